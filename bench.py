@@ -1,0 +1,170 @@
+"""Benchmark: SR steps of the multi-RBM VMC solver on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|mixed]
+                    [--N 32 --M 64 --K 2 --C 4096 --m 256 --B 64] [--no-cpu]
+
+Workload (default): trapped-ion chain N=32 spins, M=64 hidden units, K=2 states,
+h=0.595, J = power law of the N=32 ion chain, 4096 chains x 256 samples (S=2^20)
+with 64 burn-in sweeps, real weights N(0, 0.01^2), lr 0.1, reglam 40, MINRES-QLP
+rtol 1e-9.  A "step" is one full SR step: sampling + H_loc + O, moments, the
+MINRES-QLP solve with its matrix-free S*v passes, and the weight update.
+
+Prints one JSON line.  value = MC samples per second over whole SR steps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "nqes-trapped-ion_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def load_J(N):
+    path = os.path.join(REPO, "tests", "golden", f"J_N{N}.npy")
+    if os.path.exists(path):
+        return np.load(path)
+    i = np.arange(N)
+    d = np.abs(i[:, None] - i[None, :]).astype(float)
+    J = np.zeros((N, N))
+    J[d > 0] = d[d > 0] ** -1.0
+    return J / J.max()
+
+
+def initial_weights(N, M, K, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.normal(size=K * (N + M + N * M)) * 0.01).astype(np.complex128)
+
+
+def cpu_baseline(args, J, W0):
+    """The CPU oracle (clean-room FP64 OpenMP restatement of rbm.cc's SR step,
+    matrix-free S*v with O recomputed per MINRES iteration) on a bounded sample."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle
+    if not os.path.exists(pyoracle.LIB_PATH):
+        pyoracle.build()
+    threads = pyoracle.max_threads()
+    C, m, B = args.cpu_C, args.cpu_m, args.B
+    W = W0.copy()
+    t0 = time.perf_counter()
+    W, emat, info = pyoracle.sr_step(args.N, args.M, args.K, args.h, J, W, args.seed, C, m, B, args.lr,
+                                     args.reglam, args.rtol)
+    dt = time.perf_counter() - t0
+    return {
+        "value": C * m / dt,
+        "unit": "MC samples/s (whole SR step)",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one SR step with C={C} chains x m={m} samples, B={B}, same N/M/K/h/J/weights; "
+                  f"{dt:.1f}s, {info['iters']} MINRES-QLP iterations",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--N", type=int, default=32)
+    ap.add_argument("--M", type=int, default=64)
+    ap.add_argument("--K", type=int, default=2)
+    ap.add_argument("--C", type=int, default=4096)
+    ap.add_argument("--m", type=int, default=256)
+    ap.add_argument("--B", type=int, default=64)
+    ap.add_argument("--h", type=float, default=0.595)
+    ap.add_argument("--lr", type=float, default=0.1)
+    ap.add_argument("--reglam", type=float, default=40.0)
+    ap.add_argument("--rtol", type=float, default=1e-9)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--cpu-C", type=int, default=64)
+    ap.add_argument("--cpu-m", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
+                                   "design note only)"}))
+        sys.exit(2)
+
+    import nqes
+    prec = nqes.PREC_FP64 if args.precision == "fp64" else nqes.PREC_MIXED
+    J = load_J(args.N)
+    W0 = initial_weights(args.N, args.M, args.K, args.seed)
+    solver = nqes.Solver(args.N, args.M, args.K, args.h, J, precision=prec)
+    W = W0.copy()
+    S = args.C * args.m
+    reglam, lr = args.reglam, args.lr
+    step_i = 0
+
+    def one_step():
+        nonlocal reglam, lr, step_i
+        emat, st = solver.monte_carlo(W, args.seed + step_i * args.C, args.C, args.m, args.B, max(lr, 0.01),
+                                      max(reglam, 1.0), args.rtol, stats=True)
+        reglam *= 0.9
+        lr *= 0.99
+        step_i += 1
+        return emat, st
+
+    for _ in range(args.warmup):
+        one_step()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        emat, st = one_step()
+        stats.append(st)
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+
+    P = args.K * (args.N + args.M + args.N * args.M)
+    b_O = 16 if args.precision == "fp64" else 8
+    sv_calls = sum(s["sv_calls"] for s in stats)
+    sv_ms = sum(s["t_sv_ms"] for s in stats)
+    # algorithmic bytes of one S*v: two streamed passes over O, w written+read, v and result
+    sv_bytes = 2 * S * P * b_O + 2 * S * 16 + 3 * P * 16
+    sv_avg_ms = sv_ms / max(sv_calls, 1)
+    achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
+    energies = np.sort(np.linalg.eigvals(emat / args.N).real).tolist()
+    out = {
+        "metric": "MC samples/s (whole SR step); converged solves/hour; p50/p99 solve latency",
+        "value": S * args.steps / dt,
+        "unit": "MC samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (random N(0,0.01^2) real weights, seeded)",
+        "config": {"workload": f"trapped-ion SR step N={args.N} M={args.M} K={args.K} h={args.h} "
+                               f"C={args.C} m={args.m} B={args.B} rtol={args.rtol} ({args.precision})",
+                   "N": args.N, "M": args.M, "K": args.K, "chains": args.C, "samples_per_chain": args.m,
+                   "burn_in": args.B, "precision": args.precision},
+        "solves_per_hour_200_steps": 3600e3 / (200 * ms_per_step),
+        "breakdown_ms_per_step": {
+            "sample": sum(s["t_sample_ms"] for s in stats) / args.steps,
+            "moments": sum(s["t_moments_ms"] for s in stats) / args.steps,
+            "solve": sum(s["t_solve_ms"] for s in stats) / args.steps,
+            "sv_device": sv_ms / args.steps,
+        },
+        "minres_iters_per_step": [s["minres_iters"] for s in stats],
+        "accept_rate": sum(s["accepts"] for s in stats) / max(sum(s["proposals"] for s in stats), 1),
+        "energies_last_step": energies,
+        "roofline": {"bound": "hbm", "kernel": "S*v (k_sv_rowdot64 + k_sv_colacc64)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args, J, W0)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

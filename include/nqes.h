@@ -1,0 +1,127 @@
+/*
+ * nqes.h -- C ABI of the MI355X multi-RBM variational Monte Carlo solver
+ * (K lowest states of the long-range trapped-ion Ising model).
+ *
+ * The library (libnqes.so) replaces the in-process hot path of the reference
+ * executable rbm-matfree-N*-M*-K* (reference file code/rbm.cc): one
+ * stochastic-reconfiguration (SR) step = Metropolis sampling, local energies,
+ * log-derivatives O, the matrix-free S*v operator, MINRES-QLP and the weight
+ * update.  Everything is runtime-sized (the reference compiles N, M, K and J in).
+ *
+ * Conventions
+ *   complex numbers are interleaved doubles (re, im), i.e. C99 double _Complex;
+ *   weights W hold P = K*L complex numbers, L = N + M + N*M; RBM k occupies
+ *     [k*L, (k+1)*L): a[N], b[M], w[N][M] (i-major) -- the index layout of
+ *     the reference's O vector (rbm.cc:216, 223, 234);
+ *   J is N x N row-major FP64 (symmetric, zero diagonal);
+ *   spins: NW = ceil(N/32) 32-bit words per replica, bit i = spin i, a set bit
+ *     means sigma_i = -1 (rbm.cc:141); per sample K*NW words, replica-major;
+ *   K x K matrices (e_mat, H_loc) are row-major [replica k][RBM kk].
+ * Errors: every call returns an int status (NQES_OK = 0); nqes_last_error()
+ * gives a message.  No C++ exception crosses the ABI.  Calls on one context
+ * must be serialised; use one context per host thread.  Each context owns one
+ * HIP stream on its device.
+ */
+#ifndef NQES_H
+#define NQES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    NQES_OK = 0,
+    NQES_EINVAL = 1,     /* invalid argument or unsupported size            */
+    NQES_ENOMEM = 2,     /* device allocation failed / does not fit in HBM  */
+    NQES_EDEVICE = 3,    /* HIP runtime error                               */
+    NQES_ENONFINITE = 4, /* non-finite energy or gradient (rbm.cc's SIGFPE) */
+    NQES_ESTATE = 5      /* call order violated (e.g. solve before sample)  */
+};
+
+enum {
+    NQES_PREC_FP64 = 0,  /* FP64 everywhere; bit-exact with the CPU oracle's
+                            canonical order for spins, H_loc and O          */
+    NQES_PREC_MIXED = 1  /* FP32 amplitudes/ratios/O, FP64 accumulation,
+                            FP64 S*v reductions and MINRES-QLP              */
+};
+
+enum {
+    NQES_MODEL_TRAPPEDION = 0 /* E_zz = sum_{i!=j} J_ij s_i s_j (rbm.cc:1012-1031) */
+};
+
+typedef struct nqes_ctx nqes_ctx;
+
+typedef struct {
+    int minres_flag;          /* MINRES-QLP flag (minresqlp.h:14-54)          */
+    uint64_t minres_iters;    /* Lanczos iterations                           */
+    uint64_t accepts;         /* accepted Metropolis proposals, all chains    */
+    uint64_t proposals;       /* C * (B + m) * K * N                          */
+    uint64_t sv_calls;        /* S*v operator applications                    */
+    double t_sample_ms;       /* sampling + H_loc + O kernel                  */
+    double t_moments_ms;      /* column reductions o, |O|^2, E O*             */
+    double t_solve_ms;        /* MINRES-QLP including all S*v                 */
+    double t_sv_ms;           /* device time of all S*v kernels               */
+    double t_total_ms;        /* whole SR step (host wall clock)              */
+    int nonfinite;            /* 1 if a non-finite energy/gradient was seen   */
+} nqes_stats;
+
+/* Creates a context on HIP device `device`.  Returns NULL on failure. */
+nqes_ctx *nqes_create(int device);
+void nqes_destroy(nqes_ctx *ctx);
+const char *nqes_last_error(const nqes_ctx *ctx);
+
+/*
+ * Sets the problem: N spins, M hidden units (M % N == 0 as rbm.cc:148
+ * requires), K states, field h, couplings J (N x N), model, precision.
+ * Replaces the compile-time NUM_SPINS/NUM_HIDDEN_UNITS/NUM_EIGENSTATES/JMAT_INC
+ * of the reference (code/build.sh:28-42, rbm.cc:1015).
+ */
+int nqes_problem_init(nqes_ctx *ctx, int N, int M, int K, double h, const double *J,
+                      int model, int precision);
+
+/*
+ * One SR step: monte_carlo<N,M,K>(rbm, seed, C, m, B, lr, reglam, rtol, maxit, H)
+ * (rbm.cc:564-720).  Chain c uses pcg64(seed + c).  W (P complex) is updated in
+ * place (w -= lr * x); emat (K*K complex) receives the sample mean of H_loc
+ * (before the division by N that run() applies, rbm.cc:1183).  stats may be NULL.
+ */
+int nqes_sr_step(nqes_ctx *ctx, double *W, uint64_t seed, int C, int m, int B,
+                 double lr, double reglam, double rtol, uint64_t maxit,
+                 double *emat, nqes_stats *stats);
+
+/*
+ * The SR step in parts (used by parity tests and the benchmark).
+ * nqes_sample: sampling, local energies, O written to device memory, and the
+ *   normalised moments o, skk = reglam (<|O|^2> - |o|^2), f = <E O*> - <E><O*>
+ *   (rbm.cc:596-650).  Keeps them in the context.
+ * nqes_get_samples: per-sample spins (S*K*NW words), H_loc (S*K*K complex) and
+ *   per-chain accepted-proposal counts (C).  Any pointer may be NULL.
+ * nqes_get_O: rows [row0, row0+nrows) of O (row = chain*m + sample), P complex each.
+ * nqes_get_moments: o, skk, f (P complex each), eloc (1 complex), emat (K*K complex).
+ * nqes_apply_S: out = S v as the reference's matrix-free lambda (rbm.cc:661-684).
+ * nqes_solve: MINRES-QLP (minresqlp.h:114-456) on S x = f; x: P complex.
+ */
+int nqes_sample(nqes_ctx *ctx, const double *W, uint64_t seed, int C, int m, int B,
+                double reglam);
+int nqes_get_samples(nqes_ctx *ctx, uint32_t *spins, double *H, uint64_t *accepts);
+int nqes_get_O(nqes_ctx *ctx, int64_t row0, int64_t nrows, double *out);
+int nqes_get_moments(nqes_ctx *ctx, double *o, double *skk, double *f, double *eloc,
+                     double *emat);
+int nqes_apply_S(nqes_ctx *ctx, const double *v, double *out);
+int nqes_solve(nqes_ctx *ctx, double rtol, uint64_t maxit, double *x, int *flag,
+               uint64_t *iters);
+
+/* Device bytes nqes_sample needs for C chains x m samples (admission check). */
+int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision);
+
+/* Library version string. */
+const char *nqes_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NQES_H */

@@ -1,0 +1,85 @@
+"""Builds libnqes.so (HIP, gfx950) and the rbm-matfree drop-in executable in-tree.
+
+    python nqes-trapped-ion_amd/build.py [--jobs N]
+
+Outputs (git-ignored, shipped with the tree):
+    nqes-trapped-ion_amd/lib/libnqes.so
+    nqes-trapped-ion_amd/bin/rbm-matfree
+"""
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+BINDIR = os.path.join(HERE, "bin")
+OBJDIR = os.path.join(HERE, "_build")
+ARCH = os.environ.get("NQES_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+    f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(REPO, "include"),
+]
+# FP64 path: no contraction anywhere (host or device) -- canonical operation order
+EXACT = ["-ffp-contract=off"]
+
+SOURCES = [
+    ("kernels_fp64.hip", EXACT),
+    ("nqes_api.cpp", EXACT),
+]
+
+
+def _compile(src, extra):
+    out = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+    path = os.path.join(CSRC, src)
+    lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
+    cmd = [HIPCC] + COMMON + extra + lang + ["-c", path, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return out
+
+
+def build(jobs=4, verbose=False):
+    os.makedirs(OBJDIR, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(BINDIR, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(lambda se: _compile(*se), SOURCES))
+    lib = os.path.join(LIBDIR, "libnqes.so")
+    tmp = lib + ".tmp"
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, lib)
+    cli_src = os.path.join(CSRC, "cli.cpp")
+    if os.path.exists(cli_src):
+        exe = os.path.join(BINDIR, "rbm-matfree")
+        cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I", os.path.join(REPO, "include"),
+               cli_src, os.path.join(CSRC, "matv5.cpp"), os.path.join(CSRC, "eig.cpp"),
+               "-L", LIBDIR, "-lnqes", "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"cli build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
+    if verbose:
+        print("built", lib)
+    return lib
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    args = ap.parse_args()
+    build(args.jobs, verbose=True)
+    sys.exit(0)

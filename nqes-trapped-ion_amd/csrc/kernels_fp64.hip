@@ -1,0 +1,510 @@
+// kernels_fp64.hip -- FP64 device path of one SR step.
+//
+// Compiled with -ffp-contract=off: the arithmetic follows the canonical order of
+// cplx64.hpp, so spins, H_loc and O agree bit for bit with any implementation of
+// the same order (the CPU oracle is one).
+//
+// Kernels
+//   k_tables64      cosh(2w), sinh(2w) per (RBM kk, spin i, hidden j) and
+//                   exp(+-2a) per (kk, i), once per SR step (weights fixed within
+//                   a step; rbm.cc:181-186 evaluates them per proposal).
+//   k_sample64      one wavefront per Markov chain: chain start (rbm.cc:604-608),
+//                   B burn-in + m sampled Metropolis sweeps (rbm.cc:530-552), and
+//                   after each sampled sweep the local energy H_loc
+//                   (rbm.cc:1042-1064) and the log-derivative row O (rbm.cc:199-239)
+//                   written coalesced to HBM.
+//   k_moments64     column sums of O: sum O, sum O conj(O), sum E conj(O)
+//                   (rbm.cc:620-626), two-stage and deterministic.
+//   k_colsum64      column sums of a small S x n complex matrix (H_loc, E_loc).
+//   k_sv_rowdot64   w_s = O_s^T v                (first pass of S*v, rbm.cc:668)
+//   k_sv_colacc64   r_p = sum_s w_s conj(O_sp)   (second pass,        rbm.cc:668)
+//
+// Lane layout of k_sample64: lane g owns hidden units j = g + 64 q (q < JPL) for all
+// K RBMs.  A product over j (rbm.cc:159-168, 189-194) is a per-lane product over q in
+// ascending j followed by an xor-butterfly over the 64 lanes; since complex multiply
+// is commutative bit for bit, every lane ends with the pairwise-tree product of the 64
+// lane partials (strides 32, 16, ..., 1).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cplx64.hpp"
+#include "kernels.hpp"
+
+using namespace nq64;
+
+namespace {
+
+__device__ inline cd shfl_xor_cd(cd v, int m)
+{
+    return mk(__shfl_xor(v.re, m), __shfl_xor(v.im, m));
+}
+
+// all 64 lanes must be active
+__device__ inline cd wave_prod(cd v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = mul(v, shfl_xor_cd(v, off));
+    return v;
+}
+
+__device__ inline cd wave_sum(cd v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = add(v, shfl_xor_cd(v, off));
+    return v;
+}
+
+__device__ inline int spin_bit(const uint32_t *sp, int NW, int k, int i)
+{
+    return (int)((sp[k * NW + (i >> 5)] >> (i & 31)) & 1u);
+}
+
+__global__ void k_tables64(int N, int M, int K, int L, const cd *__restrict__ W, cd *__restrict__ chT,
+                           cd *__restrict__ shT, cd *__restrict__ eaP, cd *__restrict__ eaN)
+{
+    long long n = (long long)K * N * M;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        int j = (int)(idx % M);
+        long long r = idx / M;
+        int i = (int)(r % N);
+        int kk = (int)(r / N);
+        cd w2 = scale(2.0, W[(long long)kk * L + N + M + (long long)i * M + j]);
+        chT[idx] = ccosh_(w2);
+        shT[idx] = csinh_(w2);
+        if (j == 0) {
+            cd a2 = scale(2.0, W[(long long)kk * L + i]);
+            eaP[kk * N + i] = cexp_(a2);
+            eaN[kk * N + i] = cexp_(neg(a2));
+        }
+    }
+}
+
+template <int K, int JPL>
+__global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = a.N, M = a.M, NW = a.NW, L = a.L;
+    cd *tl = reinterpret_cast<cd *>(smem);                      // [k][kk][j]
+    uint32_t *sp = reinterpret_cast<uint32_t *>(tl + K * K * M); // [k][word]
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x;
+    const cd *W = static_cast<const cd *>(a.W);
+    const cd *chT = static_cast<const cd *>(a.chT);
+    const cd *shT = static_cast<const cd *>(a.shT);
+    const cd *eaP = static_cast<const cd *>(a.eaP);
+    const cd *eaN = static_cast<const cd *>(a.eaN);
+    cd *Hs = static_cast<cd *>(a.Hs);
+    cd *Es = static_cast<cd *>(a.Es);
+
+    // ---- chain start: spins (rbm.cc:99-119)
+    pcg rng = pcg_seed(a.seed + (uint64_t)c);
+    for (int w = 0; w < K * NW; w++) {
+        uint32_t word = 0;
+        int k = w / NW, i0 = (w % NW) * 32;
+        for (int b = 0; b < 32 && i0 + b < N; b++) {
+            (void)k;
+            if (canonical(pcg_next(rng)) < 0.5) word |= 1u << b;
+        }
+        if (lane == 0) sp[w] = word;
+    }
+    __syncthreads();
+
+    // ---- state_fcn (rbm.cc:148-173)
+    cd phi[K * K], phi_inv[K * K];
+    {
+        cd part[K * K];
+#pragma unroll
+        for (int q = 0; q < K * K; q++) part[q] = one();
+#pragma unroll
+        for (int q = 0; q < JPL; q++) {
+            int j = lane + 64 * q;
+            if (j < M) {
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    cd th[K];
+                    cd bj = W[(long long)kk * L + N + j];
+#pragma unroll
+                    for (int k = 0; k < K; k++) th[k] = bj;
+                    for (int i = 0; i < N; i++) {
+                        cd w = W[(long long)kk * L + N + M + (long long)i * M + j];
+#pragma unroll
+                        for (int k = 0; k < K; k++) th[k] = add(th[k], spin_bit(sp, NW, k, i) ? neg(w) : w);
+                    }
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        tl[(k * K + kk) * M + j] = ctanh_(th[k]);
+                        cd ch = ccosh_(th[k]);
+                        part[k * K + kk] = (q == 0) ? ch : mul(part[k * K + kk], ch);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                cd A = zero();
+                for (int i = 0; i < N; i++) {
+                    cd ai = W[(long long)kk * L + i];
+                    A = add(A, spin_bit(sp, NW, k, i) ? neg(ai) : ai);
+                }
+                phi[k * K + kk] = mul(cexp_(A), wave_prod(part[k * K + kk]));
+            }
+        invK<K>(phi, phi_inv);
+    }
+    __syncthreads();
+
+    unsigned long long accepts = 0;
+    const int nsweeps = a.B + a.m;
+    for (int sw = 0; sw < nsweeps; sw++) {
+        // ---- Metropolis sweep (rbm.cc:530-552)
+        cd det_phi = detK<K>(phi);
+        for (int k = 0; k < K; k++) {
+            for (int i = 0; i < N; i++) {
+                const int si = spin_bit(sp, NW, k, i);
+                cd tnew[JPL][K];
+                cd part[K];
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) part[kk] = one();
+#pragma unroll
+                for (int q = 0; q < JPL; q++) {
+                    int j = lane + 64 * q;
+                    if (j < M) {
+#pragma unroll
+                        for (int kk = 0; kk < K; kk++) {
+                            long long ti = ((long long)kk * N + i) * M + j;
+                            cd chv = chT[ti];
+                            cd shv = shT[ti];
+                            if (si) shv = neg(shv);
+                            cd th = tl[(k * K + kk) * M + j];
+                            cd denom = sub(chv, mul(th, shv));
+                            tnew[q][kk] = div(sub(mul(th, chv), shv), denom);
+                            part[kk] = (q == 0) ? denom : mul(part[kk], denom);
+                        }
+                    }
+                }
+                cd prow[K];
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    cd e = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                    cd T = wave_prod(part[kk]);
+                    prow[kk] = mul(mul(phi[k * K + kk], e), T);
+                }
+                cd phip[K * K];
+#pragma unroll
+                for (int r = 0; r < K; r++)
+#pragma unroll
+                    for (int cc = 0; cc < K; cc++) phip[r * K + cc] = (r == k) ? prow[cc] : phi[r * K + cc];
+                cd det_p = detK<K>(phip);
+                double prob = (det_phi.re == 0.0 && det_phi.im == 0.0) ? 1.0 : norm(det_p) / norm(det_phi);
+                double u = canonical(pcg_next(rng));
+                if (u <= (prob < 1.0 ? prob : 1.0)) {
+                    det_phi = det_p;
+#pragma unroll
+                    for (int q = 0; q < K * K; q++) phi[q] = phip[q];
+                    invK<K>(phi, phi_inv);
+#pragma unroll
+                    for (int q = 0; q < JPL; q++) {
+                        int j = lane + 64 * q;
+                        if (j < M)
+#pragma unroll
+                            for (int kk = 0; kk < K; kk++) tl[(k * K + kk) * M + j] = tnew[q][kk];
+                    }
+                    if (lane == 0) sp[k * NW + (i >> 5)] ^= 1u << (i & 31);
+                    accepts++;
+                    __syncthreads();
+                }
+            }
+        }
+        if (sw < a.B) continue;
+
+        // ---- one sample
+        const long long s = (long long)c * a.m + (sw - a.B);
+        __syncthreads();
+
+        // local energy (rbm.cc:1042-1064); E_zz of replica (lane % K) (rbm.cc:1012-1031)
+        cd H[K * K];
+        {
+            double ez;
+            {
+                int kr = lane % K;
+                double res = 0.0;
+                for (int i = 1; i < N; i++) {
+                    int bi = spin_bit(sp, NW, kr, i);
+                    for (int j = 0; j < i; j++) {
+                        double v = a.J[(long long)i * N + j];
+                        res += (bi ^ spin_bit(sp, NW, kr, j)) ? -v : v;
+                    }
+                }
+                ez = res * 2.0;
+            }
+            cd res[K * K], sx[K * K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                double ek = __shfl(ez, k);
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) res[k * K + kk] = scale(ek, phi[k * K + kk]);
+            }
+#pragma unroll
+            for (int q = 0; q < K * K; q++) sx[q] = zero();
+            for (int i = 0; i < N; i++) {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int si = spin_bit(sp, NW, k, i);
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        cd part = one();
+#pragma unroll
+                        for (int q = 0; q < JPL; q++) {
+                            int j = lane + 64 * q;
+                            if (j < M) {
+                                long long ti = ((long long)kk * N + i) * M + j;
+                                cd shv = shT[ti];
+                                if (si) shv = neg(shv);
+                                cd d = sub(chT[ti], mul(tl[(k * K + kk) * M + j], shv));
+                                part = (q == 0) ? d : mul(part, d);
+                            }
+                        }
+                        cd e = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                        cd v = mul(mul(phi[k * K + kk], e), wave_prod(part));
+                        sx[k * K + kk] = add(sx[k * K + kk], v);
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < K * K; q++) res[q] = sub(res[q], scale(a.h, sx[q]));
+            matmulK<K>(phi_inv, res, H);
+        }
+        cd energy = zero();
+#pragma unroll
+        for (int k = 0; k < K; k++) energy = add(energy, H[k * K + k]);
+#pragma unroll
+        for (int q = 0; q < K * K; q++)
+            if (lane == q) Hs[s * K * K + q] = H[q];
+        if (lane == 0) Es[s] = energy;
+        for (int w = lane; w < K * NW; w += 64) a.spins[s * K * NW + w] = sp[w];
+
+        // log-derivatives (rbm.cc:199-239).  Columns c != k of Phi^+ derPhi equal
+        // those of G = Phi^+ Phi (same operands, same order), so only column k is
+        // formed per entry.
+        if (a.O) {
+            cd G[K * K];
+            matmulK<K>(phi_inv, phi, G);
+            cd *Orow = static_cast<cd *>(a.O) + s * a.P;
+            for (long long p = lane; p < a.P; p += 64) {
+                int k = (int)(p / L);
+                int r = (int)(p % L);
+                cd col[K];
+                if (r < N) {
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        cd v = phi[kk * K + k];
+                        col[kk] = spin_bit(sp, NW, kk, r) ? neg(v) : v;
+                    }
+                } else if (r < N + M) {
+                    int j = r - N;
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) col[kk] = mul(phi[kk * K + k], tl[(kk * K + k) * M + j]);
+                } else {
+                    int rr = r - N - M;
+                    int i = rr / M, j = rr % M;
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        cd v = mul(phi[kk * K + k], tl[(kk * K + k) * M + j]);
+                        col[kk] = spin_bit(sp, NW, kk, i) ? neg(v) : v;
+                    }
+                }
+                cd X[K * K];
+#pragma unroll
+                for (int rw = 0; rw < K; rw++) {
+                    cd y = mul(phi_inv[rw * K], col[0]);
+#pragma unroll
+                    for (int l = 1; l < K; l++) y = add(y, mul(phi_inv[rw * K + l], col[l]));
+#pragma unroll
+                    for (int cc = 0; cc < K; cc++) X[rw * K + cc] = (cc == k) ? y : G[rw * K + cc];
+                }
+                Orow[p] = detK<K>(X);
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0) a.accepts[c] = accepts;
+}
+
+// Column sums over rows [s0, s1) of chunk blockIdx.y; thread = column.
+__global__ void k_moments64(const cd *__restrict__ O, const cd *__restrict__ E, long long S, long long P,
+                            int nchunk, cd *__restrict__ po, cd *__restrict__ ps, cd *__restrict__ pf)
+{
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    long long s0 = S * blockIdx.y / nchunk, s1 = S * (blockIdx.y + 1) / nchunk;
+    cd so = zero(), ss = zero(), sf = zero();
+    for (long long s = s0; s < s1; s++) {
+        cd v = O[s * P + p];
+        so = add(so, v);
+        ss = add(ss, mul(v, conj(v)));
+        sf = add(sf, mul(E[s], conj(v)));
+    }
+    long long o = (long long)blockIdx.y * P + p;
+    po[o] = so;
+    ps[o] = ss;
+    pf[o] = sf;
+}
+
+// out[col] = sum over chunks of part[chunk][col], chunks in order
+__global__ void k_chunksum64(const cd *__restrict__ part, long long ncol, int nchunk, cd *__restrict__ out)
+{
+    long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    cd acc = part[col];
+    for (int ch = 1; ch < nchunk; ch++) acc = add(acc, part[(long long)ch * ncol + col]);
+    out[col] = acc;
+}
+
+__global__ void k_colsum64(const cd *__restrict__ X, long long S, long long ncol, int nchunk,
+                           cd *__restrict__ part)
+{
+    long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    long long s0 = S * blockIdx.y / nchunk, s1 = S * (blockIdx.y + 1) / nchunk;
+    cd acc = zero();
+    for (long long s = s0; s < s1; s++) acc = add(acc, X[s * ncol + col]);
+    part[(long long)blockIdx.y * ncol + col] = acc;
+}
+
+// w_s = sum_p O_sp v_p; one wave per row, 4 waves per block, grid-stride over rows
+__global__ __launch_bounds__(256) void k_sv_rowdot64(const cd *__restrict__ O, const cd *__restrict__ v,
+                                                     long long S, long long P, cd *__restrict__ w)
+{
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long s = wave; s < S; s += nwaves) {
+        const cd *row = O + s * P;
+        cd acc = zero();
+        for (long long p = lane; p < P; p += 64) acc = add(acc, mul(row[p], v[p]));
+        acc = wave_sum(acc);
+        if (lane == 0) w[s] = acc;
+    }
+}
+
+// part[chunk][p] = sum_{s in chunk} w_s conj(O_sp)
+__global__ void k_sv_colacc64(const cd *__restrict__ O, const cd *__restrict__ w, long long S, long long P,
+                              int nchunk, cd *__restrict__ part)
+{
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    long long s0 = S * blockIdx.y / nchunk, s1 = S * (blockIdx.y + 1) / nchunk;
+    cd acc = zero();
+    for (long long s = s0; s < s1; s++) acc = add(acc, mul(w[s], conj(O[s * P + p])));
+    part[(long long)blockIdx.y * P + p] = acc;
+}
+
+template <int K, int JPL>
+hipError_t launch_sample_t(const Fp64SampleArgs &a, hipStream_t st)
+{
+    size_t lds = sizeof(cd) * K * K * a.M + sizeof(uint32_t) * K * a.NW;
+    hipLaunchKernelGGL((k_sample64<K, JPL>), dim3(a.C), dim3(64), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_sample_k(const Fp64SampleArgs &a, hipStream_t st)
+{
+    int jpl = (a.M + 63) / 64;
+    if (jpl <= 1) return launch_sample_t<K, 1>(a, st);
+    if (jpl <= 2) return launch_sample_t<K, 2>(a, st);
+    if (jpl <= 4) return launch_sample_t<K, 4>(a, st);
+    if (jpl <= 8) return launch_sample_t<K, 8>(a, st);
+    return hipErrorInvalidValue;
+}
+
+} // namespace
+
+hipError_t nq_launch_tables64(int N, int M, int K, int L, const void *W, void *chT, void *shT, void *eaP,
+                              void *eaN, hipStream_t st)
+{
+    long long n = (long long)K * N * M;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_tables64, dim3(blocks), dim3(256), 0, st, N, M, K, L, (const cd *)W, (cd *)chT,
+                       (cd *)shT, (cd *)eaP, (cd *)eaN);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_sample64(int K, const Fp64SampleArgs &a, hipStream_t st)
+{
+    switch (K) {
+    case 1: return launch_sample_k<1>(a, st);
+    case 2: return launch_sample_k<2>(a, st);
+    case 3: return launch_sample_k<3>(a, st);
+    case 4: return launch_sample_k<4>(a, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t nq_launch_moments64(const void *O, const void *E, long long S, long long P, int nchunk, void *po,
+                               void *ps, void *pf, void *out_o, void *out_s, void *out_f, hipStream_t st)
+{
+    dim3 g((unsigned)((P + 255) / 256), (unsigned)nchunk);
+    hipLaunchKernelGGL(k_moments64, g, dim3(256), 0, st, (const cd *)O, (const cd *)E, S, P, nchunk, (cd *)po,
+                       (cd *)ps, (cd *)pf);
+    dim3 g2((unsigned)((P + 255) / 256));
+    hipLaunchKernelGGL(k_chunksum64, g2, dim3(256), 0, st, (const cd *)po, P, nchunk, (cd *)out_o);
+    hipLaunchKernelGGL(k_chunksum64, g2, dim3(256), 0, st, (const cd *)ps, P, nchunk, (cd *)out_s);
+    hipLaunchKernelGGL(k_chunksum64, g2, dim3(256), 0, st, (const cd *)pf, P, nchunk, (cd *)out_f);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_colsum64(const void *X, long long S, long long ncol, int nchunk, void *part, void *out,
+                              hipStream_t st)
+{
+    dim3 g((unsigned)((ncol + 63) / 64), (unsigned)nchunk);
+    hipLaunchKernelGGL(k_colsum64, g, dim3(64), 0, st, (const cd *)X, S, ncol, nchunk, (cd *)part);
+    hipLaunchKernelGGL(k_chunksum64, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, st, (const cd *)part, ncol,
+                       nchunk, (cd *)out);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_sv64(const void *O, const void *v, long long S, long long P, int nchunk, void *w,
+                          void *part, void *out, hipStream_t st)
+{
+    long long blocks = (S + 3) / 4;
+    if (blocks > 16384) blocks = 16384;
+    hipLaunchKernelGGL(k_sv_rowdot64, dim3((unsigned)blocks), dim3(256), 0, st, (const cd *)O, (const cd *)v, S,
+                       P, (cd *)w);
+    dim3 g((unsigned)((P + 255) / 256), (unsigned)nchunk);
+    hipLaunchKernelGGL(k_sv_colacc64, g, dim3(256), 0, st, (const cd *)O, (const cd *)w, S, P, nchunk,
+                       (cd *)part);
+    hipLaunchKernelGGL(k_chunksum64, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, (const cd *)part, P,
+                       nchunk, (cd *)out);
+    return hipGetLastError();
+}
+
+// Device copies of the elementary functions, for bit-exactness tests.
+__global__ void k_mathprobe64(const double *in, long long n, double *out)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    cd z = mk(in[2 * t], in[2 * t + 1]);
+    cd b = mk(in[2 * ((t + 1) % n)], in[2 * ((t + 1) % n) + 1]);
+    cd r[6];
+    r[0] = cexp_(z);
+    r[1] = ccosh_(z);
+    r[2] = csinh_(z);
+    r[3] = ctanh_(z);
+    r[4] = div(z, b);
+    r[5] = mk(exp_(z.re), 0.0);
+    for (int q = 0; q < 6; q++) {
+        out[(t * 6 + q) * 2] = r[q].re;
+        out[(t * 6 + q) * 2 + 1] = r[q].im;
+    }
+}
+
+hipError_t nq_launch_mathprobe64(const void *in, long long n, void *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mathprobe64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const double *)in, n,
+                       (double *)out);
+    return hipGetLastError();
+}

@@ -1,0 +1,232 @@
+"""Python host mirror of libnqes (include/nqes.h).
+
+Mirrors the reference's SR-step seam (code/rbm.cc:564-720, ``monte_carlo``) and
+its training loop (code/rbm.cc:1173-1197, ``run``) over the C ABI.  The HIP
+library is the only compute path: if libnqes.so is missing or no GPU is
+visible, constructing a :class:`Solver` raises.
+
+Array conventions: complex arrays are numpy complex128; weights are one flat
+vector of P = K*(N+M+N*M) entries, RBM k = [a(N), b(M), w(N*M, i-major)].
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnqes.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+MODEL_TRAPPEDION = 0
+
+_cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+
+
+class NqesError(RuntimeError):
+    pass
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("minres_flag", ctypes.c_int),
+        ("minres_iters", ctypes.c_uint64),
+        ("accepts", ctypes.c_uint64),
+        ("proposals", ctypes.c_uint64),
+        ("sv_calls", ctypes.c_uint64),
+        ("t_sample_ms", ctypes.c_double),
+        ("t_moments_ms", ctypes.c_double),
+        ("t_solve_ms", ctypes.c_double),
+        ("t_sv_ms", ctypes.c_double),
+        ("t_total_ms", ctypes.c_double),
+        ("nonfinite", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+_lib = None
+
+# every symbol include/nqes.h declares
+EXPORTS = [
+    "nqes_create", "nqes_destroy", "nqes_last_error", "nqes_problem_init", "nqes_sr_step",
+    "nqes_sample", "nqes_get_samples", "nqes_get_O", "nqes_get_moments", "nqes_apply_S",
+    "nqes_solve", "nqes_required_bytes", "nqes_version",
+]
+
+
+def load_library(path=LIB_PATH):
+    """Loads libnqes.so and declares the C signatures (no device call)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NqesError(f"{path} not built: run nqes-trapped-ion_amd/build.py")
+    L = ctypes.CDLL(path)
+    c_int, c_u64, c_i64, c_dbl = ctypes.c_int, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double
+    L.nqes_create.restype = _vp
+    L.nqes_create.argtypes = [c_int]
+    L.nqes_destroy.argtypes = [_vp]
+    L.nqes_last_error.restype = ctypes.c_char_p
+    L.nqes_last_error.argtypes = [_vp]
+    L.nqes_problem_init.restype = c_int
+    L.nqes_problem_init.argtypes = [_vp, c_int, c_int, c_int, c_dbl, _dp, c_int, c_int]
+    L.nqes_sr_step.restype = c_int
+    L.nqes_sr_step.argtypes = [_vp, _cp, c_u64, c_int, c_int, c_int, c_dbl, c_dbl, c_dbl, c_u64, _cp,
+                               ctypes.POINTER(Stats)]
+    L.nqes_sample.restype = c_int
+    L.nqes_sample.argtypes = [_vp, _cp, c_u64, c_int, c_int, c_int, c_dbl]
+    L.nqes_get_samples.restype = c_int
+    L.nqes_get_samples.argtypes = [_vp, _vp, _vp, _vp]
+    L.nqes_get_O.restype = c_int
+    L.nqes_get_O.argtypes = [_vp, c_i64, c_i64, _cp]
+    L.nqes_get_moments.restype = c_int
+    L.nqes_get_moments.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+    L.nqes_apply_S.restype = c_int
+    L.nqes_apply_S.argtypes = [_vp, _cp, _cp]
+    L.nqes_solve.restype = c_int
+    L.nqes_solve.argtypes = [_vp, c_dbl, c_u64, _cp, ctypes.POINTER(c_int), ctypes.POINTER(c_u64)]
+    L.nqes_required_bytes.restype = c_i64
+    L.nqes_required_bytes.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int]
+    L.nqes_version.restype = ctypes.c_char_p
+    L.nqes_version.argtypes = []
+    _lib = L
+    return L
+
+
+def nparams(N, M, K):
+    return K * (N + M + N * M)
+
+
+def nwords(N):
+    return (N + 31) // 32
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_vp)
+
+
+class Solver:
+    """One device context with one problem (N, M, K, h, J) loaded.
+
+    ``monte_carlo`` is the reference's SR step (rbm.cc:564-567): it updates
+    ``W`` in place and returns the K x K sample mean of H_loc.
+    """
+
+    def __init__(self, N, M, K, h, J, precision=PREC_FP64, device=0, model=MODEL_TRAPPEDION):
+        self.lib = load_library()
+        self.ctx = self.lib.nqes_create(int(device))
+        if not self.ctx:
+            raise NqesError(f"nqes_create({device}) failed: no HIP device")
+        self.N, self.M, self.K, self.h = int(N), int(M), int(K), float(h)
+        self.P = nparams(N, M, K)
+        self.NW = nwords(N)
+        J = np.ascontiguousarray(J, dtype=np.float64)
+        if J.shape != (N, N):
+            raise ValueError(f"J must be {N}x{N}")
+        self._check(self.lib.nqes_problem_init(self.ctx, self.N, self.M, self.K, self.h, J.reshape(-1),
+                                               int(model), int(precision)))
+        self.C = self.m = None
+
+    def _check(self, st):
+        if st != 0:
+            msg = self.lib.nqes_last_error(self.ctx).decode()
+            raise NqesError(f"nqes status {st}: {msg}")
+
+    def close(self):
+        if self.ctx:
+            self.lib.nqes_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---------------------------------------------------------------- SR step
+    def monte_carlo(self, W, seed, C, m, B, lr, reglam, rtol, maxit=2**64 - 1, stats=False):
+        if W.dtype != np.complex128 or not W.flags.c_contiguous or W.shape != (self.P,):
+            raise ValueError("W must be a contiguous complex128 vector of length P")
+        emat = np.zeros(self.K * self.K, dtype=np.complex128)
+        st = Stats()
+        self._check(self.lib.nqes_sr_step(self.ctx, W, int(seed) % (1 << 64), int(C), int(m), int(B), float(lr),
+                                          float(reglam), float(rtol), int(maxit), emat, ctypes.byref(st)))
+        self.C, self.m = C, m
+        emat = emat.reshape(self.K, self.K)
+        return (emat, st.as_dict()) if stats else emat
+
+    # ---------------------------------------------------------------- parts
+    def sample(self, W, seed, C, m, B, reglam):
+        W = np.ascontiguousarray(W, dtype=np.complex128)
+        self._check(self.lib.nqes_sample(self.ctx, W, int(seed) % (1 << 64), int(C), int(m), int(B),
+                                         float(reglam)))
+        self.C, self.m = C, m
+
+    def samples(self):
+        S = self.C * self.m
+        spins = np.zeros((S, self.K, self.NW), dtype=np.uint32)
+        H = np.zeros((S, self.K, self.K), dtype=np.complex128)
+        acc = np.zeros(self.C, dtype=np.uint64)
+        self._check(self.lib.nqes_get_samples(self.ctx, _ptr(spins), _ptr(H), _ptr(acc)))
+        return spins, H, acc
+
+    def O_rows(self, row0, nrows):
+        out = np.zeros((nrows, self.P), dtype=np.complex128)
+        self._check(self.lib.nqes_get_O(self.ctx, int(row0), int(nrows), out.reshape(-1)))
+        return out
+
+    def moments(self):
+        o, skk, f = (np.zeros(self.P, dtype=np.complex128) for _ in range(3))
+        eloc = np.zeros(1, dtype=np.complex128)
+        emat = np.zeros(self.K * self.K, dtype=np.complex128)
+        self._check(self.lib.nqes_get_moments(self.ctx, _ptr(o), _ptr(skk), _ptr(f), _ptr(eloc), _ptr(emat)))
+        return dict(o=o, skk=skk, f=f, eloc=eloc[0], emat=emat.reshape(self.K, self.K))
+
+    def apply_S(self, v):
+        v = np.ascontiguousarray(v, dtype=np.complex128)
+        out = np.zeros(self.P, dtype=np.complex128)
+        self._check(self.lib.nqes_apply_S(self.ctx, v, out))
+        return out
+
+    def solve(self, rtol, maxit=2**64 - 1):
+        x = np.zeros(self.P, dtype=np.complex128)
+        flag = ctypes.c_int(0)
+        it = ctypes.c_uint64(0)
+        self._check(self.lib.nqes_solve(self.ctx, float(rtol), int(maxit), x, ctypes.byref(flag), ctypes.byref(it)))
+        return x, flag.value, it.value
+
+
+def run(solver, W, seed, C, m, B, niters, tol=1e-6, lrate0=0.1, lratef=0.99, lrate_cutoff=0.01,
+        reglam0=0.01, reglam_decay=0.9, reglam_cutoff=0.001, rtol=1e-10, maxit=2**64 - 1, log=None):
+    """The reference's SR loop (rbm.cc:1173-1197): schedules, seeds seed + t*C,
+    sorted real eigenvalues of e_mat/N, convergence when ||evals - best|| < tol.
+    Returns (energies per step, converged step or None, final e_mat/N)."""
+    N = solver.N
+    reglam, lrate = reglam0, lrate0
+    best = None
+    hist = []
+    emat = None
+    for it in range(niters):
+        emat = solver.monte_carlo(W, seed + it * C, C, m, B, max(lrate, lrate_cutoff), max(reglam, reglam_cutoff),
+                                  rtol, maxit) / N
+        reglam *= reglam_decay
+        lrate *= lratef
+        ev = np.sort(np.linalg.eigvals(emat).real)
+        hist.append(ev)
+        if log:
+            log(ev)
+        if it and np.linalg.norm(ev - best) < tol:
+            return hist, it + 1, emat
+        if not it or ev.sum() < best.sum():
+            best = ev
+    return hist, None, emat

@@ -1,0 +1,91 @@
+"""FP64 HIP path vs the CPU oracle, through the C ABI (libnqes.so).
+
+The FP64 device path follows the oracle's canonical operation order, so per-chain
+quantities (accept/reject sequence, spins, H_loc, O) must be bit-identical.
+Sums over samples use a different (deterministic) reduction order than the
+oracle, so moments, S*v and the SR step are compared at 1e-12 relative; the SR
+update goes through MINRES-QLP with rtol, so weights are compared at a
+tolerance tied to rtol.
+"""
+import numpy as np
+import pytest
+
+from helpers import load_J, random_weights, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # N, M, K, C, m, B
+    (8, 16, 2, 6, 4, 3),
+    (4, 8, 1, 5, 6, 2),
+    (6, 12, 3, 4, 3, 2),
+    (4, 4, 4, 3, 3, 2),
+    (8, 64, 2, 3, 2, 1),
+    (4, 128, 2, 2, 2, 1),
+    (40, 80, 2, 2, 2, 1),
+]
+
+
+@pytest.fixture(scope="module")
+def nqes():
+    import nqes as mod
+    return mod
+
+
+def oracle_samples(oracle, N, M, K, h, J, W, seed, C, m, B):
+    outs = [oracle.run_chain(N, M, K, h, J, W, seed, c, m, B, want_O=True) for c in range(C)]
+    spins = np.concatenate([o["spins"] for o in outs])
+    H = np.concatenate([o["H"] for o in outs])
+    O = np.concatenate([o["O"] for o in outs])
+    acc = np.array([o["accepts"] for o in outs], dtype=np.uint64)
+    return spins, H, O, acc
+
+
+@pytest.mark.parametrize("N,M,K,C,m,B", CASES)
+def test_sampler_bitexact(oracle, nqes, N, M, K, C, m, B):
+    print(f"case N={N} M={M} K={K} C={C} m={m} B={B}", flush=True)
+    h = 0.595
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=N * 100 + M + K, scale=0.15)
+    seed = 1234567
+    sp_o, H_o, O_o, acc_o = oracle_samples(oracle, N, M, K, h, J, W, seed, C, m, B)
+    with nqes.Solver(N, M, K, h, J) as s:
+        s.sample(W, seed, C, m, B, reglam=0.5)
+        sp_g, H_g, acc_g = s.samples()
+        O_g = s.O_rows(0, C * m)
+    assert np.array_equal(acc_g, acc_o), (acc_g, acc_o)
+    assert np.array_equal(sp_g, sp_o)
+    assert rel_err(H_g, H_o) == 0.0, rel_err(H_g, H_o)
+    assert rel_err(O_g, O_o) == 0.0, rel_err(O_g, O_o)
+
+
+@pytest.mark.parametrize("N,M,K,C,m,B", CASES[:4])
+def test_moments_sv_step(oracle, nqes, N, M, K, C, m, B):
+    print(f"case N={N} M={M} K={K} C={C} m={m} B={B}", flush=True)
+    h, reglam, lr, rtol = 0.595, 0.7, 0.1, 1e-10
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=7 + N + M + K, scale=0.15)
+    seed = 99
+    Wo, emat_o, info = oracle.sr_step(N, M, K, h, J, W, seed, C, m, B, lr, reglam, rtol, want_details=True)
+    _, _, O_o, _ = oracle_samples(oracle, N, M, K, h, J, W, seed, C, m, B)
+    with nqes.Solver(N, M, K, h, J) as s:
+        s.sample(W, seed, C, m, B, reglam)
+        mom = s.moments()
+        assert rel_err(mom["o"], info["o"]) < 1e-12
+        assert rel_err(mom["skk"], info["skk"]) < 1e-12
+        assert rel_err(mom["f"], info["f"]) < 1e-12
+        assert abs(mom["eloc"] - info["eloc"]) <= 1e-12 * abs(info["eloc"])
+        assert rel_err(mom["emat"], emat_o) < 1e-12
+        rng = np.random.default_rng(5)
+        v = rng.normal(size=s.P) + 1j * rng.normal(size=s.P)
+        sv_g = s.apply_S(v)
+        sv_o = oracle.sv_from_O(O_o, info["o"], info["skk"], v)
+        assert rel_err(sv_g, sv_o) < 1e-12
+        x, flag, iters = s.solve(rtol)
+        assert flag == info["flag"] or flag in (1, 2)
+        assert rel_err(x, info["x"]) < 1e-6, (rel_err(x, info["x"]), flag, iters, info["iters"])
+        Wg = W.copy()
+        emat_g, st = s.monte_carlo(Wg, seed, C, m, B, lr, reglam, rtol, stats=True)
+    assert rel_err(emat_g, emat_o) < 1e-12
+    assert rel_err(Wg - W, Wo - W) < 1e-6
+    assert st["accepts"] == info["accepts"]
