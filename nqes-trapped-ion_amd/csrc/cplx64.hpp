@@ -14,7 +14,7 @@
 //                     Taylor polynomials to r^17 / r^18 on |r| <= pi/4
 //   cosh, sinh        Taylor to x^20 / x^19 for |x| < 1, (e^x +- e^-x)/2 above
 //   complex exp/cosh/sinh from the real ones; tanh(z) =
-//                     (sinh 2x + i sin 2y) / (cosh 2x + cos 2y), |x| <= 20
+//                     (sinh x cosh x + i sin y cos y) / (sinh^2 x + cos^2 y), |x| <= 20
 // sin/cosh/sinh are evaluated on |argument| and the sign applied last, so cosh and
 // cos are exactly even and sinh and sin exactly odd; the sampler relies on that to
 // tabulate cosh(2w), sinh(2w) once per SR step and flip signs per proposal.
@@ -251,16 +251,16 @@ __host__ __device__ inline cd csinh_(cd z)
 
 __host__ __device__ inline cd ctanh_(cd z)
 {
-    double s2, c2;
-    sincos_(2.0 * z.im, &s2, &c2);
+    double sy, cy;
+    sincos_(z.im, &sy, &cy);
     if (fabs(z.re) > 20.0) {
         double e = exp_(-2.0 * fabs(z.re));
-        return mk(__builtin_copysign(1.0, z.re), 2.0 * s2 * e);
+        return mk(__builtin_copysign(1.0, z.re), 4.0 * (sy * cy) * e);
     }
-    double ch2, sh2;
-    coshsinh_(2.0 * z.re, &ch2, &sh2);
-    double d = ch2 + c2;
-    return mk(sh2 / d, s2 / d);
+    double ch, sh;
+    coshsinh_(z.re, &ch, &sh);
+    double d = sh * sh + cy * cy;
+    return mk((sh * ch) / d, (sy * cy) / d);
 }
 
 #undef NQ_F

@@ -227,18 +227,20 @@ static inline cplx c_sinh(cplx z)
     return cmk(sh * c, ch * s);
 }
 
+/* tanh(x+iy) = (sinh x cosh x + i sin y cos y) / (sinh^2 x + cos^2 y): the
+ * denominator equals (cosh 2x + cos 2y)/2 without its cancellation near the poles */
 static inline cplx c_tanh(cplx z)
 {
-    double s2, c2;
-    orc_sincos(2.0 * z.im, &s2, &c2);
+    double sy, cy;
+    orc_sincos(z.im, &sy, &cy);
     if (fabs(z.re) > 20.0) {
         double e = orc_exp(-2.0 * fabs(z.re));
-        return cmk(copysign(1.0, z.re), 2.0 * s2 * e);
+        return cmk(copysign(1.0, z.re), 4.0 * (sy * cy) * e);
     }
-    double ch2, sh2;
-    orc_coshsinh(2.0 * z.re, &ch2, &sh2);
-    double d = ch2 + c2;
-    return cmk(sh2 / d, s2 / d);
+    double ch, sh;
+    orc_coshsinh(z.re, &ch, &sh);
+    double d = sh * sh + cy * cy;
+    return cmk((sh * ch) / d, (sy * cy) / d);
 }
 
 /* exported scalar wrappers (tests pin them against numpy) */
@@ -468,6 +470,14 @@ static void init_spins(const prob_t *p, u128 *rng, uint32_t *s)
     for (int k = 0; k < p->K; k++)
         for (int i = 0; i < p->N; i++)
             if (canonical(pcg_next(rng)) < 0.5) s[k * p->NW + (i >> 5)] |= 1u << (i & 31);
+}
+
+/* chain start spins of chain c: pcg64(seed + c), K*N draws (tests) */
+void orc_init_spins(int N, int K, uint64_t seed, int c, uint32_t *out)
+{
+    prob_t p = mkprob(N, 1, K, 0.0, NULL);
+    u128 rng = pcg_seed(seed + (uint64_t)c);
+    init_spins(&p, &rng, out);
 }
 
 /* ------------------------------------------------------------------------- */

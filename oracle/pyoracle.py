@@ -73,6 +73,7 @@ def lib():
                                   ctypes.POINTER(c_long), vp, vp, vp, vp, vp]
         L.orc_sv_from_O.restype = c_int
         L.orc_sv_from_O.argtypes = [c_long, c_long, _cp, _cp, _cp, _cp, _cp]
+        L.orc_init_spins.argtypes = [c_int, c_int, c_u64, c_int, _up]
         L.orc_max_threads.restype = c_int
         L.orc_set_threads.argtypes = [c_int]
         _lib = L
@@ -178,6 +179,12 @@ def unpack_spins(words, N):
     for i in range(N):
         bits[..., i] = (words[..., i // 32] >> np.uint32(i % 32)) & 1
     return bits
+
+
+def init_spins(N, K, seed, c):
+    out = np.zeros(K * nwords(N), dtype=np.uint32)
+    lib().orc_init_spins(N, K, int(seed) % (1 << 64), c, out)
+    return out.reshape(K, nwords(N))
 
 
 def det(m):
