@@ -9,7 +9,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_functions():
     src = open(os.path.join(REPO, "include", "nqes.h")).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(nqes_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(nqes_[A-Za-z0-9_]+)\s*\(", src)))
 
 
 def test_library_exports_header_symbols():
