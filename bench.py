@@ -85,7 +85,22 @@ def main():
     ap.add_argument("--cpu-C", type=int, default=64)
     ap.add_argument("--cpu-m", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB mebibytes (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        # a plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
+        # give the counter scale used to turn rocprofv3 readings into HBM bytes
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_each_way": n * 4}))
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
