@@ -32,8 +32,10 @@ COMMON = [
 # FP64 path: no contraction anywhere (host or device) -- canonical operation order
 EXACT = ["-ffp-contract=off"]
 
+# mixed path: FMA contraction allowed in its kernels (FP32 work, FP64 sums)
 SOURCES = [
     ("kernels_fp64.hip", EXACT),
+    ("kernels_mixed.hip", []),
     ("nqes_api.cpp", EXACT),
 ]
 

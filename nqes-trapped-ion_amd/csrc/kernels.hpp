@@ -33,3 +33,46 @@ hipError_t nq_launch_colsum64(const void *X, long long S, long long ncol, int nc
 hipError_t nq_launch_sv64(const void *O, const void *v, long long S, long long P, int nchunk, void *w,
                           void *part, void *out, hipStream_t st);
 hipError_t nq_launch_mathprobe64(const void *in, long long n, void *out, hipStream_t st);
+
+// ---------------------------------------------------------------- mixed precision
+// Arguments of the mixed sampler (G lanes per chain).  T = float or complex float
+// (real / complex weights), D = double or complex double.  O rows have stride Pld
+// (P rounded up to 16 bytes); H_loc and E_loc are complex double in both modes.
+struct MixedSampleArgs {
+    int N, M, NW, L;
+    long long P, Pld;
+    int C, m, B;
+    double h;
+    const double *J;           // N x N
+    const void *chT, *shT;     // K*N*M T: cosh(2w), sinh(2w) [kk][i][j]
+    const void *wT, *bT;       // K*N*M T: w [kk][i][j]; K*M T: b [kk][j]
+    const void *eaP, *eaN;     // K*N D: exp(2a), exp(-2a) [kk][i]
+    const void *aD;            // K*N D: a [kk][i]
+    const void *theta0;        // C*K*K*M T: angles at chain start [c][k][kk][j]
+    const uint32_t *spins0;    // C*K*NW
+    const uint64_t *rng0;      // 2*C pcg64 states after the chain-start draws
+    int tab_lds;               // stage the cosh/sinh tables in LDS
+    void *O;                   // S*Pld T
+    void *Hs;                  // S*K*K complex double
+    void *Es;                  // S complex double
+    uint32_t *spins;           // S*K*NW
+    unsigned long long *accepts; // C
+    unsigned int *flags;       // bit 0: an angle exceeded the FP32 range
+};
+
+int nq_mixed_fused_qv(long long nvec);
+hipError_t nq_launch_tables_mx(bool cplx, int N, int M, int K, int L, const void *W, void *chT, void *shT, void *wT,
+                               void *bT, void *eaP, void *eaN, void *aD, hipStream_t st);
+hipError_t nq_launch_chain_init(int N, int K, int NW, int C, uint64_t seed, uint32_t *spins0, uint64_t *rng0,
+                                hipStream_t st);
+hipError_t nq_launch_theta_gemm(bool cplx, int N, int M, int K, int NW, int C, const uint32_t *spins0,
+                                const void *wT, const void *bT, void *theta0, hipStream_t st);
+hipError_t nq_launch_sample_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st);
+size_t nq_sample_mx_lds(bool cplx, int N, int M, int K, int G, int wpb, bool tab_lds);
+hipError_t nq_launch_moments_mx(bool cplx, const void *O, const void *E, long long S, long long P, long long Pld,
+                                int nchunk, void *po, void *ps, void *pf, void *scratch, void *out_o, void *out_s,
+                                void *out_f, hipStream_t st);
+hipError_t nq_launch_sv_mx(bool cplx, const void *O, const void *v, const void *ov, long long S, long long P,
+                           long long Pld, int nblk, void *part, void *wpart, void *wbuf, void *scratch, void *out,
+                           hipStream_t st);
+int nq_sv_wave_blocks();

@@ -1,0 +1,1064 @@
+// kernels_mixed.hip -- mixed-precision device path of one SR step.
+//
+// Per-hidden-unit work in FP32 (tanh theta, the cosh/sinh update factors, O),
+// per-chain K x K algebra, amplitude ratios and every sum in FP64.  CPLX = false
+// is the real specialisation used when all weights are real.
+//
+// Kernels
+//   k_tables_mx      FP32 cosh(2w), sinh(2w) and w, FP64 exp(+-2a), a, b (once per step)
+//   k_chain_init     per chain: pcg64(seed + c), K*N Bernoulli draws (rbm.cc:99-119)
+//   k_theta_gemm     Theta0[C*K, K*M] = Sigma[C*K, N] W[N, K*M] + b on FP32 MFMA
+//                    (v_mfma_f32_16x16x4f32), the angles at chain start (rbm.cc:135-145)
+//   k_sample_mx      G lanes per chain, 64/G chains per wavefront; Metropolis sweeps
+//                    (rbm.cc:530-552) with tanh theta in LDS updated per accepted flip;
+//                    after each sampled sweep: theta, tanh, Phi and Phi^-1 recomputed from
+//                    the spins (no drift), local energy (rbm.cc:1042-1064) and the O row
+//                    O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k}
+//                    (Cramer's rule applied to rbm.cc:199-239) written to HBM in FP32
+//   k_moments_mx     column sums of O, |O|^2, E conj(O) in FP64 with Neumaier compensation
+//   k_sv_fused_mx    S*v in one streamed pass over O: per row w_s = (O_s - o)^T v, then
+//                    acc += conj(O_s) w_s, FP64 accumulation (rbm.cc:661-684)
+//   k_sv_rowdot_mx / k_sv_colacc_mx   two-pass S*v for rows too long for registers
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "cplx64.hpp"
+#include "kernels.hpp"
+#include "mixed_math.hpp"
+
+using namespace nqm;
+using nq64::cd;
+
+namespace {
+
+template <bool CPLX> struct Ty;
+template <> struct Ty<false> {
+    using T = float;
+    using D = double;
+};
+template <> struct Ty<true> {
+    using T = cf;
+    using D = cd;
+};
+
+__device__ inline double fromW(cd w, double) { return w.re; }
+__device__ inline cd fromW(cd w, cd) { return w; }
+
+// ---------------------------------------------------------------- tables
+template <bool CPLX>
+__global__ void k_tables_mx(int N, int M, int K, int L, const cd *__restrict__ W, void *chT_, void *shT_, void *wT_,
+                            void *bT_, void *eaP_, void *eaN_, void *aD_)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    T *chT = (T *)chT_, *shT = (T *)shT_, *wT = (T *)wT_, *bT = (T *)bT_;
+    D *eaP = (D *)eaP_, *eaN = (D *)eaN_, *aD = (D *)aD_;
+    long long n = (long long)K * N * M;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x) {
+        int j = (int)(idx % M);
+        long long r = idx / M;
+        int i = (int)(r % N);
+        int kk = (int)(r / N);
+        cd w = W[(long long)kk * L + N + M + (long long)i * M + j];
+        cd w2 = nq64::scale(2.0, w);
+        chT[idx] = toT(fromW(nq64::ccosh_(w2), D()));
+        shT[idx] = toT(fromW(nq64::csinh_(w2), D()));
+        wT[idx] = toT(fromW(w, D()));
+        if (i == 0) bT[kk * M + j] = toT(fromW(W[(long long)kk * L + N + j], D()));
+        if (j == 0) {
+            cd a = W[(long long)kk * L + i];
+            cd a2 = nq64::scale(2.0, a);
+            eaP[kk * N + i] = fromW(nq64::cexp_(a2), D());
+            eaN[kk * N + i] = fromW(nq64::cexp_(nq64::neg(a2)), D());
+            aD[kk * N + i] = fromW(a, D());
+        }
+    }
+}
+
+// ---------------------------------------------------------------- chain start
+__global__ void k_chain_init(int N, int K, int NW, int C, uint64_t seed, uint32_t *__restrict__ spins0,
+                             uint64_t *__restrict__ rng0)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= C) return;
+    nq64::pcg rng = nq64::pcg_seed(seed + (uint64_t)c);
+    for (int w = 0; w < K * NW; w++) {
+        uint32_t word = 0;
+        int i0 = (w % NW) * 32;
+        for (int b = 0; b < 32 && i0 + b < N; b++)
+            if (nq64::canonical(nq64::pcg_next(rng)) < 0.5) word |= 1u << b;
+        spins0[(long long)c * K * NW + w] = word;
+    }
+    rng0[2 * c] = rng.lo;
+    rng0[2 * c + 1] = rng.hi;
+}
+
+// Theta0 = Sigma W + b with FP32 MFMA 16x16x4: one wavefront per 16x16 output tile.
+// Rows r = c*K + k (chain c, replica k), columns col over K*M*R (R = 2 for complex:
+// re/im interleaved, so the output is the [c][k][kk][j] complex layout).
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void k_theta_gemm(int N, int M, int K, int NW, int R, long long rows,
+                                                   const uint32_t *__restrict__ spins0,
+                                                   const float *__restrict__ wT, const float *__restrict__ bT,
+                                                   float *__restrict__ theta0)
+{
+    const int lane = threadIdx.x;
+    const long long ncol = (long long)K * M * R;
+    const long long r0 = (long long)blockIdx.x * 16;
+    const long long c0 = (long long)blockIdx.y * 16;
+    const long long arow = r0 + (lane & 15);
+    const long long bcol = c0 + (lane & 15);
+    const int kq = lane >> 4;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    // B column bcol -> (kk, j, part)
+    int bkk = 0, bj = 0, bpart = 0;
+    bool bok = bcol < ncol;
+    if (bok) {
+        long long e = bcol / R;
+        bpart = (int)(bcol % R);
+        bkk = (int)(e / M);
+        bj = (int)(e % M);
+    }
+    const bool aok = arow < rows;
+    const uint32_t *sp = spins0 + (aok ? arow : 0) * NW;  // row = c*K + k -> spins0[c][k][word]
+    for (int i0 = 0; i0 < N; i0 += 4) {
+        int i = i0 + kq;
+        float av = 0.f, bv = 0.f;
+        if (aok && i < N) av = ((sp[i >> 5] >> (i & 31)) & 1u) ? -1.f : 1.f;
+        if (bok && i < N) bv = wT[(((long long)bkk * N + i) * M + bj) * R + bpart];
+        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv, acc, 0, 0, 0);
+    }
+    // lane holds D[4*(lane/16) + q][lane%16]
+    const long long ocol = c0 + (lane & 15);
+    if (ocol < ncol) {
+        long long e = ocol / R;
+        int part = (int)(ocol % R);
+        int kk = (int)(e / M), j = (int)(e % M);
+        float bias = bT[((long long)kk * M + j) * R + part];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            long long orow = r0 + 4 * (lane >> 4) + q;
+            if (orow < rows) theta0[orow * ncol + ocol] = acc[q] + bias;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- sampler
+template <typename T>
+__device__ inline T ld(const T *p)
+{
+    return *p;
+}
+
+__device__ inline float coshT(float x) { return coshf(x); }
+__device__ inline cf coshT(cf z)
+{
+    float sy, cy;
+    __sincosf(z.im, &sy, &cy);
+    float e = __expf(z.re), ie = __expf(-z.re);
+    return mkf(0.5f * (e + ie) * cy, 0.5f * (e - ie) * sy);
+}
+__device__ inline float reT(float x) { return x; }
+__device__ inline float reT(cf z) { return z.re; }
+
+__device__ inline double expD(double x) { return nq64::exp_(x); }
+__device__ inline cd expD(cd z) { return nq64::cexp_(z); }
+
+template <bool CPLX, int K, int G, int JMAX>
+__global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    constexpr int CPW = 64 / G;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = a.N, M = a.M, NW = a.NW, L = a.L;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int g = lane & (G - 1), grp = lane / G;
+    const int cpb = (blockDim.x >> 6) * CPW;
+    const int cib = wv * CPW + grp;
+    const int c_raw = blockIdx.x * cpb + cib;
+    const bool live = c_raw < a.C;
+    const int c = live ? c_raw : a.C - 1;
+    const int jpl = (M + G - 1) / G;
+    const long long KNM = (long long)K * N * M;
+
+    // LDS: [ch tables][sh tables] (if a.tab_lds) [tanh per chain][spins per chain]
+    unsigned char *p = smem;
+    const T *chT = (const T *)a.chT;
+    const T *shT = (const T *)a.shT;
+    if (a.tab_lds) {
+        T *lch = (T *)p;
+        T *lsh = lch + KNM;
+        for (long long q = tid; q < KNM; q += blockDim.x) {
+            lch[q] = chT[q];
+            lsh[q] = shT[q];
+        }
+        chT = lch;
+        shT = lsh;
+        p += 2 * KNM * sizeof(T);
+    }
+    T *tl = (T *)p + (long long)cib * K * K * M;
+    uint32_t *sp = (uint32_t *)((T *)p + (long long)cpb * K * K * M) + cib * K * NW;
+    const T *wT = (const T *)a.wT;
+    const T *bT = (const T *)a.bT;
+    const D *eaP = (const D *)a.eaP;
+    const D *eaN = (const D *)a.eaN;
+    const D *aD = (const D *)a.aD;
+
+    if (g == 0)
+        for (int w = 0; w < K * NW; w++) sp[w] = a.spins0[(long long)c * K * NW + w];
+    nq64::pcg rng;
+    rng.lo = a.rng0[2 * c];
+    rng.hi = a.rng0[2 * c + 1];
+    __syncthreads();
+
+    D phi[K * K];
+    int overflow = 0;
+
+    // theta -> tanh (LDS) and Phi; source: Theta0 (chain start) or the spins
+    auto refresh = [&](bool from_theta0) {
+        D part[K * K];
+#pragma unroll
+        for (int q = 0; q < K * K; q++) part[q] = oneD<D>();
+#pragma unroll
+        for (int q = 0; q < JMAX; q++) {
+            int j = g + G * q;
+            if (q < jpl && j < M) {
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    T th[K];
+                    if (from_theta0) {
+#pragma unroll
+                        for (int k = 0; k < K; k++)
+                            th[k] = ((const T *)a.theta0)[(((long long)c * K + k) * K + kk) * M + j];
+                    } else {
+                        T b = bT[kk * M + j];
+#pragma unroll
+                        for (int k = 0; k < K; k++) th[k] = b;
+                        for (int i = 0; i < N; i++) {
+                            T w = wT[((long long)kk * N + i) * M + j];
+#pragma unroll
+                            for (int k = 0; k < K; k++) {
+                                int bit = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                                th[k] = bit ? subT(th[k], w) : addT(th[k], w);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        if (fabsf(reT(th[k])) > 80.0f) overflow = 1;
+                        tl[(k * K + kk) * M + j] = tanhT(th[k]);
+                        part[k * K + kk] = mulD(part[k * K + kk], toD(coshT(th[k])));
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                D A = zeroD<D>();
+                for (int i = 0; i < N; i++) {
+                    int bit = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                    D ai = aD[kk * N + i];
+                    A = bit ? subD(A, ai) : addD(A, ai);
+                }
+                phi[k * K + kk] = mulD(expD(A), group_prod<G>(part[k * K + kk]));
+            }
+    };
+
+    refresh(true);
+    unsigned long long accepts = 0;
+    const int nsweeps = a.B + a.m;
+    for (int sw = 0; sw < nsweeps; sw++) {
+        D det_phi = det_m<K, D>(phi);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            uint32_t cur = 0;
+            for (int i = 0; i < N; i++) {
+                if ((i & 31) == 0) cur = sp[k * NW + (i >> 5)];
+                const int si = (cur >> (i & 31)) & 1u;
+                T tnew[JMAX][K];
+                T pf[K];
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) pf[kk] = oneT<T>();
+#pragma unroll
+                for (int q = 0; q < JMAX; q++) {
+                    int j = g + G * q;
+                    if (q < jpl && j < M) {
+#pragma unroll
+                        for (int kk = 0; kk < K; kk++) {
+                            long long ti = ((long long)kk * N + i) * M + j;
+                            T chv = chT[ti];
+                            T shv = shT[ti];
+                            if (si) shv = negT(shv);
+                            T th = tl[(k * K + kk) * M + j];
+                            T den = subT(chv, mulT(th, shv));
+                            tnew[q][kk] = divT(subT(mulT(th, chv), shv), den);
+                            pf[kk] = mulT(pf[kk], den);
+                        }
+                    }
+                }
+                D prow[K];
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    D e = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                    D Tp = group_prod<G>(toD(pf[kk]));
+                    prow[kk] = mulD(mulD(phi[k * K + kk], e), Tp);
+                }
+                D phip[K * K];
+#pragma unroll
+                for (int r = 0; r < K; r++)
+#pragma unroll
+                    for (int cc = 0; cc < K; cc++) phip[r * K + cc] = (r == k) ? prow[cc] : phi[r * K + cc];
+                D det_p = det_m<K, D>(phip);
+                double prob = isZeroD(det_phi) ? 1.0 : normD(det_p) / normD(det_phi);
+                double u = nq64::canonical(nq64::pcg_next(rng));
+                const bool acc = u <= (prob < 1.0 ? prob : 1.0);
+                if (acc) {
+                    det_phi = det_p;
+#pragma unroll
+                    for (int q = 0; q < K * K; q++) phi[q] = phip[q];
+#pragma unroll
+                    for (int q = 0; q < JMAX; q++) {
+                        int j = g + G * q;
+                        if (q < jpl && j < M)
+#pragma unroll
+                            for (int kk = 0; kk < K; kk++) tl[(k * K + kk) * M + j] = tnew[q][kk];
+                    }
+                    cur ^= 1u << (i & 31);
+                    accepts++;
+                }
+                if ((i & 31) == 31 || i == N - 1) {
+                    if (g == 0) sp[k * NW + (i >> 5)] = cur;
+                }
+            }
+        }
+        if (sw < a.B) continue;
+        __syncthreads();
+
+        // ---- one sample
+        const long long s = (long long)c * a.m + (sw - a.B);
+        refresh(false);
+        D phi_inv[K * K];
+        inv_m<K, D>(phi, phi_inv);
+
+        // E_zz per replica: lanes take rows i = g, g+G, ...
+        double ez[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double r = 0.0;
+            for (int i = g; i < N; i += G) {
+                int bi = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                for (int j = 0; j < i; j++) {
+                    int bj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
+                    double v = a.J[(long long)i * N + j];
+                    r += (bi ^ bj) ? -v : v;
+                }
+            }
+            ez[k] = 2.0 * group_sum<G>(r);
+        }
+        // sum_i Phi^(i): lanes take spins i = g, g+G, ...; the j loop starts at a
+        // lane-dependent offset to spread LDS banks
+        D H[K * K];
+        {
+            D sx[K * K];
+#pragma unroll
+            for (int q = 0; q < K * K; q++) sx[q] = zeroD<D>();
+#pragma unroll
+            for (int k = 0; k < K; k++)
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    const T *tk = tl + (k * K + kk) * M;
+                    for (int i = g; i < N; i += G) {
+                        const int si = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                        const T *chr = chT + ((long long)kk * N + i) * M;
+                        const T *shr = shT + ((long long)kk * N + i) * M;
+                        D prod = oneD<D>();
+                        T pf = oneT<T>();
+                        int j = g % M;
+                        for (int jj = 0; jj < M; jj++) {
+                            T shv = shr[j];
+                            if (si) shv = negT(shv);
+                            pf = mulT(pf, subT(chr[j], mulT(tk[j], shv)));
+                            if ((jj & 7) == 7) {
+                                prod = mulD(prod, toD(pf));
+                                pf = oneT<T>();
+                            }
+                            if (++j == M) j = 0;
+                        }
+                        prod = mulD(prod, toD(pf));
+                        D e = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                        sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e), prod));
+                    }
+                }
+            D res[K * K];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    D sxs = group_sum<G>(sx[k * K + kk]);
+                    res[k * K + kk] = subD(scaleD(ez[k], phi[k * K + kk]), scaleD(a.h, sxs));
+                }
+            matmul_m<K, D>(phi_inv, res, H);
+        }
+        D energy = zeroD<D>();
+#pragma unroll
+        for (int k = 0; k < K; k++) energy = addD(energy, H[k * K + k]);
+        if (live) {
+            cd *Hs = (cd *)a.Hs;
+            cd *Es = (cd *)a.Es;
+#pragma unroll
+            for (int q = 0; q < K * K; q++)
+                if (g == q) {
+                    cd hv;
+                    if constexpr (CPLX) hv = H[q];
+                    else hv = nq64::mk(H[q], 0.0);
+                    Hs[s * K * K + q] = hv;
+                }
+            if (g == 0) {
+                cd ev;
+                if constexpr (CPLX) ev = energy;
+                else ev = nq64::mk(energy, 0.0);
+                Es[s] = ev;
+            }
+            for (int w = g; w < K * NW; w += G) a.spins[s * K * NW + w] = sp[w];
+        }
+
+        // O row: O_{k,p} = sum_kk c_{k,kk} d_{kk,p}
+        if (live && a.O) {
+            T cT[K * K];
+#pragma unroll
+            for (int k = 0; k < K; k++)
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) cT[k * K + kk] = toT(mulD(phi_inv[k * K + kk], phi[kk * K + k]));
+            T *Orow = (T *)a.O + s * a.Pld;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                T ck[K];
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) ck[kk] = cT[k * K + kk];
+                T *ob = Orow + (long long)k * L;
+                // a block: sum_kk c sigma_i^kk
+                for (int i = g; i < N; i += G) {
+                    T v = zeroT<T>();
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
+                        v = bit ? subT(v, ck[kk]) : addT(v, ck[kk]);
+                    }
+                    ob[i] = v;
+                }
+                // b block and w block: lanes over j
+                for (int j = g; j < M; j += G) {
+                    T ct[K];
+                    T v = zeroT<T>();
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        ct[kk] = mulT(ck[kk], tl[(kk * K + k) * M + j]);
+                        v = addT(v, ct[kk]);
+                    }
+                    ob[N + j] = v;
+                    for (int i = 0; i < N; i++) {
+                        T w = zeroT<T>();
+#pragma unroll
+                        for (int kk = 0; kk < K; kk++) {
+                            int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
+                            w = bit ? subT(w, ct[kk]) : addT(w, ct[kk]);
+                        }
+                        ob[N + M + (long long)i * M + j] = w;
+                    }
+                }
+            }
+            for (long long q = a.P + g; q < a.Pld; q += G) Orow[q] = zeroT<T>();
+        }
+        __syncthreads();
+    }
+    if (live && g == 0) a.accepts[c] = accepts;
+    if (live && overflow) atomicOr(a.flags, 1u);
+}
+
+// ---------------------------------------------------------------- moments
+__device__ inline void neum_add(double &s, double &comp, double x)
+{
+    double t = s + x;
+    if (fabs(s) >= fabs(x)) comp += (s - t) + x;
+    else comp += (x - t) + s;
+    s = t;
+}
+
+template <bool CPLX>
+__global__ void k_moments_mx(const void *O_, const cd *__restrict__ E, long long S, long long P, long long Pld,
+                             int nchunk, cd *__restrict__ po, cd *__restrict__ ps, cd *__restrict__ pf)
+{
+    using T = typename Ty<CPLX>::T;
+    const T *O = (const T *)O_;
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    long long s0 = S * blockIdx.y / nchunk, s1 = S * (blockIdx.y + 1) / nchunk;
+    double o_r = 0, o_rc = 0, o_i = 0, o_ic = 0, s_r = 0, s_rc = 0, f_r = 0, f_rc = 0, f_i = 0, f_ic = 0;
+    for (long long s = s0; s < s1; s++) {
+        cd v;
+        if constexpr (CPLX) v = toD(O[s * Pld + p]);
+        else v = nq64::mk(toD(O[s * Pld + p]), 0.0);
+        const cd e = E[s];
+        neum_add(o_r, o_rc, v.re);
+        neum_add(o_i, o_ic, v.im);
+        neum_add(s_r, s_rc, v.re * v.re + v.im * v.im);
+        // E conj(v)
+        neum_add(f_r, f_rc, e.re * v.re + e.im * v.im);
+        neum_add(f_i, f_ic, e.im * v.re - e.re * v.im);
+    }
+    long long o = (long long)blockIdx.y * P + p;
+    po[o] = nq64::mk(o_r + o_rc, o_i + o_ic);
+    ps[o] = nq64::mk(s_r + s_rc, 0.0);
+    pf[o] = nq64::mk(f_r + f_rc, f_i + f_ic);
+}
+
+// ---------------------------------------------------------------- S*v (fused, one pass)
+// Thread t owns 16-byte vectors u = t + 256 q of each row (4 real or 2 complex
+// columns).  v, acc in FP64 registers; rows double-buffered in registers.
+template <bool CPLX> struct Vec;
+template <> struct Vec<false> {
+    using V = float4;
+    static constexpr int W = 4;
+};
+template <> struct Vec<true> {
+    using V = float4;  // two complex
+    static constexpr int W = 2;
+};
+
+template <bool CPLX>
+__device__ inline void vec_to_D(const float4 &x, typename Ty<CPLX>::D *o)
+{
+    if constexpr (CPLX) {
+        o[0] = nq64::mk(x.x, x.y);
+        o[1] = nq64::mk(x.z, x.w);
+    } else {
+        o[0] = x.x;
+        o[1] = x.y;
+        o[2] = x.z;
+        o[3] = x.w;
+    }
+}
+
+// A block of NWV wavefronts streams a contiguous range of rows in tiles of R rows:
+// thread t owns the 16-byte column vectors u = t + 64*NWV*q (q < QV) of every row.
+// The R partial dot products of a tile are reduced with one barrier, then the same
+// registers feed the R rank-one updates.  The block width NWV is chosen per problem
+// so that QV is small (few registers, two blocks per CU: one loads while the other
+// reduces) and few column slots are idle (4 to 8 wavefronts per block).
+template <bool CPLX, int QV, int R>
+__global__ __launch_bounds__(512) void k_sv_fused_mx(const float4 *__restrict__ O, const void *v_, const void *ov_,
+                                                      long long S, long long nvec, long long rows_per_block,
+                                                      void *part_, void *wsum_)
+{
+    using D = typename Ty<CPLX>::D;
+    constexpr int W = Vec<CPLX>::W;
+    const D *v = (const D *)v_;
+    const D ov = *(const D *)ov_;
+    D *part = (D *)part_;
+    D *wsum = (D *)wsum_;
+    __shared__ D red[2][R][8];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const int nwv = blockDim.x >> 6;
+    const long long stride = blockDim.x;
+    D vr[QV][W], acc[QV][W];
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+        long long u = t + stride * q;
+#pragma unroll
+        for (int e = 0; e < W; e++) {
+            vr[q][e] = (u < nvec) ? v[u * W + e] : zeroD<D>();
+            acc[q][e] = zeroD<D>();
+        }
+    }
+    const long long s0 = (long long)blockIdx.x * rows_per_block;
+    long long s1 = s0 + rows_per_block;
+    if (s1 > S) s1 = S;
+    D wacc = zeroD<D>();
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    int it = 0;
+    for (long long s = s0; s < s1; s += R, it++) {
+        float4 x[R][QV];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int q = 0; q < QV; q++) {
+                long long u = t + stride * q;
+                x[r][q] = (u < nvec && s + r < s1) ? O[(s + r) * nvec + u] : z4;
+            }
+        const int buf = it & 1;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            D dot = zeroD<D>();
+#pragma unroll
+            for (int q = 0; q < QV; q++) {
+                D od[W];
+                vec_to_D<CPLX>(x[r][q], od);
+#pragma unroll
+                for (int e = 0; e < W; e++) dot = addD(dot, mulD(od[e], vr[q][e]));
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) dot = addD(dot, shflxD(dot, off));
+            if (lane == 0) red[buf][r][wv] = dot;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (s + r >= s1) break;
+            D w = red[buf][r][0];
+            for (int k = 1; k < nwv; k++) w = addD(w, red[buf][r][k]);
+            w = subD(w, ov);
+            wacc = addD(wacc, w);
+#pragma unroll
+            for (int q = 0; q < QV; q++) {
+                // convert again rather than keep the FP64 copies live across the barrier
+                float4 xx = x[r][q];
+                asm volatile("" : "+v"(xx.x), "+v"(xx.y), "+v"(xx.z), "+v"(xx.w));
+                D od[W];
+                vec_to_D<CPLX>(xx, od);
+#pragma unroll
+                for (int e = 0; e < W; e++) acc[q][e] = addD(acc[q][e], mulD(w, conjD(od[e])));
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+        long long u = t + stride * q;
+        if (u < nvec)
+#pragma unroll
+            for (int e = 0; e < W; e++) part[(long long)blockIdx.x * nvec * W + u * W + e] = acc[q][e];
+    }
+    if (t == 0) wsum[blockIdx.x] = wacc;
+}
+
+// Wave-per-row S*v: every wavefront streams whole rows on its own (no block
+// barrier); lane l owns the 16-byte column vectors u = l + 64 q (q < QV).  v is
+// staged once per block in LDS (FP64), the accumulator stays in registers, and the
+// row dot product is a 64-lane butterfly.  Used when a row fits 64*QV vectors.
+template <bool CPLX, int QV>
+__global__ __launch_bounds__(256) void k_sv_wave_mx(const float4 *__restrict__ O, const void *v_, const void *ov_,
+                                                    long long S, long long nvec, void *part_, void *wsum_)
+{
+    using D = typename Ty<CPLX>::D;
+    constexpr int W = Vec<CPLX>::W;
+    extern __shared__ __align__(16) unsigned char smem_sv[];
+    D *vl = (D *)smem_sv;
+    const D *v = (const D *)v_;
+    const D ov = *(const D *)ov_;
+    const int t = threadIdx.x, lane = t & 63;
+    for (long long q = t; q < nvec * W; q += blockDim.x) vl[q] = v[q];
+    __syncthreads();
+    const long long gw = ((long long)blockIdx.x * blockDim.x + t) >> 6;
+    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+    D acc[QV][W];
+#pragma unroll
+    for (int q = 0; q < QV; q++)
+#pragma unroll
+        for (int e = 0; e < W; e++) acc[q][e] = zeroD<D>();
+    D wacc = zeroD<D>();
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (long long s = gw; s < S; s += nw) {
+        float4 x[QV];
+        const float4 *row = O + s * nvec;
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            long long u = lane + 64LL * q;
+            x[q] = (u < nvec) ? row[u] : z4;
+        }
+        D dot = zeroD<D>();
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            long long u = lane + 64LL * q;
+            if (u < nvec) {
+                D od[W];
+                vec_to_D<CPLX>(x[q], od);
+#pragma unroll
+                for (int e = 0; e < W; e++) dot = addD(dot, mulD(od[e], vl[u * W + e]));
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) dot = addD(dot, shflxD(dot, off));
+        const D w = subD(dot, ov);
+        wacc = addD(wacc, w);
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            float4 xx = x[q];
+            asm volatile("" : "+v"(xx.x), "+v"(xx.y), "+v"(xx.z), "+v"(xx.w));
+            D od[W];
+            vec_to_D<CPLX>(xx, od);
+#pragma unroll
+            for (int e = 0; e < W; e++) acc[q][e] = addD(acc[q][e], mulD(w, conjD(od[e])));
+        }
+    }
+    // sum the block's wavefront accumulators in wave order through LDS (reusing the
+    // v staging area), one partial row per block
+    const int wv = t >> 6, nwb = blockDim.x >> 6;
+    for (int k = 0; k < nwb; k++) {
+        __syncthreads();
+        if (wv == k) {
+#pragma unroll
+            for (int q = 0; q < QV; q++) {
+                long long u = lane + 64LL * q;
+                if (u < nvec)
+#pragma unroll
+                    for (int e = 0; e < W; e++) vl[u * W + e] = (k == 0) ? acc[q][e] : addD(vl[u * W + e], acc[q][e]);
+            }
+        }
+    }
+    __syncthreads();
+    D *part = (D *)part_ + (long long)blockIdx.x * nvec * W;
+    for (long long q = t; q < nvec * W; q += blockDim.x) part[q] = vl[q];
+    __shared__ D wred[16];
+    if (lane == 0) wred[wv] = wacc;
+    __syncthreads();
+    if (t == 0) {
+        D ws = wred[0];
+        for (int k = 1; k < nwb; k++) ws = addD(ws, wred[k]);
+        ((D *)wsum_)[blockIdx.x] = ws;
+    }
+}
+
+// two-pass fallback: w_s = (O_s - o)^T v (one wave per row), then column sums
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_sv_rowdot_mx(const void *O_, const void *v_, const void *ov_, long long S,
+                                                      long long P, long long Pld, void *w_)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    const T *O = (const T *)O_;
+    const D *v = (const D *)v_;
+    const D ov = *(const D *)ov_;
+    D *w = (D *)w_;
+    const int lane = threadIdx.x & 63;
+    const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+    for (long long s = wave; s < S; s += nwaves) {
+        const T *row = O + s * Pld;
+        D acc = zeroD<D>();
+        for (long long p = lane; p < P; p += 64) acc = addD(acc, mulD(toD(row[p]), v[p]));
+        acc = group_sum<64>(acc);
+        if (lane == 0) w[s] = subD(acc, ov);
+    }
+}
+
+template <bool CPLX>
+__global__ void k_sv_colacc_mx(const void *O_, const void *w_, long long S, long long P, long long Pld, int nchunk,
+                               void *part_, void *wpart_)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    const T *O = (const T *)O_;
+    const D *w = (const D *)w_;
+    D *part = (D *)part_;
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long s0 = S * blockIdx.y / nchunk, s1 = S * (blockIdx.y + 1) / nchunk;
+    if (p < P) {
+        D acc = zeroD<D>();
+        for (long long s = s0; s < s1; s++) acc = addD(acc, mulD(w[s], conjD(toD(O[s * Pld + p]))));
+        part[(long long)blockIdx.y * P + p] = acc;
+    }
+    if (p == 0) {
+        D ws = zeroD<D>();
+        for (long long s = s0; s < s1; s++) ws = addD(ws, w[s]);
+        ((D *)wpart_)[blockIdx.y] = ws;
+    }
+}
+
+// out[col] = sum over chunks of part[chunk * stride + col], col < ncol, chunks in order
+template <typename D>
+__global__ void k_chunksum_mx(const D *__restrict__ part, long long ncol, long long stride, int nchunk,
+                              D *__restrict__ out)
+{
+    long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    D acc = part[col];
+    for (int ch = 1; ch < nchunk; ch++) acc = addD(acc, part[(long long)ch * stride + col]);
+    out[col] = acc;
+}
+
+// partial sums over groups of CG consecutive chunks: grid (col tiles, groups)
+template <typename D>
+__global__ void k_chunkgroup_mx(const D *__restrict__ part, long long ncol, long long stride, int nchunk, int cg,
+                                D *__restrict__ gout)
+{
+    long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    int c0 = blockIdx.y * cg, c1 = c0 + cg;
+    if (c1 > nchunk) c1 = nchunk;
+    D acc = zeroD<D>();
+    for (int ch = c0; ch < c1; ch++) acc = addD(acc, part[(long long)ch * stride + col]);
+    gout[(long long)blockIdx.y * ncol + col] = acc;
+}
+
+// out[col] = sum_chunks part[...], deterministic, parallel over column tiles and
+// chunk groups; scratch holds ngroups * ncol values
+template <typename D>
+void chunk_reduce(const D *part, long long ncol, long long stride, int nchunk, D *scratch, D *out, hipStream_t st)
+{
+    const int cg = 32;
+    int ngroups = (nchunk + cg - 1) / cg;
+    if (ngroups <= 1) {
+        hipLaunchKernelGGL(k_chunksum_mx<D>, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, st, part, ncol, stride,
+                           nchunk, out);
+        return;
+    }
+    hipLaunchKernelGGL(k_chunkgroup_mx<D>, dim3((unsigned)((ncol + 63) / 64), (unsigned)ngroups), dim3(64), 0, st,
+                       part, ncol, stride, nchunk, cg, scratch);
+    hipLaunchKernelGGL(k_chunksum_mx<D>, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, st, (const D *)scratch,
+                       ncol, ncol, ngroups, out);
+}
+
+// ---------------------------------------------------------------- launchers
+template <bool CPLX, int K, int G>
+hipError_t launch_sample_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
+{
+    using T = typename Ty<CPLX>::T;
+    const int cpb = wpb * (64 / G);
+    size_t lds = (size_t)cpb * K * K * a.M * sizeof(T) + (size_t)cpb * K * a.NW * 4;
+    if (a.tab_lds) lds += 2 * (size_t)K * a.N * a.M * sizeof(T);
+    int blocks = (a.C + cpb - 1) / cpb;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_sample_mx<CPLX, K, G, 8>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_sample_mx<CPLX, K, G, 8>), dim3(blocks), dim3(64 * wpb), lds, st, a);
+    return hipGetLastError();
+}
+
+template <bool CPLX, int K>
+hipError_t launch_sample_k(const MixedSampleArgs &a, hipStream_t st, int G, int wpb)
+{
+    switch (G) {
+    case 16: return launch_sample_g<CPLX, K, 16>(a, st, wpb);
+    case 32: return launch_sample_g<CPLX, K, 32>(a, st, wpb);
+    case 64: return launch_sample_g<CPLX, K, 64>(a, st, wpb);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <bool CPLX>
+hipError_t launch_sample_c(int K, const MixedSampleArgs &a, hipStream_t st, int G, int wpb)
+{
+    switch (K) {
+    case 1: return launch_sample_k<CPLX, 1>(a, st, G, wpb);
+    case 2: return launch_sample_k<CPLX, 2>(a, st, G, wpb);
+    case 3: return launch_sample_k<CPLX, 3>(a, st, G, wpb);
+    case 4: return launch_sample_k<CPLX, 4>(a, st, G, wpb);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <bool CPLX, int QV>
+hipError_t launch_fused(const void *O, const void *v, const void *ov, long long S, long long nvec, int nblk, int nwv,
+                        void *part, void *wsum, hipStream_t st)
+{
+    constexpr int R = (QV <= 2) ? 8 : 4;
+    long long rpb = (S + nblk - 1) / nblk;
+    hipLaunchKernelGGL((k_sv_fused_mx<CPLX, QV, R>), dim3(nblk), dim3(64 * nwv), 0, st, (const float4 *)O, v, ov, S,
+                       nvec, rpb, part, wsum);
+    return hipGetLastError();
+}
+
+template <bool CPLX>
+hipError_t launch_fused_q(int qv, const void *O, const void *v, const void *ov, long long S, long long nvec,
+                          int nblk, int nwv, void *part, void *wsum, hipStream_t st)
+{
+    switch (qv) {
+    case 1: return launch_fused<CPLX, 1>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
+    case 2: return launch_fused<CPLX, 2>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
+    case 3: return launch_fused<CPLX, 3>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
+    case 4: return launch_fused<CPLX, 4>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
+    case 5: return launch_fused<CPLX, 5>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
+    case 6: return launch_fused<CPLX, 6>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
+    case 7: return launch_fused<CPLX, 7>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
+    case 8: return launch_fused<CPLX, 8>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+
+// block width (wavefronts) and column vectors per thread of the fused S*v; qv = 0
+// selects the two-pass form
+void nq_mixed_fused_shape(long long nvec, int *nwv_out, int *qv_out)
+{
+    int best_nw = 0, best_q = 0;
+    double best = -1.0;
+    for (int nw = 4; nw <= 8; nw++) {
+        long long q = (nvec + 64LL * nw - 1) / (64LL * nw);
+        if (q > 8) continue;
+        double util = (double)nvec / (64.0 * nw * q);
+        double score = util - 0.01 * q;
+        if (score > best) {
+            best = score;
+            best_nw = nw;
+            best_q = (int)q;
+        }
+    }
+    *nwv_out = best_nw;
+    *qv_out = best_q;
+}
+
+int nq_mixed_fused_qv(long long nvec)
+{
+    int nw, q;
+    nq_mixed_fused_shape(nvec, &nw, &q);
+    return q;
+}
+
+hipError_t nq_launch_tables_mx(bool cplx, int N, int M, int K, int L, const void *W, void *chT, void *shT, void *wT,
+                               void *bT, void *eaP, void *eaN, void *aD, hipStream_t st)
+{
+    long long n = (long long)K * N * M;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 65536) blocks = 65536;
+    if (cplx)
+        hipLaunchKernelGGL(k_tables_mx<true>, dim3(blocks), dim3(256), 0, st, N, M, K, L, (const cd *)W, chT, shT, wT,
+                           bT, eaP, eaN, aD);
+    else
+        hipLaunchKernelGGL(k_tables_mx<false>, dim3(blocks), dim3(256), 0, st, N, M, K, L, (const cd *)W, chT, shT,
+                           wT, bT, eaP, eaN, aD);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_chain_init(int N, int K, int NW, int C, uint64_t seed, uint32_t *spins0, uint64_t *rng0,
+                                hipStream_t st)
+{
+    hipLaunchKernelGGL(k_chain_init, dim3((C + 127) / 128), dim3(128), 0, st, N, K, NW, C, seed, spins0, rng0);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_theta_gemm(bool cplx, int N, int M, int K, int NW, int C, const uint32_t *spins0,
+                                const void *wT, const void *bT, void *theta0, hipStream_t st)
+{
+    int R = cplx ? 2 : 1;
+    long long rows = (long long)C * K;
+    long long ncol = (long long)K * M * R;
+    dim3 grid((unsigned)((rows + 15) / 16), (unsigned)((ncol + 15) / 16));
+    hipLaunchKernelGGL(k_theta_gemm, grid, dim3(64), 0, st, N, M, K, NW, R, rows, spins0, (const float *)wT,
+                       (const float *)bT, (float *)theta0);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_sample_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st)
+{
+    return cplx ? launch_sample_c<true>(K, a, st, G, wpb) : launch_sample_c<false>(K, a, st, G, wpb);
+}
+
+size_t nq_sample_mx_lds(bool cplx, int N, int M, int K, int G, int wpb, bool tab_lds)
+{
+    size_t ts = cplx ? 8 : 4;
+    int NW = (N + 31) / 32;
+    int cpb = wpb * (64 / G);
+    size_t lds = (size_t)cpb * K * K * M * ts + (size_t)cpb * K * NW * 4;
+    if (tab_lds) lds += 2 * (size_t)K * N * M * ts;
+    return lds;
+}
+
+hipError_t nq_launch_moments_mx(bool cplx, const void *O, const void *E, long long S, long long P, long long Pld,
+                                int nchunk, void *po, void *ps, void *pf, void *scratch, void *out_o, void *out_s,
+                                void *out_f, hipStream_t st)
+{
+    dim3 g((unsigned)((P + 255) / 256), (unsigned)nchunk);
+    if (cplx)
+        hipLaunchKernelGGL(k_moments_mx<true>, g, dim3(256), 0, st, O, (const cd *)E, S, P, Pld, nchunk, (cd *)po,
+                           (cd *)ps, (cd *)pf);
+    else
+        hipLaunchKernelGGL(k_moments_mx<false>, g, dim3(256), 0, st, O, (const cd *)E, S, P, Pld, nchunk, (cd *)po,
+                           (cd *)ps, (cd *)pf);
+    // the partial buffers' first chunk rows double as scratch once reduced in order
+    chunk_reduce<cd>((const cd *)po, P, P, nchunk, (cd *)scratch, (cd *)out_o, st);
+    chunk_reduce<cd>((const cd *)ps, P, P, nchunk, (cd *)scratch, (cd *)out_s, st);
+    chunk_reduce<cd>((const cd *)pf, P, P, nchunk, (cd *)scratch, (cd *)out_f, st);
+    return hipGetLastError();
+}
+
+// S*v raw sums: out[p] = sum_s conj(O_sp) w_s (P entries of D), out[P] = sum_s w_s,
+// w_s = O_s^T v - ov.  v, ov, out are D (double or complex double).  Picks the
+// wave-per-row kernel when a row fits 20 vectors per lane, else the tiled block
+// kernel, else two passes.  scratch: >= (nblk/32 + 2) * (Pld + 1) D.
+template <bool CPLX, int QV>
+hipError_t launch_wave(const void *O, const void *v, const void *ov, long long S, long long nvec, int nblk, void *part,
+                       void *wpart, hipStream_t st)
+{
+    using D = typename Ty<CPLX>::D;
+    size_t lds = (size_t)nvec * Vec<CPLX>::W * sizeof(D);
+    hipLaunchKernelGGL((k_sv_wave_mx<CPLX, QV>), dim3(nblk), dim3(256), lds, st, (const float4 *)O, v, ov, S, nvec,
+                       part, wpart);
+    return hipGetLastError();
+}
+
+template <bool CPLX>
+hipError_t launch_wave_q(int qv, const void *O, const void *v, const void *ov, long long S, long long nvec, int nblk,
+                         void *part, void *wpart, hipStream_t st)
+{
+    switch (qv) {
+#define NQ_WQ(q)                                                                                                  \
+    case q: return launch_wave<CPLX, q>(O, v, ov, S, nvec, nblk, part, wpart, st);
+        NQ_WQ(1) NQ_WQ(2) NQ_WQ(3) NQ_WQ(4) NQ_WQ(5) NQ_WQ(6) NQ_WQ(7) NQ_WQ(8) NQ_WQ(9) NQ_WQ(10)
+        NQ_WQ(11) NQ_WQ(12) NQ_WQ(13) NQ_WQ(14) NQ_WQ(15) NQ_WQ(16) NQ_WQ(17) NQ_WQ(18) NQ_WQ(19) NQ_WQ(20)
+#undef NQ_WQ
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int nq_sv_wave_blocks()
+{
+    const char *e = getenv("NQES_SV_BLOCKS");
+    int b = (e && *e) ? atoi(e) : 256;
+    return b > 0 ? b : 256;
+}
+
+hipError_t nq_launch_sv_mx(bool cplx, const void *O, const void *v, const void *ov, long long S, long long P,
+                           long long Pld, int nblk, void *part, void *wpart, void *wbuf, void *scratch, void *out,
+                           hipStream_t st)
+{
+    const int W = cplx ? 2 : 4;
+    const long long nvec = Pld / W;
+    const long long qw = (nvec + 63) / 64;
+    hipError_t e = hipSuccess;
+    auto reduce = [&](long long ncol, long long stride) {
+        if (cplx) {
+            chunk_reduce<cd>((const cd *)part, ncol, stride, nblk, (cd *)scratch, (cd *)out, st);
+            chunk_reduce<cd>((const cd *)wpart, 1, 1, nblk, (cd *)scratch + (long long)(nblk / 32 + 1) * ncol,
+                             (cd *)out + P, st);
+        } else {
+            chunk_reduce<double>((const double *)part, ncol, stride, nblk, (double *)scratch, (double *)out, st);
+            chunk_reduce<double>((const double *)wpart, 1, 1, nblk,
+                                 (double *)scratch + (long long)(nblk / 32 + 1) * ncol, (double *)out + P, st);
+        }
+        return hipGetLastError();
+    };
+    if (qw <= 20) {
+        e = cplx ? launch_wave_q<true>((int)qw, O, v, ov, S, nvec, nblk, part, wpart, st)
+                 : launch_wave_q<false>((int)qw, O, v, ov, S, nvec, nblk, part, wpart, st);
+        if (e != hipSuccess) return e;
+        return reduce(P, Pld);
+    }
+    int nwv = 0, qv = 0;
+    nq_mixed_fused_shape(nvec, &nwv, &qv);
+    if (qv) {
+        e = cplx ? launch_fused_q<true>(qv, O, v, ov, S, nvec, nblk, nwv, part, wpart, st)
+                 : launch_fused_q<false>(qv, O, v, ov, S, nvec, nblk, nwv, part, wpart, st);
+        if (e != hipSuccess) return e;
+        return reduce(P, Pld);
+    }
+    long long blocks = (S + 3) / 4;
+    if (blocks > 16384) blocks = 16384;
+    dim3 g((unsigned)((P + 255) / 256), (unsigned)nblk);
+    if (cplx) {
+        hipLaunchKernelGGL(k_sv_rowdot_mx<true>, dim3((unsigned)blocks), dim3(256), 0, st, O, v, ov, S, P, Pld, wbuf);
+        hipLaunchKernelGGL(k_sv_colacc_mx<true>, g, dim3(256), 0, st, O, wbuf, S, P, Pld, nblk, part, wpart);
+    } else {
+        hipLaunchKernelGGL(k_sv_rowdot_mx<false>, dim3((unsigned)blocks), dim3(256), 0, st, O, v, ov, S, P, Pld,
+                           wbuf);
+        hipLaunchKernelGGL(k_sv_colacc_mx<false>, g, dim3(256), 0, st, O, wbuf, S, P, Pld, nblk, part, wpart);
+    }
+    return reduce(P, P);
+}

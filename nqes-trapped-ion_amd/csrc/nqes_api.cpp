@@ -1,20 +1,24 @@
 // nqes_api.cpp -- host side of libnqes: the C ABI of include/nqes.h.
 //
 // One SR step (reference: monte_carlo, code/rbm.cc:564-720):
-//   1. weights -> device, per-step tables (k_tables64)
-//   2. sampler kernel: one wavefront per chain, writes O (S x P) to HBM, H_loc, E_loc
+//   1. weights -> device, per-step tables
+//   2. sampler kernel: Metropolis chains, H_loc, E_loc and the O matrix (S x P) in HBM
 //   3. deterministic column reductions -> sums of O, |O|^2, E O*, H_loc, E_loc
 //   4. host normalisation (rbm.cc:641-650)
-//   5. MINRES-QLP on the host (FP64 scalars, length-P vectors), whose operator
-//      is the two streamed S*v passes over O on the device (rbm.cc:661-684)
+//   5. MINRES-QLP on the host (FP64 scalars, length-P vectors) whose operator is the
+//      streamed S*v over O on the device (rbm.cc:661-684)
 //   6. w -= lr * x (rbm.cc:699-717)
-// Host arithmetic is compiled with -ffp-contract=off as the device code is.
+// Precision NQES_PREC_FP64 follows the canonical FP64 order (bit-exact spins, H_loc,
+// O); NQES_PREC_MIXED keeps per-hidden-unit work and O in FP32 and all sums in FP64,
+// and specialises to real arithmetic when every weight is real.
+// Host arithmetic is compiled with -ffp-contract=off as the FP64 device code is.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -64,6 +68,12 @@ int choose_chunks(long long S, long long ncol, int cols_per_block)
     return (int)nch;
 }
 
+int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
 } // namespace
 
 struct nqes_ctx {
@@ -71,6 +81,7 @@ struct nqes_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::string err;
+    size_t lds_max = 65536;
 
     // problem
     bool has_problem = false;
@@ -78,18 +89,22 @@ struct nqes_ctx {
     long long P = 0;
     double h = 0;
     DevBuf dJ, dW, dchT, dshT, deaP, deaN;
+    DevBuf dwT, dbT, daD, dspins0, drng0, dtheta0, dflags;  // mixed path
 
     // sample state
     bool sampled = false;
+    bool cplx = true;          // mixed path: complex arithmetic (false: all weights real)
     int C = 0, m = 0, B = 0;
-    long long S = 0;
+    long long S = 0, Pld = 0;
+    int G = 64, wpb = 1, tab_lds = 0;
     DevBuf dO, dHs, dEs, dspins, dacc;
     DevBuf dmpart_o, dmpart_s, dmpart_f, dmo, dms, dmf, dhpart, dhsum, depart, desum;
-    DevBuf dv, dw, dsvpart, dsvout;
+    DevBuf dv, dov, dw, dsvpart, dsvwpart, dsvout, dsvscratch, dmscratch;
     int mchunks = 1, svchunks = 1;
     std::vector<cd> o, skk, f, emat;
     cd eloc{0, 0};
     std::vector<cd> svraw;
+    std::vector<double> hv;
     uint64_t sv_calls = 0;
     double sv_ms = 0;
     int nonfinite = 0;
@@ -123,9 +138,35 @@ static double event_ms(hipEvent_t a, hipEvent_t b)
     return ms;
 }
 
+// bytes of one O element and the padded row length (rows start on 16 bytes)
+static size_t o_elem(int prec, bool cplx) { return prec == NQES_PREC_FP64 ? 16 : (cplx ? 8 : 4); }
+static long long o_ld(long long P, int prec, bool cplx)
+{
+    size_t e = o_elem(prec, cplx);
+    long long per16 = 16 / (long long)e;
+    return (P + per16 - 1) / per16 * per16;
+}
+
+// lane-group size and waves per block of the mixed sampler
+static void mixed_shape(nqes_ctx *c, bool cplx)
+{
+    int G = 16;
+    while (G < 64 && (c->M + G - 1) / G > 8) G *= 2;
+    int g_env = env_int("NQES_G", 0);
+    if ((g_env == 16 || g_env == 32 || g_env == 64) && (c->M + g_env - 1) / g_env <= 8) G = g_env;
+    int wpb = env_int("NQES_WPB", 4);
+    if (wpb < 1 || wpb > 4) wpb = 4;
+    int tab = env_int("NQES_TABLDS", 1);
+    while (wpb > 1 && nq_sample_mx_lds(cplx, c->N, c->M, c->K, G, wpb, false) > c->lds_max) wpb--;
+    if (tab && nq_sample_mx_lds(cplx, c->N, c->M, c->K, G, wpb, true) > c->lds_max) tab = 0;
+    c->G = G;
+    c->wpb = wpb;
+    c->tab_lds = tab;
+}
+
 extern "C" {
 
-const char *nqes_version(void) { return "nqes-mi355x 0.1 (gfx950)"; }
+const char *nqes_version(void) { return "nqes-mi355x 0.2 (gfx950)"; }
 
 nqes_ctx *nqes_create(int device)
 {
@@ -143,6 +184,8 @@ nqes_ctx *nqes_create(int device)
             delete c;
             return nullptr;
         }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->lds_max = prop.sharedMemPerBlock;
     return c;
 }
 
@@ -165,14 +208,15 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
     long long P = (long long)K * (N + M + (long long)N * M);
     long long S = (long long)C * m;
     long long NW = (N + 31) / 32;
+    bool cplx = true;
     long long b = 0;
-    b += S * P * 16;                  // O
+    b += S * o_ld(P, precision, cplx) * (long long)o_elem(precision, cplx);  // O
     b += S * (long long)K * K * 16;   // H_loc
     b += S * 16 * 2;                  // E_loc, w
     b += S * K * NW * 4;              // spins
-    b += (long long)K * N * M * 32;   // tables
+    b += (long long)K * N * M * 48;   // tables
+    b += (long long)C * K * K * M * 8; // chain-start angles
     b += 3 * 4096LL * P * 16;         // chunk partials (upper bound)
-    (void)precision;
     return b;
 }
 
@@ -186,13 +230,17 @@ int nqes_problem_init(nqes_ctx *c, int N, int M, int K, double h, const double *
     if (M % N != 0) return c->fail(NQES_EINVAL, "M=%d must be a multiple of N=%d (rbm.cc:148)", M, N);
     if (!J) return c->fail(NQES_EINVAL, "J is NULL");
     if (model != NQES_MODEL_TRAPPEDION) return c->fail(NQES_EINVAL, "unsupported model %d", model);
-    if (precision != NQES_PREC_FP64) return c->fail(NQES_EINVAL, "unsupported precision %d", precision);
-    hipDeviceProp_t prop;
-    HIPCHK(c, hipGetDeviceProperties(&prop, c->device), "hipGetDeviceProperties");
-    size_t lds = (size_t)16 * K * K * M + 4 * (size_t)K * ((N + 31) / 32);
-    if (lds > prop.sharedMemPerBlock)
-        return c->fail(NQES_EINVAL, "K*K*M=%d hidden-unit angles (%zu B) exceed the %zu B of LDS per block", K * K * M,
-                       lds, (size_t)prop.sharedMemPerBlock);
+    if (precision != NQES_PREC_FP64 && precision != NQES_PREC_MIXED)
+        return c->fail(NQES_EINVAL, "unsupported precision %d", precision);
+    if (precision == NQES_PREC_FP64) {
+        size_t lds = (size_t)16 * K * K * M + 4 * (size_t)K * ((N + 31) / 32);
+        if (lds > c->lds_max)
+            return c->fail(NQES_EINVAL, "K*K*M=%d hidden-unit angles (%zu B) exceed the %zu B of LDS per block",
+                           K * K * M, lds, c->lds_max);
+    } else {
+        if (nq_sample_mx_lds(true, N, M, K, 64, 1, false) > c->lds_max)
+            return c->fail(NQES_EINVAL, "K*K*M=%d hidden-unit angles exceed the LDS per block", K * K * M);
+    }
     c->N = N;
     c->M = M;
     c->K = K;
@@ -209,8 +257,108 @@ int nqes_problem_init(nqes_ctx *c, int N, int M, int K, double h, const double *
     HIPCHK(c, c->dshT.ensure((size_t)16 * K * N * M), "alloc tables");
     HIPCHK(c, c->deaP.ensure((size_t)16 * K * N), "alloc tables");
     HIPCHK(c, c->deaN.ensure((size_t)16 * K * N), "alloc tables");
+    if (precision == NQES_PREC_MIXED) {
+        HIPCHK(c, c->dwT.ensure((size_t)8 * K * N * M), "alloc tables");
+        HIPCHK(c, c->dbT.ensure((size_t)8 * K * M), "alloc tables");
+        HIPCHK(c, c->daD.ensure((size_t)16 * K * N), "alloc tables");
+        HIPCHK(c, c->dflags.ensure(16), "alloc flags");
+    }
     c->has_problem = true;
     c->sampled = false;
+    return NQES_OK;
+}
+
+static int sample_fp64(nqes_ctx *c, uint64_t seed, int C, int m, int B)
+{
+    const long long S = (long long)C * m;
+    HIPCHK(c, nq_launch_tables64(c->N, c->M, c->K, c->L, c->dW.p, c->dchT.p, c->dshT.p, c->deaP.p, c->deaN.p, c->stream),
+           "tables kernel");
+    Fp64SampleArgs a;
+    a.N = c->N;
+    a.M = c->M;
+    a.NW = c->NW;
+    a.L = c->L;
+    a.P = c->P;
+    a.C = C;
+    a.m = m;
+    a.B = B;
+    a.seed = seed;
+    a.h = c->h;
+    a.J = static_cast<const double *>(c->dJ.p);
+    a.W = c->dW.p;
+    a.chT = c->dchT.p;
+    a.shT = c->dshT.p;
+    a.eaP = c->deaP.p;
+    a.eaN = c->deaN.p;
+    a.O = c->dO.p;
+    a.Hs = c->dHs.p;
+    a.Es = c->dEs.p;
+    a.spins = static_cast<uint32_t *>(c->dspins.p);
+    a.accepts = static_cast<unsigned long long *>(c->dacc.p);
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream), "event");
+    HIPCHK(c, nq_launch_sample64(c->K, a, c->stream), "sample kernel");
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
+    HIPCHK(c, nq_launch_moments64(c->dO.p, c->dEs.p, S, c->P, c->mchunks, c->dmpart_o.p, c->dmpart_s.p,
+                                  c->dmpart_f.p, c->dmo.p, c->dms.p, c->dmf.p, c->stream),
+           "moments kernel");
+    return NQES_OK;
+}
+
+static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
+{
+    const long long S = (long long)C * m;
+    const bool cplx = c->cplx;
+    HIPCHK(c, c->dspins0.ensure((size_t)C * c->K * c->NW * 4), "alloc spins0");
+    HIPCHK(c, c->drng0.ensure((size_t)C * 16), "alloc rng0");
+    HIPCHK(c, c->dtheta0.ensure((size_t)C * c->K * c->K * c->M * (cplx ? 8 : 4)), "alloc theta0");
+    HIPCHK(c, hipMemsetAsync(c->dflags.p, 0, 16, c->stream), "clear flags");
+    HIPCHK(c, nq_launch_tables_mx(cplx, c->N, c->M, c->K, c->L, c->dW.p, c->dchT.p, c->dshT.p, c->dwT.p, c->dbT.p,
+                                  c->deaP.p, c->deaN.p, c->daD.p, c->stream),
+           "tables kernel");
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream), "event");
+    HIPCHK(c, nq_launch_chain_init(c->N, c->K, c->NW, C, seed, (uint32_t *)c->dspins0.p, (uint64_t *)c->drng0.p,
+                                   c->stream),
+           "chain init kernel");
+    HIPCHK(c, nq_launch_theta_gemm(cplx, c->N, c->M, c->K, c->NW, C, (const uint32_t *)c->dspins0.p, c->dwT.p,
+                                   c->dbT.p, c->dtheta0.p, c->stream),
+           "theta GEMM kernel");
+    mixed_shape(c, cplx);
+    MixedSampleArgs a;
+    a.N = c->N;
+    a.M = c->M;
+    a.NW = c->NW;
+    a.L = c->L;
+    a.P = c->P;
+    a.Pld = c->Pld;
+    a.C = C;
+    a.m = m;
+    a.B = B;
+    a.h = c->h;
+    a.J = static_cast<const double *>(c->dJ.p);
+    a.chT = c->dchT.p;
+    a.shT = c->dshT.p;
+    a.wT = c->dwT.p;
+    a.bT = c->dbT.p;
+    a.eaP = c->deaP.p;
+    a.eaN = c->deaN.p;
+    a.aD = c->daD.p;
+    a.theta0 = c->dtheta0.p;
+    a.spins0 = (const uint32_t *)c->dspins0.p;
+    a.rng0 = (const uint64_t *)c->drng0.p;
+    a.tab_lds = c->tab_lds;
+    a.O = c->dO.p;
+    a.Hs = c->dHs.p;
+    a.Es = c->dEs.p;
+    a.spins = static_cast<uint32_t *>(c->dspins.p);
+    a.accepts = static_cast<unsigned long long *>(c->dacc.p);
+    a.flags = static_cast<unsigned int *>(c->dflags.p);
+    HIPCHK(c, nq_launch_sample_mx(cplx, c->K, a, c->G, c->wpb, c->stream), "mixed sample kernel");
+    HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
+    HIPCHK(c, c->dmscratch.ensure((size_t)(c->mchunks / 32 + 2) * c->P * 16), "alloc moments scratch");
+    HIPCHK(c, nq_launch_moments_mx(cplx, c->dO.p, c->dEs.p, S, c->P, c->Pld, c->mchunks, c->dmpart_o.p,
+                                   c->dmpart_s.p, c->dmpart_f.p, c->dmscratch.p, c->dmo.p, c->dms.p, c->dmf.p,
+                                   c->stream),
+           "moments kernel");
     return NQES_OK;
 }
 
@@ -224,12 +372,24 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     c->sampled = false;
     const int K = c->K;
     const long long P = c->P, S = (long long)C * m;
+    bool cplx = true;
+    if (c->prec == NQES_PREC_MIXED) {
+        cplx = false;
+        for (long long q = 0; q < P; q++)
+            if (W[2 * q + 1] != 0.0) {
+                cplx = true;
+                break;
+            }
+    }
+    c->cplx = cplx;
+    c->Pld = o_ld(P, c->prec, cplx);
+    const size_t obytes = (size_t)S * c->Pld * o_elem(c->prec, cplx);
     size_t freeb = 0, totalb = 0;
     HIPCHK(c, hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
-    long long need = (long long)(S * P * 16) - (long long)c->dO.bytes;
+    long long need = (long long)obytes - (long long)c->dO.bytes;
     if (need > 0 && (size_t)need + (size_t)(1ull << 30) > freeb)
-        return c->fail(NQES_ENOMEM, "O needs %.2f GB, %.2f GB free", S * P * 16 / 1e9, freeb / 1e9);
-    HIPCHK(c, c->dO.ensure((size_t)(S * P * 16)), "alloc O");
+        return c->fail(NQES_ENOMEM, "O needs %.2f GB, %.2f GB free", obytes / 1e9, freeb / 1e9);
+    HIPCHK(c, c->dO.ensure(obytes), "alloc O");
     HIPCHK(c, c->dHs.ensure((size_t)(S * K * K * 16)), "alloc H");
     HIPCHK(c, c->dEs.ensure((size_t)(S * 16)), "alloc E");
     HIPCHK(c, c->dspins.ensure((size_t)(S * K * c->NW * 4)), "alloc spins");
@@ -249,46 +409,21 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     HIPCHK(c, c->desum.ensure(16), "alloc partials");
 
     HIPCHK(c, hipMemcpyAsync(c->dW.p, W, 16 * P, hipMemcpyHostToDevice, c->stream), "copy W");
-    HIPCHK(c, nq_launch_tables64(c->N, c->M, K, c->L, c->dW.p, c->dchT.p, c->dshT.p, c->deaP.p, c->deaN.p, c->stream),
-           "tables kernel");
-    Fp64SampleArgs a;
-    a.N = c->N;
-    a.M = c->M;
-    a.NW = c->NW;
-    a.L = c->L;
-    a.P = P;
-    a.C = C;
-    a.m = m;
-    a.B = B;
-    a.seed = seed;
-    a.h = c->h;
-    a.J = static_cast<const double *>(c->dJ.p);
-    a.W = c->dW.p;
-    a.chT = c->dchT.p;
-    a.shT = c->dshT.p;
-    a.eaP = c->deaP.p;
-    a.eaN = c->deaN.p;
-    a.O = c->dO.p;
-    a.Hs = c->dHs.p;
-    a.Es = c->dEs.p;
-    a.spins = static_cast<uint32_t *>(c->dspins.p);
-    a.accepts = static_cast<unsigned long long *>(c->dacc.p);
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream), "event");
-    HIPCHK(c, nq_launch_sample64(K, a, c->stream), "sample kernel");
-    HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
-    HIPCHK(c, nq_launch_moments64(c->dO.p, c->dEs.p, S, P, c->mchunks, c->dmpart_o.p, c->dmpart_s.p, c->dmpart_f.p,
-                                  c->dmo.p, c->dms.p, c->dmf.p, c->stream),
-           "moments kernel");
+    int st = (c->prec == NQES_PREC_FP64) ? sample_fp64(c, seed, C, m, B) : sample_mixed(c, seed, C, m, B);
+    if (st != NQES_OK) return st;
     HIPCHK(c, nq_launch_colsum64(c->dHs.p, S, K * K, hch, c->dhpart.p, c->dhsum.p, c->stream), "colsum kernel");
     HIPCHK(c, nq_launch_colsum64(c->dEs.p, S, 1, ech, c->depart.p, c->desum.p, c->stream), "colsum kernel");
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
 
     std::vector<cd> so(P), ss(P), sf(P), sh(K * K), se(1);
+    unsigned int flags = 0;
     HIPCHK(c, hipMemcpyAsync(so.data(), c->dmo.p, 16 * P, hipMemcpyDeviceToHost, c->stream), "copy moments");
     HIPCHK(c, hipMemcpyAsync(ss.data(), c->dms.p, 16 * P, hipMemcpyDeviceToHost, c->stream), "copy moments");
     HIPCHK(c, hipMemcpyAsync(sf.data(), c->dmf.p, 16 * P, hipMemcpyDeviceToHost, c->stream), "copy moments");
     HIPCHK(c, hipMemcpyAsync(sh.data(), c->dhsum.p, 16 * K * K, hipMemcpyDeviceToHost, c->stream), "copy moments");
     HIPCHK(c, hipMemcpyAsync(se.data(), c->desum.p, 16, hipMemcpyDeviceToHost, c->stream), "copy moments");
+    if (c->prec == NQES_PREC_MIXED)
+        HIPCHK(c, hipMemcpyAsync(&flags, c->dflags.p, 4, hipMemcpyDeviceToHost, c->stream), "copy flags");
     HIPCHK(c, hipStreamSynchronize(c->stream), "sample sync");
 
     // normalisation (rbm.cc:641-650)
@@ -298,7 +433,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     c->o.resize(P);
     c->skk.resize(P);
     c->f.resize(P);
-    int nonfinite = !(std::isfinite(c->eloc.re) && std::isfinite(c->eloc.im));
+    int nonfinite = !(std::isfinite(c->eloc.re) && std::isfinite(c->eloc.im)) || (flags & 1u);
     for (long long q = 0; q < P; q++) {
         cd oq = divr(so[q], Sd);
         cd sq = divr(ss[q], Sd);
@@ -318,12 +453,26 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     c->m = m;
     c->B = B;
     c->S = S;
-    c->svchunks = choose_chunks(S, P, 256);
     HIPCHK(c, c->dv.ensure((size_t)P * 16), "alloc v");
+    HIPCHK(c, c->dov.ensure(16), "alloc ov");
     HIPCHK(c, c->dw.ensure((size_t)S * 16), "alloc w");
-    HIPCHK(c, c->dsvpart.ensure((size_t)c->svchunks * P * 16), "alloc S*v partials");
-    HIPCHK(c, c->dsvout.ensure((size_t)P * 16), "alloc S*v out");
-    c->svraw.resize(P);
+    if (c->prec == NQES_PREC_FP64) {
+        c->svchunks = choose_chunks(S, P, 256);
+        HIPCHK(c, c->dsvpart.ensure((size_t)c->svchunks * P * 16), "alloc S*v partials");
+    } else {
+        const long long nvec = c->Pld / (cplx ? 2 : 4);
+        long long nblk;
+        if ((nvec + 63) / 64 <= 20) nblk = nq_sv_wave_blocks();   // wave-per-row kernel
+        else if (nq_mixed_fused_qv(nvec)) nblk = 1024;            // tiled block kernel
+        else nblk = choose_chunks(S, P, 256);                     // two passes
+        if (nblk > S) nblk = S;
+        c->svchunks = (int)nblk;
+        HIPCHK(c, c->dsvpart.ensure((size_t)c->svchunks * c->Pld * 16), "alloc S*v partials");
+        HIPCHK(c, c->dsvwpart.ensure((size_t)c->svchunks * 16), "alloc S*v partials");
+        HIPCHK(c, c->dsvscratch.ensure((size_t)(c->svchunks / 32 + 2) * (c->Pld + 1) * 16), "alloc S*v scratch");
+    }
+    HIPCHK(c, c->dsvout.ensure((size_t)(P + 1) * 16), "alloc S*v out");
+    c->svraw.resize(P + 1);
     c->sv_calls = 0;
     c->sv_ms = 0;
     c->sampled = true;
@@ -348,10 +497,30 @@ int nqes_get_O(nqes_ctx *c, int64_t row0, int64_t nrows, double *out)
     if (!c->sampled) return c->fail(NQES_ESTATE, "no samples: call nqes_sample first");
     if (!out || row0 < 0 || nrows < 0 || row0 + nrows > c->S) return c->fail(NQES_EINVAL, "row range out of bounds");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    const size_t e = o_elem(c->prec, c->cplx);
+    std::vector<unsigned char> buf((size_t)nrows * c->Pld * e);
     HIPCHK(c,
-           hipMemcpy(out, static_cast<char *>(c->dO.p) + (size_t)row0 * c->P * 16, (size_t)nrows * c->P * 16,
+           hipMemcpy(buf.data(), static_cast<char *>(c->dO.p) + (size_t)row0 * c->Pld * e, buf.size(),
                      hipMemcpyDeviceToHost),
            "copy O");
+    for (int64_t r = 0; r < nrows; r++)
+        for (long long q = 0; q < c->P; q++) {
+            const unsigned char *src = buf.data() + ((size_t)r * c->Pld + q) * e;
+            double *dst = out + 2 * ((size_t)r * c->P + q);
+            if (e == 16) {
+                memcpy(dst, src, 16);
+            } else if (e == 8) {
+                float v[2];
+                memcpy(v, src, 8);
+                dst[0] = v[0];
+                dst[1] = v[1];
+            } else {
+                float v;
+                memcpy(&v, src, 4);
+                dst[0] = v;
+                dst[1] = 0.0;
+            }
+        }
     return NQES_OK;
 }
 
@@ -367,26 +536,86 @@ int nqes_get_moments(nqes_ctx *c, double *o, double *skk, double *f, double *elo
     return NQES_OK;
 }
 
-// S v = (1/S) sum_s (O_s^T v) conj(O_s) + skk .* v - (o^T v) conj(o)   (rbm.cc:661-684)
-static int apply_S(nqes_ctx *c, const cd *v, cd *out)
+// device part of S*v on the FP32 O of the mixed path: raw[p] = sum_s conj(O_sp) w_s,
+// raw[P] = sum_s w_s, w_s = O_s^T v - o^T v.  v given as D = double (real) or cd.
+static int sv_mixed_device(nqes_ctx *c, const void *v, const void *ov, size_t dsize)
 {
     const long long P = c->P;
-    HIPCHK(c, hipMemcpyAsync(c->dv.p, v, 16 * P, hipMemcpyHostToDevice, c->stream), "copy v");
+    HIPCHK(c, hipMemcpyAsync(c->dv.p, v, dsize * P, hipMemcpyHostToDevice, c->stream), "copy v");
+    HIPCHK(c, hipMemcpyAsync(c->dov.p, ov, dsize, hipMemcpyHostToDevice, c->stream), "copy ov");
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
-    HIPCHK(c, nq_launch_sv64(c->dO.p, c->dv.p, c->S, P, c->svchunks, c->dw.p, c->dsvpart.p, c->dsvout.p, c->stream),
+    HIPCHK(c, nq_launch_sv_mx(c->cplx, c->dO.p, c->dv.p, c->dov.p, c->S, P, c->Pld, c->svchunks, c->dsvpart.p,
+                              c->dsvwpart.p, c->dw.p, c->dsvscratch.p, c->dsvout.p, c->stream),
            "S*v kernels");
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
-    HIPCHK(c, hipMemcpyAsync(c->svraw.data(), c->dsvout.p, 16 * P, hipMemcpyDeviceToHost, c->stream), "copy S*v");
+    HIPCHK(c, hipMemcpyAsync(c->svraw.data(), c->dsvout.p, dsize * (P + 1), hipMemcpyDeviceToHost, c->stream),
+           "copy S*v");
     HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
     c->sv_ms += event_ms(c->ev[2], c->ev[3]);
     c->sv_calls++;
+    return NQES_OK;
+}
+
+// S v = (1/S) sum_s (O_s^T v) conj(O_s) + skk .* v - (o^T v) conj(o)   (rbm.cc:661-684)
+static int apply_S(nqes_ctx *c, const cd *v, cd *out)
+{
     using namespace nq64;
+    const long long P = c->P;
+    const double Sd = (double)c->S;
     cd ov = zero();
     for (long long q = 0; q < P; q++) ov = add(ov, mul(c->o[q], v[q]));
-    const double Sd = (double)c->S;
-    for (long long q = 0; q < P; q++) {
-        cd r = divr(c->svraw[q], Sd);
-        out[q] = add(r, sub(mul(c->skk[q], v[q]), mul(ov, conj(c->o[q]))));
+    if (c->prec == NQES_PREC_FP64) {
+        HIPCHK(c, hipMemcpyAsync(c->dv.p, v, 16 * P, hipMemcpyHostToDevice, c->stream), "copy v");
+        HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
+        HIPCHK(c,
+               nq_launch_sv64(c->dO.p, c->dv.p, c->S, P, c->svchunks, c->dw.p, c->dsvpart.p, c->dsvout.p, c->stream),
+               "S*v kernels");
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
+        HIPCHK(c, hipMemcpyAsync(c->svraw.data(), c->dsvout.p, 16 * P, hipMemcpyDeviceToHost, c->stream), "copy S*v");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
+        c->sv_ms += event_ms(c->ev[2], c->ev[3]);
+        c->sv_calls++;
+        for (long long q = 0; q < P; q++) {
+            cd r = divr(c->svraw[q], Sd);
+            out[q] = add(r, sub(mul(c->skk[q], v[q]), mul(ov, conj(c->o[q]))));
+        }
+        return NQES_OK;
+    }
+    // mixed: centred single pass, (1/S) [sum_s conj(O_s) w_s - conj(o) sum_s w_s]
+    if (c->cplx) {
+        int st = sv_mixed_device(c, v, &ov, 16);
+        if (st != NQES_OK) return st;
+        const cd wsum = c->svraw[P];
+        for (long long q = 0; q < P; q++) {
+            cd r = divr(sub(c->svraw[q], mul(conj(c->o[q]), wsum)), Sd);
+            out[q] = add(r, mul(c->skk[q], v[q]));
+        }
+        return NQES_OK;
+    }
+    // real O and o: apply to the real and imaginary parts of v separately
+    c->hv.resize(P + 1);
+    bool has_im = false;
+    for (long long q = 0; q < P; q++)
+        if (v[q].im != 0.0) has_im = true;
+    for (int part = 0; part < (has_im ? 2 : 1); part++) {
+        double ovp = 0.0;
+        for (long long q = 0; q < P; q++) {
+            c->hv[q] = part ? v[q].im : v[q].re;
+            ovp += c->o[q].re * c->hv[q];
+        }
+        int st = sv_mixed_device(c, c->hv.data(), &ovp, 8);
+        if (st != NQES_OK) return st;
+        const double *raw = reinterpret_cast<const double *>(c->svraw.data());
+        const double wsum = raw[P];
+        for (long long q = 0; q < P; q++) {
+            double r = (raw[q] - c->o[q].re * wsum) / Sd + c->skk[q].re * c->hv[q];
+            if (part == 0) {
+                out[q].re = r;
+                out[q].im = 0.0;
+            } else {
+                out[q].im = r;
+            }
+        }
     }
     return NQES_OK;
 }

@@ -1,0 +1,106 @@
+"""Mixed-precision HIP path (FP32 amplitudes/O, FP64 sums) through the C ABI.
+
+Tolerances:
+  * the mixed path draws the same pcg64 streams as the FP64 path; an accept
+    decision differs only when the FP32 amplitude ratio lands within ~1e-6 of the
+    uniform draw, so at these sizes nearly all chains keep identical spins;
+  * on chains with identical spins, H_loc and O agree with the FP64 path to 1e-4
+    relative (FP32 per-hidden-unit arithmetic, FP64 products and sums);
+  * S*v on the mixed path's own FP32 O equals the dense (1/S) O^H O - o* o^T +
+    diag(skk) product built in float64 from the same O to 1e-10 (FP64 accumulation);
+  * e_mat and the SR update agree with the FP64 path to 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+from helpers import load_J, random_weights, rel_err
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # N, M, K, C, m, B, complex weights
+    (8, 16, 2, 64, 8, 4, False),
+    (8, 16, 2, 64, 8, 4, True),
+    (6, 12, 1, 32, 8, 4, True),
+    (6, 12, 3, 32, 6, 4, False),
+    (4, 8, 4, 16, 6, 4, True),
+    (16, 64, 2, 32, 4, 2, False),
+    (32, 64, 2, 32, 4, 2, False),
+    (16, 160, 2, 16, 3, 2, True),
+    (40, 80, 2, 16, 3, 2, False),
+]
+
+
+@pytest.fixture(scope="module")
+def nqes():
+    import nqes as mod
+    return mod
+
+
+def run_both(nqes, N, M, K, cplx, C, m, B, seed=777, reglam=0.5):
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=N + M + K, scale=0.1, complex_=cplx)
+    out = {}
+    for prec in (nqes.PREC_FP64, nqes.PREC_MIXED):
+        with nqes.Solver(N, M, K, 0.595, J, precision=prec) as s:
+            s.sample(W, seed, C, m, B, reglam)
+            sp, H, acc = s.samples()
+            out[prec] = dict(spins=sp, H=H, acc=acc, O=s.O_rows(0, C * m), mom=s.moments())
+    return W, J, out
+
+
+@pytest.mark.parametrize("N,M,K,C,m,B,cplx", CASES)
+def test_mixed_matches_fp64(nqes, N, M, K, C, m, B, cplx):
+    print(f"case N={N} M={M} K={K} C={C} m={m} B={B} complex={cplx}", flush=True)
+    W, J, out = run_both(nqes, N, M, K, cplx, C, m, B)
+    a, b = out[nqes.PREC_FP64], out[nqes.PREC_MIXED]
+    S = C * m
+    same = np.all(a["spins"].reshape(S, -1) == b["spins"].reshape(S, -1), axis=1)
+    assert same.mean() >= 0.9, same.mean()
+    assert rel_err(b["H"][same], a["H"][same]) < 1e-4
+    assert rel_err(b["O"][same], a["O"][same]) < 1e-4
+    if same.all():
+        assert rel_err(b["mom"]["emat"], a["mom"]["emat"]) < 1e-4
+        assert rel_err(b["mom"]["f"], a["mom"]["f"]) < 1e-3
+    if not cplx:
+        assert not np.any(b["O"].imag)
+
+
+@pytest.mark.parametrize("N,M,K,C,m,B,cplx", [CASES[0], CASES[1], CASES[3], CASES[6], CASES[7]])
+def test_mixed_sv_exact_on_own_O(nqes, N, M, K, C, m, B, cplx):
+    print(f"case N={N} M={M} K={K} complex={cplx}", flush=True)
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=3 + N, scale=0.1, complex_=cplx)
+    reglam = 0.3
+    with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+        s.sample(W, 99, C, m, B, reglam)
+        O = s.O_rows(0, C * m)
+        mom = s.moments()
+        S = C * m
+        o = O.mean(axis=0)
+        assert rel_err(mom["o"], o) < 1e-12
+        skk = reglam * ((np.abs(O) ** 2).mean(axis=0) - np.abs(o) ** 2)
+        assert rel_err(mom["skk"].real, skk) < 1e-10
+        rng = np.random.default_rng(1)
+        for v in (rng.normal(size=s.P) + 0j, rng.normal(size=s.P) + 1j * rng.normal(size=s.P)):
+            Sv = O.conj().T @ (O @ v) / S - o.conj() * (o @ v) + mom["skk"] * v
+            assert rel_err(s.apply_S(v), Sv) < 1e-10
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_mixed_sr_step_matches_fp64(nqes, cplx):
+    N, M, K, C, m, B = 8, 16, 2, 64, 16, 8
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=21, scale=0.1, complex_=cplx)
+    res = {}
+    for prec in (nqes.PREC_FP64, nqes.PREC_MIXED):
+        with nqes.Solver(N, M, K, 0.595, J, precision=prec) as s:
+            Wn = W.copy()
+            emat, st = s.monte_carlo(Wn, 5, C, m, B, 0.1, 2.0, 1e-9, stats=True)
+            res[prec] = (Wn, emat, st)
+    (Wa, ea, sa), (Wb, eb, sb) = res[nqes.PREC_FP64], res[nqes.PREC_MIXED]
+    assert rel_err(eb, ea) < 1e-4
+    assert rel_err(Wb - W, Wa - W) < 1e-3
+    assert abs(int(sa["accepts"]) - int(sb["accepts"])) <= max(3, sa["accepts"] // 1000)
+    if not cplx:
+        assert not np.any(Wb.imag)
