@@ -114,6 +114,17 @@ int nqes_apply_S(nqes_ctx *ctx, const double *v, double *out);
 int nqes_solve(nqes_ctx *ctx, double rtol, uint64_t maxit, double *x, int *flag,
                uint64_t *iters);
 
+/*
+ * Correlation estimators of the final state: correlation_fcn (rbm.cc:730-771) with
+ * the operators corrfcn_x/z/xx/zz (rbm.cc:774-829).  Chains as in an SR step
+ * (pcg64(seed + c), B burn-in sweeps, m samples); per sample the K x K matrix
+ * Phi^+ fcn(idx) is averaged.  Outputs (K*K complex each, row-major): x[N], z[N]
+ * (index i), xx[N*N], zz[N*N] (index i*N + j).  The caller projects them on the
+ * eigenvectors of e_mat as rbm.cc:764-769 does.  Runs the FP64 sampler.
+ */
+int nqes_correlations(nqes_ctx *ctx, const double *W, uint64_t seed, int C, int m, int B,
+                      double *x, double *z, double *xx, double *zz);
+
 /* Device bytes nqes_sample needs for C chains x m samples (admission check). */
 int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision);
 

@@ -21,11 +21,13 @@ struct Fp64SampleArgs {
     void *Es;                  // S complex: trace H_loc
     uint32_t *spins;           // S*K*NW
     unsigned long long *accepts; // C
+    void *cor;                 // correlation kernel: C*(2N+2N^2)*K*K complex
 };
 
 hipError_t nq_launch_tables64(int N, int M, int K, int L, const void *W, void *chT, void *shT, void *eaP,
                               void *eaN, hipStream_t st);
 hipError_t nq_launch_sample64(int K, const Fp64SampleArgs &a, hipStream_t st);
+hipError_t nq_launch_corr64(int K, const Fp64SampleArgs &a, hipStream_t st);
 hipError_t nq_launch_moments64(const void *O, const void *E, long long S, long long P, int nchunk, void *po,
                                void *ps, void *pf, void *out_o, void *out_s, void *out_f, hipStream_t st);
 hipError_t nq_launch_colsum64(const void *X, long long S, long long ncol, int nchunk, void *part, void *out,

@@ -80,25 +80,16 @@ __global__ void k_tables64(int N, int M, int K, int L, const cd *__restrict__ W,
     }
 }
 
+// Chain start (rbm.cc:604-608): pcg64(seed + c), K*N Bernoulli draws
+// (rbm.cc:99-119), state_fcn (rbm.cc:148-173) and the inverse.
 template <int K, int JPL>
-__global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
+__device__ inline void chain_start64(const Fp64SampleArgs &a, int c, cd *tl, uint32_t *sp, pcg &rng, cd *phi,
+                                     cd *phi_inv)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
     const int N = a.N, M = a.M, NW = a.NW, L = a.L;
-    cd *tl = reinterpret_cast<cd *>(smem);                      // [k][kk][j]
-    uint32_t *sp = reinterpret_cast<uint32_t *>(tl + K * K * M); // [k][word]
     const int lane = threadIdx.x;
-    const int c = blockIdx.x;
     const cd *W = static_cast<const cd *>(a.W);
-    const cd *chT = static_cast<const cd *>(a.chT);
-    const cd *shT = static_cast<const cd *>(a.shT);
-    const cd *eaP = static_cast<const cd *>(a.eaP);
-    const cd *eaN = static_cast<const cd *>(a.eaN);
-    cd *Hs = static_cast<cd *>(a.Hs);
-    cd *Es = static_cast<cd *>(a.Es);
-
-    // ---- chain start: spins (rbm.cc:99-119)
-    pcg rng = pcg_seed(a.seed + (uint64_t)c);
+    rng = pcg_seed(a.seed + (uint64_t)c);
     for (int w = 0; w < K * NW; w++) {
         uint32_t word = 0;
         int k = w / NW, i0 = (w % NW) * 32;
@@ -110,8 +101,6 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
     }
     __syncthreads();
 
-    // ---- state_fcn (rbm.cc:148-173)
-    cd phi[K * K], phi_inv[K * K];
     {
         cd part[K * K];
 #pragma unroll
@@ -155,68 +144,103 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
     }
     __syncthreads();
 
-    unsigned long long accepts = 0;
-    const int nsweeps = a.B + a.m;
-    for (int sw = 0; sw < nsweeps; sw++) {
-        // ---- Metropolis sweep (rbm.cc:530-552)
-        cd det_phi = detK<K>(phi);
-        for (int k = 0; k < K; k++) {
-            for (int i = 0; i < N; i++) {
-                const int si = spin_bit(sp, NW, k, i);
-                cd tnew[JPL][K];
-                cd part[K];
+}
+
+// One Metropolis sweep (rbm.cc:530-552): proposals k < K, i < N in order.
+template <int K, int JPL>
+__device__ inline void sweep64(const Fp64SampleArgs &a, cd *tl, uint32_t *sp, pcg &rng, cd *phi, cd *phi_inv,
+                               unsigned long long &accepts)
+{
+    const int N = a.N, M = a.M, NW = a.NW;
+    const int lane = threadIdx.x;
+    const cd *chT = static_cast<const cd *>(a.chT);
+    const cd *shT = static_cast<const cd *>(a.shT);
+    const cd *eaP = static_cast<const cd *>(a.eaP);
+    const cd *eaN = static_cast<const cd *>(a.eaN);
+    cd det_phi = detK<K>(phi);
 #pragma unroll
-                for (int kk = 0; kk < K; kk++) part[kk] = one();
+    for (int k = 0; k < K; k++) {
+        for (int i = 0; i < N; i++) {
+            const int si = spin_bit(sp, NW, k, i);
+            cd tnew[JPL][K];
+            cd part[K];
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) part[kk] = one();
+#pragma unroll
+            for (int q = 0; q < JPL; q++) {
+                int j = lane + 64 * q;
+                if (j < M) {
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        long long ti = ((long long)kk * N + i) * M + j;
+                        cd chv = chT[ti];
+                        cd shv = shT[ti];
+                        if (si) shv = neg(shv);
+                        cd th = tl[(k * K + kk) * M + j];
+                        cd denom = sub(chv, mul(th, shv));
+                        tnew[q][kk] = div(sub(mul(th, chv), shv), denom);
+                        part[kk] = (q == 0) ? denom : mul(part[kk], denom);
+                    }
+                }
+            }
+            cd prow[K];
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                cd e = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                cd T = wave_prod(part[kk]);
+                prow[kk] = mul(mul(phi[k * K + kk], e), T);
+            }
+            cd phip[K * K];
+#pragma unroll
+            for (int r = 0; r < K; r++)
+#pragma unroll
+                for (int cc = 0; cc < K; cc++) phip[r * K + cc] = (r == k) ? prow[cc] : phi[r * K + cc];
+            cd det_p = detK<K>(phip);
+            double prob = (det_phi.re == 0.0 && det_phi.im == 0.0) ? 1.0 : norm(det_p) / norm(det_phi);
+            double u = canonical(pcg_next(rng));
+            if (u <= (prob < 1.0 ? prob : 1.0)) {
+                det_phi = det_p;
+#pragma unroll
+                for (int q = 0; q < K * K; q++) phi[q] = phip[q];
+                invK<K>(phi, phi_inv);
 #pragma unroll
                 for (int q = 0; q < JPL; q++) {
                     int j = lane + 64 * q;
-                    if (j < M) {
+                    if (j < M)
 #pragma unroll
-                        for (int kk = 0; kk < K; kk++) {
-                            long long ti = ((long long)kk * N + i) * M + j;
-                            cd chv = chT[ti];
-                            cd shv = shT[ti];
-                            if (si) shv = neg(shv);
-                            cd th = tl[(k * K + kk) * M + j];
-                            cd denom = sub(chv, mul(th, shv));
-                            tnew[q][kk] = div(sub(mul(th, chv), shv), denom);
-                            part[kk] = (q == 0) ? denom : mul(part[kk], denom);
-                        }
-                    }
+                        for (int kk = 0; kk < K; kk++) tl[(k * K + kk) * M + j] = tnew[q][kk];
                 }
-                cd prow[K];
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) {
-                    cd e = si ? eaP[kk * N + i] : eaN[kk * N + i];
-                    cd T = wave_prod(part[kk]);
-                    prow[kk] = mul(mul(phi[k * K + kk], e), T);
-                }
-                cd phip[K * K];
-#pragma unroll
-                for (int r = 0; r < K; r++)
-#pragma unroll
-                    for (int cc = 0; cc < K; cc++) phip[r * K + cc] = (r == k) ? prow[cc] : phi[r * K + cc];
-                cd det_p = detK<K>(phip);
-                double prob = (det_phi.re == 0.0 && det_phi.im == 0.0) ? 1.0 : norm(det_p) / norm(det_phi);
-                double u = canonical(pcg_next(rng));
-                if (u <= (prob < 1.0 ? prob : 1.0)) {
-                    det_phi = det_p;
-#pragma unroll
-                    for (int q = 0; q < K * K; q++) phi[q] = phip[q];
-                    invK<K>(phi, phi_inv);
-#pragma unroll
-                    for (int q = 0; q < JPL; q++) {
-                        int j = lane + 64 * q;
-                        if (j < M)
-#pragma unroll
-                            for (int kk = 0; kk < K; kk++) tl[(k * K + kk) * M + j] = tnew[q][kk];
-                    }
-                    if (lane == 0) sp[k * NW + (i >> 5)] ^= 1u << (i & 31);
-                    accepts++;
-                    __syncthreads();
-                }
+                if (lane == 0) sp[k * NW + (i >> 5)] ^= 1u << (i & 31);
+                accepts++;
+                __syncthreads();
             }
         }
+    }
+}
+
+template <int K, int JPL>
+__global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = a.N, M = a.M, NW = a.NW, L = a.L;
+    cd *tl = reinterpret_cast<cd *>(smem);                      // [k][kk][j]
+    uint32_t *sp = reinterpret_cast<uint32_t *>(tl + K * K * M); // [k][word]
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x;
+    const cd *chT = static_cast<const cd *>(a.chT);
+    const cd *shT = static_cast<const cd *>(a.shT);
+    const cd *eaP = static_cast<const cd *>(a.eaP);
+    const cd *eaN = static_cast<const cd *>(a.eaN);
+    cd *Hs = static_cast<cd *>(a.Hs);
+    cd *Es = static_cast<cd *>(a.Es);
+
+    cd phi[K * K], phi_inv[K * K];
+    pcg rng;
+    chain_start64<K, JPL>(a, c, tl, sp, rng, phi, phi_inv);
+    unsigned long long accepts = 0;
+    const int nsweeps = a.B + a.m;
+    for (int sw = 0; sw < nsweeps; sw++) {
+        sweep64<K, JPL>(a, tl, sp, rng, phi, phi_inv, accepts);
         if (sw < a.B) continue;
 
         // ---- one sample
@@ -332,6 +356,132 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
     if (lane == 0) a.accepts[c] = accepts;
 }
 
+// accumulate the K x K matrix Phi^+ X into a chain's correlation slot (lane q
+// owns entry q)
+template <int K>
+__device__ inline void cor_add(cd *slot, const cd *phi_inv, const cd *X)
+{
+    cd mat[K * K];
+    matmulK<K>(phi_inv, X, mat);
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < K * K; q++)
+        if (lane == q) slot[q] = add(slot[q], mat[q]);
+}
+
+// Correlation estimators (rbm.cc:730-771 with the operators of rbm.cc:774-829): per
+// sample and index, Phi^+ fcn(index) accumulated per chain into
+// cor[c][idx][K*K]: idx = i (sigma^x_i: spin i flipped in every replica), N + i
+// (sigma^z_i: rows with spin i down negated), 2N + i*N + j (sigma^x_i sigma^x_j),
+// 2N + N*N + i*N + j (sigma^z_i sigma^z_j).  Flipped amplitudes use the incremental
+// update of flip_state_fcn applied once (x) or twice (xx) instead of state_fcn from
+// scratch (same values, cheaper).
+template <int K, int JPL>
+__global__ __launch_bounds__(64) void k_corr64(Fp64SampleArgs a)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = a.N, M = a.M, NW = a.NW;
+    cd *tl = reinterpret_cast<cd *>(smem);
+    uint32_t *sp = reinterpret_cast<uint32_t *>(tl + K * K * M);
+    const int lane = threadIdx.x;
+    const int c = blockIdx.x;
+    const cd *chT = static_cast<const cd *>(a.chT);
+    const cd *shT = static_cast<const cd *>(a.shT);
+    const cd *eaP = static_cast<const cd *>(a.eaP);
+    const cd *eaN = static_cast<const cd *>(a.eaN);
+    const long long ncor = 2LL * N + 2LL * N * N;
+    cd *cor = static_cast<cd *>(a.cor) + (long long)c * ncor * K * K;
+    for (long long q = lane; q < ncor * K * K; q += 64) cor[q] = zero();
+
+    cd phi[K * K], phi_inv[K * K];
+    pcg rng;
+    chain_start64<K, JPL>(a, c, tl, sp, rng, phi, phi_inv);
+    unsigned long long accepts = 0;
+    for (int sw = 0; sw < a.B + a.m; sw++) {
+        sweep64<K, JPL>(a, tl, sp, rng, phi, phi_inv, accepts);
+        if (sw < a.B) continue;
+        __syncthreads();
+        // sigma^x_i and sigma^z_i
+        for (int i = 0; i < N; i++) {
+            cd X[K * K], Z[K * K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int si = spin_bit(sp, NW, k, i);
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    cd part = one();
+#pragma unroll
+                    for (int q = 0; q < JPL; q++) {
+                        int j = lane + 64 * q;
+                        if (j < M) {
+                            long long ti = ((long long)kk * N + i) * M + j;
+                            cd shv = shT[ti];
+                            if (si) shv = neg(shv);
+                            cd d = sub(chT[ti], mul(tl[(k * K + kk) * M + j], shv));
+                            part = (q == 0) ? d : mul(part, d);
+                        }
+                    }
+                    cd e = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                    X[k * K + kk] = mul(mul(phi[k * K + kk], e), wave_prod(part));
+                    Z[k * K + kk] = si ? neg(phi[k * K + kk]) : phi[k * K + kk];
+                }
+            }
+            cor_add<K>(cor + (long long)i * K * K, phi_inv, X);
+            cor_add<K>(cor + (long long)(N + i) * K * K, phi_inv, Z);
+        }
+        // sigma^x_i sigma^x_j and sigma^z_i sigma^z_j
+        for (int i = 0; i < N; i++) {
+            for (int jj = 0; jj < N; jj++) {
+                cd Z[K * K];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int f = spin_bit(sp, NW, k, i) ^ spin_bit(sp, NW, k, jj);
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) Z[k * K + kk] = f ? neg(phi[k * K + kk]) : phi[k * K + kk];
+                }
+                cor_add<K>(cor + (2LL * N + (long long)N * N + (long long)i * N + jj) * K * K, phi_inv, Z);
+                if (jj < i) continue;
+                if (jj == i) {
+                    cor_add<K>(cor + (2LL * N + (long long)i * N + i) * K * K, phi_inv, phi);
+                    continue;
+                }
+                cd X[K * K];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const int si = spin_bit(sp, NW, k, i), sj = spin_bit(sp, NW, k, jj);
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        cd part = one();
+#pragma unroll
+                        for (int q = 0; q < JPL; q++) {
+                            int j = lane + 64 * q;
+                            if (j < M) {
+                                long long ti = ((long long)kk * N + i) * M + j;
+                                long long tj = ((long long)kk * N + jj) * M + j;
+                                cd t = tl[(k * K + kk) * M + j];
+                                cd shi = shT[ti], shj = shT[tj];
+                                if (si) shi = neg(shi);
+                                if (sj) shj = neg(shj);
+                                cd d1 = sub(chT[ti], mul(t, shi));
+                                cd t1 = div(sub(mul(t, chT[ti]), shi), d1);
+                                cd d2 = sub(chT[tj], mul(t1, shj));
+                                cd d = mul(d1, d2);
+                                part = (q == 0) ? d : mul(part, d);
+                            }
+                        }
+                        cd ei = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                        cd ej = sj ? eaP[kk * N + jj] : eaN[kk * N + jj];
+                        X[k * K + kk] = mul(mul(mul(phi[k * K + kk], ei), ej), wave_prod(part));
+                    }
+                }
+                cor_add<K>(cor + (2LL * N + (long long)i * N + jj) * K * K, phi_inv, X);
+                cor_add<K>(cor + (2LL * N + (long long)jj * N + i) * K * K, phi_inv, X);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Column sums over rows [s0, s1) of chunk blockIdx.y; thread = column.
 __global__ void k_moments64(const cd *__restrict__ O, const cd *__restrict__ E, long long S, long long P,
                             int nchunk, cd *__restrict__ po, cd *__restrict__ ps, cd *__restrict__ pf)
@@ -420,7 +570,38 @@ hipError_t launch_sample_k(const Fp64SampleArgs &a, hipStream_t st)
     return hipErrorInvalidValue;
 }
 
+
+template <int K, int JPL>
+hipError_t launch_corr_t(const Fp64SampleArgs &a, hipStream_t st)
+{
+    size_t lds = sizeof(cd) * K * K * a.M + sizeof(uint32_t) * K * a.NW;
+    hipLaunchKernelGGL((k_corr64<K, JPL>), dim3(a.C), dim3(64), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_corr_k(const Fp64SampleArgs &a, hipStream_t st)
+{
+    int jpl = (a.M + 63) / 64;
+    if (jpl <= 1) return launch_corr_t<K, 1>(a, st);
+    if (jpl <= 2) return launch_corr_t<K, 2>(a, st);
+    if (jpl <= 4) return launch_corr_t<K, 4>(a, st);
+    if (jpl <= 8) return launch_corr_t<K, 8>(a, st);
+    return hipErrorInvalidValue;
+}
+
 } // namespace
+
+hipError_t nq_launch_corr64(int K, const Fp64SampleArgs &a, hipStream_t st)
+{
+    switch (K) {
+    case 1: return launch_corr_k<1>(a, st);
+    case 2: return launch_corr_k<2>(a, st);
+    case 3: return launch_corr_k<3>(a, st);
+    case 4: return launch_corr_k<4>(a, st);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t nq_launch_tables64(int N, int M, int K, int L, const void *W, void *chT, void *shT, void *eaP,
                               void *eaN, hipStream_t st)

@@ -657,6 +657,65 @@ int nqes_solve(nqes_ctx *c, double rtol, uint64_t maxit, double *x, int *flag, u
     return NQES_OK;
 }
 
+int nqes_correlations(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B, double *x, double *z,
+                      double *xx, double *zz)
+{
+    if (!c) return NQES_EINVAL;
+    if (!c->has_problem) return c->fail(NQES_ESTATE, "nqes_problem_init not called");
+    if (!W) return c->fail(NQES_EINVAL, "W is NULL");
+    if (C < 1 || m < 1 || B < 0) return c->fail(NQES_EINVAL, "invalid C=%d m=%d B=%d", C, m, B);
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    const int N = c->N, K = c->K;
+    const long long ncol = (2LL * N + 2LL * N * N) * K * K;
+    DevBuf dcor, dpart, dsum;
+    HIPCHK(c, dcor.ensure((size_t)C * ncol * 16), "alloc correlations");
+    const int nch = choose_chunks(C, ncol, 64);
+    HIPCHK(c, dpart.ensure((size_t)nch * ncol * 16), "alloc correlations");
+    HIPCHK(c, dsum.ensure((size_t)ncol * 16), "alloc correlations");
+    HIPCHK(c, c->dspins.ensure(16), "alloc");
+    HIPCHK(c, hipMemcpyAsync(c->dW.p, W, 16 * c->P, hipMemcpyHostToDevice, c->stream), "copy W");
+    HIPCHK(c, nq_launch_tables64(N, c->M, K, c->L, c->dW.p, c->dchT.p, c->dshT.p, c->deaP.p, c->deaN.p, c->stream),
+           "tables kernel");
+    Fp64SampleArgs a;
+    memset(&a, 0, sizeof(a));
+    a.N = N;
+    a.M = c->M;
+    a.NW = c->NW;
+    a.L = c->L;
+    a.P = c->P;
+    a.C = C;
+    a.m = m;
+    a.B = B;
+    a.seed = seed;
+    a.h = c->h;
+    a.J = static_cast<const double *>(c->dJ.p);
+    a.W = c->dW.p;
+    a.chT = c->dchT.p;
+    a.shT = c->dshT.p;
+    a.eaP = c->deaP.p;
+    a.eaN = c->deaN.p;
+    a.cor = dcor.p;
+    HIPCHK(c, nq_launch_corr64(K, a, c->stream), "correlation kernel");
+    HIPCHK(c, nq_launch_colsum64(dcor.p, C, ncol, nch, dpart.p, dsum.p, c->stream), "colsum kernel");
+    std::vector<cd> sum(ncol);
+    HIPCHK(c, hipMemcpyAsync(sum.data(), dsum.p, (size_t)ncol * 16, hipMemcpyDeviceToHost, c->stream), "copy");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "correlation sync");
+    const double Sd = (double)C * (double)m;
+    const long long kk2 = (long long)K * K;
+    double *outs[4] = {x, z, xx, zz};
+    const long long off[4] = {0, (long long)N, 2LL * N, 2LL * N + (long long)N * N};
+    const long long cnt[4] = {(long long)N, (long long)N, (long long)N * N, (long long)N * N};
+    for (int t = 0; t < 4; t++) {
+        if (!outs[t]) continue;
+        for (long long q = 0; q < cnt[t] * kk2; q++) {
+            cd v = nq64::divr(sum[off[t] * kk2 + q], Sd);
+            outs[t][2 * q] = v.re;
+            outs[t][2 * q + 1] = v.im;
+        }
+    }
+    return NQES_OK;
+}
+
 int nqes_sr_step(nqes_ctx *c, double *W, uint64_t seed, int C, int m, int B, double lr, double reglam, double rtol,
                  uint64_t maxit, double *emat, nqes_stats *stats)
 {

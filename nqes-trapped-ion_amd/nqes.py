@@ -54,7 +54,7 @@ _lib = None
 EXPORTS = [
     "nqes_create", "nqes_destroy", "nqes_last_error", "nqes_problem_init", "nqes_sr_step",
     "nqes_sample", "nqes_get_samples", "nqes_get_O", "nqes_get_moments", "nqes_apply_S",
-    "nqes_solve", "nqes_required_bytes", "nqes_version",
+    "nqes_solve", "nqes_correlations", "nqes_required_bytes", "nqes_version",
 ]
 
 
@@ -89,6 +89,8 @@ def load_library(path=LIB_PATH):
     L.nqes_apply_S.argtypes = [_vp, _cp, _cp]
     L.nqes_solve.restype = c_int
     L.nqes_solve.argtypes = [_vp, c_dbl, c_u64, _cp, ctypes.POINTER(c_int), ctypes.POINTER(c_u64)]
+    L.nqes_correlations.restype = c_int
+    L.nqes_correlations.argtypes = [_vp, _cp, c_u64, c_int, c_int, c_int, _cp, _cp, _cp, _cp]
     L.nqes_required_bytes.restype = c_i64
     L.nqes_required_bytes.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int]
     L.nqes_version.restype = ctypes.c_char_p
@@ -198,12 +200,36 @@ class Solver:
         self._check(self.lib.nqes_apply_S(self.ctx, v, out))
         return out
 
+    def correlations(self, W, seed, C, m, B):
+        """Sample means of Phi^+ fcn(idx) (rbm.cc:730-829): x, z of shape (N, K, K),
+        xx, zz of shape (N, N, K, K)."""
+        N, K = self.N, self.K
+        W = np.ascontiguousarray(W, dtype=np.complex128)
+        x = np.zeros(N * K * K, dtype=np.complex128)
+        z = np.zeros(N * K * K, dtype=np.complex128)
+        xx = np.zeros(N * N * K * K, dtype=np.complex128)
+        zz = np.zeros(N * N * K * K, dtype=np.complex128)
+        self._check(self.lib.nqes_correlations(self.ctx, W, int(seed) % (1 << 64), int(C), int(m), int(B),
+                                               x, z, xx, zz))
+        return x.reshape(N, K, K), z.reshape(N, K, K), xx.reshape(N, N, K, K), zz.reshape(N, N, K, K)
+
     def solve(self, rtol, maxit=2**64 - 1):
         x = np.zeros(self.P, dtype=np.complex128)
         flag = ctypes.c_int(0)
         it = ctypes.c_uint64(0)
         self._check(self.lib.nqes_solve(self.ctx, float(rtol), int(maxit), x, ctypes.byref(flag), ctypes.byref(it)))
         return x, flag.value, it.value
+
+
+def project_correlations(cor, emat):
+    """rbm.cc:764-769 and 1198-1216: res[idx, k] = Re diag(V^-1 cor[idx] V)[k] with V
+    the eigenvectors of e_mat, columns ordered by ascending real eigenvalue."""
+    evals, V = np.linalg.eig(emat)
+    order = np.argsort(evals.real, kind="stable")
+    Vi = np.linalg.inv(V)
+    flat = cor.reshape(-1, cor.shape[-2], cor.shape[-1])
+    res = np.einsum("ij,njk,ki->ni", Vi, flat, V).real
+    return res[:, order].reshape(cor.shape[:-2] + (cor.shape[-1],))
 
 
 def run(solver, W, seed, C, m, B, niters, tol=1e-6, lrate0=0.1, lratef=0.99, lrate_cutoff=0.01,

@@ -753,6 +753,85 @@ void orc_run_chain(int N, int M, int K, double h, const double *J, const double 
     chain_free(&ch);
 }
 
+/*
+ * correlation_fcn (rbm.cc:730-771) for the four operators of rbm.cc:774-829, all
+ * accumulated over one set of chains (the reference runs the same chains four
+ * times with the same seed).  sigma^x terms recompute Phi from scratch with
+ * state_fcn on the flipped configuration, as corrfcn_x / corrfcn_xx do.  Outputs
+ * the sample means of Phi^+ fcn(idx): x[N], z[N], xx[N*N], zz[N*N] (K*K complex each).
+ */
+void orc_correlations(int N, int M, int K, double h, const double *J, const double *Wd, uint64_t seed,
+                      int C, int m, int B, double *x, double *z, double *xx, double *zz)
+{
+    prob_t p = mkprob(N, M, K, h, J);
+    const cplx *W = (const cplx *)Wd;
+    long KK = (long)K * K;
+    long ncor = 2L * N + 2L * N * N;
+    cplx *cor = (cplx *)calloc((size_t)ncor * KK, sizeof(cplx));
+    for (int c = 0; c < C; c++) {
+        u128 rng = pcg_seed(seed + (uint64_t)c);
+        uint32_t s[ORC_KMAX * (ORC_NMAX / 32)], s2[ORC_KMAX * (ORC_NMAX / 32)];
+        chain_t ch;
+        chain_alloc(&p, &ch);
+        chain_start(&p, W, s, &rng, &ch);
+        for (int t = 0; t < B; t++) mc_sweep(&p, W, s, &rng, &ch);
+        for (int t = 0; t < m; t++) {
+            mc_sweep(&p, W, s, &rng, &ch);
+            cplx X[ORC_KMAX * ORC_KMAX], mat[ORC_KMAX * ORC_KMAX];
+            for (long idx = 0; idx < ncor; idx++) {
+                int i, j;
+                if (idx < N) {                                /* sigma^x_i */
+                    i = (int)idx;
+                    memcpy(s2, s, sizeof(uint32_t) * K * p.NW);
+                    for (int k = 0; k < K; k++) sflip(s2 + k * p.NW, i);
+                    state_fcn(&p, W, s2, X, NULL);
+                } else if (idx < 2L * N) {                    /* sigma^z_i */
+                    i = (int)(idx - N);
+                    for (int k = 0; k < K; k++)
+                        for (int kk = 0; kk < K; kk++) {
+                            cplx v = ch.phi[k * K + kk];
+                            X[k * K + kk] = sbit(s + k * p.NW, i) ? cneg(v) : v;
+                        }
+                } else if (idx < 2L * N + (long)N * N) {      /* sigma^x_i sigma^x_j */
+                    long r = idx - 2L * N;
+                    i = (int)(r / N);
+                    j = (int)(r % N);
+                    memcpy(s2, s, sizeof(uint32_t) * K * p.NW);
+                    for (int k = 0; k < K; k++) {
+                        sflip(s2 + k * p.NW, i);
+                        sflip(s2 + k * p.NW, j);
+                    }
+                    state_fcn(&p, W, s2, X, NULL);
+                } else {                                      /* sigma^z_i sigma^z_j */
+                    long r = idx - 2L * N - (long)N * N;
+                    i = (int)(r / N);
+                    j = (int)(r % N);
+                    for (int k = 0; k < K; k++)
+                        for (int kk = 0; kk < K; kk++) {
+                            cplx v = ch.phi[k * K + kk];
+                            int f = sbit(s + k * p.NW, i) ^ sbit(s + k * p.NW, j);
+                            X[k * K + kk] = f ? cneg(v) : v;
+                        }
+                }
+                matmulK(ch.phi_inv, X, K, mat);
+                for (long q = 0; q < KK; q++) cor[idx * KK + q] = cadd(cor[idx * KK + q], mat[q]);
+            }
+        }
+        chain_free(&ch);
+    }
+    double Sd = (double)C * (double)m;
+    double *outs[4] = {x, z, xx, zz};
+    long off[4] = {0, N, 2L * N, 2L * N + (long)N * N};
+    long cnt[4] = {N, N, (long)N * N, (long)N * N};
+    for (int t = 0; t < 4; t++)
+        for (long q = 0; q < cnt[t] * KK; q++) {
+            cplx v = cdivr(cor[off[t] * KK + q], Sd);
+            outs[t][2 * q] = v.re;
+            outs[t][2 * q + 1] = v.im;
+        }
+    free(cor);
+}
+
 /* ------------------------------------------------------------------------- */
 /* MINRES-QLP (minresqlp.h:85-390), complex vectors, real scalars             */
 /* ------------------------------------------------------------------------- */

@@ -74,6 +74,8 @@ def lib():
         L.orc_sv_from_O.restype = c_int
         L.orc_sv_from_O.argtypes = [c_long, c_long, _cp, _cp, _cp, _cp, _cp]
         L.orc_init_spins.argtypes = [c_int, c_int, c_u64, c_int, _up]
+        L.orc_correlations.argtypes = [c_int, c_int, c_int, c_dbl, _dp, _cp, c_u64, c_int, c_int, c_int,
+                                       _cp, _cp, _cp, _cp]
         L.orc_max_threads.restype = c_int
         L.orc_set_threads.argtypes = [c_int]
         _lib = L
@@ -301,6 +303,18 @@ def sv_from_O(O, o, skk, v):
                         np.ascontiguousarray(skk, dtype=np.complex128),
                         np.ascontiguousarray(v, dtype=np.complex128), out)
     return out
+
+
+def correlations(N, M, K, h, J, W, seed, C, m, B):
+    """Sample means of Phi^+ fcn(idx) for sigma^x_i, sigma^z_i, sigma^x_i sigma^x_j,
+    sigma^z_i sigma^z_j (rbm.cc:730-829); arrays (N, K, K) and (N, N, K, K)."""
+    x = np.zeros(N * K * K, dtype=np.complex128)
+    z = np.zeros(N * K * K, dtype=np.complex128)
+    xx = np.zeros(N * N * K * K, dtype=np.complex128)
+    zz = np.zeros(N * N * K * K, dtype=np.complex128)
+    lib().orc_correlations(N, M, K, float(h), np.ascontiguousarray(J, dtype=np.float64).reshape(-1),
+                           np.ascontiguousarray(W, dtype=np.complex128), int(seed) % (1 << 64), C, m, B, x, z, xx, zz)
+    return x.reshape(N, K, K), z.reshape(N, K, K), xx.reshape(N, N, K, K), zz.reshape(N, N, K, K)
 
 
 def set_threads(n):

@@ -305,3 +305,57 @@ def test_sr_loop_reaches_ed(oracle):
     assert np.allclose(ev, ed_energies(N, h, J, K), atol=3e-3), (ev, ed_energies(N, h, J, K))
     # real weights stay real through the complex algorithm
     assert not np.any(W.imag)
+
+
+# ------------------------------------------------------------------ correlation operators
+def test_correlation_operators_rayleigh_ritz(oracle):
+    # E_{|det Phi|^2}[Phi^+ fcn(idx)] = G^-1 A^H Op A for Op = sigma^x_i, sigma^z_i,
+    # sigma^x_i sigma^x_j, sigma^z_i sigma^z_j; fcn built as rbm.cc:774-829 does
+    N, M, K = 4, 4, 2
+    W = random_weights(N, M, K, seed=77, scale=0.3)
+    A = rbm_vectors(N, M, K, W)
+    G = A.conj().T @ A
+    dim = 1 << N
+    s = np.arange(dim)
+    sig = 1 - 2 * ((s[:, None] >> np.arange(N)[None, :]) & 1)
+    ops = {}
+    for i in range(N):
+        X = np.zeros((dim, dim))
+        X[s, s ^ (1 << i)] = 1
+        ops[("x", i)] = X
+        ops[("z", i)] = np.diag(sig[:, i]).astype(float)
+    acc = {key: np.zeros((K, K), complex) for key in ops}
+    acc_xx = np.zeros((N, N, K, K), complex)
+    tot = 0.0
+    for cfg in range(1 << (K * N)):
+        bits = np.array([[(cfg >> (k * N + i)) & 1 for i in range(N)] for k in range(K)])
+        phi, _ = oracle.state_fcn(N, M, K, W, oracle.pack_spins(bits))
+        phi_inv, ok = oracle.inv(phi)
+        if not ok:
+            continue
+        wgt = abs(np.linalg.det(phi)) ** 2
+        tot += wgt
+        for i in range(N):
+            b2 = bits.copy()
+            b2[:, i] ^= 1
+            phx, _ = oracle.state_fcn(N, M, K, W, oracle.pack_spins(b2))
+            acc[("x", i)] += wgt * phi_inv @ phx
+            acc[("z", i)] += wgt * phi_inv @ (np.where(bits[:, i:i + 1] == 1, -1, 1) * phi)
+            for j in range(N):
+                b3 = bits.copy()
+                b3[:, i] ^= 1
+                b3[:, j] ^= 1
+                phxx, _ = oracle.state_fcn(N, M, K, W, oracle.pack_spins(b3))
+                acc_xx[i, j] += wgt * phi_inv @ phxx
+    for key, Op in ops.items():
+        R = np.linalg.solve(G, A.conj().T @ Op @ A)
+        assert rel_err(acc[key] / tot, R) < 1e-10, key
+    for i in range(N):
+        for j in range(N):
+            R = np.linalg.solve(G, A.conj().T @ (ops[("x", i)] @ ops[("x", j)]) @ A)
+            assert rel_err(acc_xx[i, j] / tot, R) < 1e-10
+    # the oracle's sampler-side estimator uses exactly these per-configuration
+    # matrices: on a chain, its xx_ii is the identity and xx, zz are symmetric
+    x, z, xx, zz = oracle.correlations(N, M, K, 0.5, synthetic_J(N), W, 5, 3, 6, 2)
+    assert np.abs(xx[np.arange(N), np.arange(N)] - np.eye(K)).max() < 1e-12
+    assert np.array_equal(xx, xx.transpose(1, 0, 2, 3)) and np.array_equal(zz, zz.transpose(1, 0, 2, 3))
