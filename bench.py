@@ -41,6 +41,22 @@ def initial_weights(N, M, K, seed):
     return (rng.normal(size=K * (N + M + N * M)) * 0.01).astype(np.complex128)
 
 
+def profiled_traffic(args, real):
+    """HBM bytes per S*v launch measured by rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE,
+    copy-calibrated) for this exact workload, from the summaries under profiles/;
+    None when no summary covers it."""
+    key = (f"traffic: precision={args.precision} real={int(real)} N={args.N} M={args.M} K={args.K} "
+           f"C={args.C} m={args.m} bytes=")
+    pdir = os.path.join(REPO, "profiles")
+    for name in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
+        with open(os.path.join(pdir, name)) as f:
+            for line in f:
+                line = line.strip()
+                if line.startswith(key):
+                    return int(line[len(key):].split()[0])
+    return None
+
+
 def cpu_baseline(args, J, W0):
     """The CPU oracle (clean-room FP64 OpenMP restatement of rbm.cc's SR step,
     matrix-free S*v with O recomputed per MINRES iteration) on a bounded sample."""
@@ -82,8 +98,8 @@ def main():
     ap.add_argument("--reglam", type=float, default=40.0)
     ap.add_argument("--rtol", type=float, default=1e-9)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--cpu-C", type=int, default=64)
-    ap.add_argument("--cpu-m", type=int, default=16)
+    ap.add_argument("--cpu-C", type=int, default=512)
+    ap.add_argument("--cpu-m", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB mebibytes (HBM counter calibration)")
@@ -147,7 +163,7 @@ def main():
         # one streamed pass over FP32 (real) or complex64 O, v / result / o in FP64
         b_O, n_pass = (4 if real else 8), 1
         sv_bytes = n_pass * S * P * b_O + 3 * P * (8 if real else 16)
-        sv_kernel = f"S*v: k_sv_fused_mx (one pass over {'FP32' if real else 'complex64'} O)"
+        sv_kernel = f"S*v: k_sv_wave_mx (one pass over {'FP32' if real else 'complex64'} O)"
     sv_calls = sum(s["sv_calls"] for s in stats)
     sv_ms = sum(s["t_sv_ms"] for s in stats)
     sv_avg_ms = sv_ms / max(sv_calls, 1)
@@ -182,7 +198,7 @@ def main():
         "energies_last_step": energies,
         "roofline": {"bound": "hbm", "kernel": sv_kernel, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes},
+                     "traffic": profiled_traffic(args, real), "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, J, W0)
