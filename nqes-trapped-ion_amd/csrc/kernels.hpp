@@ -62,7 +62,6 @@ struct MixedSampleArgs {
     unsigned int *flags;       // bit 0: an angle exceeded the FP32 range
 };
 
-int nq_mixed_fused_qv(long long nvec);
 hipError_t nq_launch_tables_mx(bool cplx, int N, int M, int K, int L, const void *W, void *chT, void *shT, void *wT,
                                void *bT, void *eaP, void *eaN, void *aD, hipStream_t st);
 hipError_t nq_launch_chain_init(int N, int K, int NW, int C, uint64_t seed, uint32_t *spins0, uint64_t *rng0,
@@ -77,4 +76,9 @@ hipError_t nq_launch_moments_mx(bool cplx, const void *O, const void *E, long lo
 hipError_t nq_launch_sv_mx(bool cplx, const void *O, const void *v, const void *ov, long long S, long long P,
                            long long Pld, int nblk, void *part, void *wpart, void *wbuf, void *scratch, void *out,
                            hipStream_t st);
-int nq_sv_wave_blocks();
+// launch shape of the one-pass S*v (qv = 0: two-pass form)
+struct SvShape {
+    int qv, wpr, np, blocks;
+    size_t lds;
+};
+SvShape nq_sv_shape(bool cplx, long long nvec);

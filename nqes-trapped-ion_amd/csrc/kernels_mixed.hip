@@ -16,9 +16,10 @@
 //                    O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k}
 //                    (Cramer's rule applied to rbm.cc:199-239) written to HBM in FP32
 //   k_moments_mx     column sums of O, |O|^2, E conj(O) in FP64 with Neumaier compensation
-//   k_sv_fused_mx    S*v in one streamed pass over O: per row w_s = (O_s - o)^T v, then
-//                    acc += conj(O_s) w_s, FP64 accumulation (rbm.cc:661-684)
-//   k_sv_rowdot_mx / k_sv_colacc_mx   two-pass S*v for rows too long for registers
+//   k_sv_rows_mx     S*v in one streamed pass over O (rbm.cc:661-684): WPR wavefronts
+//                    share a row, w_s = (O_s - o)^T v, then acc += conj(O_s) w_s with
+//                    FP64 accumulation in registers; next row prefetched, nontemporal loads
+//   k_sv_rowdot_mx / k_sv_colacc_mx   two-pass S*v when v does not fit in LDS
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -518,16 +519,13 @@ __global__ void k_moments_mx(const void *O_, const cd *__restrict__ E, long long
     pf[o] = nq64::mk(f_r + f_rc, f_i + f_ic);
 }
 
-// ---------------------------------------------------------------- S*v (fused, one pass)
-// Thread t owns 16-byte vectors u = t + 256 q of each row (4 real or 2 complex
-// columns).  v, acc in FP64 registers; rows double-buffered in registers.
+// ---------------------------------------------------------------- S*v (one pass)
+// A 16-byte vector of an O row holds 4 real or 2 complex columns.
 template <bool CPLX> struct Vec;
 template <> struct Vec<false> {
-    using V = float4;
     static constexpr int W = 4;
 };
 template <> struct Vec<true> {
-    using V = float4;  // two complex
     static constexpr int W = 2;
 };
 
@@ -545,116 +543,41 @@ __device__ inline void vec_to_D(const float4 &x, typename Ty<CPLX>::D *o)
     }
 }
 
-// A block of NWV wavefronts streams a contiguous range of rows in tiles of R rows:
-// thread t owns the 16-byte column vectors u = t + 64*NWV*q (q < QV) of every row.
-// The R partial dot products of a tile are reduced with one barrier, then the same
-// registers feed the R rank-one updates.  The block width NWV is chosen per problem
-// so that QV is small (few registers, two blocks per CU: one loads while the other
-// reduces) and few column slots are idle (4 to 8 wavefronts per block).
-template <bool CPLX, int QV, int R>
-__global__ __launch_bounds__(512) void k_sv_fused_mx(const float4 *__restrict__ O, const void *v_, const void *ov_,
-                                                      long long S, long long nvec, long long rows_per_block,
-                                                      void *part_, void *wsum_)
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// streaming load that bypasses cache retention (O is read once per S*v)
+__device__ __forceinline__ float4 ld_stream(const float4 *p)
 {
-    using D = typename Ty<CPLX>::D;
-    constexpr int W = Vec<CPLX>::W;
-    const D *v = (const D *)v_;
-    const D ov = *(const D *)ov_;
-    D *part = (D *)part_;
-    D *wsum = (D *)wsum_;
-    __shared__ D red[2][R][8];
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    const int nwv = blockDim.x >> 6;
-    const long long stride = blockDim.x;
-    D vr[QV][W], acc[QV][W];
-#pragma unroll
-    for (int q = 0; q < QV; q++) {
-        long long u = t + stride * q;
-#pragma unroll
-        for (int e = 0; e < W; e++) {
-            vr[q][e] = (u < nvec) ? v[u * W + e] : zeroD<D>();
-            acc[q][e] = zeroD<D>();
-        }
-    }
-    const long long s0 = (long long)blockIdx.x * rows_per_block;
-    long long s1 = s0 + rows_per_block;
-    if (s1 > S) s1 = S;
-    D wacc = zeroD<D>();
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    int it = 0;
-    for (long long s = s0; s < s1; s += R, it++) {
-        float4 x[R][QV];
-#pragma unroll
-        for (int r = 0; r < R; r++)
-#pragma unroll
-            for (int q = 0; q < QV; q++) {
-                long long u = t + stride * q;
-                x[r][q] = (u < nvec && s + r < s1) ? O[(s + r) * nvec + u] : z4;
-            }
-        const int buf = it & 1;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            D dot = zeroD<D>();
-#pragma unroll
-            for (int q = 0; q < QV; q++) {
-                D od[W];
-                vec_to_D<CPLX>(x[r][q], od);
-#pragma unroll
-                for (int e = 0; e < W; e++) dot = addD(dot, mulD(od[e], vr[q][e]));
-            }
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) dot = addD(dot, shflxD(dot, off));
-            if (lane == 0) red[buf][r][wv] = dot;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            if (s + r >= s1) break;
-            D w = red[buf][r][0];
-            for (int k = 1; k < nwv; k++) w = addD(w, red[buf][r][k]);
-            w = subD(w, ov);
-            wacc = addD(wacc, w);
-#pragma unroll
-            for (int q = 0; q < QV; q++) {
-                // convert again rather than keep the FP64 copies live across the barrier
-                float4 xx = x[r][q];
-                asm volatile("" : "+v"(xx.x), "+v"(xx.y), "+v"(xx.z), "+v"(xx.w));
-                D od[W];
-                vec_to_D<CPLX>(xx, od);
-#pragma unroll
-                for (int e = 0; e < W; e++) acc[q][e] = addD(acc[q][e], mulD(w, conjD(od[e])));
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < QV; q++) {
-        long long u = t + stride * q;
-        if (u < nvec)
-#pragma unroll
-            for (int e = 0; e < W; e++) part[(long long)blockIdx.x * nvec * W + u * W + e] = acc[q][e];
-    }
-    if (t == 0) wsum[blockIdx.x] = wacc;
+    f4v r = __builtin_nontemporal_load((const f4v *)p);
+    return make_float4(r.x, r.y, r.z, r.w);
 }
 
-// Wave-per-row S*v: every wavefront streams whole rows on its own (no block
-// barrier); lane l owns the 16-byte column vectors u = l + 64 q (q < QV).  v is
-// staged once per block in LDS (FP64), the accumulator stays in registers, and the
-// row dot product is a 64-lane butterfly.  Used when a row fits 64*QV vectors.
-template <bool CPLX, int QV>
-__global__ __launch_bounds__(256) void k_sv_wave_mx(const float4 *__restrict__ O, const void *v_, const void *ov_,
-                                                    long long S, long long nvec, void *part_, void *wsum_)
+// Row-group S*v.  A block holds NP row groups of WPR wavefronts.  Lane l of wave h
+// of a group owns the 16-byte vectors u = l + 64 (h + WPR q), q < QV, of the group's
+// rows; group g of block b takes rows b NP + g + k (grid NP).  Per row: partial dot
+// with v (FP64, staged once per block in LDS), 64-lane butterfly, the WPR partials
+// summed in wave order through LDS (one barrier per row when WPR > 1; the slots
+// alternate between two buffers so one barrier suffices), then acc += conj(O_s) w_s
+// in FP64 registers.  The next row's loads are issued before the current row's
+// math.  Every wave of a block runs the same number of iterations (rows past S
+// contribute w = 0), so the barriers match.  Output: one partial row of Pld values
+// per block (groups summed in order) and the block's sum of w.
+template <bool CPLX, int QV, int WPR, int NP>
+__global__ __launch_bounds__(64 * WPR * NP) void k_sv_rows_mx(const float4 *__restrict__ O, const void *v_,
+                                                               const void *ov_, long long S, long long nvec,
+                                                               void *part_, void *wsum_)
 {
     using D = typename Ty<CPLX>::D;
     constexpr int W = Vec<CPLX>::W;
     extern __shared__ __align__(16) unsigned char smem_sv[];
+    __shared__ D red[2][NP][WPR];
+    __shared__ D wred[NP];
     D *vl = (D *)smem_sv;
     const D *v = (const D *)v_;
     const D ov = *(const D *)ov_;
-    const int t = threadIdx.x, lane = t & 63;
+    const int t = threadIdx.x, lane = t & 63, h = (t >> 6) % WPR, g = t / (64 * WPR);
     for (long long q = t; q < nvec * W; q += blockDim.x) vl[q] = v[q];
     __syncthreads();
-    const long long gw = ((long long)blockIdx.x * blockDim.x + t) >> 6;
-    const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
     D acc[QV][W];
 #pragma unroll
     for (int q = 0; q < QV; q++)
@@ -662,18 +585,28 @@ __global__ __launch_bounds__(256) void k_sv_wave_mx(const float4 *__restrict__ O
         for (int e = 0; e < W; e++) acc[q][e] = zeroD<D>();
     D wacc = zeroD<D>();
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (long long s = gw; s < S; s += nw) {
-        float4 x[QV];
-        const float4 *row = O + s * nvec;
+    const long long ns = (long long)gridDim.x * NP;
+    const long long s0 = (long long)blockIdx.x * NP;
+    long long s = s0 + g;
+    float4 x[QV];
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+        long long u = lane + 64LL * (h + WPR * q);
+        x[q] = (s < S && u < nvec) ? ld_stream(O + s * nvec + u) : z4;
+    }
+    int par = 0;
+    for (long long sb = s0; sb < S; sb += ns, s += ns) {
+        float4 y[QV];
+        const long long sn = s + ns;
 #pragma unroll
         for (int q = 0; q < QV; q++) {
-            long long u = lane + 64LL * q;
-            x[q] = (u < nvec) ? row[u] : z4;
+            long long u = lane + 64LL * (h + WPR * q);
+            y[q] = (sn < S && u < nvec) ? ld_stream(O + sn * nvec + u) : z4;
         }
         D dot = zeroD<D>();
 #pragma unroll
         for (int q = 0; q < QV; q++) {
-            long long u = lane + 64LL * q;
+            long long u = lane + 64LL * (h + WPR * q);
             if (u < nvec) {
                 D od[W];
                 vec_to_D<CPLX>(x[q], od);
@@ -683,10 +616,19 @@ __global__ __launch_bounds__(256) void k_sv_wave_mx(const float4 *__restrict__ O
         }
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) dot = addD(dot, shflxD(dot, off));
-        const D w = subD(dot, ov);
+        if constexpr (WPR > 1) {
+            if (lane == 0) red[par][g][h] = dot;
+            __syncthreads();
+            dot = red[par][g][0];
+#pragma unroll
+            for (int k = 1; k < WPR; k++) dot = addD(dot, red[par][g][k]);
+            par ^= 1;
+        }
+        const D w = (s < S) ? subD(dot, ov) : zeroD<D>();
         wacc = addD(wacc, w);
 #pragma unroll
         for (int q = 0; q < QV; q++) {
+            // re-convert from the FP32 registers instead of keeping FP64 copies alive
             float4 xx = x[q];
             asm volatile("" : "+v"(xx.x), "+v"(xx.y), "+v"(xx.z), "+v"(xx.w));
             D od[W];
@@ -694,31 +636,29 @@ __global__ __launch_bounds__(256) void k_sv_wave_mx(const float4 *__restrict__ O
 #pragma unroll
             for (int e = 0; e < W; e++) acc[q][e] = addD(acc[q][e], mulD(w, conjD(od[e])));
         }
+#pragma unroll
+        for (int q = 0; q < QV; q++) x[q] = y[q];
     }
-    // sum the block's wavefront accumulators in wave order through LDS (reusing the
-    // v staging area), one partial row per block
-    const int wv = t >> 6, nwb = blockDim.x >> 6;
-    for (int k = 0; k < nwb; k++) {
+    // sum the groups' accumulators in group order through LDS (reusing the v area)
+    for (int k = 0; k < NP; k++) {
         __syncthreads();
-        if (wv == k) {
+        if (g == k) {
 #pragma unroll
             for (int q = 0; q < QV; q++) {
-                long long u = lane + 64LL * q;
+                long long u = lane + 64LL * (h + WPR * q);
                 if (u < nvec)
 #pragma unroll
                     for (int e = 0; e < W; e++) vl[u * W + e] = (k == 0) ? acc[q][e] : addD(vl[u * W + e], acc[q][e]);
             }
         }
     }
+    if (h == 0 && lane == 0) wred[g] = wacc;
     __syncthreads();
     D *part = (D *)part_ + (long long)blockIdx.x * nvec * W;
     for (long long q = t; q < nvec * W; q += blockDim.x) part[q] = vl[q];
-    __shared__ D wred[16];
-    if (lane == 0) wred[wv] = wacc;
-    __syncthreads();
     if (t == 0) {
         D ws = wred[0];
-        for (int k = 1; k < nwb; k++) ws = addD(ws, wred[k]);
+        for (int k = 1; k < NP; k++) ws = addD(ws, wred[k]);
         ((D *)wsum_)[blockIdx.x] = ws;
     }
 }
@@ -854,63 +794,7 @@ hipError_t launch_sample_c(int K, const MixedSampleArgs &a, hipStream_t st, int 
     }
 }
 
-template <bool CPLX, int QV>
-hipError_t launch_fused(const void *O, const void *v, const void *ov, long long S, long long nvec, int nblk, int nwv,
-                        void *part, void *wsum, hipStream_t st)
-{
-    constexpr int R = (QV <= 2) ? 8 : 4;
-    long long rpb = (S + nblk - 1) / nblk;
-    hipLaunchKernelGGL((k_sv_fused_mx<CPLX, QV, R>), dim3(nblk), dim3(64 * nwv), 0, st, (const float4 *)O, v, ov, S,
-                       nvec, rpb, part, wsum);
-    return hipGetLastError();
-}
-
-template <bool CPLX>
-hipError_t launch_fused_q(int qv, const void *O, const void *v, const void *ov, long long S, long long nvec,
-                          int nblk, int nwv, void *part, void *wsum, hipStream_t st)
-{
-    switch (qv) {
-    case 1: return launch_fused<CPLX, 1>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
-    case 2: return launch_fused<CPLX, 2>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
-    case 3: return launch_fused<CPLX, 3>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
-    case 4: return launch_fused<CPLX, 4>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
-    case 5: return launch_fused<CPLX, 5>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
-    case 6: return launch_fused<CPLX, 6>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
-    case 7: return launch_fused<CPLX, 7>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
-    case 8: return launch_fused<CPLX, 8>(O, v, ov, S, nvec, nblk, nwv, part, wsum, st);
-    default: return hipErrorInvalidValue;
-    }
-}
-
 } // namespace
-
-// block width (wavefronts) and column vectors per thread of the fused S*v; qv = 0
-// selects the two-pass form
-void nq_mixed_fused_shape(long long nvec, int *nwv_out, int *qv_out)
-{
-    int best_nw = 0, best_q = 0;
-    double best = -1.0;
-    for (int nw = 4; nw <= 8; nw++) {
-        long long q = (nvec + 64LL * nw - 1) / (64LL * nw);
-        if (q > 8) continue;
-        double util = (double)nvec / (64.0 * nw * q);
-        double score = util - 0.01 * q;
-        if (score > best) {
-            best = score;
-            best_nw = nw;
-            best_q = (int)q;
-        }
-    }
-    *nwv_out = best_nw;
-    *qv_out = best_q;
-}
-
-int nq_mixed_fused_qv(long long nvec)
-{
-    int nw, q;
-    nq_mixed_fused_shape(nvec, &nw, &q);
-    return q;
-}
 
 hipError_t nq_launch_tables_mx(bool cplx, int N, int M, int K, int L, const void *W, void *chT, void *shT, void *wT,
                                void *bT, void *eaP, void *eaN, void *aD, hipStream_t st)
@@ -979,49 +863,77 @@ hipError_t nq_launch_moments_mx(bool cplx, const void *O, const void *E, long lo
     return hipGetLastError();
 }
 
-// S*v raw sums: out[p] = sum_s conj(O_sp) w_s (P entries of D), out[P] = sum_s w_s,
-// w_s = O_s^T v - ov.  v, ov, out are D (double or complex double).  Picks the
-// wave-per-row kernel when a row fits 20 vectors per lane, else the tiled block
-// kernel, else two passes.  scratch: >= (nblk/32 + 2) * (Pld + 1) D.
-template <bool CPLX, int QV>
-hipError_t launch_wave(const void *O, const void *v, const void *ov, long long S, long long nvec, int nblk, void *part,
-                       void *wpart, hipStream_t st)
+// Shape of the row-group S*v for rows of nvec 16-byte vectors: the fewest waves per
+// row (1, 2, 4, 8) that keep a lane's share at <= 9 vectors, NP groups so that a block
+// has 4 waves (8 when WPR = 8), and 2048 waves in the grid (two blocks per CU at the
+// kernel's two waves per SIMD).  qv = 0: the row is too long (two-pass form).
+// NQES_SV_WPR sets the smallest WPR tried and NQES_SV_BLOCKS the block count.
+SvShape nq_sv_shape(bool cplx, long long nvec)
 {
-    using D = typename Ty<CPLX>::D;
-    size_t lds = (size_t)nvec * Vec<CPLX>::W * sizeof(D);
-    hipLaunchKernelGGL((k_sv_wave_mx<CPLX, QV>), dim3(nblk), dim3(256), lds, st, (const float4 *)O, v, ov, S, nvec,
-                       part, wpart);
+    SvShape sh{0, 0, 0, 0, 0};
+    size_t lds = (size_t)nvec * 32;  // v in FP64: 4 doubles or 2 complex per vector
+    if (lds > 160 * 1024) return sh;
+    const char *ew = getenv("NQES_SV_WPR");
+    const int wmin = (ew && *ew) ? atoi(ew) : 1;
+    for (int wpr = 1; wpr <= 8; wpr *= 2) {
+        long long q = (nvec + 64LL * wpr - 1) / (64LL * wpr);
+        if (q <= 9 && wpr >= wmin) {
+            sh.wpr = wpr;
+            sh.qv = (int)q;
+            break;
+        }
+    }
+    if (!sh.qv) return sh;
+    if (sh.wpr > 1 && sh.qv < 5) sh.qv = 5;  // instantiated shapes only; idle slots are masked
+    sh.np = sh.wpr >= 4 ? 1 : 4 / sh.wpr;
+    sh.blocks = 2048 / (sh.wpr * sh.np);
+    const char *e = getenv("NQES_SV_BLOCKS");
+    if (e && *e && atoi(e) > 0) sh.blocks = atoi(e);
+    sh.lds = lds;
+    (void)cplx;
+    return sh;
+}
+
+template <bool CPLX, int QV, int WPR, int NP>
+hipError_t launch_rows(const void *O, const void *v, const void *ov, long long S, long long nvec, int nblk,
+                       void *part, void *wpart, size_t lds, hipStream_t st)
+{
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_sv_rows_mx<CPLX, QV, WPR, NP>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_sv_rows_mx<CPLX, QV, WPR, NP>), dim3(nblk), dim3(64 * WPR * NP), lds, st,
+                       (const float4 *)O, v, ov, S, nvec, part, wpart);
     return hipGetLastError();
 }
 
 template <bool CPLX>
-hipError_t launch_wave_q(int qv, const void *O, const void *v, const void *ov, long long S, long long nvec, int nblk,
-                         void *part, void *wpart, hipStream_t st)
+hipError_t launch_rows_shape(const SvShape &sh, const void *O, const void *v, const void *ov, long long S,
+                             long long nvec, int nblk, void *part, void *wpart, hipStream_t st)
 {
-    switch (qv) {
-#define NQ_WQ(q)                                                                                                  \
-    case q: return launch_wave<CPLX, q>(O, v, ov, S, nvec, nblk, part, wpart, st);
-        NQ_WQ(1) NQ_WQ(2) NQ_WQ(3) NQ_WQ(4) NQ_WQ(5) NQ_WQ(6) NQ_WQ(7) NQ_WQ(8) NQ_WQ(9) NQ_WQ(10)
-        NQ_WQ(11) NQ_WQ(12) NQ_WQ(13) NQ_WQ(14) NQ_WQ(15) NQ_WQ(16) NQ_WQ(17) NQ_WQ(18) NQ_WQ(19) NQ_WQ(20)
-#undef NQ_WQ
-    default: return hipErrorInvalidValue;
-    }
+#define NQ_R(q, w, n)                                                                                             \
+    if (sh.qv == q && sh.wpr == w && sh.np == n)                                                                 \
+        return launch_rows<CPLX, q, w, n>(O, v, ov, S, nvec, nblk, part, wpart, sh.lds, st);
+    NQ_R(1, 1, 4) NQ_R(2, 1, 4) NQ_R(3, 1, 4) NQ_R(4, 1, 4) NQ_R(5, 1, 4) NQ_R(6, 1, 4) NQ_R(7, 1, 4)
+    NQ_R(8, 1, 4) NQ_R(9, 1, 4)
+    NQ_R(5, 2, 2) NQ_R(6, 2, 2) NQ_R(7, 2, 2) NQ_R(8, 2, 2) NQ_R(9, 2, 2)
+    NQ_R(5, 4, 1) NQ_R(6, 4, 1) NQ_R(7, 4, 1) NQ_R(8, 4, 1) NQ_R(9, 4, 1)
+    NQ_R(5, 8, 1) NQ_R(6, 8, 1) NQ_R(7, 8, 1) NQ_R(8, 8, 1) NQ_R(9, 8, 1)
+#undef NQ_R
+    return hipErrorInvalidValue;
 }
 
-int nq_sv_wave_blocks()
-{
-    const char *e = getenv("NQES_SV_BLOCKS");
-    int b = (e && *e) ? atoi(e) : 256;
-    return b > 0 ? b : 256;
-}
-
+// S*v raw sums: out[p] = sum_s conj(O_sp) w_s (P entries of D), out[P] = sum_s w_s,
+// w_s = O_s^T v - ov.  v, ov, out are D (double or complex double).  nblk partial
+// rows: the row-group kernel's block count (nq_sv_shape) or, for rows too long for
+// it, the chunk count of the two-pass form.  scratch: >= (nblk/32 + 2) * (Pld + 1) D.
 hipError_t nq_launch_sv_mx(bool cplx, const void *O, const void *v, const void *ov, long long S, long long P,
                            long long Pld, int nblk, void *part, void *wpart, void *wbuf, void *scratch, void *out,
                            hipStream_t st)
 {
     const int W = cplx ? 2 : 4;
     const long long nvec = Pld / W;
-    const long long qw = (nvec + 63) / 64;
     hipError_t e = hipSuccess;
     auto reduce = [&](long long ncol, long long stride) {
         if (cplx) {
@@ -1035,17 +947,10 @@ hipError_t nq_launch_sv_mx(bool cplx, const void *O, const void *v, const void *
         }
         return hipGetLastError();
     };
-    if (qw <= 20) {
-        e = cplx ? launch_wave_q<true>((int)qw, O, v, ov, S, nvec, nblk, part, wpart, st)
-                 : launch_wave_q<false>((int)qw, O, v, ov, S, nvec, nblk, part, wpart, st);
-        if (e != hipSuccess) return e;
-        return reduce(P, Pld);
-    }
-    int nwv = 0, qv = 0;
-    nq_mixed_fused_shape(nvec, &nwv, &qv);
-    if (qv) {
-        e = cplx ? launch_fused_q<true>(qv, O, v, ov, S, nvec, nblk, nwv, part, wpart, st)
-                 : launch_fused_q<false>(qv, O, v, ov, S, nvec, nblk, nwv, part, wpart, st);
+    const SvShape sh = nq_sv_shape(cplx, nvec);
+    if (sh.qv) {
+        e = cplx ? launch_rows_shape<true>(sh, O, v, ov, S, nvec, nblk, part, wpart, st)
+                 : launch_rows_shape<false>(sh, O, v, ov, S, nvec, nblk, part, wpart, st);
         if (e != hipSuccess) return e;
         return reduce(P, Pld);
     }

@@ -461,11 +461,9 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
         HIPCHK(c, c->dsvpart.ensure((size_t)c->svchunks * P * 16), "alloc S*v partials");
     } else {
         const long long nvec = c->Pld / (cplx ? 2 : 4);
-        long long nblk;
-        if ((nvec + 63) / 64 <= 20) nblk = nq_sv_wave_blocks();   // wave-per-row kernel
-        else if (nq_mixed_fused_qv(nvec)) nblk = 1024;            // tiled block kernel
-        else nblk = choose_chunks(S, P, 256);                     // two passes
-        if (nblk > S) nblk = S;
+        const SvShape sh = nq_sv_shape(cplx, nvec);
+        long long nblk = sh.qv ? sh.blocks : choose_chunks(S, P, 256);  // one pass / two passes
+        if (!sh.qv && nblk > S) nblk = S;
         c->svchunks = (int)nblk;
         HIPCHK(c, c->dsvpart.ensure((size_t)c->svchunks * c->Pld * 16), "alloc S*v partials");
         HIPCHK(c, c->dsvwpart.ensure((size_t)c->svchunks * 16), "alloc S*v partials");
