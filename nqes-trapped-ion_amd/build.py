@@ -36,6 +36,7 @@ EXACT = ["-ffp-contract=off"]
 SOURCES = [
     ("kernels_fp64.hip", EXACT),
     ("kernels_mixed.hip", []),
+    ("sv_factored.hip", []),
     ("nqes_api.cpp", EXACT),
 ]
 

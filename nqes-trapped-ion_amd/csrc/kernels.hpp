@@ -60,7 +60,34 @@ struct MixedSampleArgs {
     uint32_t *spins;           // S*K*NW
     unsigned long long *accepts; // C
     unsigned int *flags;       // bit 0: an angle exceeded the FP32 range
+    void *U;                   // (S*K) x Ucols T: S*v factors (NULL: not stored)
+    int Ucols;
 };
+
+// Factored S*v (sv_factored.hip): raw[p] = sum_s conj(O_sp) w_s, raw[P] = sum_s w_s,
+// w_s = O_s^T v - ov, from the spins and the factor rows U (row r = s*K + kk:
+// [k*M + j] = c_{s,k,kk} tanh(theta)_{kk,k,j}, [K*M + k] = c_{s,k,kk}, zero padded).
+struct SvfArgs {
+    int N, M, K, NW, Ucols;
+    int n4;                    // K dimension of the forward MFMA: N rounded up to 4
+    long long S, P, L;
+    const uint32_t *spins;     // S*K*NW
+    const void *U;             // (S*K) x Ucols T
+    const void *v;             // P D
+    const void *ov;            // 1 D
+    void *wrow;                // S*K D
+    void *w;                   // S D
+    void *part;                // bwd_blocks x (P + 1) D
+};
+struct SvfShape {
+    int ucols, ppw, bwd_waves, bwd_blocks, fwd_blocks;
+};
+SvfShape nq_svf_shape(bool cplx, int N, int M, int K);
+hipError_t nq_launch_svf(bool cplx, const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st);
+// out[col] = sum over chunks of part[chunk*stride + col] in chunk order (D = double or
+// complex double); scratch >= (nchunk/32 + 1) * ncol D
+hipError_t nq_chunk_reduce(bool cplx, const void *part, long long ncol, long long stride, int nchunk,
+                           void *scratch, void *out, hipStream_t st);
 
 hipError_t nq_launch_tables_mx(bool cplx, int N, int M, int K, int L, const void *W, void *chT, void *shT, void *wT,
                                void *bT, void *eaP, void *eaN, void *aD, hipStream_t st);
@@ -69,6 +96,7 @@ hipError_t nq_launch_chain_init(int N, int K, int NW, int C, uint64_t seed, uint
 hipError_t nq_launch_theta_gemm(bool cplx, int N, int M, int K, int NW, int C, const uint32_t *spins0,
                                 const void *wT, const void *bT, void *theta0, hipStream_t st);
 hipError_t nq_launch_sample_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st);
+hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st);
 size_t nq_sample_mx_lds(bool cplx, int N, int M, int K, int G, int wpb, bool tab_lds);
 hipError_t nq_launch_moments_mx(bool cplx, const void *O, const void *E, long long S, long long P, long long Pld,
                                 int nchunk, void *po, void *ps, void *pf, void *scratch, void *out_o, void *out_s,

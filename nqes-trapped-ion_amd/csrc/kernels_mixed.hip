@@ -11,8 +11,10 @@
 //                    (v_mfma_f32_16x16x4f32), the angles at chain start (rbm.cc:135-145)
 //   k_sample_mx      G lanes per chain, 64/G chains per wavefront; Metropolis sweeps
 //                    (rbm.cc:530-552) with tanh theta in LDS updated per accepted flip;
-//                    after each sampled sweep: theta, tanh, Phi and Phi^-1 recomputed from
-//                    the spins (no drift), local energy (rbm.cc:1042-1064) and the O row
+//                    after every sweep theta, tanh and Phi are recomputed from the spins
+//                    (drift bounded to one sweep), after each sampled sweep the spins stored
+//   k_measure_mx     G lanes per sample: theta, tanh, Phi, Phi^-1 from the stored spins,
+//                    local energy (rbm.cc:1042-1064) and the O row
 //                    O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k}
 //                    (Cramer's rule applied to rbm.cc:199-239) written to HBM in FP32
 //   k_moments_mx     column sums of O, |O|^2, E conj(O) in FP64 with Neumaier compensation
@@ -32,16 +34,6 @@ using namespace nqm;
 using nq64::cd;
 
 namespace {
-
-template <bool CPLX> struct Ty;
-template <> struct Ty<false> {
-    using T = float;
-    using D = double;
-};
-template <> struct Ty<true> {
-    using T = cf;
-    using D = cd;
-};
 
 __device__ inline double fromW(cd w, double) { return w.re; }
 __device__ inline cd fromW(cd w, cd) { return w; }
@@ -168,6 +160,263 @@ __device__ inline float reT(cf z) { return z.re; }
 __device__ inline double expD(double x) { return nq64::exp_(x); }
 __device__ inline cd expD(cd z) { return nq64::cexp_(z); }
 
+// Theta -> tanh (tl, LDS) and Phi for one chain state held by a group of G lanes.
+// Lane g owns hidden units j = g + G q.  Source: Theta0 of chain c (chain start) or
+// the spins sp (LDS).  Phi_{k,kk} = exp(sum_i +-a_i) prod_j cosh(theta_j) with the
+// cosh product taken per lane in FP64, then across the group.
+template <bool CPLX, int K, int G, int JMAX>
+__device__ inline void mx_refresh(const MixedSampleArgs &a, const uint32_t *sp, typename Ty<CPLX>::T *tl, int g,
+                                  long long c, bool from_theta0, typename Ty<CPLX>::D *phi, int &overflow)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    const int N = a.N, M = a.M, NW = a.NW;
+    const int jpl = (M + G - 1) / G;
+    const T *wT = (const T *)a.wT;
+    const T *bT = (const T *)a.bT;
+    const D *aD = (const D *)a.aD;
+    D part[K * K];
+#pragma unroll
+    for (int q = 0; q < K * K; q++) part[q] = oneD<D>();
+#pragma unroll
+    for (int q = 0; q < JMAX; q++) {
+        int j = g + G * q;
+        if (q < jpl && j < M) {
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                T th[K];
+                if (from_theta0) {
+#pragma unroll
+                    for (int k = 0; k < K; k++) th[k] = ((const T *)a.theta0)[((c * K + k) * K + kk) * M + j];
+                } else {
+                    T b = bT[kk * M + j];
+#pragma unroll
+                    for (int k = 0; k < K; k++) th[k] = b;
+                    for (int i = 0; i < N; i++) {
+                        T w = wT[((long long)kk * N + i) * M + j];
+#pragma unroll
+                        for (int k = 0; k < K; k++) {
+                            int bit = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                            th[k] = bit ? subT(th[k], w) : addT(th[k], w);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    if (fabsf(reT(th[k])) > 80.0f) overflow = 1;
+                    tl[(k * K + kk) * M + j] = tanhT(th[k]);
+                    part[k * K + kk] = mulD(part[k * K + kk], toD(coshT(th[k])));
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            D A = zeroD<D>();
+            for (int i = 0; i < N; i++) {
+                int bit = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                D ai = aD[kk * N + i];
+                A = bit ? subD(A, ai) : addD(A, ai);
+            }
+            phi[k * K + kk] = mulD(expD(A), group_prod<G>(part[k * K + kk]));
+        }
+}
+
+// Local energy and O row of sample s from its state (sp, tl, phi) held by a group
+// of G lanes: E_zz (rbm.cc:1012-1031), H_loc = Phi^-1 (diag(E_zz) Phi - h sum_i
+// Phi^(i)) (rbm.cc:1042-1064), O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with
+// c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k} (Cramer's rule applied to rbm.cc:199-239).
+template <bool CPLX, int K, int G>
+__device__ inline void mx_measure(const MixedSampleArgs &a, const uint32_t *sp, const typename Ty<CPLX>::T *tl,
+                                  const typename Ty<CPLX>::T *chT, const typename Ty<CPLX>::T *shT, int g,
+                                  const typename Ty<CPLX>::D *phi, long long s, bool live)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    const int N = a.N, M = a.M, NW = a.NW, L = a.L;
+    const D *eaP = (const D *)a.eaP;
+    const D *eaN = (const D *)a.eaN;
+    D phi_inv[K * K];
+    inv_m<K, D>(phi, phi_inv);
+
+    // E_zz per replica: lanes take rows i = g, g+G, ...
+    double ez[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double r = 0.0;
+        for (int i = g; i < N; i += G) {
+            int bi = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+            for (int j = 0; j < i; j++) {
+                int bj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
+                double v = a.J[(long long)i * N + j];
+                r += (bi ^ bj) ? -v : v;
+            }
+        }
+        ez[k] = 2.0 * group_sum<G>(r);
+    }
+    // sum_i Phi^(i): lanes take spins i = g, g+G, ...; the j loop starts at a
+    // lane-dependent offset to spread LDS banks
+    D H[K * K];
+    {
+        D sx[K * K];
+#pragma unroll
+        for (int q = 0; q < K * K; q++) sx[q] = zeroD<D>();
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                const T *tk = tl + (k * K + kk) * M;
+                for (int i = g; i < N; i += G) {
+                    const int si = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                    const T *chr = chT + ((long long)kk * N + i) * M;
+                    const T *shr = shT + ((long long)kk * N + i) * M;
+                    D prod = oneD<D>();
+                    T pf = oneT<T>();
+                    int j = g % M;
+                    for (int jj = 0; jj < M; jj++) {
+                        T shv = shr[j];
+                        if (si) shv = negT(shv);
+                        pf = mulT(pf, subT(chr[j], mulT(tk[j], shv)));
+                        if ((jj & 7) == 7) {
+                            prod = mulD(prod, toD(pf));
+                            pf = oneT<T>();
+                        }
+                        if (++j == M) j = 0;
+                    }
+                    prod = mulD(prod, toD(pf));
+                    D e = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                    sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e), prod));
+                }
+            }
+        D res[K * K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                D sxs = group_sum<G>(sx[k * K + kk]);
+                res[k * K + kk] = subD(scaleD(ez[k], phi[k * K + kk]), scaleD(a.h, sxs));
+            }
+        matmul_m<K, D>(phi_inv, res, H);
+    }
+    D energy = zeroD<D>();
+#pragma unroll
+    for (int k = 0; k < K; k++) energy = addD(energy, H[k * K + k]);
+    if (live) {
+        cd *Hs = (cd *)a.Hs;
+        cd *Es = (cd *)a.Es;
+#pragma unroll
+        for (int q = 0; q < K * K; q++)
+            if (g == q) {
+                cd hv;
+                if constexpr (CPLX) hv = H[q];
+                else hv = nq64::mk(H[q], 0.0);
+                Hs[s * K * K + q] = hv;
+            }
+        if (g == 0) {
+            cd ev;
+            if constexpr (CPLX) ev = energy;
+            else ev = nq64::mk(energy, 0.0);
+            Es[s] = ev;
+        }
+    }
+
+    // O row: O_{k,p} = sum_kk c_{k,kk} d_{kk,p}; factor rows U[s K + kk]
+    if (live && (a.O || a.U)) {
+        T cT[K * K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) cT[k * K + kk] = toT(mulD(phi_inv[k * K + kk], phi[kk * K + k]));
+        T *Orow = a.O ? (T *)a.O + s * a.Pld : nullptr;
+        T *Ub = a.U ? (T *)a.U + s * K * (long long)a.Ucols : nullptr;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            T ck[K];
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) ck[kk] = cT[k * K + kk];
+            T *ob = Orow + (long long)k * L;
+            // a block: sum_kk c sigma_i^kk
+            if (Orow)
+                for (int i = g; i < N; i += G) {
+                    T v = zeroT<T>();
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
+                        v = bit ? subT(v, ck[kk]) : addT(v, ck[kk]);
+                    }
+                    ob[i] = v;
+                }
+            // b block and w block: lanes over j
+            for (int j = g; j < M; j += G) {
+                T ct[K];
+                T v = zeroT<T>();
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    ct[kk] = mulT(ck[kk], tl[(kk * K + k) * M + j]);
+                    v = addT(v, ct[kk]);
+                }
+                if (Ub)
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) Ub[(long long)kk * a.Ucols + k * M + j] = ct[kk];
+                if (!Orow) continue;
+                ob[N + j] = v;
+                for (int i = 0; i < N; i++) {
+                    T w = zeroT<T>();
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
+                        w = bit ? subT(w, ct[kk]) : addT(w, ct[kk]);
+                    }
+                    ob[N + M + (long long)i * M + j] = w;
+                }
+            }
+        }
+        if (Ub)
+            for (int col = K * M + g; col < a.Ucols; col += G)
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    T val = zeroT<T>();
+#pragma unroll
+                    for (int k = 0; k < K; k++)
+                        if (col - K * M == k) val = cT[k * K + kk];
+                    Ub[(long long)kk * a.Ucols + col] = val;
+                }
+        if (Orow)
+            for (long long q = a.P + g; q < a.Pld; q += G) Orow[q] = zeroT<T>();
+    }
+}
+
+// stage the cosh/sinh tables in LDS when a.tab_lds; returns the LDS cursor after them
+template <typename T>
+__device__ inline unsigned char *mx_stage_tables(const MixedSampleArgs &a, long long KNM, const T *&chT,
+                                                 const T *&shT, unsigned char *p)
+{
+    chT = (const T *)a.chT;
+    shT = (const T *)a.shT;
+    if (a.tab_lds) {
+        T *lch = (T *)p;
+        T *lsh = lch + KNM;
+        for (long long q = threadIdx.x; q < KNM; q += blockDim.x) {
+            lch[q] = chT[q];
+            lsh[q] = shT[q];
+        }
+        chT = lch;
+        shT = lsh;
+        p += 2 * KNM * sizeof(T);
+    }
+    return p;
+}
+
+// Metropolis chains: G lanes per chain, 64/G chains per wavefront.  Sweeps
+// (rbm.cc:530-552) with tanh theta in LDS updated per accepted flip.  After every
+// sweep (burn-in included) theta, tanh and Phi are recomputed from the spins, so the
+// FP32 rounding of the incremental updates never spans more than one sweep: with
+// nearly equal RBMs (small initial weights, K = 4) det Phi can be ~1e-6 of its scale,
+// and longer drift would bias acceptances.  After each sampled sweep the spins are
+// stored (sample s = c m + t); the local energies and O rows are computed afterwards
+// by k_measure_mx, one lane group per sample.
 template <bool CPLX, int K, int G, int JMAX>
 __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
 {
@@ -175,7 +424,7 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
     using D = typename Ty<CPLX>::D;
     constexpr int CPW = 64 / G;
     extern __shared__ __align__(16) unsigned char smem[];
-    const int N = a.N, M = a.M, NW = a.NW, L = a.L;
+    const int N = a.N, M = a.M, NW = a.NW;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int g = lane & (G - 1), grp = lane / G;
     const int cpb = (blockDim.x >> 6) * CPW;
@@ -187,27 +436,12 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
     const long long KNM = (long long)K * N * M;
 
     // LDS: [ch tables][sh tables] (if a.tab_lds) [tanh per chain][spins per chain]
-    unsigned char *p = smem;
-    const T *chT = (const T *)a.chT;
-    const T *shT = (const T *)a.shT;
-    if (a.tab_lds) {
-        T *lch = (T *)p;
-        T *lsh = lch + KNM;
-        for (long long q = tid; q < KNM; q += blockDim.x) {
-            lch[q] = chT[q];
-            lsh[q] = shT[q];
-        }
-        chT = lch;
-        shT = lsh;
-        p += 2 * KNM * sizeof(T);
-    }
+    const T *chT, *shT;
+    unsigned char *p = mx_stage_tables<T>(a, KNM, chT, shT, smem);
     T *tl = (T *)p + (long long)cib * K * K * M;
     uint32_t *sp = (uint32_t *)((T *)p + (long long)cpb * K * K * M) + cib * K * NW;
-    const T *wT = (const T *)a.wT;
-    const T *bT = (const T *)a.bT;
     const D *eaP = (const D *)a.eaP;
     const D *eaN = (const D *)a.eaN;
-    const D *aD = (const D *)a.aD;
 
     if (g == 0)
         for (int w = 0; w < K * NW; w++) sp[w] = a.spins0[(long long)c * K * NW + w];
@@ -218,60 +452,7 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
 
     D phi[K * K];
     int overflow = 0;
-
-    // theta -> tanh (LDS) and Phi; source: Theta0 (chain start) or the spins
-    auto refresh = [&](bool from_theta0) {
-        D part[K * K];
-#pragma unroll
-        for (int q = 0; q < K * K; q++) part[q] = oneD<D>();
-#pragma unroll
-        for (int q = 0; q < JMAX; q++) {
-            int j = g + G * q;
-            if (q < jpl && j < M) {
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) {
-                    T th[K];
-                    if (from_theta0) {
-#pragma unroll
-                        for (int k = 0; k < K; k++)
-                            th[k] = ((const T *)a.theta0)[(((long long)c * K + k) * K + kk) * M + j];
-                    } else {
-                        T b = bT[kk * M + j];
-#pragma unroll
-                        for (int k = 0; k < K; k++) th[k] = b;
-                        for (int i = 0; i < N; i++) {
-                            T w = wT[((long long)kk * N + i) * M + j];
-#pragma unroll
-                            for (int k = 0; k < K; k++) {
-                                int bit = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-                                th[k] = bit ? subT(th[k], w) : addT(th[k], w);
-                            }
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        if (fabsf(reT(th[k])) > 80.0f) overflow = 1;
-                        tl[(k * K + kk) * M + j] = tanhT(th[k]);
-                        part[k * K + kk] = mulD(part[k * K + kk], toD(coshT(th[k])));
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < K; k++)
-#pragma unroll
-            for (int kk = 0; kk < K; kk++) {
-                D A = zeroD<D>();
-                for (int i = 0; i < N; i++) {
-                    int bit = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-                    D ai = aD[kk * N + i];
-                    A = bit ? subD(A, ai) : addD(A, ai);
-                }
-                phi[k * K + kk] = mulD(expD(A), group_prod<G>(part[k * K + kk]));
-            }
-    };
-
-    refresh(true);
+    mx_refresh<CPLX, K, G, JMAX>(a, sp, tl, g, c, true, phi, overflow);
     unsigned long long accepts = 0;
     const int nsweeps = a.B + a.m;
     for (int sw = 0; sw < nsweeps; sw++) {
@@ -338,148 +519,55 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
                 }
             }
         }
-        if (sw < a.B) continue;
         __syncthreads();
-
-        // ---- one sample
-        const long long s = (long long)c * a.m + (sw - a.B);
-        refresh(false);
-        D phi_inv[K * K];
-        inv_m<K, D>(phi, phi_inv);
-
-        // E_zz per replica: lanes take rows i = g, g+G, ...
-        double ez[K];
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            double r = 0.0;
-            for (int i = g; i < N; i += G) {
-                int bi = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-                for (int j = 0; j < i; j++) {
-                    int bj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
-                    double v = a.J[(long long)i * N + j];
-                    r += (bi ^ bj) ? -v : v;
-                }
-            }
-            ez[k] = 2.0 * group_sum<G>(r);
-        }
-        // sum_i Phi^(i): lanes take spins i = g, g+G, ...; the j loop starts at a
-        // lane-dependent offset to spread LDS banks
-        D H[K * K];
-        {
-            D sx[K * K];
-#pragma unroll
-            for (int q = 0; q < K * K; q++) sx[q] = zeroD<D>();
-#pragma unroll
-            for (int k = 0; k < K; k++)
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) {
-                    const T *tk = tl + (k * K + kk) * M;
-                    for (int i = g; i < N; i += G) {
-                        const int si = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-                        const T *chr = chT + ((long long)kk * N + i) * M;
-                        const T *shr = shT + ((long long)kk * N + i) * M;
-                        D prod = oneD<D>();
-                        T pf = oneT<T>();
-                        int j = g % M;
-                        for (int jj = 0; jj < M; jj++) {
-                            T shv = shr[j];
-                            if (si) shv = negT(shv);
-                            pf = mulT(pf, subT(chr[j], mulT(tk[j], shv)));
-                            if ((jj & 7) == 7) {
-                                prod = mulD(prod, toD(pf));
-                                pf = oneT<T>();
-                            }
-                            if (++j == M) j = 0;
-                        }
-                        prod = mulD(prod, toD(pf));
-                        D e = si ? eaP[kk * N + i] : eaN[kk * N + i];
-                        sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e), prod));
-                    }
-                }
-            D res[K * K];
-#pragma unroll
-            for (int k = 0; k < K; k++)
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) {
-                    D sxs = group_sum<G>(sx[k * K + kk]);
-                    res[k * K + kk] = subD(scaleD(ez[k], phi[k * K + kk]), scaleD(a.h, sxs));
-                }
-            matmul_m<K, D>(phi_inv, res, H);
-        }
-        D energy = zeroD<D>();
-#pragma unroll
-        for (int k = 0; k < K; k++) energy = addD(energy, H[k * K + k]);
-        if (live) {
-            cd *Hs = (cd *)a.Hs;
-            cd *Es = (cd *)a.Es;
-#pragma unroll
-            for (int q = 0; q < K * K; q++)
-                if (g == q) {
-                    cd hv;
-                    if constexpr (CPLX) hv = H[q];
-                    else hv = nq64::mk(H[q], 0.0);
-                    Hs[s * K * K + q] = hv;
-                }
-            if (g == 0) {
-                cd ev;
-                if constexpr (CPLX) ev = energy;
-                else ev = nq64::mk(energy, 0.0);
-                Es[s] = ev;
-            }
+        if (sw >= a.B && live) {
+            const long long s = (long long)c * a.m + (sw - a.B);
             for (int w = g; w < K * NW; w += G) a.spins[s * K * NW + w] = sp[w];
         }
-
-        // O row: O_{k,p} = sum_kk c_{k,kk} d_{kk,p}
-        if (live && a.O) {
-            T cT[K * K];
-#pragma unroll
-            for (int k = 0; k < K; k++)
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) cT[k * K + kk] = toT(mulD(phi_inv[k * K + kk], phi[kk * K + k]));
-            T *Orow = (T *)a.O + s * a.Pld;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                T ck[K];
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) ck[kk] = cT[k * K + kk];
-                T *ob = Orow + (long long)k * L;
-                // a block: sum_kk c sigma_i^kk
-                for (int i = g; i < N; i += G) {
-                    T v = zeroT<T>();
-#pragma unroll
-                    for (int kk = 0; kk < K; kk++) {
-                        int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
-                        v = bit ? subT(v, ck[kk]) : addT(v, ck[kk]);
-                    }
-                    ob[i] = v;
-                }
-                // b block and w block: lanes over j
-                for (int j = g; j < M; j += G) {
-                    T ct[K];
-                    T v = zeroT<T>();
-#pragma unroll
-                    for (int kk = 0; kk < K; kk++) {
-                        ct[kk] = mulT(ck[kk], tl[(kk * K + k) * M + j]);
-                        v = addT(v, ct[kk]);
-                    }
-                    ob[N + j] = v;
-                    for (int i = 0; i < N; i++) {
-                        T w = zeroT<T>();
-#pragma unroll
-                        for (int kk = 0; kk < K; kk++) {
-                            int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
-                            w = bit ? subT(w, ct[kk]) : addT(w, ct[kk]);
-                        }
-                        ob[N + M + (long long)i * M + j] = w;
-                    }
-                }
-            }
-            for (long long q = a.P + g; q < a.Pld; q += G) Orow[q] = zeroT<T>();
-        }
+        if (sw + 1 < nsweeps) mx_refresh<CPLX, K, G, JMAX>(a, sp, tl, g, c, false, phi, overflow);
         __syncthreads();
     }
     if (live && g == 0) a.accepts[c] = accepts;
     if (live && overflow) atomicOr(a.flags, 1u);
+}
+
+// Local energies and O rows of all S samples from their stored spins: G lanes per
+// sample, 64/G samples per wavefront, each block looping over sample batches (grid
+// sized to the CUs) so that the tables are staged in LDS once per block.  Sample
+// parallelism (S) instead of chain parallelism (C) keeps every SIMD busy.
+template <bool CPLX, int K, int G, int JMAX>
+__global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    constexpr int SPW = 64 / G;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = a.N, M = a.M, NW = a.NW;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int g = lane & (G - 1), grp = lane / G;
+    const int spb = (blockDim.x >> 6) * SPW;
+    const int sib = wv * SPW + grp;
+    const long long S = (long long)a.C * a.m;
+    const long long KNM = (long long)K * N * M;
+
+    const T *chT, *shT;
+    unsigned char *p = mx_stage_tables<T>(a, KNM, chT, shT, smem);
+    T *tl = (T *)p + (long long)sib * K * K * M;
+    uint32_t *sp = (uint32_t *)((T *)p + (long long)spb * K * K * M) + sib * K * NW;
+    int overflow = 0;
+    for (long long base = (long long)blockIdx.x * spb; base < S; base += (long long)gridDim.x * spb) {
+        const long long s_raw = base + sib;
+        const bool live = s_raw < S;
+        const long long s = live ? s_raw : S - 1;
+        __syncthreads();  // previous batch done with sp / tl (and the tables staged)
+        if (g == 0)
+            for (int w = 0; w < K * NW; w++) sp[w] = a.spins[s * K * NW + w];
+        __syncthreads();
+        D phi[K * K];
+        mx_refresh<CPLX, K, G, JMAX>(a, sp, tl, g, 0, false, phi, overflow);
+        mx_measure<CPLX, K, G>(a, sp, tl, chT, shT, g, phi, s, live);
+    }
+    if (overflow) atomicOr(a.flags, 1u);
 }
 
 // ---------------------------------------------------------------- moments
@@ -771,6 +859,56 @@ hipError_t launch_sample_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
     return hipGetLastError();
 }
 
+template <bool CPLX, int K, int G>
+hipError_t launch_measure_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
+{
+    using T = typename Ty<CPLX>::T;
+    const int spb = wpb * (64 / G);
+    size_t lds = (size_t)spb * K * K * a.M * sizeof(T) + (size_t)spb * K * a.NW * 4;
+    if (a.tab_lds) lds += 2 * (size_t)K * a.N * a.M * sizeof(T);
+    const long long S = (long long)a.C * a.m;
+    long long blocks = (S + spb - 1) / spb;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_measure_mx<CPLX, K, G, 8>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    // resident blocks: limited by LDS (160 KB per CU) and by 8 waves per SIMD
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    long long per_cu = lds > 0 ? (long long)(160 * 1024) / (long long)lds : 8;
+    if (per_cu > 32 / wpb) per_cu = 32 / wpb;
+    if (per_cu < 1) per_cu = 1;
+    const char *eb = getenv("NQES_MEASURE_BLOCKS_PER_CU");
+    if (eb && *eb && atoi(eb) > 0) per_cu = atoi(eb);
+    if (blocks > per_cu * ncu) blocks = per_cu * ncu;
+    hipLaunchKernelGGL((k_measure_mx<CPLX, K, G, 8>), dim3((unsigned)blocks), dim3(64 * wpb), lds, st, a);
+    return hipGetLastError();
+}
+
+template <bool CPLX, int K>
+hipError_t launch_measure_k(const MixedSampleArgs &a, hipStream_t st, int G, int wpb)
+{
+    switch (G) {
+    case 16: return launch_measure_g<CPLX, K, 16>(a, st, wpb);
+    case 32: return launch_measure_g<CPLX, K, 32>(a, st, wpb);
+    case 64: return launch_measure_g<CPLX, K, 64>(a, st, wpb);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <bool CPLX>
+hipError_t launch_measure_c(int K, const MixedSampleArgs &a, hipStream_t st, int G, int wpb)
+{
+    switch (K) {
+    case 1: return launch_measure_k<CPLX, 1>(a, st, G, wpb);
+    case 2: return launch_measure_k<CPLX, 2>(a, st, G, wpb);
+    case 3: return launch_measure_k<CPLX, 3>(a, st, G, wpb);
+    case 4: return launch_measure_k<CPLX, 4>(a, st, G, wpb);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <bool CPLX, int K>
 hipError_t launch_sample_k(const MixedSampleArgs &a, hipStream_t st, int G, int wpb)
 {
@@ -795,6 +933,14 @@ hipError_t launch_sample_c(int K, const MixedSampleArgs &a, hipStream_t st, int 
 }
 
 } // namespace
+
+hipError_t nq_chunk_reduce(bool cplx, const void *part, long long ncol, long long stride, int nchunk,
+                           void *scratch, void *out, hipStream_t st)
+{
+    if (cplx) chunk_reduce<cd>((const cd *)part, ncol, stride, nchunk, (cd *)scratch, (cd *)out, st);
+    else chunk_reduce<double>((const double *)part, ncol, stride, nchunk, (double *)scratch, (double *)out, st);
+    return hipGetLastError();
+}
 
 hipError_t nq_launch_tables_mx(bool cplx, int N, int M, int K, int L, const void *W, void *chT, void *shT, void *wT,
                                void *bT, void *eaP, void *eaN, void *aD, hipStream_t st)
@@ -833,6 +979,11 @@ hipError_t nq_launch_theta_gemm(bool cplx, int N, int M, int K, int NW, int C, c
 hipError_t nq_launch_sample_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st)
 {
     return cplx ? launch_sample_c<true>(K, a, st, G, wpb) : launch_sample_c<false>(K, a, st, G, wpb);
+}
+
+hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st)
+{
+    return cplx ? launch_measure_c<true>(K, a, st, G, wpb) : launch_measure_c<false>(K, a, st, G, wpb);
 }
 
 size_t nq_sample_mx_lds(bool cplx, int N, int M, int K, int G, int wpb, bool tab_lds)

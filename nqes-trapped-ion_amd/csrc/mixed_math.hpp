@@ -20,6 +20,17 @@ struct cf {
     float re, im;
 };
 
+// per-element (T) and accumulation (D) types of the real / complex instantiation
+template <bool CPLX> struct Ty;
+template <> struct Ty<false> {
+    using T = float;
+    using D = double;
+};
+template <> struct Ty<true> {
+    using T = cf;
+    using D = cd;
+};
+
 // ---------------------------------------------------------------- float / cf
 __device__ __host__ inline cf mkf(float re, float im)
 {

@@ -96,10 +96,12 @@ struct nqes_ctx {
     bool cplx = true;          // mixed path: complex arithmetic (false: all weights real)
     int C = 0, m = 0, B = 0;
     long long S = 0, Pld = 0;
-    int G = 64, wpb = 1, tab_lds = 0;
     DevBuf dO, dHs, dEs, dspins, dacc;
     DevBuf dmpart_o, dmpart_s, dmpart_f, dmo, dms, dmf, dhpart, dhsum, depart, desum;
     DevBuf dv, dov, dw, dsvpart, dsvwpart, dsvout, dsvscratch, dmscratch;
+    DevBuf dU, dwrow, dsvfpart, dsvfscratch;  // factored S*v (mixed path)
+    bool svf = false;
+    SvfShape svf_shape{0, 0, 0, 0, 0};
     int mchunks = 1, svchunks = 1;
     std::vector<cd> o, skk, f, emat;
     cd eloc{0, 0};
@@ -147,21 +149,24 @@ static long long o_ld(long long P, int prec, bool cplx)
     return (P + per16 - 1) / per16 * per16;
 }
 
-// lane-group size and waves per block of the mixed sampler
-static void mixed_shape(nqes_ctx *c, bool cplx)
+// lane-group size, waves per block and table staging of a mixed kernel with G lanes
+// per chain (sweeps) or per sample (measurement); the LDS layout is the same
+struct MixedShape {
+    int G, wpb, tab;
+};
+static MixedShape mixed_shape(const nqes_ctx *c, bool cplx, int g_dflt, const char *g_env_name,
+                              const char *wpb_env_name)
 {
-    int G = 16;
+    int G = g_dflt;
     while (G < 64 && (c->M + G - 1) / G > 8) G *= 2;
-    int g_env = env_int("NQES_G", 0);
+    int g_env = env_int(g_env_name, 0);
     if ((g_env == 16 || g_env == 32 || g_env == 64) && (c->M + g_env - 1) / g_env <= 8) G = g_env;
-    int wpb = env_int("NQES_WPB", 4);
+    int wpb = env_int(wpb_env_name, 4);
     if (wpb < 1 || wpb > 4) wpb = 4;
     int tab = env_int("NQES_TABLDS", 1);
     while (wpb > 1 && nq_sample_mx_lds(cplx, c->N, c->M, c->K, G, wpb, false) > c->lds_max) wpb--;
     if (tab && nq_sample_mx_lds(cplx, c->N, c->M, c->K, G, wpb, true) > c->lds_max) tab = 0;
-    c->G = G;
-    c->wpb = wpb;
-    c->tab_lds = tab;
+    return MixedShape{G, wpb, tab};
 }
 
 extern "C" {
@@ -322,7 +327,8 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     HIPCHK(c, nq_launch_theta_gemm(cplx, c->N, c->M, c->K, c->NW, C, (const uint32_t *)c->dspins0.p, c->dwT.p,
                                    c->dbT.p, c->dtheta0.p, c->stream),
            "theta GEMM kernel");
-    mixed_shape(c, cplx);
+    const MixedShape sw = mixed_shape(c, cplx, 32, "NQES_G", "NQES_WPB");
+    const MixedShape ms = mixed_shape(c, cplx, 16, "NQES_GM", "NQES_WPBM");
     MixedSampleArgs a;
     a.N = c->N;
     a.M = c->M;
@@ -345,14 +351,18 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.theta0 = c->dtheta0.p;
     a.spins0 = (const uint32_t *)c->dspins0.p;
     a.rng0 = (const uint64_t *)c->drng0.p;
-    a.tab_lds = c->tab_lds;
+    a.tab_lds = sw.tab;
     a.O = c->dO.p;
     a.Hs = c->dHs.p;
     a.Es = c->dEs.p;
     a.spins = static_cast<uint32_t *>(c->dspins.p);
     a.accepts = static_cast<unsigned long long *>(c->dacc.p);
     a.flags = static_cast<unsigned int *>(c->dflags.p);
-    HIPCHK(c, nq_launch_sample_mx(cplx, c->K, a, c->G, c->wpb, c->stream), "mixed sample kernel");
+    a.U = c->svf ? c->dU.p : nullptr;
+    a.Ucols = c->svf_shape.ucols;
+    HIPCHK(c, nq_launch_sample_mx(cplx, c->K, a, sw.G, sw.wpb, c->stream), "mixed sample kernel");
+    a.tab_lds = ms.tab;
+    HIPCHK(c, nq_launch_measure_mx(cplx, c->K, a, ms.G, ms.wpb, c->stream), "mixed measure kernel");
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
     HIPCHK(c, c->dmscratch.ensure((size_t)(c->mchunks / 32 + 2) * c->P * 16), "alloc moments scratch");
     HIPCHK(c, nq_launch_moments_mx(cplx, c->dO.p, c->dEs.p, S, c->P, c->Pld, c->mchunks, c->dmpart_o.p,
@@ -383,6 +393,21 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     }
     c->cplx = cplx;
     c->Pld = o_ld(P, c->prec, cplx);
+    // factored S*v (mixed path) unless NQES_SV=stored or the shape does not apply
+    c->svf = false;
+    if (c->prec == NQES_PREC_MIXED) {
+        const char *e = getenv("NQES_SV");
+        c->svf_shape = nq_svf_shape(cplx, c->N, c->M, K);
+        c->svf = c->svf_shape.ppw > 0 && !(e && strcmp(e, "stored") == 0);
+    }
+    if (c->svf) {
+        const size_t tsz = cplx ? 8 : 4;
+        const long long nb = c->svf_shape.bwd_blocks;
+        HIPCHK(c, c->dU.ensure((size_t)S * K * c->svf_shape.ucols * tsz), "alloc S*v factors");
+        HIPCHK(c, c->dwrow.ensure((size_t)S * K * 16), "alloc S*v rows");
+        HIPCHK(c, c->dsvfpart.ensure((size_t)nb * (P + 1) * 16), "alloc S*v partials");
+        HIPCHK(c, c->dsvfscratch.ensure((size_t)(nb / 32 + 2) * (P + 1) * 16), "alloc S*v scratch");
+    }
     const size_t obytes = (size_t)S * c->Pld * o_elem(c->prec, cplx);
     size_t freeb = 0, totalb = 0;
     HIPCHK(c, hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
@@ -542,9 +567,31 @@ static int sv_mixed_device(nqes_ctx *c, const void *v, const void *ov, size_t ds
     HIPCHK(c, hipMemcpyAsync(c->dv.p, v, dsize * P, hipMemcpyHostToDevice, c->stream), "copy v");
     HIPCHK(c, hipMemcpyAsync(c->dov.p, ov, dsize, hipMemcpyHostToDevice, c->stream), "copy ov");
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
-    HIPCHK(c, nq_launch_sv_mx(c->cplx, c->dO.p, c->dv.p, c->dov.p, c->S, P, c->Pld, c->svchunks, c->dsvpart.p,
-                              c->dsvwpart.p, c->dw.p, c->dsvscratch.p, c->dsvout.p, c->stream),
-           "S*v kernels");
+    if (c->svf) {
+        SvfArgs f;
+        f.N = c->N;
+        f.M = c->M;
+        f.K = c->K;
+        f.NW = c->NW;
+        f.Ucols = c->svf_shape.ucols;
+        f.n4 = (c->N + 3) & ~3;
+        f.S = c->S;
+        f.P = P;
+        f.L = c->L;
+        f.spins = static_cast<const uint32_t *>(c->dspins.p);
+        f.U = c->dU.p;
+        f.v = c->dv.p;
+        f.ov = c->dov.p;
+        f.wrow = c->dwrow.p;
+        f.w = c->dw.p;
+        f.part = c->dsvfpart.p;
+        HIPCHK(c, nq_launch_svf(c->cplx, f, c->svf_shape, c->dsvfscratch.p, c->dsvout.p, c->stream),
+               "factored S*v kernels");
+    } else {
+        HIPCHK(c, nq_launch_sv_mx(c->cplx, c->dO.p, c->dv.p, c->dov.p, c->S, P, c->Pld, c->svchunks, c->dsvpart.p,
+                                  c->dsvwpart.p, c->dw.p, c->dsvscratch.p, c->dsvout.p, c->stream),
+               "S*v kernels");
+    }
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
     HIPCHK(c, hipMemcpyAsync(c->svraw.data(), c->dsvout.p, dsize * (P + 1), hipMemcpyDeviceToHost, c->stream),
            "copy S*v");

@@ -1,0 +1,328 @@
+// sv_factored.hip -- S*v of the mixed path from per-sample factors instead of O.
+//
+// With Cramer's rule, the O row of sample s is, for RBM k (rbm.cc:199-239),
+//   a block  O_{k,i}   = sum_kk c_{s,k,kk} sigma^kk_i
+//   b block  O_{k,j}   = sum_kk c_{s,k,kk} t_{s,kk,k,j}
+//   w block  O_{k,i,j} = sum_kk c_{s,k,kk} sigma^kk_i t_{s,kk,k,j}
+// where sigma^kk = +-1 are replica kk's spins and t_{s,kk,k,j} = tanh theta_j of RBM k
+// at those spins.  Rows r = s K + kk of the factor matrix U hold
+//   U[r][k M + j] = c_{s,k,kk} t_{s,kk,k,j},  U[r][K M + k] = c_{s,k,kk},
+// zero-padded to Ucols = 16 ceil((K M + K) / 16) columns.  With B[i][k M + j] =
+// v_w[k][i][j], B[i][K M + k] = v_a[k][i] and bias[k M + j] = v_b[k][j]:
+//   O_s^T v = sum_kk sum_col U[r][col] (sigma_r^T B[:, col] + bias[col])
+//   sum_s conj(O_s) w_s:  a and w blocks  G[i][col] = sum_r sigma_r[i] conj(U[r][col]) w_s(r)
+//                         b block         sum_r conj(U[r][col]) w_s(r)
+// which is the reference's S*v (rbm.cc:661-684, matrix-free: the per-sample state is
+// stored instead of O) as two GEMMs with the +-1 spin matrix as one operand.  Both run
+// on FP64 MFMA (v_mfma_f64_16x16x4f64): FP64 arithmetic on FP32 factors, so the
+// operator is linear to FP64 rounding, as the stored-O pass is.
+//
+// Pass 1 k_svf_fwd     one wavefront per 16 rows: Y = Sigma B on MFMA, then
+//                      w_r = sum_col U (Y + bias) -> wrow[r]
+// Pass 2 k_svf_combine w_s = sum_kk wrow[s K + kk] - o^T v
+// Pass 3 k_svf_bwd     blocks over row chunks, wavefronts over (column tile, i tile)
+//                      pairs: G on MFMA, b-block column sums, sum_s w_s; one partial
+//                      raw vector (P + 1) per block, summed in block order after.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "cplx64.hpp"
+#include "kernels.hpp"
+#include "mixed_math.hpp"
+
+using namespace nqm;
+using nq64::cd;
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ inline double reD(double x) { return x; }
+__device__ inline double reD(cd z) { return z.re; }
+__device__ inline double imD(double) { return 0.0; }
+__device__ inline double imD(cd z) { return z.im; }
+template <typename D> __device__ inline D mkD(double re, double im);
+template <> __device__ inline double mkD<double>(double re, double) { return re; }
+template <> __device__ inline cd mkD<cd>(double re, double im) { return nq64::mk(re, im); }
+
+// +-1 of spin i (bit set: sigma = -1) from the row's two spin words (N <= 64)
+__device__ inline double spin_pm(uint32_t w0, uint32_t w1, int i)
+{
+    const uint32_t w = (i < 32) ? w0 : w1;
+    return ((w >> (i & 31)) & 1u) ? -1.0 : 1.0;
+}
+
+// B (N4 x Ucols) and bias (Ucols) from v, in LDS
+template <typename D>
+__device__ inline void stage_B(const SvfArgs &a, D *Bl, D *bias)
+{
+    const D *v = (const D *)a.v;
+    const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
+    const int N4 = a.n4;
+    const long long KM = (long long)K * M;
+    for (long long q = threadIdx.x; q < (long long)N4 * UC; q += blockDim.x) {
+        const int i = (int)(q / UC), col = (int)(q % UC);
+        D b = zeroD<D>();
+        if (i < N) {
+            if (col < KM) {
+                const int k = col / M, j = col % M;
+                b = v[(long long)k * a.L + N + M + (long long)i * M + j];
+            } else if (col < KM + K) {
+                b = v[(long long)(col - KM) * a.L + i];
+            }
+        }
+        Bl[q] = b;
+    }
+    for (int col = threadIdx.x; col < UC; col += blockDim.x) {
+        D b = zeroD<D>();
+        if (col < KM) b = v[(long long)(col / M) * a.L + N + col % M];
+        bias[col] = b;
+    }
+}
+
+// v_mfma_f64_16x16x4f64 operand layout: lane l holds A[l%16][l/16] and B[l/16][l%16];
+// result register e of lane l is D[4 e + l/16][l%16] (rows interleaved by lane group,
+// unlike the FP32 16x16x4 form whose register e holds row 4 (l/16) + e).
+//
+// Pass 1.  A block of 4 wavefronts loops over 16-row tiles.  Lane l: A operand row
+// r0 + l%16 (spins i = 4 q + l/16), result rows r0 + 4 e + l/16 (e < 4) at column l%16
+// of each 16-column tile.  NS = N4/4 steps of the K dimension (N <= 64).
+template <bool CPLX, int NSMAX>
+__global__ __launch_bounds__(256) void k_svf_fwd(SvfArgs a)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    extern __shared__ __align__(16) unsigned char smem_f[];
+    const int N = a.N, K = a.K, NW = a.NW, UC = a.Ucols;
+    const int N4 = a.n4;
+    const int NS = N4 / 4;
+    D *Bl = (D *)smem_f;
+    D *bias = Bl + (long long)N4 * UC;
+    stage_B<D>(a, Bl, bias);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int hi = lane >> 4, lo = lane & 15;
+    const long long R = a.S * K;
+    const long long ntile = (R + 15) / 16;
+    const T *U = (const T *)a.U;
+    D *wrow = (D *)a.wrow;
+    for (long long t = (long long)blockIdx.x * 4 + wv; t < ntile; t += (long long)gridDim.x * 4) {
+        const long long r0 = t * 16;
+        const long long ra = r0 + lo;
+        uint32_t w0 = 0, w1 = 0;
+        if (ra < R) {
+            w0 = a.spins[ra * NW];
+            if (NW > 1) w1 = a.spins[ra * NW + 1];
+        }
+        double av[NSMAX];
+#pragma unroll
+        for (int q = 0; q < NSMAX; q++) {
+            const int i = 4 * q + hi;
+            av[q] = (q < NS && ra < R && i < N) ? spin_pm(w0, w1, i) : 0.0;
+        }
+        D rs[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) rs[q] = zeroD<D>();
+        for (int ct = 0; ct < UC / 16; ct++) {
+            const int col = ct * 16 + lo;
+            v4d accR = {0.0, 0.0, 0.0, 0.0};
+            v4d accI = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < NSMAX; q++) {
+                if (q < NS) {
+                    const D bv = Bl[(long long)(4 * q + hi) * UC + col];
+                    accR = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], reD(bv), accR, 0, 0, 0);
+                    if constexpr (CPLX) accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], imD(bv), accI, 0, 0, 0);
+                }
+            }
+            const D bc = bias[col];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const long long r = r0 + 4 * q + hi;
+                if (r < R) {
+                    const D z = addD(mkD<D>(accR[q], accI[q]), bc);
+                    rs[q] = addD(rs[q], mulD(toD(U[r * UC + col]), z));
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) rs[q] = addD(rs[q], shflxD(rs[q], off));
+        if (lo == 0)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const long long r = r0 + 4 * q + hi;
+                if (r < R) wrow[r] = rs[q];
+            }
+    }
+}
+
+// Pass 2.  w_s = sum_kk wrow[s K + kk] - o^T v
+template <typename D>
+__global__ void k_svf_combine(SvfArgs a)
+{
+    const D *wrow = (const D *)a.wrow;
+    D *w = (D *)a.w;
+    const D ov = *(const D *)a.ov;
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < a.S;
+         s += (long long)gridDim.x * blockDim.x) {
+        D acc = wrow[s * a.K];
+        for (int kk = 1; kk < a.K; kk++) acc = addD(acc, wrow[s * a.K + kk]);
+        w[s] = subD(acc, ov);
+    }
+}
+
+// Pass 3.  Block b takes the 4-row steps [b n4 / nblk, (b+1) n4 / nblk); wavefront wv
+// takes the pairs p = wv + NWV q (q < PPW) of (column tile ct = p % NCT, i tile
+// it = p / NCT).  Lane l: A operand sigma_{r0 + l/16}[16 it + l%16], B operand
+// conj(U[r0 + l/16][16 ct + l%16]) w_s; result rows i = 16 it + 4 e + l/16 (e < 4) at
+// column 16 ct + l%16.  Pairs with it = 0 also sum the B operand over rows (b block)
+// and wavefront 0 sums w_s over rows with kk = 0.  Output: part[b][0..P].
+template <bool CPLX, int PPW>
+__global__ __launch_bounds__(PPW <= 2 ? 1024 : 512) void k_svf_bwd(SvfArgs a)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    const int N = a.N, M = a.M, K = a.K, NW = a.NW, UC = a.Ucols;
+    const int NCT = UC / 16, NIT = (N + 15) / 16, NP = NCT * NIT;
+    const int NWV = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int hi = lane >> 4, lo = lane & 15;
+    const long long R = a.S * K;
+    const long long KM = (long long)K * M;
+    const long long n4 = (R + 3) / 4;
+    const long long q0 = n4 * blockIdx.x / gridDim.x, q1 = n4 * (blockIdx.x + 1) / gridDim.x;
+    const T *U = (const T *)a.U;
+    const D *w = (const D *)a.w;
+
+    v4d accR[PPW], accI[PPW];
+    D cs[PPW];
+#pragma unroll
+    for (int q = 0; q < PPW; q++) {
+        accR[q] = v4d{0.0, 0.0, 0.0, 0.0};
+        accI[q] = v4d{0.0, 0.0, 0.0, 0.0};
+        cs[q] = zeroD<D>();
+    }
+    D ws = zeroD<D>();
+    for (long long step = q0; step < q1; step++) {
+        const long long r = step * 4 + hi;  // this lane's row (K dimension)
+        const bool ok = r < R;
+        uint32_t w0 = 0, w1 = 0;
+        D wr = zeroD<D>();
+        if (ok) {
+            w0 = a.spins[r * NW];
+            if (NW > 1) w1 = a.spins[r * NW + 1];
+            wr = w[r / K];
+            if (wv == 0 && lo == 0 && (r % K) == 0) ws = addD(ws, wr);
+        }
+#pragma unroll
+        for (int q = 0; q < PPW; q++) {
+            const int p = wv + NWV * q;
+            if (p < NP) {
+                const int ct = p % NCT, it = p / NCT;
+                const int i = 16 * it + lo;
+                const double av = (ok && i < N) ? spin_pm(w0, w1, i) : 0.0;
+                const D bv = ok ? mulD(conjD(toD(U[r * UC + 16 * ct + lo])), wr) : zeroD<D>();
+                accR[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, reD(bv), accR[q], 0, 0, 0);
+                if constexpr (CPLX) accI[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, imD(bv), accI[q], 0, 0, 0);
+                if (it == 0) cs[q] = addD(cs[q], bv);
+            }
+        }
+    }
+    D *part = (D *)a.part + (long long)blockIdx.x * (a.P + 1);
+#pragma unroll
+    for (int q = 0; q < PPW; q++) {
+        const int p = wv + NWV * q;
+        if (p < NP) {
+            const int ct = p % NCT, it = p / NCT;
+            const int col = 16 * ct + lo;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int i = 16 * it + 4 * e + hi;
+                if (i < N) {
+                    const D g = mkD<D>(accR[q][e], accI[q][e]);
+                    if (col < KM) part[(col / M) * a.L + N + M + (long long)i * M + col % M] = g;
+                    else if (col < KM + K) part[(col - KM) * a.L + i] = g;
+                }
+            }
+            if (it == 0) {
+                D c = cs[q];
+                c = addD(c, shflxD(c, 16));
+                c = addD(c, shflxD(c, 32));
+                if (hi == 0 && col < KM) part[(col / M) * a.L + N + col % M] = c;
+            }
+        }
+    }
+    if (wv == 0) {
+        D t = ws;
+        t = addD(t, shflxD(t, 16));
+        t = addD(t, shflxD(t, 32));
+        if (lane == 0) part[a.P] = t;
+    }
+}
+
+template <bool CPLX>
+hipError_t launch_svf(const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st)
+{
+    using D = typename Ty<CPLX>::D;
+    const size_t lds = ((size_t)a.n4 * a.Ucols + a.Ucols) * sizeof(D);
+    if (lds > 65536) {
+        hipError_t e = (a.N <= 32)
+                           ? hipFuncSetAttribute((const void *)k_svf_fwd<CPLX, 8>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
+                           : hipFuncSetAttribute((const void *)k_svf_fwd<CPLX, 16>,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (a.N <= 32)
+        hipLaunchKernelGGL((k_svf_fwd<CPLX, 8>), dim3(sh.fwd_blocks), dim3(256), lds, st, a);
+    else
+        hipLaunchKernelGGL((k_svf_fwd<CPLX, 16>), dim3(sh.fwd_blocks), dim3(256), lds, st, a);
+    hipLaunchKernelGGL(k_svf_combine<D>, dim3(1024), dim3(256), 0, st, a);
+    const dim3 bb(64 * sh.bwd_waves);
+    switch (sh.ppw) {
+    case 1: hipLaunchKernelGGL((k_svf_bwd<CPLX, 1>), dim3(sh.bwd_blocks), bb, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_svf_bwd<CPLX, 2>), dim3(sh.bwd_blocks), bb, 0, st, a); break;
+    case 8: hipLaunchKernelGGL((k_svf_bwd<CPLX, 8>), dim3(sh.bwd_blocks), bb, 0, st, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    nq_chunk_reduce(CPLX, a.part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, out, st);
+    return hipGetLastError();
+}
+
+} // namespace
+
+// Factored S*v applies when N <= 64, B fits in LDS and the (column tile, i tile)
+// pairs fit one block: 16 wavefronts of 1 or 2 pairs, or 8 of 8.  ppw = 0: not
+// applicable.
+SvfShape nq_svf_shape(bool cplx, int N, int M, int K)
+{
+    SvfShape sh{0, 0, 0, 0, 0};
+    if (N > 64) return sh;
+    const int UC = 16 * ((K * M + K + 15) / 16);
+    const int npairs = (UC / 16) * ((N + 15) / 16);
+    const int N4 = (N + 3) & ~3;
+    const size_t lds = ((size_t)N4 * UC + UC) * (cplx ? 16 : 8);
+    if (lds > 160 * 1024) return sh;
+    int ppw = npairs <= 16 ? 1 : npairs <= 32 ? 2 : npairs <= 64 ? 8 : 0;
+    if (!ppw) return sh;
+    sh.ucols = UC;
+    sh.ppw = ppw;
+    sh.bwd_waves = (npairs + ppw - 1) / ppw;
+    sh.bwd_blocks = 512;
+    sh.fwd_blocks = 1024;
+    const char *e = getenv("NQES_SVF_BWD_BLOCKS");
+    if (e && *e && atoi(e) > 0) sh.bwd_blocks = atoi(e);
+    e = getenv("NQES_SVF_FWD_BLOCKS");
+    if (e && *e && atoi(e) > 0) sh.fwd_blocks = atoi(e);
+    return sh;
+}
+
+hipError_t nq_launch_svf(bool cplx, const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st)
+{
+    return cplx ? launch_svf<true>(a, sh, scratch, out, st) : launch_svf<false>(a, sh, scratch, out, st);
+}

@@ -7,7 +7,11 @@ Tolerances:
   * on chains with identical spins, H_loc and O agree with the FP64 path to 1e-4
     relative (FP32 per-hidden-unit arithmetic, FP64 products and sums);
   * S*v on the mixed path's own FP32 O equals the dense (1/S) O^H O - o* o^T +
-    diag(skk) product built in float64 from the same O to 1e-10 (FP64 accumulation);
+    diag(skk) product built in float64 from the same O to 1e-10 (FP64 accumulation)
+    with the streamed pass over O (NQES_SV=stored);
+  * the factored S*v (default: FP64 MFMA passes over the spins and the FP32 factor
+    rows c t) differs from it only by the FP32 rounding of O's sum over replicas,
+    so it agrees to 1e-6 relative (1e-12 for K = 1, where that sum has one term);
   * e_mat and the SR update agree with the FP64 path to 1e-4 relative.
 """
 import numpy as np
@@ -66,8 +70,13 @@ def test_mixed_matches_fp64(nqes, N, M, K, C, m, B, cplx):
         assert not np.any(b["O"].imag)
 
 
+@pytest.fixture()
+def sv_stored(monkeypatch):
+    monkeypatch.setenv("NQES_SV", "stored")
+
+
 @pytest.mark.parametrize("N,M,K,C,m,B,cplx", [CASES[0], CASES[1], CASES[3], CASES[6], CASES[7]])
-def test_mixed_sv_exact_on_own_O(nqes, N, M, K, C, m, B, cplx):
+def test_mixed_sv_exact_on_own_O(nqes, sv_stored, N, M, K, C, m, B, cplx):
     print(f"case N={N} M={M} K={K} complex={cplx}", flush=True)
     J = load_J(N)
     W = random_weights(N, M, K, seed=3 + N, scale=0.1, complex_=cplx)
@@ -85,6 +94,32 @@ def test_mixed_sv_exact_on_own_O(nqes, N, M, K, C, m, B, cplx):
         for v in (rng.normal(size=s.P) + 0j, rng.normal(size=s.P) + 1j * rng.normal(size=s.P)):
             Sv = O.conj().T @ (O @ v) / S - o.conj() * (o @ v) + mom["skk"] * v
             assert rel_err(s.apply_S(v), Sv) < 1e-10
+
+
+@pytest.mark.parametrize("N,M,K,C,m,B,cplx", [
+    (8, 16, 2, 64, 8, 4, False), (8, 16, 2, 64, 8, 4, True), (6, 12, 1, 32, 8, 4, True),
+    (6, 12, 3, 32, 6, 4, False), (8, 16, 4, 16, 6, 4, True), (32, 64, 2, 64, 4, 2, False),
+    (40, 80, 2, 16, 3, 2, True), (64, 64, 3, 8, 3, 2, False), (17, 34, 4, 9, 5, 2, True)])
+def test_factored_sv_matches_stored(nqes, monkeypatch, N, M, K, C, m, B, cplx):
+    """Samples whose Phi is singular (two replicas on the same spins) give unbounded
+    Cramer weights c, so O is ill-defined there in both forms (the reference's
+    pseudo-inverse would give O = det(Phi^+ derPhi) = 0).  With 2^N spin states per
+    replica that needs tiny N; K = 4 cases use N >= 8."""
+    print(f"case N={N} M={M} K={K} C={C} m={m} complex={cplx}", flush=True)
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=11 + N, scale=0.1, complex_=cplx)
+    rng = np.random.default_rng(5)
+    vs = [rng.normal(size=K * (N + M + N * M)) + 0j,
+          rng.normal(size=K * (N + M + N * M)) + 1j * rng.normal(size=K * (N + M + N * M))]
+    out = {}
+    for mode in ("factored", "stored"):
+        monkeypatch.setenv("NQES_SV", mode)
+        with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+            s.sample(W, 4321, C, m, B, 0.7)
+            out[mode] = [s.apply_S(v) for v in vs]
+    tol = 1e-12 if K == 1 else 1e-6
+    for a, b in zip(out["factored"], out["stored"]):
+        assert rel_err(a, b) < tol, rel_err(a, b)
 
 
 @pytest.mark.parametrize("cplx", [False, True])
