@@ -20,9 +20,9 @@
 // Pass 1 k_svf_fwd     one wavefront per 16 rows: Y = Sigma B on MFMA, then
 //                      w_r = sum_col U (Y + bias) -> wrow[r]
 // Pass 2 k_svf_combine w_s = sum_kk wrow[s K + kk] - o^T v
-// Pass 3 k_svf_bwd     blocks over row chunks, wavefronts over (column tile, i tile)
-//                      pairs: G on MFMA, b-block column sums, sum_s w_s; one partial
-//                      raw vector (P + 1) per block, summed in block order after.
+// Pass 3 k_svf_bwd     blocks over row chunks, wavefronts over column tiles (each with
+//                      all its i tiles): G on MFMA, b-block column sums, sum_s w_s; one
+//                      partial raw vector (P + 1) per block, summed in block order after.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -175,19 +175,19 @@ __global__ void k_svf_combine(SvfArgs a)
 }
 
 // Pass 3.  Block b takes the 4-row steps [b n4 / nblk, (b+1) n4 / nblk); wavefront wv
-// takes the pairs p = wv + NWV q (q < PPW) of (column tile ct = p % NCT, i tile
-// it = p / NCT).  Lane l: A operand sigma_{r0 + l/16}[16 it + l%16], B operand
-// conj(U[r0 + l/16][16 ct + l%16]) w_s; result rows i = 16 it + 4 e + l/16 (e < 4) at
-// column 16 ct + l%16.  Pairs with it = 0 also sum the B operand over rows (b block)
-// and wavefront 0 sums w_s over rows with kk = 0.  Output: part[b][0..P].
-template <bool CPLX, int PPW>
-__global__ __launch_bounds__(PPW <= 2 ? 1024 : 512) void k_svf_bwd(SvfArgs a)
+// owns the column tiles ct = wv CTW + c (c < CTW) with all their i tiles
+// it < NIT = ceil(N / 16), so each U value is loaded once and feeds NIT MFMAs.  Lane
+// l: A operand sigma_{r0 + l/16}[16 it + l%16], B operand conj(U[r0 + l/16][16 ct +
+// l%16]) w_s; result rows i = 16 it + 4 e + l/16 (e < 4) at column 16 ct + l%16.  The
+// B operand is also summed over rows (b block), and wavefront 0 sums w_s over rows
+// with kk = 0.  Output: part[b][0..P].
+template <bool CPLX, int CTW, int NITM>
+__global__ __launch_bounds__(1024) void k_svf_bwd(SvfArgs a)
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
     const int N = a.N, M = a.M, K = a.K, NW = a.NW, UC = a.Ucols;
-    const int NCT = UC / 16, NIT = (N + 15) / 16, NP = NCT * NIT;
-    const int NWV = blockDim.x >> 6;
+    const int NCT = UC / 16, NIT = (N + 15) / 16;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int hi = lane >> 4, lo = lane & 15;
     const long long R = a.S * K;
@@ -197,13 +197,16 @@ __global__ __launch_bounds__(PPW <= 2 ? 1024 : 512) void k_svf_bwd(SvfArgs a)
     const T *U = (const T *)a.U;
     const D *w = (const D *)a.w;
 
-    v4d accR[PPW], accI[PPW];
-    D cs[PPW];
+    v4d accR[CTW][NITM], accI[CTW][NITM];
+    D cs[CTW];
 #pragma unroll
-    for (int q = 0; q < PPW; q++) {
-        accR[q] = v4d{0.0, 0.0, 0.0, 0.0};
-        accI[q] = v4d{0.0, 0.0, 0.0, 0.0};
-        cs[q] = zeroD<D>();
+    for (int c = 0; c < CTW; c++) {
+#pragma unroll
+        for (int it = 0; it < NITM; it++) {
+            accR[c][it] = v4d{0.0, 0.0, 0.0, 0.0};
+            accI[c][it] = v4d{0.0, 0.0, 0.0, 0.0};
+        }
+        cs[c] = zeroD<D>();
     }
     D ws = zeroD<D>();
     for (long long step = q0; step < q1; step++) {
@@ -211,48 +214,66 @@ __global__ __launch_bounds__(PPW <= 2 ? 1024 : 512) void k_svf_bwd(SvfArgs a)
         const bool ok = r < R;
         uint32_t w0 = 0, w1 = 0;
         D wr = zeroD<D>();
+        T uv[CTW];
+#pragma unroll
+        for (int c = 0; c < CTW; c++) uv[c] = zeroT<T>();
         if (ok) {
             w0 = a.spins[r * NW];
             if (NW > 1) w1 = a.spins[r * NW + 1];
             wr = w[r / K];
+#pragma unroll
+            for (int c = 0; c < CTW; c++) {
+                const int ct = wv * CTW + c;
+                if (ct < NCT) uv[c] = U[r * UC + 16 * ct + lo];
+            }
             if (wv == 0 && lo == 0 && (r % K) == 0) ws = addD(ws, wr);
         }
+        double av[NITM];
 #pragma unroll
-        for (int q = 0; q < PPW; q++) {
-            const int p = wv + NWV * q;
-            if (p < NP) {
-                const int ct = p % NCT, it = p / NCT;
-                const int i = 16 * it + lo;
-                const double av = (ok && i < N) ? spin_pm(w0, w1, i) : 0.0;
-                const D bv = ok ? mulD(conjD(toD(U[r * UC + 16 * ct + lo])), wr) : zeroD<D>();
-                accR[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, reD(bv), accR[q], 0, 0, 0);
-                if constexpr (CPLX) accI[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, imD(bv), accI[q], 0, 0, 0);
-                if (it == 0) cs[q] = addD(cs[q], bv);
+        for (int it = 0; it < NITM; it++) {
+            const int i = 16 * it + lo;
+            av[it] = (ok && it < NIT && i < N) ? spin_pm(w0, w1, i) : 0.0;
+        }
+#pragma unroll
+        for (int c = 0; c < CTW; c++) {
+            if (wv * CTW + c < NCT) {
+                const D bv = mulD(conjD(toD(uv[c])), wr);
+                cs[c] = addD(cs[c], bv);
+#pragma unroll
+                for (int it = 0; it < NITM; it++) {
+                    if (it < NIT) {
+                        accR[c][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[it], reD(bv), accR[c][it], 0, 0, 0);
+                        if constexpr (CPLX)
+                            accI[c][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[it], imD(bv), accI[c][it], 0, 0, 0);
+                    }
+                }
             }
         }
     }
     D *part = (D *)a.part + (long long)blockIdx.x * (a.P + 1);
 #pragma unroll
-    for (int q = 0; q < PPW; q++) {
-        const int p = wv + NWV * q;
-        if (p < NP) {
-            const int ct = p % NCT, it = p / NCT;
+    for (int c = 0; c < CTW; c++) {
+        const int ct = wv * CTW + c;
+        if (ct < NCT) {
             const int col = 16 * ct + lo;
 #pragma unroll
-            for (int e = 0; e < 4; e++) {
-                const int i = 16 * it + 4 * e + hi;
-                if (i < N) {
-                    const D g = mkD<D>(accR[q][e], accI[q][e]);
-                    if (col < KM) part[(col / M) * a.L + N + M + (long long)i * M + col % M] = g;
-                    else if (col < KM + K) part[(col - KM) * a.L + i] = g;
+            for (int it = 0; it < NITM; it++) {
+                if (it < NIT) {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int i = 16 * it + 4 * e + hi;
+                        if (i < N) {
+                            const D g = mkD<D>(accR[c][it][e], accI[c][it][e]);
+                            if (col < KM) part[(col / M) * a.L + N + M + (long long)i * M + col % M] = g;
+                            else if (col < KM + K) part[(col - KM) * a.L + i] = g;
+                        }
+                    }
                 }
             }
-            if (it == 0) {
-                D c = cs[q];
-                c = addD(c, shflxD(c, 16));
-                c = addD(c, shflxD(c, 32));
-                if (hi == 0 && col < KM) part[(col / M) * a.L + N + col % M] = c;
-            }
+            D t = cs[c];
+            t = addD(t, shflxD(t, 16));
+            t = addD(t, shflxD(t, 32));
+            if (hi == 0 && col < KM) part[(col / M) * a.L + N + col % M] = t;
         }
     }
     if (wv == 0) {
@@ -282,12 +303,13 @@ hipError_t launch_svf(const SvfArgs &a, const SvfShape &sh, void *scratch, void 
         hipLaunchKernelGGL((k_svf_fwd<CPLX, 16>), dim3(sh.fwd_blocks), dim3(256), lds, st, a);
     hipLaunchKernelGGL(k_svf_combine<D>, dim3(1024), dim3(256), 0, st, a);
     const dim3 bb(64 * sh.bwd_waves);
-    switch (sh.ppw) {
-    case 1: hipLaunchKernelGGL((k_svf_bwd<CPLX, 1>), dim3(sh.bwd_blocks), bb, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_svf_bwd<CPLX, 2>), dim3(sh.bwd_blocks), bb, 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_svf_bwd<CPLX, 8>), dim3(sh.bwd_blocks), bb, 0, st, a); break;
-    default: return hipErrorInvalidValue;
-    }
+    const bool n32 = a.N <= 32;
+    if (sh.ppw == 1 && n32) hipLaunchKernelGGL((k_svf_bwd<CPLX, 1, 2>), dim3(sh.bwd_blocks), bb, 0, st, a);
+    else if (sh.ppw == 1) hipLaunchKernelGGL((k_svf_bwd<CPLX, 1, 4>), dim3(sh.bwd_blocks), bb, 0, st, a);
+    else if constexpr (!CPLX) {  // complex shapes keep one tile per wavefront (registers)
+        if (n32) hipLaunchKernelGGL((k_svf_bwd<CPLX, 2, 2>), dim3(sh.bwd_blocks), bb, 0, st, a);
+        else hipLaunchKernelGGL((k_svf_bwd<CPLX, 2, 4>), dim3(sh.bwd_blocks), bb, 0, st, a);
+    } else return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     nq_chunk_reduce(CPLX, a.part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, out, st);
@@ -296,24 +318,25 @@ hipError_t launch_svf(const SvfArgs &a, const SvfShape &sh, void *scratch, void 
 
 } // namespace
 
-// Factored S*v applies when N <= 64, B fits in LDS and the (column tile, i tile)
-// pairs fit one block: 16 wavefronts of 1 or 2 pairs, or 8 of 8.  ppw = 0: not
-// applicable.
+// Factored S*v applies when N <= 64, B fits in LDS and the column tiles fit one
+// block of at most 16 wavefronts with 1 or (real weights) 2 tiles each (ppw = tiles
+// per wavefront).
+// ppw = 0: not applicable.
 SvfShape nq_svf_shape(bool cplx, int N, int M, int K)
 {
     SvfShape sh{0, 0, 0, 0, 0};
     if (N > 64) return sh;
     const int UC = 16 * ((K * M + K + 15) / 16);
-    const int npairs = (UC / 16) * ((N + 15) / 16);
+    const int nct = UC / 16;
     const int N4 = (N + 3) & ~3;
     const size_t lds = ((size_t)N4 * UC + UC) * (cplx ? 16 : 8);
     if (lds > 160 * 1024) return sh;
-    int ppw = npairs <= 16 ? 1 : npairs <= 32 ? 2 : npairs <= 64 ? 8 : 0;
+    const int ppw = nct <= 16 ? 1 : (nct <= 32 && !cplx) ? 2 : 0;
     if (!ppw) return sh;
+    sh.bwd_waves = (nct + ppw - 1) / ppw;
     sh.ucols = UC;
     sh.ppw = ppw;
-    sh.bwd_waves = (npairs + ppw - 1) / ppw;
-    sh.bwd_blocks = 512;
+    sh.bwd_blocks = 2048;
     sh.fwd_blocks = 1024;
     const char *e = getenv("NQES_SVF_BWD_BLOCKS");
     if (e && *e && atoi(e) > 0) sh.bwd_blocks = atoi(e);
