@@ -420,6 +420,41 @@ __host__ __device__ inline uint64_t pcg_next(pcg &s)
     return (x >> rot) | (x << ((64u - rot) & 63u));
 }
 
+// XSL-RR output of a state
+__host__ __device__ inline uint64_t pcg_out(const pcg &s)
+{
+    unsigned rot = (unsigned)(s.hi >> 58);
+    uint64_t x = s.hi ^ s.lo;
+    return (x >> rot) | (x << ((64u - rot) & 63u));
+}
+
+// jump-ahead: the affine map s -> a s + c (mod 2^128) applied d times is
+// s -> a_d s + c_d; (a, c) holds (a_d, c_d).  pcg_affine applies it.
+__host__ __device__ inline pcg pcg_affine(const pcg &s, const pcg &a, const pcg &c)
+{
+    uint64_t lo = s.lo * a.lo;
+    uint64_t hi = umulhi64(s.lo, a.lo) + s.lo * a.hi + s.hi * a.lo;
+    uint64_t nlo = lo + c.lo;
+    hi += c.hi + (nlo < lo ? 1u : 0u);
+    pcg r;
+    r.lo = nlo;
+    r.hi = hi;
+    return r;
+}
+
+// (a_d, c_d) of d >= 1 steps of the pcg64 state transition
+__host__ __device__ inline void pcg_jump(unsigned d, pcg &a, pcg &c)
+{
+    const pcg A{NQ_PCG_MULT_LO, NQ_PCG_MULT_HI}, C{NQ_PCG_INC_LO, NQ_PCG_INC_HI};
+    a = A;
+    c = C;
+    for (unsigned k = 1; k < d; k++) {  // f o f^k: (A a_k, A c_k + C)
+        const pcg zero{0, 0};
+        a = pcg_affine(a, A, zero);
+        c = pcg_affine(c, A, C);
+    }
+}
+
 // generate_canonical<double, 53> over a 64-bit engine
 __host__ __device__ inline double canonical(uint64_t u)
 {

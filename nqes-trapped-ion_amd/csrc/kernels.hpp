@@ -78,12 +78,19 @@ struct SvfArgs {
     void *wrow;                // S*K D
     void *w;                   // S D
     void *part;                // bwd_blocks x (P + 1) D
+    const void *E;             // S complex double: local energies (moments)
+    int wsel;                  // backward-pass weights: 0 w, 1 ones, 2 E
 };
 struct SvfShape {
     int ucols, ppw, bwd_waves, bwd_blocks, fwd_blocks;
 };
 SvfShape nq_svf_shape(bool cplx, int N, int M, int K);
 hipError_t nq_launch_svf(bool cplx, const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st);
+// moments from the factors: so = sum_s O_s, ss = sum_s |O_s|^2, sf = sum_s E_s conj(O_s)
+// (P complex double each); part: >= max(1, K(K-1)/2) * bwd_blocks * (P + 1) D,
+// scratch: >= that / 32 + 2 (P + 1) D, tmp: 3 (P + 1) D
+hipError_t nq_launch_svf_moments(bool cplx, const SvfArgs &a, const SvfShape &sh, void *part, void *scratch, void *tmp,
+                                 void *so, void *ss, void *sf, hipStream_t st);
 // out[col] = sum over chunks of part[chunk*stride + col] in chunk order (D = double or
 // complex double); scratch >= (nchunk/32 + 1) * ncol D
 hipError_t nq_chunk_reduce(bool cplx, const void *part, long long ncol, long long stride, int nchunk,

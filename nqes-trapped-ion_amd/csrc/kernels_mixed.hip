@@ -140,6 +140,58 @@ __global__ __launch_bounds__(64) void k_theta_gemm(int N, int M, int K, int NW, 
 }
 
 // ---------------------------------------------------------------- sampler
+// Per-step tables read by the sampler and the measurement: cosh(2w), sinh(2w), w
+// ([kk][i][j]), b ([kk][j]) in T and exp(2a), exp(-2a), a ([kk][i]) in D; in LDS when
+// a.tab_lds (mx_tab_bytes), else in global memory.
+template <bool CPLX> struct MxTab {
+    const typename Ty<CPLX>::T *ch, *sh, *w, *b;
+    const typename Ty<CPLX>::D *eaP, *eaN, *aD;
+};
+
+__host__ __device__ inline size_t mx_tab_bytes(bool cplx, int N, int M, int K)
+{
+    const size_t ts = cplx ? 8 : 4, ds = cplx ? 16 : 8;
+    return 3 * (size_t)K * N * ds + (3 * (size_t)K * N * M + (size_t)K * M) * ts;
+}
+
+// point tb at the tables (copied into LDS at p when a.tab_lds); returns the LDS
+// cursor after them.  D arrays first to keep their alignment.
+template <bool CPLX>
+__device__ inline unsigned char *mx_stage_tables(const MixedSampleArgs &a, int K, MxTab<CPLX> &tb, unsigned char *p)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    tb.ch = (const T *)a.chT;
+    tb.sh = (const T *)a.shT;
+    tb.w = (const T *)a.wT;
+    tb.b = (const T *)a.bT;
+    tb.eaP = (const D *)a.eaP;
+    tb.eaN = (const D *)a.eaN;
+    tb.aD = (const D *)a.aD;
+    if (!a.tab_lds) return p;
+    const long long KN = (long long)K * a.N, KNM = KN * a.M, KM = (long long)K * a.M;
+    D *d = (D *)p;
+    for (long long q = threadIdx.x; q < KN; q += blockDim.x) {
+        d[q] = tb.eaP[q];
+        d[KN + q] = tb.eaN[q];
+        d[2 * KN + q] = tb.aD[q];
+    }
+    T *t = (T *)(d + 3 * KN);
+    for (long long q = threadIdx.x; q < KNM; q += blockDim.x) {
+        t[q] = tb.ch[q];
+        t[KNM + q] = tb.sh[q];
+        t[2 * KNM + q] = tb.w[q];
+    }
+    for (long long q = threadIdx.x; q < KM; q += blockDim.x) t[3 * KNM + q] = tb.b[q];
+    tb.eaP = d;
+    tb.eaN = d + KN;
+    tb.aD = d + 2 * KN;
+    tb.ch = t;
+    tb.sh = t + KNM;
+    tb.w = t + 2 * KNM;
+    tb.b = t + 3 * KNM;
+    return (unsigned char *)(t + 3 * KNM + KM);
+}
 template <typename T>
 __device__ inline T ld(const T *p)
 {
@@ -165,16 +217,17 @@ __device__ inline cd expD(cd z) { return nq64::cexp_(z); }
 // the spins sp (LDS).  Phi_{k,kk} = exp(sum_i +-a_i) prod_j cosh(theta_j) with the
 // cosh product taken per lane in FP64, then across the group.
 template <bool CPLX, int K, int G, int JMAX>
-__device__ inline void mx_refresh(const MixedSampleArgs &a, const uint32_t *sp, typename Ty<CPLX>::T *tl, int g,
-                                  long long c, bool from_theta0, typename Ty<CPLX>::D *phi, int &overflow)
+__device__ inline void mx_refresh(const MixedSampleArgs &a, const MxTab<CPLX> &tb, const uint32_t *sp,
+                                  typename Ty<CPLX>::T *tl, int g, long long c, bool from_theta0,
+                                  typename Ty<CPLX>::D *phi, int &overflow)
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
     const int N = a.N, M = a.M, NW = a.NW;
     const int jpl = (M + G - 1) / G;
-    const T *wT = (const T *)a.wT;
-    const T *bT = (const T *)a.bT;
-    const D *aD = (const D *)a.aD;
+    const T *wT = tb.w;
+    const T *bT = tb.b;
+    const D *aD = tb.aD;
     D part[K * K];
 #pragma unroll
     for (int q = 0; q < K * K; q++) part[q] = oneD<D>();
@@ -229,15 +282,16 @@ __device__ inline void mx_refresh(const MixedSampleArgs &a, const uint32_t *sp, 
 // Phi^(i)) (rbm.cc:1042-1064), O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with
 // c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k} (Cramer's rule applied to rbm.cc:199-239).
 template <bool CPLX, int K, int G>
-__device__ inline void mx_measure(const MixedSampleArgs &a, const uint32_t *sp, const typename Ty<CPLX>::T *tl,
-                                  const typename Ty<CPLX>::T *chT, const typename Ty<CPLX>::T *shT, int g,
-                                  const typename Ty<CPLX>::D *phi, long long s, bool live)
+__device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &tb, const uint32_t *sp,
+                                  const typename Ty<CPLX>::T *tl, int g, const typename Ty<CPLX>::D *phi,
+                                  long long s, bool live)
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
     const int N = a.N, M = a.M, NW = a.NW, L = a.L;
-    const D *eaP = (const D *)a.eaP;
-    const D *eaN = (const D *)a.eaN;
+    const T *chT = tb.ch, *shT = tb.sh;
+    const D *eaP = tb.eaP;
+    const D *eaN = tb.eaN;
     D phi_inv[K * K];
     inv_m<K, D>(phi, phi_inv);
 
@@ -388,27 +442,6 @@ __device__ inline void mx_measure(const MixedSampleArgs &a, const uint32_t *sp, 
     }
 }
 
-// stage the cosh/sinh tables in LDS when a.tab_lds; returns the LDS cursor after them
-template <typename T>
-__device__ inline unsigned char *mx_stage_tables(const MixedSampleArgs &a, long long KNM, const T *&chT,
-                                                 const T *&shT, unsigned char *p)
-{
-    chT = (const T *)a.chT;
-    shT = (const T *)a.shT;
-    if (a.tab_lds) {
-        T *lch = (T *)p;
-        T *lsh = lch + KNM;
-        for (long long q = threadIdx.x; q < KNM; q += blockDim.x) {
-            lch[q] = chT[q];
-            lsh[q] = shT[q];
-        }
-        chT = lch;
-        shT = lsh;
-        p += 2 * KNM * sizeof(T);
-    }
-    return p;
-}
-
 // Metropolis chains: G lanes per chain, 64/G chains per wavefront.  Sweeps
 // (rbm.cc:530-552) with tanh theta in LDS updated per accepted flip.  After every
 // sweep (burn-in included) theta, tanh and Phi are recomputed from the spins, so the
@@ -433,15 +466,15 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
     const bool live = c_raw < a.C;
     const int c = live ? c_raw : a.C - 1;
     const int jpl = (M + G - 1) / G;
-    const long long KNM = (long long)K * N * M;
 
-    // LDS: [ch tables][sh tables] (if a.tab_lds) [tanh per chain][spins per chain]
-    const T *chT, *shT;
-    unsigned char *p = mx_stage_tables<T>(a, KNM, chT, shT, smem);
+    // LDS: [tables] (if a.tab_lds) [tanh per chain][spins per chain]
+    MxTab<CPLX> tb;
+    unsigned char *p = mx_stage_tables<CPLX>(a, K, tb, smem);
+    const T *chT = tb.ch, *shT = tb.sh;
     T *tl = (T *)p + (long long)cib * K * K * M;
     uint32_t *sp = (uint32_t *)((T *)p + (long long)cpb * K * K * M) + cib * K * NW;
-    const D *eaP = (const D *)a.eaP;
-    const D *eaN = (const D *)a.eaN;
+    const D *eaP = tb.eaP;
+    const D *eaN = tb.eaN;
 
     if (g == 0)
         for (int w = 0; w < K * NW; w++) sp[w] = a.spins0[(long long)c * K * NW + w];
@@ -452,7 +485,14 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
 
     D phi[K * K];
     int overflow = 0;
-    mx_refresh<CPLX, K, G, JMAX>(a, sp, tl, g, c, true, phi, overflow);
+    mx_refresh<CPLX, K, G, JMAX>(a, tb, sp, tl, g, c, true, phi, overflow);
+    // Draws in batches of G: lane g of the group holds draw g of the batch, computed
+    // with one jump of g + 1 steps from the batch base (the same stream as G
+    // sequential steps); proposal t of the batch reads lane t's draw.
+    nq64::pcg jmp_a, jmp_c;
+    nq64::pcg_jump((unsigned)g + 1u, jmp_a, jmp_c);
+    double ub = 0.0;
+    int tdraw = G;
     unsigned long long accepts = 0;
     const int nsweeps = a.B + a.m;
     for (int sw = 0; sw < nsweeps; sw++) {
@@ -497,9 +537,25 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
 #pragma unroll
                     for (int cc = 0; cc < K; cc++) phip[r * K + cc] = (r == k) ? prow[cc] : phi[r * K + cc];
                 D det_p = det_m<K, D>(phip);
-                double prob = isZeroD(det_phi) ? 1.0 : normD(det_p) / normD(det_phi);
-                double u = nq64::canonical(nq64::pcg_next(rng));
-                const bool acc = u <= (prob < 1.0 ? prob : 1.0);
+                if (tdraw == G) {  // next batch of G draws (wave-uniform: chains step in lockstep)
+                    const nq64::pcg sg = nq64::pcg_affine(rng, jmp_a, jmp_c);
+                    ub = nq64::canonical(nq64::pcg_out(sg));
+                    rng.lo = __shfl(sg.lo, G - 1, G);
+                    rng.hi = __shfl(sg.hi, G - 1, G);
+                    tdraw = 0;
+                }
+                // draw tdraw of each group's batch: scalar reads of lane grp G + tdraw
+                double u = 0.0;
+#pragma unroll
+                for (int q = 0; q < CPW; q++) {
+                    const uint64_t ubits = __builtin_bit_cast(uint64_t, ub);
+                    const uint32_t lo32 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)ubits, q * G + tdraw);
+                    const uint32_t hi32 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(ubits >> 32), q * G + tdraw);
+                    if (grp == q) u = __builtin_bit_cast(double, (uint64_t)lo32 | ((uint64_t)hi32 << 32));
+                }
+                tdraw++;
+                // U <= min(1, |det'|^2 / |det|^2) without the division (rbm.cc:542-543)
+                const bool acc = isZeroD(det_phi) || u * normD(det_phi) <= normD(det_p);
                 if (acc) {
                     det_phi = det_p;
 #pragma unroll
@@ -524,7 +580,7 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
             const long long s = (long long)c * a.m + (sw - a.B);
             for (int w = g; w < K * NW; w += G) a.spins[s * K * NW + w] = sp[w];
         }
-        if (sw + 1 < nsweeps) mx_refresh<CPLX, K, G, JMAX>(a, sp, tl, g, c, false, phi, overflow);
+        if (sw + 1 < nsweeps) mx_refresh<CPLX, K, G, JMAX>(a, tb, sp, tl, g, c, false, phi, overflow);
         __syncthreads();
     }
     if (live && g == 0) a.accepts[c] = accepts;
@@ -548,10 +604,9 @@ __global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a)
     const int spb = (blockDim.x >> 6) * SPW;
     const int sib = wv * SPW + grp;
     const long long S = (long long)a.C * a.m;
-    const long long KNM = (long long)K * N * M;
 
-    const T *chT, *shT;
-    unsigned char *p = mx_stage_tables<T>(a, KNM, chT, shT, smem);
+    MxTab<CPLX> tb;
+    unsigned char *p = mx_stage_tables<CPLX>(a, K, tb, smem);
     T *tl = (T *)p + (long long)sib * K * K * M;
     uint32_t *sp = (uint32_t *)((T *)p + (long long)spb * K * K * M) + sib * K * NW;
     int overflow = 0;
@@ -564,8 +619,8 @@ __global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a)
             for (int w = 0; w < K * NW; w++) sp[w] = a.spins[s * K * NW + w];
         __syncthreads();
         D phi[K * K];
-        mx_refresh<CPLX, K, G, JMAX>(a, sp, tl, g, 0, false, phi, overflow);
-        mx_measure<CPLX, K, G>(a, sp, tl, chT, shT, g, phi, s, live);
+        mx_refresh<CPLX, K, G, JMAX>(a, tb, sp, tl, g, 0, false, phi, overflow);
+        mx_measure<CPLX, K, G>(a, tb, sp, tl, g, phi, s, live);
     }
     if (overflow) atomicOr(a.flags, 1u);
 }
@@ -848,7 +903,7 @@ hipError_t launch_sample_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
     using T = typename Ty<CPLX>::T;
     const int cpb = wpb * (64 / G);
     size_t lds = (size_t)cpb * K * K * a.M * sizeof(T) + (size_t)cpb * K * a.NW * 4;
-    if (a.tab_lds) lds += 2 * (size_t)K * a.N * a.M * sizeof(T);
+    if (a.tab_lds) lds += mx_tab_bytes(CPLX, a.N, a.M, K);
     int blocks = (a.C + cpb - 1) / cpb;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)k_sample_mx<CPLX, K, G, 8>,
@@ -865,7 +920,7 @@ hipError_t launch_measure_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
     using T = typename Ty<CPLX>::T;
     const int spb = wpb * (64 / G);
     size_t lds = (size_t)spb * K * K * a.M * sizeof(T) + (size_t)spb * K * a.NW * 4;
-    if (a.tab_lds) lds += 2 * (size_t)K * a.N * a.M * sizeof(T);
+    if (a.tab_lds) lds += mx_tab_bytes(CPLX, a.N, a.M, K);
     const long long S = (long long)a.C * a.m;
     long long blocks = (S + spb - 1) / spb;
     if (lds > 65536) {
@@ -992,7 +1047,7 @@ size_t nq_sample_mx_lds(bool cplx, int N, int M, int K, int G, int wpb, bool tab
     int NW = (N + 31) / 32;
     int cpb = wpb * (64 / G);
     size_t lds = (size_t)cpb * K * K * M * ts + (size_t)cpb * K * NW * 4;
-    if (tab_lds) lds += 2 * (size_t)K * N * M * ts;
+    if (tab_lds) lds += mx_tab_bytes(cplx, N, M, K);
     return lds;
 }
 

@@ -112,19 +112,66 @@ __device__ inline cf toT(cd a) { return mkf((float)a.re, (float)a.im); }
 __device__ inline double shflxD(double v, int m) { return __shfl_xor(v, m); }
 __device__ inline cd shflxD(cd v, int m) { return nq64::mk(__shfl_xor(v.re, m), __shfl_xor(v.im, m)); }
 
-// product / sum over the G lanes of a lane group (G a power of two <= 64)
+// Lane exchange x <- x[lane ^ MASK] without the LDS crossbar: DPP quad permutes
+// (MASK 1, 2), row half-mirror (7) and row mirror (15) within 16-lane rows, and the
+// gfx950 permlane swaps across rows (16) and halves (32).
+template <int MASK>
+__device__ inline uint32_t xor_lane(uint32_t x)
+{
+    if constexpr (MASK == 1) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0xB1, 0xF, 0xF, false);
+    } else if constexpr (MASK == 2) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x4E, 0xF, 0xF, false);
+    } else if constexpr (MASK == 7) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x141, 0xF, 0xF, false);
+    } else if constexpr (MASK == 15) {
+        return (uint32_t)__builtin_amdgcn_update_dpp((int)x, (int)x, 0x140, 0xF, 0xF, false);
+    } else if constexpr (MASK == 16) {
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);  // rows [0,0,2,2] / [1,1,3,3]
+        return (__lane_id() & 16) ? r[0] : r[1];
+    } else {
+        static_assert(MASK == 32, "unsupported lane mask");
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);  // halves [0,0] / [1,1]
+        return (__lane_id() & 32) ? r[0] : r[1];
+    }
+}
+template <int MASK>
+__device__ inline double xor_lane(double v)
+{
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint64_t r = (uint64_t)xor_lane<MASK>((uint32_t)b) | ((uint64_t)xor_lane<MASK>((uint32_t)(b >> 32)) << 32);
+    return __builtin_bit_cast(double, r);
+}
+template <int MASK>
+__device__ inline cd xor_lane(cd v)
+{
+    return nq64::mk(xor_lane<MASK>(v.re), xor_lane<MASK>(v.im));
+}
+
+// Product / sum over the G lanes of a lane group (G a power of two <= 64) by a
+// butterfly over the lane masks 1, 2, 7, 15, 16, 32 (a basis of the group's index
+// space): lanes i and i ^ m combine the same two values at every step, so every lane
+// of the group ends with the bitwise same result.  All lanes must be active.
 template <int G, typename D>
 __device__ inline D group_prod(D v)
 {
-#pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1) v = mulD(v, shflxD(v, off));
+    if constexpr (G >= 2) v = mulD(v, xor_lane<1>(v));
+    if constexpr (G >= 4) v = mulD(v, xor_lane<2>(v));
+    if constexpr (G >= 8) v = mulD(v, xor_lane<7>(v));
+    if constexpr (G >= 16) v = mulD(v, xor_lane<15>(v));
+    if constexpr (G >= 32) v = mulD(v, xor_lane<16>(v));
+    if constexpr (G >= 64) v = mulD(v, xor_lane<32>(v));
     return v;
 }
 template <int G, typename D>
 __device__ inline D group_sum(D v)
 {
-#pragma unroll
-    for (int off = G / 2; off >= 1; off >>= 1) v = addD(v, shflxD(v, off));
+    if constexpr (G >= 2) v = addD(v, xor_lane<1>(v));
+    if constexpr (G >= 4) v = addD(v, xor_lane<2>(v));
+    if constexpr (G >= 8) v = addD(v, xor_lane<7>(v));
+    if constexpr (G >= 16) v = addD(v, xor_lane<15>(v));
+    if constexpr (G >= 32) v = addD(v, xor_lane<16>(v));
+    if constexpr (G >= 64) v = addD(v, xor_lane<32>(v));
     return v;
 }
 

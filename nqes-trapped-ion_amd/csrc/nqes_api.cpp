@@ -14,6 +14,7 @@
 // Host arithmetic is compiled with -ffp-contract=off as the FP64 device code is.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -99,7 +100,7 @@ struct nqes_ctx {
     DevBuf dO, dHs, dEs, dspins, dacc;
     DevBuf dmpart_o, dmpart_s, dmpart_f, dmo, dms, dmf, dhpart, dhsum, depart, desum;
     DevBuf dv, dov, dw, dsvpart, dsvwpart, dsvout, dsvscratch, dmscratch;
-    DevBuf dU, dwrow, dsvfpart, dsvfscratch;  // factored S*v (mixed path)
+    DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
     bool svf = false;
     SvfShape svf_shape{0, 0, 0, 0, 0};
     int mchunks = 1, svchunks = 1;
@@ -295,7 +296,7 @@ static int sample_fp64(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.shT = c->dshT.p;
     a.eaP = c->deaP.p;
     a.eaN = c->deaN.p;
-    a.O = c->dO.p;
+    a.O = c->svf ? nullptr : c->dO.p;
     a.Hs = c->dHs.p;
     a.Es = c->dEs.p;
     a.spins = static_cast<uint32_t *>(c->dspins.p);
@@ -307,6 +308,31 @@ static int sample_fp64(nqes_ctx *c, uint64_t seed, int C, int m, int B)
                                   c->dmpart_f.p, c->dmo.p, c->dms.p, c->dmf.p, c->stream),
            "moments kernel");
     return NQES_OK;
+}
+
+// arguments of the factored S*v / moments kernels for S samples
+static SvfArgs svf_args(nqes_ctx *c, long long S)
+{
+    SvfArgs f;
+    f.N = c->N;
+    f.M = c->M;
+    f.K = c->K;
+    f.NW = c->NW;
+    f.Ucols = c->svf_shape.ucols;
+    f.n4 = (c->N + 3) & ~3;
+    f.S = S;
+    f.P = c->P;
+    f.L = c->L;
+    f.spins = static_cast<const uint32_t *>(c->dspins.p);
+    f.U = c->dU.p;
+    f.v = c->dv.p;
+    f.ov = c->dov.p;
+    f.wrow = c->dwrow.p;
+    f.w = c->dw.p;
+    f.part = c->dsvfpart.p;
+    f.E = c->dEs.p;
+    f.wsel = 0;
+    return f;
 }
 
 static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
@@ -352,7 +378,7 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.spins0 = (const uint32_t *)c->dspins0.p;
     a.rng0 = (const uint64_t *)c->drng0.p;
     a.tab_lds = sw.tab;
-    a.O = c->dO.p;
+    a.O = c->svf ? nullptr : c->dO.p;
     a.Hs = c->dHs.p;
     a.Es = c->dEs.p;
     a.spins = static_cast<uint32_t *>(c->dspins.p);
@@ -364,6 +390,13 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.tab_lds = ms.tab;
     HIPCHK(c, nq_launch_measure_mx(cplx, c->K, a, ms.G, ms.wpb, c->stream), "mixed measure kernel");
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
+    if (c->svf) {
+        SvfArgs f = svf_args(c, S);
+        HIPCHK(c, nq_launch_svf_moments(cplx, f, c->svf_shape, c->dsvfpart.p, c->dsvfscratch.p, c->dsvftmp.p, c->dmo.p,
+                                        c->dms.p, c->dmf.p, c->stream),
+               "factored moments kernels");
+        return NQES_OK;
+    }
     HIPCHK(c, c->dmscratch.ensure((size_t)(c->mchunks / 32 + 2) * c->P * 16), "alloc moments scratch");
     HIPCHK(c, nq_launch_moments_mx(cplx, c->dO.p, c->dEs.p, S, c->P, c->Pld, c->mchunks, c->dmpart_o.p,
                                    c->dmpart_s.p, c->dmpart_f.p, c->dmscratch.p, c->dmo.p, c->dms.p, c->dmf.p,
@@ -403,12 +436,15 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     if (c->svf) {
         const size_t tsz = cplx ? 8 : 4;
         const long long nb = c->svf_shape.bwd_blocks;
+        const long long npair = K > 1 ? (long long)K * (K - 1) / 2 : 1;
+        const long long nchunk = nb * npair;
         HIPCHK(c, c->dU.ensure((size_t)S * K * c->svf_shape.ucols * tsz), "alloc S*v factors");
         HIPCHK(c, c->dwrow.ensure((size_t)S * K * 16), "alloc S*v rows");
-        HIPCHK(c, c->dsvfpart.ensure((size_t)nb * (P + 1) * 16), "alloc S*v partials");
-        HIPCHK(c, c->dsvfscratch.ensure((size_t)(nb / 32 + 2) * (P + 1) * 16), "alloc S*v scratch");
+        HIPCHK(c, c->dsvfpart.ensure((size_t)std::max(nb * 16, nchunk * 8) * (P + 1)), "alloc S*v partials");
+        HIPCHK(c, c->dsvfscratch.ensure((size_t)(nchunk / 32 + 2) * (P + 1) * 16), "alloc S*v scratch");
+        HIPCHK(c, c->dsvftmp.ensure((size_t)3 * (P + 1) * 16), "alloc moments");
     }
-    const size_t obytes = (size_t)S * c->Pld * o_elem(c->prec, cplx);
+    const size_t obytes = c->svf ? 16 : (size_t)S * c->Pld * o_elem(c->prec, cplx);  // factored: no O
     size_t freeb = 0, totalb = 0;
     HIPCHK(c, hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
     long long need = (long long)obytes - (long long)c->dO.bytes;
@@ -514,12 +550,58 @@ int nqes_get_samples(nqes_ctx *c, uint32_t *spins, double *H, uint64_t *accepts)
     return NQES_OK;
 }
 
+// O rows rebuilt from the factor rows and spins with the FP32 additions (in replica
+// order) of the measurement kernel; they equal the rows the streamed path stores up to
+// the FP32 rounding of c * tanh(theta), which the kernel may contract into an FMA:
+// a_i = sum_kk +-c_{k,kk}, b_j = sum_kk U_kk[kM+j], w_ij = sum_kk +-U_kk[kM+j]
+static int get_O_factored(nqes_ctx *c, int64_t row0, int64_t nrows, double *out)
+{
+    const int N = c->N, M = c->M, K = c->K, NW = c->NW, UC = c->svf_shape.ucols, L = c->L;
+    const size_t tsz = c->cplx ? 8 : 4;
+    std::vector<float> U((size_t)nrows * K * UC * (c->cplx ? 2 : 1));
+    std::vector<uint32_t> sp((size_t)nrows * K * NW);
+    HIPCHK(c, hipMemcpy(U.data(), static_cast<char *>(c->dU.p) + (size_t)row0 * K * UC * tsz, U.size() * 4,
+                        hipMemcpyDeviceToHost),
+           "copy S*v factors");
+    HIPCHK(c, hipMemcpy(sp.data(), static_cast<uint32_t *>(c->dspins.p) + (size_t)row0 * K * NW, sp.size() * 4,
+                        hipMemcpyDeviceToHost),
+           "copy spins");
+    const int R = c->cplx ? 2 : 1;
+    for (int64_t r = 0; r < nrows; r++) {
+        auto u = [&](int kk, int col, int part) { return U[(((size_t)r * K + kk) * UC + col) * R + part]; };
+        auto bit = [&](int kk, int i) { return (sp[((size_t)r * K + kk) * NW + (i >> 5)] >> (i & 31)) & 1u; };
+        double *row = out + 2 * (size_t)r * c->P;
+        for (int k = 0; k < K; k++)
+            for (int part = 0; part < R; part++) {
+                for (int i = 0; i < N; i++) {
+                    float v = 0.0f;
+                    for (int kk = 0; kk < K; kk++) v = bit(kk, i) ? v - u(kk, K * M + k, part) : v + u(kk, K * M + k, part);
+                    row[2 * ((size_t)k * L + i) + part] = v;
+                }
+                for (int j = 0; j < M; j++) {
+                    float v = 0.0f;
+                    for (int kk = 0; kk < K; kk++) v = v + u(kk, k * M + j, part);
+                    row[2 * ((size_t)k * L + N + j) + part] = v;
+                    for (int i = 0; i < N; i++) {
+                        float w = 0.0f;
+                        for (int kk = 0; kk < K; kk++) w = bit(kk, i) ? w - u(kk, k * M + j, part) : w + u(kk, k * M + j, part);
+                        row[2 * ((size_t)k * L + N + M + (size_t)i * M + j) + part] = w;
+                    }
+                }
+            }
+        if (R == 1)
+            for (long long q = 0; q < c->P; q++) row[2 * q + 1] = 0.0;
+    }
+    return NQES_OK;
+}
+
 int nqes_get_O(nqes_ctx *c, int64_t row0, int64_t nrows, double *out)
 {
     if (!c) return NQES_EINVAL;
     if (!c->sampled) return c->fail(NQES_ESTATE, "no samples: call nqes_sample first");
     if (!out || row0 < 0 || nrows < 0 || row0 + nrows > c->S) return c->fail(NQES_EINVAL, "row range out of bounds");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    if (c->svf) return get_O_factored(c, row0, nrows, out);
     const size_t e = o_elem(c->prec, c->cplx);
     std::vector<unsigned char> buf((size_t)nrows * c->Pld * e);
     HIPCHK(c,
@@ -568,23 +650,7 @@ static int sv_mixed_device(nqes_ctx *c, const void *v, const void *ov, size_t ds
     HIPCHK(c, hipMemcpyAsync(c->dov.p, ov, dsize, hipMemcpyHostToDevice, c->stream), "copy ov");
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
     if (c->svf) {
-        SvfArgs f;
-        f.N = c->N;
-        f.M = c->M;
-        f.K = c->K;
-        f.NW = c->NW;
-        f.Ucols = c->svf_shape.ucols;
-        f.n4 = (c->N + 3) & ~3;
-        f.S = c->S;
-        f.P = P;
-        f.L = c->L;
-        f.spins = static_cast<const uint32_t *>(c->dspins.p);
-        f.U = c->dU.p;
-        f.v = c->dv.p;
-        f.ov = c->dov.p;
-        f.wrow = c->dwrow.p;
-        f.w = c->dw.p;
-        f.part = c->dsvfpart.p;
+        const SvfArgs f = svf_args(c, c->S);
         HIPCHK(c, nq_launch_svf(c->cplx, f, c->svf_shape, c->dsvfscratch.p, c->dsvout.p, c->stream),
                "factored S*v kernels");
     } else {

@@ -181,7 +181,7 @@ __global__ void k_svf_combine(SvfArgs a)
 // l%16]) w_s; result rows i = 16 it + 4 e + l/16 (e < 4) at column 16 ct + l%16.  The
 // B operand is also summed over rows (b block), and wavefront 0 sums w_s over rows
 // with kk = 0.  Output: part[b][0..P].
-template <bool CPLX, int CTW, int NITM>
+template <bool CPLX, int CTW, int NITM, int WSEL>
 __global__ __launch_bounds__(1024) void k_svf_bwd(SvfArgs a)
 {
     using T = typename Ty<CPLX>::T;
@@ -220,7 +220,12 @@ __global__ __launch_bounds__(1024) void k_svf_bwd(SvfArgs a)
         if (ok) {
             w0 = a.spins[r * NW];
             if (NW > 1) w1 = a.spins[r * NW + 1];
-            wr = w[r / K];
+            if constexpr (WSEL == 0) wr = w[r / K];
+            else if constexpr (WSEL == 1) wr = mkD<D>(1.0, 0.0);
+            else {
+                const cd e = ((const cd *)a.E)[r / K];
+                wr = mkD<D>(e.re, e.im);
+            }
 #pragma unroll
             for (int c = 0; c < CTW; c++) {
                 const int ct = wv * CTW + c;
@@ -284,6 +289,21 @@ __global__ __launch_bounds__(1024) void k_svf_bwd(SvfArgs a)
     }
 }
 
+// backward pass with weights w (WSEL 0), ones (1) or the local energies (2)
+template <bool CPLX, int WSEL>
+hipError_t launch_bwd(const SvfArgs &a, const SvfShape &sh, hipStream_t st)
+{
+    const dim3 bb(64 * sh.bwd_waves);
+    const bool n32 = a.N <= 32;
+    if (sh.ppw == 1 && n32) hipLaunchKernelGGL((k_svf_bwd<CPLX, 1, 2, WSEL>), dim3(sh.bwd_blocks), bb, 0, st, a);
+    else if (sh.ppw == 1) hipLaunchKernelGGL((k_svf_bwd<CPLX, 1, 4, WSEL>), dim3(sh.bwd_blocks), bb, 0, st, a);
+    else if constexpr (!CPLX) {  // complex shapes keep one tile per wavefront (registers)
+        if (n32) hipLaunchKernelGGL((k_svf_bwd<CPLX, 2, 2, WSEL>), dim3(sh.bwd_blocks), bb, 0, st, a);
+        else hipLaunchKernelGGL((k_svf_bwd<CPLX, 2, 4, WSEL>), dim3(sh.bwd_blocks), bb, 0, st, a);
+    } else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 template <bool CPLX>
 hipError_t launch_svf(const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st)
 {
@@ -303,13 +323,8 @@ hipError_t launch_svf(const SvfArgs &a, const SvfShape &sh, void *scratch, void 
         hipLaunchKernelGGL((k_svf_fwd<CPLX, 16>), dim3(sh.fwd_blocks), dim3(256), lds, st, a);
     hipLaunchKernelGGL(k_svf_combine<D>, dim3(1024), dim3(256), 0, st, a);
     const dim3 bb(64 * sh.bwd_waves);
-    const bool n32 = a.N <= 32;
-    if (sh.ppw == 1 && n32) hipLaunchKernelGGL((k_svf_bwd<CPLX, 1, 2>), dim3(sh.bwd_blocks), bb, 0, st, a);
-    else if (sh.ppw == 1) hipLaunchKernelGGL((k_svf_bwd<CPLX, 1, 4>), dim3(sh.bwd_blocks), bb, 0, st, a);
-    else if constexpr (!CPLX) {  // complex shapes keep one tile per wavefront (registers)
-        if (n32) hipLaunchKernelGGL((k_svf_bwd<CPLX, 2, 2>), dim3(sh.bwd_blocks), bb, 0, st, a);
-        else hipLaunchKernelGGL((k_svf_bwd<CPLX, 2, 4>), dim3(sh.bwd_blocks), bb, 0, st, a);
-    } else return hipErrorInvalidValue;
+    hipError_t eb = launch_bwd<CPLX, 0>(a, sh, st);
+    if (eb != hipSuccess) return eb;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     nq_chunk_reduce(CPLX, a.part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, out, st);
@@ -317,6 +332,155 @@ hipError_t launch_svf(const SvfArgs &a, const SvfShape &sh, void *scratch, void 
 }
 
 } // namespace
+
+// Sum over samples of |O_sp|^2 from the factors.  With U_kk = U[s K + kk][col]:
+//   a and w blocks  sum_s sum_kk |U_kk|^2 + 2 sum_{kk < kk'} sum_s tau_s[i] Re(U_kk conj U_kk'),
+//                   tau_s[i] = sigma^kk_i sigma^kk'_i (+-1),
+//   b block         sum_s |sum_kk U_kk|^2.
+// Grid (blocks over sample chunks, replica pairs); wavefronts over column tiles.  The
+// cross term of pair (kk, kk') is a GEMM [N x S] [S x Ucols] on FP64 MFMA (A operand tau,
+// B operand Re(U_kk conj U_kk')).  Pair 0 also adds the diagonal and b-block sums.
+// Block (b, pair) writes a full partial vector part[pair nblk + b][0..P).
+template <bool CPLX, int NITM>
+__global__ __launch_bounds__(1024) void k_svf_sq(SvfArgs a)
+{
+    using T = typename Ty<CPLX>::T;
+    const int N = a.N, M = a.M, K = a.K, NW = a.NW, UC = a.Ucols;
+    const int NCT = UC / 16, NIT = (N + 15) / 16;
+    const int NWV = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int hi = lane >> 4, lo = lane & 15;
+    const long long S = a.S, KM = (long long)K * M;
+    const long long n4 = (S + 3) / 4;
+    const long long q0 = n4 * blockIdx.x / gridDim.x, q1 = n4 * (blockIdx.x + 1) / gridDim.x;
+    // replica pair of this block: pairs enumerated (0,1), (0,2), .., (1,2), ..; K = 1: none
+    const int pair = blockIdx.y;
+    int ka = 0, kb = 0;
+    {
+        int q = 0;
+        for (int x = 0; x < K; x++)
+            for (int y = x + 1; y < K; y++, q++)
+                if (q == pair) {
+                    ka = x;
+                    kb = y;
+                }
+    }
+    const bool cross = K > 1;
+    const T *U = (const T *)a.U;
+    double *part = (double *)a.part + ((long long)pair * gridDim.x + blockIdx.x) * (a.P + 1);
+    for (int ct = wv; ct < NCT; ct += NWV) {
+        const int col = 16 * ct + lo;
+        v4d acc[NITM];
+#pragma unroll
+        for (int it = 0; it < NITM; it++) acc[it] = v4d{0.0, 0.0, 0.0, 0.0};
+        double dg = 0.0, bq = 0.0;
+        for (long long step = q0; step < q1; step++) {
+            const long long s = step * 4 + hi;
+            const bool ok = s < S;
+            double vab = 0.0;
+            uint32_t t0 = 0, t1 = 0;  // spin words of tau: bits of kk xor kk'
+            if (ok) {
+                const T *Us = U + s * K * UC + col;
+                if (cross) {
+                    const T ua = Us[ka * UC], ub = Us[kb * UC];
+                    if constexpr (CPLX) vab = (double)ua.re * ub.re + (double)ua.im * ub.im;
+                    else vab = (double)ua * ub;
+                    t0 = a.spins[(s * K + ka) * NW] ^ a.spins[(s * K + kb) * NW];
+                    if (NW > 1) t1 = a.spins[(s * K + ka) * NW + 1] ^ a.spins[(s * K + kb) * NW + 1];
+                }
+                if (pair == 0) {
+                    double br = 0.0, bi = 0.0;
+                    for (int kk = 0; kk < K; kk++) {
+                        const T u = Us[kk * UC];
+                        if constexpr (CPLX) {
+                            dg += (double)u.re * u.re + (double)u.im * u.im;
+                            br += u.re;
+                            bi += u.im;
+                        } else {
+                            dg += (double)u * u;
+                            br += u;
+                        }
+                    }
+                    bq += br * br + bi * bi;
+                }
+            }
+            if (cross) {
+#pragma unroll
+                for (int it = 0; it < NITM; it++) {
+                    if (it < NIT) {
+                        const int i = 16 * it + lo;
+                        const double av = (ok && i < N) ? spin_pm(t0, t1, i) : 0.0;
+                        acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, vab, acc[it], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        // column sums over the 4 sample lanes of each column
+        dg += __shfl_xor(dg, 16);
+        dg += __shfl_xor(dg, 32);
+        bq += __shfl_xor(bq, 16);
+        bq += __shfl_xor(bq, 32);
+        if (col < KM + K) {
+            const double dcol = dg;  // every lane of column col holds its diagonal sum
+#pragma unroll
+            for (int it = 0; it < NITM; it++) {
+                if (it < NIT) {
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        const int i = 16 * it + 4 * e + hi;
+                        if (i < N) {
+                            const double g = 2.0 * acc[it][e] + (pair == 0 ? dcol : 0.0);
+                            if (col < KM) part[(col / M) * a.L + N + M + (long long)i * M + col % M] = g;
+                            else part[(col - KM) * a.L + i] = g;
+                        }
+                    }
+                }
+            }
+            if (hi == 0 && col < KM) part[(col / M) * a.L + N + col % M] = (pair == 0) ? bq : 0.0;
+        }
+    }
+}
+
+// so = conj(sum_s conj O_s) (w = 1), sf = sum_s E conj O_s, ss = sum |O|^2 -> complex arrays
+template <typename D>
+__global__ void k_svf_pack_moments(long long P, const D *r1, const D *rE, const double *rsq, cd *so, cd *ss, cd *sf)
+{
+    for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < P; p += (long long)gridDim.x * blockDim.x) {
+        const cd x1 = nq64::mk(reD(r1[p]), imD(r1[p])), xe = nq64::mk(reD(rE[p]), imD(rE[p]));
+        so[p] = nq64::conj(x1);
+        sf[p] = xe;
+        ss[p] = nq64::mk(rsq[p], 0.0);
+    }
+}
+
+template <bool CPLX>
+hipError_t launch_svf_moments(const SvfArgs &a0, const SvfShape &sh, void *part, void *scratch, void *tmp, void *so,
+                              void *ss, void *sf, hipStream_t st)
+{
+    using D = typename Ty<CPLX>::D;
+    const long long P = a0.P;
+    D *r1 = (D *)tmp, *rE = r1 + (P + 1);
+    double *rsq = (double *)(rE + (P + 1));
+    const bool n32 = a0.N <= 32;
+    SvfArgs a = a0;
+    a.part = part;
+    for (int pass = 0; pass < 2; pass++) {
+        hipError_t e = pass == 0 ? launch_bwd<CPLX, 1>(a, sh, st) : launch_bwd<CPLX, 2>(a, sh, st);
+        if (e != hipSuccess) return e;
+        nq_chunk_reduce(CPLX, part, P + 1, P + 1, sh.bwd_blocks, scratch, pass == 0 ? (void *)r1 : (void *)rE, st);
+    }
+    const int npair = a0.K > 1 ? a0.K * (a0.K - 1) / 2 : 1;
+    const int nct = a0.Ucols / 16;
+    const int waves = nct < 16 ? nct : 16;
+    if (n32) hipLaunchKernelGGL((k_svf_sq<CPLX, 2>), dim3(sh.bwd_blocks, npair), dim3(64 * waves), 0, st, a);
+    else hipLaunchKernelGGL((k_svf_sq<CPLX, 4>), dim3(sh.bwd_blocks, npair), dim3(64 * waves), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    nq_chunk_reduce(false, part, P, P + 1, sh.bwd_blocks * npair, scratch, rsq, st);
+    hipLaunchKernelGGL(k_svf_pack_moments<D>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, P, r1, rE, rsq,
+                       (cd *)so, (cd *)ss, (cd *)sf);
+    return hipGetLastError();
+}
 
 // Factored S*v applies when N <= 64, B fits in LDS and the column tiles fit one
 // block of at most 16 wavefronts with 1 or (real weights) 2 tiles each (ppw = tiles
@@ -343,6 +507,13 @@ SvfShape nq_svf_shape(bool cplx, int N, int M, int K)
     e = getenv("NQES_SVF_FWD_BLOCKS");
     if (e && *e && atoi(e) > 0) sh.fwd_blocks = atoi(e);
     return sh;
+}
+
+hipError_t nq_launch_svf_moments(bool cplx, const SvfArgs &a, const SvfShape &sh, void *part, void *scratch, void *tmp,
+                                 void *so, void *ss, void *sf, hipStream_t st)
+{
+    return cplx ? launch_svf_moments<true>(a, sh, part, scratch, tmp, so, ss, sf, st)
+                : launch_svf_moments<false>(a, sh, part, scratch, tmp, so, ss, sf, st);
 }
 
 hipError_t nq_launch_svf(bool cplx, const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st)

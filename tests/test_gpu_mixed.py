@@ -111,15 +111,29 @@ def test_factored_sv_matches_stored(nqes, monkeypatch, N, M, K, C, m, B, cplx):
     rng = np.random.default_rng(5)
     vs = [rng.normal(size=K * (N + M + N * M)) + 0j,
           rng.normal(size=K * (N + M + N * M)) + 1j * rng.normal(size=K * (N + M + N * M))]
-    out = {}
+    out, mom, Orows = {}, {}, {}
     for mode in ("factored", "stored"):
         monkeypatch.setenv("NQES_SV", mode)
         with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
             s.sample(W, 4321, C, m, B, 0.7)
             out[mode] = [s.apply_S(v) for v in vs]
+            mom[mode] = s.moments()
+            Orows[mode] = s.O_rows(0, C * m)
     tol = 1e-12 if K == 1 else 1e-6
     for a, b in zip(out["factored"], out["stored"]):
         assert rel_err(a, b) < tol, rel_err(a, b)
+    # O rows rebuilt from the factors equal the stored rows to FP32 rounding (the
+    # kernels may contract c * tanh(theta) into an FMA differently on the two paths)
+    assert rel_err(Orows["factored"], Orows["stored"]) < 1e-6
+    # moments from the factors (FP64 on the exact factors) vs from the FP32 O rows
+    O = Orows["stored"]
+    o = O.mean(axis=0)
+    for key in ("o", "skk", "f"):
+        assert rel_err(mom["factored"][key], mom["stored"][key]) < tol, (key, rel_err(mom["factored"][key], mom["stored"][key]))
+    assert rel_err(mom["factored"]["o"], o) < tol
+    skk = 0.7 * ((np.abs(O) ** 2).mean(axis=0) - np.abs(o) ** 2)
+    assert rel_err(mom["factored"]["skk"].real, skk) < max(tol, 1e-10)
+    assert mom["factored"]["emat"] is not None and np.array_equal(mom["factored"]["emat"], mom["stored"]["emat"])
 
 
 @pytest.mark.parametrize("cplx", [False, True])
