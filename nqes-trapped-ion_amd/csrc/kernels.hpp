@@ -80,9 +80,13 @@ struct SvfArgs {
     void *part;                // bwd_blocks x (P + 1) D
     const void *E;             // S complex double: local energies (moments)
     int wsel;                  // backward-pass weights: 0 w, 1 ones, 2 E
+    int ncg;                   // forward column groups (wrow holds ncg x S*K partial sums)
 };
+// ok = 0: the factored form does not apply (N > 64).  The forward pass splits the
+// Ucols/16 column tiles into fwd_cgs groups of fwd_cgt (B of a group fits in LDS); the
+// backward pass into bwd_cgs groups of up to bwd_waves tiles, one wavefront per tile.
 struct SvfShape {
-    int ucols, ppw, bwd_waves, bwd_blocks, fwd_blocks;
+    int ucols, ok, bwd_waves, bwd_blocks, bwd_cgs, fwd_blocks, fwd_cgt, fwd_cgs;
 };
 SvfShape nq_svf_shape(bool cplx, int N, int M, int K);
 hipError_t nq_launch_svf(bool cplx, const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st);

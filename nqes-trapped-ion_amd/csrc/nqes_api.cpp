@@ -102,7 +102,7 @@ struct nqes_ctx {
     DevBuf dv, dov, dw, dsvpart, dsvwpart, dsvout, dsvscratch, dmscratch;
     DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
     bool svf = false;
-    SvfShape svf_shape{0, 0, 0, 0, 0};
+    SvfShape svf_shape{};
     int mchunks = 1, svchunks = 1;
     std::vector<cd> o, skk, f, emat;
     cd eloc{0, 0};
@@ -431,7 +431,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     if (c->prec == NQES_PREC_MIXED) {
         const char *e = getenv("NQES_SV");
         c->svf_shape = nq_svf_shape(cplx, c->N, c->M, K);
-        c->svf = c->svf_shape.ppw > 0 && !(e && strcmp(e, "stored") == 0);
+        c->svf = c->svf_shape.ok && !(e && strcmp(e, "stored") == 0);
     }
     if (c->svf) {
         const size_t tsz = cplx ? 8 : 4;
@@ -439,7 +439,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
         const long long npair = K > 1 ? (long long)K * (K - 1) / 2 : 1;
         const long long nchunk = nb * npair;
         HIPCHK(c, c->dU.ensure((size_t)S * K * c->svf_shape.ucols * tsz), "alloc S*v factors");
-        HIPCHK(c, c->dwrow.ensure((size_t)S * K * 16), "alloc S*v rows");
+        HIPCHK(c, c->dwrow.ensure((size_t)c->svf_shape.fwd_cgs * S * K * 16), "alloc S*v rows");
         HIPCHK(c, c->dsvfpart.ensure((size_t)std::max(nb * 16, nchunk * 8) * (P + 1)), "alloc S*v partials");
         HIPCHK(c, c->dsvfscratch.ensure((size_t)(nchunk / 32 + 2) * (P + 1) * 16), "alloc S*v scratch");
         HIPCHK(c, c->dsvftmp.ensure((size_t)3 * (P + 1) * 16), "alloc moments");

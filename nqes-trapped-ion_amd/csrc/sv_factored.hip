@@ -53,16 +53,16 @@ __device__ inline double spin_pm(uint32_t w0, uint32_t w1, int i)
     return ((w >> (i & 31)) & 1u) ? -1.0 : 1.0;
 }
 
-// B (N4 x Ucols) and bias (Ucols) from v, in LDS
+// columns [c0, c0 + W) of B (N4 x W) and of bias (W) from v, in LDS
 template <typename D>
-__device__ inline void stage_B(const SvfArgs &a, D *Bl, D *bias)
+__device__ inline void stage_B(const SvfArgs &a, D *Bl, D *bias, int c0, int W)
 {
     const D *v = (const D *)a.v;
-    const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
+    const int N = a.N, M = a.M, K = a.K;
     const int N4 = a.n4;
     const long long KM = (long long)K * M;
-    for (long long q = threadIdx.x; q < (long long)N4 * UC; q += blockDim.x) {
-        const int i = (int)(q / UC), col = (int)(q % UC);
+    for (long long q = threadIdx.x; q < (long long)N4 * W; q += blockDim.x) {
+        const int i = (int)(q / W), col = c0 + (int)(q % W);
         D b = zeroD<D>();
         if (i < N) {
             if (col < KM) {
@@ -74,10 +74,11 @@ __device__ inline void stage_B(const SvfArgs &a, D *Bl, D *bias)
         }
         Bl[q] = b;
     }
-    for (int col = threadIdx.x; col < UC; col += blockDim.x) {
+    for (int q = threadIdx.x; q < W; q += blockDim.x) {
+        const int col = c0 + q;
         D b = zeroD<D>();
         if (col < KM) b = v[(long long)(col / M) * a.L + N + col % M];
-        bias[col] = b;
+        bias[q] = b;
     }
 }
 
@@ -85,11 +86,13 @@ __device__ inline void stage_B(const SvfArgs &a, D *Bl, D *bias)
 // result register e of lane l is D[4 e + l/16][l%16] (rows interleaved by lane group,
 // unlike the FP32 16x16x4 form whose register e holds row 4 (l/16) + e).
 //
-// Pass 1.  A block of 4 wavefronts loops over 16-row tiles.  Lane l: A operand row
-// r0 + l%16 (spins i = 4 q + l/16), result rows r0 + 4 e + l/16 (e < 4) at column l%16
-// of each 16-column tile.  NS = N4/4 steps of the K dimension (N <= 64).
+// Pass 1.  Block (b, g) handles column group g (tiles [g cgt, min((g+1) cgt, NCT)),
+// its part of B staged in LDS) and loops with its 4 wavefronts over 16-row tiles.
+// Lane l: A operand row r0 + l%16 (spins i = 4 q + l/16), result rows r0 + 4 e + l/16
+// (e < 4) at column l%16 of each 16-column tile.  NS = N4/4 steps of the K dimension
+// (N <= 64).  Output: the group's partial row sums wrow[g R + r].
 template <bool CPLX, int NSMAX>
-__global__ __launch_bounds__(256) void k_svf_fwd(SvfArgs a)
+__global__ __launch_bounds__(256) void k_svf_fwd(SvfArgs a, int cgt)
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
@@ -97,16 +100,20 @@ __global__ __launch_bounds__(256) void k_svf_fwd(SvfArgs a)
     const int N = a.N, K = a.K, NW = a.NW, UC = a.Ucols;
     const int N4 = a.n4;
     const int NS = N4 / 4;
+    const int NCT = UC / 16;
+    const int ct0 = blockIdx.y * cgt;
+    const int nct = (NCT - ct0 < cgt) ? NCT - ct0 : cgt;
+    const int W = 16 * nct;
     D *Bl = (D *)smem_f;
-    D *bias = Bl + (long long)N4 * UC;
-    stage_B<D>(a, Bl, bias);
+    D *bias = Bl + (long long)N4 * W;
+    stage_B<D>(a, Bl, bias, 16 * ct0, W);
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int hi = lane >> 4, lo = lane & 15;
     const long long R = a.S * K;
     const long long ntile = (R + 15) / 16;
     const T *U = (const T *)a.U;
-    D *wrow = (D *)a.wrow;
+    D *wrow = (D *)a.wrow + (long long)blockIdx.y * R;
     for (long long t = (long long)blockIdx.x * 4 + wv; t < ntile; t += (long long)gridDim.x * 4) {
         const long long r0 = t * 16;
         const long long ra = r0 + lo;
@@ -124,19 +131,19 @@ __global__ __launch_bounds__(256) void k_svf_fwd(SvfArgs a)
         D rs[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) rs[q] = zeroD<D>();
-        for (int ct = 0; ct < UC / 16; ct++) {
-            const int col = ct * 16 + lo;
+        for (int c = 0; c < nct; c++) {
+            const int cl = c * 16 + lo, col = 16 * ct0 + cl;
             v4d accR = {0.0, 0.0, 0.0, 0.0};
             v4d accI = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int q = 0; q < NSMAX; q++) {
                 if (q < NS) {
-                    const D bv = Bl[(long long)(4 * q + hi) * UC + col];
+                    const D bv = Bl[(long long)(4 * q + hi) * W + cl];
                     accR = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], reD(bv), accR, 0, 0, 0);
                     if constexpr (CPLX) accI = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], imD(bv), accI, 0, 0, 0);
                 }
             }
-            const D bc = bias[col];
+            const D bc = bias[cl];
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const long long r = r0 + 4 * q + hi;
@@ -159,37 +166,44 @@ __global__ __launch_bounds__(256) void k_svf_fwd(SvfArgs a)
     }
 }
 
-// Pass 2.  w_s = sum_kk wrow[s K + kk] - o^T v
+// Pass 2.  w_s = sum_g sum_kk wrow[g R + s K + kk] - o^T v
 template <typename D>
 __global__ void k_svf_combine(SvfArgs a)
 {
     const D *wrow = (const D *)a.wrow;
     D *w = (D *)a.w;
     const D ov = *(const D *)a.ov;
+    const long long R = a.S * a.K;
     for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < a.S;
          s += (long long)gridDim.x * blockDim.x) {
-        D acc = wrow[s * a.K];
-        for (int kk = 1; kk < a.K; kk++) acc = addD(acc, wrow[s * a.K + kk]);
+        D acc = zeroD<D>();
+        for (int g = 0; g < a.ncg; g++)
+            for (int kk = 0; kk < a.K; kk++) acc = addD(acc, wrow[g * R + s * a.K + kk]);
         w[s] = subD(acc, ov);
     }
 }
 
-// Pass 3.  Block b takes the 4-row steps [b n4 / nblk, (b+1) n4 / nblk); wavefront wv
-// owns the column tiles ct = wv CTW + c (c < CTW) with all their i tiles
-// it < NIT = ceil(N / 16), so each U value is loaded once and feeds NIT MFMAs.  Lane
-// l: A operand sigma_{r0 + l/16}[16 it + l%16], B operand conj(U[r0 + l/16][16 ct +
-// l%16]) w_s; result rows i = 16 it + 4 e + l/16 (e < 4) at column 16 ct + l%16.  The
-// B operand is also summed over rows (b block), and wavefront 0 sums w_s over rows
-// with kk = 0.  Output: part[b][0..P].
-template <bool CPLX, int CTW, int NITM, int WSEL>
+// Pass 3.  Block (b, g) takes the 4-row steps [b n4 / nblk, (b+1) n4 / nblk) and the
+// column tiles of group g: wavefront wv owns tile ct = g GS + wv (GS wavefronts per
+// block) with all its i tiles it < NIT = ceil(N / 16), so each U value is loaded once
+// and feeds NIT MFMAs.  Lane l: A operand sigma_{r0 + l/16}[16 it + l%16], B operand
+// conj(U[r0 + l/16][16 ct + l%16]) w_s; result rows i = 16 it + 4 e + l/16 (e < 4) at
+// column 16 ct + l%16.  The B operand is also summed over rows (b block), and group 0's
+// wavefront 0 sums w_s over rows with kk = 0.  The groups of block row b write disjoint
+// entries of part[b][0..P].
+template <bool CPLX, int NITM, int WSEL>
 __global__ __launch_bounds__(1024) void k_svf_bwd(SvfArgs a)
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
     const int N = a.N, M = a.M, K = a.K, NW = a.NW, UC = a.Ucols;
     const int NCT = UC / 16, NIT = (N + 15) / 16;
+    const int GS = blockDim.x >> 6;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int hi = lane >> 4, lo = lane & 15;
+    const int ct = blockIdx.y * GS + wv;
+    const bool tile = ct < NCT;
+    const bool wsum = blockIdx.y == 0 && wv == 0;
     const long long R = a.S * K;
     const long long KM = (long long)K * M;
     const long long n4 = (R + 3) / 4;
@@ -197,91 +211,70 @@ __global__ __launch_bounds__(1024) void k_svf_bwd(SvfArgs a)
     const T *U = (const T *)a.U;
     const D *w = (const D *)a.w;
 
-    v4d accR[CTW][NITM], accI[CTW][NITM];
-    D cs[CTW];
+    v4d accR[NITM], accI[NITM];
 #pragma unroll
-    for (int c = 0; c < CTW; c++) {
-#pragma unroll
-        for (int it = 0; it < NITM; it++) {
-            accR[c][it] = v4d{0.0, 0.0, 0.0, 0.0};
-            accI[c][it] = v4d{0.0, 0.0, 0.0, 0.0};
-        }
-        cs[c] = zeroD<D>();
+    for (int it = 0; it < NITM; it++) {
+        accR[it] = v4d{0.0, 0.0, 0.0, 0.0};
+        accI[it] = v4d{0.0, 0.0, 0.0, 0.0};
     }
-    D ws = zeroD<D>();
-    for (long long step = q0; step < q1; step++) {
-        const long long r = step * 4 + hi;  // this lane's row (K dimension)
-        const bool ok = r < R;
-        uint32_t w0 = 0, w1 = 0;
-        D wr = zeroD<D>();
-        T uv[CTW];
-#pragma unroll
-        for (int c = 0; c < CTW; c++) uv[c] = zeroT<T>();
-        if (ok) {
-            w0 = a.spins[r * NW];
-            if (NW > 1) w1 = a.spins[r * NW + 1];
-            if constexpr (WSEL == 0) wr = w[r / K];
-            else if constexpr (WSEL == 1) wr = mkD<D>(1.0, 0.0);
-            else {
-                const cd e = ((const cd *)a.E)[r / K];
-                wr = mkD<D>(e.re, e.im);
+    D cs = zeroD<D>(), ws = zeroD<D>();
+    if (tile || wsum)
+        for (long long step = q0; step < q1; step++) {
+            const long long r = step * 4 + hi;  // this lane's row (K dimension)
+            const bool ok = r < R;
+            uint32_t w0 = 0, w1 = 0;
+            D wr = zeroD<D>();
+            T uv = zeroT<T>();
+            if (ok) {
+                w0 = a.spins[r * NW];
+                if (NW > 1) w1 = a.spins[r * NW + 1];
+                if constexpr (WSEL == 0) wr = w[r / K];
+                else if constexpr (WSEL == 1) wr = mkD<D>(1.0, 0.0);
+                else {
+                    const cd e = ((const cd *)a.E)[r / K];
+                    wr = mkD<D>(e.re, e.im);
+                }
+                if (tile) uv = U[r * UC + 16 * ct + lo];
+                if (wsum && lo == 0 && (r % K) == 0) ws = addD(ws, wr);
             }
-#pragma unroll
-            for (int c = 0; c < CTW; c++) {
-                const int ct = wv * CTW + c;
-                if (ct < NCT) uv[c] = U[r * UC + 16 * ct + lo];
-            }
-            if (wv == 0 && lo == 0 && (r % K) == 0) ws = addD(ws, wr);
-        }
-        double av[NITM];
-#pragma unroll
-        for (int it = 0; it < NITM; it++) {
-            const int i = 16 * it + lo;
-            av[it] = (ok && it < NIT && i < N) ? spin_pm(w0, w1, i) : 0.0;
-        }
-#pragma unroll
-        for (int c = 0; c < CTW; c++) {
-            if (wv * CTW + c < NCT) {
-                const D bv = mulD(conjD(toD(uv[c])), wr);
-                cs[c] = addD(cs[c], bv);
+            if (tile) {
+                const D bv = mulD(conjD(toD(uv)), wr);
+                cs = addD(cs, bv);
 #pragma unroll
                 for (int it = 0; it < NITM; it++) {
                     if (it < NIT) {
-                        accR[c][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[it], reD(bv), accR[c][it], 0, 0, 0);
+                        const int i = 16 * it + lo;
+                        const double av = (ok && i < N) ? spin_pm(w0, w1, i) : 0.0;
+                        accR[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, reD(bv), accR[it], 0, 0, 0);
                         if constexpr (CPLX)
-                            accI[c][it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[it], imD(bv), accI[c][it], 0, 0, 0);
+                            accI[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, imD(bv), accI[it], 0, 0, 0);
                     }
                 }
             }
         }
-    }
     D *part = (D *)a.part + (long long)blockIdx.x * (a.P + 1);
+    if (tile) {
+        const int col = 16 * ct + lo;
 #pragma unroll
-    for (int c = 0; c < CTW; c++) {
-        const int ct = wv * CTW + c;
-        if (ct < NCT) {
-            const int col = 16 * ct + lo;
+        for (int it = 0; it < NITM; it++) {
+            if (it < NIT) {
 #pragma unroll
-            for (int it = 0; it < NITM; it++) {
-                if (it < NIT) {
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        const int i = 16 * it + 4 * e + hi;
-                        if (i < N) {
-                            const D g = mkD<D>(accR[c][it][e], accI[c][it][e]);
-                            if (col < KM) part[(col / M) * a.L + N + M + (long long)i * M + col % M] = g;
-                            else if (col < KM + K) part[(col - KM) * a.L + i] = g;
-                        }
+                for (int e = 0; e < 4; e++) {
+                    const int i = 16 * it + 4 * e + hi;
+                    if (i < N) {
+                        const D g = mkD<D>(accR[it][e], accI[it][e]);
+                        if (col < KM) part[(col / M) * a.L + N + M + (long long)i * M + col % M] = g;
+                        else if (col < KM + K) part[(col - KM) * a.L + i] = g;
                     }
                 }
             }
-            D t = cs[c];
-            t = addD(t, shflxD(t, 16));
-            t = addD(t, shflxD(t, 32));
-            if (hi == 0 && col < KM) part[(col / M) * a.L + N + col % M] = t;
         }
+        D t = cs;
+        t = addD(t, shflxD(t, 16));
+        t = addD(t, shflxD(t, 32));
+        if (hi == 0 && col < KM) part[(col / M) * a.L + N + col % M] = t;
     }
-    if (wv == 0) {
+    if (wsum) {
         D t = ws;
         t = addD(t, shflxD(t, 16));
         t = addD(t, shflxD(t, 32));
@@ -293,36 +286,27 @@ __global__ __launch_bounds__(1024) void k_svf_bwd(SvfArgs a)
 template <bool CPLX, int WSEL>
 hipError_t launch_bwd(const SvfArgs &a, const SvfShape &sh, hipStream_t st)
 {
-    const dim3 bb(64 * sh.bwd_waves);
-    const bool n32 = a.N <= 32;
-    if (sh.ppw == 1 && n32) hipLaunchKernelGGL((k_svf_bwd<CPLX, 1, 2, WSEL>), dim3(sh.bwd_blocks), bb, 0, st, a);
-    else if (sh.ppw == 1) hipLaunchKernelGGL((k_svf_bwd<CPLX, 1, 4, WSEL>), dim3(sh.bwd_blocks), bb, 0, st, a);
-    else if constexpr (!CPLX) {  // complex shapes keep one tile per wavefront (registers)
-        if (n32) hipLaunchKernelGGL((k_svf_bwd<CPLX, 2, 2, WSEL>), dim3(sh.bwd_blocks), bb, 0, st, a);
-        else hipLaunchKernelGGL((k_svf_bwd<CPLX, 2, 4, WSEL>), dim3(sh.bwd_blocks), bb, 0, st, a);
-    } else return hipErrorInvalidValue;
+    const dim3 grid(sh.bwd_blocks, sh.bwd_cgs), bb(64 * sh.bwd_waves);
+    if (a.N <= 32) hipLaunchKernelGGL((k_svf_bwd<CPLX, 2, WSEL>), grid, bb, 0, st, a);
+    else hipLaunchKernelGGL((k_svf_bwd<CPLX, 4, WSEL>), grid, bb, 0, st, a);
     return hipGetLastError();
 }
 
 template <bool CPLX>
-hipError_t launch_svf(const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st)
+hipError_t launch_svf(const SvfArgs &a0, const SvfShape &sh, void *scratch, void *out, hipStream_t st)
 {
     using D = typename Ty<CPLX>::D;
-    const size_t lds = ((size_t)a.n4 * a.Ucols + a.Ucols) * sizeof(D);
-    if (lds > 65536) {
-        hipError_t e = (a.N <= 32)
-                           ? hipFuncSetAttribute((const void *)k_svf_fwd<CPLX, 8>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
-                           : hipFuncSetAttribute((const void *)k_svf_fwd<CPLX, 16>,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    if (a.N <= 32)
-        hipLaunchKernelGGL((k_svf_fwd<CPLX, 8>), dim3(sh.fwd_blocks), dim3(256), lds, st, a);
+    SvfArgs a = a0;
+    a.ncg = sh.fwd_cgs;
+    const int W = 16 * sh.fwd_cgt;
+    const size_t lds = ((size_t)a.n4 * W + W) * sizeof(D);
+    if (lds > 65536) return hipErrorInvalidValue;  // groups are sized to fit
+    const dim3 grid(sh.fwd_blocks, sh.fwd_cgs);
+    if (a.n4 <= 32)
+        hipLaunchKernelGGL((k_svf_fwd<CPLX, 8>), grid, dim3(256), lds, st, a, sh.fwd_cgt);
     else
-        hipLaunchKernelGGL((k_svf_fwd<CPLX, 16>), dim3(sh.fwd_blocks), dim3(256), lds, st, a);
+        hipLaunchKernelGGL((k_svf_fwd<CPLX, 16>), grid, dim3(256), lds, st, a, sh.fwd_cgt);
     hipLaunchKernelGGL(k_svf_combine<D>, dim3(1024), dim3(256), 0, st, a);
-    const dim3 bb(64 * sh.bwd_waves);
     hipError_t eb = launch_bwd<CPLX, 0>(a, sh, st);
     if (eb != hipSuccess) return eb;
     hipError_t e = hipGetLastError();
@@ -482,24 +466,26 @@ hipError_t launch_svf_moments(const SvfArgs &a0, const SvfShape &sh, void *part,
     return hipGetLastError();
 }
 
-// Factored S*v applies when N <= 64, B fits in LDS and the column tiles fit one
-// block of at most 16 wavefronts with 1 or (real weights) 2 tiles each (ppw = tiles
-// per wavefront).
-// ppw = 0: not applicable.
+// Factored S*v applies when N <= 64 (two spin words per row, four i tiles).  Forward
+// column groups: as many 16-column tiles as keep B (N4 x 16 cgt in D) within 64 KB of
+// LDS.  Backward column groups: 16 tiles (one wavefront each) per block.
 SvfShape nq_svf_shape(bool cplx, int N, int M, int K)
 {
-    SvfShape sh{0, 0, 0, 0, 0};
+    SvfShape sh{};
     if (N > 64) return sh;
     const int UC = 16 * ((K * M + K + 15) / 16);
     const int nct = UC / 16;
     const int N4 = (N + 3) & ~3;
-    const size_t lds = ((size_t)N4 * UC + UC) * (cplx ? 16 : 8);
-    if (lds > 160 * 1024) return sh;
-    const int ppw = nct <= 16 ? 1 : (nct <= 32 && !cplx) ? 2 : 0;
-    if (!ppw) return sh;
-    sh.bwd_waves = (nct + ppw - 1) / ppw;
+    const size_t dsz = cplx ? 16 : 8;
+    int cgt = (int)(65536 / ((size_t)(N4 + 1) * 16 * dsz));
+    if (cgt > nct) cgt = nct;
+    if (cgt < 1) return sh;
     sh.ucols = UC;
-    sh.ppw = ppw;
+    sh.ok = 1;
+    sh.fwd_cgt = cgt;
+    sh.fwd_cgs = (nct + cgt - 1) / cgt;
+    sh.bwd_waves = nct < 16 ? nct : 16;
+    sh.bwd_cgs = (nct + 15) / 16;
     sh.bwd_blocks = 2048;
     sh.fwd_blocks = 1024;
     const char *e = getenv("NQES_SVF_BWD_BLOCKS");
