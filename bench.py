@@ -23,6 +23,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "nqes-trapped-ion_amd"))
 
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix, published
 
 
 def load_J(N):
@@ -41,11 +42,11 @@ def initial_weights(N, M, K, seed):
     return (rng.normal(size=K * (N + M + N * M)) * 0.01).astype(np.complex128)
 
 
-def profiled_traffic(args, real):
-    """HBM bytes per S*v launch measured by rocprofv3 (FETCH_SIZE x 2 + WRITE_SIZE,
-    copy-calibrated) for this exact workload, from the summaries under profiles/;
-    None when no summary covers it."""
-    key = (f"traffic: precision={args.precision} real={int(real)} N={args.N} M={args.M} K={args.K} "
+def profiled_traffic(args, real, mode):
+    """HBM bytes per S*v (all its kernels) measured by rocprofv3 (FETCH_SIZE x 2 +
+    WRITE_SIZE, copy-calibrated) for this exact workload and S*v form, from the
+    summaries under profiles/; None when no summary covers it."""
+    key = (f"traffic: precision={args.precision} sv={mode} real={int(real)} N={args.N} M={args.M} K={args.K} "
            f"C={args.C} m={args.m} bytes=")
     pdir = os.path.join(REPO, "profiles")
     for name in sorted(os.listdir(pdir)) if os.path.isdir(pdir) else []:
@@ -154,20 +155,36 @@ def main():
 
     P = args.K * (args.N + args.M + args.N * args.M)
     real = not np.any(W0.imag)
+    factored = args.precision == "mixed" and args.N <= 64 and os.environ.get("NQES_SV") != "stored"
     if args.precision == "fp64":
         # two streamed passes over complex128 O, w written and read, v / result / o
         b_O, n_pass = 16, 2
         sv_bytes = n_pass * S * P * b_O + 2 * S * 16 + 3 * P * 16
         sv_kernel = "S*v: k_sv_rowdot64 + k_sv_colacc64 (two passes over complex128 O)"
+    elif factored:
+        # two FP64-MFMA passes over the factors: sigma^T B (R x N x (K M + K)) and the
+        # transposed product, R = S K rows; complex weights double the real products
+        R = S * args.K
+        sv_flops = 2 * 2 * R * args.N * (args.K * args.M + args.K) * (1 if real else 2)
+        sv_kernel = "S*v: k_svf_fwd + k_svf_bwd (FP64 MFMA over spins and c*tanh(theta) factors)"
     else:
         # one streamed pass over FP32 (real) or complex64 O, v / result / o in FP64
         b_O, n_pass = (4 if real else 8), 1
         sv_bytes = n_pass * S * P * b_O + 3 * P * (8 if real else 16)
-        sv_kernel = f"S*v: k_sv_wave_mx (one pass over {'FP32' if real else 'complex64'} O)"
+        sv_kernel = f"S*v: k_sv_rows_mx (one pass over {'FP32' if real else 'complex64'} O)"
     sv_calls = sum(s["sv_calls"] for s in stats)
     sv_ms = sum(s["t_sv_ms"] for s in stats)
     sv_avg_ms = sv_ms / max(sv_calls, 1)
-    achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
+    if factored:
+        achieved = sv_flops / (sv_avg_ms * 1e-3) / 1e12 if sv_calls else 0.0
+        roofline = {"bound": "mfma", "kernel": sv_kernel, "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
+                    "traffic": profiled_traffic(args, real, "factored"), "sv_avg_ms": sv_avg_ms, "sv_flops": sv_flops}
+    else:
+        achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
+        roofline = {"bound": "hbm", "kernel": sv_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args, real, "stored"),
+                    "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes}
     energies = np.sort(np.linalg.eigvals(emat / args.N).real).tolist()
     out = {
         "metric": "MC samples/s (whole SR step); converged solves/hour; p50/p99 solve latency",
@@ -196,9 +213,7 @@ def main():
         "minres_iters_per_step": [s["minres_iters"] for s in stats],
         "accept_rate": sum(s["accepts"] for s in stats) / max(sum(s["proposals"] for s in stats), 1),
         "energies_last_step": energies,
-        "roofline": {"bound": "hbm", "kernel": sv_kernel, "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic(args, real), "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes},
+        "roofline": roofline,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, J, W0)
