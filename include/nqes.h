@@ -125,7 +125,10 @@ int nqes_solve(nqes_ctx *ctx, double rtol, uint64_t maxit, double *x, int *flag,
 int nqes_correlations(nqes_ctx *ctx, const double *W, uint64_t seed, int C, int m, int B,
                       double *x, double *z, double *xx, double *zz);
 
-/* Device bytes nqes_sample needs for C chains x m samples (admission check). */
+/* Device bytes nqes_sample needs for C chains x m samples (admission check), an upper
+ * bound that assumes complex weights.  For NQES_PREC_MIXED it counts the factored
+ * S*v state (spins and c*tanh(theta) rows, no O) when N <= 64 and NQES_SV is not
+ * "stored", else the stored O rows. */
 int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision);
 
 /* Library version string. */

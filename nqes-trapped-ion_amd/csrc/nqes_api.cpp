@@ -214,9 +214,25 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
     long long P = (long long)K * (N + M + (long long)N * M);
     long long S = (long long)C * m;
     long long NW = (N + 31) / 32;
-    bool cplx = true;
+    bool cplx = true;  // upper bound: complex weights
     long long b = 0;
-    b += S * o_ld(P, precision, cplx) * (long long)o_elem(precision, cplx);  // O
+    SvfShape sh{};
+    bool svf = false;
+    if (precision == NQES_PREC_MIXED) {
+        const char *e = getenv("NQES_SV");
+        sh = nq_svf_shape(cplx, N, M, K);
+        svf = sh.ok && !(e && strcmp(e, "stored") == 0);
+    }
+    if (svf) {  // factor rows, per-row sums and block partials instead of O
+        long long npair = K > 1 ? (long long)K * (K - 1) / 2 : 1;
+        long long nchunk = (long long)sh.bwd_blocks * npair;
+        b += S * K * (long long)sh.ucols * 8;
+        b += (long long)sh.fwd_cgs * S * K * 16;
+        b += std::max((long long)sh.bwd_blocks * 16, nchunk * 8) * (P + 1);
+        b += (nchunk / 32 + 5) * (P + 1) * 16;
+    } else {
+        b += S * o_ld(P, precision, cplx) * (long long)o_elem(precision, cplx);  // O
+    }
     b += S * (long long)K * K * 16;   // H_loc
     b += S * 16 * 2;                  // E_loc, w
     b += S * K * NW * 4;              // spins

@@ -30,3 +30,7 @@ def test_version_and_sizing_need_no_device():
     b = lib.nqes_required_bytes(32, 64, 2, 4096, 256, 0)
     assert b >= (1 << 20) * 4288 * 16
     assert lib.nqes_required_bytes(0, 64, 2, 4096, 256, 0) == -1
+    # mixed, factored S*v: spins and c*tanh(theta) rows (2^21 rows x 144 complex64),
+    # far below the 2^20 x 4288 FP32 O of the streamed form
+    mb = lib.nqes_required_bytes(32, 64, 2, 4096, 256, 1)
+    assert (1 << 21) * 144 * 8 <= mb < (1 << 20) * 4288 * 4
