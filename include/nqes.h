@@ -131,6 +131,67 @@ int nqes_correlations(nqes_ctx *ctx, const double *W, uint64_t seed, int C, int 
  * "stored", else the stored O rows. */
 int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision);
 
+/*
+ * Solve server: continuous batching of whole solves on one device.
+ *
+ * A solve is the reference's run() (rbm.cc:1173-1197): up to niters SR steps with
+ * the lrate/reglam schedules, chain seeds seed + t*C + c for step t, and its
+ * convergence test; each request carries its own N, M, K, h, J, model and
+ * precision (the reference compiles one binary per N, M, K, J).  Requests queue in
+ * arrival order and are admitted while fewer than max_slots solves run and the
+ * device bytes they need (nqes_required_bytes plus a fixed per-solve allowance)
+ * fit in the budget.  A request that does not fit waits; later requests that fit
+ * may pass it, at most 2*max_slots times, after which admission waits for it.
+ * Every admitted solve runs on its own HIP stream, so the kernels of concurrent
+ * solves overlap on the device.  A solve's result depends only on its request:
+ * the same request gives the same bits alone and in any mix.
+ */
+typedef struct nqes_server nqes_server;
+
+typedef struct {
+    int N, M, K;              /* as nqes_problem_init                          */
+    double h;
+    const double *J;          /* N x N row-major; copied at submit             */
+    int model, precision;
+    const double *W0;         /* initial weights, P complex; copied at submit  */
+    uint64_t seed;            /* chain c of SR step t: pcg64(seed + t*C + c)   */
+    int C, m, B;              /* chains, samples per chain, burn-in sweeps     */
+    int niters;               /* maximum SR steps (-n)                         */
+    double tol;               /* convergence tolerance (--tol)                 */
+    double lrate_init, lrate_decay, lrate_cutoff;
+    double reglam_init, reglam_decay, reglam_cutoff;
+    double rtol;              /* MINRES-QLP relative tolerance                 */
+    uint64_t maxit;           /* MINRES-QLP iteration limit                    */
+} nqes_request;
+
+typedef struct {
+    int status;               /* NQES_OK or the status of the failed call      */
+    int steps;                /* SR steps taken                                */
+    int converged;            /* 1 if the convergence test ended the solve     */
+    int K;
+    double energies[4];       /* ascending E/N of the last step (K entries)    */
+    double t_submit_ms;       /* server clock: ms since nqes_server_create     */
+    double t_admit_ms;
+    double t_done_ms;
+    double t_device_ms;       /* sum of the SR steps' wall times               */
+    int64_t bytes;            /* device bytes reserved for the solve           */
+} nqes_result;
+
+/* budget <= 0: 90 % of the device memory free at creation.  max_slots >= 1.
+ * Returns NULL on failure (no device, bad arguments). */
+nqes_server *nqes_server_create(int device, int max_slots, int64_t budget);
+/* Waits for every queued and running solve, then frees the server. */
+void nqes_server_destroy(nqes_server *srv);
+const char *nqes_server_last_error(const nqes_server *srv);
+/* Validates and queues a request; *id names it.  NQES_EINVAL for bad sizes,
+ * NQES_ENOMEM if it can never fit in the budget. */
+int nqes_server_submit(nqes_server *srv, const nqes_request *req, uint64_t *id);
+/* *done = 1 once solve `id` has finished. */
+int nqes_server_poll(nqes_server *srv, uint64_t id, int *done);
+/* Blocks until solve `id` finishes; copies its final weights (P complex, may be
+ * NULL) and result, then forgets the solve.  Returns the solve's status. */
+int nqes_server_wait(nqes_server *srv, uint64_t id, double *W, nqes_result *res);
+
 /* Library version string. */
 const char *nqes_version(void);
 

@@ -38,6 +38,8 @@ SOURCES = [
     ("kernels_mixed.hip", []),
     ("sv_factored.hip", []),
     ("nqes_api.cpp", EXACT),
+    ("serve.cpp", EXACT),
+    ("eig.cpp", EXACT),
 ]
 
 
@@ -69,8 +71,8 @@ def build(jobs=4, verbose=False):
     if os.path.exists(cli_src):
         exe = os.path.join(BINDIR, "rbm-matfree")
         cmd = ["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-I", os.path.join(REPO, "include"),
-               cli_src, os.path.join(CSRC, "matv5.cpp"), os.path.join(CSRC, "eig.cpp"),
-               "-L", LIBDIR, "-lnqes", "-lz", "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"]
+               cli_src, os.path.join(CSRC, "matv5.cpp"),
+               "-L", LIBDIR, "-lnqes", "-lz", "-pthread", "-Wl,-rpath,$ORIGIN/../lib", "-o", exe + ".tmp"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"cli build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -31,6 +31,7 @@
 #include "../../include/nqes.h"
 #include "eig.hpp"
 #include "matv5.hpp"
+#include "srloop.hpp"
 
 using cplx = std::complex<double>;
 
@@ -305,15 +306,7 @@ std::string format_row(const std::vector<double> &v)
     return out.str();
 }
 
-std::vector<double> sorted_real_eigs(const std::vector<cplx> &A, int K)
-{
-    std::vector<cplx> ev(K);
-    eigk::eig(K, A.data(), ev.data(), nullptr);
-    std::vector<double> r(K);
-    for (int k = 0; k < K; k++) r[k] = ev[k].real();
-    std::sort(r.begin(), r.end());
-    return r;
-}
+using srloop::sorted_real_eigs;
 
 [[noreturn]] void fp_error()
 {
@@ -489,13 +482,13 @@ int main(int argc, char **argv)
         };
         check(nqes_problem_init(ctx, N, M, K, h, J.data(), NQES_MODEL_TRAPPEDION, precision));
 
-        double reglam = reglam0, lrate = lrate0;
-        std::vector<double> best(K, 0.0);
+        srloop::Schedule sched{lrate0, lratef, lrate_cut, reglam0, reglam_decay, reglam_cut};
+        srloop::Convergence conv_test(K);
         std::vector<cplx> emat(K * K);
         for (int iter = 0; iter < niters; iter++) {
             nqes_stats st;
             int rc = nqes_sr_step(ctx, reinterpret_cast<double *>(W.data()), seed + (uint64_t)iter * C, C, m, B,
-                                  std::max(lrate, lrate_cut), std::max(reglam, reglam_cut), rtol, maxit,
+                                  sched.step_lrate(), sched.step_reglam(), rtol, maxit,
                                   reinterpret_cast<double *>(emat.data()), &st);
             if (rc == NQES_ENONFINITE && !throw_fp) {
                 // the reference would carry NaNs on; report and stop cleanly
@@ -507,25 +500,17 @@ int main(int argc, char **argv)
             }
             check(rc);
             for (auto &e : emat) e /= (double)N;
-            reglam *= reglam_decay;
-            lrate *= lratef;
+            sched.advance();
             std::vector<double> evals = sorted_real_eigs(emat, K);
             std::cout << format_row(evals) << std::endl;
             if (want_stats)
                 std::cerr << "step " << iter << ": " << st.t_total_ms << " ms (sample " << st.t_sample_ms
                           << ", solve " << st.t_solve_ms << ", " << st.minres_iters << " MINRES-QLP iterations)"
                           << std::endl;
-            double dn = 0.0, se = 0.0, sb = 0.0;
-            for (int k = 0; k < K; k++) {
-                dn += (evals[k] - best[k]) * (evals[k] - best[k]);
-                se += evals[k];
-                sb += best[k];
-            }
-            if (iter && std::sqrt(dn) < tol) {
+            if (conv_test.update(iter, evals, tol)) {
                 std::cout << "Converged after " << iter + 1 << " steps." << std::endl;
                 break;
             }
-            if (!iter || se < sb) best = evals;
         }
 
         // correlation functions of the final state (rbm.cc:1198-1229)

@@ -210,35 +210,49 @@ const char *nqes_last_error(const nqes_ctx *c) { return c ? c->err.c_str() : "nu
 
 int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
 {
+    // the allocations of nqes_problem_init and nqes_sample, for complex weights
+    // (real weights need less on the mixed path)
     if (N <= 0 || M <= 0 || K <= 0 || C <= 0 || m <= 0) return -1;
-    long long P = (long long)K * (N + M + (long long)N * M);
-    long long S = (long long)C * m;
-    long long NW = (N + 31) / 32;
-    bool cplx = true;  // upper bound: complex weights
-    long long b = 0;
+    if (precision != NQES_PREC_FP64 && precision != NQES_PREC_MIXED) return -1;
+    const bool mixed = precision == NQES_PREC_MIXED, cplx = true;
+    const long long P = (long long)K * (N + M + (long long)N * M);
+    const long long S = (long long)C * m;
+    const long long NW = (N + 31) / 32;
+    const long long Pld = o_ld(P, precision, cplx);
+    long long b = 8LL * N * N + 16 * P + 32LL * K * N * M + 32LL * K * N;  // J, W, tables
+    if (mixed) b += 8LL * K * N * M + 8LL * K * M + 16LL * K * N + 16;
+    if (mixed) b += (long long)C * K * NW * 4 + C * 16LL + (long long)C * K * K * M * 8;  // chain starts
     SvfShape sh{};
     bool svf = false;
-    if (precision == NQES_PREC_MIXED) {
+    if (mixed) {
         const char *e = getenv("NQES_SV");
         sh = nq_svf_shape(cplx, N, M, K);
         svf = sh.ok && !(e && strcmp(e, "stored") == 0);
     }
+    const long long mch = choose_chunks(S, P, 256);
     if (svf) {  // factor rows, per-row sums and block partials instead of O
-        long long npair = K > 1 ? (long long)K * (K - 1) / 2 : 1;
-        long long nchunk = (long long)sh.bwd_blocks * npair;
+        const long long npair = K > 1 ? (long long)K * (K - 1) / 2 : 1;
+        const long long nchunk = (long long)sh.bwd_blocks * npair;
         b += S * K * (long long)sh.ucols * 8;
         b += (long long)sh.fwd_cgs * S * K * 16;
         b += std::max((long long)sh.bwd_blocks * 16, nchunk * 8) * (P + 1);
-        b += (nchunk / 32 + 5) * (P + 1) * 16;
+        b += (nchunk / 32 + 2) * (P + 1) * 16 + 3 * (P + 1) * 16;
     } else {
-        b += S * o_ld(P, precision, cplx) * (long long)o_elem(precision, cplx);  // O
+        b += S * Pld * (long long)o_elem(precision, cplx);  // O
+        if (mixed) b += (mch / 32 + 2) * P * 16;              // moments scratch
     }
-    b += S * (long long)K * K * 16;   // H_loc
-    b += S * 16 * 2;                  // E_loc, w
-    b += S * K * NW * 4;              // spins
-    b += (long long)K * N * M * 48;   // tables
-    b += (long long)C * K * K * M * 8; // chain-start angles
-    b += 3 * 4096LL * P * 16;         // chunk partials (upper bound)
+    b += S * K * K * 16 + S * 16 + S * K * NW * 4 + C * 8LL;  // H_loc, E_loc, spins, accepts
+    b += (svf ? 0 : 3 * mch * P * 16) + 3 * P * 16;           // moment partials and sums
+    b += (long long)choose_chunks(S, (long long)K * K, 64) * K * K * 16 + K * K * 16;
+    b += (long long)choose_chunks(S, 1, 64) * 16 + 16;
+    b += P * 16 + 16 + S * 16 + (P + 1) * 16;  // v, o^T v, w, S*v result
+    if (!mixed) {
+        b += mch * P * 16;
+    } else if (!svf) {
+        const SvShape sv = nq_sv_shape(cplx, Pld / 2);
+        long long nblk = sv.qv ? sv.blocks : std::min<long long>(mch, S);
+        b += nblk * Pld * 16 + nblk * 16 + (nblk / 32 + 2) * (Pld + 1) * 16;
+    }
     return b;
 }
 
@@ -472,9 +486,10 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     HIPCHK(c, c->dspins.ensure((size_t)(S * K * c->NW * 4)), "alloc spins");
     HIPCHK(c, c->dacc.ensure((size_t)C * 8), "alloc accepts");
     c->mchunks = choose_chunks(S, P, 256);
-    HIPCHK(c, c->dmpart_o.ensure((size_t)c->mchunks * P * 16), "alloc partials");
-    HIPCHK(c, c->dmpart_s.ensure((size_t)c->mchunks * P * 16), "alloc partials");
-    HIPCHK(c, c->dmpart_f.ensure((size_t)c->mchunks * P * 16), "alloc partials");
+    const size_t mpart = c->svf ? 16 : (size_t)c->mchunks * P * 16;  // factored: moments from the factors
+    HIPCHK(c, c->dmpart_o.ensure(mpart), "alloc partials");
+    HIPCHK(c, c->dmpart_s.ensure(mpart), "alloc partials");
+    HIPCHK(c, c->dmpart_f.ensure(mpart), "alloc partials");
     HIPCHK(c, c->dmo.ensure((size_t)P * 16), "alloc moments");
     HIPCHK(c, c->dms.ensure((size_t)P * 16), "alloc moments");
     HIPCHK(c, c->dmf.ensure((size_t)P * 16), "alloc moments");
@@ -536,7 +551,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     if (c->prec == NQES_PREC_FP64) {
         c->svchunks = choose_chunks(S, P, 256);
         HIPCHK(c, c->dsvpart.ensure((size_t)c->svchunks * P * 16), "alloc S*v partials");
-    } else {
+    } else if (!c->svf) {
         const long long nvec = c->Pld / (cplx ? 2 : 4);
         const SvShape sh = nq_sv_shape(cplx, nvec);
         long long nblk = sh.qv ? sh.blocks : choose_chunks(S, P, 256);  // one pass / two passes

@@ -1,0 +1,275 @@
+// serve.cpp -- the solve server of nqes.h: continuous batching of whole solves.
+//
+// One solve is the reference's run() loop (rbm.cc:1173-1197) over nqes_sr_step.
+// The server keeps a FIFO of requests and a pool of max_slots worker threads.
+// Admission happens under the server's lock whenever a request arrives or a solve
+// ends: a request is admitted while a slot is free and its device bytes fit in the
+// budget minus what running solves hold.  A request that does not fit blocks the
+// ones behind it unless they fit as well; each pass of that kind is counted, and
+// after 2*max_slots passes the blocked request is waited for.  Admitted solves go
+// to a ready list that the workers drain; each worker makes its own context (own
+// HIP stream, own device buffers) for the solve and frees it at the end, so what a
+// solve holds is returned as soon as it finishes and concurrent solves share the
+// device only through the hardware queues.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <complex>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/nqes.h"
+#include "srloop.hpp"
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+// device bytes a context needs beyond nqes_required_bytes: stream, events, J,
+// tables, moments and the solver's vectors (generous; checked against the
+// measured free memory in the tests)
+constexpr int64_t kPerSolveBytes = 64ll << 20;
+
+struct Job {
+    uint64_t id = 0;
+    nqes_request req{};
+    std::vector<double> J, W;
+    int64_t bytes = 0;
+    int passed = 0;  // times a later request was admitted ahead of this one
+    bool done = false;
+    nqes_result res{};
+    std::string err;  // message of a failed solve
+};
+
+double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
+
+} // namespace
+
+struct nqes_server {
+    int device = 0;
+    int max_slots = 1;
+    int64_t budget = 0;
+    clk::time_point t0;
+    std::string err;
+
+    std::mutex mu;
+    std::condition_variable cv_ready;  // workers: ready list non-empty or stopping
+    std::condition_variable cv_done;   // waiters: some solve finished
+    std::deque<Job *> queue;           // submitted, not admitted (arrival order)
+    std::deque<Job *> ready;           // admitted, not yet taken by a worker
+    std::map<uint64_t, std::unique_ptr<Job>> jobs;
+    int64_t held = 0;                  // bytes of admitted, unfinished solves
+    int active = 0;                    // admitted, unfinished solves
+    uint64_t next_id = 1;
+    bool stopping = false;
+    std::vector<std::thread> workers;
+
+    // under mu
+    void admit()
+    {
+        int64_t avail = budget - held;
+        Job *blocked = nullptr;
+        for (auto it = queue.begin(); it != queue.end() && active < max_slots;) {
+            Job *j = *it;
+            if (j->bytes <= avail && !(blocked && blocked->passed >= 2 * max_slots)) {
+                if (blocked) blocked->passed++;
+                avail -= j->bytes;
+                held += j->bytes;
+                active++;
+                j->res.t_admit_ms = ms_since(t0);
+                ready.push_back(j);
+                it = queue.erase(it);
+                cv_ready.notify_one();
+            } else {
+                if (!blocked) blocked = j;
+                ++it;
+            }
+        }
+    }
+
+    void run(Job *j)
+    {
+        const nqes_request &q = j->req;
+        nqes_result &r = j->res;
+        r.K = q.K;
+        nqes_ctx *ctx = nqes_create(device);
+        if (!ctx) {
+            r.status = NQES_EDEVICE;
+            j->err = "cannot create a context on the device";
+            return;
+        }
+        int rc = nqes_problem_init(ctx, q.N, q.M, q.K, q.h, j->J.data(), q.model, q.precision);
+        srloop::Schedule sched{q.lrate_init,  q.lrate_decay,  q.lrate_cutoff,
+                               q.reglam_init, q.reglam_decay, q.reglam_cutoff};
+        srloop::Convergence conv(q.K);
+        std::vector<std::complex<double>> emat(q.K * q.K);
+        for (int it = 0; rc == NQES_OK && it < q.niters; it++) {
+            nqes_stats st{};
+            rc = nqes_sr_step(ctx, j->W.data(), q.seed + (uint64_t)it * q.C, q.C, q.m, q.B, sched.step_lrate(),
+                              sched.step_reglam(), q.rtol, q.maxit, reinterpret_cast<double *>(emat.data()), &st);
+            if (rc != NQES_OK) break;
+            r.t_device_ms += st.t_total_ms;
+            r.steps = it + 1;
+            for (auto &e : emat) e /= (double)q.N;
+            sched.advance();
+            std::vector<double> ev = srloop::sorted_real_eigs(emat, q.K);
+            for (int k = 0; k < q.K; k++) r.energies[k] = ev[k];
+            if (conv.update(it, ev, q.tol)) {
+                r.converged = 1;
+                break;
+            }
+        }
+        r.status = rc;
+        if (rc != NQES_OK) j->err = nqes_last_error(ctx);
+        nqes_destroy(ctx);
+    }
+
+    void worker()
+    {
+        for (;;) {
+            Job *j = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_ready.wait(lk, [&] { return !ready.empty() || (stopping && queue.empty() && active == 0); });
+                if (ready.empty()) return;
+                j = ready.front();
+                ready.pop_front();
+            }
+            run(j);
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                j->res.t_done_ms = ms_since(t0);
+                j->done = true;
+                held -= j->bytes;
+                active--;
+                admit();
+                if (stopping && queue.empty() && active == 0) cv_ready.notify_all();
+            }
+            cv_done.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+nqes_server *nqes_server_create(int device, int max_slots, int64_t budget)
+{
+    int n = 0;
+    if (max_slots < 1 || max_slots > 64) return nullptr;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    if (budget <= 0) {
+        size_t freeb = 0, totalb = 0;
+        if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) return nullptr;
+        budget = (int64_t)(0.9 * (double)freeb);
+    }
+    auto *s = new nqes_server();
+    s->device = device;
+    s->max_slots = max_slots;
+    s->budget = budget;
+    s->t0 = clk::now();
+    for (int i = 0; i < max_slots; i++)
+        s->workers.emplace_back([s, device] {
+            (void)hipSetDevice(device);
+            s->worker();
+        });
+    return s;
+}
+
+void nqes_server_destroy(nqes_server *s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stopping = true;
+    }
+    s->cv_ready.notify_all();
+    for (auto &t : s->workers) t.join();
+    delete s;
+}
+
+const char *nqes_server_last_error(const nqes_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int nqes_server_submit(nqes_server *s, const nqes_request *q, uint64_t *id)
+{
+    if (!s || !q || !id) return NQES_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto bad = [&](const char *msg) {
+        s->err = msg;
+        return NQES_EINVAL;
+    };
+    if (s->stopping) return bad("server is stopping");
+    if (q->N < 1 || q->N > 256 || q->M < 1 || q->M > 512 || q->K < 1 || q->K > 4) return bad("N, M or K out of range");
+    if (q->M % q->N != 0) return bad("M must be a multiple of N (rbm.cc:148)");
+    if (!q->J || !q->W0) return bad("J and W0 are required");
+    if (q->C < 1 || q->m < 1 || q->B < 0 || q->niters < 0) return bad("C, m, B or niters out of range");
+    if (q->model != NQES_MODEL_TRAPPEDION) return bad("unsupported model");
+    if (q->precision != NQES_PREC_FP64 && q->precision != NQES_PREC_MIXED) return bad("unsupported precision");
+    int64_t need = nqes_required_bytes(q->N, q->M, q->K, q->C, q->m, q->precision);
+    if (need < 0) return bad("bad sizes");
+    need += kPerSolveBytes;
+    if (need > s->budget) {
+        s->err = "request needs " + std::to_string(need) + " device bytes, budget is " + std::to_string(s->budget);
+        return NQES_ENOMEM;
+    }
+    auto j = std::make_unique<Job>();
+    j->id = s->next_id++;
+    j->req = *q;
+    j->J.assign(q->J, q->J + (size_t)q->N * q->N);
+    const size_t P = (size_t)q->K * (q->N + q->M + (size_t)q->N * q->M);
+    j->W.assign(q->W0, q->W0 + 2 * P);
+    j->req.J = nullptr;
+    j->req.W0 = nullptr;
+    j->bytes = need;
+    j->res.bytes = need;
+    j->res.t_submit_ms = ms_since(s->t0);
+    *id = j->id;
+    s->queue.push_back(j.get());
+    s->jobs.emplace(j->id, std::move(j));
+    s->admit();
+    return NQES_OK;
+}
+
+int nqes_server_poll(nqes_server *s, uint64_t id, int *done)
+{
+    if (!s || !done) return NQES_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) {
+        s->err = "unknown request id";
+        return NQES_EINVAL;
+    }
+    *done = it->second->done ? 1 : 0;
+    return NQES_OK;
+}
+
+int nqes_server_wait(nqes_server *s, uint64_t id, double *W, nqes_result *res)
+{
+    if (!s) return NQES_EINVAL;
+    std::unique_ptr<Job> j;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) {
+            s->err = "unknown request id";
+            return NQES_EINVAL;
+        }
+        Job *jp = it->second.get();
+        s->cv_done.wait(lk, [&] { return jp->done; });
+        j = std::move(it->second);
+        s->jobs.erase(it);
+        if (j->res.status != NQES_OK) s->err = j->err;
+    }
+    if (W) std::memcpy(W, j->W.data(), j->W.size() * sizeof(double));
+    if (res) *res = j->res;
+    return j->res.status;
+}
+
+} // extern "C"

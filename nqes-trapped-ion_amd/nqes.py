@@ -48,6 +48,45 @@ class Stats(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class Request(ctypes.Structure):
+    """nqes_request: one whole solve (run() of rbm.cc:1173-1197)."""
+    _fields_ = [
+        ("N", ctypes.c_int), ("M", ctypes.c_int), ("K", ctypes.c_int),
+        ("h", ctypes.c_double),
+        ("J", ctypes.c_void_p),
+        ("model", ctypes.c_int), ("precision", ctypes.c_int),
+        ("W0", ctypes.c_void_p),
+        ("seed", ctypes.c_uint64),
+        ("C", ctypes.c_int), ("m", ctypes.c_int), ("B", ctypes.c_int),
+        ("niters", ctypes.c_int),
+        ("tol", ctypes.c_double),
+        ("lrate_init", ctypes.c_double), ("lrate_decay", ctypes.c_double), ("lrate_cutoff", ctypes.c_double),
+        ("reglam_init", ctypes.c_double), ("reglam_decay", ctypes.c_double), ("reglam_cutoff", ctypes.c_double),
+        ("rtol", ctypes.c_double),
+        ("maxit", ctypes.c_uint64),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("status", ctypes.c_int),
+        ("steps", ctypes.c_int),
+        ("converged", ctypes.c_int),
+        ("K", ctypes.c_int),
+        ("energies", ctypes.c_double * 4),
+        ("t_submit_ms", ctypes.c_double),
+        ("t_admit_ms", ctypes.c_double),
+        ("t_done_ms", ctypes.c_double),
+        ("t_device_ms", ctypes.c_double),
+        ("bytes", ctypes.c_int64),
+    ]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "energies"}
+        d["energies"] = list(self.energies)[: self.K]
+        return d
+
+
 _lib = None
 
 # every symbol include/nqes.h declares
@@ -55,6 +94,8 @@ EXPORTS = [
     "nqes_create", "nqes_destroy", "nqes_last_error", "nqes_problem_init", "nqes_sr_step",
     "nqes_sample", "nqes_get_samples", "nqes_get_O", "nqes_get_moments", "nqes_apply_S",
     "nqes_solve", "nqes_correlations", "nqes_required_bytes", "nqes_version",
+    "nqes_server_create", "nqes_server_destroy", "nqes_server_last_error", "nqes_server_submit",
+    "nqes_server_poll", "nqes_server_wait",
 ]
 
 
@@ -95,6 +136,17 @@ def load_library(path=LIB_PATH):
     L.nqes_required_bytes.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int]
     L.nqes_version.restype = ctypes.c_char_p
     L.nqes_version.argtypes = []
+    L.nqes_server_create.restype = _vp
+    L.nqes_server_create.argtypes = [c_int, c_int, c_i64]
+    L.nqes_server_destroy.argtypes = [_vp]
+    L.nqes_server_last_error.restype = ctypes.c_char_p
+    L.nqes_server_last_error.argtypes = [_vp]
+    L.nqes_server_submit.restype = c_int
+    L.nqes_server_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(c_u64)]
+    L.nqes_server_poll.restype = c_int
+    L.nqes_server_poll.argtypes = [_vp, c_u64, ctypes.POINTER(c_int)]
+    L.nqes_server_wait.restype = c_int
+    L.nqes_server_wait.argtypes = [_vp, c_u64, _vp, ctypes.POINTER(Result)]
     _lib = L
     return L
 
@@ -256,3 +308,78 @@ def run(solver, W, seed, C, m, B, niters, tol=1e-6, lrate0=0.1, lratef=0.99, lra
         if not it or ev.sum() < best.sum():
             best = ev
     return hist, None, emat
+
+
+class Server:
+    """Continuous batching of whole solves on one device (nqes_server_*).
+
+    ``submit`` copies the request (J, initial weights) and returns an id; solves run
+    concurrently, each on its own HIP stream, up to ``max_slots`` at a time and
+    within ``budget`` device bytes (default: 90 % of the free memory).  ``wait``
+    returns (final weights, result dict) and raises on a failed solve.
+    """
+
+    def __init__(self, max_slots=4, budget=0, device=0):
+        self.lib = load_library()
+        self.srv = self.lib.nqes_server_create(int(device), int(max_slots), int(budget))
+        if not self.srv:
+            raise NqesError(f"nqes_server_create({device}, {max_slots}) failed: no HIP device or bad arguments")
+        self._P = {}
+
+    def _err(self):
+        return self.lib.nqes_server_last_error(self.srv).decode()
+
+    def submit(self, N, M, K, h, J, W0, seed, C, m, B, niters, precision=PREC_MIXED, model=MODEL_TRAPPEDION,
+               tol=1e-6, lrate0=0.1, lratef=0.99, lrate_cutoff=0.01, reglam0=0.01, reglam_decay=0.9,
+               reglam_cutoff=0.001, rtol=1e-10, maxit=2**64 - 1):
+        J = np.ascontiguousarray(J, dtype=np.float64)
+        W0 = np.ascontiguousarray(W0, dtype=np.complex128)
+        if J.shape != (N, N) or W0.shape != (nparams(N, M, K),):
+            raise ValueError("J must be N x N and W0 of length P")
+        q = Request(int(N), int(M), int(K), float(h), J.ctypes.data, int(model), int(precision), W0.ctypes.data,
+                    int(seed) % (1 << 64), int(C), int(m), int(B), int(niters), float(tol), float(lrate0),
+                    float(lratef), float(lrate_cutoff), float(reglam0), float(reglam_decay), float(reglam_cutoff),
+                    float(rtol), int(maxit))
+        rid = ctypes.c_uint64(0)
+        st = self.lib.nqes_server_submit(self.srv, ctypes.byref(q), ctypes.byref(rid))
+        if st != 0:
+            raise NqesError(f"nqes_server_submit status {st}: {self._err()}")
+        self._P[rid.value] = nparams(N, M, K)
+        return rid.value
+
+    def done(self, rid):
+        d = ctypes.c_int(0)
+        st = self.lib.nqes_server_poll(self.srv, int(rid), ctypes.byref(d))
+        if st != 0:
+            raise NqesError(f"nqes_server_poll status {st}: {self._err()}")
+        return bool(d.value)
+
+    def wait(self, rid, raise_on_error=True):
+        """(final weights, result dict).  A failed solve raises, or with
+        raise_on_error=False returns its result with the status and message."""
+        W = np.zeros(self._P.pop(rid), dtype=np.complex128)
+        res = Result()
+        st = self.lib.nqes_server_wait(self.srv, int(rid), W.ctypes.data, ctypes.byref(res))
+        if st != 0 and raise_on_error:
+            raise NqesError(f"solve {rid} failed with status {st}: {self._err()}")
+        d = res.as_dict()
+        if st != 0:
+            d["error"] = self._err()
+        return W, d
+
+    def close(self):
+        if self.srv:
+            self.lib.nqes_server_destroy(self.srv)
+            self.srv = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,131 @@
+"""Solve server: continuous batching of whole solves on one device (nqes_server_*).
+
+* A request gives the same final weights, energies and step count, bit for bit, when
+  it runs alone (one slot) and when it runs in a mix of concurrent solves that arrive
+  while others are on the device (different sizes, K, precisions and real/complex
+  weights).
+* The server's solve is the reference's run() loop (rbm.cc:1173-1197): its final
+  weights equal, bit for bit, those of the Python mirror's loop (nqes.run) over a
+  Solver on the same request, and its energies agree to 1e-12 (numpy's eigvals vs
+  the library's QR).
+* Admission by device bytes: with a budget that holds one solve at a time, solves
+  run one after another in arrival order; a request larger than the budget is
+  refused at submit; invalid sizes are refused.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import nqes
+from helpers import load_J, random_weights, synthetic_J
+
+pytestmark = pytest.mark.gpu
+
+PER_SOLVE = 64 << 20  # the server's fixed per-solve allowance
+
+# (N, M, K, precision, complex weights, C, m, B, niters, seed)
+MIX = [
+    (8, 16, 2, nqes.PREC_MIXED, False, 64, 32, 16, 6, 11),
+    (12, 24, 3, nqes.PREC_FP64, True, 32, 16, 8, 4, 12),
+    (16, 16, 1, nqes.PREC_MIXED, True, 128, 32, 16, 5, 13),
+    (32, 64, 2, nqes.PREC_MIXED, False, 256, 64, 16, 3, 14),
+    (10, 10, 4, nqes.PREC_FP64, False, 32, 16, 8, 3, 15),
+    (20, 40, 3, nqes.PREC_MIXED, True, 64, 32, 8, 4, 16),
+]
+
+SCHED = dict(tol=1e-6, lrate0=0.1, lratef=0.99, lrate_cutoff=0.01, reglam0=40.0, reglam_decay=0.9,
+             reglam_cutoff=1.0, rtol=1e-9)
+
+
+def request(case):
+    N, M, K, prec, cplx, C, m, B, niters, seed = case
+    J = load_J(N) if N in (8, 16, 32) else synthetic_J(N)
+    W0 = random_weights(N, M, K, seed, scale=0.05 if cplx else 0.01, complex_=cplx)
+    h = 0.3 + 0.05 * seed % 1.0
+    return dict(N=N, M=M, K=K, h=h, J=J, W0=W0, seed=1000 * seed, C=C, m=m, B=B, niters=niters, precision=prec)
+
+
+def run_alone(q):
+    with nqes.Server(max_slots=1) as srv:
+        rid = srv.submit(**q, **SCHED)
+        return srv.wait(rid)
+
+
+@pytest.fixture(scope="module")
+def alone():
+    out = []
+    for case in MIX:
+        print("alone:", case, flush=True)
+        out.append(run_alone(request(case)))
+    return out
+
+
+def test_same_bits_alone_and_in_mix(alone):
+    print("mix of", len(MIX), "solves, 4 slots, staggered arrivals", flush=True)
+    with nqes.Server(max_slots=4) as srv:
+        ids = []
+        for case in MIX:
+            ids.append(srv.submit(**request(case), **SCHED))
+            time.sleep(0.03)
+        res = [srv.wait(i) for i in ids]
+    overlap = 0
+    for case, (Wa, ra), (Wm, rm) in zip(MIX, alone, res):
+        assert np.array_equal(Wa, Wm), case
+        assert ra["energies"] == rm["energies"], case
+        assert ra["steps"] == rm["steps"] and ra["converged"] == rm["converged"], case
+        assert rm["status"] == 0 and rm["steps"] >= 1
+    # later arrivals were admitted while earlier solves were still running
+    for a in range(len(res)):
+        for b in range(a + 1, len(res)):
+            ra, rb = res[a][1], res[b][1]
+            if rb["t_admit_ms"] < ra["t_done_ms"] and ra["t_admit_ms"] < rb["t_done_ms"]:
+                overlap += 1
+    assert overlap >= 3
+
+
+@pytest.mark.parametrize("idx", [0, 1, 5])
+def test_server_equals_python_loop(alone, idx):
+    case = MIX[idx]
+    q = request(case)
+    print("python loop:", case, flush=True)
+    W = q["W0"].copy()
+    with nqes.Solver(q["N"], q["M"], q["K"], q["h"], q["J"], precision=q["precision"]) as s:
+        hist, conv, _ = nqes.run(s, W, q["seed"], q["C"], q["m"], q["B"], q["niters"], tol=SCHED["tol"],
+                                 lrate0=SCHED["lrate0"], lratef=SCHED["lratef"],
+                                 lrate_cutoff=SCHED["lrate_cutoff"], reglam0=SCHED["reglam0"],
+                                 reglam_decay=SCHED["reglam_decay"], reglam_cutoff=SCHED["reglam_cutoff"],
+                                 rtol=SCHED["rtol"])
+    Ws, rs = alone[idx]
+    assert np.array_equal(W, Ws)
+    assert rs["steps"] == len(hist)
+    assert rs["converged"] == (conv is not None)
+    np.testing.assert_allclose(rs["energies"], hist[-1], rtol=1e-12, atol=1e-12)
+
+
+def test_admission_by_device_bytes(alone):
+    lib = nqes.load_library()
+    cases = [MIX[0], MIX[2], MIX[0]]  # similar footprints, so a budget can hold one but never two
+    need = [lib.nqes_required_bytes(c[0], c[1], c[2], c[5], c[6], c[3]) + PER_SOLVE for c in cases]
+    budget = int(max(need) * 1.5)  # any two together exceed it
+    assert all(a + b > budget for a in need for b in need)
+    print("budget", budget, "needs", need, flush=True)
+    with nqes.Server(max_slots=4, budget=budget) as srv:
+        ids = [srv.submit(**request(c), **SCHED) for c in cases]
+        res = [srv.wait(i) for i in ids]
+    for c, (W, r) in zip(cases, res):
+        Wa, ra = alone[MIX.index(c)]
+        assert np.array_equal(W, Wa) and r["energies"] == ra["energies"]
+        assert r["bytes"] == need[cases.index(c)]
+    for a, b in zip(res, res[1:]):  # one at a time, arrival order
+        assert b[1]["t_admit_ms"] >= a[1]["t_done_ms"]
+
+
+def test_refusals():
+    q = request(MIX[0])
+    with nqes.Server(max_slots=1, budget=PER_SOLVE) as srv:
+        with pytest.raises(nqes.NqesError, match="status 2"):
+            srv.submit(**q, **SCHED)
+        bad = dict(q, M=12, W0=random_weights(8, 12, 2, 1))  # M % N != 0
+        with pytest.raises(nqes.NqesError, match="status 1"):
+            srv.submit(**bad, **SCHED)
