@@ -37,6 +37,7 @@ SOURCES = [
     ("kernels_fp64.hip", EXACT),
     ("kernels_mixed.hip", []),
     ("sv_factored.hip", []),
+    ("sv_q8.hip", []),
     ("nqes_api.cpp", EXACT),
     ("serve.cpp", EXACT),
     ("eig.cpp", EXACT),

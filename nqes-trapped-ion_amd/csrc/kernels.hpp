@@ -81,6 +81,11 @@ struct SvfArgs {
     const void *E;             // S complex double: local energies (moments)
     int wsel;                  // backward-pass weights: 0 w, 1 ones, 2 E
     int ncg;                   // forward column groups (wrow holds ncg x S*K partial sums)
+    const void *Uq;            // int8 form: quantised factors (16 x 16 tiles of digit words)
+    const double *escale;      // int8 form: per-row factor scale 2^(e_r - 26)
+    void *vq;                  // int8 form: v operands, column scales and biases
+    void *q8w;                 // int8 form: backward block scales and weight digits
+    void *q8m;                 // int8 form: spin masks of the backward A operand
 };
 // ok = 0: the factored form does not apply (N > 64).  The forward pass splits the
 // Ucols/16 column tiles into fwd_cgs groups of fwd_cgt (B of a group fits in LDS); the
@@ -89,6 +94,18 @@ struct SvfShape {
     int ucols, ok, bwd_waves, bwd_blocks, bwd_cgs, fwd_blocks, fwd_cgt, fwd_cgs;
 };
 SvfShape nq_svf_shape(bool cplx, int N, int M, int K);
+hipError_t nq_launch_svf_combine(bool cplx, const SvfArgs &a, hipStream_t st);
+// int8 form of the factored S*v (real weights, N <= 64): factors quantised once per
+// sample set (nq_q8_pack), S*v on int8 MFMA (nq_q8_apply)
+struct Q8Shape {
+    int ok, ucols, nsp, fwd_cgt, fwd_cgs, fwd_blocks, bwd_waves, bwd_cgs, bwd_blocks;
+    size_t fwd_lds, vq_bytes, w_bytes, m_bytes;
+    long long rpad;  // rows of Uq / escale (multiple of 64)
+    int gblk_ints;
+};
+Q8Shape nq_q8_shape(int N, int M, int K, long long R);
+hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
+hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st);
 hipError_t nq_launch_svf(bool cplx, const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st);
 // moments from the factors: so = sum_s O_s, ss = sum_s |O_s|^2, sf = sum_s E_s conj(O_s)
 // (P complex double each); part: >= max(1, K(K-1)/2) * bwd_blocks * (P + 1) D,

@@ -101,8 +101,10 @@ struct nqes_ctx {
     DevBuf dmpart_o, dmpart_s, dmpart_f, dmo, dms, dmf, dhpart, dhsum, depart, desum;
     DevBuf dv, dov, dw, dsvpart, dsvwpart, dsvout, dsvscratch, dmscratch;
     DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
-    bool svf = false;
+    DevBuf dUq, descale, dvq, dq8w, dq8m;              // its int8 form (real weights)
+    bool svf = false, q8 = false;
     SvfShape svf_shape{};
+    Q8Shape q8_shape{};
     int mchunks = 1, svchunks = 1;
     std::vector<cd> o, skk, f, emat;
     cd eloc{0, 0};
@@ -237,6 +239,9 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
         b += (long long)sh.fwd_cgs * S * K * 16;
         b += std::max((long long)sh.bwd_blocks * 16, nchunk * 8) * (P + 1);
         b += (nchunk / 32 + 2) * (P + 1) * 16 + 3 * (P + 1) * 16;
+        const Q8Shape q8 = nq_q8_shape(N, M, K, S * K);  // int8 form, used for real weights
+        if (q8.ok)
+            b += q8.rpad * (long long)q8.ucols * 4 + q8.rpad * 8 + (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes);
     } else {
         b += S * Pld * (long long)o_elem(precision, cplx);  // O
         if (mixed) b += (mch / 32 + 2) * P * 16;              // moments scratch
@@ -362,6 +367,12 @@ static SvfArgs svf_args(nqes_ctx *c, long long S)
     f.part = c->dsvfpart.p;
     f.E = c->dEs.p;
     f.wsel = 0;
+    f.ncg = 1;
+    f.Uq = c->dUq.p;
+    f.escale = static_cast<const double *>(c->descale.p);
+    f.vq = c->dvq.p;
+    f.q8w = c->dq8w.p;
+    f.q8m = c->dq8m.p;
     return f;
 }
 
@@ -425,6 +436,7 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
         HIPCHK(c, nq_launch_svf_moments(cplx, f, c->svf_shape, c->dsvfpart.p, c->dsvfscratch.p, c->dsvftmp.p, c->dmo.p,
                                         c->dms.p, c->dmf.p, c->stream),
                "factored moments kernels");
+        if (c->q8) HIPCHK(c, nq_q8_pack(f, c->q8_shape, c->stream), "factor quantisation kernel");
         return NQES_OK;
     }
     HIPCHK(c, c->dmscratch.ensure((size_t)(c->mchunks / 32 + 2) * c->P * 16), "alloc moments scratch");
@@ -456,20 +468,37 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     }
     c->cplx = cplx;
     c->Pld = o_ld(P, c->prec, cplx);
-    // factored S*v (mixed path) unless NQES_SV=stored or the shape does not apply
-    c->svf = false;
+    // factored S*v (mixed path) unless NQES_SV=stored or the shape does not apply; for
+    // real weights its int8 form unless NQES_SV=fp64mfma
+    c->svf = c->q8 = false;
     if (c->prec == NQES_PREC_MIXED) {
         const char *e = getenv("NQES_SV");
         c->svf_shape = nq_svf_shape(cplx, c->N, c->M, K);
         c->svf = c->svf_shape.ok && !(e && strcmp(e, "stored") == 0);
+        if (c->svf && !cplx) {
+            c->q8_shape = nq_q8_shape(c->N, c->M, K, S * K);
+            c->q8 = c->q8_shape.ok && !(e && strcmp(e, "fp64mfma") == 0);
+        }
     }
     if (c->svf) {
         const size_t tsz = cplx ? 8 : 4;
-        const long long nb = c->svf_shape.bwd_blocks;
+        long long nb = c->svf_shape.bwd_blocks;
         const long long npair = K > 1 ? (long long)K * (K - 1) / 2 : 1;
-        const long long nchunk = nb * npair;
+        long long nchunk = nb * npair;
+        long long cgs = c->svf_shape.fwd_cgs;
+        if (c->q8) {
+            nb = std::max(nb, (long long)c->q8_shape.bwd_blocks);
+            nchunk = std::max(nchunk, (long long)c->q8_shape.bwd_blocks);
+            cgs = std::max(cgs, (long long)c->q8_shape.fwd_cgs);
+            const long long rp = c->q8_shape.rpad;
+            HIPCHK(c, c->dUq.ensure((size_t)rp * c->q8_shape.ucols * 4), "alloc quantised factors");
+            HIPCHK(c, c->descale.ensure((size_t)rp * 8), "alloc factor scales");
+            HIPCHK(c, c->dvq.ensure(c->q8_shape.vq_bytes), "alloc S*v operands");
+            HIPCHK(c, c->dq8w.ensure(c->q8_shape.w_bytes), "alloc S*v weight digits");
+            HIPCHK(c, c->dq8m.ensure(c->q8_shape.m_bytes), "alloc spin masks");
+        }
         HIPCHK(c, c->dU.ensure((size_t)S * K * c->svf_shape.ucols * tsz), "alloc S*v factors");
-        HIPCHK(c, c->dwrow.ensure((size_t)c->svf_shape.fwd_cgs * S * K * 16), "alloc S*v rows");
+        HIPCHK(c, c->dwrow.ensure((size_t)cgs * S * K * 16), "alloc S*v rows");
         HIPCHK(c, c->dsvfpart.ensure((size_t)std::max(nb * 16, nchunk * 8) * (P + 1)), "alloc S*v partials");
         HIPCHK(c, c->dsvfscratch.ensure((size_t)(nchunk / 32 + 2) * (P + 1) * 16), "alloc S*v scratch");
         HIPCHK(c, c->dsvftmp.ensure((size_t)3 * (P + 1) * 16), "alloc moments");
@@ -680,7 +709,10 @@ static int sv_mixed_device(nqes_ctx *c, const void *v, const void *ov, size_t ds
     HIPCHK(c, hipMemcpyAsync(c->dv.p, v, dsize * P, hipMemcpyHostToDevice, c->stream), "copy v");
     HIPCHK(c, hipMemcpyAsync(c->dov.p, ov, dsize, hipMemcpyHostToDevice, c->stream), "copy ov");
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
-    if (c->svf) {
+    if (c->q8) {
+        const SvfArgs f = svf_args(c, c->S);
+        HIPCHK(c, nq_q8_apply(f, c->q8_shape, c->dsvfscratch.p, c->dsvout.p, c->stream), "int8 factored S*v kernels");
+    } else if (c->svf) {
         const SvfArgs f = svf_args(c, c->S);
         HIPCHK(c, nq_launch_svf(c->cplx, f, c->svf_shape, c->dsvfscratch.p, c->dsvout.p, c->stream),
                "factored S*v kernels");

@@ -502,6 +502,13 @@ hipError_t nq_launch_svf_moments(bool cplx, const SvfArgs &a, const SvfShape &sh
                 : launch_svf_moments<false>(a, sh, part, scratch, tmp, so, ss, sf, st);
 }
 
+hipError_t nq_launch_svf_combine(bool cplx, const SvfArgs &a, hipStream_t st)
+{
+    if (cplx) hipLaunchKernelGGL(k_svf_combine<cd>, dim3(1024), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(k_svf_combine<double>, dim3(1024), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t nq_launch_svf(bool cplx, const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st)
 {
     return cplx ? launch_svf<true>(a, sh, scratch, out, st) : launch_svf<false>(a, sh, scratch, out, st);

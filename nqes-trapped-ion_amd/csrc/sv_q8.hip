@@ -1,0 +1,617 @@
+// sv_q8.hip -- S*v of the mixed path (real weights) from the factors on int8 MFMA.
+//
+// The factored S*v (rbm.cc:661-684 with O_s written as sum_kk c sigma^kk t, see
+// sv_factored.hip) is two GEMMs with the +-1 spin matrix as one operand.  Here both
+// run on v_mfma_i32_16x16x{32,64}_i8, whose integer results are exact.  Numbers are
+// split into balanced base-256 digits (int8, least significant first):
+//   factors   per row r a power of two 2^e_r > max_col |U[r][col]|, and
+//             X = rint(U 2^(30 - e_r)) (|X| <= 2^30), four digits in the bytes of one
+//             word -- the quantised factor U~ = X 2^(e_r - 30) differs from U by at
+//             most 2^-31 of the row's largest entry; done once per SR step;
+//   v         per column col of B (spins i x columns) a power of two 2^g > max_i |B|,
+//             Q = rint(B 2^(46 - g)), six digits;
+//   weights   per backward block a power of two 2^G > max |w_s(r) 2^(e_r - 30)|,
+//             Omega = rint(w 2^(e_r - 30) 2^(46 - G)), six digits.
+// The forward pass forms Y = Sigma Q (six MFMAs per tile, exact) and
+//   w_r = 2^(e_r - 30) sum_col X (Y 2^(g - 46) + bias) in FP64.
+// The backward pass writes sigma = 1 - 2 b (b = 1 for spin down) and forms
+//   sum_r sigma_r[i] X_r[col] Omega_r = T[col] - 2 sum_r b_r[i] X_r[col] Omega_r,
+// the second sum digit by digit on MFMA (A operand = weight digits masked by b, 24
+// MFMAs per tile pair), T (also the b-block column sums) with v_dot4_i32_i8; products
+// of equal digit weight 256^s share an int32 accumulator (exact), which goes to FP64
+// once per block.  The operator is the exact product of U~ with v and w quantised to
+// 2^-47 of their scale (w per block of rows): linear and symmetric to ~1e-13.
+//
+// Layout of the quantised factors Uq: 16 x 16 tiles (row tile rt, column tile ct),
+// tile (rt, ct) at word (rt NCT + ct) 256; lane l of a tile holds words 4 l + j for
+// rows 16 rt + 4 (l/16) + j (j < 4) at column 16 ct + l%16.  That is the result
+// layout of the 16x16 MFMA (register j of lane l = row 4 (l/16) + j) for the forward
+// pass, and, over four row tiles, the B operand of the 16x16x64 MFMA for the
+// backward pass after a 4 x 4 byte transpose per row tile (lane l byte 4 q + j <->
+// row 16 q + 4 (l/16) + j).  Rows are padded to a multiple of 64 with zeros.
+//
+// MFMA int8 layouts (measured): A operand lane l byte e = A[l%16][k(l, e)], B operand
+// lane l byte e = B[k(l, e)][l%16] for the same k(l, e) (any bijection onto the K
+// dimension), result register e of lane l = D[4 (l/16) + e][l%16].
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#include "cplx64.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int QU = 4;            // factor digits
+constexpr int QV = 6;            // v digits
+constexpr int QW = 6;            // weight digits
+constexpr int NS = QU + QW - 1;  // digit weights 256^s of the backward products
+constexpr int XBITS = 30;        // |X| <= 2^30
+constexpr int VBITS = 46;        // |Q| <= 2^46
+constexpr int WBITS = 46;        // |Omega| <= 2^46
+constexpr int MAX_BLOCK_CHUNKS = 256;  // 16384 rows: |int32 sums| <= 4 2^14 2^14 = 2^30
+
+__device__ inline long long bal8l(long long x) { return (long long)(int8_t)(uint8_t)(x & 255); }
+
+// four 0/1 bits -> four 0/1 bytes
+__device__ inline uint32_t expand4(uint32_t b) { return ((b & 15u) * 0x00204081u) & 0x01010101u; }
+
+// X of a factor word (balanced digits d_t in byte t): the raw word minus the borrows
+// of the negative digits below the top one
+__device__ inline int decode_x(uint32_t wd) { return (int)(wd - (((wd >> 7) & 0x010101u) << 8)); }
+
+__device__ inline double xor_sum16(double x)
+{
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) x += __shfl_xor(x, off);
+    return x;
+}
+
+// Pack: U (FP32, [R][UC] row-major) -> Uq tiles and escale[r] = 2^(e_r - 30).  One
+// wavefront per 16-row tile, rows [R, Rp) written as zeros.
+__global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, long long ntile, int UC, uint32_t *Uq,
+                                                 double *escale)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = lane >> 4, lo = lane & 15;
+    const int NCT = UC / 16;
+    for (long long rt = (long long)blockIdx.x * 4 + wv; rt < ntile; rt += (long long)gridDim.x * 4) {
+        float mx[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const long long r = rt * 16 + 4 * g + j;
+                if (r < R) mx[j] = fmaxf(mx[j], fabsf(U[r * UC + 16 * ct + lo]));
+            }
+        double sc[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) mx[j] = fmaxf(mx[j], __shfl_xor(mx[j], off));
+            int e = 0;
+            if (mx[j] > 0.f) (void)frexpf(mx[j], &e);  // mx < 2^e
+            sc[j] = ldexp(1.0, XBITS - e);
+            const long long r = rt * 16 + 4 * g + j;
+            if (lo == 0) escale[r] = (r < R && mx[j] > 0.f) ? ldexp(1.0, e - XBITS) : 0.0;
+        }
+        for (int ct = 0; ct < NCT; ct++) {
+            uint32_t wd[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const long long r = rt * 16 + 4 * g + j;
+                const float u = (r < R) ? U[r * UC + 16 * ct + lo] : 0.f;
+                long long x = (long long)rint((double)u * sc[j]);
+                uint32_t word = 0;
+#pragma unroll
+                for (int t = 0; t < QU; t++) {
+                    const long long d = (t < QU - 1) ? bal8l(x) : x;
+                    word |= ((uint32_t)d & 255u) << (8 * t);
+                    x = (x - d) >> 8;
+                }
+                wd[j] = word;
+            }
+            uint4 o;
+            o.x = wd[0];
+            o.y = wd[1];
+            o.z = wd[2];
+            o.w = wd[3];
+            *(uint4 *)(Uq + ((rt * NCT + ct) * 64 + lane) * 4) = o;
+        }
+    }
+}
+
+// v -> forward B operands.  Thread per column col < UC: B[i][col] = v_w[k][i][j]
+// (col = k M + j < K M), v_a[k][i] (col = K M + k), else 0; its scale and six digits
+// per spin i (u = 0 least significant).  vfrag[(ct QV + u) 64 + lane] holds BPL bytes: spin k(l, e) = BPL (l/16) + e,
+// column 16 ct + l%16.  colscale[col] = 2^(g - 46); bias[col] = v_b[k][j] or 0.
+__global__ void k_q8_prep_v(SvfArgs a, int nsp, uint8_t *vfrag, double *colscale, double *bias)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int UC = a.Ucols;
+    if (col >= UC) return;
+    const int N = a.N, M = a.M, K = a.K;
+    const long long KM = (long long)K * M;
+    const double *v = (const double *)a.v;
+    auto Bval = [&](int i) -> double {
+        if (col < KM) return v[(long long)(col / M) * a.L + N + M + (long long)i * M + col % M];
+        if (col < KM + K) return v[(long long)(col - KM) * a.L + i];
+        return 0.0;
+    };
+    double mx = 0.0;
+    for (int i = 0; i < N; i++) mx = fmax(mx, fabs(Bval(i)));
+    int g = 0;
+    if (mx > 0.0) (void)frexp(mx, &g);
+    const double sc = ldexp(1.0, VBITS - g);
+    colscale[col] = mx > 0.0 ? ldexp(1.0, g - VBITS) : 0.0;
+    bias[col] = col < KM ? v[(long long)(col / M) * a.L + N + col % M] : 0.0;
+    const int bpl = nsp / 4;
+    const int ct = col / 16, lo = col % 16;
+    for (int i = 0; i < nsp; i++) {
+        long long q = (i < N) ? (long long)rint(Bval(i) * sc) : 0;
+        int dg[QV];  // least significant first
+#pragma unroll
+        for (int u = 0; u < QV; u++) {
+            const long long d = (u < QV - 1) ? bal8l(q) : q;
+            dg[u] = (int)d;
+            q = (q - d) >> 8;
+        }
+        const int lane = (i / bpl) * 16 + lo, e = i % bpl;
+#pragma unroll
+        for (int u = 0; u < QV; u++) vfrag[(((long long)ct * QV + u) * 64 + lane) * bpl + e] = (uint8_t)(dg[u] & 255);
+    }
+}
+
+template <int NSP> struct FwdOp;
+template <> struct FwdOp<32> {
+    typedef long long T;
+    static __device__ inline v4i mfma(T a, T b) { return __builtin_amdgcn_mfma_i32_16x16x32_i8(a, b, v4i{0, 0, 0, 0}, 0, 0, 0); }
+};
+template <> struct FwdOp<64> {
+    typedef v4i T;
+    static __device__ inline v4i mfma(T a, T b) { return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, v4i{0, 0, 0, 0}, 0, 0, 0); }
+};
+
+// +-1 bytes (0 for spins >= N or rows >= R) of spins [first, first + 4) of a row
+__device__ inline uint32_t spin_bytes4(uint32_t bits, uint32_t valid)
+{
+    const uint32_t one = expand4(valid), neg = expand4(bits & valid);
+    return one ^ (neg * 0xFEu);
+}
+
+// Forward pass.  Block (b, g) handles column tiles [g cgt, ...) (their v operands in
+// LDS) and loops with 4 wavefronts over 16-row tiles.  Lane l: A operand = spins
+// BPL (l/16) .. + BPL of row 16 rt + l%16; result rows 16 rt + 4 (l/16) + e at column
+// l%16 of each tile, matching the lane's four Uq words.  Output wrow[g R + r].
+template <int NSP>
+__global__ __launch_bounds__(256) void k_q8_fwd(SvfArgs a, int cgt)
+{
+    using Op = FwdOp<NSP>;
+    using AT = typename Op::T;
+    constexpr int BPL = NSP / 4;
+    extern __shared__ __align__(16) unsigned char smem_q[];
+    const int N = a.N, K = a.K, NW = a.NW, UC = a.Ucols;
+    const int NCT = UC / 16;
+    const int ct0 = blockIdx.y * cgt;
+    const int nct = (NCT - ct0 < cgt) ? NCT - ct0 : cgt;
+    uint8_t *fr = smem_q;
+    double *cs = (double *)(smem_q + (size_t)cgt * QV * 64 * BPL);
+    double *bs = cs + 16 * cgt;
+    {
+        const uint4 *src = (const uint4 *)((const uint8_t *)a.vq + (size_t)ct0 * QV * 64 * BPL);
+        uint4 *dst = (uint4 *)fr;
+        const int n16 = nct * QV * 64 * BPL / 16;
+        for (int q = threadIdx.x; q < n16; q += blockDim.x) dst[q] = src[q];
+        const double *gcs = (const double *)((const uint8_t *)a.vq + (size_t)NCT * QV * 64 * 16);
+        const double *gbs = gcs + UC;
+        for (int q = threadIdx.x; q < 16 * nct; q += blockDim.x) {
+            cs[q] = gcs[16 * ct0 + q];
+            bs[q] = gbs[16 * ct0 + q];
+        }
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = lane >> 4, lo = lane & 15;
+    const long long R = a.S * K;
+    const long long ntile = (R + 15) / 16;
+    const uint32_t *Uq = (const uint32_t *)a.Uq;
+    double *wrow = (double *)a.wrow + (long long)blockIdx.y * R;
+    for (long long rt = (long long)blockIdx.x * 4 + wv; rt < ntile; rt += (long long)gridDim.x * 4) {
+        const long long ra = rt * 16 + lo;  // A operand row
+        uint32_t w0 = 0, w1 = 0;
+        if (ra < R) {
+            w0 = a.spins[ra * NW];
+            if (NW > 1) w1 = a.spins[ra * NW + 1];
+        }
+        AT av;
+        {
+            const int i0 = BPL * g;  // first spin of this lane
+            uint32_t bytes[BPL / 4];
+#pragma unroll
+            for (int h = 0; h < BPL / 4; h++) {
+                const int i = i0 + 4 * h;
+                const uint32_t word = i < 32 ? w0 : w1;
+                const uint32_t bits = (word >> (i & 31)) & 15u;
+                const int nv = N - i;
+                const uint32_t valid = (ra < R && nv > 0) ? (nv >= 4 ? 15u : ((1u << nv) - 1u)) : 0u;
+                bytes[h] = spin_bytes4(bits, valid);
+            }
+            __builtin_memcpy(&av, bytes, sizeof(av));
+        }
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < nct; c++) {
+            v4i y[QV];
+#pragma unroll
+            for (int u = 0; u < QV; u++) {
+                const AT bv = *(const AT *)(fr + ((size_t)(c * QV + u) * 64 + lane) * BPL);
+                y[u] = Op::mfma(av, bv);
+            }
+            const uint4 wq = *(const uint4 *)(Uq + ((rt * NCT + ct0 + c) * 64 + lane) * 4);
+            const uint32_t wds[4] = {wq.x, wq.y, wq.z, wq.w};
+            const double csc = cs[16 * c + lo], bias = bs[16 * c + lo];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int h = (y[5][e] * 256 + y[4][e]) * 256 + y[3][e];
+                const int l = (y[2][e] * 256 + y[1][e]) * 256 + y[0][e];
+                const double yv = fma((double)h, 16777216.0, (double)l);
+                const double z = fma(yv, csc, bias);
+                acc[e] = fma((double)decode_x(wds[e]), z, acc[e]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const double s = xor_sum16(acc[e]);
+            const long long r = rt * 16 + 4 * g + e;
+            if (lo == 0 && r < R) wrow[r] = s * a.escale[r];
+        }
+    }
+}
+
+// Chunks of the backward pass: 64 rows, row t = 16 q + 4 g + j of a chunk sits in
+// byte j of dword q of lane group g (the k(l, e) of the 16x16x64 operands).  Block b
+// of the backward grid takes chunks [b n64 / nblk, (b+1) n64 / nblk).
+__device__ inline void block_chunks(long long n64, long long b, long long nb, long long &c0, long long &c1)
+{
+    c0 = n64 * b / nb;
+    c1 = n64 * (b + 1) / nb;
+}
+
+// Spin masks, once per sample set: msk[(chunk NI + i) 4 + g] = uint4 over q of dwords
+// with byte j = 0xff where sigma_{16 q + 4 g + j}[i] = -1 (0 for spins >= N, rows >= R).
+// Thread per (chunk, i, g).
+__global__ void k_q8_masks(const uint32_t *spins, long long R, int N, int NW, int NI, long long n64, uint4 *msk)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n64 * NI * 4) return;
+    const int g = (int)(idx & 3), i = (int)((idx >> 2) % NI);
+    const long long ch = idx / (4LL * NI);
+    uint32_t d[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        uint32_t bits = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long long r = ch * 64 + 16 * q + 4 * g + j;
+            if (r < R && i < N) bits |= ((spins[r * NW + (i >> 5)] >> (i & 31)) & 1u) << j;
+        }
+        d[q] = expand4(bits) * 0xffu;
+    }
+    msk[idx] = uint4{d[0], d[1], d[2], d[3]};
+}
+
+// Row weights of the backward pass, per call: omega_r = wt(s) 2^(e_r - 30) with
+// wt = w_s, 1 or E_s (WSEL 0, 1, 2).  Block b (one per backward block): G_b from
+// max |omega| over its rows (gblk[b]), then per chunk the digit dwords
+// pd[(chunk QW + u) 4 + g] = uint4 over q of Omega = rint(omega 2^(46 - G_b)) (digit u,
+// least significant first); and part[b][P] = sum of wt(s) over its rows with r % K == 0.
+template <int WSEL>
+__global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *pd)
+{
+    __shared__ uint32_t st[QW][4][4];
+    __shared__ double red[4], rsum[4];
+    const int K = a.K;
+    const long long R = a.S * K;
+    const long long n64 = (R + 63) / 64;
+    long long c0, c1;
+    block_chunks(n64, blockIdx.x, gridDim.x, c0, c1);
+    const double *w = (const double *)a.w;
+    const double *es = a.escale;
+    auto wt = [&](long long r) -> double {
+        if constexpr (WSEL == 0) return w[r / K];
+        else if constexpr (WSEL == 1) return 1.0;
+        else return ((const nq64::cd *)a.E)[r / K].re;
+    };
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    double mx = 0.0, sw = 0.0;
+    for (long long r = c0 * 64 + threadIdx.x; r < c1 * 64 && r < R; r += blockDim.x) {
+        const double x = wt(r);
+        mx = fmax(mx, fabs(x * es[r]));
+        if (r % K == 0) sw += x;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        mx = fmax(mx, __shfl_xor(mx, off));
+        sw += __shfl_xor(sw, off);
+    }
+    if (lane == 0) {
+        red[wv] = mx;
+        rsum[wv] = sw;
+    }
+    __syncthreads();
+    double m = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) m = fmax(m, red[q]);
+    int G = 0;
+    if (m > 0.0) (void)frexp(m, &G);  // m < 2^G
+    if (threadIdx.x == 0) {
+        gblk[blockIdx.x] = G;
+        double t = 0.0;
+        for (int q = 0; q < (int)(blockDim.x >> 6); q++) t += rsum[q];
+        ((double *)a.part)[(long long)blockIdx.x * (a.P + 1) + a.P] = t;
+    }
+    const double wsc = ldexp(1.0, WBITS - G);
+    for (long long ch = c0; ch < c1; ch++) {
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            const int t = threadIdx.x;
+            const long long r = ch * 64 + t;
+            long long q = r < R ? (long long)rint(wt(r) * es[r] * wsc) : 0;
+            const int qq = t >> 4, gg = (t >> 2) & 3, j = t & 3;
+#pragma unroll
+            for (int u = 0; u < QW; u++) {
+                const long long d = (u < QW - 1) ? bal8l(q) : q;
+                ((uint8_t *)&st[u][gg][qq])[j] = (uint8_t)(d & 255);
+                q = (q - d) >> 8;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < QW * 4) {
+            const int u = threadIdx.x >> 2, g = threadIdx.x & 3;
+            pd[(ch * QW + u) * 4 + g] = uint4{st[u][g][0], st[u][g][1], st[u][g][2], st[u][g][3]};
+        }
+    }
+}
+
+// Backward pass.  Block b: GS wavefronts, wavefront wv takes column tile
+// ct = blockIdx.y GS + wv with NIT spin tiles (slice blockIdx.z of the spins), over the
+// chunks of block b.  The
+// chunk's weight digits and spin masks are staged in LDS (double-buffered: the next
+// chunk's are loaded from global memory before the current chunk is multiplied and
+// stored after it), the lane's Uq words one chunk ahead in registers.  Per chunk: B
+// operand of factor digit t = bytes t of the lane's four words of row tile q (a 4 x 4
+// byte transpose); A operand of spin tile it, weight digit u: dword q = digits & mask.
+// Accumulator s = t + u collects sum_r b X Omega products of weight 256^s, tacc[s]
+// the column sums T.  Writes part[b][0..P) for its columns:
+// a and w blocks T - 2 sum b X Omega, b block T.
+template <int NIT>
+__global__ __launch_bounds__(512) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk)
+{
+    constexpr int NI = NIT * 16;             // spins of this block (blockIdx.z slice)
+    constexpr int NSTAGE = QW * 4 + NI * 4;  // uint4 per chunk: digits, masks
+    const int NIA = 16 * ((a.N + 15) / 16);  // spins of the mask array
+    const int i0 = blockIdx.z * NI;
+    __shared__ uint4 stg[2][NSTAGE];
+    const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
+    const int NCT = UC / 16;
+    const int GS = blockDim.x >> 6;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = lane >> 4, lo = lane & 15;
+    const int ct = blockIdx.y * GS + wv;
+    const bool tile = ct < NCT;
+    const long long R = a.S * K;
+    const long long KM = (long long)K * M;
+    const long long n64 = (R + 63) / 64;
+    long long c0, c1;
+    block_chunks(n64, blockIdx.x, gridDim.x, c0, c1);
+    const int G = gblk[blockIdx.x];
+    auto stage_load = [&](long long ch, int idx) -> uint4 {
+        return idx < QW * 4 ? pd[ch * QW * 4 + idx] : msk[(ch * NIA + i0) * 4 + (idx - QW * 4)];
+    };
+
+    v4i acc[NIT][NS];
+#pragma unroll
+    for (int it = 0; it < NIT; it++)
+#pragma unroll
+        for (int s = 0; s < NS; s++) acc[it][s] = v4i{0, 0, 0, 0};
+    int tacc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) tacc[s] = 0;
+    const uint4 *Uq = (const uint4 *)a.Uq;
+    const int ctl = tile ? ct : 0;
+    auto load_words = [&](long long ch, uint4 *wq) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) wq[q] = Uq[((ch * 4 + q) * NCT + ctl) * 64 + lane];
+    };
+    // pipeline: the words of chunk ch + 1 and its staged data are loaded while chunk ch
+    // is multiplied; cur is only ever a copy of nxt, so waiting for it never waits for
+    // the loads just issued
+    uint4 cur[4], nxt[4];
+    if (c0 < c1) {
+        for (int idx = threadIdx.x; idx < NSTAGE; idx += blockDim.x) stg[0][idx] = stage_load(c0, idx);
+        load_words(c0, nxt);
+    }
+    __syncthreads();
+    for (long long ch = c0; ch < c1; ch++) {
+        const int buf = (int)((ch - c0) & 1);
+        const bool more = ch + 1 < c1;
+#pragma unroll
+        for (int q = 0; q < 4; q++) cur[q] = nxt[q];
+        uint4 sx = uint4{0, 0, 0, 0};
+        const int sidx = threadIdx.x;
+        if (more) {
+            if (sidx < NSTAGE) sx = stage_load(ch + 1, sidx);
+            load_words(ch + 1, nxt);
+        }
+        if (tile) {
+            v4i bp[QU];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint4 wq = cur[q];
+                const uint32_t t0 = __builtin_amdgcn_perm(wq.y, wq.x, 0x05010400u);
+                const uint32_t t1 = __builtin_amdgcn_perm(wq.y, wq.x, 0x07030602u);
+                const uint32_t t2 = __builtin_amdgcn_perm(wq.w, wq.z, 0x05010400u);
+                const uint32_t t3 = __builtin_amdgcn_perm(wq.w, wq.z, 0x07030602u);
+                bp[0][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x05040100u);
+                bp[1][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x07060302u);
+                bp[2][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x05040100u);
+                bp[3][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x07060302u);
+            }
+            uint4 mq[NIT];
+#pragma unroll
+            for (int it = 0; it < NIT; it++) mq[it] = stg[buf][QW * 4 + (16 * it + lo) * 4 + g];
+#pragma unroll
+            for (int u = 0; u < QW; u++) {
+                const uint4 ps = stg[buf][u * 4 + g];
+                const uint32_t pw[4] = {ps.x, ps.y, ps.z, ps.w};
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int t = 0; t < QU; t++)
+                        tacc[t + u] = __builtin_amdgcn_sdot4((int)pw[q], bp[t][q], tacc[t + u], false);
+#pragma unroll
+                for (int it = 0; it < NIT; it++) {
+                    const v4i av = v4i{(int)(pw[0] & mq[it].x), (int)(pw[1] & mq[it].y), (int)(pw[2] & mq[it].z),
+                                       (int)(pw[3] & mq[it].w)};
+#pragma unroll
+                    for (int t = 0; t < QU; t++)
+                        acc[it][t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bp[t], acc[it][t + u], 0, 0, 0);
+                }
+            }
+        }
+        if (more && sidx < NSTAGE) stg[buf ^ 1][sidx] = sx;
+        __syncthreads();
+    }
+    if (!tile) return;
+    double *part = (double *)a.part + (long long)blockIdx.x * (a.P + 1);
+    const double osc = ldexp(1.0, G - WBITS);
+    const int col = 16 * ct + lo;
+    int ts[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        int x = tacc[s];
+        x += __shfl_xor(x, 16);
+        x += __shfl_xor(x, 32);
+        ts[s] = x;
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; it++) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            double hsum = 0.0;
+#pragma unroll
+            for (int s = NS - 1; s >= 0; s--)
+                hsum = fma(hsum, 256.0, fma(-2.0, (double)acc[it][s][e], (double)ts[s]));
+            const int i = i0 + 16 * it + 4 * g + e;
+            if (i < N) {
+                const double val = hsum * osc;
+                if (col < KM) part[(col / M) * a.L + N + M + (long long)i * M + col % M] = val;
+                else if (col < KM + K) part[(col - KM) * a.L + i] = val;
+            }
+        }
+    }
+    double hsum = 0.0;
+#pragma unroll
+    for (int s = NS - 1; s >= 0; s--) hsum = fma(hsum, 256.0, (double)ts[s]);
+    if (g == 0 && col < KM && blockIdx.z == 0) part[(col / M) * a.L + N + col % M] = hsum * osc;
+}
+
+int nit_of(int N) { return (N + 15) / 16; }
+
+template <int WSEL>
+hipError_t launch_bwd(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+{
+    int *gblk = (int *)a.q8w;
+    uint4 *pd = (uint4 *)(gblk + sh.gblk_ints);
+    hipLaunchKernelGGL((k_q8_omega<WSEL>), dim3(sh.bwd_blocks), dim3(256), 0, st, a, gblk, pd);
+    const uint4 *msk = (const uint4 *)a.q8m;
+    const dim3 bb(64 * sh.bwd_waves);
+    const int nit = nit_of(a.N);
+    if (nit == 1) {
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, sh.bwd_cgs, 1), bb, 0, st, a, gblk, pd, msk);
+    } else if (nit % 2 == 0) {  // slices of two spin tiles
+        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(sh.bwd_blocks, sh.bwd_cgs, nit / 2), bb, 0, st, a, gblk, pd, msk);
+    } else {
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, sh.bwd_cgs, nit), bb, 0, st, a, gblk, pd, msk);
+    }
+    return hipGetLastError();
+}
+
+} // namespace
+
+// Applies for real weights and N <= 64.  Forward: column groups of cgt tiles whose v
+// operands fit in 64 KB of LDS.  Backward: GS <= 8 wavefronts (one column tile each)
+// per block, groups balanced; at most MAX_BLOCK_CHUNKS chunks per block (int32 sums
+// stay exact).
+// Work buffers: q8w (block scales, weight digits; per call), q8m (spin masks; per
+// sample set).
+Q8Shape nq_q8_shape(int N, int M, int K, long long R)
+{
+    Q8Shape sh{};
+    if (N > 64) return sh;
+    const int UC = 16 * ((K * M + K + 15) / 16);
+    const int nct = UC / 16;
+    sh.nsp = N <= 32 ? 32 : 64;
+    const int bpl = sh.nsp / 4;
+    const size_t per_tile = (size_t)QV * 64 * bpl + 2 * 16 * sizeof(double);
+    int cgt = (int)(65536 / per_tile);
+    if (cgt > nct) cgt = nct;
+    sh.ucols = UC;
+    sh.fwd_cgt = cgt;
+    sh.fwd_cgs = (nct + cgt - 1) / cgt;
+    sh.fwd_lds = (size_t)cgt * per_tile;
+    sh.fwd_blocks = 1024;
+    const int ngrp = (nct + 7) / 8;
+    sh.bwd_cgs = ngrp;
+    sh.bwd_waves = (nct + ngrp - 1) / ngrp;
+    const long long n64 = (R + 63) / 64;
+    long long nb = 2048;
+    if (nb > n64) nb = n64 > 0 ? n64 : 1;
+    while (nb * MAX_BLOCK_CHUNKS < n64) nb *= 2;
+    sh.bwd_blocks = (int)nb;
+    const char *e = getenv("NQES_Q8_BWD_BLOCKS");
+    if (e && *e && atoi(e) > 0 && (long long)atoi(e) * MAX_BLOCK_CHUNKS >= n64) sh.bwd_blocks = atoi(e);
+    e = getenv("NQES_Q8_FWD_BLOCKS");
+    if (e && *e && atoi(e) > 0) sh.fwd_blocks = atoi(e);
+    sh.rpad = n64 * 64;
+    sh.vq_bytes = (size_t)nct * QV * 64 * 16 + 2 * (size_t)UC * sizeof(double);
+    sh.gblk_ints = (sh.bwd_blocks + 3) / 4 * 4;  // keeps pn 16-byte aligned
+    sh.w_bytes = (size_t)sh.gblk_ints * 4 + (size_t)n64 * QW * 4 * 16;
+    sh.m_bytes = (size_t)n64 * 16 * ((N + 15) / 16) * 4 * 16;
+    sh.ok = 1;
+    return sh;
+}
+
+hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+{
+    const long long R = a.S * a.K;
+    const long long ntile = sh.rpad / 16;
+    long long blocks = (ntile + 3) / 4;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_q8_pack, dim3((unsigned)blocks), dim3(256), 0, st, (const float *)a.U, R, ntile, a.Ucols,
+                       (uint32_t *)a.Uq, (double *)a.escale);
+    const int NI = 16 * nit_of(a.N);
+    const long long n64 = sh.rpad / 64, nthr = n64 * NI * 4;
+    hipLaunchKernelGGL(k_q8_masks, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, a.spins, R, a.N, a.NW, NI,
+                       n64, (uint4 *)a.q8m);
+    return hipGetLastError();
+}
+
+hipError_t nq_q8_apply(const SvfArgs &a0, const Q8Shape &sh, void *scratch, void *out, hipStream_t st)
+{
+    SvfArgs a = a0;
+    a.ncg = sh.fwd_cgs;
+    const int UC = a.Ucols, nct = UC / 16;
+    uint8_t *vfrag = (uint8_t *)a.vq;
+    double *colscale = (double *)(vfrag + (size_t)nct * QV * 64 * 16);
+    hipLaunchKernelGGL(k_q8_prep_v, dim3((UC + 63) / 64), dim3(64), 0, st, a, sh.nsp, vfrag, colscale, colscale + UC);
+    const dim3 grid(sh.fwd_blocks, sh.fwd_cgs);
+    if (sh.nsp == 32) hipLaunchKernelGGL((k_q8_fwd<32>), grid, dim3(256), sh.fwd_lds, st, a, sh.fwd_cgt);
+    else hipLaunchKernelGGL((k_q8_fwd<64>), grid, dim3(256), sh.fwd_lds, st, a, sh.fwd_cgt);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = nq_launch_svf_combine(false, a, st);
+    if (e != hipSuccess) return e;
+    e = launch_bwd<0>(a, sh, st);
+    if (e != hipSuccess) return e;
+    return nq_chunk_reduce(false, a.part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, out, st);
+}

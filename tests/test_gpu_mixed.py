@@ -136,6 +136,39 @@ def test_factored_sv_matches_stored(nqes, monkeypatch, N, M, K, C, m, B, cplx):
     assert mom["factored"]["emat"] is not None and np.array_equal(mom["factored"]["emat"], mom["stored"]["emat"])
 
 
+@pytest.mark.parametrize("N,M,K,C,m", [(8, 16, 1, 37, 5), (8, 16, 2, 64, 8), (17, 34, 3, 29, 7), (32, 64, 2, 96, 9),
+                                        (40, 80, 2, 33, 5), (64, 128, 4, 21, 3), (24, 24, 4, 50, 4)])
+def test_q8_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
+    """The int8 form of the factored S*v (real weights) against the FP64-MFMA form on
+    the same samples: equal to the 2^-27 quantisation of the factors (1e-7 relative),
+    linear and symmetric to 1e-11 (v and the row weights are quantised to 2^-47 and
+    2^-41 of their scale, the weights per block of rows).  Row counts S K that are not multiples of 64 exercise the
+    padding; N = 40 and 64 the 64-spin MFMA, N = 64 M = 128 K = 4 several column
+    groups in both passes."""
+    print(f"case N={N} M={M} K={K} C={C} m={m}", flush=True)
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=7 + N, scale=0.1, complex_=False)
+    P = K * (N + M + N * M)
+    rng = np.random.default_rng(9)
+    v1, v2 = rng.normal(size=P) + 0j, rng.normal(size=P) * 1e-3 + 0j
+    out = {}
+    for mode in ("fp64mfma", "q8"):
+        if mode == "q8":
+            monkeypatch.delenv("NQES_SV", raising=False)
+        else:
+            monkeypatch.setenv("NQES_SV", mode)
+        with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+            s.sample(W, 77, C, m, 3, 0.5)
+            out[mode] = [s.apply_S(v) for v in (v1, v2, v1 + 2.5 * v2)]
+    a1, a2, a12 = out["q8"]
+    b1, b2, _ = out["fp64mfma"]
+    assert rel_err(a1, b1) < 1e-7, rel_err(a1, b1)
+    assert rel_err(a2, b2) < 1e-7, rel_err(a2, b2)
+    assert rel_err(a12, a1 + 2.5 * a2) < 1e-11, rel_err(a12, a1 + 2.5 * a2)
+    sym = abs(np.vdot(v2, a1) - np.vdot(v1, a2)) / (np.linalg.norm(v1) * np.linalg.norm(a2))
+    assert sym < 1e-11, sym
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 def test_mixed_sr_step_matches_fp64(nqes, cplx):
     N, M, K, C, m, B = 8, 16, 2, 64, 16, 8
