@@ -240,14 +240,17 @@ __global__ __launch_bounds__(256) void k_q8_fwd(SvfArgs a, int cgt)
             __builtin_memcpy(&av, bytes, sizeof(av));
         }
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        const uint4 *wrt = (const uint4 *)Uq + (rt * NCT + ct0) * 64 + lane;
+        uint4 wn = wrt[0];  // words of column tile c + 1 load while tile c is multiplied
         for (int c = 0; c < nct; c++) {
+            const uint4 wq = wn;
+            if (c + 1 < nct) wn = wrt[(c + 1) * 64];
             v4i y[QV];
 #pragma unroll
             for (int u = 0; u < QV; u++) {
                 const AT bv = *(const AT *)(fr + ((size_t)(c * QV + u) * 64 + lane) * BPL);
                 y[u] = Op::mfma(av, bv);
             }
-            const uint4 wq = *(const uint4 *)(Uq + ((rt * NCT + ct0 + c) * 64 + lane) * 4);
             const uint32_t wds[4] = {wq.x, wq.y, wq.z, wq.w};
             const double csc = cs[16 * c + lo], bias = bs[16 * c + lo];
 #pragma unroll
@@ -372,41 +375,32 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
     }
 }
 
-// Backward pass.  Block b: GS wavefronts, wavefront wv takes column tile
-// ct = blockIdx.y GS + wv with NIT spin tiles (slice blockIdx.z of the spins), over the
-// chunks of block b.  The
-// chunk's weight digits and spin masks are staged in LDS (double-buffered: the next
-// chunk's are loaded from global memory before the current chunk is multiplied and
-// stored after it), the lane's Uq words one chunk ahead in registers.  Per chunk: B
-// operand of factor digit t = bytes t of the lane's four words of row tile q (a 4 x 4
-// byte transpose); A operand of spin tile it, weight digit u: dword q = digits & mask.
-// Accumulator s = t + u collects sum_r b X Omega products of weight 256^s, tacc[s]
-// the column sums T.  Writes part[b][0..P) for its columns:
-// a and w blocks T - 2 sum b X Omega, b block T.
+// Backward pass.  Wavefront wv of block (b, y, z) takes column tile ct = 4 y + wv and
+// spin tiles [z NIT, z NIT + NIT) over the chunks of block row b (every slice z forms
+// T, slice 0 writes the b block); wavefronts share nothing (no LDS, no barriers).  Per chunk: B operand of factor digit t = bytes t of
+// the lane's four Uq words of row tile q (a 4 x 4 byte transpose); A operand of spin
+// tile it, weight digit u: dword q = digits & mask.  Accumulator s = t + u collects
+// sum_r b X Omega products of weight 256^s, tacc[s] the column sums T.  The next
+// chunk's words, digits and masks are loaded while the current chunk is multiplied
+// (the current copies are only ever register copies of those loads).  Writes
+// part[b][0..P) for its columns: a and w blocks T - 2 sum b X Omega, b block T.
 template <int NIT>
-__global__ __launch_bounds__(512) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk)
+__global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk)
 {
-    constexpr int NI = NIT * 16;             // spins of this block (blockIdx.z slice)
-    constexpr int NSTAGE = QW * 4 + NI * 4;  // uint4 per chunk: digits, masks
-    const int NIA = 16 * ((a.N + 15) / 16);  // spins of the mask array
-    const int i0 = blockIdx.z * NI;
-    __shared__ uint4 stg[2][NSTAGE];
     const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
     const int NCT = UC / 16;
-    const int GS = blockDim.x >> 6;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
-    const int ct = blockIdx.y * GS + wv;
-    const bool tile = ct < NCT;
+    const int ct = blockIdx.y * 4 + wv;
+    if (ct >= NCT) return;
+    const int NIA = 16 * ((N + 15) / 16);  // spins of the mask array
+    const int i0 = blockIdx.z * NIT * 16;
     const long long R = a.S * K;
     const long long KM = (long long)K * M;
     const long long n64 = (R + 63) / 64;
     long long c0, c1;
     block_chunks(n64, blockIdx.x, gridDim.x, c0, c1);
     const int G = gblk[blockIdx.x];
-    auto stage_load = [&](long long ch, int idx) -> uint4 {
-        return idx < QW * 4 ? pd[ch * QW * 4 + idx] : msk[(ch * NIA + i0) * 4 + (idx - QW * 4)];
-    };
 
     v4i acc[NIT][NS];
 #pragma unroll
@@ -417,71 +411,56 @@ __global__ __launch_bounds__(512) void k_q8_bwd(SvfArgs a, const int *gblk, cons
 #pragma unroll
     for (int s = 0; s < NS; s++) tacc[s] = 0;
     const uint4 *Uq = (const uint4 *)a.Uq;
-    const int ctl = tile ? ct : 0;
-    auto load_words = [&](long long ch, uint4 *wq) {
+    uint4 nw[4], nd[QW], nm[NIT];
+    auto load_chunk = [&](long long ch) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) wq[q] = Uq[((ch * 4 + q) * NCT + ctl) * 64 + lane];
+        for (int q = 0; q < 4; q++) nw[q] = Uq[((ch * 4 + q) * NCT + ct) * 64 + lane];
+#pragma unroll
+        for (int u = 0; u < QW; u++) nd[u] = pd[(ch * QW + u) * 4 + g];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) nm[it] = msk[(ch * NIA + i0 + 16 * it + lo) * 4 + g];
     };
-    // pipeline: the words of chunk ch + 1 and its staged data are loaded while chunk ch
-    // is multiplied; cur is only ever a copy of nxt, so waiting for it never waits for
-    // the loads just issued
-    uint4 cur[4], nxt[4];
-    if (c0 < c1) {
-        for (int idx = threadIdx.x; idx < NSTAGE; idx += blockDim.x) stg[0][idx] = stage_load(c0, idx);
-        load_words(c0, nxt);
-    }
-    __syncthreads();
+    if (c0 < c1) load_chunk(c0);
     for (long long ch = c0; ch < c1; ch++) {
-        const int buf = (int)((ch - c0) & 1);
-        const bool more = ch + 1 < c1;
+        uint4 cw[4], cd[QW], cm[NIT];
 #pragma unroll
-        for (int q = 0; q < 4; q++) cur[q] = nxt[q];
-        uint4 sx = uint4{0, 0, 0, 0};
-        const int sidx = threadIdx.x;
-        if (more) {
-            if (sidx < NSTAGE) sx = stage_load(ch + 1, sidx);
-            load_words(ch + 1, nxt);
+        for (int q = 0; q < 4; q++) cw[q] = nw[q];
+#pragma unroll
+        for (int u = 0; u < QW; u++) cd[u] = nd[u];
+#pragma unroll
+        for (int it = 0; it < NIT; it++) cm[it] = nm[it];
+        if (ch + 1 < c1) load_chunk(ch + 1);
+        v4i bp[QU];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 wq = cw[q];
+            const uint32_t t0 = __builtin_amdgcn_perm(wq.y, wq.x, 0x05010400u);
+            const uint32_t t1 = __builtin_amdgcn_perm(wq.y, wq.x, 0x07030602u);
+            const uint32_t t2 = __builtin_amdgcn_perm(wq.w, wq.z, 0x05010400u);
+            const uint32_t t3 = __builtin_amdgcn_perm(wq.w, wq.z, 0x07030602u);
+            bp[0][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x05040100u);
+            bp[1][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x07060302u);
+            bp[2][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x05040100u);
+            bp[3][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x07060302u);
         }
-        if (tile) {
-            v4i bp[QU];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint4 wq = cur[q];
-                const uint32_t t0 = __builtin_amdgcn_perm(wq.y, wq.x, 0x05010400u);
-                const uint32_t t1 = __builtin_amdgcn_perm(wq.y, wq.x, 0x07030602u);
-                const uint32_t t2 = __builtin_amdgcn_perm(wq.w, wq.z, 0x05010400u);
-                const uint32_t t3 = __builtin_amdgcn_perm(wq.w, wq.z, 0x07030602u);
-                bp[0][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x05040100u);
-                bp[1][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x07060302u);
-                bp[2][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x05040100u);
-                bp[3][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x07060302u);
-            }
-            uint4 mq[NIT];
+        for (int u = 0; u < QW; u++) {
+            const uint32_t pw[4] = {cd[u].x, cd[u].y, cd[u].z, cd[u].w};
 #pragma unroll
-            for (int it = 0; it < NIT; it++) mq[it] = stg[buf][QW * 4 + (16 * it + lo) * 4 + g];
+            for (int q = 0; q < 4; q++)
 #pragma unroll
-            for (int u = 0; u < QW; u++) {
-                const uint4 ps = stg[buf][u * 4 + g];
-                const uint32_t pw[4] = {ps.x, ps.y, ps.z, ps.w};
+                for (int t = 0; t < QU; t++)
+                    tacc[t + u] = __builtin_amdgcn_sdot4((int)pw[q], bp[t][q], tacc[t + u], false);
 #pragma unroll
-                for (int q = 0; q < 4; q++)
+            for (int it = 0; it < NIT; it++) {
+                const v4i av = v4i{(int)(pw[0] & cm[it].x), (int)(pw[1] & cm[it].y), (int)(pw[2] & cm[it].z),
+                                   (int)(pw[3] & cm[it].w)};
 #pragma unroll
-                    for (int t = 0; t < QU; t++)
-                        tacc[t + u] = __builtin_amdgcn_sdot4((int)pw[q], bp[t][q], tacc[t + u], false);
-#pragma unroll
-                for (int it = 0; it < NIT; it++) {
-                    const v4i av = v4i{(int)(pw[0] & mq[it].x), (int)(pw[1] & mq[it].y), (int)(pw[2] & mq[it].z),
-                                       (int)(pw[3] & mq[it].w)};
-#pragma unroll
-                    for (int t = 0; t < QU; t++)
-                        acc[it][t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bp[t], acc[it][t + u], 0, 0, 0);
-                }
+                for (int t = 0; t < QU; t++)
+                    acc[it][t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bp[t], acc[it][t + u], 0, 0, 0);
             }
         }
-        if (more && sidx < NSTAGE) stg[buf ^ 1][sidx] = sx;
-        __syncthreads();
     }
-    if (!tile) return;
     double *part = (double *)a.part + (long long)blockIdx.x * (a.P + 1);
     const double osc = ldexp(1.0, G - WBITS);
     const int col = 16 * ct + lo;
@@ -524,14 +503,14 @@ hipError_t launch_bwd(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
     uint4 *pd = (uint4 *)(gblk + sh.gblk_ints);
     hipLaunchKernelGGL((k_q8_omega<WSEL>), dim3(sh.bwd_blocks), dim3(256), 0, st, a, gblk, pd);
     const uint4 *msk = (const uint4 *)a.q8m;
-    const dim3 bb(64 * sh.bwd_waves);
-    const int nit = nit_of(a.N);
+    const dim3 bb(256);
+    const int nit = nit_of(a.N), ny = (a.Ucols / 16 + 3) / 4;
     if (nit == 1) {
-        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, sh.bwd_cgs, 1), bb, 0, st, a, gblk, pd, msk);
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, ny, 1), bb, 0, st, a, gblk, pd, msk);
     } else if (nit % 2 == 0) {  // slices of two spin tiles
-        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(sh.bwd_blocks, sh.bwd_cgs, nit / 2), bb, 0, st, a, gblk, pd, msk);
+        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(sh.bwd_blocks, ny, nit / 2), bb, 0, st, a, gblk, pd, msk);
     } else {
-        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, sh.bwd_cgs, nit), bb, 0, st, a, gblk, pd, msk);
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, ny, nit), bb, 0, st, a, gblk, pd, msk);
     }
     return hipGetLastError();
 }
@@ -559,12 +538,12 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R)
     sh.fwd_cgt = cgt;
     sh.fwd_cgs = (nct + cgt - 1) / cgt;
     sh.fwd_lds = (size_t)cgt * per_tile;
-    sh.fwd_blocks = 1024;
+    sh.fwd_blocks = 1280;  // measured: 5 wavefronts per SIMD
     const int ngrp = (nct + 7) / 8;
     sh.bwd_cgs = ngrp;
     sh.bwd_waves = (nct + ngrp - 1) / ngrp;
     const long long n64 = (R + 63) / 64;
-    long long nb = 2048;
+    long long nb = 1024;  // measured: fewer, longer row ranges beat more partial sums
     if (nb > n64) nb = n64 > 0 ? n64 : 1;
     while (nb * MAX_BLOCK_CHUNKS < n64) nb *= 2;
     sh.bwd_blocks = (int)nb;
