@@ -35,6 +35,7 @@ sys.path.insert(0, os.path.join(REPO, "nqes-trapped-ion_amd"))
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix, published
+INT8_MFMA_PEAK_TOPS = 5000.0  # MI355X dense int8 matrix, published
 
 
 def load_J(N):
@@ -266,12 +267,24 @@ def main():
 
     P = args.K * (args.N + args.M + args.N * args.M)
     real = not np.any(W0.imag)
-    factored = args.precision == "mixed" and args.N <= 64 and os.environ.get("NQES_SV") != "stored"
+    sv_env = os.environ.get("NQES_SV")
+    factored = args.precision == "mixed" and args.N <= 64 and sv_env != "stored"
+    q8 = factored and real and sv_env != "fp64mfma"
     if args.precision == "fp64":
         # two streamed passes over complex128 O, w written and read, v / result / o
         b_O, n_pass = 16, 2
         sv_bytes = n_pass * S * P * b_O + 2 * S * 16 + 3 * P * 16
         sv_kernel = "S*v: k_sv_rowdot64 + k_sv_colacc64 (two passes over complex128 O)"
+    elif q8:
+        # int8 form: each pass reads the quantised factor words once (4 B per row and
+        # column), plus per row its scale (8 B) and spin words, and w once per sample
+        R = S * args.K
+        ucols = 16 * ((args.K * args.M + args.K + 15) // 16)
+        nw = (args.N + 31) // 32
+        sv_bytes = 2 * R * ucols * 4 + 2 * R * 8 + R * 4 * nw + 2 * S * 8
+        nsp = 32 if args.N <= 32 else 64
+        sv_ops = 2 * R * ucols * (6 * nsp + 24 * 16 * ((args.N + 15) // 16))  # int8 MFMA ops, both passes
+        sv_kernel = "S*v: k_q8_fwd + k_q8_bwd (int8 MFMA over quantised c*tanh(theta) factors)"
     elif factored:
         # two FP64-MFMA passes over the factors: sigma^T B (R x N x (K M + K)) and the
         # transposed product, R = S K rows; complex weights double the real products
@@ -286,7 +299,14 @@ def main():
     sv_calls = sum(s["sv_calls"] for s in stats)
     sv_ms = sum(s["t_sv_ms"] for s in stats)
     sv_avg_ms = sv_ms / max(sv_calls, 1)
-    if factored:
+    if q8:
+        achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
+        roofline = {"bound": "hbm", "kernel": sv_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args, real, "q8"),
+                    "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes,
+                    "int8_mfma_tops": sv_ops / (sv_avg_ms * 1e-3) / 1e12 if sv_calls else 0.0,
+                    "int8_mfma_peak_tops": INT8_MFMA_PEAK_TOPS}
+    elif factored:
         achieved = sv_flops / (sv_avg_ms * 1e-3) / 1e12 if sv_calls else 0.0
         roofline = {"bound": "mfma", "kernel": sv_kernel, "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
                     "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS,

@@ -62,6 +62,7 @@ struct MixedSampleArgs {
     unsigned int *flags;       // bit 0: an angle exceeded the FP32 range
     void *U;                   // (S*K) x Ucols T: S*v factors (NULL: not stored)
     int Ucols;
+    int j_lds;                 // measurement: stage J in LDS (after the tables)
 };
 
 // Factored S*v (sv_factored.hip): raw[p] = sum_s conj(O_sp) w_s, raw[P] = sum_s w_s,

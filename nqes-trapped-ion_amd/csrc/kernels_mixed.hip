@@ -282,9 +282,9 @@ __device__ inline void mx_refresh(const MixedSampleArgs &a, const MxTab<CPLX> &t
 // Phi^(i)) (rbm.cc:1042-1064), O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with
 // c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k} (Cramer's rule applied to rbm.cc:199-239).
 template <bool CPLX, int K, int G>
-__device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &tb, const uint32_t *sp,
-                                  const typename Ty<CPLX>::T *tl, int g, const typename Ty<CPLX>::D *phi,
-                                  long long s, bool live)
+__device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &tb, const double *Jm,
+                                  const uint32_t *sp, const typename Ty<CPLX>::T *tl, int g,
+                                  const typename Ty<CPLX>::D *phi, long long s, bool live)
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
@@ -295,55 +295,90 @@ __device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &t
     D phi_inv[K * K];
     inv_m<K, D>(phi, phi_inv);
 
-    // E_zz per replica: lanes take rows i = g, g+G, ...
+    // E_zz per replica: lanes take rows i = g, g+G, ..., and the columns j < i of row i
+    // starting at a lane-dependent offset (rows of J share LDS banks)
     double ez[K];
 #pragma unroll
     for (int k = 0; k < K; k++) {
         double r = 0.0;
         for (int i = g; i < N; i += G) {
-            int bi = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-            for (int j = 0; j < i; j++) {
-                int bj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
-                double v = a.J[(long long)i * N + j];
-                r += (bi ^ bj) ? -v : v;
+            const int bi = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+            const double *Ji = Jm + (long long)i * N;
+            int j = g % N;
+            for (int jj = 0; jj < N; jj++) {
+                if (j < i) {
+                    const int bj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
+                    const double v = Ji[j];
+                    r += (bi ^ bj) ? -v : v;
+                }
+                if (++j == N) j = 0;
             }
         }
         ez[k] = 2.0 * group_sum<G>(r);
     }
-    // sum_i Phi^(i): lanes take spins i = g, g+G, ...; the j loop starts at a
-    // lane-dependent offset to spread LDS banks
+    // sum_i Phi^(i): lanes take spins i = g, g + G, ... two at a time; each table entry
+    // cosh/sinh(2 w_{kk,i,j}) read feeds all K replicas, so a lane runs 2 K independent
+    // product chains.  The j loop starts at a lane-dependent offset to spread LDS banks.
     D H[K * K];
     {
         D sx[K * K];
 #pragma unroll
         for (int q = 0; q < K * K; q++) sx[q] = zeroD<D>();
 #pragma unroll
-        for (int k = 0; k < K; k++)
+        for (int kk = 0; kk < K; kk++) {
+            for (int i0 = g; i0 < N; i0 += 2 * G) {
+                const int i1 = (i0 + G < N) ? i0 + G : i0;  // i1 == i0: second chain discarded
+                const bool two = i0 + G < N;
+                int si0[K], si1[K];
 #pragma unroll
-            for (int kk = 0; kk < K; kk++) {
-                const T *tk = tl + (k * K + kk) * M;
-                for (int i = g; i < N; i += G) {
-                    const int si = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-                    const T *chr = chT + ((long long)kk * N + i) * M;
-                    const T *shr = shT + ((long long)kk * N + i) * M;
-                    D prod = oneD<D>();
-                    T pf = oneT<T>();
-                    int j = g % M;
-                    for (int jj = 0; jj < M; jj++) {
-                        T shv = shr[j];
-                        if (si) shv = negT(shv);
-                        pf = mulT(pf, subT(chr[j], mulT(tk[j], shv)));
-                        if ((jj & 7) == 7) {
-                            prod = mulD(prod, toD(pf));
-                            pf = oneT<T>();
-                        }
-                        if (++j == M) j = 0;
+                for (int k = 0; k < K; k++) {
+                    si0[k] = (sp[k * NW + (i0 >> 5)] >> (i0 & 31)) & 1u;
+                    si1[k] = (sp[k * NW + (i1 >> 5)] >> (i1 & 31)) & 1u;
+                }
+                const T *ch0 = chT + ((long long)kk * N + i0) * M, *sh0 = shT + ((long long)kk * N + i0) * M;
+                const T *ch1 = chT + ((long long)kk * N + i1) * M, *sh1 = shT + ((long long)kk * N + i1) * M;
+                D prod0[K], prod1[K];
+                T pf0[K], pf1[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    prod0[k] = oneD<D>();
+                    prod1[k] = oneD<D>();
+                    pf0[k] = oneT<T>();
+                    pf1[k] = oneT<T>();
+                }
+                int j = g % M;
+                for (int jj = 0; jj < M; jj++) {
+                    const T c0 = ch0[j], s0 = sh0[j], c1 = ch1[j], s1 = sh1[j];
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const T t = tl[(k * K + kk) * M + j];
+                        pf0[k] = mulT(pf0[k], subT(c0, mulT(t, si0[k] ? negT(s0) : s0)));
+                        pf1[k] = mulT(pf1[k], subT(c1, mulT(t, si1[k] ? negT(s1) : s1)));
                     }
-                    prod = mulD(prod, toD(pf));
-                    D e = si ? eaP[kk * N + i] : eaN[kk * N + i];
-                    sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e), prod));
+                    if ((jj & 7) == 7) {
+#pragma unroll
+                        for (int k = 0; k < K; k++) {
+                            prod0[k] = mulD(prod0[k], toD(pf0[k]));
+                            prod1[k] = mulD(prod1[k], toD(pf1[k]));
+                            pf0[k] = oneT<T>();
+                            pf1[k] = oneT<T>();
+                        }
+                    }
+                    if (++j == M) j = 0;
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    prod0[k] = mulD(prod0[k], toD(pf0[k]));
+                    const D e0 = si0[k] ? eaP[kk * N + i0] : eaN[kk * N + i0];
+                    sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e0), prod0[k]));
+                    if (two) {
+                        prod1[k] = mulD(prod1[k], toD(pf1[k]));
+                        const D e1 = si1[k] ? eaP[kk * N + i1] : eaN[kk * N + i1];
+                        sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e1), prod1[k]));
+                    }
                 }
             }
+        }
         D res[K * K];
 #pragma unroll
         for (int k = 0; k < K; k++)
@@ -607,6 +642,13 @@ __global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a)
 
     MxTab<CPLX> tb;
     unsigned char *p = mx_stage_tables<CPLX>(a, K, tb, smem);
+    const double *Jm = a.J;
+    if (a.j_lds) {
+        double *jl = (double *)p;
+        for (int q = tid; q < N * N; q += blockDim.x) jl[q] = a.J[q];
+        Jm = jl;
+        p = (unsigned char *)(jl + N * N);
+    }
     T *tl = (T *)p + (long long)sib * K * K * M;
     uint32_t *sp = (uint32_t *)((T *)p + (long long)spb * K * K * M) + sib * K * NW;
     int overflow = 0;
@@ -620,7 +662,7 @@ __global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a)
         __syncthreads();
         D phi[K * K];
         mx_refresh<CPLX, K, G, JMAX>(a, tb, sp, tl, g, 0, false, phi, overflow);
-        mx_measure<CPLX, K, G>(a, tb, sp, tl, g, phi, s, live);
+        mx_measure<CPLX, K, G>(a, tb, Jm, sp, tl, g, phi, s, live);
     }
     if (overflow) atomicOr(a.flags, 1u);
 }
@@ -921,6 +963,7 @@ hipError_t launch_measure_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
     const int spb = wpb * (64 / G);
     size_t lds = (size_t)spb * K * K * a.M * sizeof(T) + (size_t)spb * K * a.NW * 4;
     if (a.tab_lds) lds += mx_tab_bytes(CPLX, a.N, a.M, K);
+    if (a.j_lds) lds += (size_t)a.N * a.N * sizeof(double);
     const long long S = (long long)a.C * a.m;
     long long blocks = (S + spb - 1) / spb;
     if (lds > 65536) {

@@ -396,7 +396,7 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
            "theta GEMM kernel");
     const MixedShape sw = mixed_shape(c, cplx, 32, "NQES_G", "NQES_WPB");
     const MixedShape ms = mixed_shape(c, cplx, 16, "NQES_GM", "NQES_WPBM");
-    MixedSampleArgs a;
+    MixedSampleArgs a{};
     a.N = c->N;
     a.M = c->M;
     a.NW = c->NW;
@@ -429,6 +429,8 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.Ucols = c->svf_shape.ucols;
     HIPCHK(c, nq_launch_sample_mx(cplx, c->K, a, sw.G, sw.wpb, c->stream), "mixed sample kernel");
     a.tab_lds = ms.tab;
+    a.j_lds = ms.tab && nq_sample_mx_lds(cplx, c->N, c->M, c->K, ms.G, ms.wpb, true) +
+                                (size_t)c->N * c->N * sizeof(double) <= c->lds_max;
     HIPCHK(c, nq_launch_measure_mx(cplx, c->K, a, ms.G, ms.wpb, c->stream), "mixed measure kernel");
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
     if (c->svf) {
