@@ -49,7 +49,13 @@ enum {
 };
 
 enum {
-    NQES_MODEL_TRAPPEDION = 0 /* E_zz = sum_{i!=j} J_ij s_i s_j (rbm.cc:1012-1031) */
+    NQES_MODEL_TRAPPEDION = 0, /* ising-like (rbm.cc:1042-1064): E_zz = 2 sum_{i>j} J_ij s_i s_j
+                                  (rbm.cc:1012-1031) and the field -h sum_i sigma^x_i; also
+                                  the ising, periodicising and longrange models with their J */
+    NQES_MODEL_HS = 1,         /* hs-like (rbm.cc:1092-1119) with delta = 1:
+                                  sum_{i>j} J_ij (XX + YY + delta ZZ), h unused; J = hs_model
+                                  (rbm.cc:1066-1078) for the Haldane-Shastry chain */
+    NQES_MODEL_XXZ = 2         /* hs-like with delta = -1; J = xxz_model (rbm.cc:1083-1089) */
 };
 
 typedef struct nqes_ctx nqes_ctx;

@@ -254,7 +254,8 @@ void dump_rbm(const std::string &path, const std::vector<cplx> &W, int N, int M,
 }
 
 // ------------------------------------------------------------------ couplings
-// E_zz(s) = sum_{i != j} J_ij s_i s_j for the diagonal models of rbm.cc:965-1031
+// Couplings of the models of rbm.cc:965-1031 (diagonal: E_zz(s) = 2 sum_{i>j} J_ij s_i s_j)
+// and rbm.cc:1066-1089 (pair-flip: J_ij of sum_{i>j} J_ij (XX + YY + delta ZZ))
 std::vector<double> model_J(const std::string &model, int N, const std::string &jmat)
 {
     std::vector<double> J((size_t)N * N, 0.0);
@@ -282,11 +283,30 @@ std::vector<double> model_J(const std::string &model, int N, const std::string &
         for (int i = 0; i < N; i++)
             for (int j = 0; j < N; j++)
                 if (i != j) J[(size_t)i * N + j] = std::pow((double)std::abs(i - j), -0.2);
+    } else if (model == "hs") {
+        // hs_model (rbm.cc:1066-1078): J_ij = 1/d^2, d = sin(pi (i-j)/N) N/pi, i > j
+        for (int i = 1; i < N; i++)
+            for (int j = 0; j < i; j++) {
+                const double d = std::sin(M_PI * (i - j) / N) * N / M_PI;
+                J[(size_t)i * N + j] = 1.0 / (d * d);
+            }
+    } else if (model == "xxz") {
+        // xxz_model (rbm.cc:1083-1089): -1 on the bonds (i, i-1) and (N-1, 0)
+        for (int i = 1; i < N; i++)
+            for (int j = 0; j < i; j++)
+                if (i - j == 1 || (i == N - 1 && j == 0)) J[(size_t)i * N + j] = -1.0;
     } else {
-        throw std::runtime_error("model " + model + " needs pair-flip local energies; this build supports "
-                                 "trappedion, ising, periodicising and longrange");
+        throw std::runtime_error("unknown model " + model);
     }
     return J;
+}
+
+// the library's model: ising-like (trappedion and the diagonal models) or hs-like
+int model_id(const std::string &model)
+{
+    if (model == "hs") return NQES_MODEL_HS;
+    if (model == "xxz") return NQES_MODEL_XXZ;
+    return NQES_MODEL_TRAPPEDION;
 }
 
 // Eigen's default row-vector formatting: stream precision 6, every coefficient right
@@ -480,7 +500,7 @@ int main(int argc, char **argv)
             if (st == NQES_ENONFINITE && throw_fp) fp_error();
             throw std::runtime_error(nqes_last_error(ctx));
         };
-        check(nqes_problem_init(ctx, N, M, K, h, J.data(), NQES_MODEL_TRAPPEDION, precision));
+        check(nqes_problem_init(ctx, N, M, K, h, J.data(), model_id(model), precision));
 
         srloop::Schedule sched{lrate0, lratef, lrate_cut, reglam0, reglam_decay, reglam_cut};
         srloop::Convergence conv_test(K);

@@ -13,6 +13,8 @@ struct Fp64SampleArgs {
     uint64_t seed;
     double h;
     const double *J;           // N x N
+    int model;                 // 0 ising-like (rbm.cc:1042), 1 / 2 hs-like, delta = 1 / -1 (rbm.cc:1092)
+    double delta;
     const void *W;             // P complex
     const void *chT, *shT;     // K*N*M complex: cosh(2w), sinh(2w), [kk][i][j]
     const void *eaP, *eaN;     // K*N complex: exp(2a), exp(-2a), [kk][i]
@@ -46,6 +48,8 @@ struct MixedSampleArgs {
     int C, m, B;
     double h;
     const double *J;           // N x N
+    int model;                 // 0 ising-like, 1 / 2 hs-like with delta = 1 / -1
+    double delta;
     const void *chT, *shT;     // K*N*M T: cosh(2w), sinh(2w) [kk][i][j]
     const void *wT, *bT;       // K*N*M T: w [kk][i][j]; K*M T: b [kk][j]
     const void *eaP, *eaN;     // K*N D: exp(2a), exp(-2a) [kk][i]

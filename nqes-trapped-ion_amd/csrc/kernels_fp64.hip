@@ -218,6 +218,80 @@ __device__ inline void sweep64(const Fp64SampleArgs &a, cd *tl, uint32_t *sp, pc
     }
 }
 
+// hs-like local energy (rbm.cc:1092-1119): res row k = sum over pairs i > j (i, then j
+// ascending) of J_ij (2 Phi^(ij)_k - delta Phi_k) if s^k_i != s^k_j, else
+// (J_ij delta) Phi_k; H = Phi^+ res.  Phi^(ij)_k: spin i, then spin j flipped in
+// replica k (two flip_state_fcn steps, rbm.cc:1106-1108), the second on the angles
+// left by the first; each product over hidden units is a lane product and a
+// butterfly, as in the sweep.  Pairs with J_ij == 0 add exact zeros and are skipped.
+template <int K, int JPL>
+__device__ inline void hloc_hs64(const Fp64SampleArgs &a, const cd *tl, const uint32_t *sp, const cd *phi,
+                                 const cd *phi_inv, cd *H)
+{
+    const int N = a.N, M = a.M, NW = a.NW;
+    const int lane = threadIdx.x;
+    const cd *chT = static_cast<const cd *>(a.chT);
+    const cd *shT = static_cast<const cd *>(a.shT);
+    const cd *eaP = static_cast<const cd *>(a.eaP);
+    const cd *eaN = static_cast<const cd *>(a.eaN);
+    const double delta = a.delta;
+    cd res[K * K];
+#pragma unroll
+    for (int q = 0; q < K * K; q++) res[q] = zero();
+    for (int i = 1; i < N; i++)
+        for (int j = 0; j < i; j++) {
+            const double pij = a.J[(long long)i * N + j];
+            if (pij == 0.0) continue;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int si = spin_bit(sp, NW, k, i), sj = spin_bit(sp, NW, k, j);
+                if (si == sj) {
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) res[k * K + kk] = add(res[k * K + kk], scale(pij * delta, phi[k * K + kk]));
+                    continue;
+                }
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    cd t1[JPL];
+                    cd part = one();
+#pragma unroll
+                    for (int q = 0; q < JPL; q++) {
+                        const int hj = lane + 64 * q;
+                        if (hj < M) {
+                            const long long ti = ((long long)kk * N + i) * M + hj;
+                            const cd chv = chT[ti];
+                            cd shv = shT[ti];
+                            if (si) shv = neg(shv);
+                            const cd th = tl[(k * K + kk) * M + hj];
+                            const cd den = sub(chv, mul(th, shv));
+                            t1[q] = div(sub(mul(th, chv), shv), den);
+                            part = (q == 0) ? den : mul(part, den);
+                        }
+                    }
+                    const cd e = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                    cd v = mul(mul(phi[k * K + kk], e), wave_prod(part));
+                    cd part2 = one();
+#pragma unroll
+                    for (int q = 0; q < JPL; q++) {
+                        const int hj = lane + 64 * q;
+                        if (hj < M) {
+                            const long long tj = ((long long)kk * N + j) * M + hj;
+                            cd shv = shT[tj];
+                            if (sj) shv = neg(shv);
+                            const cd den = sub(chT[tj], mul(t1[q], shv));
+                            part2 = (q == 0) ? den : mul(part2, den);
+                        }
+                    }
+                    const cd e2 = sj ? eaP[kk * N + j] : eaN[kk * N + j];
+                    v = mul(mul(v, e2), wave_prod(part2));
+                    const cd d = sub(scale(2.0, v), scale(delta, phi[k * K + kk]));
+                    res[k * K + kk] = add(res[k * K + kk], scale(pij, d));
+                }
+            }
+        }
+    matmulK<K>(phi_inv, res, H);
+}
+
 template <int K, int JPL>
 __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
 {
@@ -247,9 +321,12 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
         const long long s = (long long)c * a.m + (sw - a.B);
         __syncthreads();
 
-        // local energy (rbm.cc:1042-1064); E_zz of replica (lane % K) (rbm.cc:1012-1031)
+        // local energy: hs-like (rbm.cc:1092-1119) or ising-like (rbm.cc:1042-1064)
         cd H[K * K];
-        {
+        if (a.model != 0) {
+            hloc_hs64<K, JPL>(a, tl, sp, phi, phi_inv, H);
+        } else {
+            // E_zz of replica (lane % K) (rbm.cc:1012-1031)
             double ez;
             {
                 int kr = lane % K;

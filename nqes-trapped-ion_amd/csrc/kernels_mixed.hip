@@ -277,9 +277,75 @@ __device__ inline void mx_refresh(const MixedSampleArgs &a, const MxTab<CPLX> &t
         }
 }
 
+// hs-like local energy (rbm.cc:1092-1119) of one sample held by a group of G lanes:
+// res row k = sum over pairs i > j of J_ij (2 Phi^(ij)_k - delta Phi_k) if
+// s^k_i != s^k_j, else J_ij delta Phi_k; H = Phi^-1 res.  Phi^(ij)_k flips spin i,
+// then spin j of replica k (the second flip on the angles after the first,
+// rbm.cc:1106-1108).  Lane g takes the pairs with index = g mod G; pairs with
+// J_ij == 0 are skipped.  Products over hidden units in T, flushed to D every 8.
+template <bool CPLX, int K, int G>
+__device__ inline void mx_hloc_hs(const MixedSampleArgs &a, const MxTab<CPLX> &tb, const double *Jm,
+                                  const uint32_t *sp, const typename Ty<CPLX>::T *tl, int g,
+                                  const typename Ty<CPLX>::D *phi, const typename Ty<CPLX>::D *phi_inv,
+                                  typename Ty<CPLX>::D *H)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    const int N = a.N, M = a.M, NW = a.NW;
+    const double delta = a.delta;
+    D res[K * K];
+#pragma unroll
+    for (int q = 0; q < K * K; q++) res[q] = zeroD<D>();
+    int pr = 0;
+    for (int i = 1; i < N; i++)
+        for (int j = 0; j < i; j++, pr++) {
+            if (pr % G != g) continue;
+            const double pij = Jm[(long long)i * N + j];
+            if (pij == 0.0) continue;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int si = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                const int sj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
+                if (si == sj) {
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) res[k * K + kk] = addD(res[k * K + kk], scaleD(pij * delta, phi[k * K + kk]));
+                    continue;
+                }
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    const T *chi = tb.ch + ((long long)kk * N + i) * M, *shi = tb.sh + ((long long)kk * N + i) * M;
+                    const T *chj = tb.ch + ((long long)kk * N + j) * M, *shj = tb.sh + ((long long)kk * N + j) * M;
+                    const T *tk = tl + (k * K + kk) * M;
+                    D prod = oneD<D>();
+                    T pf = oneT<T>();
+                    for (int hj = 0; hj < M; hj++) {
+                        const T t = tk[hj];
+                        const T ci = chi[hj], s_i = si ? negT(shi[hj]) : shi[hj];
+                        const T den1 = subT(ci, mulT(t, s_i));
+                        const T t1 = divT(subT(mulT(t, ci), s_i), den1);
+                        const T s_j = sj ? negT(shj[hj]) : shj[hj];
+                        const T den2 = subT(chj[hj], mulT(t1, s_j));
+                        pf = mulT(pf, mulT(den1, den2));
+                        if ((hj & 7) == 7) {
+                            prod = mulD(prod, toD(pf));
+                            pf = oneT<T>();
+                        }
+                    }
+                    prod = mulD(prod, toD(pf));
+                    const D e = mulD(si ? tb.eaP[kk * N + i] : tb.eaN[kk * N + i], sj ? tb.eaP[kk * N + j] : tb.eaN[kk * N + j]);
+                    const D v = mulD(mulD(phi[k * K + kk], e), prod);
+                    res[k * K + kk] = addD(res[k * K + kk], scaleD(pij, subD(scaleD(2.0, v), scaleD(delta, phi[k * K + kk]))));
+                }
+            }
+        }
+#pragma unroll
+    for (int q = 0; q < K * K; q++) res[q] = group_sum<G>(res[q]);
+    matmul_m<K, D>(phi_inv, res, H);
+}
+
 // Local energy and O row of sample s from its state (sp, tl, phi) held by a group
 // of G lanes: E_zz (rbm.cc:1012-1031), H_loc = Phi^-1 (diag(E_zz) Phi - h sum_i
-// Phi^(i)) (rbm.cc:1042-1064), O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with
+// Phi^(i)) (rbm.cc:1042-1064) or the hs-like H_loc above, O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with
 // c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k} (Cramer's rule applied to rbm.cc:199-239).
 template <bool CPLX, int K, int G>
 __device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &tb, const double *Jm,
@@ -295,6 +361,10 @@ __device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &t
     D phi_inv[K * K];
     inv_m<K, D>(phi, phi_inv);
 
+    D H[K * K];
+    if (a.model != 0) {
+        mx_hloc_hs<CPLX, K, G>(a, tb, Jm, sp, tl, g, phi, phi_inv, H);
+    } else {
     // E_zz per replica: lanes take rows i = g, g+G, ..., and the columns j < i of row i
     // starting at a lane-dependent offset (rows of J share LDS banks)
     double ez[K];
@@ -319,7 +389,6 @@ __device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &t
     // sum_i Phi^(i): lanes take spins i = g, g + G, ... two at a time; each table entry
     // cosh/sinh(2 w_{kk,i,j}) read feeds all K replicas, so a lane runs 2 K independent
     // product chains.  The j loop starts at a lane-dependent offset to spread LDS banks.
-    D H[K * K];
     {
         D sx[K * K];
 #pragma unroll
@@ -388,6 +457,7 @@ __device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &t
                 res[k * K + kk] = subD(scaleD(ez[k], phi[k * K + kk]), scaleD(a.h, sxs));
             }
         matmul_m<K, D>(phi_inv, res, H);
+    }
     }
     D energy = zeroD<D>();
 #pragma unroll

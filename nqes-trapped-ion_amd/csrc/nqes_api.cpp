@@ -270,7 +270,8 @@ int nqes_problem_init(nqes_ctx *c, int N, int M, int K, double h, const double *
     if (K < 1 || K > 4) return c->fail(NQES_EINVAL, "K=%d outside [1, 4]", K);
     if (M % N != 0) return c->fail(NQES_EINVAL, "M=%d must be a multiple of N=%d (rbm.cc:148)", M, N);
     if (!J) return c->fail(NQES_EINVAL, "J is NULL");
-    if (model != NQES_MODEL_TRAPPEDION) return c->fail(NQES_EINVAL, "unsupported model %d", model);
+    if (model != NQES_MODEL_TRAPPEDION && model != NQES_MODEL_HS && model != NQES_MODEL_XXZ)
+        return c->fail(NQES_EINVAL, "unsupported model %d", model);
     if (precision != NQES_PREC_FP64 && precision != NQES_PREC_MIXED)
         return c->fail(NQES_EINVAL, "unsupported precision %d", precision);
     if (precision == NQES_PREC_FP64) {
@@ -314,7 +315,7 @@ static int sample_fp64(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     const long long S = (long long)C * m;
     HIPCHK(c, nq_launch_tables64(c->N, c->M, c->K, c->L, c->dW.p, c->dchT.p, c->dshT.p, c->deaP.p, c->deaN.p, c->stream),
            "tables kernel");
-    Fp64SampleArgs a;
+    Fp64SampleArgs a{};
     a.N = c->N;
     a.M = c->M;
     a.NW = c->NW;
@@ -325,6 +326,8 @@ static int sample_fp64(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.B = B;
     a.seed = seed;
     a.h = c->h;
+    a.model = c->model;
+    a.delta = c->model == NQES_MODEL_XXZ ? -1.0 : 1.0;
     a.J = static_cast<const double *>(c->dJ.p);
     a.W = c->dW.p;
     a.chT = c->dchT.p;
@@ -407,6 +410,8 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.m = m;
     a.B = B;
     a.h = c->h;
+    a.model = c->model;
+    a.delta = c->model == NQES_MODEL_XXZ ? -1.0 : 1.0;
     a.J = static_cast<const double *>(c->dJ.p);
     a.chT = c->dchT.p;
     a.shT = c->dshT.p;
@@ -852,7 +857,7 @@ int nqes_correlations(nqes_ctx *c, const double *W, uint64_t seed, int C, int m,
     HIPCHK(c, hipMemcpyAsync(c->dW.p, W, 16 * c->P, hipMemcpyHostToDevice, c->stream), "copy W");
     HIPCHK(c, nq_launch_tables64(N, c->M, K, c->L, c->dW.p, c->dchT.p, c->dshT.p, c->deaP.p, c->deaN.p, c->stream),
            "tables kernel");
-    Fp64SampleArgs a;
+    Fp64SampleArgs a{};
     memset(&a, 0, sizeof(a));
     a.N = N;
     a.M = c->M;
@@ -864,6 +869,8 @@ int nqes_correlations(nqes_ctx *c, const double *W, uint64_t seed, int C, int m,
     a.B = B;
     a.seed = seed;
     a.h = c->h;
+    a.model = c->model;
+    a.delta = c->model == NQES_MODEL_XXZ ? -1.0 : 1.0;
     a.J = static_cast<const double *>(c->dJ.p);
     a.W = c->dW.p;
     a.chT = c->dchT.p;

@@ -18,7 +18,9 @@ LIB_PATH = os.path.join(HERE, "lib", "libnqes.so")
 
 PREC_FP64 = 0
 PREC_MIXED = 1
-MODEL_TRAPPEDION = 0
+MODEL_TRAPPEDION = 0  # ising-like: trapped-ion and the diagonal models (J given)
+MODEL_HS = 1          # hs-like, delta = 1 (Haldane-Shastry with its J)
+MODEL_XXZ = 2         # hs-like, delta = -1
 
 _cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")

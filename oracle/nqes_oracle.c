@@ -17,6 +17,7 @@
  *   - monte_carlo (SR step, matrix-free)        code/rbm.cc:564-720
  *   - trapped_ion_model                         code/rbm.cc:1012-1031
  *   - ising_like_hamiltonian                    code/rbm.cc:1042-1064
+ *   - hs_like_hamiltonian (models hs, xxz)      code/rbm.cc:1066-1119
  *   - minresqlp / sym_ortho                     code/minresqlp.h:85-390
  *
  * Canonical floating-point order.  The reference's exact bits depend on glibc's
@@ -440,6 +441,7 @@ typedef struct {
     long L, P;          /* L = N + M + N*M, P = K*L       */
     double h;
     const double *J;    /* N x N row-major                */
+    int model;          /* 0 ising-like, 1 hs-like delta = 1, 2 hs-like delta = -1 */
 } prob_t;
 
 static prob_t mkprob(int N, int M, int K, double h, const double *J)
@@ -448,6 +450,7 @@ static prob_t mkprob(int N, int M, int K, double h, const double *J)
     p.N = N; p.M = M; p.K = K; p.NW = (N + 31) / 32;
     p.L = (long)N + M + (long)N * M; p.P = (long)K * p.L;
     p.h = h; p.J = J;
+    p.model = 0;
     return p;
 }
 
@@ -549,8 +552,8 @@ static double ezz(const prob_t *p, const uint32_t *sk)
 /* ising_like_hamiltonian (rbm.cc:1042-1064):
  * H = Phi^+ (diag(E_zz(s^k)) Phi - h sum_i Phi^(i)), Phi^(i) = Phi with spin i
  * flipped in every replica. */
-static void hloc(const prob_t *p, const cplx *W, const uint32_t *s, const cplx *phi,
-                 const cplx *phi_inv, const cplx *tanh_, cplx *H, cplx *scratch)
+static void hloc_ising(const prob_t *p, const cplx *W, const uint32_t *s, const cplx *phi,
+                       const cplx *phi_inv, const cplx *tanh_, cplx *H, cplx *scratch)
 {
     int N = p->N, M = p->M, K = p->K;
     cplx res[ORC_KMAX * ORC_KMAX], sx[ORC_KMAX * ORC_KMAX];
@@ -578,6 +581,68 @@ static void hloc(const prob_t *p, const cplx *W, const uint32_t *s, const cplx *
     }
     for (int q = 0; q < K * K; q++) res[q] = csub(res[q], cscale(p->h, sx[q]));
     matmulK(phi_inv, res, K, H);
+}
+
+/* hs_like_hamiltonian (rbm.cc:1092-1119), H = sum_{i<j} J_ij (sx sx + sy sy + delta sz sz):
+ * res row k = sum over pairs i > j (i ascending, then j ascending) of
+ *   J_ij (2 Phi^(ij)_k - delta Phi_k)  if s^k_i != s^k_j,   (J_ij delta) Phi_k  else,
+ * H = Phi^+ res.  Phi^(ij)_k is row k of Phi with spin i, then spin j flipped in
+ * replica k: two flip_state_fcn steps (rbm.cc:1106-1108), the second on the angles
+ * left by the first.  Pairs with J_ij == 0 add exact zeros and are skipped.
+ * scratch: 2 M complex. */
+static void hloc_hs(const prob_t *p, const cplx *W, const uint32_t *s, const cplx *phi,
+                    const cplx *phi_inv, const cplx *tanh_, cplx *H, cplx *scratch)
+{
+    int N = p->N, M = p->M, K = p->K;
+    double delta = p->model == 1 ? 1.0 : -1.0;
+    cplx res[ORC_KMAX * ORC_KMAX];
+    cplx *den = scratch, *t1 = scratch + M;
+    for (int q = 0; q < K * K; q++) res[q] = cmk(0.0, 0.0);
+    for (int i = 1; i < N; i++)
+        for (int j = 0; j < i; j++) {
+            double pij = p->J[(long)i * N + j];
+            if (pij == 0.0) continue;
+            for (int k = 0; k < K; k++) {
+                int si = sbit(s + k * p->NW, i), sj = sbit(s + k * p->NW, j);
+                if (si == sj) {
+                    for (int kk = 0; kk < K; kk++)
+                        res[k * K + kk] = cadd(res[k * K + kk], cscale(pij * delta, phi[k * K + kk]));
+                    continue;
+                }
+                for (int kk = 0; kk < K; kk++) {
+                    const cplx *t = tanh_ + (k * K + kk) * M;
+                    cplx a2 = cscale(2.0, Wa(p, W, kk, i));
+                    cplx e = c_exp(si ? a2 : cneg(a2));
+                    for (int q = 0; q < M; q++) {
+                        cplx w2 = cscale(2.0, Ww(p, W, kk, i, q));
+                        cplx ws = si ? cneg(w2) : w2;
+                        cplx chv = c_cosh(ws), shv = c_sinh(ws);
+                        den[q] = csub(chv, cmul(t[q], shv));
+                        t1[q] = cdiv(csub(cmul(t[q], chv), shv), den[q]);
+                    }
+                    cplx v = cmul(cmul(phi[k * K + kk], e), tree_prod(den, M));
+                    cplx b2 = cscale(2.0, Wa(p, W, kk, j));
+                    cplx e2 = c_exp(sj ? b2 : cneg(b2));
+                    for (int q = 0; q < M; q++) {
+                        cplx w2 = cscale(2.0, Ww(p, W, kk, j, q));
+                        cplx ws = sj ? cneg(w2) : w2;
+                        den[q] = csub(c_cosh(ws), cmul(t1[q], c_sinh(ws)));
+                    }
+                    v = cmul(cmul(v, e2), tree_prod(den, M));
+                    cplx d = csub(cscale(2.0, v), cscale(delta, phi[k * K + kk]));
+                    res[k * K + kk] = cadd(res[k * K + kk], cscale(pij, d));
+                }
+            }
+        }
+    matmulK(phi_inv, res, K, H);
+}
+
+/* the model's local energy; scratch: 2 M complex */
+static void hloc(const prob_t *p, const cplx *W, const uint32_t *s, const cplx *phi,
+                 const cplx *phi_inv, const cplx *tanh_, cplx *H, cplx *scratch)
+{
+    if (p->model) hloc_hs(p, W, s, phi, phi_inv, tanh_, H, scratch);
+    else hloc_ising(p, W, s, phi, phi_inv, tanh_, H, scratch);
 }
 
 /* deriv (rbm.cc:199-239): O_p = det(Phi^+ derPhi), derPhi = Phi with column k
@@ -643,13 +708,20 @@ double orc_ezz(int N, const double *J, const uint32_t *s)
     return ezz(&p, s);
 }
 
+void orc_hloc_model(int model, int N, int M, int K, double h, const double *J, const double *W, const uint32_t *s,
+                    const double *phi, const double *phi_inv, const double *tanh_, double *H)
+{
+    prob_t p = mkprob(N, M, K, h, J);
+    p.model = model;
+    cplx *scr = (cplx *)malloc(sizeof(cplx) * 2 * M);
+    hloc(&p, (const cplx *)W, s, (const cplx *)phi, (const cplx *)phi_inv, (const cplx *)tanh_, (cplx *)H, scr);
+    free(scr);
+}
+
 void orc_hloc(int N, int M, int K, double h, const double *J, const double *W, const uint32_t *s,
               const double *phi, const double *phi_inv, const double *tanh_, double *H)
 {
-    prob_t p = mkprob(N, M, K, h, J);
-    cplx *scr = (cplx *)malloc(sizeof(cplx) * M);
-    hloc(&p, (const cplx *)W, s, (const cplx *)phi, (const cplx *)phi_inv, (const cplx *)tanh_, (cplx *)H, scr);
-    free(scr);
+    orc_hloc_model(0, N, M, K, h, J, W, s, phi, phi_inv, tanh_, H);
 }
 
 void orc_deriv(int N, int M, int K, const uint32_t *s, const double *phi, const double *phi_inv,
@@ -678,7 +750,7 @@ static void chain_alloc(const prob_t *p, chain_t *c)
     c->phip = (cplx *)malloc(sizeof(cplx) * K * K);
     c->tanh_ = (cplx *)malloc(sizeof(cplx) * M * K * K);
     c->tanhp = (cplx *)malloc(sizeof(cplx) * M * K * K);
-    c->scratch = (cplx *)malloc(sizeof(cplx) * M);
+    c->scratch = (cplx *)malloc(sizeof(cplx) * 2 * M);
     c->accepts = 0;
 }
 
@@ -727,12 +799,13 @@ static void chain_start(const prob_t *p, const cplx *W, uint32_t *s, u128 *rng, 
  * sample: spins (K*NW words), H (K*K complex), optionally O (P complex) and the
  * state (phi, phi_inv: K*K each; tanh: M*K*K).  Any output may be NULL.
  */
-void orc_run_chain(int N, int M, int K, double h, const double *J, const double *Wd,
-                   uint64_t seed, int c, int m, int B,
-                   uint32_t *spins_out, double *H_out, double *O_out,
-                   double *phi_out, double *phi_inv_out, double *tanh_out, long *accepts)
+void orc_run_chain_model(int model, int N, int M, int K, double h, const double *J, const double *Wd,
+                         uint64_t seed, int c, int m, int B,
+                         uint32_t *spins_out, double *H_out, double *O_out,
+                         double *phi_out, double *phi_inv_out, double *tanh_out, long *accepts)
 {
     prob_t p = mkprob(N, M, K, h, J);
+    p.model = model;
     const cplx *W = (const cplx *)Wd;
     u128 rng = pcg_seed(seed + (uint64_t)c);
     uint32_t s[ORC_KMAX * (ORC_NMAX / 32)];
@@ -751,6 +824,15 @@ void orc_run_chain(int N, int M, int K, double h, const double *J, const double 
     }
     if (accepts) *accepts = ch.accepts;
     chain_free(&ch);
+}
+
+void orc_run_chain(int N, int M, int K, double h, const double *J, const double *Wd,
+                   uint64_t seed, int c, int m, int B,
+                   uint32_t *spins_out, double *H_out, double *O_out,
+                   double *phi_out, double *phi_inv_out, double *tanh_out, long *accepts)
+{
+    orc_run_chain_model(0, N, M, K, h, J, Wd, seed, c, m, B, spins_out, H_out, O_out, phi_out, phi_inv_out,
+                        tanh_out, accepts);
 }
 
 /*
@@ -1188,13 +1270,14 @@ static void sv_apply(void *vc, const cplx *v, cplx *out)
  * o/skk/f (P complex each, after normalisation), eloc (1 complex).
  * Returns the MINRES-QLP flag.
  */
-int orc_sr_step(int N, int M, int K, double h, const double *J, double *Wd,
-                uint64_t seed, int C, int m, int B, double lr, double reglam,
-                double rtol, uint64_t maxit, int store_O,
-                double *emat_out, uint64_t *iters_out, long *accepts_out,
-                double *x_out, double *o_out, double *skk_out, double *f_out, double *eloc_out)
+int orc_sr_step_model(int model, int N, int M, int K, double h, const double *J, double *Wd,
+                      uint64_t seed, int C, int m, int B, double lr, double reglam,
+                      double rtol, uint64_t maxit, int store_O,
+                      double *emat_out, uint64_t *iters_out, long *accepts_out,
+                      double *x_out, double *o_out, double *skk_out, double *f_out, double *eloc_out)
 {
     prob_t p = mkprob(N, M, K, h, J);
+    p.model = model;
     cplx *W = (cplx *)Wd;
     long P = p.P, S = (long)C * m;
     int NW = p.NW;
@@ -1315,6 +1398,16 @@ int orc_sr_step(int N, int M, int K, double h, const double *J, double *Wd,
     free(x); free(o); free(f); free(skk);
     free(sv.spins); free(sv.phis); free(sv.phi_invs); free(sv.tanhs); free(sv.O_store);
     return flag;
+}
+
+int orc_sr_step(int N, int M, int K, double h, const double *J, double *Wd,
+                uint64_t seed, int C, int m, int B, double lr, double reglam,
+                double rtol, uint64_t maxit, int store_O,
+                double *emat_out, uint64_t *iters_out, long *accepts_out,
+                double *x_out, double *o_out, double *skk_out, double *f_out, double *eloc_out)
+{
+    return orc_sr_step_model(0, N, M, K, h, J, Wd, seed, C, m, B, lr, reglam, rtol, maxit, store_O, emat_out,
+                             iters_out, accepts_out, x_out, o_out, skk_out, f_out, eloc_out);
 }
 
 /* S*v on a given set of stored O rows (tests: matrix-free vs dense S) */

@@ -62,15 +62,19 @@ def lib():
         L.orc_ezz.restype = c_dbl
         L.orc_ezz.argtypes = [c_int, _dp, _up]
         L.orc_hloc.argtypes = [c_int, c_int, c_int, c_dbl, _dp, _cp, _up, _cp, _cp, _cp, _cp]
+        L.orc_hloc_model.argtypes = [c_int] + L.orc_hloc.argtypes
         L.orc_deriv.argtypes = [c_int, c_int, c_int, _up, _cp, _cp, _cp, _cp]
         L.orc_run_chain.argtypes = [c_int, c_int, c_int, c_dbl, _dp, _cp, c_u64, c_int, c_int, c_int,
                                     vp, vp, vp, vp, vp, vp, ctypes.POINTER(c_long)]
+        L.orc_run_chain_model.argtypes = [c_int] + L.orc_run_chain.argtypes
         L.orc_minresqlp_dense.restype = c_int
         L.orc_minresqlp_dense.argtypes = [c_long, _cp, _cp, _cp, c_dbl, c_u64, ctypes.POINTER(c_u64)]
         L.orc_sr_step.restype = c_int
         L.orc_sr_step.argtypes = [c_int, c_int, c_int, c_dbl, _dp, _cp, c_u64, c_int, c_int, c_int,
                                   c_dbl, c_dbl, c_dbl, c_u64, c_int, _cp, ctypes.POINTER(c_u64),
                                   ctypes.POINTER(c_long), vp, vp, vp, vp, vp]
+        L.orc_sr_step_model.restype = c_int
+        L.orc_sr_step_model.argtypes = [c_int] + L.orc_sr_step.argtypes
         L.orc_sv_from_O.restype = c_int
         L.orc_sv_from_O.argtypes = [c_long, c_long, _cp, _cp, _cp, _cp, _cp]
         L.orc_init_spins.argtypes = [c_int, c_int, c_u64, c_int, _up]
@@ -226,9 +230,12 @@ def ezz(N, J, spins_k):
                          np.ascontiguousarray(spins_k, dtype=np.uint32).reshape(-1))
 
 
-def hloc(N, M, K, h, J, W, spins, phi, phi_inv, tanh):
+MODEL_TRAPPEDION, MODEL_HS, MODEL_XXZ = 0, 1, 2  # ising-like; hs-like with delta = 1, -1
+
+
+def hloc(N, M, K, h, J, W, spins, phi, phi_inv, tanh, model=MODEL_TRAPPEDION):
     H = np.zeros(K * K, dtype=np.complex128)
-    lib().orc_hloc(N, M, K, float(h), np.ascontiguousarray(J, dtype=np.float64).reshape(-1),
+    lib().orc_hloc_model(int(model), N, M, K, float(h), np.ascontiguousarray(J, dtype=np.float64).reshape(-1),
                    np.ascontiguousarray(W), np.ascontiguousarray(spins, dtype=np.uint32).reshape(-1),
                    np.ascontiguousarray(phi).reshape(-1), np.ascontiguousarray(phi_inv).reshape(-1),
                    np.ascontiguousarray(tanh).reshape(-1), H)
@@ -243,7 +250,7 @@ def deriv(N, M, K, spins, phi, phi_inv, tanh):
     return O
 
 
-def run_chain(N, M, K, h, J, W, seed, c, m, B, want_O=False, want_state=False):
+def run_chain(N, M, K, h, J, W, seed, c, m, B, want_O=False, want_state=False, model=MODEL_TRAPPEDION):
     """One chain of an SR step: returns dict of per-sample spins, H, [O], [state]."""
     NW = nwords(N)
     P = nparams(N, M, K)
@@ -254,7 +261,7 @@ def run_chain(N, M, K, h, J, W, seed, c, m, B, want_O=False, want_state=False):
     phi_inv = np.zeros((m, K, K), dtype=np.complex128) if want_state else None
     tanh = np.zeros((m, K, K, M), dtype=np.complex128) if want_state else None
     acc = ctypes.c_long(0)
-    lib().orc_run_chain(N, M, K, float(h), np.ascontiguousarray(J, dtype=np.float64).reshape(-1),
+    lib().orc_run_chain_model(int(model), N, M, K, float(h), np.ascontiguousarray(J, dtype=np.float64).reshape(-1),
                         np.ascontiguousarray(W, dtype=np.complex128), int(seed) % (1 << 64), c, m, B,
                         _ptr(spins), _ptr(H), _ptr(O), _ptr(phi), _ptr(phi_inv), _ptr(tanh),
                         ctypes.byref(acc))
@@ -276,7 +283,7 @@ def minresqlp_dense(A, b, rtol, maxit=2**64 - 1):
 
 
 def sr_step(N, M, K, h, J, W, seed, C, m, B, lr, reglam, rtol, maxit=2**64 - 1, store_O=False,
-            want_details=False):
+            want_details=False, model=MODEL_TRAPPEDION):
     """One SR step (rbm.cc monte_carlo).  Returns (W_new, emat, info)."""
     P = nparams(N, M, K)
     Wn = np.ascontiguousarray(W, dtype=np.complex128).copy()
@@ -285,7 +292,7 @@ def sr_step(N, M, K, h, J, W, seed, C, m, B, lr, reglam, rtol, maxit=2**64 - 1, 
     acc = ctypes.c_long(0)
     det_arrays = [np.zeros(P, dtype=np.complex128) for _ in range(4)] if want_details else [None] * 4
     eloc = np.zeros(1, dtype=np.complex128) if want_details else None
-    flag = lib().orc_sr_step(N, M, K, float(h), np.ascontiguousarray(J, dtype=np.float64).reshape(-1),
+    flag = lib().orc_sr_step_model(int(model), N, M, K, float(h), np.ascontiguousarray(J, dtype=np.float64).reshape(-1),
                              Wn, int(seed) % (1 << 64), C, m, B, float(lr), float(reglam), float(rtol),
                              int(maxit), int(bool(store_O)), emat, ctypes.byref(it), ctypes.byref(acc),
                              *[_ptr(a) for a in det_arrays], _ptr(eloc))

@@ -46,6 +46,46 @@ def dense_hamiltonian(N, h, J):
     return H
 
 
+def hs_J(N):
+    """Haldane-Shastry couplings (lower triangle): J_ij = 1/d^2, d = (N/pi) sin(pi (i-j)/N)."""
+    J = np.zeros((N, N))
+    for i in range(1, N):
+        for j in range(i):
+            d = np.sin(np.pi * (i - j) / N) * N / np.pi
+            J[i, j] = 1.0 / (d * d)
+    return J
+
+
+def xxz_J(N):
+    """XXZ chain couplings (lower triangle): -1 on the bonds (i, i-1) and (N-1, 0)."""
+    J = np.zeros((N, N))
+    for i in range(1, N):
+        J[i, i - 1] = -1.0
+    if N > 2:
+        J[N - 1, 0] = -1.0
+    return J
+
+
+def dense_heisenberg(N, J, delta):
+    """H = sum_{i>j} J_ij (X_i X_j + Y_i Y_j + delta Z_i Z_j) on 2^N states, J's lower
+    triangle (bit i set = sigma_i = -1); XX + YY maps |..01..> to 2 |..10..>."""
+    dim = 1 << N
+    H = np.zeros((dim, dim))
+    for s in range(dim):
+        for i in range(1, N):
+            for j in range(i):
+                p = J[i, j]
+                if p == 0.0:
+                    continue
+                bi, bj = (s >> i) & 1, (s >> j) & 1
+                if bi == bj:
+                    H[s, s] += p * delta
+                else:
+                    H[s, s] -= p * delta
+                    H[s ^ (1 << i) ^ (1 << j), s] += 2.0 * p
+    return H
+
+
 def ed_energies(N, h, J, K):
     return np.linalg.eigvalsh(dense_hamiltonian(N, h, J))[:K] / N
 

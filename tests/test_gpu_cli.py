@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 import scipy.io as sio
 
-from helpers import load_J, random_weights
+from helpers import hs_J, load_J, random_weights, xxz_J
 
 pytestmark = pytest.mark.gpu
 
@@ -90,6 +90,30 @@ def test_cli_matches_python_mirror_and_writes_files(exe, tmp_path):
     # <sigma^x_i sigma^x_i> = <sigma^z_i sigma^z_i> = 1 for every state
     assert np.allclose(c["xx"][np.arange(N), np.arange(N)], 1.0, atol=1e-10)
     assert np.allclose(c["zz"][np.arange(N), np.arange(N)], 1.0, atol=1e-10)
+
+
+@pytest.mark.parametrize("model,mid", [("xxz", 2), ("hs", 1)])
+def test_cli_pairflip_models_match_mirror(exe, tmp_path, model, mid):
+    """--model xxz / hs: couplings of rbm.cc:1066-1089 built by the executable and the
+    pair-flip local energy; the energy lines equal the Python mirror's loop over a
+    Solver with the test's own hs_J / xxz_J."""
+    import nqes
+    N, M, K = 8, 16, 2
+    W = random_weights(N, M, K, seed=8, scale=0.1)
+    weights_to_mat(str(tmp_path / "w0.mat"), W, N, M, K)
+    cmd = [exe, "--model", model, "-h", "0", "-c", "16", "-m", "20", "--seed", "77", "-n", "3",
+           "--reglam-init", "40", "--reglam-cutoff", "1", "--minres-rtol", "1e-9",
+           "--correlation-data", str(tmp_path / "corr.mat"), "--load-rbm", str(tmp_path / "w0.mat")]
+    print("run:", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    printed = [[float(x) for x in ln.split()] for ln in r.stdout.splitlines()[1:4]]
+    J = hs_J(N) if model == "hs" else xxz_J(N)
+    with nqes.Solver(N, M, K, 0.0, J, model=mid) as s:
+        hist, _, _ = nqes.run(s, W.copy(), 77, 16, 20, 64, 3, reglam0=40, reglam_cutoff=1, rtol=1e-9)
+    assert len(printed) == 3
+    for a, b in zip(printed, hist):
+        assert np.allclose(a, b, rtol=1e-5, atol=0), (a, b)
 
 
 def test_cli_gauge_and_single_rbm(exe, tmp_path):

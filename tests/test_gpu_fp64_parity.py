@@ -10,7 +10,7 @@ tolerance tied to rtol.
 import numpy as np
 import pytest
 
-from helpers import load_J, random_weights, rel_err
+from helpers import hs_J, load_J, random_weights, rel_err, xxz_J
 
 pytestmark = pytest.mark.gpu
 
@@ -32,8 +32,8 @@ def nqes():
     return mod
 
 
-def oracle_samples(oracle, N, M, K, h, J, W, seed, C, m, B):
-    outs = [oracle.run_chain(N, M, K, h, J, W, seed, c, m, B, want_O=True) for c in range(C)]
+def oracle_samples(oracle, N, M, K, h, J, W, seed, C, m, B, model=0):
+    outs = [oracle.run_chain(N, M, K, h, J, W, seed, c, m, B, want_O=True, model=model) for c in range(C)]
     spins = np.concatenate([o["spins"] for o in outs])
     H = np.concatenate([o["H"] for o in outs])
     O = np.concatenate([o["O"] for o in outs])
@@ -57,6 +57,27 @@ def test_sampler_bitexact(oracle, nqes, N, M, K, C, m, B):
     assert np.array_equal(sp_g, sp_o)
     assert rel_err(H_g, H_o) == 0.0, rel_err(H_g, H_o)
     assert rel_err(O_g, O_o) == 0.0, rel_err(O_g, O_o)
+
+
+@pytest.mark.parametrize("model", [1, 2])
+@pytest.mark.parametrize("N,M,K,C,m,B", [(8, 16, 2, 4, 3, 2), (6, 12, 3, 3, 2, 1), (5, 70, 1, 3, 2, 1),
+                                        (40, 80, 2, 2, 1, 1)])
+def test_sampler_bitexact_pairflip(oracle, nqes, model, N, M, K, C, m, B):
+    """hs (model 1) and xxz (model 2) local energies (rbm.cc:1092-1119): spins, H_loc
+    and O bit-identical with the oracle (M = 70 puts two hidden units on some lanes)."""
+    print(f"case model={model} N={N} M={M} K={K} C={C} m={m} B={B}", flush=True)
+    J = hs_J(N) if model == 1 else xxz_J(N)
+    W = random_weights(N, M, K, seed=N * 10 + M + K + model, scale=0.15)
+    seed = 4242
+    sp_o, H_o, O_o, acc_o = oracle_samples(oracle, N, M, K, 0.0, J, W, seed, C, m, B, model=model)
+    with nqes.Solver(N, M, K, 0.0, J, model=model) as s:
+        s.sample(W, seed, C, m, B, reglam=0.5)
+        sp_g, H_g, acc_g = s.samples()
+        O_g = s.O_rows(0, C * m)
+    assert np.array_equal(acc_g, acc_o)
+    assert np.array_equal(sp_g, sp_o)
+    assert rel_err(H_g, H_o) == 0.0, rel_err(H_g, H_o)
+    assert rel_err(O_g, O_o) == 0.0
 
 
 @pytest.mark.parametrize("N,M,K,C,m,B", CASES[:4])

@@ -17,7 +17,7 @@ Tolerances:
 import numpy as np
 import pytest
 
-from helpers import load_J, random_weights, rel_err
+from helpers import load_J, random_weights, rel_err, hs_J, xxz_J
 
 pytestmark = pytest.mark.gpu
 
@@ -167,6 +167,30 @@ def test_q8_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
     assert rel_err(a12, a1 + 2.5 * a2) < 1e-11, rel_err(a12, a1 + 2.5 * a2)
     sym = abs(np.vdot(v2, a1) - np.vdot(v1, a2)) / (np.linalg.norm(v1) * np.linalg.norm(a2))
     assert sym < 1e-11, sym
+
+
+@pytest.mark.parametrize("model", [1, 2])
+@pytest.mark.parametrize("cplx", [False, True])
+def test_mixed_pairflip_matches_fp64(nqes, model, cplx):
+    """hs / xxz local energies on the mixed path (FP32 angles and pair-flip ratios,
+    FP64 sums) against the FP64 path on the same chains: H_loc samples to 1e-4, one SR
+    step's energies to 1e-4."""
+    N, M, K, C, m, B = 10, 20, 2, 32, 8, 4
+    J = hs_J(N) if model == 1 else xxz_J(N)
+    W = random_weights(N, M, K, seed=31 + model, scale=0.1, complex_=cplx)
+    res = {}
+    for prec in (nqes.PREC_FP64, nqes.PREC_MIXED):
+        with nqes.Solver(N, M, K, 0.0, J, precision=prec, model=model) as s:
+            s.sample(W, 17, C, m, B, 1.0)
+            sp, H, _ = s.samples()
+            Wn = W.copy()
+            emat = s.monte_carlo(Wn, 17, C, m, B, 0.05, 2.0, 1e-9)
+            res[prec] = (sp, H, emat)
+    (sa, Ha, ea), (sb, Hb, eb) = res[nqes.PREC_FP64], res[nqes.PREC_MIXED]
+    same = np.all(sa.reshape(C * m, -1) == sb.reshape(C * m, -1), axis=1)
+    assert same.mean() > 0.9
+    assert rel_err(Hb[same], Ha[same]) < 1e-4
+    assert rel_err(eb, ea) < 1e-4
 
 
 @pytest.mark.parametrize("cplx", [False, True])

@@ -10,7 +10,8 @@ from fractions import Fraction
 import numpy as np
 import pytest
 
-from helpers import dense_hamiltonian, ed_energies, load_J, random_weights, rel_err, synthetic_J
+from helpers import (dense_hamiltonian, dense_heisenberg, ed_energies, hs_J, load_J, random_weights, rel_err,
+                     synthetic_J, xxz_J)
 
 PCG_INC = (6364136223846793005 << 64) | 1442695040888963407
 PCG_MULT = (2549297995355413924 << 64) | 4865540595714422341
@@ -156,7 +157,7 @@ def test_ezz(oracle):
 
 
 # ------------------------------------------------------------------ exact enumeration vs ED
-def enumerate_all(oracle, N, M, K, h, J, W):
+def enumerate_all(oracle, N, M, K, h, J, W, model=0):
     """|det Phi|^2-weighted sums over all 2^(K N) multi-spin configurations."""
     tot_w, tot_H = 0.0, np.zeros((K, K), complex)
     Os = []
@@ -168,7 +169,7 @@ def enumerate_all(oracle, N, M, K, h, J, W):
         if not ok:
             continue
         wgt = abs(np.linalg.det(phi)) ** 2
-        H = oracle.hloc(N, M, K, h, J, W, sp, phi, phi_inv, tanh)
+        H = oracle.hloc(N, M, K, h, J, W, sp, phi, phi_inv, tanh, model=model)
         tot_w += wgt
         tot_H += wgt * H
         Os.append((wgt, H, oracle.deriv(N, M, K, sp, phi, phi_inv, tanh)))
@@ -204,6 +205,38 @@ def test_hloc_expectation_equals_rayleigh_ritz(oracle, N, M, K):
     # a transposed estimator would not pass
     if K > 1:
         assert rel_err(Hmean, R.T) > 1e-3
+
+
+@pytest.mark.parametrize("model,delta", [(1, 1.0), (2, -1.0)])
+@pytest.mark.parametrize("N,M,K", [(6, 6, 1), (4, 4, 2), (3, 3, 3)])
+def test_hs_hloc_expectation_equals_rayleigh_ritz(oracle, model, delta, N, M, K):
+    # the same identity for the pair-flip local energy of rbm.cc:1092-1119 with the
+    # dense sum_{i>j} J_ij (XX + YY + delta ZZ); random dense couplings exercise every
+    # pair, the model couplings (hs: all pairs, xxz: ring bonds) the shipped ones
+    rng = np.random.default_rng(5 * N + K + model)
+    W = random_weights(N, M, K, seed=13 * N + K + model, scale=0.3)
+    A = rbm_vectors(N, M, K, W)
+    G = A.conj().T @ A
+    for J in (np.tril(rng.normal(size=(N, N)), -1), hs_J(N) if model == 1 else xxz_J(N)):
+        Hmean, _, _ = enumerate_all(oracle, N, M, K, 0.3, J, W, model=model)
+        R = np.linalg.solve(G, A.conj().T @ dense_heisenberg(N, J, delta) @ A)
+        assert rel_err(Hmean, R) < 1e-10, (Hmean, R)
+        if K > 1:
+            assert rel_err(Hmean, R.T) > 1e-3
+
+
+def test_hs_xxz_couplings_and_spectra():
+    # hs_model / xxz_model (rbm.cc:1066-1089) and textbook spectra of the chains they
+    # define: Haldane-Shastry ground state E0 = -(pi^2/24)(N + 5/N) in units of
+    # sum J (S.S) with S = sigma/2 (here H = 4 sum J S.S), and the XXZ ring at
+    # delta = -1 is -sum (XX + YY - ZZ): by the sublattice rotation equal to the
+    # Heisenberg antiferromagnet sum (XX + YY + ZZ), whose N = 4 ground state is -8
+    for N in (4, 6, 8):
+        E0 = np.linalg.eigvalsh(dense_heisenberg(N, hs_J(N), 1.0))[0]
+        assert abs(E0 - 4 * (-(np.pi ** 2 / 24) * (N + 5.0 / N))) < 1e-9, (N, E0)
+    assert abs(np.linalg.eigvalsh(dense_heisenberg(4, xxz_J(4), -1.0))[0] + 8.0) < 1e-12
+    J = xxz_J(6)
+    assert J[1, 0] == -1.0 and J[5, 0] == -1.0 and J[5, 4] == -1.0 and np.count_nonzero(J) == 6
 
 
 def test_ed_matches_published_values():
