@@ -138,6 +138,14 @@ int nqes_correlations(nqes_ctx *ctx, const double *W, uint64_t seed, int C, int 
 int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision);
 
 /*
+ * S form of the following SR steps: 0 = matrix-free (default; rbm-matfree-N*,
+ * rbm.cc:661-684), 1 = dense P x P matrix oo = <conj(O) O^T> - conj(o) o^T formed
+ * after sampling and applied as skk .* v + oo v (the rbm-N* executables,
+ * rbm.cc:651-657, 686-696).  Dense needs NQES_PREC_FP64 and P*P*16 device bytes.
+ */
+int nqes_set_dense_S(nqes_ctx *ctx, int dense);
+
+/*
  * Solve server: continuous batching of whole solves on one device.
  *
  * A solve is the reference's run() (rbm.cc:1173-1197): up to niters SR steps with

@@ -1,5 +1,7 @@
 // cli.cpp -- rbm-matfree: drop-in replacement for the reference executables
-// rbm-matfree-N{N}-M{M}-K{K}[-opt] (code/build.sh:40-41) on top of libnqes.
+// rbm-matfree-N{N}-M{M}-K{K}[-opt] (code/build.sh:40-41) and, installed under the
+// name rbm-N{N}-M{M}-K{K}[-opt], for the dense-S executables (code/build.sh:37-39),
+// on top of libnqes.
 //
 // Same options and semantics as the reference's CmdArgs (rbm.cc:900-953, parsed
 // with the rules of code/argparse.h:435-554), same stdout (seed line, one line of
@@ -400,6 +402,7 @@ int main(int argc, char **argv)
         if (!M) M = std::stoi(mt[2]);
         if (!K) K = std::stoi(mt[3]);
     }
+    const bool dense_name = base.rfind("rbm-N", 0) == 0;
     if (!N || !M || !K) {
         std::cerr << "N, M, K unknown: name the executable rbm-matfree-N<N>-M<M>-K<K> or pass "
                      "--num-spins/--num-hidden/--num-states"
@@ -501,6 +504,9 @@ int main(int argc, char **argv)
             throw std::runtime_error(nqes_last_error(ctx));
         };
         check(nqes_problem_init(ctx, N, M, K, h, J.data(), model_id(model), precision));
+        // installed as rbm-N<N>-M<M>-K<K>[-opt] (the reference's dense-S build, code/build.sh:37-39):
+        // S as a dense matrix on the FP64 path
+        if (dense_name && precision == NQES_PREC_FP64) check(nqes_set_dense_S(ctx, 1));
 
         srloop::Schedule sched{lrate0, lratef, lrate_cut, reglam0, reglam_decay, reglam_cut};
         srloop::Convergence conv_test(K);

@@ -29,6 +29,10 @@ struct Fp64SampleArgs {
 hipError_t nq_launch_tables64(int N, int M, int K, int L, const void *W, void *chT, void *shT, void *eaP,
                               void *eaN, hipStream_t st);
 hipError_t nq_launch_sample64(int K, const Fp64SampleArgs &a, hipStream_t st);
+// dense S (FP64): oo = <conj(O) O^T> - conj(o) o^T, and out = skk .* v + oo v
+hipError_t nq_launch_dense_gram64(const void *O, long long S, long long P, const void *o, void *oo, hipStream_t st);
+hipError_t nq_launch_dense_apply64(const void *oo, const void *skk, const void *v, long long P, void *out,
+                                   hipStream_t st);
 hipError_t nq_launch_corr64(int K, const Fp64SampleArgs &a, hipStream_t st);
 hipError_t nq_launch_moments64(const void *O, const void *E, long long S, long long P, int nchunk, void *po,
                                void *ps, void *pf, void *out_o, void *out_s, void *out_f, hipStream_t st);

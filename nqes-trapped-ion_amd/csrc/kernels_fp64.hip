@@ -433,6 +433,43 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
     if (lane == 0) a.accepts[c] = accepts;
 }
 
+// Dense S of the rbm-N* executables (rbm.cc:651-657, 686-696): oo[l][l'] =
+// (1/S) sum_s conj(O_sl) O_sl' - conj(o_l) o_l'.  One 16 x 16 tile per block, samples
+// in chunks of 16 staged in LDS, each element summed in ascending sample order.
+__global__ __launch_bounds__(256) void k_dense_gram64(const cd *__restrict__ O, long long S, long long P,
+                                                      const cd *__restrict__ o, cd *__restrict__ oo)
+{
+    __shared__ cd A[16][17], B[16][17];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const long long l0 = (long long)blockIdx.y * 16, m0 = (long long)blockIdx.x * 16;
+    cd acc = zero();
+    for (long long s0 = 0; s0 < S; s0 += 16) {
+        const long long s = s0 + ty;
+        A[ty][tx] = (s < S && l0 + tx < P) ? O[s * P + l0 + tx] : zero();
+        B[ty][tx] = (s < S && m0 + tx < P) ? O[s * P + m0 + tx] : zero();
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc = add(acc, mul(conj(A[q][ty]), B[q][tx]));
+        __syncthreads();
+    }
+    const long long l = l0 + ty, m = m0 + tx;
+    if (l < P && m < P) oo[l * P + m] = sub(divr(acc, (double)S), mul(conj(o[l]), o[m]));
+}
+
+// Dense operator (rbm.cc:686-696): out[l] = skk[l] v[l] + sum_l' oo[l][l'] v[l'].  One
+// wavefront per row, lanes over l', butterfly sum.
+__global__ __launch_bounds__(256) void k_dense_apply64(const cd *__restrict__ oo, const cd *__restrict__ skk,
+                                                       const cd *__restrict__ v, long long P, cd *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const long long l = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (l >= P) return;
+    cd acc = zero();
+    for (long long q = lane; q < P; q += 64) acc = add(acc, mul(oo[l * P + q], v[q]));
+    acc = wave_sum(acc);
+    if (lane == 0) out[l] = add(mul(skk[l], v[l]), acc);
+}
+
 // accumulate the K x K matrix Phi^+ X into a chain's correlation slot (lane q
 // owns entry q)
 template <int K>
@@ -700,6 +737,21 @@ hipError_t nq_launch_sample64(int K, const Fp64SampleArgs &a, hipStream_t st)
     case 4: return launch_sample_k<4>(a, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t nq_launch_dense_gram64(const void *O, long long S, long long P, const void *o, void *oo, hipStream_t st)
+{
+    const unsigned nt = (unsigned)((P + 15) / 16);
+    hipLaunchKernelGGL(k_dense_gram64, dim3(nt, nt), dim3(256), 0, st, (const cd *)O, S, P, (const cd *)o, (cd *)oo);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_dense_apply64(const void *oo, const void *skk, const void *v, long long P, void *out,
+                                   hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dense_apply64, dim3((unsigned)((P + 3) / 4)), dim3(256), 0, st, (const cd *)oo,
+                       (const cd *)skk, (const cd *)v, P, (cd *)out);
+    return hipGetLastError();
 }
 
 hipError_t nq_launch_moments64(const void *O, const void *E, long long S, long long P, int nchunk, void *po,

@@ -102,6 +102,8 @@ struct nqes_ctx {
     DevBuf dv, dov, dw, dsvpart, dsvwpart, dsvout, dsvscratch, dmscratch;
     DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
     DevBuf dUq, descale, dvq, dq8w, dq8m;              // its int8 form (real weights)
+    bool dense = false;                                // dense S (FP64): oo P x P
+    DevBuf doo, dvo, dskk;
     bool svf = false, q8 = false;
     SvfShape svf_shape{};
     Q8Shape q8_shape{};
@@ -306,6 +308,7 @@ int nqes_problem_init(nqes_ctx *c, int N, int M, int K, double h, const double *
         HIPCHK(c, c->dflags.ensure(16), "alloc flags");
     }
     c->has_problem = true;
+    c->dense = false;
     c->sampled = false;
     return NQES_OK;
 }
@@ -601,7 +604,26 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     c->svraw.resize(P + 1);
     c->sv_calls = 0;
     c->sv_ms = 0;
+    if (c->dense) {  // the dense S of rbm.cc:651-657 from the stored O rows
+        HIPCHK(c, c->doo.ensure((size_t)P * P * 16), "alloc dense S");
+        HIPCHK(c, c->dvo.ensure((size_t)P * 16), "alloc o");
+        HIPCHK(c, c->dskk.ensure((size_t)P * 16), "alloc skk");
+        HIPCHK(c, hipMemcpyAsync(c->dvo.p, c->o.data(), 16 * P, hipMemcpyHostToDevice, c->stream), "copy o");
+        HIPCHK(c, hipMemcpyAsync(c->dskk.p, c->skk.data(), 16 * P, hipMemcpyHostToDevice, c->stream), "copy skk");
+        HIPCHK(c, nq_launch_dense_gram64(c->dO.p, S, P, c->dvo.p, c->doo.p, c->stream), "dense S kernel");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "dense S sync");
+    }
     c->sampled = true;
+    return NQES_OK;
+}
+
+int nqes_set_dense_S(nqes_ctx *c, int dense)
+{
+    if (!c) return NQES_EINVAL;
+    if (!c->has_problem) return c->fail(NQES_ESTATE, "no problem: call nqes_problem_init first");
+    if (dense && c->prec != NQES_PREC_FP64) return c->fail(NQES_EINVAL, "the dense S form needs NQES_PREC_FP64");
+    c->dense = dense != 0;
+    c->sampled = false;
     return NQES_OK;
 }
 
@@ -745,6 +767,17 @@ static int apply_S(nqes_ctx *c, const cd *v, cd *out)
     const double Sd = (double)c->S;
     cd ov = zero();
     for (long long q = 0; q < P; q++) ov = add(ov, mul(c->o[q], v[q]));
+    if (c->dense) {  // rbm.cc:686-696
+        HIPCHK(c, hipMemcpyAsync(c->dv.p, v, 16 * P, hipMemcpyHostToDevice, c->stream), "copy v");
+        HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
+        HIPCHK(c, nq_launch_dense_apply64(c->doo.p, c->dskk.p, c->dv.p, P, c->dsvout.p, c->stream), "dense S kernel");
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
+        HIPCHK(c, hipMemcpyAsync(out, c->dsvout.p, 16 * P, hipMemcpyDeviceToHost, c->stream), "copy S*v");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
+        c->sv_ms += event_ms(c->ev[2], c->ev[3]);
+        c->sv_calls++;
+        return NQES_OK;
+    }
     if (c->prec == NQES_PREC_FP64) {
         HIPCHK(c, hipMemcpyAsync(c->dv.p, v, 16 * P, hipMemcpyHostToDevice, c->stream), "copy v");
         HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");

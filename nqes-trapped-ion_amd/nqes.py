@@ -96,6 +96,7 @@ EXPORTS = [
     "nqes_create", "nqes_destroy", "nqes_last_error", "nqes_problem_init", "nqes_sr_step",
     "nqes_sample", "nqes_get_samples", "nqes_get_O", "nqes_get_moments", "nqes_apply_S",
     "nqes_solve", "nqes_correlations", "nqes_required_bytes", "nqes_version",
+    "nqes_set_dense_S",
     "nqes_server_create", "nqes_server_destroy", "nqes_server_last_error", "nqes_server_submit",
     "nqes_server_poll", "nqes_server_wait",
 ]
@@ -138,6 +139,8 @@ def load_library(path=LIB_PATH):
     L.nqes_required_bytes.argtypes = [c_int, c_int, c_int, c_int, c_int, c_int]
     L.nqes_version.restype = ctypes.c_char_p
     L.nqes_version.argtypes = []
+    L.nqes_set_dense_S.restype = c_int
+    L.nqes_set_dense_S.argtypes = [_vp, c_int]
     L.nqes_server_create.restype = _vp
     L.nqes_server_create.argtypes = [c_int, c_int, c_i64]
     L.nqes_server_destroy.argtypes = [_vp]
@@ -208,6 +211,11 @@ class Solver:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_dense_S(self, dense=True):
+        """S as a dense P x P matrix (the reference's rbm-N* executables) instead of
+        the matrix-free operator; FP64 only."""
+        self._check(self.lib.nqes_set_dense_S(self.ctx, int(bool(dense))))
 
     # ---------------------------------------------------------------- SR step
     def monte_carlo(self, W, seed, C, m, B, lr, reglam, rtol, maxit=2**64 - 1, stats=False):

@@ -116,6 +116,24 @@ def test_cli_pairflip_models_match_mirror(exe, tmp_path, model, mid):
         assert np.allclose(a, b, rtol=1e-5, atol=0), (a, b)
 
 
+def test_cli_dense_name_selects_dense_S(tmp_path):
+    """Installed as rbm-N8-M16-K2-opt (the reference's dense-S build) the executable
+    solves with the dense S: same samples, so the first energy line is identical to
+    the matrix-free executable's and later lines agree to MINRES-QLP rounding."""
+    N, M, K = 8, 16, 2
+    W = random_weights(N, M, K, seed=9, scale=0.1)
+    weights_to_mat(str(tmp_path / "w0.mat"), W, N, M, K)
+    outs = {}
+    for name in ("rbm-N8-M16-K2-opt", "rbm-matfree-N8-M16-K2-opt"):
+        link = tmp_path / name
+        os.symlink(BIN, link)
+        r = run_cli(str(link), tmp_path, ["-n", "3", "--load-rbm", str(tmp_path / "w0.mat"), "--stats"])
+        outs[name] = [[float(x) for x in ln.split()] for ln in r.stdout.splitlines()[1:4]]
+    a, b = outs["rbm-N8-M16-K2-opt"], outs["rbm-matfree-N8-M16-K2-opt"]
+    assert a[0] == b[0]
+    assert np.allclose(a, b, rtol=1e-6, atol=0)
+
+
 def test_cli_gauge_and_single_rbm(exe, tmp_path):
     N, M, K = 8, 16, 2
     W = random_weights(N, M, K, seed=6, scale=0.1)

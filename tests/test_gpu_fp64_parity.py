@@ -110,3 +110,37 @@ def test_moments_sv_step(oracle, nqes, N, M, K, C, m, B):
     assert rel_err(emat_g, emat_o) < 1e-12
     assert rel_err(Wg - W, Wo - W) < 1e-6
     assert st["accepts"] == info["accepts"]
+
+
+@pytest.mark.parametrize("N,M,K,C,m,B", [(8, 16, 2, 16, 8, 4), (6, 12, 3, 8, 6, 2), (4, 8, 1, 10, 5, 2)])
+def test_dense_S_matches_matrix_free(nqes, N, M, K, C, m, B):
+    """The dense S of the rbm-N* executables (rbm.cc:651-657, 686-696) is the same
+    operator as the matrix-free one: S*v equal to 1e-12 on the same samples, and equal
+    to skk v + oo v with oo = O^H O / S - conj(o) o^T built in numpy from the O rows;
+    one SR step gives the same e_mat and weights to MINRES-QLP rounding."""
+    print(f"case N={N} M={M} K={K} C={C} m={m} B={B}", flush=True)
+    h, reglam = 0.595, 0.7
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=3 * N + K, scale=0.15)
+    rng = np.random.default_rng(2)
+    P = K * (N + M + N * M)
+    v = rng.normal(size=P) + 1j * rng.normal(size=P)
+    out = {}
+    for dense in (False, True):
+        with nqes.Solver(N, M, K, h, J) as s:
+            if dense:
+                s.set_dense_S(True)
+            s.sample(W, 55, C, m, B, reglam)
+            mom = s.moments()
+            O = s.O_rows(0, C * m)
+            Wn = W.copy()
+            emat = s.monte_carlo(Wn, 55, C, m, B, 0.1, reglam, 1e-11)
+            s.sample(W, 55, C, m, B, reglam)
+            out[dense] = (s.apply_S(v), emat, Wn)
+    o = O.mean(axis=0)
+    oo = O.conj().T @ O / (C * m) - np.outer(o.conj(), o)
+    ref = mom["skk"] * v + oo @ v
+    assert rel_err(out[True][0], ref) < 1e-12
+    assert rel_err(out[True][0], out[False][0]) < 1e-12
+    assert np.array_equal(out[True][1], out[False][1])
+    assert rel_err(out[True][2] - W, out[False][2] - W) < 1e-8
