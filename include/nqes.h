@@ -146,6 +146,18 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision);
 int nqes_set_dense_S(nqes_ctx *ctx, int dense);
 
 /*
+ * Exhaustive evaluation behind the reference's --dump-states/ham/deriv files
+ * (dump_states, rbm.cc:474-526, 1149-1172; needs N*K < 32): for every multi-spin
+ * configuration cfg in [cfg0, cfg0 + ncfg) (ncfg <= 2^24; replica k's spins are bits
+ * [k N, k N + N) of cfg) with weights W, on the FP64 path: phi (ncfg x K*K complex,
+ * state_fcn), res (ncfg x K*K: the local-energy numerator of the model, H_loc =
+ * Phi^+ res, rbm.cc:1042-1064 / 1092-1119) and tanh (ncfg x K*K*M, tanh(theta) of RBM
+ * kk at replica k, [k][kk][j]).  Any output may be NULL.  Invalidates the samples.
+ */
+int nqes_enumerate(nqes_ctx *ctx, const double *W, int64_t cfg0, int64_t ncfg, double *phi, double *res,
+                   double *tanh_);
+
+/*
  * Solve server: continuous batching of whole solves on one device.
  *
  * A solve is the reference's run() (rbm.cc:1173-1197): up to niters SR steps with

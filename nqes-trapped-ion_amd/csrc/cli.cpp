@@ -330,6 +330,93 @@ std::string format_row(const std::vector<double> &v)
 
 using srloop::sorted_real_eigs;
 
+// dump_states (rbm.cc:474-526): variable s, complex, dims [2^(K N), rows, cols] with
+// unit dimensions dropped, column-major; fcn(cfg) is Phi (what 0), H_loc (1) or O (2)
+// of the multi-spin configuration cfg (replica k = bits [k N, k N + N)).  Phi^+ is
+// the pseudo-inverse, as in rbm.cc:1158, 1167.
+void dump_configs(nqes_ctx *ctx, const std::string &prefix, int what, int N, int M, int K,
+                  const std::vector<cplx> &W)
+{
+    const int64_t dim0 = 1ll << (N * K), L = N + M + (int64_t)N * M, P = K * L;
+    const int rows = what == 2 ? (int)P : K, cols = what == 2 ? 1 : K;
+    std::vector<int64_t> dims{dim0};
+    if (rows > 1) dims.push_back(rows);
+    if (cols > 1) dims.push_back(cols);
+    matv5::Var v{"s", dims, true, 6, {}, {}};
+    v.re.assign((size_t)v.numel(), 0.0);
+    v.im.assign((size_t)v.numel(), 0.0);
+    const int64_t batch = 1 << 16;
+    const size_t kk = (size_t)K * K;
+    std::vector<cplx> phi(batch * kk), res(what == 1 ? batch * kk : 0), th(what == 2 ? batch * kk * M : 0);
+    std::vector<cplx> pinv(kk), G(kk), X(kk), val((size_t)rows * cols), colk(K);
+    for (int64_t c0 = 0; c0 < dim0; c0 += batch) {
+        const int64_t nb = std::min(batch, dim0 - c0);
+        int st = nqes_enumerate(ctx, reinterpret_cast<const double *>(W.data()), c0, nb,
+                                reinterpret_cast<double *>(phi.data()),
+                                what == 1 ? reinterpret_cast<double *>(res.data()) : nullptr,
+                                what == 2 ? reinterpret_cast<double *>(th.data()) : nullptr);
+        if (st != NQES_OK) throw std::runtime_error(nqes_last_error(ctx));
+        for (int64_t e = 0; e < nb; e++) {
+            const cplx *ph = phi.data() + e * kk;
+            if (what == 0) {
+                for (size_t q = 0; q < kk; q++) val[q] = ph[q];
+            } else {
+                eigk::pinv(K, ph, pinv.data());
+                if (what == 1) {
+                    const cplx *r = res.data() + e * kk;
+                    for (int i = 0; i < K; i++)
+                        for (int j = 0; j < K; j++) {
+                            cplx acc(0.0, 0.0);
+                            for (int l = 0; l < K; l++) acc += pinv[i * K + l] * r[l * K + j];
+                            val[i * K + j] = acc;
+                        }
+                } else {
+                    // deriv (rbm.cc:199-239): det(Phi^+ derPhi); columns != k of Phi^+ derPhi
+                    // are those of G = Phi^+ Phi
+                    const uint64_t cfg = (uint64_t)(c0 + e);
+                    const cplx *t = th.data() + e * kk * M;
+                    for (int i = 0; i < K; i++)
+                        for (int j = 0; j < K; j++) {
+                            cplx acc(0.0, 0.0);
+                            for (int l = 0; l < K; l++) acc += pinv[i * K + l] * ph[l * K + j];
+                            G[i * K + j] = acc;
+                        }
+                    auto spin = [&](int kk2, int i) { return (cfg >> (kk2 * N + i)) & 1u; };
+                    for (int64_t p = 0; p < P; p++) {
+                        const int k = (int)(p / L);
+                        const int64_t r = p % L;
+                        for (int kk2 = 0; kk2 < K; kk2++) {
+                            const cplx f = ph[kk2 * K + k];
+                            if (r < N) colk[kk2] = spin(kk2, (int)r) ? -f : f;
+                            else if (r < N + M) colk[kk2] = f * t[(kk2 * K + k) * M + (r - N)];
+                            else {
+                                const int i = (int)((r - N - M) / M), j = (int)((r - N - M) % M);
+                                const cplx fv = f * t[(kk2 * K + k) * M + j];
+                                colk[kk2] = spin(kk2, i) ? -fv : fv;
+                            }
+                        }
+                        for (int i = 0; i < K; i++) {
+                            cplx y(0.0, 0.0);
+                            for (int l = 0; l < K; l++) y += pinv[i * K + l] * colk[l];
+                            for (int j = 0; j < K; j++) X[i * K + j] = (j == k) ? y : G[i * K + j];
+                        }
+                        val[p] = eigk::det(K, X.data());
+                    }
+                }
+            }
+            const int64_t cfg = c0 + e;
+            for (int i = 0; i < rows; i++)
+                for (int j = 0; j < cols; j++) {
+                    const size_t dst = (size_t)j * rows * dim0 + (size_t)i * dim0 + cfg;
+                    v.re[dst] = val[(size_t)i * cols + j].real();
+                    v.im[dst] = val[(size_t)i * cols + j].imag();
+                }
+        }
+    }
+    matv5::write_file(prefix + "N" + std::to_string(N) + "M" + std::to_string(M) + "K" + std::to_string(K) + ".mat",
+                      {v});
+}
+
 [[noreturn]] void fp_error()
 {
     std::cerr << "Floating point error occured. Exiting." << std::endl;
@@ -364,6 +451,9 @@ int main(int argc, char **argv)
     o.add("apply-gauge", "Apply a_i -> a_i - i pi/2 (odd i) to dumped weights", "false", true);
     o.add("inverse-gauge", "With --apply-gauge: a_i -> a_i + i pi/2 (odd i) on loaded weights", "false", true);
     o.add("correlation-data", "File name for the correlation function data", std::nullopt);
+    o.add("dump-states", "If N*K < 32: dump Phi of every spin configuration to <prefix>N<N>M<M>K<K>.mat", "");
+    o.add("dump-ham", "If N*K < 32: dump H_loc of every spin configuration", "");
+    o.add("dump-deriv", "If N*K < 32: dump the derivatives O of every spin configuration", "");
     // extensions of this build
     o.add("num-spins", "N (default: from the executable name)", "0");
     o.add("num-hidden", "M (default: from the executable name)", "0");
@@ -409,6 +499,12 @@ int main(int argc, char **argv)
                   << std::endl;
         return 255;
     }
+    // the reference defines the dump options only when N*K < 32 (rbm.cc:943-953)
+    for (const char *d : {"dump-states", "dump-ham", "dump-deriv"})
+        if (N * K >= 32 && !o.get(d).empty()) {
+            std::cerr << "Unrecognized argument --" << d << " (only for N*K < 32)" << std::endl;
+            return 255;
+        }
     std::string model = o.get("model");
     for (auto &ch : model) ch = (char)std::tolower((unsigned char)ch);
     const double h = conv<double>(o, "h", to_dbl);
@@ -507,6 +603,10 @@ int main(int argc, char **argv)
         // installed as rbm-N<N>-M<M>-K<K>[-opt] (the reference's dense-S build, code/build.sh:37-39):
         // S as a dense matrix on the FP64 path
         if (dense_name && precision == NQES_PREC_FP64) check(nqes_set_dense_S(ctx, 1));
+        // --dump-states/ham/deriv with the initial weights (rbm.cc:1149-1172)
+        const std::string dumps[3] = {o.get("dump-states"), o.get("dump-ham"), o.get("dump-deriv")};
+        for (int what = 0; what < 3; what++)
+            if (!dumps[what].empty()) dump_configs(ctx, dumps[what], what, N, M, K, W);
 
         srloop::Schedule sched{lrate0, lratef, lrate_cut, reglam0, reglam_decay, reglam_cut};
         srloop::Convergence conv_test(K);

@@ -162,4 +162,82 @@ bool eig(int n, const cplx *A, cplx *evals, cplx *V)
     return ok;
 }
 
+int pinv(int n, const cplx *A, cplx *X, double rtol)
+{
+    // B = A (columns b_p), V = I; rotate column pairs until mutually orthogonal
+    std::vector<cplx> B(A, A + (size_t)n * n), V((size_t)n * n, cplx(0.0, 0.0));
+    for (int i = 0; i < n; i++) V[(size_t)i * n + i] = 1.0;
+    auto col = [&](std::vector<cplx> &Mx, int r, int c) -> cplx & { return Mx[(size_t)r * n + c]; };
+    for (int sweep = 0; sweep < 60; sweep++) {
+        bool rotated = false;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++) {
+                double alpha = 0.0, beta = 0.0;
+                cplx gamma(0.0, 0.0);
+                for (int r = 0; r < n; r++) {
+                    alpha += std::norm(col(B, r, p));
+                    beta += std::norm(col(B, r, q));
+                    gamma += std::conj(col(B, r, p)) * col(B, r, q);
+                }
+                const double ag = std::abs(gamma);
+                if (ag == 0.0 || ag <= 1e-15 * std::sqrt(alpha * beta)) continue;
+                rotated = true;
+                const cplx ph = gamma / ag;  // b_q conj(ph) has a real overlap with b_p
+                const double zeta = (beta - alpha) / (2.0 * ag);
+                const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::abs(zeta) + std::sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / std::sqrt(1.0 + t * t), s = c * t;
+                for (int r = 0; r < n; r++) {
+                    const cplx bp = col(B, r, p), bq = col(B, r, q) * std::conj(ph);
+                    col(B, r, p) = c * bp - s * bq;
+                    col(B, r, q) = s * bp + c * bq;
+                    const cplx vp = col(V, r, p), vq = col(V, r, q) * std::conj(ph);
+                    col(V, r, p) = c * vp - s * vq;
+                    col(V, r, q) = s * vp + c * vq;
+                }
+            }
+        if (!rotated) break;
+    }
+    // A V = B = U S: s_p = |b_p|, u_p = b_p / s_p; pinv(A) = V S^+ U^H = sum_p v_p b_p^H / s_p^2
+    std::vector<double> sv(n);
+    double smax = 0.0;
+    for (int p = 0; p < n; p++) {
+        double a2 = 0.0;
+        for (int r = 0; r < n; r++) a2 += std::norm(col(B, r, p));
+        sv[p] = std::sqrt(a2);
+        smax = std::max(smax, sv[p]);
+    }
+    for (int i = 0; i < n * n; i++) X[i] = 0.0;
+    int rank = 0;
+    for (int p = 0; p < n; p++) {
+        if (!(sv[p] > rtol * smax)) continue;
+        rank++;
+        const double inv2 = 1.0 / (sv[p] * sv[p]);
+        for (int r = 0; r < n; r++)
+            for (int c = 0; c < n; c++) X[(size_t)r * n + c] += col(V, r, p) * std::conj(col(B, c, p)) * inv2;
+    }
+    return rank;
+}
+
+cplx det(int n, const cplx *A)
+{
+    std::vector<cplx> L(A, A + (size_t)n * n);
+    cplx d(1.0, 0.0);
+    for (int k = 0; k < n; k++) {
+        int piv = k;
+        for (int r = k + 1; r < n; r++)
+            if (std::abs(L[(size_t)r * n + k]) > std::abs(L[(size_t)piv * n + k])) piv = r;
+        if (L[(size_t)piv * n + k] == cplx(0.0, 0.0)) return cplx(0.0, 0.0);
+        if (piv != k) {
+            for (int c = 0; c < n; c++) std::swap(L[(size_t)k * n + c], L[(size_t)piv * n + c]);
+            d = -d;
+        }
+        d *= L[(size_t)k * n + k];
+        for (int r = k + 1; r < n; r++) {
+            const cplx f = L[(size_t)r * n + k] / L[(size_t)k * n + k];
+            for (int c = k; c < n; c++) L[(size_t)r * n + c] -= f * L[(size_t)k * n + c];
+        }
+    }
+    return d;
+}
+
 } // namespace eigk

@@ -17,4 +17,13 @@ using cplx = std::complex<double>;
 // is a unit eigenvector for evals[i].  Returns false if QR did not converge.
 bool eig(int n, const cplx *A, cplx *evals, cplx *V);
 
+// Moore-Penrose pseudo-inverse of the n x n row-major A (n <= 16), the role of
+// Eigen's completeOrthogonalDecomposition().pseudoInverse() at rbm.cc:547, 608, 1158:
+// one-sided Jacobi SVD A = U S V^H (Hestenes), singular values below rtol * s_max
+// treated as zero.  X: n x n row-major.  Returns the numerical rank.
+int pinv(int n, const cplx *A, cplx *X, double rtol = 1e-12);
+
+// determinant of the n x n row-major A (LU with partial pivoting)
+cplx det(int n, const cplx *A);
+
 } // namespace eigk

@@ -29,6 +29,10 @@ struct Fp64SampleArgs {
 hipError_t nq_launch_tables64(int N, int M, int K, int L, const void *W, void *chT, void *shT, void *eaP,
                               void *eaN, hipStream_t st);
 hipError_t nq_launch_sample64(int K, const Fp64SampleArgs &a, hipStream_t st);
+// every multi-spin configuration [cfg0, cfg0 + ncfg) (N K < 32): Phi, the local-energy
+// numerator res (H_loc = Phi^+ res) and tanh(theta); outputs may be NULL
+hipError_t nq_launch_enum64(int K, const Fp64SampleArgs &a, long long cfg0, long long ncfg, void *phi, void *res,
+                           void *tanh_, hipStream_t st);
 // dense S (FP64): oo = <conj(O) O^T> - conj(o) o^T, and out = skk .* v + oo v
 hipError_t nq_launch_dense_gram64(const void *O, long long S, long long P, const void *o, void *oo, hipStream_t st);
 hipError_t nq_launch_dense_apply64(const void *oo, const void *skk, const void *v, long long P, void *out,

@@ -82,25 +82,14 @@ __global__ void k_tables64(int N, int M, int K, int L, const cd *__restrict__ W,
 
 // Chain start (rbm.cc:604-608): pcg64(seed + c), K*N Bernoulli draws
 // (rbm.cc:99-119), state_fcn (rbm.cc:148-173) and the inverse.
+// state_fcn (rbm.cc:148-173) of the spins in sp (LDS): tanh(theta) into tl, Phi and
+// its inverse
 template <int K, int JPL>
-__device__ inline void chain_start64(const Fp64SampleArgs &a, int c, cd *tl, uint32_t *sp, pcg &rng, cd *phi,
-                                     cd *phi_inv)
+__device__ inline void state64(const Fp64SampleArgs &a, cd *tl, const uint32_t *sp, cd *phi, cd *phi_inv)
 {
     const int N = a.N, M = a.M, NW = a.NW, L = a.L;
     const int lane = threadIdx.x;
     const cd *W = static_cast<const cd *>(a.W);
-    rng = pcg_seed(a.seed + (uint64_t)c);
-    for (int w = 0; w < K * NW; w++) {
-        uint32_t word = 0;
-        int k = w / NW, i0 = (w % NW) * 32;
-        for (int b = 0; b < 32 && i0 + b < N; b++) {
-            (void)k;
-            if (canonical(pcg_next(rng)) < 0.5) word |= 1u << b;
-        }
-        if (lane == 0) sp[w] = word;
-    }
-    __syncthreads();
-
     {
         cd part[K * K];
 #pragma unroll
@@ -143,7 +132,26 @@ __device__ inline void chain_start64(const Fp64SampleArgs &a, int c, cd *tl, uin
         invK<K>(phi, phi_inv);
     }
     __syncthreads();
+}
 
+template <int K, int JPL>
+__device__ inline void chain_start64(const Fp64SampleArgs &a, int c, cd *tl, uint32_t *sp, pcg &rng, cd *phi,
+                                     cd *phi_inv)
+{
+    const int N = a.N, NW = a.NW;
+    const int lane = threadIdx.x;
+    rng = pcg_seed(a.seed + (uint64_t)c);
+    for (int w = 0; w < K * NW; w++) {
+        uint32_t word = 0;
+        int k = w / NW, i0 = (w % NW) * 32;
+        for (int b = 0; b < 32 && i0 + b < N; b++) {
+            (void)k;
+            if (canonical(pcg_next(rng)) < 0.5) word |= 1u << b;
+        }
+        if (lane == 0) sp[w] = word;
+    }
+    __syncthreads();
+    state64<K, JPL>(a, tl, sp, phi, phi_inv);
 }
 
 // One Metropolis sweep (rbm.cc:530-552): proposals k < K, i < N in order.
@@ -225,8 +233,7 @@ __device__ inline void sweep64(const Fp64SampleArgs &a, cd *tl, uint32_t *sp, pc
 // left by the first; each product over hidden units is a lane product and a
 // butterfly, as in the sweep.  Pairs with J_ij == 0 add exact zeros and are skipped.
 template <int K, int JPL>
-__device__ inline void hloc_hs64(const Fp64SampleArgs &a, const cd *tl, const uint32_t *sp, const cd *phi,
-                                 const cd *phi_inv, cd *H)
+__device__ inline void hres_hs64(const Fp64SampleArgs &a, const cd *tl, const uint32_t *sp, const cd *phi, cd *res)
 {
     const int N = a.N, M = a.M, NW = a.NW;
     const int lane = threadIdx.x;
@@ -235,7 +242,6 @@ __device__ inline void hloc_hs64(const Fp64SampleArgs &a, const cd *tl, const ui
     const cd *eaP = static_cast<const cd *>(a.eaP);
     const cd *eaN = static_cast<const cd *>(a.eaN);
     const double delta = a.delta;
-    cd res[K * K];
 #pragma unroll
     for (int q = 0; q < K * K; q++) res[q] = zero();
     for (int i = 1; i < N; i++)
@@ -289,7 +295,68 @@ __device__ inline void hloc_hs64(const Fp64SampleArgs &a, const cd *tl, const ui
                 }
             }
         }
-    matmulK<K>(phi_inv, res, H);
+}
+
+// ising-like numerator (rbm.cc:1042-1064): res = diag(E_zz) Phi - h sum_i Phi^(i), E_zz
+// of replica (lane % K) (rbm.cc:1012-1031); all 64 lanes active
+template <int K, int JPL>
+__device__ inline void hres_ising64(const Fp64SampleArgs &a, const cd *tl, const uint32_t *sp, const cd *phi,
+                                    cd *res)
+{
+    const int N = a.N, M = a.M, NW = a.NW;
+    const int lane = threadIdx.x;
+    const cd *chT = static_cast<const cd *>(a.chT);
+    const cd *shT = static_cast<const cd *>(a.shT);
+    const cd *eaP = static_cast<const cd *>(a.eaP);
+    const cd *eaN = static_cast<const cd *>(a.eaN);
+    double ez;
+    {
+        int kr = lane % K;
+        double r = 0.0;
+        for (int i = 1; i < N; i++) {
+            int bi = spin_bit(sp, NW, kr, i);
+            for (int j = 0; j < i; j++) {
+                double v = a.J[(long long)i * N + j];
+                r += (bi ^ spin_bit(sp, NW, kr, j)) ? -v : v;
+            }
+        }
+        ez = r * 2.0;
+    }
+    cd sx[K * K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double ek = __shfl(ez, k);
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) res[k * K + kk] = scale(ek, phi[k * K + kk]);
+    }
+#pragma unroll
+    for (int q = 0; q < K * K; q++) sx[q] = zero();
+    for (int i = 0; i < N; i++) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int si = spin_bit(sp, NW, k, i);
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                cd part = one();
+#pragma unroll
+                for (int q = 0; q < JPL; q++) {
+                    int j = lane + 64 * q;
+                    if (j < M) {
+                        long long ti = ((long long)kk * N + i) * M + j;
+                        cd shv = shT[ti];
+                        if (si) shv = neg(shv);
+                        cd d = sub(chT[ti], mul(tl[(k * K + kk) * M + j], shv));
+                        part = (q == 0) ? d : mul(part, d);
+                    }
+                }
+                cd e = si ? eaP[kk * N + i] : eaN[kk * N + i];
+                cd v = mul(mul(phi[k * K + kk], e), wave_prod(part));
+                sx[k * K + kk] = add(sx[k * K + kk], v);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < K * K; q++) res[q] = sub(res[q], scale(a.h, sx[q]));
 }
 
 template <int K, int JPL>
@@ -301,10 +368,6 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
     uint32_t *sp = reinterpret_cast<uint32_t *>(tl + K * K * M); // [k][word]
     const int lane = threadIdx.x;
     const int c = blockIdx.x;
-    const cd *chT = static_cast<const cd *>(a.chT);
-    const cd *shT = static_cast<const cd *>(a.shT);
-    const cd *eaP = static_cast<const cd *>(a.eaP);
-    const cd *eaN = static_cast<const cd *>(a.eaN);
     cd *Hs = static_cast<cd *>(a.Hs);
     cd *Es = static_cast<cd *>(a.Es);
 
@@ -323,58 +386,10 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
 
         // local energy: hs-like (rbm.cc:1092-1119) or ising-like (rbm.cc:1042-1064)
         cd H[K * K];
-        if (a.model != 0) {
-            hloc_hs64<K, JPL>(a, tl, sp, phi, phi_inv, H);
-        } else {
-            // E_zz of replica (lane % K) (rbm.cc:1012-1031)
-            double ez;
-            {
-                int kr = lane % K;
-                double res = 0.0;
-                for (int i = 1; i < N; i++) {
-                    int bi = spin_bit(sp, NW, kr, i);
-                    for (int j = 0; j < i; j++) {
-                        double v = a.J[(long long)i * N + j];
-                        res += (bi ^ spin_bit(sp, NW, kr, j)) ? -v : v;
-                    }
-                }
-                ez = res * 2.0;
-            }
-            cd res[K * K], sx[K * K];
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                double ek = __shfl(ez, k);
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) res[k * K + kk] = scale(ek, phi[k * K + kk]);
-            }
-#pragma unroll
-            for (int q = 0; q < K * K; q++) sx[q] = zero();
-            for (int i = 0; i < N; i++) {
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    const int si = spin_bit(sp, NW, k, i);
-#pragma unroll
-                    for (int kk = 0; kk < K; kk++) {
-                        cd part = one();
-#pragma unroll
-                        for (int q = 0; q < JPL; q++) {
-                            int j = lane + 64 * q;
-                            if (j < M) {
-                                long long ti = ((long long)kk * N + i) * M + j;
-                                cd shv = shT[ti];
-                                if (si) shv = neg(shv);
-                                cd d = sub(chT[ti], mul(tl[(k * K + kk) * M + j], shv));
-                                part = (q == 0) ? d : mul(part, d);
-                            }
-                        }
-                        cd e = si ? eaP[kk * N + i] : eaN[kk * N + i];
-                        cd v = mul(mul(phi[k * K + kk], e), wave_prod(part));
-                        sx[k * K + kk] = add(sx[k * K + kk], v);
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < K * K; q++) res[q] = sub(res[q], scale(a.h, sx[q]));
+        {
+            cd res[K * K];
+            if (a.model != 0) hres_hs64<K, JPL>(a, tl, sp, phi, res);
+            else hres_ising64<K, JPL>(a, tl, sp, phi, res);
             matmulK<K>(phi_inv, res, H);
         }
         cd energy = zero();
@@ -468,6 +483,39 @@ __global__ __launch_bounds__(256) void k_dense_apply64(const cd *__restrict__ oo
     for (long long q = lane; q < P; q += 64) acc = add(acc, mul(oo[l * P + q], v[q]));
     acc = wave_sum(acc);
     if (lane == 0) out[l] = add(mul(skk[l], v[l]), acc);
+}
+
+// Exhaustive evaluation for the --dump-states/ham/deriv files (rbm.cc:474-526,
+// 1149-1172): one wavefront per multi-spin configuration cfg = cfg0 + blockIdx.x
+// (replica k's spins are bits [k N, k N + N) of cfg, N K < 32).  Writes Phi (K*K),
+// the local-energy numerator res (H_loc = Phi^+ res, K*K) and tanh(theta)
+// ([k][kk][j], K*K*M); any output may be NULL.
+template <int K, int JPL>
+__global__ __launch_bounds__(64) void k_enum64(Fp64SampleArgs a, long long cfg0, cd *phi_out, cd *res_out,
+                                               cd *tanh_out)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = a.N, M = a.M;
+    cd *tl = reinterpret_cast<cd *>(smem);
+    uint32_t *sp = reinterpret_cast<uint32_t *>(tl + K * K * M);
+    const int lane = threadIdx.x;
+    const long long cfg = cfg0 + blockIdx.x;
+    if (lane < K) sp[lane] = (uint32_t)((cfg >> (lane * N)) & ((1ll << N) - 1));
+    __syncthreads();
+    cd phi[K * K], phi_inv[K * K];
+    state64<K, JPL>(a, tl, sp, phi, phi_inv);
+    cd res[K * K];
+    if (a.model != 0) hres_hs64<K, JPL>(a, tl, sp, phi, res);
+    else hres_ising64<K, JPL>(a, tl, sp, phi, res);
+    const long long e = (long long)blockIdx.x;
+#pragma unroll
+    for (int q = 0; q < K * K; q++)
+        if (lane == q) {
+            if (phi_out) phi_out[e * K * K + q] = phi[q];
+            if (res_out) res_out[e * K * K + q] = res[q];
+        }
+    if (tanh_out)
+        for (int q = lane; q < K * K * M; q += 64) tanh_out[e * K * K * M + q] = tl[q];
 }
 
 // accumulate the K x K matrix Phi^+ X into a chain's correlation slot (lane q
@@ -686,6 +734,28 @@ hipError_t launch_sample_k(const Fp64SampleArgs &a, hipStream_t st)
 
 
 template <int K, int JPL>
+hipError_t launch_enum_t(const Fp64SampleArgs &a, long long cfg0, long long ncfg, void *phi, void *res, void *tanh_,
+                         hipStream_t st)
+{
+    size_t lds = sizeof(cd) * K * K * a.M + sizeof(uint32_t) * K * a.NW;
+    hipLaunchKernelGGL((k_enum64<K, JPL>), dim3((unsigned)ncfg), dim3(64), lds, st, a, cfg0, (cd *)phi, (cd *)res,
+                       (cd *)tanh_);
+    return hipGetLastError();
+}
+
+template <int K>
+hipError_t launch_enum_k(const Fp64SampleArgs &a, long long cfg0, long long ncfg, void *phi, void *res, void *tanh_,
+                         hipStream_t st)
+{
+    int jpl = (a.M + 63) / 64;
+    if (jpl <= 1) return launch_enum_t<K, 1>(a, cfg0, ncfg, phi, res, tanh_, st);
+    if (jpl <= 2) return launch_enum_t<K, 2>(a, cfg0, ncfg, phi, res, tanh_, st);
+    if (jpl <= 4) return launch_enum_t<K, 4>(a, cfg0, ncfg, phi, res, tanh_, st);
+    if (jpl <= 8) return launch_enum_t<K, 8>(a, cfg0, ncfg, phi, res, tanh_, st);
+    return hipErrorInvalidValue;
+}
+
+template <int K, int JPL>
 hipError_t launch_corr_t(const Fp64SampleArgs &a, hipStream_t st)
 {
     size_t lds = sizeof(cd) * K * K * a.M + sizeof(uint32_t) * K * a.NW;
@@ -735,6 +805,18 @@ hipError_t nq_launch_sample64(int K, const Fp64SampleArgs &a, hipStream_t st)
     case 2: return launch_sample_k<2>(a, st);
     case 3: return launch_sample_k<3>(a, st);
     case 4: return launch_sample_k<4>(a, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t nq_launch_enum64(int K, const Fp64SampleArgs &a, long long cfg0, long long ncfg, void *phi, void *res,
+                           void *tanh_, hipStream_t st)
+{
+    switch (K) {
+    case 1: return launch_enum_k<1>(a, cfg0, ncfg, phi, res, tanh_, st);
+    case 2: return launch_enum_k<2>(a, cfg0, ncfg, phi, res, tanh_, st);
+    case 3: return launch_enum_k<3>(a, cfg0, ncfg, phi, res, tanh_, st);
+    case 4: return launch_enum_k<4>(a, cfg0, ncfg, phi, res, tanh_, st);
     default: return hipErrorInvalidValue;
     }
 }

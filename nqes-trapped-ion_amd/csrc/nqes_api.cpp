@@ -104,6 +104,7 @@ struct nqes_ctx {
     DevBuf dUq, descale, dvq, dq8w, dq8m;              // its int8 form (real weights)
     bool dense = false;                                // dense S (FP64): oo P x P
     DevBuf doo, dvo, dskk;
+    DevBuf denum;                                      // exhaustive evaluation outputs
     bool svf = false, q8 = false;
     SvfShape svf_shape{};
     Q8Shape q8_shape{};
@@ -614,6 +615,49 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
         HIPCHK(c, hipStreamSynchronize(c->stream), "dense S sync");
     }
     c->sampled = true;
+    return NQES_OK;
+}
+
+int nqes_enumerate(nqes_ctx *c, const double *W, int64_t cfg0, int64_t ncfg, double *phi, double *res,
+                   double *tanh_)
+{
+    if (!c) return NQES_EINVAL;
+    if (!c->has_problem) return c->fail(NQES_ESTATE, "no problem: call nqes_problem_init first");
+    if (!W) return c->fail(NQES_EINVAL, "W is NULL");
+    const int N = c->N, M = c->M, K = c->K;
+    if (N * K >= 32) return c->fail(NQES_EINVAL, "N*K = %d: exhaustive evaluation needs N*K < 32", N * K);
+    if (cfg0 < 0 || ncfg < 0 || cfg0 + ncfg > (1ll << (N * K)) || ncfg > (1ll << 24))
+        return c->fail(NQES_EINVAL, "configurations [%lld, %lld) outside [0, 2^%d) or more than 2^24 at once",
+                       (long long)cfg0, (long long)(cfg0 + ncfg), N * K);
+    if (ncfg == 0) return NQES_OK;
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    const size_t kk = (size_t)K * K, nphi = 16 * kk * ncfg, ntanh = 16 * kk * M * ncfg;
+    HIPCHK(c, c->denum.ensure(2 * nphi + ntanh), "alloc evaluation outputs");
+    char *base = static_cast<char *>(c->denum.p);
+    HIPCHK(c, hipMemcpyAsync(c->dW.p, W, 16 * c->P, hipMemcpyHostToDevice, c->stream), "copy W");
+    HIPCHK(c, nq_launch_tables64(N, M, K, c->L, c->dW.p, c->dchT.p, c->dshT.p, c->deaP.p, c->deaN.p, c->stream),
+           "tables kernel");
+    Fp64SampleArgs a{};
+    a.N = N;
+    a.M = M;
+    a.NW = c->NW;
+    a.L = c->L;
+    a.P = c->P;
+    a.h = c->h;
+    a.model = c->model;
+    a.delta = c->model == NQES_MODEL_XXZ ? -1.0 : 1.0;
+    a.J = static_cast<const double *>(c->dJ.p);
+    a.W = c->dW.p;
+    a.chT = c->dchT.p;
+    a.shT = c->dshT.p;
+    a.eaP = c->deaP.p;
+    a.eaN = c->deaN.p;
+    HIPCHK(c, nq_launch_enum64(K, a, cfg0, ncfg, base, base + nphi, base + 2 * nphi, c->stream), "evaluation kernel");
+    if (phi) HIPCHK(c, hipMemcpyAsync(phi, base, nphi, hipMemcpyDeviceToHost, c->stream), "copy Phi");
+    if (res) HIPCHK(c, hipMemcpyAsync(res, base + nphi, nphi, hipMemcpyDeviceToHost, c->stream), "copy res");
+    if (tanh_) HIPCHK(c, hipMemcpyAsync(tanh_, base + 2 * nphi, ntanh, hipMemcpyDeviceToHost, c->stream), "copy tanh");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "evaluation sync");
+    c->sampled = false;  // the tables now hold W's
     return NQES_OK;
 }
 

@@ -96,7 +96,7 @@ EXPORTS = [
     "nqes_create", "nqes_destroy", "nqes_last_error", "nqes_problem_init", "nqes_sr_step",
     "nqes_sample", "nqes_get_samples", "nqes_get_O", "nqes_get_moments", "nqes_apply_S",
     "nqes_solve", "nqes_correlations", "nqes_required_bytes", "nqes_version",
-    "nqes_set_dense_S",
+    "nqes_set_dense_S", "nqes_enumerate",
     "nqes_server_create", "nqes_server_destroy", "nqes_server_last_error", "nqes_server_submit",
     "nqes_server_poll", "nqes_server_wait",
 ]
@@ -141,6 +141,8 @@ def load_library(path=LIB_PATH):
     L.nqes_version.argtypes = []
     L.nqes_set_dense_S.restype = c_int
     L.nqes_set_dense_S.argtypes = [_vp, c_int]
+    L.nqes_enumerate.restype = c_int
+    L.nqes_enumerate.argtypes = [_vp, _cp, c_i64, c_i64, _vp, _vp, _vp]
     L.nqes_server_create.restype = _vp
     L.nqes_server_create.argtypes = [c_int, c_int, c_i64]
     L.nqes_server_destroy.argtypes = [_vp]
@@ -216,6 +218,19 @@ class Solver:
         """S as a dense P x P matrix (the reference's rbm-N* executables) instead of
         the matrix-free operator; FP64 only."""
         self._check(self.lib.nqes_set_dense_S(self.ctx, int(bool(dense))))
+
+    def enumerate(self, W, cfg0=0, ncfg=None):
+        """Phi (ncfg, K, K), local-energy numerator res (H_loc = Phi^+ res) and tanh
+        (ncfg, K, K, M) of the configurations [cfg0, cfg0 + ncfg) (N K < 32)."""
+        N, M, K = self.N, self.M, self.K
+        if ncfg is None:
+            ncfg = (1 << (N * K)) - cfg0
+        W = np.ascontiguousarray(W, dtype=np.complex128)
+        phi = np.zeros((ncfg, K, K), dtype=np.complex128)
+        res = np.zeros((ncfg, K, K), dtype=np.complex128)
+        tanh = np.zeros((ncfg, K, K, M), dtype=np.complex128)
+        self._check(self.lib.nqes_enumerate(self.ctx, W, int(cfg0), int(ncfg), _ptr(phi), _ptr(res), _ptr(tanh)))
+        return phi, res, tanh
 
     # ---------------------------------------------------------------- SR step
     def monte_carlo(self, W, seed, C, m, B, lr, reglam, rtol, maxit=2**64 - 1, stats=False):
