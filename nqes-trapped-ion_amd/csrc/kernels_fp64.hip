@@ -674,6 +674,24 @@ __global__ void k_chunksum64(const cd *__restrict__ part, long long ncol, int nc
     out[col] = acc;
 }
 
+// the same sum for few columns: block per column, thread t sums chunks t, t + 256, ...
+// in ascending order, then a fixed pairwise tree over the 256 partials
+__global__ __launch_bounds__(256) void k_chunksum_tree64(const cd *__restrict__ part, long long ncol, int nchunk,
+                                                         cd *__restrict__ out)
+{
+    __shared__ cd red[256];
+    const long long col = blockIdx.x;
+    cd acc = zero();
+    for (int ch = threadIdx.x; ch < nchunk; ch += 256) acc = add(acc, part[(long long)ch * ncol + col]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = add(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[col] = red[0];
+}
+
 __global__ void k_colsum64(const cd *__restrict__ X, long long S, long long ncol, int nchunk,
                            cd *__restrict__ part)
 {
@@ -854,8 +872,12 @@ hipError_t nq_launch_colsum64(const void *X, long long S, long long ncol, int nc
 {
     dim3 g((unsigned)((ncol + 63) / 64), (unsigned)nchunk);
     hipLaunchKernelGGL(k_colsum64, g, dim3(64), 0, st, (const cd *)X, S, ncol, nchunk, (cd *)part);
-    hipLaunchKernelGGL(k_chunksum64, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, st, (const cd *)part, ncol,
-                       nchunk, (cd *)out);
+    if (ncol <= 64)
+        hipLaunchKernelGGL(k_chunksum_tree64, dim3((unsigned)ncol), dim3(256), 0, st, (const cd *)part, ncol, nchunk,
+                           (cd *)out);
+    else
+        hipLaunchKernelGGL(k_chunksum64, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, st, (const cd *)part, ncol,
+                           nchunk, (cd *)out);
     return hipGetLastError();
 }
 

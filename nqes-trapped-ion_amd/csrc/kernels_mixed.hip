@@ -556,7 +556,7 @@ __device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &t
 // stored (sample s = c m + t); the local energies and O rows are computed afterwards
 // by k_measure_mx, one lane group per sample.
 template <bool CPLX, int K, int G, int JMAX>
-__global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a)
+__global__ __launch_bounds__(512) void k_sample_mx(MixedSampleArgs a)
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;

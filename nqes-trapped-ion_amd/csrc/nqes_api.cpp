@@ -102,6 +102,10 @@ struct nqes_ctx {
     DevBuf dv, dov, dw, dsvpart, dsvwpart, dsvout, dsvscratch, dmscratch;
     DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
     DevBuf dUq, descale, dvq, dq8w, dq8m;              // its int8 form (real weights)
+    hipGraph_t svg = nullptr;                          // int8 S*v sequence, captured once per
+    hipGraphExec_t svx = nullptr;                      // sample set and replayed per MINRES step
+    double *hv_pin = nullptr, *hr_pin = nullptr;       // pinned v | o^T v and result (P + 1 each)
+    size_t pin_len = 0;
     bool dense = false;                                // dense S (FP64): oo P x P
     DevBuf doo, dvo, dskk;
     DevBuf denum;                                      // exhaustive evaluation outputs
@@ -155,20 +159,29 @@ static long long o_ld(long long P, int prec, bool cplx)
     return (P + per16 - 1) / per16 * per16;
 }
 
+// the captured S*v graph refers to the sample set's buffers
+static void drop_sv_graph(nqes_ctx *c)
+{
+    if (c->svx) (void)hipGraphExecDestroy(c->svx);
+    if (c->svg) (void)hipGraphDestroy(c->svg);
+    c->svx = nullptr;
+    c->svg = nullptr;
+}
+
 // lane-group size, waves per block and table staging of a mixed kernel with G lanes
 // per chain (sweeps) or per sample (measurement); the LDS layout is the same
 struct MixedShape {
     int G, wpb, tab;
 };
 static MixedShape mixed_shape(const nqes_ctx *c, bool cplx, int g_dflt, const char *g_env_name,
-                              const char *wpb_env_name)
+                              const char *wpb_env_name, int max_wpb)
 {
     int G = g_dflt;
     while (G < 64 && (c->M + G - 1) / G > 8) G *= 2;
     int g_env = env_int(g_env_name, 0);
     if ((g_env == 16 || g_env == 32 || g_env == 64) && (c->M + g_env - 1) / g_env <= 8) G = g_env;
     int wpb = env_int(wpb_env_name, 4);
-    if (wpb < 1 || wpb > 4) wpb = 4;
+    if (wpb < 1 || wpb > max_wpb) wpb = 4;
     int tab = env_int("NQES_TABLDS", 1);
     while (wpb > 1 && nq_sample_mx_lds(cplx, c->N, c->M, c->K, G, wpb, false) > c->lds_max) wpb--;
     if (tab && nq_sample_mx_lds(cplx, c->N, c->M, c->K, G, wpb, true) > c->lds_max) tab = 0;
@@ -205,6 +218,10 @@ void nqes_destroy(nqes_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->svx) (void)hipGraphExecDestroy(c->svx);
+    if (c->svg) (void)hipGraphDestroy(c->svg);
+    if (c->hv_pin) (void)hipHostFree(c->hv_pin);
+    if (c->hr_pin) (void)hipHostFree(c->hr_pin);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -401,8 +418,8 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     HIPCHK(c, nq_launch_theta_gemm(cplx, c->N, c->M, c->K, c->NW, C, (const uint32_t *)c->dspins0.p, c->dwT.p,
                                    c->dbT.p, c->dtheta0.p, c->stream),
            "theta GEMM kernel");
-    const MixedShape sw = mixed_shape(c, cplx, 32, "NQES_G", "NQES_WPB");
-    const MixedShape ms = mixed_shape(c, cplx, 16, "NQES_GM", "NQES_WPBM");
+    const MixedShape sw = mixed_shape(c, cplx, 32, "NQES_G", "NQES_WPB", 8);   // k_sample_mx: 512 threads
+    const MixedShape ms = mixed_shape(c, cplx, 16, "NQES_GM", "NQES_WPBM", 4);  // k_measure_mx: 256
     MixedSampleArgs a{};
     a.N = c->N;
     a.M = c->M;
@@ -479,6 +496,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     }
     c->cplx = cplx;
     c->Pld = o_ld(P, c->prec, cplx);
+    drop_sv_graph(c);  // buffers may move
     // factored S*v (mixed path) unless NQES_SV=stored or the shape does not apply; for
     // real weights its int8 form unless NQES_SV=fp64mfma
     c->svf = c->q8 = false;
@@ -774,11 +792,59 @@ int nqes_get_moments(nqes_ctx *c, double *o, double *skk, double *f, double *elo
     return NQES_OK;
 }
 
+// the int8 S*v of the current sample set as one graph: v and o^T v from pinned host
+// memory, the kernels of nq_q8_apply, the raw result back to pinned memory (no memset
+// nodes).  NQES_GRAPH=0 launches the same sequence eagerly instead.
+static int build_sv_graph(nqes_ctx *c)
+{
+    const long long P = c->P;
+    if ((size_t)(P + 1) > c->pin_len) {
+        if (c->hv_pin) (void)hipHostFree(c->hv_pin);
+        if (c->hr_pin) (void)hipHostFree(c->hr_pin);
+        c->hv_pin = c->hr_pin = nullptr;
+        c->pin_len = 0;
+        HIPCHK(c, hipHostMalloc((void **)&c->hv_pin, 8 * (P + 1), hipHostMallocDefault), "pinned v");
+        HIPCHK(c, hipHostMalloc((void **)&c->hr_pin, 8 * (P + 1), hipHostMallocDefault), "pinned result");
+        c->pin_len = (size_t)(P + 1);
+    }
+    const SvfArgs f = svf_args(c, c->S);
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin S*v capture");
+    hipError_t e = hipMemcpyAsync(c->dv.p, c->hv_pin, 8 * P, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->dov.p, c->hv_pin + P, 8, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = nq_q8_apply(f, c->q8_shape, c->dsvfscratch.p, c->dsvout.p, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->hr_pin, c->dsvout.p, 8 * (P + 1), hipMemcpyDeviceToHost, c->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return c->hipfail(e != hipSuccess ? e : e2, "S*v capture");
+    }
+    c->svg = g;
+    HIPCHK(c, hipGraphInstantiate(&c->svx, c->svg, nullptr, nullptr, 0), "S*v graph instantiate");
+    return NQES_OK;
+}
+
 // device part of S*v on the FP32 O of the mixed path: raw[p] = sum_s conj(O_sp) w_s,
 // raw[P] = sum_s w_s, w_s = O_s^T v - o^T v.  v given as D = double (real) or cd.
 static int sv_mixed_device(nqes_ctx *c, const void *v, const void *ov, size_t dsize)
 {
     const long long P = c->P;
+    if (c->q8 && env_int("NQES_GRAPH", 1) != 0) {
+        if (!c->svx) {
+            int st = build_sv_graph(c);
+            if (st != NQES_OK) return st;
+        }
+        memcpy(c->hv_pin, v, 8 * P);
+        memcpy(c->hv_pin + P, ov, 8);
+        HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
+        HIPCHK(c, hipGraphLaunch(c->svx, c->stream), "S*v graph launch");
+        HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
+        HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
+        memcpy(c->svraw.data(), c->hr_pin, 8 * (P + 1));
+        c->sv_ms += event_ms(c->ev[2], c->ev[3]);
+        c->sv_calls++;
+        return NQES_OK;
+    }
     HIPCHK(c, hipMemcpyAsync(c->dv.p, v, dsize * P, hipMemcpyHostToDevice, c->stream), "copy v");
     HIPCHK(c, hipMemcpyAsync(c->dov.p, ov, dsize, hipMemcpyHostToDevice, c->stream), "copy ov");
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");

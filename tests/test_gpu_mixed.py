@@ -169,6 +169,36 @@ def test_q8_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
     assert sym < 1e-11, sym
 
 
+def test_q8_graph_replays_equal_eager(nqes, monkeypatch):
+    """The int8 S*v runs as a captured HIP graph (one per sample set): every replay,
+    with eager work and synchronisations between replays and a second sample set
+    (graph rebuilt), equals the eagerly launched sequence (NQES_GRAPH=0) bit for bit."""
+    N, M, K, C, m = 16, 32, 2, 64, 8
+    J = load_J(16)
+    W = random_weights(N, M, K, seed=41, scale=0.1, complex_=False)
+    P = K * (N + M + N * M)
+    rng = np.random.default_rng(3)
+    vs = [rng.normal(size=P) + 0j for _ in range(4)] + [rng.normal(size=P) + 1j * rng.normal(size=P)]
+    out = {}
+    for graph in ("1", "0"):
+        monkeypatch.setenv("NQES_GRAPH", graph)
+        monkeypatch.delenv("NQES_SV", raising=False)
+        res = []
+        with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+            for seed in (5, 6):
+                s.sample(W, seed, C, m, 2, 0.5)
+                for v in vs:
+                    res.append(s.apply_S(v))
+                    s.moments()
+                    s.samples()
+            Wn = W.copy()
+            res.append(s.monte_carlo(Wn, 7, C, m, 2, 0.1, 1.0, 1e-9))
+            res.append(Wn)
+        out[graph] = res
+    for a, b in zip(out["1"], out["0"]):
+        assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("model", [1, 2])
 @pytest.mark.parametrize("cplx", [False, True])
 def test_mixed_pairflip_matches_fp64(nqes, model, cplx):
