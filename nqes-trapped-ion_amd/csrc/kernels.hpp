@@ -108,6 +108,9 @@ struct SvfShape {
 };
 SvfShape nq_svf_shape(bool cplx, int N, int M, int K);
 hipError_t nq_launch_svf_combine(bool cplx, const SvfArgs &a, hipStream_t st);
+// |O|^2 sums and the packed moments, given the w = 1 and w = E sums at tmp, tmp + (P+1) D
+hipError_t nq_launch_svf_sq_pack(bool cplx, const SvfArgs &a, const SvfShape &sh, void *part, void *scratch,
+                                 void *tmp, void *so, void *ss, void *sf, hipStream_t st);
 // int8 form of the factored S*v (real weights, N <= 64): factors quantised once per
 // sample set (nq_q8_pack), S*v on int8 MFMA (nq_q8_apply)
 struct Q8Shape {
@@ -119,6 +122,10 @@ struct Q8Shape {
 Q8Shape nq_q8_shape(int N, int M, int K, long long R);
 hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
 hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st);
+// moments from the quantised factors (sums with w = 1 and w = E on the int8 backward
+// pass, |O|^2 on FP64 MFMA); after nq_q8_pack
+hipError_t nq_q8_moments(const SvfArgs &a, const Q8Shape &sh, const SvfShape &svf, void *part, void *scratch,
+                         void *tmp, void *so, void *ss, void *sf, hipStream_t st);
 hipError_t nq_launch_svf(bool cplx, const SvfArgs &a, const SvfShape &sh, void *scratch, void *out, hipStream_t st);
 // moments from the factors: so = sum_s O_s, ss = sum_s |O_s|^2, sf = sum_s E_s conj(O_s)
 // (P complex double each); part: >= max(1, K(K-1)/2) * bwd_blocks * (P + 1) D,

@@ -461,10 +461,16 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
     if (c->svf) {
         SvfArgs f = svf_args(c, S);
-        HIPCHK(c, nq_launch_svf_moments(cplx, f, c->svf_shape, c->dsvfpart.p, c->dsvfscratch.p, c->dsvftmp.p, c->dmo.p,
-                                        c->dms.p, c->dmf.p, c->stream),
-               "factored moments kernels");
-        if (c->q8) HIPCHK(c, nq_q8_pack(f, c->q8_shape, c->stream), "factor quantisation kernel");
+        if (c->q8) {
+            HIPCHK(c, nq_q8_pack(f, c->q8_shape, c->stream), "factor quantisation kernel");
+            HIPCHK(c, nq_q8_moments(f, c->q8_shape, c->svf_shape, c->dsvfpart.p, c->dsvfscratch.p, c->dsvftmp.p,
+                                    c->dmo.p, c->dms.p, c->dmf.p, c->stream),
+                   "int8 moments kernels");
+        } else {
+            HIPCHK(c, nq_launch_svf_moments(cplx, f, c->svf_shape, c->dsvfpart.p, c->dsvfscratch.p, c->dsvftmp.p,
+                                            c->dmo.p, c->dms.p, c->dmf.p, c->stream),
+                   "factored moments kernels");
+        }
         return NQES_OK;
     }
     HIPCHK(c, c->dmscratch.ensure((size_t)(c->mchunks / 32 + 2) * c->P * 16), "alloc moments scratch");

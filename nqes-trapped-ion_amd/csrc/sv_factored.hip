@@ -437,6 +437,31 @@ __global__ void k_svf_pack_moments(long long P, const D *r1, const D *rE, const 
     }
 }
 
+// sum |O|^2 into rsq = tmp + 2 (P + 1) D, then the three moment vectors from r1 (w = 1,
+// at tmp) and rE (w = E, at tmp + (P + 1) D)
+template <bool CPLX>
+hipError_t launch_sq_pack(const SvfArgs &a0, const SvfShape &sh, void *part, void *scratch, void *tmp, void *so,
+                          void *ss, void *sf, hipStream_t st)
+{
+    using D = typename Ty<CPLX>::D;
+    const long long P = a0.P;
+    D *r1 = (D *)tmp, *rE = r1 + (P + 1);
+    double *rsq = (double *)(rE + (P + 1));
+    SvfArgs a = a0;
+    a.part = part;
+    const int npair = a0.K > 1 ? a0.K * (a0.K - 1) / 2 : 1;
+    const int nct = a0.Ucols / 16;
+    const int waves = nct < 16 ? nct : 16;
+    if (a0.N <= 32) hipLaunchKernelGGL((k_svf_sq<CPLX, 2>), dim3(sh.bwd_blocks, npair), dim3(64 * waves), 0, st, a);
+    else hipLaunchKernelGGL((k_svf_sq<CPLX, 4>), dim3(sh.bwd_blocks, npair), dim3(64 * waves), 0, st, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    nq_chunk_reduce(false, part, P, P + 1, sh.bwd_blocks * npair, scratch, rsq, st);
+    hipLaunchKernelGGL(k_svf_pack_moments<D>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, P, r1, rE, rsq,
+                       (cd *)so, (cd *)ss, (cd *)sf);
+    return hipGetLastError();
+}
+
 template <bool CPLX>
 hipError_t launch_svf_moments(const SvfArgs &a0, const SvfShape &sh, void *part, void *scratch, void *tmp, void *so,
                               void *ss, void *sf, hipStream_t st)
@@ -453,17 +478,7 @@ hipError_t launch_svf_moments(const SvfArgs &a0, const SvfShape &sh, void *part,
         if (e != hipSuccess) return e;
         nq_chunk_reduce(CPLX, part, P + 1, P + 1, sh.bwd_blocks, scratch, pass == 0 ? (void *)r1 : (void *)rE, st);
     }
-    const int npair = a0.K > 1 ? a0.K * (a0.K - 1) / 2 : 1;
-    const int nct = a0.Ucols / 16;
-    const int waves = nct < 16 ? nct : 16;
-    if (n32) hipLaunchKernelGGL((k_svf_sq<CPLX, 2>), dim3(sh.bwd_blocks, npair), dim3(64 * waves), 0, st, a);
-    else hipLaunchKernelGGL((k_svf_sq<CPLX, 4>), dim3(sh.bwd_blocks, npair), dim3(64 * waves), 0, st, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    nq_chunk_reduce(false, part, P, P + 1, sh.bwd_blocks * npair, scratch, rsq, st);
-    hipLaunchKernelGGL(k_svf_pack_moments<D>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, P, r1, rE, rsq,
-                       (cd *)so, (cd *)ss, (cd *)sf);
-    return hipGetLastError();
+    return launch_sq_pack<CPLX>(a0, sh, part, scratch, tmp, so, ss, sf, st);
 }
 
 // Factored S*v applies when N <= 64 (two spin words per row, four i tiles).  Forward
@@ -500,6 +515,13 @@ hipError_t nq_launch_svf_moments(bool cplx, const SvfArgs &a, const SvfShape &sh
 {
     return cplx ? launch_svf_moments<true>(a, sh, part, scratch, tmp, so, ss, sf, st)
                 : launch_svf_moments<false>(a, sh, part, scratch, tmp, so, ss, sf, st);
+}
+
+hipError_t nq_launch_svf_sq_pack(bool cplx, const SvfArgs &a, const SvfShape &sh, void *part, void *scratch,
+                                 void *tmp, void *so, void *ss, void *sf, hipStream_t st)
+{
+    return cplx ? launch_sq_pack<true>(a, sh, part, scratch, tmp, so, ss, sf, st)
+                : launch_sq_pack<false>(a, sh, part, scratch, tmp, so, ss, sf, st);
 }
 
 hipError_t nq_launch_svf_combine(bool cplx, const SvfArgs &a, hipStream_t st)

@@ -594,3 +594,17 @@ hipError_t nq_q8_apply(const SvfArgs &a0, const Q8Shape &sh, void *scratch, void
     if (e != hipSuccess) return e;
     return nq_chunk_reduce(false, a.part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, out, st);
 }
+
+hipError_t nq_q8_moments(const SvfArgs &a0, const Q8Shape &sh, const SvfShape &svf, void *part, void *scratch,
+                         void *tmp, void *so, void *ss, void *sf, hipStream_t st)
+{
+    SvfArgs a = a0;
+    a.part = part;
+    double *r1 = (double *)tmp, *rE = r1 + (a.P + 1);
+    hipError_t e = launch_bwd<1>(a, sh, st);
+    if (e == hipSuccess) e = nq_chunk_reduce(false, part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, r1, st);
+    if (e == hipSuccess) e = launch_bwd<2>(a, sh, st);
+    if (e == hipSuccess) e = nq_chunk_reduce(false, part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, rE, st);
+    if (e != hipSuccess) return e;
+    return nq_launch_svf_sq_pack(false, a0, svf, part, scratch, tmp, so, ss, sf, st);
+}
