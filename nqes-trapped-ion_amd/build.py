@@ -36,6 +36,10 @@ EXACT = ["-ffp-contract=off"]
 SOURCES = [
     ("kernels_fp64.hip", EXACT),
     ("kernels_mixed.hip", []),
+    ("sampler_mx.hip", ["-DNQES_SWEEP_K=1"], "sampler_mx_k1"),
+    ("sampler_mx.hip", ["-DNQES_SWEEP_K=2"], "sampler_mx_k2"),
+    ("sampler_mx.hip", ["-DNQES_SWEEP_K=3"], "sampler_mx_k3"),
+    ("sampler_mx.hip", ["-DNQES_SWEEP_K=4"], "sampler_mx_k4"),
     ("sv_factored.hip", []),
     ("sv_q8.hip", []),
     ("nqes_api.cpp", EXACT),
@@ -44,8 +48,8 @@ SOURCES = [
 ]
 
 
-def _compile(src, extra):
-    out = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+def _compile(src, extra, obj=None):
+    out = os.path.join(OBJDIR, (obj or os.path.basename(src)) + ".o")
     path = os.path.join(CSRC, src)
     lang = ["-x", "hip"] if src.endswith(".hip") else ["-x", "hip"]
     cmd = [HIPCC] + COMMON + extra + lang + ["-c", path, "-o", out]
@@ -55,7 +59,7 @@ def _compile(src, extra):
     return out
 
 
-def build(jobs=4, verbose=False):
+def build(jobs=8, verbose=False):
     os.makedirs(OBJDIR, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     os.makedirs(BINDIR, exist_ok=True)
@@ -85,7 +89,7 @@ def build(jobs=4, verbose=False):
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--jobs", type=int, default=8)
     args = ap.parse_args()
     build(args.jobs, verbose=True)
     sys.exit(0)

@@ -143,7 +143,17 @@ hipError_t nq_launch_chain_init(int N, int K, int NW, int C, uint64_t seed, uint
                                 hipStream_t st);
 hipError_t nq_launch_theta_gemm(bool cplx, int N, int M, int K, int NW, int C, const uint32_t *spins0,
                                 const void *wT, const void *bT, void *theta0, hipStream_t st);
-hipError_t nq_launch_sample_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st);
+// Metropolis sweeps of the mixed path (sampler_mx.hip): G lanes per chain, JQ hidden
+// units per lane (tables padded to G*JQ), wpb wavefronts per block, tables in LDS if tab.
+struct SweepShape {
+    int ok, G, JQ, wpb, tab;
+    size_t tab_bytes;  // padded tables (device buffer passed to nq_launch_sweep_mx)
+};
+SweepShape nq_sweep_shape(bool cplx, int N, int M, int K, size_t lds_max);
+template <int K>
+hipError_t nq_sweep_k(bool cplx, const MixedSampleArgs &a, const void *tabp, const SweepShape &sh, hipStream_t st);
+hipError_t nq_launch_sweep_mx(bool cplx, int K, const MixedSampleArgs &a, const SweepShape &sh, void *tabp,
+                              hipStream_t st);
 hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st);
 size_t nq_sample_mx_lds(bool cplx, int N, int M, int K, int G, int wpb, bool tab_lds);
 hipError_t nq_launch_moments_mx(bool cplx, const void *O, const void *E, long long S, long long P, long long Pld,

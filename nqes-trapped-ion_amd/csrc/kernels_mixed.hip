@@ -9,10 +9,7 @@
 //   k_chain_init     per chain: pcg64(seed + c), K*N Bernoulli draws (rbm.cc:99-119)
 //   k_theta_gemm     Theta0[C*K, K*M] = Sigma[C*K, N] W[N, K*M] + b on FP32 MFMA
 //                    (v_mfma_f32_16x16x4f32), the angles at chain start (rbm.cc:135-145)
-//   k_sample_mx      G lanes per chain, 64/G chains per wavefront; Metropolis sweeps
-//                    (rbm.cc:530-552) with tanh theta in LDS updated per accepted flip;
-//                    after every sweep theta, tanh and Phi are recomputed from the spins
-//                    (drift bounded to one sweep), after each sampled sweep the spins stored
+//   (k_sample_mx, the Metropolis sweeps, is in sampler_mx.hip)
 //   k_measure_mx     G lanes per sample: theta, tanh, Phi, Phi^-1 from the stored spins,
 //                    local energy (rbm.cc:1042-1064) and the O row
 //                    O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k}
@@ -547,151 +544,6 @@ __device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &t
     }
 }
 
-// Metropolis chains: G lanes per chain, 64/G chains per wavefront.  Sweeps
-// (rbm.cc:530-552) with tanh theta in LDS updated per accepted flip.  After every
-// sweep (burn-in included) theta, tanh and Phi are recomputed from the spins, so the
-// FP32 rounding of the incremental updates never spans more than one sweep: with
-// nearly equal RBMs (small initial weights, K = 4) det Phi can be ~1e-6 of its scale,
-// and longer drift would bias acceptances.  After each sampled sweep the spins are
-// stored (sample s = c m + t); the local energies and O rows are computed afterwards
-// by k_measure_mx, one lane group per sample.
-template <bool CPLX, int K, int G, int JMAX>
-__global__ __launch_bounds__(512) void k_sample_mx(MixedSampleArgs a)
-{
-    using T = typename Ty<CPLX>::T;
-    using D = typename Ty<CPLX>::D;
-    constexpr int CPW = 64 / G;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int N = a.N, M = a.M, NW = a.NW;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int g = lane & (G - 1), grp = lane / G;
-    const int cpb = (blockDim.x >> 6) * CPW;
-    const int cib = wv * CPW + grp;
-    const int c_raw = blockIdx.x * cpb + cib;
-    const bool live = c_raw < a.C;
-    const int c = live ? c_raw : a.C - 1;
-    const int jpl = (M + G - 1) / G;
-
-    // LDS: [tables] (if a.tab_lds) [tanh per chain][spins per chain]
-    MxTab<CPLX> tb;
-    unsigned char *p = mx_stage_tables<CPLX>(a, K, tb, smem);
-    const T *chT = tb.ch, *shT = tb.sh;
-    T *tl = (T *)p + (long long)cib * K * K * M;
-    uint32_t *sp = (uint32_t *)((T *)p + (long long)cpb * K * K * M) + cib * K * NW;
-    const D *eaP = tb.eaP;
-    const D *eaN = tb.eaN;
-
-    if (g == 0)
-        for (int w = 0; w < K * NW; w++) sp[w] = a.spins0[(long long)c * K * NW + w];
-    nq64::pcg rng;
-    rng.lo = a.rng0[2 * c];
-    rng.hi = a.rng0[2 * c + 1];
-    __syncthreads();
-
-    D phi[K * K];
-    int overflow = 0;
-    mx_refresh<CPLX, K, G, JMAX>(a, tb, sp, tl, g, c, true, phi, overflow);
-    // Draws in batches of G: lane g of the group holds draw g of the batch, computed
-    // with one jump of g + 1 steps from the batch base (the same stream as G
-    // sequential steps); proposal t of the batch reads lane t's draw.
-    nq64::pcg jmp_a, jmp_c;
-    nq64::pcg_jump((unsigned)g + 1u, jmp_a, jmp_c);
-    double ub = 0.0;
-    int tdraw = G;
-    unsigned long long accepts = 0;
-    const int nsweeps = a.B + a.m;
-    for (int sw = 0; sw < nsweeps; sw++) {
-        D det_phi = det_m<K, D>(phi);
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            uint32_t cur = 0;
-            for (int i = 0; i < N; i++) {
-                if ((i & 31) == 0) cur = sp[k * NW + (i >> 5)];
-                const int si = (cur >> (i & 31)) & 1u;
-                T tnew[JMAX][K];
-                T pf[K];
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) pf[kk] = oneT<T>();
-#pragma unroll
-                for (int q = 0; q < JMAX; q++) {
-                    int j = g + G * q;
-                    if (q < jpl && j < M) {
-#pragma unroll
-                        for (int kk = 0; kk < K; kk++) {
-                            long long ti = ((long long)kk * N + i) * M + j;
-                            T chv = chT[ti];
-                            T shv = shT[ti];
-                            if (si) shv = negT(shv);
-                            T th = tl[(k * K + kk) * M + j];
-                            T den = subT(chv, mulT(th, shv));
-                            tnew[q][kk] = divT(subT(mulT(th, chv), shv), den);
-                            pf[kk] = mulT(pf[kk], den);
-                        }
-                    }
-                }
-                D prow[K];
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) {
-                    D e = si ? eaP[kk * N + i] : eaN[kk * N + i];
-                    D Tp = group_prod<G>(toD(pf[kk]));
-                    prow[kk] = mulD(mulD(phi[k * K + kk], e), Tp);
-                }
-                D phip[K * K];
-#pragma unroll
-                for (int r = 0; r < K; r++)
-#pragma unroll
-                    for (int cc = 0; cc < K; cc++) phip[r * K + cc] = (r == k) ? prow[cc] : phi[r * K + cc];
-                D det_p = det_m<K, D>(phip);
-                if (tdraw == G) {  // next batch of G draws (wave-uniform: chains step in lockstep)
-                    const nq64::pcg sg = nq64::pcg_affine(rng, jmp_a, jmp_c);
-                    ub = nq64::canonical(nq64::pcg_out(sg));
-                    rng.lo = __shfl(sg.lo, G - 1, G);
-                    rng.hi = __shfl(sg.hi, G - 1, G);
-                    tdraw = 0;
-                }
-                // draw tdraw of each group's batch: scalar reads of lane grp G + tdraw
-                double u = 0.0;
-#pragma unroll
-                for (int q = 0; q < CPW; q++) {
-                    const uint64_t ubits = __builtin_bit_cast(uint64_t, ub);
-                    const uint32_t lo32 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)ubits, q * G + tdraw);
-                    const uint32_t hi32 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(ubits >> 32), q * G + tdraw);
-                    if (grp == q) u = __builtin_bit_cast(double, (uint64_t)lo32 | ((uint64_t)hi32 << 32));
-                }
-                tdraw++;
-                // U <= min(1, |det'|^2 / |det|^2) without the division (rbm.cc:542-543)
-                const bool acc = isZeroD(det_phi) || u * normD(det_phi) <= normD(det_p);
-                if (acc) {
-                    det_phi = det_p;
-#pragma unroll
-                    for (int q = 0; q < K * K; q++) phi[q] = phip[q];
-#pragma unroll
-                    for (int q = 0; q < JMAX; q++) {
-                        int j = g + G * q;
-                        if (q < jpl && j < M)
-#pragma unroll
-                            for (int kk = 0; kk < K; kk++) tl[(k * K + kk) * M + j] = tnew[q][kk];
-                    }
-                    cur ^= 1u << (i & 31);
-                    accepts++;
-                }
-                if ((i & 31) == 31 || i == N - 1) {
-                    if (g == 0) sp[k * NW + (i >> 5)] = cur;
-                }
-            }
-        }
-        __syncthreads();
-        if (sw >= a.B && live) {
-            const long long s = (long long)c * a.m + (sw - a.B);
-            for (int w = g; w < K * NW; w += G) a.spins[s * K * NW + w] = sp[w];
-        }
-        if (sw + 1 < nsweeps) mx_refresh<CPLX, K, G, JMAX>(a, tb, sp, tl, g, c, false, phi, overflow);
-        __syncthreads();
-    }
-    if (live && g == 0) a.accepts[c] = accepts;
-    if (live && overflow) atomicOr(a.flags, 1u);
-}
-
 // Local energies and O rows of all S samples from their stored spins: G lanes per
 // sample, 64/G samples per wavefront, each block looping over sample batches (grid
 // sized to the CUs) so that the tables are staged in LDS once per block.  Sample
@@ -1010,23 +862,6 @@ void chunk_reduce(const D *part, long long ncol, long long stride, int nchunk, D
 
 // ---------------------------------------------------------------- launchers
 template <bool CPLX, int K, int G>
-hipError_t launch_sample_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
-{
-    using T = typename Ty<CPLX>::T;
-    const int cpb = wpb * (64 / G);
-    size_t lds = (size_t)cpb * K * K * a.M * sizeof(T) + (size_t)cpb * K * a.NW * 4;
-    if (a.tab_lds) lds += mx_tab_bytes(CPLX, a.N, a.M, K);
-    int blocks = (a.C + cpb - 1) / cpb;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_sample_mx<CPLX, K, G, 8>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((k_sample_mx<CPLX, K, G, 8>), dim3(blocks), dim3(64 * wpb), lds, st, a);
-    return hipGetLastError();
-}
-
-template <bool CPLX, int K, int G>
 hipError_t launch_measure_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
 {
     using T = typename Ty<CPLX>::T;
@@ -1077,29 +912,6 @@ hipError_t launch_measure_c(int K, const MixedSampleArgs &a, hipStream_t st, int
     }
 }
 
-template <bool CPLX, int K>
-hipError_t launch_sample_k(const MixedSampleArgs &a, hipStream_t st, int G, int wpb)
-{
-    switch (G) {
-    case 16: return launch_sample_g<CPLX, K, 16>(a, st, wpb);
-    case 32: return launch_sample_g<CPLX, K, 32>(a, st, wpb);
-    case 64: return launch_sample_g<CPLX, K, 64>(a, st, wpb);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-template <bool CPLX>
-hipError_t launch_sample_c(int K, const MixedSampleArgs &a, hipStream_t st, int G, int wpb)
-{
-    switch (K) {
-    case 1: return launch_sample_k<CPLX, 1>(a, st, G, wpb);
-    case 2: return launch_sample_k<CPLX, 2>(a, st, G, wpb);
-    case 3: return launch_sample_k<CPLX, 3>(a, st, G, wpb);
-    case 4: return launch_sample_k<CPLX, 4>(a, st, G, wpb);
-    default: return hipErrorInvalidValue;
-    }
-}
-
 } // namespace
 
 hipError_t nq_chunk_reduce(bool cplx, const void *part, long long ncol, long long stride, int nchunk,
@@ -1142,11 +954,6 @@ hipError_t nq_launch_theta_gemm(bool cplx, int N, int M, int K, int NW, int C, c
     hipLaunchKernelGGL(k_theta_gemm, grid, dim3(64), 0, st, N, M, K, NW, R, rows, spins0, (const float *)wT,
                        (const float *)bT, (float *)theta0);
     return hipGetLastError();
-}
-
-hipError_t nq_launch_sample_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st)
-{
-    return cplx ? launch_sample_c<true>(K, a, st, G, wpb) : launch_sample_c<false>(K, a, st, G, wpb);
 }
 
 hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st)

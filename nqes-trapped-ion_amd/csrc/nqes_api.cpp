@@ -91,6 +91,7 @@ struct nqes_ctx {
     double h = 0;
     DevBuf dJ, dW, dchT, dshT, deaP, deaN;
     DevBuf dwT, dbT, daD, dspins0, drng0, dtheta0, dflags;  // mixed path
+    DevBuf dtabp;                                      // its sampler's padded tables
 
     // sample state
     bool sampled = false;
@@ -244,6 +245,7 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
     long long b = 8LL * N * N + 16 * P + 32LL * K * N * M + 32LL * K * N;  // J, W, tables
     if (mixed) b += 8LL * K * N * M + 8LL * K * M + 16LL * K * N + 16;
     if (mixed) b += (long long)C * K * NW * 4 + C * 16LL + (long long)C * K * K * M * 8;  // chain starts
+    if (mixed) b += (long long)nq_sweep_shape(cplx, N, M, K, 0).tab_bytes;                 // sampler tables
     SvfShape sh{};
     bool svf = false;
     if (mixed) {
@@ -418,7 +420,9 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     HIPCHK(c, nq_launch_theta_gemm(cplx, c->N, c->M, c->K, c->NW, C, (const uint32_t *)c->dspins0.p, c->dwT.p,
                                    c->dbT.p, c->dtheta0.p, c->stream),
            "theta GEMM kernel");
-    const MixedShape sw = mixed_shape(c, cplx, 32, "NQES_G", "NQES_WPB", 8);   // k_sample_mx: 512 threads
+    const SweepShape sw = nq_sweep_shape(cplx, c->N, c->M, c->K, c->lds_max);  // k_sample_mx
+    if (!sw.ok) return c->fail(NQES_EINVAL, "M=%d too large for the sampler", c->M);
+    HIPCHK(c, c->dtabp.ensure(sw.tab_bytes), "alloc sampler tables");
     const MixedShape ms = mixed_shape(c, cplx, 16, "NQES_GM", "NQES_WPBM", 4);  // k_measure_mx: 256
     MixedSampleArgs a{};
     a.N = c->N;
@@ -444,7 +448,6 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.theta0 = c->dtheta0.p;
     a.spins0 = (const uint32_t *)c->dspins0.p;
     a.rng0 = (const uint64_t *)c->drng0.p;
-    a.tab_lds = sw.tab;
     a.O = c->svf ? nullptr : c->dO.p;
     a.Hs = c->dHs.p;
     a.Es = c->dEs.p;
@@ -453,7 +456,7 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.flags = static_cast<unsigned int *>(c->dflags.p);
     a.U = c->svf ? c->dU.p : nullptr;
     a.Ucols = c->svf_shape.ucols;
-    HIPCHK(c, nq_launch_sample_mx(cplx, c->K, a, sw.G, sw.wpb, c->stream), "mixed sample kernel");
+    HIPCHK(c, nq_launch_sweep_mx(cplx, c->K, a, sw, c->dtabp.p, c->stream), "mixed sample kernel");
     a.tab_lds = ms.tab;
     a.j_lds = ms.tab && nq_sample_mx_lds(cplx, c->N, c->M, c->K, ms.G, ms.wpb, true) +
                                 (size_t)c->N * c->N * sizeof(double) <= c->lds_max;
