@@ -65,7 +65,7 @@ struct MixedSampleArgs {
     const void *theta0;        // C*K*K*M T: angles at chain start [c][k][kk][j]
     const uint32_t *spins0;    // C*K*NW
     const uint64_t *rng0;      // 2*C pcg64 states after the chain-start draws
-    int tab_lds;               // stage the cosh/sinh tables in LDS
+    int tab_lds;               // measurement: stage the tables and J in LDS
     void *O;                   // S*Pld T
     void *Hs;                  // S*K*K complex double
     void *Es;                  // S complex double
@@ -74,7 +74,6 @@ struct MixedSampleArgs {
     unsigned int *flags;       // bit 0: an angle exceeded the FP32 range
     void *U;                   // (S*K) x Ucols T: S*v factors (NULL: not stored)
     int Ucols;
-    int j_lds;                 // measurement: stage J in LDS (after the tables)
 };
 
 // Factored S*v (sv_factored.hip): raw[p] = sum_s conj(O_sp) w_s, raw[P] = sum_s w_s,

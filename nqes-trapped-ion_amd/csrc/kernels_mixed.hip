@@ -139,7 +139,7 @@ __global__ __launch_bounds__(64) void k_theta_gemm(int N, int M, int K, int NW, 
 // ---------------------------------------------------------------- sampler
 // Per-step tables read by the sampler and the measurement: cosh(2w), sinh(2w), w
 // ([kk][i][j]), b ([kk][j]) in T and exp(2a), exp(-2a), a ([kk][i]) in D; in LDS when
-// a.tab_lds (mx_tab_bytes), else in global memory.
+// the kernel's TAB parameter (mx_tab_bytes), else in global memory.
 template <bool CPLX> struct MxTab {
     const typename Ty<CPLX>::T *ch, *sh, *w, *b;
     const typename Ty<CPLX>::D *eaP, *eaN, *aD;
@@ -151,9 +151,9 @@ __host__ __device__ inline size_t mx_tab_bytes(bool cplx, int N, int M, int K)
     return 3 * (size_t)K * N * ds + (3 * (size_t)K * N * M + (size_t)K * M) * ts;
 }
 
-// point tb at the tables (copied into LDS at p when a.tab_lds); returns the LDS
+// point tb at the tables (copied into LDS at p when TAB); returns the LDS
 // cursor after them.  D arrays first to keep their alignment.
-template <bool CPLX>
+template <bool CPLX, bool TAB>
 __device__ inline unsigned char *mx_stage_tables(const MixedSampleArgs &a, int K, MxTab<CPLX> &tb, unsigned char *p)
 {
     using T = typename Ty<CPLX>::T;
@@ -165,7 +165,7 @@ __device__ inline unsigned char *mx_stage_tables(const MixedSampleArgs &a, int K
     tb.eaP = (const D *)a.eaP;
     tb.eaN = (const D *)a.eaN;
     tb.aD = (const D *)a.aD;
-    if (!a.tab_lds) return p;
+    if constexpr (!TAB) return p;
     const long long KN = (long long)K * a.N, KNM = KN * a.M, KM = (long long)K * a.M;
     D *d = (D *)p;
     for (long long q = threadIdx.x; q < KN; q += blockDim.x) {
@@ -548,7 +548,7 @@ __device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &t
 // sample, 64/G samples per wavefront, each block looping over sample batches (grid
 // sized to the CUs) so that the tables are staged in LDS once per block.  Sample
 // parallelism (S) instead of chain parallelism (C) keeps every SIMD busy.
-template <bool CPLX, int K, int G, int JMAX>
+template <bool CPLX, int K, int G, int JMAX, bool TAB>
 __global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a)
 {
     using T = typename Ty<CPLX>::T;
@@ -563,9 +563,9 @@ __global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a)
     const long long S = (long long)a.C * a.m;
 
     MxTab<CPLX> tb;
-    unsigned char *p = mx_stage_tables<CPLX>(a, K, tb, smem);
+    unsigned char *p = mx_stage_tables<CPLX, TAB>(a, K, tb, smem);
     const double *Jm = a.J;
-    if (a.j_lds) {
+    if constexpr (TAB) {
         double *jl = (double *)p;
         for (int q = tid; q < N * N; q += blockDim.x) jl[q] = a.J[q];
         Jm = jl;
@@ -867,13 +867,13 @@ hipError_t launch_measure_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
     using T = typename Ty<CPLX>::T;
     const int spb = wpb * (64 / G);
     size_t lds = (size_t)spb * K * K * a.M * sizeof(T) + (size_t)spb * K * a.NW * 4;
-    if (a.tab_lds) lds += mx_tab_bytes(CPLX, a.N, a.M, K);
-    if (a.j_lds) lds += (size_t)a.N * a.N * sizeof(double);
+    const bool tab = a.tab_lds;  // tables and J in LDS
+    if (tab) lds += mx_tab_bytes(CPLX, a.N, a.M, K) + (size_t)a.N * a.N * sizeof(double);
     const long long S = (long long)a.C * a.m;
     long long blocks = (S + spb - 1) / spb;
+    const void *fn = tab ? (const void *)k_measure_mx<CPLX, K, G, 8, true> : (const void *)k_measure_mx<CPLX, K, G, 8, false>;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_measure_mx<CPLX, K, G, 8>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     // resident blocks: limited by LDS (160 KB per CU) and by 8 waves per SIMD
@@ -885,7 +885,8 @@ hipError_t launch_measure_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
     const char *eb = getenv("NQES_MEASURE_BLOCKS_PER_CU");
     if (eb && *eb && atoi(eb) > 0) per_cu = atoi(eb);
     if (blocks > per_cu * ncu) blocks = per_cu * ncu;
-    hipLaunchKernelGGL((k_measure_mx<CPLX, K, G, 8>), dim3((unsigned)blocks), dim3(64 * wpb), lds, st, a);
+    if (tab) hipLaunchKernelGGL((k_measure_mx<CPLX, K, G, 8, true>), dim3((unsigned)blocks), dim3(64 * wpb), lds, st, a);
+    else hipLaunchKernelGGL((k_measure_mx<CPLX, K, G, 8, false>), dim3((unsigned)blocks), dim3(64 * wpb), lds, st, a);
     return hipGetLastError();
 }
 

@@ -457,9 +457,8 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.U = c->svf ? c->dU.p : nullptr;
     a.Ucols = c->svf_shape.ucols;
     HIPCHK(c, nq_launch_sweep_mx(cplx, c->K, a, sw, c->dtabp.p, c->stream), "mixed sample kernel");
-    a.tab_lds = ms.tab;
-    a.j_lds = ms.tab && nq_sample_mx_lds(cplx, c->N, c->M, c->K, ms.G, ms.wpb, true) +
-                                (size_t)c->N * c->N * sizeof(double) <= c->lds_max;
+    a.tab_lds = ms.tab && nq_sample_mx_lds(cplx, c->N, c->M, c->K, ms.G, ms.wpb, true) +
+                                  (size_t)c->N * c->N * sizeof(double) <= c->lds_max;
     HIPCHK(c, nq_launch_measure_mx(cplx, c->K, a, ms.G, ms.wpb, c->stream), "mixed measure kernel");
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
     if (c->svf) {

@@ -183,6 +183,7 @@ __device__ __attribute__((always_inline)) inline void sweep_refresh(const MixedS
 #pragma unroll
             for (int k = 0; k < K; k++) bits[k] = sp[k * NW + w0];
             const int iend = (N - 32 * w0 < 32) ? N - 32 * w0 : 32;
+#pragma unroll 4
             for (int ii = 0; ii < iend; ii++) {
                 const int i = 32 * w0 + ii;
                 T wv[K][JQ];
@@ -203,7 +204,7 @@ __device__ __attribute__((always_inline)) inline void sweep_refresh(const MixedS
         }
     }
     // angles to the chain's LDS state, then one (replica, RBM) pair at a time: tanh in
-    // place, Phi_{k,kk} = exp(sum_i +-a_{kk,i}) prod_j cosh(theta_j) (sums in i order)
+    // place, Phi_{k,kk} = exp(sum_i +-a_{kk,i}) prod_j cosh(theta_j)
 #pragma unroll
     for (int k = 0; k < K; k++)
 #pragma unroll
@@ -222,13 +223,13 @@ __device__ __attribute__((always_inline)) inline void sweep_refresh(const MixedS
             *tp = tanhT(t);
             part = mulD(part, toD(coshT(t)));
         }
+        // sum_i +-a_{kk,i}: lane g takes i = g, g + G, ..., then a sum over the group
         D A = zeroD<D>();
-        for (int w0 = 0; w0 < NW; w0++) {
-            const uint32_t bits = sp[k * NW + w0];
-            const int iend = (N - 32 * w0 < 32) ? N - 32 * w0 : 32;
-            const D *ak = aD + kk * N + 32 * w0;
-            for (int ii = 0; ii < iend; ii++) A = ((bits >> ii) & 1u) ? subD(A, ak[ii]) : addD(A, ak[ii]);
+        for (int i = g; i < N; i += G) {
+            const D ai = aD[kk * N + i];
+            A = ((sp[k * NW + (i >> 5)] >> (i & 31)) & 1u) ? subD(A, ai) : addD(A, ai);
         }
+        A = group_sum<G>(A);
         const D ph = mulD(expD(A), group_prod<G>(part));
         if (g == 0) phis[pr] = ph;
     }
@@ -479,8 +480,8 @@ hipError_t nq_sweep_k<NQES_SWEEP_K>(bool cplx, const MixedSampleArgs &a, const v
 
 #if NQES_SWEEP_K == 1
 
-// G = 32 lanes per chain, or 64 when a lane would otherwise update more than 8 angles
-// per flip (K*JQ) or M > 256; NQES_G = 32 / 64 overrides.  JQ = ceil(M / G) rounded up to
+// G = 32 lanes per chain (measured faster than 64 at K = 2 and 4), 64 when M > 256;
+// NQES_G = 32 / 64 overrides.  JQ = ceil(M / G) rounded up to
 // 1, 2, 4 or 8.  Tables in LDS when they fit next to the spins.
 static int round_jq(int M, int G)
 {
@@ -493,7 +494,7 @@ static int round_jq(int M, int G)
 SweepShape nq_sweep_shape(bool cplx, int N, int M, int K, size_t lds_max)
 {
     SweepShape sh{};
-    int G = (round_jq(M, 32) > 8 || K * round_jq(M, 32) > 8) ? 64 : 32;
+    int G = round_jq(M, 32) > 8 ? 64 : 32;
     const char *e = getenv("NQES_G");
     if (e && (atoi(e) == 32 || atoi(e) == 64) && round_jq(M, atoi(e)) <= 8) G = atoi(e);
     const int JQ = round_jq(M, G);
