@@ -163,6 +163,41 @@ __device__ inline D group_prod(D v)
     if constexpr (G >= 64) v = mulD(v, xor_lane<32>(v));
     return v;
 }
+// FP32 exchange x[lane ^ MASK] within 8 lanes (MASK 1, 2, 7) written as a bare DPP
+// move, which the compiler folds into the consuming multiply (one VALU op per step).
+template <int MASK>
+__device__ inline float xor_lane_f(float x)
+{
+    static_assert(MASK == 1 || MASK == 2 || MASK == 7, "unsupported lane mask");
+    constexpr int ctrl = MASK == 1 ? 0xB1 : (MASK == 2 ? 0x4E : 0x141);
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), ctrl, 0xF, 0xF, true));
+}
+__device__ inline float mul_lane1(float v) { return v * xor_lane_f<1>(v); }
+__device__ inline float mul_lane2(float v) { return v * xor_lane_f<2>(v); }
+__device__ inline float mul_lane7(float v) { return v * xor_lane_f<7>(v); }
+__device__ inline cf cmul_lanes(cf v, float re, float im) { return mkf(v.re * re - v.im * im, v.re * im + v.im * re); }
+__device__ inline cf mul_lane1(cf v) { return cmul_lanes(v, xor_lane_f<1>(v.re), xor_lane_f<1>(v.im)); }
+__device__ inline cf mul_lane2(cf v) { return cmul_lanes(v, xor_lane_f<2>(v.re), xor_lane_f<2>(v.im)); }
+__device__ inline cf mul_lane7(cf v) { return cmul_lanes(v, xor_lane_f<7>(v.re), xor_lane_f<7>(v.im)); }
+
+// Product over the G lanes of a group of per-lane FP32 (T) factors: the steps within
+// 8 lanes in T, the rest in the FP64 type D.  With at most 8 factors per lane the T
+// part multiplies at most 64 factors of magnitude e^{+-2|w|}, far inside the FP32
+// range for any weights the solver reaches; the rounding it adds is of the same size
+// as that of the FP32 factors themselves.  Every lane ends with the same bits.
+template <int G, typename T, typename D>
+__device__ inline D group_prod_tf(T v)
+{
+    if constexpr (G >= 2) v = mul_lane1(v);
+    if constexpr (G >= 4) v = mul_lane2(v);
+    if constexpr (G >= 8) v = mul_lane7(v);
+    D d = toD(v);
+    if constexpr (G >= 16) d = mulD(d, xor_lane<15>(d));
+    if constexpr (G >= 32) d = mulD(d, xor_lane<16>(d));
+    if constexpr (G >= 64) d = mulD(d, xor_lane<32>(d));
+    return d;
+}
+
 template <int G, typename D>
 __device__ inline D group_sum(D v)
 {

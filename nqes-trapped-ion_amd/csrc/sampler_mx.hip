@@ -10,7 +10,7 @@
 //
 // Per flip of spin i of replica k:
 //   den_j = cosh(2w) - t_j sinh(+-2w),   t'_j = (t_j cosh(2w) - sinh(+-2w)) / den_j,
-//   Phi'_{k,kk} = Phi_{k,kk} exp(-+2a_i) prod_j den_j   (FP32 per lane, FP64 across lanes),
+//   Phi'_{k,kk} = Phi_{k,kk} exp(-+2a_i) prod_j den_j   (FP32 within 8 lanes, then FP64),
 //   det Phi' = sum_kk Phi'_{k,kk} cof_{k,kk}(Phi)       (Laplace along row k),
 // and the move is taken if u |det Phi|^2 <= |det Phi'|^2.  The cofactors of row k do
 // not depend on row k, so they are formed once per replica and sweep.  After every
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
                 D det_p = zeroD<D>();
 #pragma unroll
                 for (int kk = 0; kk < K; kk++) {
-                    const D Tp = group_prod<G>(toD(pf[kk]));
+                    const D Tp = group_prod_tf<G, T, D>(pf[kk]);
                     prow[kk] = mulD(mulD(prc[kk], ea[kk * N + i]), Tp);
                     det_p = addD(det_p, mulD(prow[kk], cof[kk]));
                 }
