@@ -65,7 +65,6 @@ struct MixedSampleArgs {
     const void *theta0;        // C*K*K*M T: angles at chain start [c][k][kk][j]
     const uint32_t *spins0;    // C*K*NW
     const uint64_t *rng0;      // 2*C pcg64 states after the chain-start draws
-    int tab_lds;               // measurement: stage the tables and J in LDS
     void *O;                   // S*Pld T
     void *Hs;                  // S*K*K complex double
     void *Es;                  // S complex double
@@ -151,10 +150,18 @@ struct SweepShape {
 SweepShape nq_sweep_shape(bool cplx, int N, int M, int K, size_t lds_max);
 template <int K>
 hipError_t nq_sweep_k(bool cplx, const MixedSampleArgs &a, const void *tabp, const SweepShape &sh, hipStream_t st);
+// Local energies and O / factor rows (sampler_mx.hip) from the sampler's padded tables
+struct MeasureShape {
+    int ok, G, JQ, wpb, tab;
+};
+MeasureShape nq_measure_shape(bool cplx, int N, int K, const SweepShape &sw, size_t lds_max);
+template <int K>
+hipError_t nq_measure_k(bool cplx, const MixedSampleArgs &a, const void *tabp, const MeasureShape &sh, hipStream_t st);
+hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, const MeasureShape &sh, const void *tabp,
+                                hipStream_t st);
 hipError_t nq_launch_sweep_mx(bool cplx, int K, const MixedSampleArgs &a, const SweepShape &sh, void *tabp,
                               hipStream_t st);
-hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st);
-size_t nq_sample_mx_lds(bool cplx, int N, int M, int K, int G, int wpb, bool tab_lds);
+
 hipError_t nq_launch_moments_mx(bool cplx, const void *O, const void *E, long long S, long long P, long long Pld,
                                 int nchunk, void *po, void *ps, void *pf, void *scratch, void *out_o, void *out_s,
                                 void *out_f, hipStream_t st);

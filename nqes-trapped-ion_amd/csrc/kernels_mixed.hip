@@ -9,11 +9,8 @@
 //   k_chain_init     per chain: pcg64(seed + c), K*N Bernoulli draws (rbm.cc:99-119)
 //   k_theta_gemm     Theta0[C*K, K*M] = Sigma[C*K, N] W[N, K*M] + b on FP32 MFMA
 //                    (v_mfma_f32_16x16x4f32), the angles at chain start (rbm.cc:135-145)
-//   (k_sample_mx, the Metropolis sweeps, is in sampler_mx.hip)
-//   k_measure_mx     G lanes per sample: theta, tanh, Phi, Phi^-1 from the stored spins,
-//                    local energy (rbm.cc:1042-1064) and the O row
-//                    O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k}
-//                    (Cramer's rule applied to rbm.cc:199-239) written to HBM in FP32
+//   (k_sample_mx and k_measure_mx, the Metropolis sweeps and the local energies with
+//   the O rows, are in sampler_mx.hip)
 //   k_moments_mx     column sums of O, |O|^2, E conj(O) in FP64 with Neumaier compensation
 //   k_sv_rows_mx     S*v in one streamed pass over O (rbm.cc:661-684): WPR wavefronts
 //                    share a row, w_s = (O_s - o)^T v, then acc += conj(O_s) w_s with
@@ -134,459 +131,6 @@ __global__ __launch_bounds__(64) void k_theta_gemm(int N, int M, int K, int NW, 
             if (orow < rows) theta0[orow * ncol + ocol] = acc[q] + bias;
         }
     }
-}
-
-// ---------------------------------------------------------------- sampler
-// Per-step tables read by the sampler and the measurement: cosh(2w), sinh(2w), w
-// ([kk][i][j]), b ([kk][j]) in T and exp(2a), exp(-2a), a ([kk][i]) in D; in LDS when
-// the kernel's TAB parameter (mx_tab_bytes), else in global memory.
-template <bool CPLX> struct MxTab {
-    const typename Ty<CPLX>::T *ch, *sh, *w, *b;
-    const typename Ty<CPLX>::D *eaP, *eaN, *aD;
-};
-
-__host__ __device__ inline size_t mx_tab_bytes(bool cplx, int N, int M, int K)
-{
-    const size_t ts = cplx ? 8 : 4, ds = cplx ? 16 : 8;
-    return 3 * (size_t)K * N * ds + (3 * (size_t)K * N * M + (size_t)K * M) * ts;
-}
-
-// point tb at the tables (copied into LDS at p when TAB); returns the LDS
-// cursor after them.  D arrays first to keep their alignment.
-template <bool CPLX, bool TAB>
-__device__ inline unsigned char *mx_stage_tables(const MixedSampleArgs &a, int K, MxTab<CPLX> &tb, unsigned char *p)
-{
-    using T = typename Ty<CPLX>::T;
-    using D = typename Ty<CPLX>::D;
-    tb.ch = (const T *)a.chT;
-    tb.sh = (const T *)a.shT;
-    tb.w = (const T *)a.wT;
-    tb.b = (const T *)a.bT;
-    tb.eaP = (const D *)a.eaP;
-    tb.eaN = (const D *)a.eaN;
-    tb.aD = (const D *)a.aD;
-    if constexpr (!TAB) return p;
-    const long long KN = (long long)K * a.N, KNM = KN * a.M, KM = (long long)K * a.M;
-    D *d = (D *)p;
-    for (long long q = threadIdx.x; q < KN; q += blockDim.x) {
-        d[q] = tb.eaP[q];
-        d[KN + q] = tb.eaN[q];
-        d[2 * KN + q] = tb.aD[q];
-    }
-    T *t = (T *)(d + 3 * KN);
-    for (long long q = threadIdx.x; q < KNM; q += blockDim.x) {
-        t[q] = tb.ch[q];
-        t[KNM + q] = tb.sh[q];
-        t[2 * KNM + q] = tb.w[q];
-    }
-    for (long long q = threadIdx.x; q < KM; q += blockDim.x) t[3 * KNM + q] = tb.b[q];
-    tb.eaP = d;
-    tb.eaN = d + KN;
-    tb.aD = d + 2 * KN;
-    tb.ch = t;
-    tb.sh = t + KNM;
-    tb.w = t + 2 * KNM;
-    tb.b = t + 3 * KNM;
-    return (unsigned char *)(t + 3 * KNM + KM);
-}
-template <typename T>
-__device__ inline T ld(const T *p)
-{
-    return *p;
-}
-
-__device__ inline float coshT(float x) { return coshf(x); }
-__device__ inline cf coshT(cf z)
-{
-    float sy, cy;
-    __sincosf(z.im, &sy, &cy);
-    float e = __expf(z.re), ie = __expf(-z.re);
-    return mkf(0.5f * (e + ie) * cy, 0.5f * (e - ie) * sy);
-}
-__device__ inline float reT(float x) { return x; }
-__device__ inline float reT(cf z) { return z.re; }
-
-__device__ inline double expD(double x) { return nq64::exp_(x); }
-__device__ inline cd expD(cd z) { return nq64::cexp_(z); }
-
-// Theta -> tanh (tl, LDS) and Phi for one chain state held by a group of G lanes.
-// Lane g owns hidden units j = g + G q.  Source: Theta0 of chain c (chain start) or
-// the spins sp (LDS).  Phi_{k,kk} = exp(sum_i +-a_i) prod_j cosh(theta_j) with the
-// cosh product taken per lane in FP64, then across the group.
-template <bool CPLX, int K, int G, int JMAX>
-__device__ inline void mx_refresh(const MixedSampleArgs &a, const MxTab<CPLX> &tb, const uint32_t *sp,
-                                  typename Ty<CPLX>::T *tl, int g, long long c, bool from_theta0,
-                                  typename Ty<CPLX>::D *phi, int &overflow)
-{
-    using T = typename Ty<CPLX>::T;
-    using D = typename Ty<CPLX>::D;
-    const int N = a.N, M = a.M, NW = a.NW;
-    const int jpl = (M + G - 1) / G;
-    const T *wT = tb.w;
-    const T *bT = tb.b;
-    const D *aD = tb.aD;
-    D part[K * K];
-#pragma unroll
-    for (int q = 0; q < K * K; q++) part[q] = oneD<D>();
-#pragma unroll
-    for (int q = 0; q < JMAX; q++) {
-        int j = g + G * q;
-        if (q < jpl && j < M) {
-#pragma unroll
-            for (int kk = 0; kk < K; kk++) {
-                T th[K];
-                if (from_theta0) {
-#pragma unroll
-                    for (int k = 0; k < K; k++) th[k] = ((const T *)a.theta0)[((c * K + k) * K + kk) * M + j];
-                } else {
-                    T b = bT[kk * M + j];
-#pragma unroll
-                    for (int k = 0; k < K; k++) th[k] = b;
-                    for (int i = 0; i < N; i++) {
-                        T w = wT[((long long)kk * N + i) * M + j];
-#pragma unroll
-                        for (int k = 0; k < K; k++) {
-                            int bit = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-                            th[k] = bit ? subT(th[k], w) : addT(th[k], w);
-                        }
-                    }
-                }
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    if (fabsf(reT(th[k])) > 80.0f) overflow = 1;
-                    tl[(k * K + kk) * M + j] = tanhT(th[k]);
-                    part[k * K + kk] = mulD(part[k * K + kk], toD(coshT(th[k])));
-                }
-            }
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < K; k++)
-#pragma unroll
-        for (int kk = 0; kk < K; kk++) {
-            D A = zeroD<D>();
-            for (int i = 0; i < N; i++) {
-                int bit = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-                D ai = aD[kk * N + i];
-                A = bit ? subD(A, ai) : addD(A, ai);
-            }
-            phi[k * K + kk] = mulD(expD(A), group_prod<G>(part[k * K + kk]));
-        }
-}
-
-// hs-like local energy (rbm.cc:1092-1119) of one sample held by a group of G lanes:
-// res row k = sum over pairs i > j of J_ij (2 Phi^(ij)_k - delta Phi_k) if
-// s^k_i != s^k_j, else J_ij delta Phi_k; H = Phi^-1 res.  Phi^(ij)_k flips spin i,
-// then spin j of replica k (the second flip on the angles after the first,
-// rbm.cc:1106-1108).  Lane g takes the pairs with index = g mod G; pairs with
-// J_ij == 0 are skipped.  Products over hidden units in T, flushed to D every 8.
-template <bool CPLX, int K, int G>
-__device__ inline void mx_hloc_hs(const MixedSampleArgs &a, const MxTab<CPLX> &tb, const double *Jm,
-                                  const uint32_t *sp, const typename Ty<CPLX>::T *tl, int g,
-                                  const typename Ty<CPLX>::D *phi, const typename Ty<CPLX>::D *phi_inv,
-                                  typename Ty<CPLX>::D *H)
-{
-    using T = typename Ty<CPLX>::T;
-    using D = typename Ty<CPLX>::D;
-    const int N = a.N, M = a.M, NW = a.NW;
-    const double delta = a.delta;
-    D res[K * K];
-#pragma unroll
-    for (int q = 0; q < K * K; q++) res[q] = zeroD<D>();
-    int pr = 0;
-    for (int i = 1; i < N; i++)
-        for (int j = 0; j < i; j++, pr++) {
-            if (pr % G != g) continue;
-            const double pij = Jm[(long long)i * N + j];
-            if (pij == 0.0) continue;
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                const int si = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-                const int sj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
-                if (si == sj) {
-#pragma unroll
-                    for (int kk = 0; kk < K; kk++) res[k * K + kk] = addD(res[k * K + kk], scaleD(pij * delta, phi[k * K + kk]));
-                    continue;
-                }
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) {
-                    const T *chi = tb.ch + ((long long)kk * N + i) * M, *shi = tb.sh + ((long long)kk * N + i) * M;
-                    const T *chj = tb.ch + ((long long)kk * N + j) * M, *shj = tb.sh + ((long long)kk * N + j) * M;
-                    const T *tk = tl + (k * K + kk) * M;
-                    D prod = oneD<D>();
-                    T pf = oneT<T>();
-                    for (int hj = 0; hj < M; hj++) {
-                        const T t = tk[hj];
-                        const T ci = chi[hj], s_i = si ? negT(shi[hj]) : shi[hj];
-                        const T den1 = subT(ci, mulT(t, s_i));
-                        const T t1 = divT(subT(mulT(t, ci), s_i), den1);
-                        const T s_j = sj ? negT(shj[hj]) : shj[hj];
-                        const T den2 = subT(chj[hj], mulT(t1, s_j));
-                        pf = mulT(pf, mulT(den1, den2));
-                        if ((hj & 7) == 7) {
-                            prod = mulD(prod, toD(pf));
-                            pf = oneT<T>();
-                        }
-                    }
-                    prod = mulD(prod, toD(pf));
-                    const D e = mulD(si ? tb.eaP[kk * N + i] : tb.eaN[kk * N + i], sj ? tb.eaP[kk * N + j] : tb.eaN[kk * N + j]);
-                    const D v = mulD(mulD(phi[k * K + kk], e), prod);
-                    res[k * K + kk] = addD(res[k * K + kk], scaleD(pij, subD(scaleD(2.0, v), scaleD(delta, phi[k * K + kk]))));
-                }
-            }
-        }
-#pragma unroll
-    for (int q = 0; q < K * K; q++) res[q] = group_sum<G>(res[q]);
-    matmul_m<K, D>(phi_inv, res, H);
-}
-
-// Local energy and O row of sample s from its state (sp, tl, phi) held by a group
-// of G lanes: E_zz (rbm.cc:1012-1031), H_loc = Phi^-1 (diag(E_zz) Phi - h sum_i
-// Phi^(i)) (rbm.cc:1042-1064) or the hs-like H_loc above, O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with
-// c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k} (Cramer's rule applied to rbm.cc:199-239).
-template <bool CPLX, int K, int G>
-__device__ inline void mx_measure(const MixedSampleArgs &a, const MxTab<CPLX> &tb, const double *Jm,
-                                  const uint32_t *sp, const typename Ty<CPLX>::T *tl, int g,
-                                  const typename Ty<CPLX>::D *phi, long long s, bool live)
-{
-    using T = typename Ty<CPLX>::T;
-    using D = typename Ty<CPLX>::D;
-    const int N = a.N, M = a.M, NW = a.NW, L = a.L;
-    const T *chT = tb.ch, *shT = tb.sh;
-    const D *eaP = tb.eaP;
-    const D *eaN = tb.eaN;
-    D phi_inv[K * K];
-    inv_m<K, D>(phi, phi_inv);
-
-    D H[K * K];
-    if (a.model != 0) {
-        mx_hloc_hs<CPLX, K, G>(a, tb, Jm, sp, tl, g, phi, phi_inv, H);
-    } else {
-    // E_zz per replica: lanes take rows i = g, g+G, ..., and the columns j < i of row i
-    // starting at a lane-dependent offset (rows of J share LDS banks)
-    double ez[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        double r = 0.0;
-        for (int i = g; i < N; i += G) {
-            const int bi = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
-            const double *Ji = Jm + (long long)i * N;
-            int j = g % N;
-            for (int jj = 0; jj < N; jj++) {
-                if (j < i) {
-                    const int bj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
-                    const double v = Ji[j];
-                    r += (bi ^ bj) ? -v : v;
-                }
-                if (++j == N) j = 0;
-            }
-        }
-        ez[k] = 2.0 * group_sum<G>(r);
-    }
-    // sum_i Phi^(i): lanes take spins i = g, g + G, ... two at a time; each table entry
-    // cosh/sinh(2 w_{kk,i,j}) read feeds all K replicas, so a lane runs 2 K independent
-    // product chains.  The j loop starts at a lane-dependent offset to spread LDS banks.
-    {
-        D sx[K * K];
-#pragma unroll
-        for (int q = 0; q < K * K; q++) sx[q] = zeroD<D>();
-#pragma unroll
-        for (int kk = 0; kk < K; kk++) {
-            for (int i0 = g; i0 < N; i0 += 2 * G) {
-                const int i1 = (i0 + G < N) ? i0 + G : i0;  // i1 == i0: second chain discarded
-                const bool two = i0 + G < N;
-                int si0[K], si1[K];
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    si0[k] = (sp[k * NW + (i0 >> 5)] >> (i0 & 31)) & 1u;
-                    si1[k] = (sp[k * NW + (i1 >> 5)] >> (i1 & 31)) & 1u;
-                }
-                const T *ch0 = chT + ((long long)kk * N + i0) * M, *sh0 = shT + ((long long)kk * N + i0) * M;
-                const T *ch1 = chT + ((long long)kk * N + i1) * M, *sh1 = shT + ((long long)kk * N + i1) * M;
-                D prod0[K], prod1[K];
-                T pf0[K], pf1[K];
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    prod0[k] = oneD<D>();
-                    prod1[k] = oneD<D>();
-                    pf0[k] = oneT<T>();
-                    pf1[k] = oneT<T>();
-                }
-                int j = g % M;
-                for (int jj = 0; jj < M; jj++) {
-                    const T c0 = ch0[j], s0 = sh0[j], c1 = ch1[j], s1 = sh1[j];
-#pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        const T t = tl[(k * K + kk) * M + j];
-                        pf0[k] = mulT(pf0[k], subT(c0, mulT(t, si0[k] ? negT(s0) : s0)));
-                        pf1[k] = mulT(pf1[k], subT(c1, mulT(t, si1[k] ? negT(s1) : s1)));
-                    }
-                    if ((jj & 7) == 7) {
-#pragma unroll
-                        for (int k = 0; k < K; k++) {
-                            prod0[k] = mulD(prod0[k], toD(pf0[k]));
-                            prod1[k] = mulD(prod1[k], toD(pf1[k]));
-                            pf0[k] = oneT<T>();
-                            pf1[k] = oneT<T>();
-                        }
-                    }
-                    if (++j == M) j = 0;
-                }
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    prod0[k] = mulD(prod0[k], toD(pf0[k]));
-                    const D e0 = si0[k] ? eaP[kk * N + i0] : eaN[kk * N + i0];
-                    sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e0), prod0[k]));
-                    if (two) {
-                        prod1[k] = mulD(prod1[k], toD(pf1[k]));
-                        const D e1 = si1[k] ? eaP[kk * N + i1] : eaN[kk * N + i1];
-                        sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e1), prod1[k]));
-                    }
-                }
-            }
-        }
-        D res[K * K];
-#pragma unroll
-        for (int k = 0; k < K; k++)
-#pragma unroll
-            for (int kk = 0; kk < K; kk++) {
-                D sxs = group_sum<G>(sx[k * K + kk]);
-                res[k * K + kk] = subD(scaleD(ez[k], phi[k * K + kk]), scaleD(a.h, sxs));
-            }
-        matmul_m<K, D>(phi_inv, res, H);
-    }
-    }
-    D energy = zeroD<D>();
-#pragma unroll
-    for (int k = 0; k < K; k++) energy = addD(energy, H[k * K + k]);
-    if (live) {
-        cd *Hs = (cd *)a.Hs;
-        cd *Es = (cd *)a.Es;
-#pragma unroll
-        for (int q = 0; q < K * K; q++)
-            if (g == q) {
-                cd hv;
-                if constexpr (CPLX) hv = H[q];
-                else hv = nq64::mk(H[q], 0.0);
-                Hs[s * K * K + q] = hv;
-            }
-        if (g == 0) {
-            cd ev;
-            if constexpr (CPLX) ev = energy;
-            else ev = nq64::mk(energy, 0.0);
-            Es[s] = ev;
-        }
-    }
-
-    // O row: O_{k,p} = sum_kk c_{k,kk} d_{kk,p}; factor rows U[s K + kk]
-    if (live && (a.O || a.U)) {
-        T cT[K * K];
-#pragma unroll
-        for (int k = 0; k < K; k++)
-#pragma unroll
-            for (int kk = 0; kk < K; kk++) cT[k * K + kk] = toT(mulD(phi_inv[k * K + kk], phi[kk * K + k]));
-        T *Orow = a.O ? (T *)a.O + s * a.Pld : nullptr;
-        T *Ub = a.U ? (T *)a.U + s * K * (long long)a.Ucols : nullptr;
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            T ck[K];
-#pragma unroll
-            for (int kk = 0; kk < K; kk++) ck[kk] = cT[k * K + kk];
-            T *ob = Orow + (long long)k * L;
-            // a block: sum_kk c sigma_i^kk
-            if (Orow)
-                for (int i = g; i < N; i += G) {
-                    T v = zeroT<T>();
-#pragma unroll
-                    for (int kk = 0; kk < K; kk++) {
-                        int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
-                        v = bit ? subT(v, ck[kk]) : addT(v, ck[kk]);
-                    }
-                    ob[i] = v;
-                }
-            // b block and w block: lanes over j
-            for (int j = g; j < M; j += G) {
-                T ct[K];
-                T v = zeroT<T>();
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) {
-                    ct[kk] = mulT(ck[kk], tl[(kk * K + k) * M + j]);
-                    v = addT(v, ct[kk]);
-                }
-                if (Ub)
-#pragma unroll
-                    for (int kk = 0; kk < K; kk++) Ub[(long long)kk * a.Ucols + k * M + j] = ct[kk];
-                if (!Orow) continue;
-                ob[N + j] = v;
-                for (int i = 0; i < N; i++) {
-                    T w = zeroT<T>();
-#pragma unroll
-                    for (int kk = 0; kk < K; kk++) {
-                        int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
-                        w = bit ? subT(w, ct[kk]) : addT(w, ct[kk]);
-                    }
-                    ob[N + M + (long long)i * M + j] = w;
-                }
-            }
-        }
-        if (Ub)
-            for (int col = K * M + g; col < a.Ucols; col += G)
-#pragma unroll
-                for (int kk = 0; kk < K; kk++) {
-                    T val = zeroT<T>();
-#pragma unroll
-                    for (int k = 0; k < K; k++)
-                        if (col - K * M == k) val = cT[k * K + kk];
-                    Ub[(long long)kk * a.Ucols + col] = val;
-                }
-        if (Orow)
-            for (long long q = a.P + g; q < a.Pld; q += G) Orow[q] = zeroT<T>();
-    }
-}
-
-// Local energies and O rows of all S samples from their stored spins: G lanes per
-// sample, 64/G samples per wavefront, each block looping over sample batches (grid
-// sized to the CUs) so that the tables are staged in LDS once per block.  Sample
-// parallelism (S) instead of chain parallelism (C) keeps every SIMD busy.
-template <bool CPLX, int K, int G, int JMAX, bool TAB>
-__global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a)
-{
-    using T = typename Ty<CPLX>::T;
-    using D = typename Ty<CPLX>::D;
-    constexpr int SPW = 64 / G;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int N = a.N, M = a.M, NW = a.NW;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int g = lane & (G - 1), grp = lane / G;
-    const int spb = (blockDim.x >> 6) * SPW;
-    const int sib = wv * SPW + grp;
-    const long long S = (long long)a.C * a.m;
-
-    MxTab<CPLX> tb;
-    unsigned char *p = mx_stage_tables<CPLX, TAB>(a, K, tb, smem);
-    const double *Jm = a.J;
-    if constexpr (TAB) {
-        double *jl = (double *)p;
-        for (int q = tid; q < N * N; q += blockDim.x) jl[q] = a.J[q];
-        Jm = jl;
-        p = (unsigned char *)(jl + N * N);
-    }
-    T *tl = (T *)p + (long long)sib * K * K * M;
-    uint32_t *sp = (uint32_t *)((T *)p + (long long)spb * K * K * M) + sib * K * NW;
-    int overflow = 0;
-    for (long long base = (long long)blockIdx.x * spb; base < S; base += (long long)gridDim.x * spb) {
-        const long long s_raw = base + sib;
-        const bool live = s_raw < S;
-        const long long s = live ? s_raw : S - 1;
-        __syncthreads();  // previous batch done with sp / tl (and the tables staged)
-        if (g == 0)
-            for (int w = 0; w < K * NW; w++) sp[w] = a.spins[s * K * NW + w];
-        __syncthreads();
-        D phi[K * K];
-        mx_refresh<CPLX, K, G, JMAX>(a, tb, sp, tl, g, 0, false, phi, overflow);
-        mx_measure<CPLX, K, G>(a, tb, Jm, sp, tl, g, phi, s, live);
-    }
-    if (overflow) atomicOr(a.flags, 1u);
 }
 
 // ---------------------------------------------------------------- moments
@@ -860,59 +404,6 @@ void chunk_reduce(const D *part, long long ncol, long long stride, int nchunk, D
                        ncol, ncol, ngroups, out);
 }
 
-// ---------------------------------------------------------------- launchers
-template <bool CPLX, int K, int G>
-hipError_t launch_measure_g(const MixedSampleArgs &a, hipStream_t st, int wpb)
-{
-    using T = typename Ty<CPLX>::T;
-    const int spb = wpb * (64 / G);
-    size_t lds = (size_t)spb * K * K * a.M * sizeof(T) + (size_t)spb * K * a.NW * 4;
-    const bool tab = a.tab_lds;  // tables and J in LDS
-    if (tab) lds += mx_tab_bytes(CPLX, a.N, a.M, K) + (size_t)a.N * a.N * sizeof(double);
-    const long long S = (long long)a.C * a.m;
-    long long blocks = (S + spb - 1) / spb;
-    const void *fn = tab ? (const void *)k_measure_mx<CPLX, K, G, 8, true> : (const void *)k_measure_mx<CPLX, K, G, 8, false>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    // resident blocks: limited by LDS (160 KB per CU) and by 8 waves per SIMD
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    long long per_cu = lds > 0 ? (long long)(160 * 1024) / (long long)lds : 8;
-    if (per_cu > 32 / wpb) per_cu = 32 / wpb;
-    if (per_cu < 1) per_cu = 1;
-    const char *eb = getenv("NQES_MEASURE_BLOCKS_PER_CU");
-    if (eb && *eb && atoi(eb) > 0) per_cu = atoi(eb);
-    if (blocks > per_cu * ncu) blocks = per_cu * ncu;
-    if (tab) hipLaunchKernelGGL((k_measure_mx<CPLX, K, G, 8, true>), dim3((unsigned)blocks), dim3(64 * wpb), lds, st, a);
-    else hipLaunchKernelGGL((k_measure_mx<CPLX, K, G, 8, false>), dim3((unsigned)blocks), dim3(64 * wpb), lds, st, a);
-    return hipGetLastError();
-}
-
-template <bool CPLX, int K>
-hipError_t launch_measure_k(const MixedSampleArgs &a, hipStream_t st, int G, int wpb)
-{
-    switch (G) {
-    case 16: return launch_measure_g<CPLX, K, 16>(a, st, wpb);
-    case 32: return launch_measure_g<CPLX, K, 32>(a, st, wpb);
-    case 64: return launch_measure_g<CPLX, K, 64>(a, st, wpb);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-template <bool CPLX>
-hipError_t launch_measure_c(int K, const MixedSampleArgs &a, hipStream_t st, int G, int wpb)
-{
-    switch (K) {
-    case 1: return launch_measure_k<CPLX, 1>(a, st, G, wpb);
-    case 2: return launch_measure_k<CPLX, 2>(a, st, G, wpb);
-    case 3: return launch_measure_k<CPLX, 3>(a, st, G, wpb);
-    case 4: return launch_measure_k<CPLX, 4>(a, st, G, wpb);
-    default: return hipErrorInvalidValue;
-    }
-}
-
 } // namespace
 
 hipError_t nq_chunk_reduce(bool cplx, const void *part, long long ncol, long long stride, int nchunk,
@@ -955,21 +446,6 @@ hipError_t nq_launch_theta_gemm(bool cplx, int N, int M, int K, int NW, int C, c
     hipLaunchKernelGGL(k_theta_gemm, grid, dim3(64), 0, st, N, M, K, NW, R, rows, spins0, (const float *)wT,
                        (const float *)bT, (float *)theta0);
     return hipGetLastError();
-}
-
-hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, int G, int wpb, hipStream_t st)
-{
-    return cplx ? launch_measure_c<true>(K, a, st, G, wpb) : launch_measure_c<false>(K, a, st, G, wpb);
-}
-
-size_t nq_sample_mx_lds(bool cplx, int N, int M, int K, int G, int wpb, bool tab_lds)
-{
-    size_t ts = cplx ? 8 : 4;
-    int NW = (N + 31) / 32;
-    int cpb = wpb * (64 / G);
-    size_t lds = (size_t)cpb * K * K * M * ts + (size_t)cpb * K * NW * 4;
-    if (tab_lds) lds += mx_tab_bytes(cplx, N, M, K);
-    return lds;
 }
 
 hipError_t nq_launch_moments_mx(bool cplx, const void *O, const void *E, long long S, long long P, long long Pld,

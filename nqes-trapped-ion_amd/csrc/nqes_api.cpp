@@ -171,24 +171,6 @@ static void drop_sv_graph(nqes_ctx *c)
 
 // lane-group size, waves per block and table staging of a mixed kernel with G lanes
 // per chain (sweeps) or per sample (measurement); the LDS layout is the same
-struct MixedShape {
-    int G, wpb, tab;
-};
-static MixedShape mixed_shape(const nqes_ctx *c, bool cplx, int g_dflt, const char *g_env_name,
-                              const char *wpb_env_name, int max_wpb)
-{
-    int G = g_dflt;
-    while (G < 64 && (c->M + G - 1) / G > 8) G *= 2;
-    int g_env = env_int(g_env_name, 0);
-    if ((g_env == 16 || g_env == 32 || g_env == 64) && (c->M + g_env - 1) / g_env <= 8) G = g_env;
-    int wpb = env_int(wpb_env_name, 4);
-    if (wpb < 1 || wpb > max_wpb) wpb = 4;
-    int tab = env_int("NQES_TABLDS", 1);
-    while (wpb > 1 && nq_sample_mx_lds(cplx, c->N, c->M, c->K, G, wpb, false) > c->lds_max) wpb--;
-    if (tab && nq_sample_mx_lds(cplx, c->N, c->M, c->K, G, wpb, true) > c->lds_max) tab = 0;
-    return MixedShape{G, wpb, tab};
-}
-
 extern "C" {
 
 const char *nqes_version(void) { return "nqes-mi355x 0.2 (gfx950)"; }
@@ -302,7 +284,8 @@ int nqes_problem_init(nqes_ctx *c, int N, int M, int K, double h, const double *
             return c->fail(NQES_EINVAL, "K*K*M=%d hidden-unit angles (%zu B) exceed the %zu B of LDS per block",
                            K * K * M, lds, c->lds_max);
     } else {
-        if (nq_sample_mx_lds(true, N, M, K, 64, 1, false) > c->lds_max)
+        const SweepShape sw = nq_sweep_shape(true, N, M, K, c->lds_max);
+        if (!sw.ok || !nq_measure_shape(true, N, K, sw, c->lds_max).ok)
             return c->fail(NQES_EINVAL, "K*K*M=%d hidden-unit angles exceed the LDS per block", K * K * M);
     }
     c->N = N;
@@ -423,7 +406,8 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     const SweepShape sw = nq_sweep_shape(cplx, c->N, c->M, c->K, c->lds_max);  // k_sample_mx
     if (!sw.ok) return c->fail(NQES_EINVAL, "M=%d too large for the sampler", c->M);
     HIPCHK(c, c->dtabp.ensure(sw.tab_bytes), "alloc sampler tables");
-    const MixedShape ms = mixed_shape(c, cplx, 16, "NQES_GM", "NQES_WPBM", 4);  // k_measure_mx: 256
+    const MeasureShape ms = nq_measure_shape(cplx, c->N, c->K, sw, c->lds_max);  // k_measure_mx
+    if (!ms.ok) return c->fail(NQES_EINVAL, "K*K*M=%d hidden-unit angles exceed the LDS per block", c->K * c->K * c->M);
     MixedSampleArgs a{};
     a.N = c->N;
     a.M = c->M;
@@ -457,9 +441,7 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.U = c->svf ? c->dU.p : nullptr;
     a.Ucols = c->svf_shape.ucols;
     HIPCHK(c, nq_launch_sweep_mx(cplx, c->K, a, sw, c->dtabp.p, c->stream), "mixed sample kernel");
-    a.tab_lds = ms.tab && nq_sample_mx_lds(cplx, c->N, c->M, c->K, ms.G, ms.wpb, true) +
-                                  (size_t)c->N * c->N * sizeof(double) <= c->lds_max;
-    HIPCHK(c, nq_launch_measure_mx(cplx, c->K, a, ms.G, ms.wpb, c->stream), "mixed measure kernel");
+    HIPCHK(c, nq_launch_measure_mx(cplx, c->K, a, ms, c->dtabp.p, c->stream), "mixed measure kernel");
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
     if (c->svf) {
         SvfArgs f = svf_args(c, S);

@@ -49,18 +49,27 @@ __device__ inline float reT(cf z) { return z.re; }
 __device__ inline double expD(double x) { return nq64::exp_(x); }
 __device__ inline cd expD(cd z) { return nq64::cexp_(z); }
 
-// padded table set: D eaP[K N], eaN[K N], aD[K N] (section rounded up to 16 bytes);
-// ChSh cs[K N Mp]; T w[K N Mp]; T b[K Mp].  Mp is a multiple of 32, so the total is a
-// multiple of 16 bytes.
-__host__ __device__ inline size_t pad_tab_dbytes(bool cplx, int N, int K)
+// Padded table set (byte offsets from its start, every section 16-byte aligned):
+//   D   eaP[K N], eaN[K N], aD[K N]
+//   cs  ChSh [K N][Mp + 1]   cosh/sinh pairs; the extra entry per row (1, 0) makes
+//                            the row stride odd, so lanes reading one hidden unit of
+//                            different rows hit different LDS banks
+//   w   T [K N][Mp],  b  T [K][Mp]
+struct PadLayout {
+    size_t cs, w, b, total;
+};
+__host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
+__host__ __device__ inline PadLayout pad_layout(bool cplx, int N, int K, int Mp)
 {
-    return (3 * (size_t)K * N * (cplx ? 16 : 8) + 15) & ~(size_t)15;
+    const size_t ts = cplx ? 8 : 4, ds = cplx ? 16 : 8;
+    PadLayout L;
+    L.cs = al16(3 * (size_t)K * N * ds);
+    L.w = L.cs + al16((size_t)K * N * (Mp + 1) * 2 * ts);
+    L.b = L.w + (size_t)K * N * Mp * ts;
+    L.total = al16(L.b + (size_t)K * Mp * ts);
+    return L;
 }
-__host__ __device__ inline size_t pad_tab_bytes(bool cplx, int N, int K, int Mp)
-{
-    const size_t ts = cplx ? 8 : 4;
-    return pad_tab_dbytes(cplx, N, K) + (size_t)K * N * Mp * 3 * ts + (size_t)K * Mp * ts;
-}
+__host__ __device__ inline size_t pad_tab_bytes(bool cplx, int N, int K, int Mp) { return pad_layout(cplx, N, K, Mp).total; }
 
 #if !defined(NQES_SWEEP_K) || NQES_SWEEP_K == 1
 template <bool CPLX>
@@ -70,35 +79,35 @@ __global__ void k_pad_tables_mx(int N, int M, int K, int Mp, const void *chT_, c
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
     const T *chT = (const T *)chT_, *shT = (const T *)shT_, *wT = (const T *)wT_, *bT = (const T *)bT_;
-    const long long KN = (long long)K * N, KNM = KN * Mp;
+    const PadLayout L = pad_layout(CPLX, N, K, Mp);
+    const long long KN = (long long)K * N, CSR = Mp + 1;
     D *d = (D *)out;
-    ChSh<T> *cs = (ChSh<T> *)((unsigned char *)out + pad_tab_dbytes(CPLX, N, K));
-    T *w = (T *)(cs + KNM);
-    T *b = w + KNM;
+    ChSh<T> *cs = (ChSh<T> *)((unsigned char *)out + L.cs);
+    T *w = (T *)((unsigned char *)out + L.w);
+    T *b = (T *)((unsigned char *)out + L.b);
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < KNM; q += stride) {
+    const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (long long q = t0; q < KN * CSR; q += stride) {
+        const long long row = q / CSR;
+        const int j = (int)(q % CSR);
+        ChSh<T> v;
+        v.ch = j < M ? chT[row * M + j] : oneT<T>();
+        v.sh = j < M ? shT[row * M + j] : zeroT<T>();
+        cs[q] = v;
+    }
+    for (long long q = t0; q < KN * Mp; q += stride) {
         const long long row = q / Mp;
         const int j = (int)(q % Mp);
-        ChSh<T> v;
-        v.ch = oneT<T>();
-        v.sh = zeroT<T>();
-        T wv = zeroT<T>();
-        if (j < M) {
-            v.ch = chT[row * M + j];
-            v.sh = shT[row * M + j];
-            wv = wT[row * M + j];
-        }
-        cs[q] = v;
-        w[q] = wv;
-        if (q < KN) {
-            d[q] = ((const D *)eaP_)[q];
-            d[KN + q] = ((const D *)eaN_)[q];
-            d[2 * KN + q] = ((const D *)aD_)[q];
-        }
-        if (q < (long long)K * Mp) {
-            const int kk = (int)(q / Mp), jj = (int)(q % Mp);
-            b[q] = jj < M ? bT[(long long)kk * M + jj] : zeroT<T>();
-        }
+        w[q] = j < M ? wT[row * M + j] : zeroT<T>();
+    }
+    for (long long q = t0; q < (long long)K * Mp; q += stride) {
+        const int kk = (int)(q / Mp), j = (int)(q % Mp);
+        b[q] = j < M ? bT[(long long)kk * M + j] : zeroT<T>();
+    }
+    for (long long q = t0; q < KN; q += stride) {
+        d[q] = ((const D *)eaP_)[q];
+        d[KN + q] = ((const D *)eaN_)[q];
+        d[2 * KN + q] = ((const D *)aD_)[q];
     }
 }
 #endif
@@ -272,10 +281,12 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
     } else {
         dtab = (const D *)tabp;
     }
+    const PadLayout PL = pad_layout(CPLX, N, K, Mp);
     const D *eaP = dtab, *eaN = dtab + KN, *aD = dtab + 2 * KN;
-    const ChSh<T> *cs = (const ChSh<T> *)((const unsigned char *)dtab + pad_tab_dbytes(CPLX, N, K));
-    const T *wp = (const T *)(cs + KN * Mp);
-    const T *bp = wp + KN * Mp;
+    const ChSh<T> *cs = (const ChSh<T> *)((const unsigned char *)dtab + PL.cs);
+    const T *wp = (const T *)((const unsigned char *)dtab + PL.w);
+    const T *bp = (const T *)((const unsigned char *)dtab + PL.b);
+    constexpr int CSR = Mp + 1;
     unsigned char *st = p + (size_t)cib * chain_lds_bytes<CPLX, K, G, JQ>(NW);
     D *phis = (D *)st;
     T *tls = (T *)(phis + K * K);
@@ -329,7 +340,7 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
 #pragma unroll
                 for (int kk = 0; kk < K; kk++)
 #pragma unroll
-                    for (int q = 0; q < JQ; q++) nx[kk][q] = cs[((long long)kk * N) * Mp + g + G * q];
+                    for (int q = 0; q < JQ; q++) nx[kk][q] = cs[((long long)kk * N) * CSR + g + G * q];
             }
             uint32_t cur = sp[k * NW];
             for (int i = 0; i < N; i++) {
@@ -338,7 +349,7 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
 #pragma unroll
                     for (int kk = 0; kk < K; kk++)
 #pragma unroll
-                        for (int q = 0; q < JQ; q++) cv[kk][q] = cs[((long long)kk * N + i) * Mp + g + G * q];
+                        for (int q = 0; q < JQ; q++) cv[kk][q] = cs[((long long)kk * N + i) * CSR + g + G * q];
                 } else {
                     const int i1 = (i + 1 < N) ? i + 1 : i;
 #pragma unroll
@@ -346,7 +357,7 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
 #pragma unroll
                         for (int q = 0; q < JQ; q++) {
                             cv[kk][q] = nx[kk][q];
-                            nx[kk][q] = cs[((long long)kk * N + i1) * Mp + g + G * q];
+                            nx[kk][q] = cs[((long long)kk * N + i1) * CSR + g + G * q];
                         }
                 }
                 if (i > 0 && (i & 31) == 0) cur = sp[k * NW + (i >> 5)];
@@ -424,6 +435,391 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
     if (live && overflow) atomicOr(a.flags, 1u);
 }
 
+// ---------------------------------------------------------------- measurement
+// hs-like local energy (rbm.cc:1092-1119) of one sample held by a group of G lanes:
+// res row k = sum over pairs i > j of J_ij (2 Phi^(ij)_k - delta Phi_k) if
+// s^k_i != s^k_j, else J_ij delta Phi_k; H = Phi^-1 res.  Phi^(ij)_k flips spin i,
+// then spin j of replica k (the second flip on the angles after the first,
+// rbm.cc:1106-1108).  Lane g takes the pairs with index = g mod G; pairs with
+// J_ij == 0 are skipped.  Products over hidden units in T, flushed to D every 8.
+template <bool CPLX, int K, int G, int Mp>
+__device__ inline void hloc_pairflip(const MixedSampleArgs &a, const ChSh<typename Ty<CPLX>::T> *cs,
+                                     const typename Ty<CPLX>::D *eaP, const typename Ty<CPLX>::D *eaN,
+                                     const double *Jm, const uint32_t *sp, const typename Ty<CPLX>::T *tl, int g,
+                                     const typename Ty<CPLX>::D *phi, const typename Ty<CPLX>::D *phi_inv,
+                                     typename Ty<CPLX>::D *H)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    constexpr int CSR = Mp + 1;
+    const int N = a.N, M = a.M, NW = a.NW;
+    const double delta = a.delta;
+    D res[K * K];
+#pragma unroll
+    for (int q = 0; q < K * K; q++) res[q] = zeroD<D>();
+    int pr = 0;
+    for (int i = 1; i < N; i++)
+        for (int j = 0; j < i; j++, pr++) {
+            if (pr % G != g) continue;
+            const double pij = Jm[(long long)i * N + j];
+            if (pij == 0.0) continue;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const int si = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                const int sj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
+                if (si == sj) {
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) res[k * K + kk] = addD(res[k * K + kk], scaleD(pij * delta, phi[k * K + kk]));
+                    continue;
+                }
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    const ChSh<T> *ri = cs + ((long long)kk * N + i) * CSR, *rj = cs + ((long long)kk * N + j) * CSR;
+                    const T *tk = tl + (k * K + kk) * Mp;
+                    D prod = oneD<D>();
+                    T pf = oneT<T>();
+                    for (int hj = 0; hj < M; hj++) {
+                        const T t = tk[hj];
+                        const ChSh<T> ci = ri[hj], cj = rj[hj];
+                        const T s_i = si ? negT(ci.sh) : ci.sh;
+                        const T den1 = subT(ci.ch, mulT(t, s_i));
+                        const T t1 = divT(subT(mulT(t, ci.ch), s_i), den1);
+                        const T s_j = sj ? negT(cj.sh) : cj.sh;
+                        const T den2 = subT(cj.ch, mulT(t1, s_j));
+                        pf = mulT(pf, mulT(den1, den2));
+                        if ((hj & 7) == 7) {
+                            prod = mulD(prod, toD(pf));
+                            pf = oneT<T>();
+                        }
+                    }
+                    prod = mulD(prod, toD(pf));
+                    const D e = mulD(si ? eaP[kk * N + i] : eaN[kk * N + i], sj ? eaP[kk * N + j] : eaN[kk * N + j]);
+                    const D v = mulD(mulD(phi[k * K + kk], e), prod);
+                    res[k * K + kk] = addD(res[k * K + kk], scaleD(pij, subD(scaleD(2.0, v), scaleD(delta, phi[k * K + kk]))));
+                }
+            }
+        }
+#pragma unroll
+    for (int q = 0; q < K * K; q++) res[q] = group_sum<G>(res[q]);
+    matmul_m<K, D>(phi_inv, res, H);
+}
+
+// Local energy and factor rows of sample s from its state (sp, tl [k][kk][Mp], phi)
+// held by a group of G lanes: E_zz (rbm.cc:1012-1031), H_loc = Phi^-1 (diag(E_zz) Phi
+// - h sum_i Phi^(i)) (rbm.cc:1042-1064) or the pair-flip H_loc above, and
+// O_{k,p} = sum_kk c_{k,kk} d_{kk,p} with c_{k,kk} = (Phi^-1)_{k,kk} Phi_{kk,k}
+// (Cramer's rule applied to rbm.cc:199-239) as the O row or the factor rows U.
+// sum_i Phi^(i): lanes take spins i = g, g + G, ... two at a time and run over all
+// hidden units j; every cosh/sinh pair read feeds the K replicas and every angle read
+// (the same address for the whole group: an LDS broadcast) feeds both spins.
+template <bool CPLX, int K, int G, int Mp>
+__device__ inline void measure_sample(const MixedSampleArgs &a, const ChSh<typename Ty<CPLX>::T> *cs,
+                                      const typename Ty<CPLX>::D *eaP, const typename Ty<CPLX>::D *eaN,
+                                      const double *Jm, const uint32_t *sp, const typename Ty<CPLX>::T *tl, int g,
+                                      const typename Ty<CPLX>::D *phi, long long s, bool live)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    constexpr int CSR = Mp + 1;
+    const int N = a.N, M = a.M, NW = a.NW, L = a.L;
+    D phi_inv[K * K];
+    inv_m<K, D>(phi, phi_inv);
+
+    D H[K * K];
+    if (a.model != 0) {
+        hloc_pairflip<CPLX, K, G, Mp>(a, cs, eaP, eaN, Jm, sp, tl, g, phi, phi_inv, H);
+    } else {
+        // E_zz per replica: lanes take rows i = g, g+G, ..., and the columns j < i of
+        // row i starting at a lane-dependent offset (rows of J share LDS banks)
+        double ez[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double r = 0.0;
+            for (int i = g; i < N; i += G) {
+                const int bi = (sp[k * NW + (i >> 5)] >> (i & 31)) & 1u;
+                const double *Ji = Jm + (long long)i * N;
+                int j = g % N;
+                for (int jj = 0; jj < N; jj++) {
+                    if (j < i) {
+                        const int bj = (sp[k * NW + (j >> 5)] >> (j & 31)) & 1u;
+                        const double v = Ji[j];
+                        r += (bi ^ bj) ? -v : v;
+                    }
+                    if (++j == N) j = 0;
+                }
+            }
+            ez[k] = 2.0 * group_sum<G>(r);
+        }
+        D sx[K * K];
+#pragma unroll
+        for (int q = 0; q < K * K; q++) sx[q] = zeroD<D>();
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            for (int i0 = g; i0 < N; i0 += 2 * G) {
+                const bool two = i0 + G < N;
+                const int i1 = two ? i0 + G : i0;  // i1 == i0: second product discarded
+                bool dn0[K], dn1[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    dn0[k] = (sp[k * NW + (i0 >> 5)] >> (i0 & 31)) & 1u;
+                    dn1[k] = (sp[k * NW + (i1 >> 5)] >> (i1 & 31)) & 1u;
+                }
+                const ChSh<T> *r0 = cs + ((long long)kk * N + i0) * CSR, *r1 = cs + ((long long)kk * N + i1) * CSR;
+                D prod0[K], prod1[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    prod0[k] = oneD<D>();
+                    prod1[k] = oneD<D>();
+                }
+                for (int j0 = 0; j0 < Mp; j0 += 8) {
+                    T pf0[K], pf1[K];
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        pf0[k] = oneT<T>();
+                        pf1[k] = oneT<T>();
+                    }
+#pragma unroll
+                    for (int jj = 0; jj < 8; jj++) {
+                        const ChSh<T> c0 = r0[j0 + jj], c1 = r1[j0 + jj];
+#pragma unroll
+                        for (int k = 0; k < K; k++) {
+                            const T t = tl[(k * K + kk) * Mp + j0 + jj];
+                            pf0[k] = mulT(pf0[k], subT(c0.ch, mulT(t, dn0[k] ? negT(c0.sh) : c0.sh)));
+                            pf1[k] = mulT(pf1[k], subT(c1.ch, mulT(t, dn1[k] ? negT(c1.sh) : c1.sh)));
+                        }
+                    }
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        prod0[k] = mulD(prod0[k], toD(pf0[k]));
+                        prod1[k] = mulD(prod1[k], toD(pf1[k]));
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    const D e0 = dn0[k] ? eaP[kk * N + i0] : eaN[kk * N + i0];
+                    sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e0), prod0[k]));
+                    if (two) {
+                        const D e1 = dn1[k] ? eaP[kk * N + i1] : eaN[kk * N + i1];
+                        sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e1), prod1[k]));
+                    }
+                }
+            }
+        }
+        D res[K * K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) {
+                const D sxs = group_sum<G>(sx[k * K + kk]);
+                res[k * K + kk] = subD(scaleD(ez[k], phi[k * K + kk]), scaleD(a.h, sxs));
+            }
+        matmul_m<K, D>(phi_inv, res, H);
+    }
+    D energy = zeroD<D>();
+#pragma unroll
+    for (int k = 0; k < K; k++) energy = addD(energy, H[k * K + k]);
+    if (live) {
+        cd *Hs = (cd *)a.Hs;
+        cd *Es = (cd *)a.Es;
+#pragma unroll
+        for (int q = 0; q < K * K; q++)
+            if (g == q) {
+                cd hv;
+                if constexpr (CPLX) hv = H[q];
+                else hv = nq64::mk(H[q], 0.0);
+                Hs[s * K * K + q] = hv;
+            }
+        if (g == 0) {
+            cd ev;
+            if constexpr (CPLX) ev = energy;
+            else ev = nq64::mk(energy, 0.0);
+            Es[s] = ev;
+        }
+    }
+
+    // O row: O_{k,p} = sum_kk c_{k,kk} d_{kk,p}; factor rows U[s K + kk]
+    if (live && (a.O || a.U)) {
+        T cT[K * K];
+#pragma unroll
+        for (int k = 0; k < K; k++)
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) cT[k * K + kk] = toT(mulD(phi_inv[k * K + kk], phi[kk * K + k]));
+        T *Orow = a.O ? (T *)a.O + s * a.Pld : nullptr;
+        T *Ub = a.U ? (T *)a.U + s * K * (long long)a.Ucols : nullptr;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            T ck[K];
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) ck[kk] = cT[k * K + kk];
+            T *ob = Orow + (long long)k * L;
+            // a block: sum_kk c sigma_i^kk
+            if (Orow)
+                for (int i = g; i < N; i += G) {
+                    T v = zeroT<T>();
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
+                        v = bit ? subT(v, ck[kk]) : addT(v, ck[kk]);
+                    }
+                    ob[i] = v;
+                }
+            // b block and w block: lanes over j
+            for (int j = g; j < M; j += G) {
+                T ct[K];
+                T v = zeroT<T>();
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    ct[kk] = mulT(ck[kk], tl[(kk * K + k) * Mp + j]);
+                    v = addT(v, ct[kk]);
+                }
+                if (Ub)
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) Ub[(long long)kk * a.Ucols + k * M + j] = ct[kk];
+                if (!Orow) continue;
+                ob[N + j] = v;
+                for (int i = 0; i < N; i++) {
+                    T w = zeroT<T>();
+#pragma unroll
+                    for (int kk = 0; kk < K; kk++) {
+                        int bit = (sp[kk * NW + (i >> 5)] >> (i & 31)) & 1u;
+                        w = bit ? subT(w, ct[kk]) : addT(w, ct[kk]);
+                    }
+                    ob[N + M + (long long)i * M + j] = w;
+                }
+            }
+        }
+        if (Ub)
+            for (int col = K * M + g; col < a.Ucols; col += G)
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    T val = zeroT<T>();
+#pragma unroll
+                    for (int k = 0; k < K; k++)
+                        if (col - K * M == k) val = cT[k * K + kk];
+                    Ub[(long long)kk * a.Ucols + col] = val;
+                }
+        if (Orow)
+            for (long long q = a.P + g; q < a.Pld; q += G) Orow[q] = zeroT<T>();
+    }
+}
+
+// Local energies and O rows (or factor rows) of all S samples from their stored
+// spins: G lanes per sample, 64/G samples per wavefront, each block looping over
+// sample batches (grid sized to the CUs) so that the tables are staged in LDS once
+// per block.  Angles and Phi from the spins as in the sweeps (sweep_refresh, lanes
+// over hidden units), then measure_sample.
+template <bool CPLX, int K, int G, int JQ, bool TAB>
+__global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a, const void *tabp)
+{
+    using T = typename Ty<CPLX>::T;
+    using D = typename Ty<CPLX>::D;
+    constexpr int SPW = 64 / G;
+    constexpr int Mp = G * JQ;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int N = a.N, NW = a.NW;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int g = lane & (G - 1), grp = lane / G;
+    const int spb = (blockDim.x >> 6) * SPW;
+    const int sib = wv * SPW + grp;
+    const long long S = (long long)a.C * a.m;
+    const long long KN = (long long)K * N;
+
+    const D *dtab;
+    const double *Jm = a.J;
+    unsigned char *p = smem;
+    const PadLayout PL = pad_layout(CPLX, N, K, Mp);
+    if constexpr (TAB) {
+        const uint4 *src = (const uint4 *)tabp;
+        uint4 *dst = (uint4 *)smem;
+        for (size_t q = tid; q < PL.total / 16; q += blockDim.x) dst[q] = src[q];
+        double *jl = (double *)(smem + PL.total);
+        for (int q = tid; q < N * N; q += blockDim.x) jl[q] = a.J[q];
+        dtab = (const D *)smem;
+        Jm = jl;
+        p = smem + al16(PL.total + (size_t)N * N * sizeof(double));
+    } else {
+        dtab = (const D *)tabp;
+    }
+    const D *eaP = dtab, *eaN = dtab + KN, *aD = dtab + 2 * KN;
+    const ChSh<T> *cs = (const ChSh<T> *)((const unsigned char *)dtab + PL.cs);
+    const T *wp = (const T *)((const unsigned char *)dtab + PL.w);
+    const T *bp = (const T *)((const unsigned char *)dtab + PL.b);
+    unsigned char *st = p + (size_t)sib * chain_lds_bytes<CPLX, K, G, JQ>(NW);
+    D *phis = (D *)st;
+    T *tls = (T *)(phis + K * K);
+    uint32_t *sp = (uint32_t *)(tls + K * K * Mp);
+    __syncthreads();
+    int overflow = 0;
+    for (long long base = (long long)blockIdx.x * spb; base < S; base += (long long)gridDim.x * spb) {
+        const long long s_raw = base + sib;
+        const bool live = s_raw < S;
+        const long long s = live ? s_raw : S - 1;
+        if (g == 0)
+            for (int w = 0; w < K * NW; w++) sp[w] = a.spins[s * K * NW + w];
+        __builtin_amdgcn_wave_barrier();
+        sweep_refresh<CPLX, K, G, JQ>(a, aD, wp, bp, sp, tls, phis, g, 0, false, overflow);
+        D phi[K * K];
+#pragma unroll
+        for (int q = 0; q < K * K; q++) phi[q] = phis[q];
+        measure_sample<CPLX, K, G, Mp>(a, cs, eaP, eaN, Jm, sp, tls, g, phi, s, live);
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (overflow) atomicOr(a.flags, 1u);
+}
+
+template <bool CPLX, int K, int G, int JQ>
+hipError_t launch_measure_jq(const MixedSampleArgs &a, const void *tabp, const MeasureShape &sh, hipStream_t st)
+{
+    const int spb = sh.wpb * (64 / G);
+    size_t lds = (size_t)spb * chain_lds_bytes<CPLX, K, G, JQ>(a.NW);
+    if (sh.tab) lds += al16(pad_tab_bytes(CPLX, a.N, K, G * JQ) + (size_t)a.N * a.N * sizeof(double));
+    const long long S = (long long)a.C * a.m;
+    long long blocks = (S + spb - 1) / spb;
+    const void *fn = sh.tab ? (const void *)k_measure_mx<CPLX, K, G, JQ, true> : (const void *)k_measure_mx<CPLX, K, G, JQ, false>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    // resident blocks: limited by LDS (160 KB per CU) and by 8 waves per SIMD
+    int dev = 0, ncu = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    long long per_cu = lds > 0 ? (long long)(160 * 1024) / (long long)lds : 8;
+    if (per_cu > 32 / sh.wpb) per_cu = 32 / sh.wpb;
+    if (per_cu < 1) per_cu = 1;
+    const char *eb = getenv("NQES_MEASURE_BLOCKS_PER_CU");
+    if (eb && *eb && atoi(eb) > 0) per_cu = atoi(eb);
+    if (blocks > per_cu * ncu) blocks = per_cu * ncu;
+    if (sh.tab) hipLaunchKernelGGL((k_measure_mx<CPLX, K, G, JQ, true>), dim3((unsigned)blocks), dim3(64 * sh.wpb), lds, st, a, tabp);
+    else hipLaunchKernelGGL((k_measure_mx<CPLX, K, G, JQ, false>), dim3((unsigned)blocks), dim3(64 * sh.wpb), lds, st, a, tabp);
+    return hipGetLastError();
+}
+
+template <bool CPLX, int K>
+hipError_t launch_measure_k(const MixedSampleArgs &a, const void *tabp, const MeasureShape &sh, hipStream_t st)
+{
+    // the shapes nq_measure_shape picks for this K
+    const int key = sh.G * 16 + sh.JQ;
+    if constexpr (K <= 2) {
+        switch (key) {
+        case 16 * 16 + 2: return launch_measure_jq<CPLX, K, 16, 2>(a, tabp, sh, st);
+        case 16 * 16 + 4: return launch_measure_jq<CPLX, K, 16, 4>(a, tabp, sh, st);
+        case 16 * 16 + 8: return launch_measure_jq<CPLX, K, 16, 8>(a, tabp, sh, st);
+        case 32 * 16 + 8: return launch_measure_jq<CPLX, K, 32, 8>(a, tabp, sh, st);
+        case 64 * 16 + 8: return launch_measure_jq<CPLX, K, 64, 8>(a, tabp, sh, st);
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (key) {
+        case 16 * 16 + 2: return launch_measure_jq<CPLX, K, 16, 2>(a, tabp, sh, st);
+        case 32 * 16 + 2: return launch_measure_jq<CPLX, K, 32, 2>(a, tabp, sh, st);
+        case 64 * 16 + 2: return launch_measure_jq<CPLX, K, 64, 2>(a, tabp, sh, st);
+        case 64 * 16 + 4: return launch_measure_jq<CPLX, K, 64, 4>(a, tabp, sh, st);
+        case 64 * 16 + 8: return launch_measure_jq<CPLX, K, 64, 8>(a, tabp, sh, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
+}
+
 template <bool CPLX, int K, int G, int JQ>
 hipError_t launch_jq(const MixedSampleArgs &a, const void *tabp, bool tab, int wpb, hipStream_t st)
 {
@@ -478,6 +874,14 @@ hipError_t nq_sweep_k<NQES_SWEEP_K>(bool cplx, const MixedSampleArgs &a, const v
     return cplx ? launch_k<true, NQES_SWEEP_K>(a, tabp, sh, st) : launch_k<false, NQES_SWEEP_K>(a, tabp, sh, st);
 }
 
+template <>
+hipError_t nq_measure_k<NQES_SWEEP_K>(bool cplx, const MixedSampleArgs &a, const void *tabp, const MeasureShape &sh,
+                                      hipStream_t st)
+{
+    return cplx ? launch_measure_k<true, NQES_SWEEP_K>(a, tabp, sh, st)
+                : launch_measure_k<false, NQES_SWEEP_K>(a, tabp, sh, st);
+}
+
 #if NQES_SWEEP_K == 1
 
 // G = 32 lanes per chain (measured faster than 64 at K = 2 and 4), 64 when M > 256;
@@ -519,6 +923,47 @@ SweepShape nq_sweep_shape(bool cplx, int N, int M, int K, size_t lds_max)
     sh.tab = (tb + states(wpb) <= lds_max) && !(et && *et && atoi(et) == 0);
     sh.tab_bytes = pad_tab_bytes(cplx, N, K, G * JQ);
     return sh;
+}
+
+// Measurement: the fewest lanes per sample (16, 32, 64) that keep the refresh's
+// K*K*JQ angles per lane (JQ = Mp / G) at <= 32 with JQ <= 8, else 64 lanes; tables
+// and J in LDS when they fit next to the per-sample states of 4 wavefronts.
+MeasureShape nq_measure_shape(bool cplx, int N, int K, const SweepShape &sw, size_t lds_max)
+{
+    MeasureShape sh{};
+    if (!sw.ok) return sh;
+    const int Mp = sw.G * sw.JQ;
+    sh.G = 64;
+    for (int G = 16; G <= 64; G *= 2)
+        if (Mp / G <= 8 && K * K * (Mp / G) <= 32) {
+            sh.G = G;
+            break;
+        }
+    sh.JQ = Mp / sh.G;
+    sh.wpb = 4;
+    const int NW = (N + 31) / 32;
+    const size_t per_sample =
+        (size_t)K * K * Mp * (cplx ? 8 : 4) + (size_t)K * K * (cplx ? 16 : 8) + (size_t)K * NW * 4;
+    const size_t states = (size_t)sh.wpb * (64 / sh.G) * per_sample;
+    if (lds_max > 0 && states > lds_max) return sh;
+    const size_t tb = al16(pad_tab_bytes(cplx, N, K, Mp) + (size_t)N * N * sizeof(double));
+    const char *et = getenv("NQES_TABLDS");
+    sh.tab = (tb + states <= lds_max) && !(et && *et && atoi(et) == 0);
+    sh.ok = 1;
+    return sh;
+}
+
+hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, const MeasureShape &sh, const void *tabp,
+                                hipStream_t st)
+{
+    if (!sh.ok) return hipErrorInvalidValue;
+    switch (K) {
+    case 1: return nq_measure_k<1>(cplx, a, tabp, sh, st);
+    case 2: return nq_measure_k<2>(cplx, a, tabp, sh, st);
+    case 3: return nq_measure_k<3>(cplx, a, tabp, sh, st);
+    case 4: return nq_measure_k<4>(cplx, a, tabp, sh, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t nq_launch_sweep_mx(bool cplx, int K, const MixedSampleArgs &a, const SweepShape &sh, void *tabp,
