@@ -112,7 +112,7 @@ hipError_t nq_launch_svf_sq_pack(bool cplx, const SvfArgs &a, const SvfShape &sh
 // int8 form of the factored S*v (real weights, N <= 64): factors quantised once per
 // sample set (nq_q8_pack), S*v on int8 MFMA (nq_q8_apply)
 struct Q8Shape {
-    int ok, ucols, nsp, fwd_cgt, fwd_cgs, fwd_blocks, bwd_waves, bwd_cgs, bwd_blocks;
+    int ok, ucols, nsp, fwd_cgt, fwd_cgs, fwd_blocks, fwd_threads, bwd_waves, bwd_cgs, bwd_blocks;
     size_t fwd_lds, vq_bytes, w_bytes, m_bytes;
     long long rpad;  // rows of Uq / escale (multiple of 64)
     int gblk_ints;

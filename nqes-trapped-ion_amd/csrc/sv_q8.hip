@@ -181,11 +181,11 @@ __device__ inline uint32_t spin_bytes4(uint32_t bits, uint32_t valid)
 }
 
 // Forward pass.  Block (b, g) handles column tiles [g cgt, ...) (their v operands in
-// LDS) and loops with 4 wavefronts over 16-row tiles.  Lane l: A operand = spins
+// LDS) and loops with its 4 or 8 wavefronts over 16-row tiles.  Lane l: A operand = spins
 // BPL (l/16) .. + BPL of row 16 rt + l%16; result rows 16 rt + 4 (l/16) + e at column
 // l%16 of each tile, matching the lane's four Uq words.  Output wrow[g R + r].
 template <int NSP>
-__global__ __launch_bounds__(256) void k_q8_fwd(SvfArgs a, int cgt)
+__global__ __launch_bounds__(512) void k_q8_fwd(SvfArgs a, int cgt)
 {
     using Op = FwdOp<NSP>;
     using AT = typename Op::T;
@@ -217,7 +217,8 @@ __global__ __launch_bounds__(256) void k_q8_fwd(SvfArgs a, int cgt)
     const long long ntile = (R + 15) / 16;
     const uint32_t *Uq = (const uint32_t *)a.Uq;
     double *wrow = (double *)a.wrow + (long long)blockIdx.y * R;
-    for (long long rt = (long long)blockIdx.x * 4 + wv; rt < ntile; rt += (long long)gridDim.x * 4) {
+    const int wpb = blockDim.x >> 6;
+    for (long long rt = (long long)blockIdx.x * wpb + wv; rt < ntile; rt += (long long)gridDim.x * wpb) {
         const long long ra = rt * 16 + lo;  // A operand row
         uint32_t w0 = 0, w1 = 0;
         if (ra < R) {
@@ -375,9 +376,11 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
     }
 }
 
-// Backward pass.  Wavefront wv of block (b, y, z) takes column tile ct = 4 y + wv and
-// spin tiles [z NIT, z NIT + NIT) over the chunks of block row b (every slice z forms
-// T, slice 0 writes the b block); wavefronts share nothing (no LDS, no barriers).  Per chunk: B operand of factor digit t = bytes t of
+// Backward pass.  Wavefront wv of block (b, y, z) takes column tile ct = (4 / zw) y +
+// wv / zw and spin tiles [z' NIT, z' NIT + NIT), z' = z zw + wv % zw, over the chunks of
+// block row b (every slice forms T, slice 0 writes the b block); the zw slices of one
+// column tile run in the same block, so the second reads the factor words from cache.
+// Wavefronts share nothing (no LDS, no barriers).  Per chunk: B operand of factor digit t = bytes t of
 // the lane's four Uq words of row tile q (a 4 x 4 byte transpose); A operand of spin
 // tile it, weight digit u: dword q = digits & mask.  Accumulator s = t + u collects
 // sum_r b X Omega products of weight 256^s, tacc[s] the column sums T.  The next
@@ -385,16 +388,18 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
 // (the current copies are only ever register copies of those loads).  Writes
 // part[b][0..P) for its columns: a and w blocks T - 2 sum b X Omega, b block T.
 template <int NIT>
-__global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk)
+__global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk,
+                                                int zw)
 {
     const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
     const int NCT = UC / 16;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
-    const int ct = blockIdx.y * 4 + wv;
+    const int ct = blockIdx.y * (4 / zw) + wv / zw;
     if (ct >= NCT) return;
     const int NIA = 16 * ((N + 15) / 16);  // spins of the mask array
-    const int i0 = blockIdx.z * NIT * 16;
+    const int zs = blockIdx.z * zw + wv % zw;
+    const int i0 = zs * NIT * 16;
     const long long R = a.S * K;
     const long long KM = (long long)K * M;
     const long long n64 = (R + 63) / 64;
@@ -491,7 +496,7 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
     double hsum = 0.0;
 #pragma unroll
     for (int s = NS - 1; s >= 0; s--) hsum = fma(hsum, 256.0, (double)ts[s]);
-    if (g == 0 && col < KM && blockIdx.z == 0) part[(col / M) * a.L + N + col % M] = hsum * osc;
+    if (g == 0 && col < KM && zs == 0) part[(col / M) * a.L + N + col % M] = hsum * osc;
 }
 
 int nit_of(int N) { return (N + 15) / 16; }
@@ -504,13 +509,15 @@ hipError_t launch_bwd(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
     hipLaunchKernelGGL((k_q8_omega<WSEL>), dim3(sh.bwd_blocks), dim3(256), 0, st, a, gblk, pd);
     const uint4 *msk = (const uint4 *)a.q8m;
     const dim3 bb(256);
-    const int nit = nit_of(a.N), ny = (a.Ucols / 16 + 3) / 4;
+    const int nit = nit_of(a.N), nct = a.Ucols / 16;
     if (nit == 1) {
-        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, ny, 1), bb, 0, st, a, gblk, pd, msk);
-    } else if (nit % 2 == 0) {  // slices of two spin tiles
-        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(sh.bwd_blocks, ny, nit / 2), bb, 0, st, a, gblk, pd, msk);
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, (nct + 3) / 4, 1), bb, 0, st, a, gblk, pd, msk, 1);
+    } else if (nit == 2) {  // one slice of two spin tiles
+        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(sh.bwd_blocks, (nct + 3) / 4, 1), bb, 0, st, a, gblk, pd, msk, 1);
+    } else if (nit == 4) {  // two slices of two spin tiles, in one block
+        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(sh.bwd_blocks, (nct + 1) / 2, 1), bb, 0, st, a, gblk, pd, msk, 2);
     } else {
-        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, ny, nit), bb, 0, st, a, gblk, pd, msk);
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, (nct + 3) / 4, nit), bb, 0, st, a, gblk, pd, msk, 1);
     }
     return hipGetLastError();
 }
@@ -539,6 +546,11 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R)
     sh.fwd_cgs = (nct + cgt - 1) / cgt;
     sh.fwd_lds = (size_t)cgt * per_tile;
     sh.fwd_blocks = 1280;  // measured: 5 wavefronts per SIMD
+    sh.fwd_threads = 256;
+    if ((160 * 1024 / sh.fwd_lds) * 4 < 16) {  // LDS allows < 4 wavefronts per SIMD: 8 per block
+        sh.fwd_threads = 512;
+        sh.fwd_blocks = 640;
+    }
     const int ngrp = (nct + 7) / 8;
     sh.bwd_cgs = ngrp;
     sh.bwd_waves = (nct + ngrp - 1) / ngrp;
@@ -584,8 +596,8 @@ hipError_t nq_q8_apply(const SvfArgs &a0, const Q8Shape &sh, void *scratch, void
     double *colscale = (double *)(vfrag + (size_t)nct * QV * 64 * 16);
     hipLaunchKernelGGL(k_q8_prep_v, dim3((UC + 63) / 64), dim3(64), 0, st, a, sh.nsp, vfrag, colscale, colscale + UC);
     const dim3 grid(sh.fwd_blocks, sh.fwd_cgs);
-    if (sh.nsp == 32) hipLaunchKernelGGL((k_q8_fwd<32>), grid, dim3(256), sh.fwd_lds, st, a, sh.fwd_cgt);
-    else hipLaunchKernelGGL((k_q8_fwd<64>), grid, dim3(256), sh.fwd_lds, st, a, sh.fwd_cgt);
+    if (sh.nsp == 32) hipLaunchKernelGGL((k_q8_fwd<32>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
+    else hipLaunchKernelGGL((k_q8_fwd<64>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = nq_launch_svf_combine(false, a, st);
