@@ -32,6 +32,8 @@ CASES = [
     (32, 64, 2, 32, 4, 2, False),
     (16, 160, 2, 16, 3, 2, True),
     (40, 80, 2, 16, 3, 2, False),
+    (80, 80, 2, 16, 3, 2, False),  # three spin words per replica (streamed S*v above 64 spins)
+    (72, 72, 1, 16, 3, 2, True),
 ]
 
 
