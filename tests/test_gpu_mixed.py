@@ -72,6 +72,30 @@ def test_mixed_matches_fp64(nqes, N, M, K, C, m, B, cplx):
         assert not np.any(b["O"].imag)
 
 
+@pytest.mark.parametrize("N,M,K,C,m,B,cplx", [CASES[0], CASES[1], CASES[3], CASES[6], CASES[8]])
+def test_tables_in_lds_or_global_same_bits(nqes, monkeypatch, N, M, K, C, m, B, cplx):
+    """The sweeps and the measurement read their padded cosh/sinh, w, b and exp(+-2a)
+    tables from LDS when they fit and from global memory otherwise (NQES_TABLDS=0
+    forces the latter); the arithmetic is the same, so spins, H_loc, O and the moments
+    agree bit for bit."""
+    print(f"case N={N} M={M} K={K} C={C} m={m} B={B} complex={cplx}", flush=True)
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=N + 3 * M + K, scale=0.1, complex_=cplx)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("NQES_TABLDS", mode)
+        with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+            s.sample(W, 4242, C, m, B, 0.5)
+            sp, H, acc = s.samples()
+            out[mode] = (sp, H, acc, s.O_rows(0, C * m), s.moments())
+    a, b = out["1"], out["0"]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[2], b[2])
+    assert np.array_equal(a[1], b[1])
+    assert np.array_equal(a[3], b[3])
+    for key in ("o", "skk", "f", "emat"):
+        assert np.array_equal(a[4][key], b[4][key]), key
+
+
 @pytest.fixture()
 def sv_stored(monkeypatch):
     monkeypatch.setenv("NQES_SV", "stored")
