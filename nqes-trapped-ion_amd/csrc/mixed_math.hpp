@@ -185,16 +185,41 @@ __device__ inline cf mul_lane7(cf v) { return cmul_lanes(v, xor_lane_f<7>(v.re),
 // part multiplies at most 64 factors of magnitude e^{+-2|w|}, far inside the FP32
 // range for any weights the solver reaches; the rounding it adds is of the same size
 // as that of the FP32 factors themselves.  Every lane ends with the same bits.
-template <int G, typename T, typename D>
+// x[lane ^ MASK] through the LDS crossbar (ds_bpermute): no VALU work besides the
+// consumer, for kernels bound by VALU issue
+template <int MASK>
+__device__ inline double xor_lane_lds(double v)
+{
+    const int addr = (int)((__lane_id() ^ MASK) << 2);
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)b);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, (uint64_t)lo | ((uint64_t)hi << 32));
+}
+template <int MASK>
+__device__ inline cd xor_lane_lds(cd v)
+{
+    return nq64::mk(xor_lane_lds<MASK>(v.re), xor_lane_lds<MASK>(v.im));
+}
+
+// LDSX: the FP64 exchanges through ds_bpermute (fewer VALU instructions, LDS latency
+// on the chain) instead of DPP / permlane moves
+template <int G, typename T, typename D, bool LDSX = false>
 __device__ inline D group_prod_tf(T v)
 {
     if constexpr (G >= 2) v = mul_lane1(v);
     if constexpr (G >= 4) v = mul_lane2(v);
     if constexpr (G >= 8) v = mul_lane7(v);
     D d = toD(v);
-    if constexpr (G >= 16) d = mulD(d, xor_lane<15>(d));
-    if constexpr (G >= 32) d = mulD(d, xor_lane<16>(d));
-    if constexpr (G >= 64) d = mulD(d, xor_lane<32>(d));
+    if constexpr (LDSX) {
+        if constexpr (G >= 16) d = mulD(d, xor_lane_lds<15>(d));
+        if constexpr (G >= 32) d = mulD(d, xor_lane_lds<16>(d));
+        if constexpr (G >= 64) d = mulD(d, xor_lane_lds<32>(d));
+    } else {
+        if constexpr (G >= 16) d = mulD(d, xor_lane<15>(d));
+        if constexpr (G >= 32) d = mulD(d, xor_lane<16>(d));
+        if constexpr (G >= 64) d = mulD(d, xor_lane<32>(d));
+    }
     return d;
 }
 

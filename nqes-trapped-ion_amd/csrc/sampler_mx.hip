@@ -382,7 +382,11 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
                 D det_p = zeroD<D>();
 #pragma unroll
                 for (int kk = 0; kk < K; kk++) {
-                    const D Tp = group_prod_tf<G, T, D>(pf[kk]);
+                    // K <= 2: VALU issue bounds the sweeps (2 wavefronts per SIMD), so the
+                    // cross-row exchanges go through the LDS crossbar; with more
+                    // replicas the register-heavy kernel runs fewer wavefronts and the
+                    // LDS latency would show
+                    const D Tp = group_prod_tf<G, T, D, (K <= 2)>(pf[kk]);
                     prow[kk] = mulD(mulD(prc[kk], ea[kk * N + i]), Tp);
                     det_p = addD(det_p, mulD(prow[kk], cof[kk]));
                 }
