@@ -397,14 +397,15 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
                     rng.hi = __shfl(sg.hi, G - 1, G);
                     tdraw = 0;
                 }
-                // draw tdraw of each group's batch: scalar reads of lane grp G + tdraw
-                double u = 0.0;
-#pragma unroll
-                for (int q = 0; q < CPW; q++) {
+                // draw tdraw of each group's batch: lane grp G + tdraw's value, fetched
+                // through the LDS crossbar (no VALU work but the address)
+                double u;
+                {
+                    const int addr = (grp * G + tdraw) << 2;
                     const uint64_t ubits = __builtin_bit_cast(uint64_t, ub);
-                    const uint32_t lo32 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)ubits, q * G + tdraw);
-                    const uint32_t hi32 = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(ubits >> 32), q * G + tdraw);
-                    if (grp == q) u = __builtin_bit_cast(double, (uint64_t)lo32 | ((uint64_t)hi32 << 32));
+                    const uint32_t lo32 = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)ubits);
+                    const uint32_t hi32 = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(ubits >> 32));
+                    u = __builtin_bit_cast(double, (uint64_t)lo32 | ((uint64_t)hi32 << 32));
                 }
                 tdraw++;
                 // U <= min(1, |det'|^2 / |det|^2) without the division (rbm.cc:542-543)
