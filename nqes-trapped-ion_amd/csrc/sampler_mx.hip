@@ -346,10 +346,13 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
             for (int i = 0; i < N; i++) {
                 ChSh<T> cv[K][JQ];
                 if constexpr (TAB) {
+                    // row (kk, i) = row (0, i) + kk N CSR: one lane address per flip, the
+                    // rest constant offsets
+                    const ChSh<T> *r0 = cs + (i * CSR + g);
 #pragma unroll
                     for (int kk = 0; kk < K; kk++)
 #pragma unroll
-                        for (int q = 0; q < JQ; q++) cv[kk][q] = cs[((long long)kk * N + i) * CSR + g + G * q];
+                        for (int q = 0; q < JQ; q++) cv[kk][q] = r0[kk * N * CSR + G * q];
                 } else {
                     const int i1 = (i + 1 < N) ? i + 1 : i;
 #pragma unroll
