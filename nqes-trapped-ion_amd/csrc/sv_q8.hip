@@ -19,7 +19,8 @@
 // The backward pass writes sigma = 1 - 2 b (b = 1 for spin down) and forms
 //   sum_r sigma_r[i] X_r[col] Omega_r = T[col] - 2 sum_r b_r[i] X_r[col] Omega_r,
 // the second sum digit by digit on MFMA (A operand = weight digits masked by b, 18
-// MFMAs per tile pair), T (also the b-block column sums) with v_dot4_i32_i8; products
+// MFMAs per tile pair), T (also the b-block column sums) by the same MFMAs on the
+// unmasked weight digits (18 per column tile and chunk); products
 // of equal digit weight 256^s share an int32 accumulator (exact), which goes to FP64
 // once per block.  The operator is the exact product of U~ with v and w quantised to
 // 2^-47 of their scale (w per block of rows): linear and symmetric to ~1e-13.
@@ -412,9 +413,9 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
     for (int it = 0; it < NIT; it++)
 #pragma unroll
         for (int s = 0; s < NS; s++) acc[it][s] = v4i{0, 0, 0, 0};
-    int tacc[NS];
+    v4i tacc[NS];  // T: the same products with the weight digits unmasked (all 16 rows equal)
 #pragma unroll
-    for (int s = 0; s < NS; s++) tacc[s] = 0;
+    for (int s = 0; s < NS; s++) tacc[s] = v4i{0, 0, 0, 0};
     const uint3 *Uq = (const uint3 *)a.Uq;
     uint3 nw[4];
     uint4 nd[QW], nm[NIT];
@@ -447,11 +448,12 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
 #pragma unroll
         for (int u = 0; u < QW; u++) {
             const uint32_t pw[4] = {cd[u].x, cd[u].y, cd[u].z, cd[u].w};
-#pragma unroll
-            for (int q = 0; q < 4; q++)
+            {
+                const v4i aall = v4i{(int)pw[0], (int)pw[1], (int)pw[2], (int)pw[3]};
 #pragma unroll
                 for (int t = 0; t < QU; t++)
-                    tacc[t + u] = __builtin_amdgcn_sdot4((int)pw[q], bp[t][q], tacc[t + u], false);
+                    tacc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aall, bp[t], tacc[t + u], 0, 0, 0);
+            }
 #pragma unroll
             for (int it = 0; it < NIT; it++) {
                 const v4i av = v4i{(int)(pw[0] & cm[it].x), (int)(pw[1] & cm[it].y), (int)(pw[2] & cm[it].z),
@@ -465,14 +467,9 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
     double *part = (double *)a.part + (long long)blockIdx.x * (a.P + 1);
     const double osc = ldexp(1.0, G - WBITS);
     const int col = 16 * ct + lo;
-    int ts[NS];
+    int ts[NS];  // column sums: every row of the T products holds them; row 4 g of lane l
 #pragma unroll
-    for (int s = 0; s < NS; s++) {
-        int x = tacc[s];
-        x += __shfl_xor(x, 16);
-        x += __shfl_xor(x, 32);
-        ts[s] = x;
-    }
+    for (int s = 0; s < NS; s++) ts[s] = tacc[s][0];
 #pragma unroll
     for (int it = 0; it < NIT; it++) {
 #pragma unroll
