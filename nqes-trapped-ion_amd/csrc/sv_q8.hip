@@ -390,13 +390,21 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
 // part[b][0..P) for its columns: a and w blocks T - 2 sum b X Omega, b block T.
 template <int NIT>
 __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk,
-                                                int zw)
+                                                int zw, int nyl)
 {
     const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
     const int NCT = UC / 16;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
-    const int ct = blockIdx.y * (4 / zw) + wv / zw;
+    // 1-D grid of nb ny blocks: the ny column-tile blocks of one row range are adjacent
+    // and the row ranges run from the last to the first, the reverse of the forward
+    // pass, so the rows it read last (still in the device's last-level cache) come first
+    const int ny = nyl;
+    const long long lin = blockIdx.x;
+    const long long nb = (long long)gridDim.x / ny;
+    const long long bx = nb - 1 - lin / ny;
+    const int by = (int)(lin % ny);
+    const int ct = by * (4 / zw) + wv / zw;
     if (ct >= NCT) return;
     const int NIA = 16 * ((N + 15) / 16);  // spins of the mask array
     const int zs = blockIdx.z * zw + wv % zw;
@@ -405,8 +413,8 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
     const long long KM = (long long)K * M;
     const long long n64 = (R + 63) / 64;
     long long c0, c1;
-    block_chunks(n64, blockIdx.x, gridDim.x, c0, c1);
-    const int G = gblk[blockIdx.x];
+    block_chunks(n64, bx, nb, c0, c1);
+    const int G = gblk[bx];
 
     v4i acc[NIT][NS];
 #pragma unroll
@@ -427,8 +435,9 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
 #pragma unroll
         for (int it = 0; it < NIT; it++) nm[it] = msk[(ch * NIA + i0 + 16 * it + lo) * 4 + g];
     };
-    if (c0 < c1) load_chunk(c0);
-    for (long long ch = c0; ch < c1; ch++) {
+    // chunks from the last to the first (the forward pass went the other way)
+    if (c0 < c1) load_chunk(c1 - 1);
+    for (long long ch = c1 - 1; ch >= c0; ch--) {
         uint3 cw[4];
         uint4 cd[QW], cm[NIT];
 #pragma unroll
@@ -437,7 +446,7 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
         for (int u = 0; u < QW; u++) cd[u] = nd[u];
 #pragma unroll
         for (int it = 0; it < NIT; it++) cm[it] = nm[it];
-        if (ch + 1 < c1) load_chunk(ch + 1);
+        if (ch - 1 >= c0) load_chunk(ch - 1);
         v4i bp[QU];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -464,7 +473,7 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
             }
         }
     }
-    double *part = (double *)a.part + (long long)blockIdx.x * (a.P + 1);
+    double *part = (double *)a.part + bx * (a.P + 1);
     const double osc = ldexp(1.0, G - WBITS);
     const int col = 16 * ct + lo;
     int ts[NS];  // column sums: every row of the T products holds them; row 4 g of lane l
@@ -503,14 +512,19 @@ hipError_t launch_bwd(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
     const uint4 *msk = (const uint4 *)a.q8m;
     const dim3 bb(256);
     const int nit = nit_of(a.N), nct = a.Ucols / 16;
+    const unsigned nb = (unsigned)sh.bwd_blocks;
     if (nit == 1) {
-        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, (nct + 3) / 4, 1), bb, 0, st, a, gblk, pd, msk, 1);
+        const int ny = (nct + 3) / 4;
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(nb * ny, 1, 1), bb, 0, st, a, gblk, pd, msk, 1, ny);
     } else if (nit == 2) {  // one slice of two spin tiles
-        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(sh.bwd_blocks, (nct + 3) / 4, 1), bb, 0, st, a, gblk, pd, msk, 1);
+        const int ny = (nct + 3) / 4;
+        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(nb * ny, 1, 1), bb, 0, st, a, gblk, pd, msk, 1, ny);
     } else if (nit == 4) {  // two slices of two spin tiles, in one block
-        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(sh.bwd_blocks, (nct + 1) / 2, 1), bb, 0, st, a, gblk, pd, msk, 2);
+        const int ny = (nct + 1) / 2;
+        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(nb * ny, 1, 1), bb, 0, st, a, gblk, pd, msk, 2, ny);
     } else {
-        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(sh.bwd_blocks, (nct + 3) / 4, nit), bb, 0, st, a, gblk, pd, msk, 1);
+        const int ny = (nct + 3) / 4;
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(nb * ny, 1, nit), bb, 0, st, a, gblk, pd, msk, 1, ny);
     }
     return hipGetLastError();
 }
