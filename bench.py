@@ -283,8 +283,17 @@ def main():
         nw = (args.N + 31) // 32
         sv_bytes = 2 * R * ucols * 3 + 2 * R * 8 + R * 4 * nw + 2 * S * 8
         nsp = 32 if args.N <= 32 else 64
-        sv_ops = 2 * R * ucols * (6 * nsp + 18 * 16 * ((args.N + 15) // 16))  # int8 MFMA ops, both passes
-        sv_kernel = "S*v: k_q8_fwd + k_q8_bwd (int8 MFMA over quantised c*tanh(theta) factors)"
+        nst = (args.N + 15) // 16
+        ncg64 = (ucols + 63) // 64
+        gemm = args.N <= 32 and ncg64 <= 4 and os.environ.get("NQES_Q8_FWD") != "tiles"
+        # int8 MFMA ops issued: backward 18 16x16x64 MFMAs per 64 rows, column tile and
+        # spin tile; forward either 6 per 16 rows and column tile (tiled form) or, as one
+        # GEMM, 3 (6 nst + 1) 16x16x64 MFMAs per 16 rows and 64 columns
+        bwd_ops = 2 * R * ucols * 18 * 16 * nst
+        fwd_ops = R * ncg64 * 3 * (6 * nst + 1) * 2 * 16 * 64 if gemm else 2 * R * ucols * 6 * nsp
+        sv_ops = fwd_ops + bwd_ops
+        fwd_name = "k_q8_fwdg" if gemm else "k_q8_fwd"
+        sv_kernel = f"S*v: {fwd_name} + k_q8_bwd (int8 MFMA over quantised c*tanh(theta) factors)"
     elif factored:
         # two FP64-MFMA passes over the factors: sigma^T B (R x N x (K M + K)) and the
         # transposed product, R = S K rows; complex weights double the real products

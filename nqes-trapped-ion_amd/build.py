@@ -42,6 +42,7 @@ SOURCES = [
     ("sampler_mx.hip", ["-DNQES_SWEEP_K=4"], "sampler_mx_k4"),
     ("sv_factored.hip", []),
     ("sv_q8.hip", []),
+    ("sv_q8_gemm.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("nqes_api.cpp", EXACT),
     ("serve.cpp", EXACT),
     ("eig.cpp", EXACT),

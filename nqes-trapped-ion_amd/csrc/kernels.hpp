@@ -92,8 +92,9 @@ struct SvfArgs {
     const void *E;             // S complex double: local energies (moments)
     int wsel;                  // backward-pass weights: 0 w, 1 ones, 2 E
     int ncg;                   // forward column groups (wrow holds ncg x S*K partial sums)
-    const void *Uq;            // int8 form: quantised factors (16 x 16 tiles of digit words)
-    const double *escale;      // int8 form: per-row factor scale 2^(e_r - 26)
+    const void *Uq;            // int8 form: quantised factors (16 x 16 tiles of digit planes)
+    const void *Uf;            // int8 form, GEMM forward pass: the same digits row-major per tile
+    const double *escale;      // int8 form: per-row factor scale
     void *vq;                  // int8 form: v operands, column scales and biases
     void *q8w;                 // int8 form: backward block scales and weight digits
     void *q8m;                 // int8 form: spin masks of the backward A operand
@@ -113,13 +114,16 @@ hipError_t nq_launch_svf_sq_pack(bool cplx, const SvfArgs &a, const SvfShape &sh
 // sample set (nq_q8_pack), S*v on int8 MFMA (nq_q8_apply)
 struct Q8Shape {
     int ok, ucols, nsp, fwd_cgt, fwd_cgs, fwd_blocks, fwd_threads, bwd_waves, bwd_cgs, bwd_blocks;
-    size_t fwd_lds, vq_bytes, w_bytes, m_bytes;
+    int gemm, ncg64, nst;  // GEMM forward pass (N <= 32): 64-column groups, spin tiles
+    size_t fwd_lds, vq_bytes, w_bytes, m_bytes, uf_bytes, qg_bytes;
     long long rpad;  // rows of Uq / escale (multiple of 64)
     int gblk_ints;
 };
 Q8Shape nq_q8_shape(int N, int M, int K, long long R);
 hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
 hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st);
+// its forward pass as one GEMM over the columns (sh.gemm; sv_q8_gemm.hip): wrow[r], a.ncg = 1
+hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
 // moments from the quantised factors (sums with w = 1 and w = E on the int8 backward
 // pass, |O|^2 on FP64 MFMA); after nq_q8_pack
 hipError_t nq_q8_moments(const SvfArgs &a, const Q8Shape &sh, const SvfShape &svf, void *part, void *scratch,

@@ -103,6 +103,7 @@ struct nqes_ctx {
     DevBuf dv, dov, dw, dsvpart, dsvwpart, dsvout, dsvscratch, dmscratch;
     DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
     DevBuf dUq, descale, dvq, dq8w, dq8m;              // its int8 form (real weights)
+    DevBuf dUf;                                        // row-major digits (GEMM forward pass)
     hipGraph_t svg = nullptr;                          // int8 S*v sequence, captured once per
     hipGraphExec_t svx = nullptr;                      // sample set and replayed per MINRES step
     double *hv_pin = nullptr, *hr_pin = nullptr;       // pinned v | o^T v and result (P + 1 each)
@@ -245,7 +246,8 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
         b += (nchunk / 32 + 2) * (P + 1) * 16 + 3 * (P + 1) * 16;
         const Q8Shape q8 = nq_q8_shape(N, M, K, S * K);  // int8 form, used for real weights
         if (q8.ok)
-            b += q8.rpad * (long long)q8.ucols * 3 + q8.rpad * 8 + (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes);
+            b += q8.rpad * (long long)q8.ucols * 3 + q8.rpad * 8 +
+                 (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes + q8.uf_bytes);
     } else {
         b += S * Pld * (long long)o_elem(precision, cplx);  // O
         if (mixed) b += (mch / 32 + 2) * P * 16;              // moments scratch
@@ -378,6 +380,7 @@ static SvfArgs svf_args(nqes_ctx *c, long long S)
     f.wsel = 0;
     f.ncg = 1;
     f.Uq = c->dUq.p;
+    f.Uf = c->dUf.p;
     f.escale = static_cast<const double *>(c->descale.p);
     f.vq = c->dvq.p;
     f.q8w = c->dq8w.p;
@@ -515,6 +518,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
             HIPCHK(c, c->dvq.ensure(c->q8_shape.vq_bytes), "alloc S*v operands");
             HIPCHK(c, c->dq8w.ensure(c->q8_shape.w_bytes), "alloc S*v weight digits");
             HIPCHK(c, c->dq8m.ensure(c->q8_shape.m_bytes), "alloc spin masks");
+            if (c->q8_shape.gemm) HIPCHK(c, c->dUf.ensure(c->q8_shape.uf_bytes), "alloc row-major factor digits");
         }
         HIPCHK(c, c->dU.ensure((size_t)S * K * c->svf_shape.ucols * tsz), "alloc S*v factors");
         HIPCHK(c, c->dwrow.ensure((size_t)cgs * S * K * 16), "alloc S*v rows");

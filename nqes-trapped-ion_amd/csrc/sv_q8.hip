@@ -16,6 +16,10 @@
 //             Omega = rint(w / f_r 2^(46 - G)), six digits.
 // The forward pass forms Y = Sigma Q (six MFMAs per tile, exact) and
 //   w_r = (1 / f_r) sum_col X (Y 2^(g - 46) + bias) in FP64.
+// For N <= 32 and at most 256 columns the forward pass is instead one GEMM over the
+// columns with one scale for all of v (sv_q8_gemm.hip, on a row-major copy Uf of the
+// factor digits that the pack kernel writes next to the tiles); NQES_Q8_FWD=tiles keeps
+// the tiled form.
 // The backward pass writes sigma = 1 - 2 b (b = 1 for spin down) and forms
 //   sum_r sigma_r[i] X_r[col] Omega_r = T[col] - 2 sum_r b_r[i] X_r[col] Omega_r,
 // the second sum digit by digit on MFMA (A operand = weight digits masked by b, 18
@@ -40,24 +44,13 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "cplx64.hpp"
 #include "kernels.hpp"
+#include "sv_q8_common.hpp"
 
 namespace {
-
-typedef int v4i __attribute__((ext_vector_type(4)));
-
-constexpr int QU = 3;            // factor digits
-constexpr int QV = 6;            // v digits
-constexpr int QW = 6;            // weight digits
-constexpr int NS = QU + QW - 1;  // digit weights 256^s of the backward products
-constexpr double XMAX = 8355711.0;  // 127 (1 + 256 + 65536): |X| <= XMAX
-constexpr int VBITS = 46;        // |Q| <= 2^46
-constexpr int WBITS = 46;        // |Omega| <= 2^46
-constexpr int MAX_BLOCK_CHUNKS = 256;  // 16384 rows: |int32 sums| <= 3 2^14 2^14 < 2^30
-
-__device__ inline long long bal8l(long long x) { return (long long)(int8_t)(uint8_t)(x & 255); }
 
 // four 0/1 bits -> four 0/1 bytes
 __device__ inline uint32_t expand4(uint32_t b) { return ((b & 15u) * 0x00204081u) & 0x01010101u; }
@@ -79,7 +72,7 @@ __device__ inline double xor_sum16(double x)
 // Pack: U (FP32, [R][UC] row-major) -> Uq tiles and escale[r] = 1 / f_r.  One
 // wavefront per 16-row tile, rows [R, Rp) written as zeros.
 __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, long long ntile, int UC, uint32_t *Uq,
-                                                 double *escale)
+                                                 double *escale, uint4 *Uf)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
@@ -120,6 +113,39 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
             uint32_t *o = Uq + ((rt * NCT + ct) * 64 + lane) * QU;
 #pragma unroll
             for (int t = 0; t < QU; t++) o[t] = pl[t];
+        }
+        if (Uf) {
+            // row-major copy for the GEMM forward pass: chunk (ct, lo) of plane t holds the
+            // 16 digits of row 16 rt + lo at columns 16 ct .. 16 ct + 15; this lane writes
+            // the chunks ct = g, g + 4, ...  Its row's scale comes from the lane holding it.
+            double scr = 0.0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const double x = __shfl(sc[j], 16 * (lo >> 2) + lo);
+                if ((lo & 3) == j) scr = x;
+            }
+            const long long r = rt * 16 + lo;
+            for (int ct = g; ct < NCT; ct += 4) {
+                uint32_t pl[QU][4] = {};
+#pragma unroll
+                for (int e4 = 0; e4 < 4; e4++) {
+                    const float4 u4 = r < R ? *(const float4 *)(U + r * UC + 16 * ct + 4 * e4) : float4{0.f, 0.f, 0.f, 0.f};
+                    const float uu[4] = {u4.x, u4.y, u4.z, u4.w};
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        long long x = (long long)rint((double)uu[e] * scr);
+#pragma unroll
+                        for (int t = 0; t < QU; t++) {
+                            const long long d = (t < QU - 1) ? bal8l(x) : x;
+                            pl[t][e4] |= ((uint32_t)d & 255u) << (8 * e);
+                            x = (x - d) >> 8;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < QU; t++)
+                    Uf[(rt * QU + t) * NCT * 16 + 16 * ct + lo] = uint4{pl[t][0], pl[t][1], pl[t][2], pl[t][3]};
+            }
         }
     }
 }
@@ -572,6 +598,20 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R)
     if (e && *e && atoi(e) > 0) sh.fwd_blocks = atoi(e);
     sh.rpad = n64 * 64;
     sh.vq_bytes = (size_t)nct * QV * 64 * 16 + 2 * (size_t)UC * sizeof(double);
+    // GEMM forward pass: N <= 32, at most 4 groups of 64 columns (their v digits in
+    // 52 KB of LDS); a row-major copy of the factor digits (3 R Ucols bytes)
+    sh.ncg64 = (UC + 63) / 64;
+    sh.nst = (N + 15) / 16;
+    sh.qg_bytes = (size_t)sh.ncg64 * (sh.nst * QV + 1) * 64 * 16 + 16;
+    const char *eg = getenv("NQES_Q8_FWD");  // "tiles": the tiled forward pass instead
+    sh.gemm = N <= 32 && sh.ncg64 <= 4 && !(eg && strcmp(eg, "tiles") == 0);
+    sh.uf_bytes = sh.gemm ? (size_t)(n64 * 4) * QU * UC * 16 : 0;
+    if (sh.gemm && sh.qg_bytes > sh.vq_bytes) sh.vq_bytes = sh.qg_bytes;
+    if (sh.gemm) {
+        sh.fwd_blocks = 512;  // 2 wavefronts per SIMD (accumulators and the next tile's digits)
+        e = getenv("NQES_Q8_FWD_BLOCKS");
+        if (e && *e && atoi(e) > 0) sh.fwd_blocks = atoi(e);
+    }
     sh.gblk_ints = (sh.bwd_blocks + 3) / 4 * 4;  // keeps pn 16-byte aligned
     sh.w_bytes = (size_t)sh.gblk_ints * 4 + (size_t)n64 * QW * 4 * 16;
     sh.m_bytes = (size_t)n64 * 16 * ((N + 15) / 16) * 4 * 16;
@@ -586,7 +626,7 @@ hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
     long long blocks = (ntile + 3) / 4;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_q8_pack, dim3((unsigned)blocks), dim3(256), 0, st, (const float *)a.U, R, ntile, a.Ucols,
-                       (uint32_t *)a.Uq, (double *)a.escale);
+                       (uint32_t *)a.Uq, (double *)a.escale, sh.gemm ? (uint4 *)a.Uf : (uint4 *)nullptr);
     const int NI = 16 * nit_of(a.N);
     const long long n64 = sh.rpad / 64, nthr = n64 * NI * 4;
     hipLaunchKernelGGL(k_q8_masks, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, a.spins, R, a.N, a.NW, NI,
@@ -597,14 +637,20 @@ hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
 hipError_t nq_q8_apply(const SvfArgs &a0, const Q8Shape &sh, void *scratch, void *out, hipStream_t st)
 {
     SvfArgs a = a0;
-    a.ncg = sh.fwd_cgs;
     const int UC = a.Ucols, nct = UC / 16;
-    uint8_t *vfrag = (uint8_t *)a.vq;
-    double *colscale = (double *)(vfrag + (size_t)nct * QV * 64 * 16);
-    hipLaunchKernelGGL(k_q8_prep_v, dim3((UC + 63) / 64), dim3(64), 0, st, a, sh.nsp, vfrag, colscale, colscale + UC);
-    const dim3 grid(sh.fwd_blocks, sh.fwd_cgs);
-    if (sh.nsp == 32) hipLaunchKernelGGL((k_q8_fwd<32>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
-    else hipLaunchKernelGGL((k_q8_fwd<64>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
+    if (sh.gemm) {
+        a.ncg = 1;
+        const hipError_t e = nq_q8_fwd_gemm(a, sh, st);
+        if (e != hipSuccess) return e;
+    } else {
+        a.ncg = sh.fwd_cgs;
+        uint8_t *vfrag = (uint8_t *)a.vq;
+        double *colscale = (double *)(vfrag + (size_t)nct * QV * 64 * 16);
+        hipLaunchKernelGGL(k_q8_prep_v, dim3((UC + 63) / 64), dim3(64), 0, st, a, sh.nsp, vfrag, colscale, colscale + UC);
+        const dim3 grid(sh.fwd_blocks, sh.fwd_cgs);
+        if (sh.nsp == 32) hipLaunchKernelGGL((k_q8_fwd<32>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
+        else hipLaunchKernelGGL((k_q8_fwd<64>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = nq_launch_svf_combine(false, a, st);

@@ -1,0 +1,275 @@
+// sv_q8_gemm.hip -- forward pass of the int8 S*v (real weights, N <= 32) as one GEMM
+// over the columns: P_ri = sum_c X_rc Q_ci on v_mfma_i32_16x16x64_i8, then the signs of
+// the spins and the digit weights per row.  The factor digits come in a row-major copy
+// (Uf, written by the pack kernel of sv_q8.hip next to its tiles): per 16-row tile and
+// digit plane t, 16 Ucols bytes, chunk 16 ct + l (l < 16) = the 16 digits of row l of
+// the tile at columns 16 ct .. 16 ct + 15.  Compiled with MFMA results in VGPRs (the
+// sums are read by VALU code once per tile).  Reference: the factored S*v of
+// rbm.cc:661-684, as in sv_q8.hip.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+#include "kernels.hpp"
+#include "sv_q8_common.hpp"
+
+namespace {
+
+// sum over the 16 lanes of a row by DPP (quad swaps, then half-row and row mirrors);
+// every lane ends with the same value
+__device__ inline double dpp_sum16(double x)
+{
+    auto step = [](double v, auto ctrl) -> double {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_mov_dpp((int)b, decltype(ctrl)::value, 0xF, 0xF, true);
+        const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), decltype(ctrl)::value, 0xF, 0xF, true);
+        return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    };
+    x = step(x, std::integral_constant<int, 0xB1>{});
+    x = step(x, std::integral_constant<int, 0x4E>{});
+    x = step(x, std::integral_constant<int, 0x141>{});
+    x = step(x, std::integral_constant<int, 0x140>{});
+    return x;
+}
+
+// ---- GEMM forward pass (N <= 32, Ucols <= 256).
+//   w_r = (1 / f_r) 2^(g - 46) sum_c X_rc (sum_i sigma_ri Q_ci + bq_c)
+// with one scale 2^g > max |v| for all of v and its bias (absolute error 2^-47 of the
+// largest entry, the size of FP64 rounding in a dot product).  The sum over columns
+// comes first, on int8 MFMAs of 64 columns, P_ri = sum_c X_rc Q_ci for 16 rows x 16
+// spins per spin tile and digit pair (t, u); the signs after it.  The bias takes one
+// more tile per column group whose spin slot u holds digit u of bq, so that its 18 digit
+// pairs cost 3 MFMAs.  Operands: A lane l, byte e = digit of row 16 rt + l%16 at column
+// 64 cg + 16 (l/16) + e (the Uf chunk 64 cg + l); B lane l, byte e = digit of the same
+// column for spin 16 st + l%16 (slot st QV + u), or digit l%16 of bq (slot NST QV).
+// Qg: [cg][slot][lane] uint4, then the scale 2^(g - 46) as a double.
+__global__ __launch_bounds__(1024) void k_q8_prep_vg(SvfArgs a, int ncg, int nst, uint4 *qg)
+{
+    __shared__ double red[16];
+    const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
+    const long long KM = (long long)K * M;
+    const double *v = (const double *)a.v;
+    auto Bval = [&](int col, int i) -> double {
+        if (col < KM) return v[(long long)(col / M) * a.L + N + M + (long long)i * M + col % M];
+        if (col < KM + K) return v[(long long)(col - KM) * a.L + i];
+        return 0.0;
+    };
+    auto bias = [&](int col) -> double { return col < KM ? v[(long long)(col / M) * a.L + N + col % M] : 0.0; };
+    // maximum over v: the parameters of the K chains (L apart), N + M + N M each
+    double mx = 0.0;
+    const long long NP = (long long)N + M + (long long)N * M;
+    for (long long q = threadIdx.x; q < K * NP; q += blockDim.x) mx = fmax(mx, fabs(v[(q / NP) * a.L + q % NP]));
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) mx = fmax(mx, __shfl_xor(mx, off));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) mx = fmax(mx, red[w]);
+    int gex = 0;
+    if (mx > 0.0) (void)frexp(mx, &gex);
+    const double sc = ldexp(1.0, VBITS - gex);
+    // thread (cg, st, lane); st = nst: the bias
+    const int lane = threadIdx.x & 63, lo = lane & 15;
+    const int cg = (threadIdx.x >> 6) / (nst + 1), st = (threadIdx.x >> 6) % (nst + 1);
+    const int nslot = nst * QV + 1;
+    if (cg < ncg) {
+        uint32_t dg[QV][4] = {};
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int col = 64 * cg + 16 * (lane >> 4) + e;
+            double val = 0.0;
+            if (col < UC) {
+                if (st < nst) val = 16 * st + lo < N ? Bval(col, 16 * st + lo) : 0.0;
+                else val = bias(col);
+            }
+            long long x = (long long)rint(val * sc);
+#pragma unroll
+            for (int u = 0; u < QV; u++) {
+                const long long d = (u < QV - 1) ? bal8l(x) : x;
+                dg[u][e >> 2] |= ((uint32_t)d & 255u) << (8 * (e & 3));
+                x = (x - d) >> 8;
+            }
+        }
+        uint4 *o = qg + (size_t)cg * nslot * 64;
+        if (st < nst) {
+#pragma unroll
+            for (int u = 0; u < QV; u++) o[(st * QV + u) * 64 + lane] = uint4{dg[u][0], dg[u][1], dg[u][2], dg[u][3]};
+        } else {
+            // slot nst QV, lane (g, lo): digit lo of the bias columns of lane group g
+            uint4 b = {0u, 0u, 0u, 0u};
+#pragma unroll
+            for (int u = 0; u < QV; u++) {
+                const int src = 16 * (lane >> 4) + u;  // every lane of the group holds the bias digits
+                const uint4 x = {__shfl(dg[u][0], src), __shfl(dg[u][1], src), __shfl(dg[u][2], src), __shfl(dg[u][3], src)};
+                if (lo == u) b = x;
+            }
+            o[nst * QV * 64 + lane] = b;
+        }
+    }
+    if (threadIdx.x == 0) *(double *)(qg + (size_t)ncg * nslot * 64) = mx > 0.0 ? ldexp(1.0, gex - VBITS) : 0.0;
+}
+
+// GEMM forward pass.  Wavefronts loop over 16-row tiles; the digits of the next tile
+// load while this one is multiplied (NCG (NST QV + 1) QU MFMAs into 8 int32 accumulators
+// per spin tile, digit weights 256^(t + u), and 3 for the bias).  Per row: each lane
+// sums its spin (FP64 Horner over the digit weights, sign of its spin) and bias digit,
+// then 16 lanes by DPP.  Output wrow[r].
+template <int NST, int NCG>
+__global__ __launch_bounds__(256, 2) void k_q8_fwdg(SvfArgs a)
+{
+    constexpr int NSF = QU + QV - 1;
+    constexpr int NSLOT = NST * QV + 1;
+    constexpr int NQ = NCG * NSLOT * 64;
+    extern __shared__ __align__(16) unsigned char smem_g[];
+    uint4 *qs = (uint4 *)smem_g;
+    {
+        const uint4 *src = (const uint4 *)a.vq;
+        for (int q = threadIdx.x; q < NQ; q += blockDim.x) qs[q] = src[q];
+    }
+    const double vsc = *(const double *)((const uint4 *)a.vq + NQ);
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int g = lane >> 4, lo = lane & 15;
+    const int NW = a.NW, NCT = a.Ucols / 16;
+    const long long R = a.S * a.K;
+    const long long ntile = (R + 15) / 16;
+    const uint4 *Uf = (const uint4 *)a.Uf;
+    double *wrow = (double *)a.wrow;
+    const double pw = lo < QV ? ldexp(1.0, 8 * lo) : 0.0;  // weight of this lane's bias digit
+    const long long step = (long long)gridDim.x * (blockDim.x >> 6);
+    // Loads are unconditional (no branch between a load and its use, so the waits count
+    // only older loads): lanes past the last column read chunk 16 NCT - 1, whose products
+    // meet zero v digits; rows past R read row R - 1 (their sums are not written); past
+    // the last tile, the last tile again.
+    const int nq16 = 16 * NCT;
+    uint4 an[NCG][QU];  // the next tile's digits, spin words and row scales
+    uint32_t wn[4];
+    double en[4];
+    auto load = [&](long long rt) {
+#pragma unroll
+        for (int cg = 0; cg < NCG; cg++) {
+            const int q = 64 * cg + lane < nq16 ? 64 * cg + lane : nq16 - 1;
+#pragma unroll
+            for (int t = 0; t < QU; t++) an[cg][t] = Uf[(rt * QU + t) * nq16 + q];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const long long r = rt * 16 + 4 * g + e, rc = r < R ? r : R - 1;
+            wn[e] = a.spins[rc * NW];
+            en[e] = a.escale[rc];
+        }
+    };
+    long long rt = (long long)blockIdx.x * (blockDim.x >> 6) + wv;
+    if (rt < ntile) load(rt);
+    for (; rt < ntile; rt += step) {
+        uint4 av[NCG][QU];
+        uint32_t word[4];
+        double es[4];
+#pragma unroll
+        for (int cg = 0; cg < NCG; cg++)
+#pragma unroll
+            for (int t = 0; t < QU; t++) av[cg][t] = an[cg][t];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            word[e] = wn[e];
+            es[e] = en[e];
+        }
+        load(rt + step < ntile ? rt + step : rt);
+        // an opaque lane offset keeps the v operand reads inside the loop (hoisted, they
+        // would hold NCG NSLOT 4 registers)
+        int ql = lane;
+        asm volatile("" : "+v"(ql));
+        // Phases: the bias tile, then each spin tile, the FP64 sums of one phase written
+        // after the MFMAs of the next (independent of them: they issue while those run;
+        // measured 6% faster than all sums after all MFMAs).
+        // Lane (g, lo), register e: row 16 rt + 4 g + e, spin 16 st + lo / bias digit lo.
+        double h[4];
+        v4i accb[QU];
+#pragma unroll
+        for (int t = 0; t < QU; t++) accb[t] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int cg = 0; cg < NCG; cg++) {
+            v4i at[QU], bb;
+#pragma unroll
+            for (int t = 0; t < QU; t++) __builtin_memcpy(&at[t], &av[cg][t], 16);
+            const uint4 b4 = qs[(cg * NSLOT + NST * QV) * 64 + ql];
+            __builtin_memcpy(&bb, &b4, 16);
+#pragma unroll
+            for (int t = 0; t < QU; t++) accb[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(at[t], bb, accb[t], 0, 0, 0);
+        }
+        v4i accp[NSF];  // sums of the previous spin tile
+#pragma unroll
+        for (int st = 0; st <= NST; st++) {
+            v4i acc[NSF];
+            if (st < NST) {
+#pragma unroll
+                for (int s = 0; s < NSF; s++) acc[s] = v4i{0, 0, 0, 0};
+#pragma unroll
+                for (int cg = 0; cg < NCG; cg++) {
+                    v4i at[QU];
+#pragma unroll
+                    for (int t = 0; t < QU; t++) __builtin_memcpy(&at[t], &av[cg][t], 16);
+#pragma unroll
+                    for (int u = 0; u < QV; u++) {
+                        v4i bv;
+                        const uint4 b4 = qs[(cg * NSLOT + st * QV + u) * 64 + ql];
+                        __builtin_memcpy(&bv, &b4, 16);
+#pragma unroll
+                        for (int t = 0; t < QU; t++)
+                            acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(at[t], bv, acc[t + u], 0, 0, 0);
+                    }
+                }
+            }
+            if (st == 0) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    double x = (double)accb[QU - 1][e];
+#pragma unroll
+                    for (int t = QU - 2; t >= 0; t--) x = fma(x, 256.0, (double)accb[t][e]);
+                    h[e] = x * pw;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    double hs = (double)accp[NSF - 1][e];
+#pragma unroll
+                    for (int s = NSF - 2; s >= 0; s--) hs = fma(hs, 256.0, (double)accp[s][e]);
+                    const bool dn = (word[e] >> (16 * (st - 1) + lo)) & 1u;
+                    h[e] += dn ? -hs : hs;
+                }
+            }
+            if (st < NST) {
+#pragma unroll
+                for (int s = 0; s < NSF; s++) accp[s] = acc[s];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const double hr = dpp_sum16(h[e]);
+            const long long r = rt * 16 + 4 * g + e;
+            if (lo == 0 && r < R) wrow[r] = hr * vsc * es[e];
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_q8_prep_vg, dim3(1), dim3(64 * sh.ncg64 * (sh.nst + 1)), 0, st, a, sh.ncg64, sh.nst, (uint4 *)a.vq);
+    const size_t lds = sh.qg_bytes - 16;
+    const dim3 grid(sh.fwd_blocks), blk(256);
+#define NQ_FWDG(NST, NCG) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG>), grid, blk, lds, st, a)
+    switch (sh.nst * 8 + sh.ncg64) {
+    case 9: NQ_FWDG(1, 1); break;
+    case 10: NQ_FWDG(1, 2); break;
+    case 11: NQ_FWDG(1, 3); break;
+    case 12: NQ_FWDG(1, 4); break;
+    case 17: NQ_FWDG(2, 1); break;
+    case 18: NQ_FWDG(2, 2); break;
+    case 19: NQ_FWDG(2, 3); break;
+    default: NQ_FWDG(2, 4); break;
+    }
+#undef NQ_FWDG
+    return hipGetLastError();
+}

@@ -197,6 +197,35 @@ def test_q8_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
     assert sym < 1e-11, sym
 
 
+@pytest.mark.parametrize("N,M,K,C,m", [(8, 16, 1, 37, 5), (17, 34, 3, 29, 7), (32, 64, 2, 96, 9), (24, 24, 4, 50, 4)])
+def test_q8_forward_passes_agree(nqes, monkeypatch, N, M, K, C, m):
+    """For N <= 32 the int8 forward pass runs as one GEMM over the columns (v quantised
+    with one scale for all of it, the bias as digit slots of an extra spin tile); the
+    tiled form (per-column scales, NQES_Q8_FWD=tiles) computes the same products of the
+    same factor digits.  Both quantise v to 2^-47 of a scale, so S*v agrees to ~1e-13
+    relative (bound 1e-11); v with blocks of very different size (1e-3) included."""
+    print(f"case N={N} M={M} K={K} C={C} m={m}", flush=True)
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=11 + N, scale=0.1, complex_=False)
+    P = K * (N + M + N * M)
+    rng = np.random.default_rng(13)
+    v1 = rng.normal(size=P) + 0j
+    v2 = v1.copy()
+    v2[: P // 3] *= 1e-3
+    out = {}
+    monkeypatch.delenv("NQES_SV", raising=False)
+    for mode in ("gemm", "tiles"):
+        if mode == "tiles":
+            monkeypatch.setenv("NQES_Q8_FWD", "tiles")
+        else:
+            monkeypatch.delenv("NQES_Q8_FWD", raising=False)
+        with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+            s.sample(W, 78, C, m, 3, 0.5)
+            out[mode] = [s.apply_S(v) for v in (v1, v2)]
+    for a, b in zip(out["gemm"], out["tiles"]):
+        assert rel_err(a, b) < 1e-11, rel_err(a, b)
+
+
 def test_q8_graph_replays_equal_eager(nqes, monkeypatch):
     """The int8 S*v runs as a captured HIP graph (one per sample set): every replay,
     with eager work and synchronisations between replays and a second sample set
