@@ -69,11 +69,13 @@ __device__ inline double xor_sum16(double x)
     return x;
 }
 
-// Pack: U (FP32, [R][UC] row-major) -> Uq tiles and escale[r] = 1 / f_r.  One
-// wavefront per 16-row tile, rows [R, Rp) written as zeros.
+// Pack: U (FP32, [R][UC] row-major) -> Uq tiles (and, when Uf is given, the same digits
+// row-major for the GEMM forward pass) and escale[r] = 1 / f_r.  One wavefront per
+// 16-row tile, rows [R, Rp) written as zeros.
 __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, long long ntile, int UC, uint32_t *Uq,
                                                  double *escale, uint4 *Uf)
 {
+    __shared__ __align__(16) uint8_t tile_t[4][QU * 256];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
     const int NCT = UC / 16;
@@ -113,38 +115,21 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
             uint32_t *o = Uq + ((rt * NCT + ct) * 64 + lane) * QU;
 #pragma unroll
             for (int t = 0; t < QU; t++) o[t] = pl[t];
-        }
-        if (Uf) {
-            // row-major copy for the GEMM forward pass: chunk (ct, lo) of plane t holds the
-            // 16 digits of row 16 rt + lo at columns 16 ct .. 16 ct + 15; this lane writes
-            // the chunks ct = g, g + 4, ...  Its row's scale comes from the lane holding it.
-            double scr = 0.0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const double x = __shfl(sc[j], 16 * (lo >> 2) + lo);
-                if ((lo & 3) == j) scr = x;
-            }
-            const long long r = rt * 16 + lo;
-            for (int ct = g; ct < NCT; ct += 4) {
-                uint32_t pl[QU][4] = {};
-#pragma unroll
-                for (int e4 = 0; e4 < 4; e4++) {
-                    const float4 u4 = r < R ? *(const float4 *)(U + r * UC + 16 * ct + 4 * e4) : float4{0.f, 0.f, 0.f, 0.f};
-                    const float uu[4] = {u4.x, u4.y, u4.z, u4.w};
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        long long x = (long long)rint((double)uu[e] * scr);
-#pragma unroll
-                        for (int t = 0; t < QU; t++) {
-                            const long long d = (t < QU - 1) ? bal8l(x) : x;
-                            pl[t][e4] |= ((uint32_t)d & 255u) << (8 * e);
-                            x = (x - d) >> 8;
-                        }
-                    }
-                }
+            if (Uf) {
+                // row-major copy for the GEMM forward pass, transposed through this
+                // wavefront's 768 bytes of LDS: chunk 16 ct + row of plane t = the 16
+                // digits of row 16 rt + row at columns 16 ct .. 16 ct + 15
+                uint8_t *tl = tile_t[wv];
 #pragma unroll
                 for (int t = 0; t < QU; t++)
-                    Uf[(rt * QU + t) * NCT * 16 + 16 * ct + lo] = uint4{pl[t][0], pl[t][1], pl[t][2], pl[t][3]};
+#pragma unroll
+                    for (int j = 0; j < 4; j++) tl[t * 256 + (4 * g + j) * 16 + lo] = (uint8_t)(pl[t] >> (8 * j));
+                __builtin_amdgcn_wave_barrier();
+                if (lane < 16 * QU) {
+                    const int t = lane >> 4, row = lane & 15;
+                    Uf[(rt * QU + t) * NCT * 16 + 16 * ct + row] = *(const uint4 *)(tl + t * 256 + row * 16);
+                }
+                __builtin_amdgcn_wave_barrier();
             }
         }
     }
@@ -608,7 +593,7 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R)
     sh.uf_bytes = sh.gemm ? (size_t)(n64 * 4) * QU * UC * 16 : 0;
     if (sh.gemm && sh.qg_bytes > sh.vq_bytes) sh.vq_bytes = sh.qg_bytes;
     if (sh.gemm) {
-        sh.fwd_blocks = 512;  // 2 wavefronts per SIMD (accumulators and the next tile's digits)
+        sh.fwd_blocks = sh.nst * sh.ncg64 > 6 ? 512 : 768;  // 2 or 3 wavefronts per SIMD, as k_q8_fwdg's bounds
         e = getenv("NQES_Q8_FWD_BLOCKS");
         if (e && *e && atoi(e) > 0) sh.fwd_blocks = atoi(e);
     }

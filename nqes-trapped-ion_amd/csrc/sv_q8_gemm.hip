@@ -8,6 +8,7 @@
 // rbm.cc:661-684, as in sv_q8.hip.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <type_traits>
 
 #include "kernels.hpp"
@@ -109,13 +110,13 @@ __global__ __launch_bounds__(1024) void k_q8_prep_vg(SvfArgs a, int ncg, int nst
     if (threadIdx.x == 0) *(double *)(qg + (size_t)ncg * nslot * 64) = mx > 0.0 ? ldexp(1.0, gex - VBITS) : 0.0;
 }
 
-// GEMM forward pass.  Wavefronts loop over 16-row tiles; the digits of the next tile
-// load while this one is multiplied (NCG (NST QV + 1) QU MFMAs into 8 int32 accumulators
-// per spin tile, digit weights 256^(t + u), and 3 for the bias).  Per row: each lane
+// GEMM forward pass.  Wavefronts loop over 16-row tiles (NCG (NST QV + 1) QU MFMAs
+// into 8 int32 accumulators per spin tile, digit weights 256^(t + u), and 3 for the
+// bias).  Per row: each lane
 // sums its spin (FP64 Horner over the digit weights, sign of its spin) and bias digit,
 // then 16 lanes by DPP.  Output wrow[r].
 template <int NST, int NCG>
-__global__ __launch_bounds__(256, 2) void k_q8_fwdg(SvfArgs a)
+__global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs a)
 {
     constexpr int NSF = QU + QV - 1;
     constexpr int NSLOT = NST * QV + 1;
@@ -139,42 +140,27 @@ __global__ __launch_bounds__(256, 2) void k_q8_fwdg(SvfArgs a)
     const long long step = (long long)gridDim.x * (blockDim.x >> 6);
     // Loads are unconditional (no branch between a load and its use, so the waits count
     // only older loads): lanes past the last column read chunk 16 NCT - 1, whose products
-    // meet zero v digits; rows past R read row R - 1 (their sums are not written); past
-    // the last tile, the last tile again.
+    // meet zero v digits; rows past R read row R - 1 (their sums are not written).  Three
+    // wavefronts per SIMD (two for NST NCG = 8, whose 168 registers would spill) cover
+    // each other's load latency (measured 5 % faster than two with the next tile's
+    // digits loaded into registers ahead).
     const int nq16 = 16 * NCT;
-    uint4 an[NCG][QU];  // the next tile's digits, spin words and row scales
-    uint32_t wn[4];
-    double en[4];
-    auto load = [&](long long rt) {
-#pragma unroll
-        for (int cg = 0; cg < NCG; cg++) {
-            const int q = 64 * cg + lane < nq16 ? 64 * cg + lane : nq16 - 1;
-#pragma unroll
-            for (int t = 0; t < QU; t++) an[cg][t] = Uf[(rt * QU + t) * nq16 + q];
-        }
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const long long r = rt * 16 + 4 * g + e, rc = r < R ? r : R - 1;
-            wn[e] = a.spins[rc * NW];
-            en[e] = a.escale[rc];
-        }
-    };
-    long long rt = (long long)blockIdx.x * (blockDim.x >> 6) + wv;
-    if (rt < ntile) load(rt);
-    for (; rt < ntile; rt += step) {
+    for (long long rt = (long long)blockIdx.x * (blockDim.x >> 6) + wv; rt < ntile; rt += step) {
         uint4 av[NCG][QU];
         uint32_t word[4];
         double es[4];
 #pragma unroll
-        for (int cg = 0; cg < NCG; cg++)
+        for (int cg = 0; cg < NCG; cg++) {
+            const int q = 64 * cg + lane < nq16 ? 64 * cg + lane : nq16 - 1;
 #pragma unroll
-            for (int t = 0; t < QU; t++) av[cg][t] = an[cg][t];
+            for (int t = 0; t < QU; t++) av[cg][t] = Uf[(rt * QU + t) * nq16 + q];
+        }
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            word[e] = wn[e];
-            es[e] = en[e];
+            const long long r = rt * 16 + 4 * g + e, rc = r < R ? r : R - 1;
+            word[e] = a.spins[rc * NW];
+            es[e] = a.escale[rc];
         }
-        load(rt + step < ntile ? rt + step : rt);
         // an opaque lane offset keeps the v operand reads inside the loop (hoisted, they
         // would hold NCG NSLOT 4 registers)
         int ql = lane;
