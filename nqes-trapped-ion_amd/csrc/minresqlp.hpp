@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "cplx64.hpp"
@@ -85,16 +86,30 @@ inline double vdot_re(const std::vector<cd> &x, const std::vector<cd> &y)
     return s;
 }
 
-// Op: callable (const std::vector<cd>& v, std::vector<cd>& out) -> bool (false aborts)
-template <typename Op>
-Result solve(Op &&A, const std::vector<cd> &b, std::vector<cd> &x, double rtol, uint64_t maxit, bool &aborted)
+// The operator as two calls: launch(v) starts A v (v stays untouched until the matching
+// finish), finish(out) writes it (false aborts).  The solver launches A v_{k+1} as soon
+// as v_{k+1} = r3 / betan and the stopping decision are known, and updates w, x while it
+// runs: every element and every reduction is computed as in the one-call order, so the
+// results are the same bit for bit.
+template <typename AOp>
+Result solve_async(AOp &A, const std::vector<cd> &b, std::vector<cd> &x, double rtol, uint64_t maxit, bool &aborted)
 {
     using namespace nq64;
     const double maxxnorm = 1e7, Acondlim = 1e15, TranCond = 1e7;
     const size_t n = b.size();
-    std::vector<cd> r1(b), r2(b), r3(b), v(n), w(n, zero()), wl(n, zero()), wl2(n, zero()), xl2(n, zero());
+    // r1, r2, r3 and w, wl, wl2 rotate through three buffers each (the reference copies
+    // r1 = r2, r2 = r3, wl2 = wl, wl = w); v and the next v alternate between two
+    std::vector<cd> rb[3] = {b, b, b}, wb[3] = {std::vector<cd>(n, zero()), std::vector<cd>(n, zero()),
+                                                std::vector<cd>(n, zero())};
+    std::vector<cd> vb[2] = {std::vector<cd>(n), std::vector<cd>(n)}, xl2(n, zero());
+    int ir1 = 0, ir2 = 1, ir3 = 2;
+    cd *r1 = rb[ir1].data(), *r2 = rb[ir2].data(), *r3 = rb[ir3].data();
+    cd *w = wb[0].data(), *wl = wb[1].data(), *wl2 = wb[2].data();
+    int vcur = 0;
     aborted = false;
-    double beta1 = vnorm(r2);
+    double beta1 = 0.0;
+    for (size_t i = 0; i < n; i++) beta1 += b[i].re * b[i].re + b[i].im * b[i].im;
+    beta1 = std::sqrt(beta1);
 
     const int flag0 = InternalError;
     int flag = flag0;
@@ -109,27 +124,51 @@ Result solve(Op &&A, const std::vector<cd> &b, std::vector<cd> &x, double rtol, 
     x.assign(n, zero());
     if (beta1 == 0.0) flag = Zero;
 
+    bool launched = true;
+    if (flag == flag0 && iters < maxit) {  // v_1 = r3 / beta1
+        cd *v = vb[vcur].data();
+        for (size_t i = 0; i < n; i++) v[i] = divr(r3[i], betan);
+        launched = A.launch(vb[vcur]);
+    }
     while (flag == flag0 && iters < maxit) {
         iters++;
         double betal = beta;
         beta = betan;
-        for (size_t i = 0; i < n; i++) v[i] = divr(r3[i], beta);
-        if (!A(v, r3)) {
+        cd *v = vb[vcur].data();
+        if (!launched || !A.finish(rb[ir3])) {
             aborted = true;
             break;
         }
+        // r3 -= (beta / betal) r1 with alfa = r3^H v, then r3 -= (alfa / beta) r2 with
+        // betan = |r3| (each reduction in index order, as a separate pass would)
+        double alfa = 0.0;
         if (iters > 1) {
-            double f = beta / betal;
-            for (size_t i = 0; i < n; i++) r3[i] = sub(r3[i], scale(f, r1[i]));
+            const double f = beta / betal;
+            for (size_t i = 0; i < n; i++) {
+                r3[i] = sub(r3[i], scale(f, r1[i]));
+                alfa += r3[i].re * v[i].re + r3[i].im * v[i].im;
+            }
+        } else {
+            for (size_t i = 0; i < n; i++) alfa += r3[i].re * v[i].re + r3[i].im * v[i].im;
         }
-        double alfa = vdot_re(r3, v);
         {
-            double f = alfa / beta;
-            for (size_t i = 0; i < n; i++) r3[i] = sub(r3[i], scale(f, r2[i]));
+            const double f = alfa / beta;
+            double s2 = 0.0;
+            for (size_t i = 0; i < n; i++) {
+                r3[i] = sub(r3[i], scale(f, r2[i]));
+                s2 += r3[i].re * r3[i].re + r3[i].im * r3[i].im;
+            }
+            betan = std::sqrt(s2);
         }
-        r1 = r2;
-        r2 = r3;
-        betan = vnorm(r3);
+        {
+            const int t = ir1;
+            ir1 = ir2;
+            ir2 = ir3;
+            ir3 = t;
+            r1 = rb[ir1].data();
+            r2 = rb[ir2].data();
+            r3 = rb[ir3].data();
+        }
         if (iters == 1 && betan == 0.0) {
             if (alfa == 0.0) {
                 flag = Zero;
@@ -192,54 +231,22 @@ Result solve(Op &&A, const std::vector<cd> &b, std::vector<cd> &x, double rtol, 
         xl2norm = std::sqrt(xl2norm * xl2norm + ul2 * ul2);
         xnorm = std::sqrt(xl2norm * xl2norm + ul * ul + u * u);
 
-        if (Acond < TranCond && flag != flag0 && QLPiter == 0) {
-            wl2 = wl;
-            wl = w;
-            for (size_t i = 0; i < n; i++)
-                w[i] = divr(sub(sub(v[i], scale(epln, wl2[i])), scale(dlta_QLP, wl[i])), gama_tmp);
-            if (xnorm < maxxnorm) {
-                for (size_t i = 0; i < n; i++) x[i] = add(x[i], scale(tau, w[i]));
-            } else {
-                flag = NormLimitExceeded;
-            }
+        // which update of w and x this iteration makes, and the scalars it reads (the
+        // tail below moves some of them on)
+        const bool minres_step = Acond < TranCond && flag != flag0 && QLPiter == 0;
+        bool x_step = false, qlp_first = false;
+        if (minres_step) {
+            x_step = xnorm < maxxnorm;
+            if (!x_step) flag = NormLimitExceeded;
         } else {
             QLPiter++;
-            if (QLPiter == 1 && iters > 1) {
-                if (iters > 3)
-                    for (size_t i = 0; i < n; i++)
-                        wl2[i] = add(add(scale(gamal3, wl2[i]), scale(veplnl2, wl[i])), scale(etal, w[i]));
-                if (iters > 2)
-                    for (size_t i = 0; i < n; i++) wl[i] = add(scale(gamal_QLP, wl[i]), scale(vepln_QLP, w[i]));
-                for (size_t i = 0; i < n; i++) w[i] = scale(gama_QLP, w[i]);
-                for (size_t i = 0; i < n; i++) xl2[i] = sub(sub(x[i], scale(ul_QLP, wl[i])), scale(u_QLP, w[i]));
-            }
-            if (iters == 1) {
-                wl2 = wl;
-                for (size_t i = 0; i < n; i++) wl[i] = scale(sr1, v[i]);
-                for (size_t i = 0; i < n; i++) w[i] = scale(cr1, neg(v[i]));
-            } else if (iters == 2) {
-                wl2 = wl;
-                for (size_t i = 0; i < n; i++) {
-                    cd wi = w[i], vi = v[i];
-                    wl[i] = add(scale(cr1, wi), scale(sr1, vi));
-                    w[i] = sub(scale(sr1, wi), scale(cr1, vi));
-                }
-            } else {
-                for (size_t i = 0; i < n; i++) {
-                    cd o_wl = wl[i], o_w = w[i], vi = v[i];
-                    cd nw = sub(scale(sr2, o_wl), scale(cr2, vi));
-                    cd nwl2 = add(scale(cr2, o_wl), scale(sr2, vi));
-                    cd nv = add(scale(cr1, o_w), scale(sr1, nw));
-                    cd nw2 = sub(scale(sr1, o_w), scale(cr1, nw));
-                    wl2[i] = nwl2;
-                    v[i] = nv;
-                    w[i] = nw2;
-                    wl[i] = nv;
-                }
-            }
-            for (size_t i = 0; i < n; i++) xl2[i] = add(xl2[i], scale(ul2, wl2[i]));
-            for (size_t i = 0; i < n; i++) x[i] = add(add(xl2[i], scale(ul, wl[i])), scale(u, w[i]));
+            qlp_first = QLPiter == 1 && iters > 1;
         }
+        const uint64_t it = iters;
+        const double s_epln = epln, s_dlta = dlta_QLP, s_gama = gama_tmp, s_tau = tau;
+        const double s_gamal3 = gamal3, s_veplnl2 = veplnl2, s_etal = etal, s_gamalQ = gamal_QLP,
+                     s_veplnQ = vepln_QLP, s_gamaQ = gama_QLP, s_ulQ = ul_QLP, s_uQ = u_QLP;
+        const double s_cr1 = cr1, s_sr1 = sr1, s_cr2 = cr2, s_sr2 = sr2, s_ul2 = ul2, s_ul = ul, s_u = u;
 
         double gamal_tmp = gamal;
         sym_ortho(gamal, eplnn, cr2, sr2, gamal);
@@ -285,12 +292,90 @@ Result solve(Op &&A, const std::vector<cd> &b, std::vector<cd> &x, double rtol, 
             rnorm = rnorml;
             relres = relresl;
         }
+
+        // the next operator call starts here: v_{k+1} = r3 / betan (r3 is now r2)
+        if (flag == flag0 && iters < maxit) {
+            cd *vn = vb[1 - vcur].data();
+            for (size_t i = 0; i < n; i++) vn[i] = divr(r2[i], betan);
+            launched = A.launch(vb[1 - vcur]);
+        }
+
+        if (minres_step) {
+            cd *t = wl2;
+            wl2 = wl;
+            wl = w;
+            w = t;
+            for (size_t i = 0; i < n; i++) w[i] = divr(sub(sub(v[i], scale(s_epln, wl2[i])), scale(s_dlta, wl[i])), s_gama);
+            if (x_step)
+                for (size_t i = 0; i < n; i++) x[i] = add(x[i], scale(s_tau, w[i]));
+        } else {
+            if (qlp_first) {
+                if (it > 3)
+                    for (size_t i = 0; i < n; i++)
+                        wl2[i] = add(add(scale(s_gamal3, wl2[i]), scale(s_veplnl2, wl[i])), scale(s_etal, w[i]));
+                if (it > 2)
+                    for (size_t i = 0; i < n; i++) wl[i] = add(scale(s_gamalQ, wl[i]), scale(s_veplnQ, w[i]));
+                for (size_t i = 0; i < n; i++) w[i] = scale(s_gamaQ, w[i]);
+                for (size_t i = 0; i < n; i++) xl2[i] = sub(sub(x[i], scale(s_ulQ, wl[i])), scale(s_uQ, w[i]));
+            }
+            if (it == 1) {
+                cd *t = wl2;
+                wl2 = wl;
+                wl = t;
+                for (size_t i = 0; i < n; i++) wl[i] = scale(s_sr1, v[i]);
+                for (size_t i = 0; i < n; i++) w[i] = scale(s_cr1, neg(v[i]));
+            } else if (it == 2) {
+                cd *t = wl2;
+                wl2 = wl;
+                wl = t;
+                for (size_t i = 0; i < n; i++) {
+                    cd wi = w[i], vi = v[i];
+                    wl[i] = add(scale(s_cr1, wi), scale(s_sr1, vi));
+                    w[i] = sub(scale(s_sr1, wi), scale(s_cr1, vi));
+                }
+            } else {
+                for (size_t i = 0; i < n; i++) {
+                    cd o_wl = wl[i], o_w = w[i], vi = v[i];
+                    cd nw = sub(scale(s_sr2, o_wl), scale(s_cr2, vi));
+                    cd nwl2 = add(scale(s_cr2, o_wl), scale(s_sr2, vi));
+                    cd nv = add(scale(s_cr1, o_w), scale(s_sr1, nw));
+                    cd nw2 = sub(scale(s_sr1, o_w), scale(s_cr1, nw));
+                    wl2[i] = nwl2;
+                    w[i] = nw2;
+                    wl[i] = nv;
+                }
+            }
+            for (size_t i = 0; i < n; i++) xl2[i] = add(xl2[i], scale(s_ul2, wl2[i]));
+            for (size_t i = 0; i < n; i++) x[i] = add(add(xl2[i], scale(s_ul, wl[i])), scale(s_u, w[i]));
+        }
+        vcur = 1 - vcur;
     }
     Result res;
     res.flag = flag;
     res.iters = iters;
     res.qlp_iters = QLPiter;
     return res;
+}
+
+// Op: callable (const std::vector<cd>& v, std::vector<cd>& out) -> bool (false aborts),
+// run as launch (remember v) and finish (call it)
+template <typename Op>
+struct SyncOp {
+    Op &A;
+    const std::vector<cd> *v = nullptr;
+    bool launch(const std::vector<cd> &vv)
+    {
+        v = &vv;
+        return true;
+    }
+    bool finish(std::vector<cd> &out) { return A(*v, out); }
+};
+
+template <typename Op>
+Result solve(Op &&A, const std::vector<cd> &b, std::vector<cd> &x, double rtol, uint64_t maxit, bool &aborted)
+{
+    SyncOp<typename std::remove_reference<Op>::type> op{A};
+    return solve_async(op, b, x, rtol, maxit, aborted);
 }
 
 } // namespace nqmq

@@ -818,26 +818,42 @@ static int build_sv_graph(nqes_ctx *c)
     return NQES_OK;
 }
 
+// the int8 S*v graph in two halves: launch (v and o^T v to pinned memory, graph on the
+// stream) and finish (wait, raw result to svraw); the host may work in between
+static int sv_graph_launch(nqes_ctx *c, const double *v, double ov)
+{
+    const long long P = c->P;
+    if (!c->svx) {
+        int st = build_sv_graph(c);
+        if (st != NQES_OK) return st;
+    }
+    memcpy(c->hv_pin, v, 8 * P);
+    c->hv_pin[P] = ov;
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
+    HIPCHK(c, hipGraphLaunch(c->svx, c->stream), "S*v graph launch");
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
+    return NQES_OK;
+}
+
+static int sv_graph_finish(nqes_ctx *c)
+{
+    HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
+    memcpy(c->svraw.data(), c->hr_pin, 8 * (c->P + 1));
+    c->sv_ms += event_ms(c->ev[2], c->ev[3]);
+    c->sv_calls++;
+    return NQES_OK;
+}
+
+static bool sv_graph_on(const nqes_ctx *c) { return c->q8 && env_int("NQES_GRAPH", 1) != 0; }
+
 // device part of S*v on the FP32 O of the mixed path: raw[p] = sum_s conj(O_sp) w_s,
 // raw[P] = sum_s w_s, w_s = O_s^T v - o^T v.  v given as D = double (real) or cd.
 static int sv_mixed_device(nqes_ctx *c, const void *v, const void *ov, size_t dsize)
 {
     const long long P = c->P;
-    if (c->q8 && env_int("NQES_GRAPH", 1) != 0) {
-        if (!c->svx) {
-            int st = build_sv_graph(c);
-            if (st != NQES_OK) return st;
-        }
-        memcpy(c->hv_pin, v, 8 * P);
-        memcpy(c->hv_pin + P, ov, 8);
-        HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
-        HIPCHK(c, hipGraphLaunch(c->svx, c->stream), "S*v graph launch");
-        HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
-        HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
-        memcpy(c->svraw.data(), c->hr_pin, 8 * (P + 1));
-        c->sv_ms += event_ms(c->ev[2], c->ev[3]);
-        c->sv_calls++;
-        return NQES_OK;
+    if (sv_graph_on(c)) {
+        int st = sv_graph_launch(c, (const double *)v, *(const double *)ov);
+        return st != NQES_OK ? st : sv_graph_finish(c);
     }
     HIPCHK(c, hipMemcpyAsync(c->dv.p, v, dsize * P, hipMemcpyHostToDevice, c->stream), "copy v");
     HIPCHK(c, hipMemcpyAsync(c->dov.p, ov, dsize, hipMemcpyHostToDevice, c->stream), "copy ov");
@@ -861,6 +877,38 @@ static int sv_mixed_device(nqes_ctx *c, const void *v, const void *ov, size_t ds
     c->sv_ms += event_ms(c->ev[2], c->ev[3]);
     c->sv_calls++;
     return NQES_OK;
+}
+
+// real O and o (mixed path, real weights): part 0 / 1 of v (real / imaginary) into hv,
+// returns o^T hv
+static double mixed_real_prepare(nqes_ctx *c, const cd *v, int part)
+{
+    const long long P = c->P;
+    c->hv.resize(P + 1);
+    double ovp = 0.0;
+    for (long long q = 0; q < P; q++) {
+        c->hv[q] = part ? v[q].im : v[q].re;
+        ovp += c->o[q].re * c->hv[q];
+    }
+    return ovp;
+}
+
+// (1/S) [raw - o sum_s w_s] + skk .* hv into part 0 (imaginary part cleared) or part 1 of out
+static void mixed_real_result(nqes_ctx *c, cd *out, int part)
+{
+    const long long P = c->P;
+    const double Sd = (double)c->S;
+    const double *raw = reinterpret_cast<const double *>(c->svraw.data());
+    const double wsum = raw[P];
+    for (long long q = 0; q < P; q++) {
+        double r = (raw[q] - c->o[q].re * wsum) / Sd + c->skk[q].re * c->hv[q];
+        if (part == 0) {
+            out[q].re = r;
+            out[q].im = 0.0;
+        } else {
+            out[q].im = r;
+        }
+    }
 }
 
 // S v = (1/S) sum_s (O_s^T v) conj(O_s) + skk .* v - (o^T v) conj(o)   (rbm.cc:661-684)
@@ -911,32 +959,52 @@ static int apply_S(nqes_ctx *c, const cd *v, cd *out)
         return NQES_OK;
     }
     // real O and o: apply to the real and imaginary parts of v separately
-    c->hv.resize(P + 1);
     bool has_im = false;
     for (long long q = 0; q < P; q++)
         if (v[q].im != 0.0) has_im = true;
     for (int part = 0; part < (has_im ? 2 : 1); part++) {
-        double ovp = 0.0;
-        for (long long q = 0; q < P; q++) {
-            c->hv[q] = part ? v[q].im : v[q].re;
-            ovp += c->o[q].re * c->hv[q];
-        }
+        double ovp = mixed_real_prepare(c, v, part);
         int st = sv_mixed_device(c, c->hv.data(), &ovp, 8);
         if (st != NQES_OK) return st;
-        const double *raw = reinterpret_cast<const double *>(c->svraw.data());
-        const double wsum = raw[P];
-        for (long long q = 0; q < P; q++) {
-            double r = (raw[q] - c->o[q].re * wsum) / Sd + c->skk[q].re * c->hv[q];
-            if (part == 0) {
-                out[q].re = r;
-                out[q].im = 0.0;
-            } else {
-                out[q].im = r;
-            }
-        }
+        mixed_real_result(c, out, part);
     }
     return NQES_OK;
 }
+
+// S*v as launch and finish for the solver: the int8 graph path with a real v (the
+// real-weight solve) returns from the launch while the device works; every other form
+// runs whole in the finish.  Results equal apply_S's bit for bit.
+struct SvOperator {
+    nqes_ctx *c;
+    int status = NQES_OK;
+    bool split = false;
+    const std::vector<cd> *v = nullptr;
+    bool launch(const std::vector<cd> &vv)
+    {
+        v = &vv;
+        split = false;
+        if (c->dense || c->prec != NQES_PREC_MIXED || c->cplx || !sv_graph_on(c)) return true;
+        const long long P = c->P;
+        for (long long q = 0; q < P; q++)
+            if (vv[q].im != 0.0) return true;
+        const double ovp = mixed_real_prepare(c, vv.data(), 0);
+        status = sv_graph_launch(c, c->hv.data(), ovp);
+        split = status == NQES_OK;
+        return split;
+    }
+    bool finish(std::vector<cd> &out)
+    {
+        if (!split) {
+            status = apply_S(c, v->data(), out.data());
+            return status == NQES_OK;
+        }
+        split = false;
+        status = sv_graph_finish(c);
+        if (status != NQES_OK) return false;
+        mixed_real_result(c, out.data(), 0);
+        return true;
+    }
+};
 
 int nqes_apply_S(nqes_ctx *c, const double *v, double *out)
 {
@@ -949,15 +1017,10 @@ int nqes_apply_S(nqes_ctx *c, const double *v, double *out)
 
 static int solve_impl(nqes_ctx *c, double rtol, uint64_t maxit, std::vector<cd> &x, nqmq::Result &res)
 {
-    int status = NQES_OK;
     bool aborted = false;
-    auto op = [&](const std::vector<cd> &v, std::vector<cd> &out) -> bool {
-        status = apply_S(c, v.data(), out.data());
-        return status == NQES_OK;
-    };
-    res = nqmq::solve(op, c->f, x, rtol, maxit, aborted);
-    if (aborted) return status;
-    return NQES_OK;
+    SvOperator op{c};
+    res = nqmq::solve_async(op, c->f, x, rtol, maxit, aborted);
+    return aborted ? op.status : NQES_OK;
 }
 
 int nqes_solve(nqes_ctx *c, double rtol, uint64_t maxit, double *x, int *flag, uint64_t *iters)
