@@ -73,6 +73,7 @@ struct MixedSampleArgs {
     unsigned int *flags;       // bit 0: an angle exceeded the FP32 range
     void *U;                   // (S*K) x Ucols T: S*v factors (NULL: not stored)
     int Ucols;
+    float *Umax;               // S*K: max |U| of each factor row (real weights; NULL: not written)
 };
 
 // Factored S*v (sv_factored.hip): raw[p] = sum_s conj(O_sp) w_s, raw[P] = sum_s w_s,
@@ -95,6 +96,7 @@ struct SvfArgs {
     const void *Uq;            // int8 form: quantised factors (16 x 16 tiles of digit planes)
     const void *Uf;            // int8 form, GEMM forward pass: the same digits row-major per tile
     const double *escale;      // int8 form: per-row factor scale
+    const float *Umax;         // int8 form: max |U| per factor row (from the measurement kernel)
     void *vq;                  // int8 form: v operands, column scales and biases
     void *q8w;                 // int8 form: backward block scales and weight digits
     void *q8m;                 // int8 form: spin masks of the backward A operand

@@ -104,6 +104,7 @@ struct nqes_ctx {
     DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
     DevBuf dUq, descale, dvq, dq8w, dq8m;              // its int8 form (real weights)
     DevBuf dUf;                                        // row-major digits (GEMM forward pass)
+    DevBuf dUmax;                                      // max |U| per factor row
     hipGraph_t svg = nullptr;                          // int8 S*v sequence, captured once per
     hipGraphExec_t svx = nullptr;                      // sample set and replayed per MINRES step
     double *hv_pin = nullptr, *hr_pin = nullptr;       // pinned v | o^T v and result (P + 1 each)
@@ -246,7 +247,7 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
         b += (nchunk / 32 + 2) * (P + 1) * 16 + 3 * (P + 1) * 16;
         const Q8Shape q8 = nq_q8_shape(N, M, K, S * K);  // int8 form, used for real weights
         if (q8.ok)
-            b += q8.rpad * (long long)q8.ucols * 3 + q8.rpad * 8 +
+            b += q8.rpad * (long long)q8.ucols * 3 + q8.rpad * 12 +
                  (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes + q8.uf_bytes);
     } else {
         b += S * Pld * (long long)o_elem(precision, cplx);  // O
@@ -381,6 +382,7 @@ static SvfArgs svf_args(nqes_ctx *c, long long S)
     f.ncg = 1;
     f.Uq = c->dUq.p;
     f.Uf = c->dUf.p;
+    f.Umax = (const float *)c->dUmax.p;
     f.escale = static_cast<const double *>(c->descale.p);
     f.vq = c->dvq.p;
     f.q8w = c->dq8w.p;
@@ -443,6 +445,7 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.flags = static_cast<unsigned int *>(c->dflags.p);
     a.U = c->svf ? c->dU.p : nullptr;
     a.Ucols = c->svf_shape.ucols;
+    a.Umax = c->q8 ? (float *)c->dUmax.p : nullptr;
     HIPCHK(c, nq_launch_sweep_mx(cplx, c->K, a, sw, c->dtabp.p, c->stream), "mixed sample kernel");
     HIPCHK(c, nq_launch_measure_mx(cplx, c->K, a, ms, c->dtabp.p, c->stream), "mixed measure kernel");
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
@@ -515,6 +518,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
             const long long rp = c->q8_shape.rpad;
             HIPCHK(c, c->dUq.ensure((size_t)rp * c->q8_shape.ucols * 3), "alloc quantised factors");
             HIPCHK(c, c->descale.ensure((size_t)rp * 8), "alloc factor scales");
+            HIPCHK(c, c->dUmax.ensure((size_t)rp * 4), "alloc factor row maxima");
             HIPCHK(c, c->dvq.ensure(c->q8_shape.vq_bytes), "alloc S*v operands");
             HIPCHK(c, c->dq8w.ensure(c->q8_shape.w_bytes), "alloc S*v weight digits");
             HIPCHK(c, c->dq8m.ensure(c->q8_shape.m_bytes), "alloc spin masks");

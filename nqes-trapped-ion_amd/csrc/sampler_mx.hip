@@ -654,6 +654,9 @@ __device__ inline void measure_sample(const MixedSampleArgs &a, const ChSh<typen
             for (int kk = 0; kk < K; kk++) cT[k * K + kk] = toT(mulD(phi_inv[k * K + kk], phi[kk * K + k]));
         T *Orow = a.O ? (T *)a.O + s * a.Pld : nullptr;
         T *Ub = a.U ? (T *)a.U + s * K * (long long)a.Ucols : nullptr;
+        float umx[K];  // real weights: this lane's max |U| of each of the sample's K rows
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) umx[kk] = 0.f;
 #pragma unroll
         for (int k = 0; k < K; k++) {
             T ck[K];
@@ -682,7 +685,10 @@ __device__ inline void measure_sample(const MixedSampleArgs &a, const ChSh<typen
                 }
                 if (Ub)
 #pragma unroll
-                    for (int kk = 0; kk < K; kk++) Ub[(long long)kk * a.Ucols + k * M + j] = ct[kk];
+                    for (int kk = 0; kk < K; kk++) {
+                        Ub[(long long)kk * a.Ucols + k * M + j] = ct[kk];
+                        if constexpr (!CPLX) umx[kk] = fmaxf(umx[kk], fabsf(ct[kk]));
+                    }
                 if (!Orow) continue;
                 ob[N + j] = v;
                 for (int i = 0; i < N; i++) {
@@ -706,6 +712,22 @@ __device__ inline void measure_sample(const MixedSampleArgs &a, const ChSh<typen
                         if (col - K * M == k) val = cT[k * K + kk];
                     Ub[(long long)kk * a.Ucols + col] = val;
                 }
+        if constexpr (!CPLX) {
+            // max |U| of each of the sample's K rows (the int8 form's row scales): the
+            // lane's entries and the c columns, then the G lanes of the sample (exact: a
+            // maximum)
+            if (a.Umax && Ub) {
+#pragma unroll
+                for (int kk = 0; kk < K; kk++) {
+                    float mx = umx[kk];
+#pragma unroll
+                    for (int k = 0; k < K; k++) mx = fmaxf(mx, fabsf(cT[k * K + kk]));
+#pragma unroll
+                    for (int off = 1; off < G; off <<= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+                    if (g == 0) a.Umax[s * K + kk] = mx;
+                }
+            }
+        }
         if (Orow)
             for (long long q = a.P + g; q < a.Pld; q += G) Orow[q] = zeroT<T>();
     }

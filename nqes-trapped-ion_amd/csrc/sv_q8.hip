@@ -70,10 +70,11 @@ __device__ inline double xor_sum16(double x)
 }
 
 // Pack: U (FP32, [R][UC] row-major) -> Uq tiles (and, when Uf is given, the same digits
-// row-major for the GEMM forward pass) and escale[r] = 1 / f_r.  One wavefront per
-// 16-row tile, rows [R, Rp) written as zeros.
+// row-major for the GEMM forward pass) and escale[r] = 1 / f_r, the scale from the row
+// maxima Umax when given (else a first pass over U).  One wavefront per 16-row tile,
+// rows [R, Rp) written as zeros.
 __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, long long ntile, int UC, uint32_t *Uq,
-                                                 double *escale, uint4 *Uf)
+                                                 double *escale, uint4 *Uf, const float *Umax)
 {
     __shared__ __align__(16) uint8_t tile_t[4][QU * 256];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -81,12 +82,20 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
     const int NCT = UC / 16;
     for (long long rt = (long long)blockIdx.x * 4 + wv; rt < ntile; rt += (long long)gridDim.x * 4) {
         float mx[4] = {0.f, 0.f, 0.f, 0.f};
-        for (int ct = 0; ct < NCT; ct++)
+        if (Umax) {  // the row maxima the measurement kernel wrote with the rows
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const long long r = rt * 16 + 4 * g + j;
-                if (r < R) mx[j] = fmaxf(mx[j], fabsf(U[r * UC + 16 * ct + lo]));
+                if (r < R) mx[j] = Umax[r];
             }
+        } else {
+            for (int ct = 0; ct < NCT; ct++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const long long r = rt * 16 + 4 * g + j;
+                    if (r < R) mx[j] = fmaxf(mx[j], fabsf(U[r * UC + 16 * ct + lo]));
+                }
+        }
         double sc[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) {
@@ -611,7 +620,8 @@ hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
     long long blocks = (ntile + 3) / 4;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(k_q8_pack, dim3((unsigned)blocks), dim3(256), 0, st, (const float *)a.U, R, ntile, a.Ucols,
-                       (uint32_t *)a.Uq, (double *)a.escale, sh.gemm ? (uint4 *)a.Uf : (uint4 *)nullptr);
+                       (uint32_t *)a.Uq, (double *)a.escale, sh.gemm ? (uint4 *)a.Uf : (uint4 *)nullptr,
+                       a.Umax);
     const int NI = 16 * nit_of(a.N);
     const long long n64 = sh.rpad / 64, nthr = n64 * NI * 4;
     hipLaunchKernelGGL(k_q8_masks, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, a.spins, R, a.N, a.NW, NI,
