@@ -360,6 +360,23 @@ __global__ void k_sv_colacc_mx(const void *O_, const void *w_, long long S, long
     }
 }
 
+// acc + part[c0] + ... + part[c1 - 1] (rows of part, stride apart), added in order; the
+// loads go eight ahead of the adds (a dependent load per add would wait out its latency)
+template <typename D>
+__device__ inline D chunk_run(D acc, const D *__restrict__ part, long long stride, int c0, int c1)
+{
+    int ch = c0;
+    for (; ch + 8 <= c1; ch += 8) {
+        D x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = part[(long long)(ch + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = addD(acc, x[u]);
+    }
+    for (; ch < c1; ch++) acc = addD(acc, part[(long long)ch * stride]);
+    return acc;
+}
+
 // out[col] = sum over chunks of part[chunk * stride + col], col < ncol, chunks in order
 template <typename D>
 __global__ void k_chunksum_mx(const D *__restrict__ part, long long ncol, long long stride, int nchunk,
@@ -367,9 +384,7 @@ __global__ void k_chunksum_mx(const D *__restrict__ part, long long ncol, long l
 {
     long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncol) return;
-    D acc = part[col];
-    for (int ch = 1; ch < nchunk; ch++) acc = addD(acc, part[(long long)ch * stride + col]);
-    out[col] = acc;
+    out[col] = chunk_run(part[col], part + col, stride, 1, nchunk);
 }
 
 // partial sums over groups of CG consecutive chunks: grid (col tiles, groups)
@@ -381,9 +396,7 @@ __global__ void k_chunkgroup_mx(const D *__restrict__ part, long long ncol, long
     if (col >= ncol) return;
     int c0 = blockIdx.y * cg, c1 = c0 + cg;
     if (c1 > nchunk) c1 = nchunk;
-    D acc = zeroD<D>();
-    for (int ch = c0; ch < c1; ch++) acc = addD(acc, part[(long long)ch * stride + col]);
-    gout[(long long)blockIdx.y * ncol + col] = acc;
+    gout[(long long)blockIdx.y * ncol + col] = chunk_run(zeroD<D>(), part + col, stride, c0, c1);
 }
 
 // out[col] = sum_chunks part[...], deterministic, parallel over column tiles and
