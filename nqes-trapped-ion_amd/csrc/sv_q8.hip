@@ -333,7 +333,7 @@ __global__ void k_q8_masks(const uint32_t *spins, long long R, int N, int NW, in
 template <int WSEL>
 __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *pd)
 {
-    __shared__ uint32_t st[QW][4][4];
+    __shared__ uint32_t stg[4][QW][4][4];  // per wavefront: a chunk's digits in the A layout
     __shared__ double red[4], rsum[4];
     const int K = a.K;
     const long long R = a.S * K;
@@ -375,10 +375,13 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
         ((double *)a.part)[(long long)blockIdx.x * (a.P + 1) + a.P] = t;
     }
     const double wsc = ldexp(1.0, WBITS - G);
-    for (long long ch = c0; ch < c1; ch++) {
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            const int t = threadIdx.x;
+    // wavefront wv takes chunks c0 + wv, c0 + wv + 4, ...: lane t computes row t's
+    // digits into the wavefront's LDS slice, 24 lanes store them in the A layout (LDS
+    // operations of one wavefront run in order; no block barrier)
+    uint32_t(*st)[4][4] = stg[wv];
+    for (long long ch = c0 + wv; ch < c1; ch += 4) {
+        {
+            const int t = lane;
             const long long r = ch * 64 + t;
             long long q = r < R ? (long long)rint(wt(r) * es[r] * wsc) : 0;
             const int qq = t >> 4, gg = (t >> 2) & 3, j = t & 3;
@@ -389,11 +392,12 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
                 q = (q - d) >> 8;
             }
         }
-        __syncthreads();
-        if (threadIdx.x < QW * 4) {
-            const int u = threadIdx.x >> 2, g = threadIdx.x & 3;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < QW * 4) {
+            const int u = lane >> 2, g = lane & 3;
             pd[(ch * QW + u) * 4 + g] = uint4{st[u][g][0], st[u][g][1], st[u][g][2], st[u][g][3]};
         }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
