@@ -699,7 +699,16 @@ __global__ void k_colsum64(const cd *__restrict__ X, long long S, long long ncol
     if (col >= ncol) return;
     long long s0 = S * blockIdx.y / nchunk, s1 = S * (blockIdx.y + 1) / nchunk;
     cd acc = zero();
-    for (long long s = s0; s < s1; s++) acc = add(acc, X[s * ncol + col]);
+    // rows added in order; the loads run eight ahead of the adds
+    long long s = s0;
+    for (; s + 8 <= s1; s += 8) {
+        cd x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = X[(s + u) * ncol + col];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc = add(acc, x[u]);
+    }
+    for (; s < s1; s++) acc = add(acc, X[s * ncol + col]);
     part[(long long)blockIdx.y * ncol + col] = acc;
 }
 
