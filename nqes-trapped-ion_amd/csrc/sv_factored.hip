@@ -24,6 +24,7 @@
 //                      all its i tiles): G on MFMA, b-block column sums, sum_s w_s; one
 //                      partial raw vector (P + 1) per block, summed in block order after.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 #include <stdlib.h>
 
@@ -325,11 +326,12 @@ hipError_t launch_svf(const SvfArgs &a0, const SvfShape &sh, void *scratch, void
 // cross term of pair (kk, kk') is a GEMM [N x S] [S x Ucols] on FP64 MFMA (A operand tau,
 // B operand Re(U_kk conj U_kk')).  Pair 0 also adds the diagonal and b-block sums.
 // Block (b, pair) writes a full partial vector part[pair nblk + b][0..P).
-template <bool CPLX, int NITM>
+template <bool CPLX, int NITM, int K>
 __global__ __launch_bounds__(1024) void k_svf_sq(SvfArgs a)
 {
     using T = typename Ty<CPLX>::T;
-    const int N = a.N, M = a.M, K = a.K, NW = a.NW, UC = a.Ucols;
+    constexpr int NW = NITM / 2;  // spin words: 1 for up to 32 spins, 2 for up to 64
+    const int N = a.N, M = a.M, UC = a.Ucols;
     const int NCT = UC / 16, NIT = (N + 15) / 16;
     const int NWV = blockDim.x >> 6;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(1024) void k_svf_sq(SvfArgs a)
                     kb = y;
                 }
     }
-    const bool cross = K > 1;
+    constexpr bool cross = K > 1;
     const T *U = (const T *)a.U;
     double *part = (double *)a.part + ((long long)pair * gridDim.x + blockIdx.x) * (a.P + 1);
     for (int ct = wv; ct < NCT; ct += NWV) {
@@ -358,47 +360,91 @@ __global__ __launch_bounds__(1024) void k_svf_sq(SvfArgs a)
 #pragma unroll
         for (int it = 0; it < NITM; it++) acc[it] = v4d{0.0, 0.0, 0.0, 0.0};
         double dg = 0.0, bq = 0.0;
-        for (long long step = q0; step < q1; step++) {
-            const long long s = step * 4 + hi;
-            const bool ok = s < S;
-            double vab = 0.0;
-            uint32_t t0 = 0, t1 = 0;  // spin words of tau: bits of kk xor kk'
-            if (ok) {
-                const T *Us = U + s * K * UC + col;
-                if (cross) {
-                    const T ua = Us[ka * UC], ub = Us[kb * UC];
-                    if constexpr (CPLX) vab = (double)ua.re * ub.re + (double)ua.im * ub.im;
-                    else vab = (double)ua * ub;
-                    t0 = a.spins[(s * K + ka) * NW] ^ a.spins[(s * K + kb) * NW];
-                    if (NW > 1) t1 = a.spins[(s * K + ka) * NW + 1] ^ a.spins[(s * K + kb) * NW + 1];
+        // loads unconditional (sample index clamped; a step past S adds exact zeros), K,
+        // the spin words and the pair-0 terms fixed at compile time, MFMAs of spin tiles
+        // past N fed zeros: a step is straight-line code, and four steps issue their loads
+        // before their sums, which run in the original order
+        // a step: its loads (Step) and its sums and MFMAs (use)
+        struct Step {
+            bool ok;
+            T u[K];
+            uint32_t wa0, wb0, wa1, wb1;  // spin words of replicas kk, kk' (tau: their xor)
+        };
+        auto load = [&](long long step, Step &x) __attribute__((always_inline)) {
+            const long long s0 = step * 4 + hi;
+            x.ok = s0 < S;
+            const long long s = x.ok ? s0 : S - 1;
+            const T *Us = U + s * K * UC + col;
+#pragma unroll
+            for (int kk = 0; kk < K; kk++) x.u[kk] = Us[kk * UC];
+            x.wa0 = x.wb0 = x.wa1 = x.wb1 = 0;
+            if constexpr (cross) {
+                x.wa0 = a.spins[(s * K + ka) * NW];
+                x.wb0 = a.spins[(s * K + kb) * NW];
+                if constexpr (NW > 1) {
+                    x.wa1 = a.spins[(s * K + ka) * NW + 1];
+                    x.wb1 = a.spins[(s * K + kb) * NW + 1];
                 }
-                if (pair == 0) {
+            }
+        };
+        auto steps = [&](auto p0c) {
+            constexpr bool P0 = decltype(p0c)::value;
+            auto use = [&](const Step &x) __attribute__((always_inline)) {
+                const bool ok = x.ok;
+                if constexpr (P0) {
                     double br = 0.0, bi = 0.0;
+#pragma unroll
                     for (int kk = 0; kk < K; kk++) {
-                        const T u = Us[kk * UC];
+                        const T u = x.u[kk];
                         if constexpr (CPLX) {
-                            dg += (double)u.re * u.re + (double)u.im * u.im;
+                            dg += ok ? (double)u.re * u.re + (double)u.im * u.im : 0.0;
                             br += u.re;
                             bi += u.im;
                         } else {
-                            dg += (double)u * u;
+                            dg += ok ? (double)u * u : 0.0;
                             br += u;
                         }
                     }
-                    bq += br * br + bi * bi;
+                    bq += ok ? br * br + bi * bi : 0.0;
                 }
-            }
-            if (cross) {
+                if constexpr (cross) {
+                    // ka, kb are runtime: pick the pair's two factors
+                    T ua = x.u[0], ub = x.u[0];
 #pragma unroll
-                for (int it = 0; it < NITM; it++) {
-                    if (it < NIT) {
+                    for (int kk = 0; kk < K; kk++) {
+                        if (kk == ka) ua = x.u[kk];
+                        if (kk == kb) ub = x.u[kk];
+                    }
+                    double vab;
+                    if constexpr (CPLX) vab = (double)ua.re * ub.re + (double)ua.im * ub.im;
+                    else vab = (double)ua * ub;
+                    vab = ok ? vab : 0.0;
+#pragma unroll
+                    for (int it = 0; it < NITM; it++) {
                         const int i = 16 * it + lo;
-                        const double av = (ok && i < N) ? spin_pm(t0, t1, i) : 0.0;
+                        const double av = (ok && i < N) ? spin_pm(x.wa0 ^ x.wb0, x.wa1 ^ x.wb1, i) : 0.0;
                         acc[it] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, vab, acc[it], 0, 0, 0);
                     }
                 }
+            };
+            // unrolled by hand (the compiler does not unroll a loop of MFMAs with a
+            // runtime trip count): four steps' loads, then their sums in order
+            long long step = q0;
+            for (; step + 4 <= q1; step += 4) {
+                Step x[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) load(step + u, x[u]);
+#pragma unroll
+                for (int u = 0; u < 4; u++) use(x[u]);
             }
-        }
+            for (; step < q1; step++) {
+                Step x;
+                load(step, x);
+                use(x);
+            }
+        };
+        if (pair == 0) steps(std::true_type{});
+        else steps(std::false_type{});
         // column sums over the 4 sample lanes of each column
         dg += __shfl_xor(dg, 16);
         dg += __shfl_xor(dg, 32);
@@ -452,8 +498,21 @@ hipError_t launch_sq_pack(const SvfArgs &a0, const SvfShape &sh, void *part, voi
     const int npair = a0.K > 1 ? a0.K * (a0.K - 1) / 2 : 1;
     const int nct = a0.Ucols / 16;
     const int waves = nct < 16 ? nct : 16;
-    if (a0.N <= 32) hipLaunchKernelGGL((k_svf_sq<CPLX, 2>), dim3(sh.bwd_blocks, npair), dim3(64 * waves), 0, st, a);
-    else hipLaunchKernelGGL((k_svf_sq<CPLX, 4>), dim3(sh.bwd_blocks, npair), dim3(64 * waves), 0, st, a);
+    const dim3 grid(sh.bwd_blocks, npair), blk(64 * waves);
+#define NQ_SQ(NITM, KK) hipLaunchKernelGGL((k_svf_sq<CPLX, NITM, KK>), grid, blk, 0, st, a)
+    if (a0.NW != (a0.N <= 32 ? 1 : 2)) return hipErrorInvalidValue;
+    switch ((a0.N <= 32 ? 0 : 8) + a0.K) {
+    case 1: NQ_SQ(2, 1); break;
+    case 2: NQ_SQ(2, 2); break;
+    case 3: NQ_SQ(2, 3); break;
+    case 4: NQ_SQ(2, 4); break;
+    case 9: NQ_SQ(4, 1); break;
+    case 10: NQ_SQ(4, 2); break;
+    case 11: NQ_SQ(4, 3); break;
+    case 12: NQ_SQ(4, 4); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef NQ_SQ
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     nq_chunk_reduce(false, part, P, P + 1, sh.bwd_blocks * npair, scratch, rsq, st);
