@@ -44,9 +44,9 @@ __device__ inline double dpp_sum16(double x)
 // 64 cg + 16 (l/16) + e (the Uf chunk 64 cg + l); B lane l, byte e = digit of the same
 // column for spin 16 st + l%16 (slot st QV + u), or digit l%16 of bq (slot NST QV).
 // Qg: [cg][slot][lane] uint4, then the scale 2^(g - 46) as a double.
-__global__ __launch_bounds__(1024) void k_q8_prep_vg(SvfArgs a, int ncg, int nst, uint4 *qg)
+__global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst, uint4 *qg)
 {
-    __shared__ double red[16];
+    __shared__ double red[4];
     const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
     const long long KM = (long long)K * M;
     const double *v = (const double *)a.v;
@@ -56,58 +56,67 @@ __global__ __launch_bounds__(1024) void k_q8_prep_vg(SvfArgs a, int ncg, int nst
         return 0.0;
     };
     auto bias = [&](int col) -> double { return col < KM ? v[(long long)(col / M) * a.L + N + col % M] : 0.0; };
-    // maximum over v: the parameters of the K chains (L apart), N + M + N M each
+    // maximum over v (the parameters of the K chains, L apart, N + M + N M each); every
+    // block takes it, loads four ahead
     double mx = 0.0;
-    const long long NP = (long long)N + M + (long long)N * M;
-    for (long long q = threadIdx.x; q < K * NP; q += blockDim.x) mx = fmax(mx, fabs(v[(q / NP) * a.L + q % NP]));
+    {
+        const long long NP = (long long)N + M + (long long)N * M, nv = K * NP, step = blockDim.x;
+        for (long long q0 = threadIdx.x; q0 < nv; q0 += 4 * step) {
+            double x[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const long long q = q0 + u * step;
+                x[u] = q < nv ? v[(q / NP) * a.L + q % NP] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) mx = fmax(mx, fabs(x[u]));
+        }
+    }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) mx = fmax(mx, __shfl_xor(mx, off));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
     __syncthreads();
-    mx = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) mx = fmax(mx, red[w]);
+    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
     int gex = 0;
     if (mx > 0.0) (void)frexp(mx, &gex);
     const double sc = ldexp(1.0, VBITS - gex);
-    // thread (cg, st, lane); st = nst: the bias
-    const int lane = threadIdx.x & 63, lo = lane & 15;
-    const int cg = (threadIdx.x >> 6) / (nst + 1), st = (threadIdx.x >> 6) % (nst + 1);
+    // block (cg, st), st = nst: the bias; wavefront w forms dword w of every lane's uint4
+    // (its columns e = 4 w .. 4 w + 3 of the lane's 16)
+    const int lane = threadIdx.x & 63, lo = lane & 15, w = threadIdx.x >> 6;
+    const int cg = blockIdx.x / (nst + 1), st = blockIdx.x % (nst + 1);
     const int nslot = nst * QV + 1;
-    if (cg < ncg) {
-        uint32_t dg[QV][4] = {};
+    uint32_t dg[QV] = {};
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
-            const int col = 64 * cg + 16 * (lane >> 4) + e;
-            double val = 0.0;
-            if (col < UC) {
-                if (st < nst) val = 16 * st + lo < N ? Bval(col, 16 * st + lo) : 0.0;
-                else val = bias(col);
-            }
-            long long x = (long long)rint(val * sc);
-#pragma unroll
-            for (int u = 0; u < QV; u++) {
-                const long long d = (u < QV - 1) ? bal8l(x) : x;
-                dg[u][e >> 2] |= ((uint32_t)d & 255u) << (8 * (e & 3));
-                x = (x - d) >> 8;
-            }
+    for (int e4 = 0; e4 < 4; e4++) {
+        const int col = 64 * cg + 16 * (lane >> 4) + 4 * w + e4;
+        double val = 0.0;
+        if (col < UC) {
+            if (st < nst) val = 16 * st + lo < N ? Bval(col, 16 * st + lo) : 0.0;
+            else val = bias(col);
         }
-        uint4 *o = qg + (size_t)cg * nslot * 64;
-        if (st < nst) {
+        long long x = (long long)rint(val * sc);
 #pragma unroll
-            for (int u = 0; u < QV; u++) o[(st * QV + u) * 64 + lane] = uint4{dg[u][0], dg[u][1], dg[u][2], dg[u][3]};
-        } else {
-            // slot nst QV, lane (g, lo): digit lo of the bias columns of lane group g
-            uint4 b = {0u, 0u, 0u, 0u};
-#pragma unroll
-            for (int u = 0; u < QV; u++) {
-                const int src = 16 * (lane >> 4) + u;  // every lane of the group holds the bias digits
-                const uint4 x = {__shfl(dg[u][0], src), __shfl(dg[u][1], src), __shfl(dg[u][2], src), __shfl(dg[u][3], src)};
-                if (lo == u) b = x;
-            }
-            o[nst * QV * 64 + lane] = b;
+        for (int u = 0; u < QV; u++) {
+            const long long d = (u < QV - 1) ? bal8l(x) : x;
+            dg[u] |= ((uint32_t)d & 255u) << (8 * e4);
+            x = (x - d) >> 8;
         }
     }
-    if (threadIdx.x == 0) *(double *)(qg + (size_t)ncg * nslot * 64) = mx > 0.0 ? ldexp(1.0, gex - VBITS) : 0.0;
+    uint32_t *o = (uint32_t *)(qg + (size_t)cg * nslot * 64);
+    if (st < nst) {
+#pragma unroll
+        for (int u = 0; u < QV; u++) o[((st * QV + u) * 64 + lane) * 4 + w] = dg[u];
+    } else {
+        // slot nst QV, lane (g, lo): digit lo of the bias columns of lane group g (every
+        // lane of the group computed the same bias values)
+        uint32_t bw = 0u;
+#pragma unroll
+        for (int u = 0; u < QV; u++)
+            if (lo == u) bw = dg[u];
+        o[(nst * QV * 64 + lane) * 4 + w] = bw;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        *(double *)(qg + (size_t)ncg * nslot * 64) = mx > 0.0 ? ldexp(1.0, gex - VBITS) : 0.0;
 }
 
 // GEMM forward pass.  Wavefronts loop over 16-row tiles (NCG (NST QV + 1) QU MFMAs
@@ -242,7 +251,7 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
 
 hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_q8_prep_vg, dim3(1), dim3(64 * sh.ncg64 * (sh.nst + 1)), 0, st, a, sh.ncg64, sh.nst, (uint4 *)a.vq);
+    hipLaunchKernelGGL(k_q8_prep_vg, dim3(sh.ncg64 * (sh.nst + 1)), dim3(256), 0, st, a, sh.ncg64, sh.nst, (uint4 *)a.vq);
     const size_t lds = sh.qg_bytes - 16;
     const dim3 grid(sh.fwd_blocks), blk(256);
 #define NQ_FWDG(NST, NCG) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG>), grid, blk, lds, st, a)
