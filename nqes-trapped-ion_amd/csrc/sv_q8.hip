@@ -413,7 +413,7 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
 // (the current copies are only ever register copies of those loads).  Writes
 // part[b][0..P) for its columns: a and w blocks T - 2 sum b X Omega, b block T.
 template <int NIT>
-__global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk,
+__global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk,
                                                 int zw, int nyl)
 {
     const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
@@ -459,18 +459,12 @@ __global__ __launch_bounds__(256) void k_q8_bwd(SvfArgs a, const int *gblk, cons
 #pragma unroll
         for (int it = 0; it < NIT; it++) nm[it] = msk[(ch * NIA + i0 + 16 * it + lo) * 4 + g];
     };
-    // chunks from the last to the first (the forward pass went the other way)
-    if (c0 < c1) load_chunk(c1 - 1);
+    // chunks from the last to the first (the forward pass went the other way); no
+    // register prefetch: three wavefronts per SIMD cover each other's loads
     for (long long ch = c1 - 1; ch >= c0; ch--) {
-        uint3 cw[4];
-        uint4 cd[QW], cm[NIT];
-#pragma unroll
-        for (int q = 0; q < 4; q++) cw[q] = nw[q];
-#pragma unroll
-        for (int u = 0; u < QW; u++) cd[u] = nd[u];
-#pragma unroll
-        for (int it = 0; it < NIT; it++) cm[it] = nm[it];
-        if (ch - 1 >= c0) load_chunk(ch - 1);
+        load_chunk(ch);
+        const uint3 *cw = nw;
+        const uint4 *cd = nd, *cm = nm;
         v4i bp[QU];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
