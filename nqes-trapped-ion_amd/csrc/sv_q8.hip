@@ -408,9 +408,10 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
 // Wavefronts share nothing (no LDS, no barriers).  Per chunk: B operand of factor digit t = dword t of
 // the lane's digit planes of row tile q (q = 0..3); A operand of spin
 // tile it, weight digit u: dword q = digits & mask.  Accumulator s = t + u collects
-// sum_r b X Omega products of weight 256^s, tacc[s] the column sums T.  The next
-// chunk's words, digits and masks are loaded while the current chunk is multiplied
-// (the current copies are only ever register copies of those loads).  Writes
+// sum_r b X Omega products of weight 256^s, tacc[s] the column sums T.  Each chunk's
+// words, digits and masks load at the top of its iteration; three wavefronts per SIMD
+// (166 registers, accumulators in VGPRs) cover each other's loads (measured faster than
+// two wavefronts with the next chunk's operands prefetched into registers).  Writes
 // part[b][0..P) for its columns: a and w blocks T - 2 sum b X Omega, b block T.
 template <int NIT>
 __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk,
