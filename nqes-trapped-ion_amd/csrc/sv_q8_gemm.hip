@@ -16,21 +16,30 @@
 
 namespace {
 
-// sum over the 16 lanes of a row by DPP (quad swaps, then half-row and row mirrors);
-// every lane ends with the same value
-__device__ inline double dpp_sum16(double x)
+// DPP move of a double: lane l takes the value of its partner under CTRL
+template <int CTRL>
+__device__ inline double dpp_mov(double v)
 {
-    auto step = [](double v, auto ctrl) -> double {
-        const long long b = __double_as_longlong(v);
-        const int lo = __builtin_amdgcn_mov_dpp((int)b, decltype(ctrl)::value, 0xF, 0xF, true);
-        const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), decltype(ctrl)::value, 0xF, 0xF, true);
-        return v + __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-    };
-    x = step(x, std::integral_constant<int, 0xB1>{});
-    x = step(x, std::integral_constant<int, 0x4E>{});
-    x = step(x, std::integral_constant<int, 0x141>{});
-    x = step(x, std::integral_constant<int, 0x140>{});
-    return x;
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// The four sums h[e] over the 16 lanes of a row at once: a row mirror splits them in
+// pairs (lanes 0-7 keep e = 0, 1, lanes 8-15 e = 2, 3), a half-row mirror in singles,
+// two quad swaps finish them.  Lane l returns the sum of h[l / 4] (every lane of the
+// row contributes exactly once).
+__device__ inline double dpp_sum16x4(const double (&h)[4], int l)
+{
+    const bool lowhalf = l < 8, lowquad = (l & 7) < 4;
+    const double k0 = lowhalf ? h[0] : h[2], k1 = lowhalf ? h[1] : h[3];
+    const double s0 = lowhalf ? h[2] : h[0], s1 = lowhalf ? h[3] : h[1];
+    const double a0 = k0 + dpp_mov<0x140>(s0), a1 = k1 + dpp_mov<0x140>(s1);
+    double b = (lowquad ? a0 : a1) + dpp_mov<0x141>(lowquad ? a1 : a0);
+    b += dpp_mov<0x4E>(b);
+    b += dpp_mov<0xB1>(b);
+    return b;
 }
 
 // ---- GEMM forward pass (N <= 32, Ucols <= 256).
@@ -238,11 +247,12 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
                 for (int s = 0; s < NSF; s++) accp[s] = acc[s];
             }
         }
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const double hr = dpp_sum16(h[e]);
+        {
+            const double hr = dpp_sum16x4(h, lo);  // row e = lo / 4
+            const int e = lo >> 2;
+            const double ese = e == 0 ? es[0] : (e == 1 ? es[1] : (e == 2 ? es[2] : es[3]));
             const long long r = rt * 16 + 4 * g + e;
-            if (lo == 0 && r < R) wrow[r] = hr * vsc * es[e];
+            if ((lo & 3) == 0 && r < R) wrow[r] = hr * vsc * ese;
         }
     }
 }
