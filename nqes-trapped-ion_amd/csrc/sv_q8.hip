@@ -647,8 +647,10 @@ hipError_t nq_q8_apply(const SvfArgs &a0, const Q8Shape &sh, void *scratch, void
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = nq_launch_svf_combine(false, a, st);
-    if (e != hipSuccess) return e;
+    if (!(sh.gemm && (a.K == 1 || a.K == 2 || a.K == 4))) {  // else the forward pass wrote w
+        e = nq_launch_svf_combine(false, a, st);
+        if (e != hipSuccess) return e;
+    }
     e = launch_bwd<0>(a, sh, st);
     if (e != hipSuccess) return e;
     return nq_chunk_reduce(false, a.part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, out, st);

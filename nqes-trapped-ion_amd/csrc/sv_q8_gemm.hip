@@ -133,7 +133,9 @@ __global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst,
 // bias).  Per row: each lane
 // sums its spin (FP64 Horner over the digit weights, sign of its spin) and bias digit,
 // then 16 lanes by DPP.  Output wrow[r].
-template <int NST, int NCG>
+// KW = K in 1, 2, 4: the sample weights w_s written directly (no combine step); KW = 0:
+// the row sums wrow[r]
+template <int NST, int NCG, int KW>
 __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs a)
 {
     constexpr int NSF = QU + QV - 1;
@@ -153,7 +155,8 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
     const long long R = a.S * a.K;
     const long long ntile = (R + 15) / 16;
     const uint4 *Uf = (const uint4 *)a.Uf;
-    double *wrow = (double *)a.wrow;
+    double *wrow = (double *)a.wrow, *wout = (double *)a.w;
+    const double ov = KW ? *(const double *)a.ov : 0.0;
     const double pw = lo < QV ? ldexp(1.0, 8 * lo) : 0.0;  // weight of this lane's bias digit
     const long long step = (long long)gridDim.x * (blockDim.x >> 6);
     // Loads are unconditional (no branch between a load and its use, so the waits count
@@ -252,7 +255,20 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
             const int e = lo >> 2;
             const double ese = e == 0 ? es[0] : (e == 1 ? es[1] : (e == 2 ? es[2] : es[3]));
             const long long r = rt * 16 + 4 * g + e;
-            if ((lo & 3) == 0 && r < R) wrow[r] = hr * vsc * ese;
+            const double wr = hr * vsc * ese;  // the row sum, at lanes l % 4 == 0
+            if (KW == 0) {
+                if ((lo & 3) == 0 && r < R) wrow[r] = wr;
+            } else {
+                // w_s = sum over the sample's KW rows (lanes 4 apart, added in order as the
+                // combine kernel does) - o^T v, at the lane of its first row
+                double acc = 0.0 + wr;
+                if (KW >= 2) acc += dpp_mov<0x104>(wr);  // row shift left by 4: lane l + 4
+                if (KW == 4) {
+                    acc += dpp_mov<0x108>(wr);
+                    acc += dpp_mov<0x10C>(wr);
+                }
+                if ((lo & (4 * KW - 1)) == 0 && r < R) wout[r / KW] = acc - ov;
+            }
         }
     }
 }
@@ -264,7 +280,13 @@ hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
     hipLaunchKernelGGL(k_q8_prep_vg, dim3(sh.ncg64 * (sh.nst + 1)), dim3(256), 0, st, a, sh.ncg64, sh.nst, (uint4 *)a.vq);
     const size_t lds = sh.qg_bytes - 16;
     const dim3 grid(sh.fwd_blocks), blk(256);
-#define NQ_FWDG(NST, NCG) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG>), grid, blk, lds, st, a)
+#define NQ_FWDG(NST, NCG)                                                                              \
+    do {                                                                                              \
+        if (a.K == 1) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 1>), grid, blk, lds, st, a);           \
+        else if (a.K == 2) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 2>), grid, blk, lds, st, a);      \
+        else if (a.K == 4) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 4>), grid, blk, lds, st, a);      \
+        else hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0>), grid, blk, lds, st, a);                    \
+    } while (0)
     switch (sh.nst * 8 + sh.ncg64) {
     case 9: NQ_FWDG(1, 1); break;
     case 10: NQ_FWDG(1, 2); break;
