@@ -805,10 +805,12 @@ static int build_sv_graph(nqes_ctx *c)
         HIPCHK(c, hipHostMalloc((void **)&c->hr_pin, 8 * (P + 1), hipHostMallocDefault), "pinned result");
         c->pin_len = (size_t)(P + 1);
     }
-    const SvfArgs f = svf_args(c, c->S);
+    // v and o^T v arrive in one copy: the graph's kernels read o^T v right after v (the
+    // v buffer holds 16 P bytes, room for P + 1 doubles)
+    SvfArgs f = svf_args(c, c->S);
+    f.ov = (const double *)c->dv.p + P;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin S*v capture");
-    hipError_t e = hipMemcpyAsync(c->dv.p, c->hv_pin, 8 * P, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->dov.p, c->hv_pin + P, 8, hipMemcpyHostToDevice, c->stream);
+    hipError_t e = hipMemcpyAsync(c->dv.p, c->hv_pin, 8 * (P + 1), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = nq_q8_apply(f, c->q8_shape, c->dsvfscratch.p, c->dsvout.p, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(c->hr_pin, c->dsvout.p, 8 * (P + 1), hipMemcpyDeviceToHost, c->stream);
     hipGraph_t g = nullptr;
