@@ -72,6 +72,12 @@ typedef struct {
     double t_sv_ms;           /* device time of all S*v kernels               */
     double t_total_ms;        /* whole SR step (host wall clock)              */
     int nonfinite;            /* 1 if a non-finite energy/gradient was seen   */
+    uint64_t ulp_ties;        /* FP64 path: accept decisions whose uniform draw u
+                                 lies within 2^-40 min(1, p) of min(1, p), p the
+                                 amplitude ratio |det Phi'|^2 / |det Phi|^2
+                                 (rbm.cc:540-543): decisions that an implementation
+                                 rounding p in another order may take the other
+                                 way; 0 on the mixed path (not counted)         */
 } nqes_stats;
 
 /* Creates a context on HIP device `device`.  Returns NULL on failure. */
@@ -116,6 +122,11 @@ int nqes_get_samples(nqes_ctx *ctx, uint32_t *spins, double *H, uint64_t *accept
 int nqes_get_O(nqes_ctx *ctx, int64_t row0, int64_t nrows, double *out);
 int nqes_get_moments(nqes_ctx *ctx, double *o, double *skk, double *f, double *eloc,
                      double *emat);
+/* Near-tie accept decisions of the last nqes_sample (FP64 path; see nqes_stats.ulp_ties):
+ * per chain (C entries, may be NULL) and their total (may be NULL).  The reference
+ * counterpart is the accept test of monte_carlo_step (rbm.cc:540-543), which has no
+ * such count; 0 on the mixed path. */
+int nqes_get_ulp_ties(nqes_ctx *ctx, uint64_t *per_chain, uint64_t *total);
 int nqes_apply_S(nqes_ctx *ctx, const double *v, double *out);
 int nqes_solve(nqes_ctx *ctx, double rtol, uint64_t maxit, double *x, int *flag,
                uint64_t *iters);

@@ -631,8 +631,8 @@ int main(int argc, char **argv)
             std::cout << format_row(evals) << std::endl;
             if (want_stats)
                 std::cerr << "step " << iter << ": " << st.t_total_ms << " ms (sample " << st.t_sample_ms
-                          << ", solve " << st.t_solve_ms << ", " << st.minres_iters << " MINRES-QLP iterations)"
-                          << std::endl;
+                          << ", solve " << st.t_solve_ms << ", " << st.minres_iters << " MINRES-QLP iterations, "
+                          << st.ulp_ties << " near-tie accept decisions)" << std::endl;
             if (conv_test.update(iter, evals, tol)) {
                 std::cout << "Converged after " << iter + 1 << " steps." << std::endl;
                 break;

@@ -18,11 +18,14 @@ struct Fp64SampleArgs {
     const void *W;             // P complex
     const void *chT, *shT;     // K*N*M complex: cosh(2w), sinh(2w), [kk][i][j]
     const void *eaP, *eaN;     // K*N complex: exp(2a), exp(-2a), [kk][i]
-    void *O;                   // S*P complex (NULL: not stored)
+    void *O;                   // S rows of O (NULL: not stored): P complex, or Pld doubles when Oreal
+    int Oreal;                 // 1: real weights, store Re O only (rows padded with zeros to Pld)
+    long long Pld;
     void *Hs;                  // S*K*K complex
     void *Es;                  // S complex: trace H_loc
     uint32_t *spins;           // S*K*NW
     unsigned long long *accepts; // C
+    unsigned long long *ties;  // C: accept decisions within the near-tie window (NULL: not written)
     void *cor;                 // correlation kernel: C*(2N+2N^2)*K*K complex
 };
 
@@ -44,7 +47,27 @@ hipError_t nq_launch_colsum64(const void *X, long long S, long long ncol, int nc
                               hipStream_t st);
 hipError_t nq_launch_sv64(const void *O, const void *v, long long S, long long P, int nchunk, void *w,
                           void *part, void *out, hipStream_t st);
-hipError_t nq_launch_mathprobe64(const void *in, long long n, void *out, hipStream_t st);
+// moments of real O (rows of stride ld doubles), partials equal to nq_launch_moments64's
+hipError_t nq_launch_moments64r(const void *O, long long ld, const void *E, long long S, long long P, int nchunk,
+                                void *po, void *ps, void *pf, void *out_o, void *out_s, void *out_f, hipStream_t st);
+// One-pass FP64 S*v over rows of nvec 16-byte vectors (2 real or 1 complex entries):
+// wpr waves per row, qv vectors per lane, `blocks` partial rows.  ok = 0: v (nvec*16 B)
+// exceeds lds_max or a row exceeds 16 waves x 9 vectors.
+struct Sv64Shape {
+    int ok, qv, wpr, blocks;
+};
+Sv64Shape nq_sv64_shape(long long nvec, long long S, size_t lds_max);
+// raw sums: out[p] = sum_s conj(O_sp) w_s (p < P), out[P] = sum_s w_s, w_s = O_s^T v - ov
+// (v, ov, out on the device, D = double or complex double); part: blocks * nvec * W D,
+// wsum: blocks D, scratch: (blocks/32 + 1) * (P + 1) D
+hipError_t nq_launch_sv_rows64(bool cplx, const Sv64Shape &sh, const void *O, const void *v, const void *ov,
+                               long long S, long long nvec, long long P, void *part, void *wsum, void *scratch,
+                               void *out, hipStream_t st);
+// out = o^T v (one D), a fixed summation order
+hipError_t nq_launch_dot64(bool cplx, const void *o, const void *v, long long P, void *out, hipStream_t st);
+// out[p] = (raw[p] - conj(o_p) raw[P]) / S + skk_p v_p
+hipError_t nq_launch_sv_finish64(bool cplx, const void *raw, const void *o, const void *skk, const void *v,
+                                 long long P, double S, void *out, hipStream_t st);
 
 // ---------------------------------------------------------------- mixed precision
 // Arguments of the mixed sampler (G lanes per chain).  T = float or complex float

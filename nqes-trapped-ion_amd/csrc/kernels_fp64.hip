@@ -154,10 +154,19 @@ __device__ inline void chain_start64(const Fp64SampleArgs &a, int c, cd *tl, uin
     state64<K, JPL>(a, tl, sp, phi, phi_inv);
 }
 
-// One Metropolis sweep (rbm.cc:530-552): proposals k < K, i < N in order.
+// Width of an accept decision's near-tie window, relative to the acceptance
+// probability min(1, p): 2^-40 (about 4096 ulps).  The amplitude ratio p is a product
+// over M hidden units and a K x K determinant, so two implementations that round in a
+// different order (the reference's libm, Eigen and FMA contraction against the
+// canonical order here) can differ by many ulps of p; a uniform draw closer to
+// min(1, p) than this window may be decided the other way by them.
+#define NQ_TIE_REL 0x1p-40
+
+// One Metropolis sweep (rbm.cc:530-552): proposals k < K, i < N in order.  ties
+// counts the decisions whose draw falls within the near-tie window of min(1, p).
 template <int K, int JPL>
 __device__ inline void sweep64(const Fp64SampleArgs &a, cd *tl, uint32_t *sp, pcg &rng, cd *phi, cd *phi_inv,
-                               unsigned long long &accepts)
+                               unsigned long long &accepts, unsigned long long &ties)
 {
     const int N = a.N, M = a.M, NW = a.NW;
     const int lane = threadIdx.x;
@@ -206,7 +215,9 @@ __device__ inline void sweep64(const Fp64SampleArgs &a, cd *tl, uint32_t *sp, pc
             cd det_p = detK<K>(phip);
             double prob = (det_phi.re == 0.0 && det_phi.im == 0.0) ? 1.0 : norm(det_p) / norm(det_phi);
             double u = canonical(pcg_next(rng));
-            if (u <= (prob < 1.0 ? prob : 1.0)) {
+            const double pmin = prob < 1.0 ? prob : 1.0;
+            if (fabs(u - pmin) <= pmin * NQ_TIE_REL) ties++;
+            if (u <= pmin) {
                 det_phi = det_p;
 #pragma unroll
                 for (int q = 0; q < K * K; q++) phi[q] = phip[q];
@@ -374,10 +385,10 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
     cd phi[K * K], phi_inv[K * K];
     pcg rng;
     chain_start64<K, JPL>(a, c, tl, sp, rng, phi, phi_inv);
-    unsigned long long accepts = 0;
+    unsigned long long accepts = 0, ties = 0;
     const int nsweeps = a.B + a.m;
     for (int sw = 0; sw < nsweeps; sw++) {
-        sweep64<K, JPL>(a, tl, sp, rng, phi, phi_inv, accepts);
+        sweep64<K, JPL>(a, tl, sp, rng, phi, phi_inv, accepts, ties);
         if (sw < a.B) continue;
 
         // ---- one sample
@@ -408,6 +419,9 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
             cd G[K * K];
             matmulK<K>(phi_inv, phi, G);
             cd *Orow = static_cast<cd *>(a.O) + s * a.P;
+            double *Orow_r = static_cast<double *>(a.O) + s * a.Pld;  // real weights: Re O only
+            if (a.Oreal)
+                for (long long p = a.P + lane; p < a.Pld; p += 64) Orow_r[p] = 0.0;
             for (long long p = lane; p < a.P; p += 64) {
                 int k = (int)(p / L);
                 int r = (int)(p % L);
@@ -440,12 +454,15 @@ __global__ __launch_bounds__(64) void k_sample64(Fp64SampleArgs a)
 #pragma unroll
                     for (int cc = 0; cc < K; cc++) X[rw * K + cc] = (cc == k) ? y : G[rw * K + cc];
                 }
-                Orow[p] = detK<K>(X);
+                const cd op = detK<K>(X);
+                if (a.Oreal) Orow_r[p] = op.re;
+                else Orow[p] = op;
             }
         }
         __syncthreads();
     }
     if (lane == 0) a.accepts[c] = accepts;
+    if (lane == 0 && a.ties) a.ties[c] = ties;
 }
 
 // Dense S of the rbm-N* executables (rbm.cc:651-657, 686-696): oo[l][l'] =
@@ -558,9 +575,9 @@ __global__ __launch_bounds__(64) void k_corr64(Fp64SampleArgs a)
     cd phi[K * K], phi_inv[K * K];
     pcg rng;
     chain_start64<K, JPL>(a, c, tl, sp, rng, phi, phi_inv);
-    unsigned long long accepts = 0;
+    unsigned long long accepts = 0, ties = 0;
     for (int sw = 0; sw < a.B + a.m; sw++) {
-        sweep64<K, JPL>(a, tl, sp, rng, phi, phi_inv, accepts);
+        sweep64<K, JPL>(a, tl, sp, rng, phi, phi_inv, accepts, ties);
         if (sw < a.B) continue;
         __syncthreads();
         // sigma^x_i and sigma^z_i
@@ -664,6 +681,32 @@ __global__ void k_moments64(const cd *__restrict__ O, const cd *__restrict__ E, 
     pf[o] = sf;
 }
 
+// The same sums for real O (real weights: every imaginary part of O is zero), rows of
+// stride ld doubles.  Each sum takes the real operations the complex one reduces to
+// when Im O = 0, in the same order, so the partials equal k_moments64's bit for bit:
+// so = (sum O, 0), ss = (sum O*O, 0), sf = (sum E.re*O, sum E.im*O).
+__global__ void k_moments64r(const double *__restrict__ O, long long ld, const cd *__restrict__ E, long long S,
+                             long long P, int nchunk, cd *__restrict__ po, cd *__restrict__ ps,
+                             cd *__restrict__ pf)
+{
+    long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    long long s0 = S * blockIdx.y / nchunk, s1 = S * (blockIdx.y + 1) / nchunk;
+    double so = 0.0, ss = 0.0, fr = 0.0, fi = 0.0;
+    for (long long s = s0; s < s1; s++) {
+        const double v = O[s * ld + p];
+        const cd e = E[s];
+        so = so + v;
+        ss = ss + v * v;
+        fr = fr + e.re * v;
+        fi = fi + e.im * v;
+    }
+    long long o = (long long)blockIdx.y * P + p;
+    po[o] = mk(so, 0.0);
+    ps[o] = mk(ss, 0.0);
+    pf[o] = mk(fr, fi);
+}
+
 // out[col] = sum over chunks of part[chunk][col], chunks in order
 __global__ void k_chunksum64(const cd *__restrict__ part, long long ncol, int nchunk, cd *__restrict__ out)
 {
@@ -738,6 +781,212 @@ __global__ void k_sv_colacc64(const cd *__restrict__ O, const cd *__restrict__ w
     cd acc = zero();
     for (long long s = s0; s < s1; s++) acc = add(acc, mul(w[s], conj(O[s * P + p])));
     part[(long long)blockIdx.y * P + p] = acc;
+}
+
+// ---------------------------------------------------------------- S*v, one pass
+// FP64 O rows as 16-byte vectors: 2 real entries (real weights: O is real) or 1
+// complex entry.
+template <bool CPLX> struct Row64;
+template <> struct Row64<false> {
+    using D = double;
+    static constexpr int W = 2;
+};
+template <> struct Row64<true> {
+    using D = cd;
+    static constexpr int W = 1;
+};
+
+__device__ inline double d_add(double a, double b) { return a + b; }
+__device__ inline cd d_add(cd a, cd b) { return add(a, b); }
+__device__ inline double d_sub(double a, double b) { return a - b; }
+__device__ inline cd d_sub(cd a, cd b) { return sub(a, b); }
+__device__ inline double d_mul(double a, double b) { return a * b; }
+__device__ inline cd d_mul(cd a, cd b) { return mul(a, b); }
+__device__ inline double d_conj(double a) { return a; }
+__device__ inline cd d_conj(cd a) { return conj(a); }
+__device__ inline double d_xor(double a, int m) { return __shfl_xor(a, m); }
+__device__ inline cd d_xor(cd a, int m) { return shfl_xor_cd(a, m); }
+template <typename D> __device__ inline D d_zero();
+template <> __device__ inline double d_zero<double>() { return 0.0; }
+template <> __device__ inline cd d_zero<cd>() { return zero(); }
+
+template <bool CPLX>
+__device__ inline void row_vec(const double2 &x, typename Row64<CPLX>::D *o)
+{
+    if constexpr (CPLX) {
+        o[0] = mk(x.x, x.y);
+    } else {
+        o[0] = x.x;
+        o[1] = x.y;
+    }
+}
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+// O is streamed once per S*v: loads that do not keep it in the caches
+__device__ __forceinline__ double2 ld_stream2(const double2 *p)
+{
+    d2v r = __builtin_nontemporal_load((const d2v *)p);
+    return make_double2(r.x, r.y);
+}
+
+// S*v in one streamed pass over FP64 O rows (rbm.cc:661-684).  A block of WPR
+// wavefronts owns one row at a time (rows blockIdx.x, blockIdx.x + gridDim.x, ...);
+// lane l of wave h holds the row's 16-byte vectors u = l + 64 (h + WPR q), q < QV.
+// Per row: w_s = O_s^T v - o^T v (v staged once per block in LDS; per-lane partial
+// dot in ascending q, 64-lane butterfly, the WPR wave partials added in wave order
+// through LDS, one barrier per row on alternating slots), then acc += conj(O_s) w_s
+// from the same registers, so O is read once.  The next row's loads are issued before
+// the current row's arithmetic.  Every wave of a block takes the same rows, so the
+// barriers match.  Output: the block's partial row (its own vectors, no cross-wave
+// sum) and its sum of w_s.  Entries of v at p >= P and O's row padding are zero.
+template <bool CPLX, int QV, int WPR>
+__global__ __launch_bounds__(64 * WPR) void k_sv_rows64(const double2 *__restrict__ O, const void *v_,
+                                                         const void *ov_, long long S, long long nvec, long long P,
+                                                         void *part_, void *wsum_)
+{
+    using D = typename Row64<CPLX>::D;
+    constexpr int W = Row64<CPLX>::W;
+    extern __shared__ __align__(16) unsigned char smem_sv64[];
+    __shared__ D red[2][WPR];
+    D *vl = (D *)smem_sv64;
+    const D *v = (const D *)v_;
+    const D ov = *(const D *)ov_;
+    const int t = threadIdx.x, lane = t & 63, h = t >> 6;
+    for (long long q = t; q < nvec * W; q += blockDim.x) vl[q] = q < P ? v[q] : d_zero<D>();
+    __syncthreads();
+    D acc[QV][W];
+#pragma unroll
+    for (int q = 0; q < QV; q++)
+#pragma unroll
+        for (int e = 0; e < W; e++) acc[q][e] = d_zero<D>();
+    D wacc = d_zero<D>();
+    const double2 z2 = make_double2(0.0, 0.0);
+    const long long ns = gridDim.x;
+    long long s = blockIdx.x;
+    double2 x[QV];
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+        const long long u = lane + 64LL * (h + WPR * q);
+        x[q] = (s < S && u < nvec) ? ld_stream2(O + s * nvec + u) : z2;
+    }
+    int par = 0;
+    for (; s < S; s += ns) {
+        double2 y[QV];
+        const long long sn = s + ns;
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            const long long u = lane + 64LL * (h + WPR * q);
+            y[q] = (sn < S && u < nvec) ? ld_stream2(O + sn * nvec + u) : z2;
+        }
+        D dot = d_zero<D>();
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            const long long u = lane + 64LL * (h + WPR * q);
+            if (u < nvec) {
+                D od[W];
+                row_vec<CPLX>(x[q], od);
+#pragma unroll
+                for (int e = 0; e < W; e++) dot = d_add(dot, d_mul(od[e], vl[u * W + e]));
+            }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) dot = d_add(dot, d_xor(dot, off));
+        if constexpr (WPR > 1) {
+            if (lane == 0) red[par][h] = dot;
+            __syncthreads();
+            dot = red[par][0];
+#pragma unroll
+            for (int k = 1; k < WPR; k++) dot = d_add(dot, red[par][k]);
+            par ^= 1;
+        }
+        const D w = d_sub(dot, ov);
+        wacc = d_add(wacc, w);
+#pragma unroll
+        for (int q = 0; q < QV; q++) {
+            D od[W];
+            row_vec<CPLX>(x[q], od);
+#pragma unroll
+            for (int e = 0; e < W; e++) acc[q][e] = d_add(acc[q][e], d_mul(w, d_conj(od[e])));
+        }
+#pragma unroll
+        for (int q = 0; q < QV; q++) x[q] = y[q];
+    }
+    D *part = (D *)part_ + (long long)blockIdx.x * nvec * W;
+#pragma unroll
+    for (int q = 0; q < QV; q++) {
+        const long long u = lane + 64LL * (h + WPR * q);
+        if (u < nvec)
+#pragma unroll
+            for (int e = 0; e < W; e++) part[u * W + e] = acc[q][e];
+    }
+    if (t == 0) ((D *)wsum_)[blockIdx.x] = wacc;
+}
+
+// o^T v in one block: thread t sums p = t, t + 256, ... in ascending order, then a
+// fixed pairwise tree over the 256 partials (the same value on every call)
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_dot64(const void *o_, const void *v_, long long P, void *out_)
+{
+    using D = typename Row64<CPLX>::D;
+    __shared__ D red[256];
+    const D *o = (const D *)o_, *v = (const D *)v_;
+    D acc = d_zero<D>();
+    for (long long p = threadIdx.x; p < P; p += 256) acc = d_add(acc, d_mul(o[p], v[p]));
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = d_add(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *(D *)out_ = red[0];
+}
+
+// S v from the raw sums (rbm.cc:661-684, centred form):
+// out[p] = (raw[p] - conj(o_p) raw[P]) / S + skk_p v_p
+template <bool CPLX>
+__global__ void k_sv_finish64(const void *raw_, const void *o_, const void *skk_, const void *v_, long long P,
+                              double Sd, void *out_)
+{
+    using D = typename Row64<CPLX>::D;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const D *raw = (const D *)raw_, *o = (const D *)o_, *skk = (const D *)skk_, *v = (const D *)v_;
+    D r = d_sub(raw[p], d_mul(d_conj(o[p]), raw[P]));
+    if constexpr (CPLX) r = divr(r, Sd);
+    else r = r / Sd;
+    ((D *)out_)[p] = d_add(r, d_mul(skk[p], v[p]));
+}
+
+template <bool CPLX, int QV, int WPR>
+hipError_t launch_rows64(const void *O, const void *v, const void *ov, long long S, long long nvec, long long P,
+                         int nblk, void *part, void *wsum, hipStream_t st)
+{
+    const size_t lds = (size_t)nvec * 16;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k_sv_rows64<CPLX, QV, WPR>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_sv_rows64<CPLX, QV, WPR>), dim3(nblk), dim3(64 * WPR), lds, st, (const double2 *)O, v, ov,
+                       S, nvec, P, part, wsum);
+    return hipGetLastError();
+}
+
+template <bool CPLX>
+hipError_t launch_rows64_shape(const Sv64Shape &sh, const void *O, const void *v, const void *ov, long long S,
+                               long long nvec, long long P, void *part, void *wsum, hipStream_t st)
+{
+#define NQ_R64(q, w)                                                                                              \
+    if (sh.qv == q && sh.wpr == w) return launch_rows64<CPLX, q, w>(O, v, ov, S, nvec, P, sh.blocks, part, wsum, st);
+    NQ_R64(1, 1) NQ_R64(2, 1) NQ_R64(3, 1) NQ_R64(4, 1) NQ_R64(5, 1) NQ_R64(6, 1) NQ_R64(7, 1) NQ_R64(8, 1)
+    NQ_R64(9, 1)
+    NQ_R64(5, 2) NQ_R64(6, 2) NQ_R64(7, 2) NQ_R64(8, 2) NQ_R64(9, 2)
+    NQ_R64(5, 4) NQ_R64(6, 4) NQ_R64(7, 4) NQ_R64(8, 4) NQ_R64(9, 4)
+    NQ_R64(5, 8) NQ_R64(6, 8) NQ_R64(7, 8) NQ_R64(8, 8) NQ_R64(9, 8)
+    NQ_R64(5, 16) NQ_R64(6, 16) NQ_R64(7, 16) NQ_R64(8, 16) NQ_R64(9, 16)
+#undef NQ_R64
+    return hipErrorInvalidValue;
 }
 
 template <int K, int JPL>
@@ -890,6 +1139,72 @@ hipError_t nq_launch_colsum64(const void *X, long long S, long long ncol, int nc
     return hipGetLastError();
 }
 
+Sv64Shape nq_sv64_shape(long long nvec, long long S, size_t lds_max)
+{
+    Sv64Shape sh{0, 0, 0, 0};
+    if (nvec < 1 || (size_t)nvec * 16 > lds_max) return sh;
+    for (int wpr = 1; wpr <= 16; wpr *= 2) {
+        const long long q = (nvec + 64LL * wpr - 1) / (64LL * wpr);
+        if (q <= 9) {
+            sh.wpr = wpr;
+            sh.qv = (int)q;
+            break;
+        }
+    }
+    if (!sh.qv) return sh;
+    if (sh.wpr > 1 && sh.qv < 5) sh.qv = 5;  // instantiated shapes only; idle slots are masked
+    long long nb = 2048 / sh.wpr;
+    if (nb > S) nb = S > 0 ? S : 1;
+    sh.blocks = (int)nb;
+    sh.ok = 1;
+    return sh;
+}
+
+hipError_t nq_launch_sv_rows64(bool cplx, const Sv64Shape &sh, const void *O, const void *v, const void *ov,
+                               long long S, long long nvec, long long P, void *part, void *wsum, void *scratch,
+                               void *out, hipStream_t st)
+{
+    if (!sh.ok) return hipErrorInvalidValue;
+    hipError_t e = cplx ? launch_rows64_shape<true>(sh, O, v, ov, S, nvec, P, part, wsum, st)
+                        : launch_rows64_shape<false>(sh, O, v, ov, S, nvec, P, part, wsum, st);
+    if (e != hipSuccess) return e;
+    const size_t dsz = cplx ? 16 : 8;
+    const long long stride = nvec * (cplx ? 1 : 2);
+    e = nq_chunk_reduce(cplx, part, P, stride, sh.blocks, scratch, out, st);
+    if (e != hipSuccess) return e;
+    char *scr2 = static_cast<char *>(scratch) + (size_t)(sh.blocks / 32 + 1) * P * dsz;
+    return nq_chunk_reduce(cplx, wsum, 1, 1, sh.blocks, scr2, static_cast<char *>(out) + (size_t)P * dsz, st);
+}
+
+hipError_t nq_launch_dot64(bool cplx, const void *o, const void *v, long long P, void *out, hipStream_t st)
+{
+    if (cplx) hipLaunchKernelGGL(k_dot64<true>, dim3(1), dim3(256), 0, st, o, v, P, out);
+    else hipLaunchKernelGGL(k_dot64<false>, dim3(1), dim3(256), 0, st, o, v, P, out);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_sv_finish64(bool cplx, const void *raw, const void *o, const void *skk, const void *v,
+                                 long long P, double S, void *out, hipStream_t st)
+{
+    const dim3 g((unsigned)((P + 255) / 256));
+    if (cplx) hipLaunchKernelGGL(k_sv_finish64<true>, g, dim3(256), 0, st, raw, o, skk, v, P, S, out);
+    else hipLaunchKernelGGL(k_sv_finish64<false>, g, dim3(256), 0, st, raw, o, skk, v, P, S, out);
+    return hipGetLastError();
+}
+
+hipError_t nq_launch_moments64r(const void *O, long long ld, const void *E, long long S, long long P, int nchunk,
+                                void *po, void *ps, void *pf, void *out_o, void *out_s, void *out_f, hipStream_t st)
+{
+    dim3 g((unsigned)((P + 255) / 256), (unsigned)nchunk);
+    hipLaunchKernelGGL(k_moments64r, g, dim3(256), 0, st, (const double *)O, ld, (const cd *)E, S, P, nchunk,
+                       (cd *)po, (cd *)ps, (cd *)pf);
+    dim3 g2((unsigned)((P + 255) / 256));
+    hipLaunchKernelGGL(k_chunksum64, g2, dim3(256), 0, st, (const cd *)po, P, nchunk, (cd *)out_o);
+    hipLaunchKernelGGL(k_chunksum64, g2, dim3(256), 0, st, (const cd *)ps, P, nchunk, (cd *)out_s);
+    hipLaunchKernelGGL(k_chunksum64, g2, dim3(256), 0, st, (const cd *)pf, P, nchunk, (cd *)out_f);
+    return hipGetLastError();
+}
+
 hipError_t nq_launch_sv64(const void *O, const void *v, long long S, long long P, int nchunk, void *w,
                           void *part, void *out, hipStream_t st)
 {
@@ -902,32 +1217,5 @@ hipError_t nq_launch_sv64(const void *O, const void *v, long long S, long long P
                        (cd *)part);
     hipLaunchKernelGGL(k_chunksum64, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, (const cd *)part, P,
                        nchunk, (cd *)out);
-    return hipGetLastError();
-}
-
-// Device copies of the elementary functions, for bit-exactness tests.
-__global__ void k_mathprobe64(const double *in, long long n, double *out)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    cd z = mk(in[2 * t], in[2 * t + 1]);
-    cd b = mk(in[2 * ((t + 1) % n)], in[2 * ((t + 1) % n) + 1]);
-    cd r[6];
-    r[0] = cexp_(z);
-    r[1] = ccosh_(z);
-    r[2] = csinh_(z);
-    r[3] = ctanh_(z);
-    r[4] = div(z, b);
-    r[5] = mk(exp_(z.re), 0.0);
-    for (int q = 0; q < 6; q++) {
-        out[(t * 6 + q) * 2] = r[q].re;
-        out[(t * 6 + q) * 2 + 1] = r[q].im;
-    }
-}
-
-hipError_t nq_launch_mathprobe64(const void *in, long long n, void *out, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_mathprobe64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const double *)in, n,
-                       (double *)out);
     return hipGetLastError();
 }

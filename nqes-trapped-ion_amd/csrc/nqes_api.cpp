@@ -98,7 +98,7 @@ struct nqes_ctx {
     bool cplx = true;          // mixed path: complex arithmetic (false: all weights real)
     int C = 0, m = 0, B = 0;
     long long S = 0, Pld = 0;
-    DevBuf dO, dHs, dEs, dspins, dacc;
+    DevBuf dO, dHs, dEs, dspins, dacc, dties;
     DevBuf dmpart_o, dmpart_s, dmpart_f, dmo, dms, dmf, dhpart, dhsum, depart, desum;
     DevBuf dv, dov, dw, dsvpart, dsvwpart, dsvout, dsvscratch, dmscratch;
     DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
@@ -113,6 +113,9 @@ struct nqes_ctx {
     DevBuf doo, dvo, dskk;
     DevBuf denum;                                      // exhaustive evaluation outputs
     bool svf = false, q8 = false;
+    bool oreal = false;                                // FP64 path, real weights: O stored as doubles
+    Sv64Shape sv64{};                                  // FP64 path: one-pass S*v shape (ok = 0: two passes)
+    DevBuf dsvwsum;
     SvfShape svf_shape{};
     Q8Shape q8_shape{};
     int mchunks = 1, svchunks = 1;
@@ -154,7 +157,7 @@ static double event_ms(hipEvent_t a, hipEvent_t b)
 }
 
 // bytes of one O element and the padded row length (rows start on 16 bytes)
-static size_t o_elem(int prec, bool cplx) { return prec == NQES_PREC_FP64 ? 16 : (cplx ? 8 : 4); }
+static size_t o_elem(int prec, bool cplx) { return prec == NQES_PREC_FP64 ? (cplx ? 16 : 8) : (cplx ? 8 : 4); }
 static long long o_ld(long long P, int prec, bool cplx)
 {
     size_t e = o_elem(prec, cplx);
@@ -260,6 +263,8 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
     b += P * 16 + 16 + S * 16 + (P + 1) * 16;  // v, o^T v, w, S*v result
     if (!mixed) {
         b += mch * P * 16;
+        const Sv64Shape s64 = nq_sv64_shape(P, S, 160 * 1024);  // complex rows: the larger partials
+        if (s64.ok) b += (long long)s64.blocks * (P * 16 + 16) + (s64.blocks / 32 + 2) * (P + 1) * 16;
     } else if (!svf) {
         const SvShape sv = nq_sv_shape(cplx, Pld / 2);
         long long nblk = sv.qv ? sv.blocks : std::min<long long>(mch, S);
@@ -344,16 +349,24 @@ static int sample_fp64(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.eaP = c->deaP.p;
     a.eaN = c->deaN.p;
     a.O = c->svf ? nullptr : c->dO.p;
+    a.Oreal = c->oreal ? 1 : 0;
+    a.Pld = c->Pld;
     a.Hs = c->dHs.p;
     a.Es = c->dEs.p;
     a.spins = static_cast<uint32_t *>(c->dspins.p);
     a.accepts = static_cast<unsigned long long *>(c->dacc.p);
+    a.ties = static_cast<unsigned long long *>(c->dties.p);
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream), "event");
     HIPCHK(c, nq_launch_sample64(c->K, a, c->stream), "sample kernel");
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
-    HIPCHK(c, nq_launch_moments64(c->dO.p, c->dEs.p, S, c->P, c->mchunks, c->dmpart_o.p, c->dmpart_s.p,
-                                  c->dmpart_f.p, c->dmo.p, c->dms.p, c->dmf.p, c->stream),
-           "moments kernel");
+    if (c->oreal)
+        HIPCHK(c, nq_launch_moments64r(c->dO.p, c->Pld, c->dEs.p, S, c->P, c->mchunks, c->dmpart_o.p, c->dmpart_s.p,
+                                       c->dmpart_f.p, c->dmo.p, c->dms.p, c->dmf.p, c->stream),
+               "moments kernel");
+    else
+        HIPCHK(c, nq_launch_moments64(c->dO.p, c->dEs.p, S, c->P, c->mchunks, c->dmpart_o.p, c->dmpart_s.p,
+                                      c->dmpart_f.p, c->dmo.p, c->dms.p, c->dmf.p, c->stream),
+               "moments kernel");
     return NQES_OK;
 }
 
@@ -481,17 +494,26 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     c->sampled = false;
     const int K = c->K;
     const long long P = c->P, S = (long long)C * m;
-    bool cplx = true;
-    if (c->prec == NQES_PREC_MIXED) {
-        cplx = false;
-        for (long long q = 0; q < P; q++)
-            if (W[2 * q + 1] != 0.0) {
-                cplx = true;
-                break;
-            }
-    }
+    bool cplx = false;
+    for (long long q = 0; q < P; q++)
+        if (W[2 * q + 1] != 0.0) {
+            cplx = true;
+            break;
+        }
     c->cplx = cplx;
-    c->Pld = o_ld(P, c->prec, cplx);
+    // FP64 path: real weights make O real (every imaginary part is an exact zero), so
+    // it is stored as doubles when the one-pass S*v applies to its rows; the dense S
+    // form keeps complex rows
+    c->oreal = false;
+    c->sv64 = Sv64Shape{0, 0, 0, 0};
+    if (c->prec == NQES_PREC_FP64 && !c->dense) {
+        if (!cplx) {
+            c->sv64 = nq_sv64_shape(o_ld(P, NQES_PREC_FP64, false) / 2, S, c->lds_max);
+            c->oreal = c->sv64.ok != 0;
+        }
+        if (!c->oreal) c->sv64 = nq_sv64_shape(P, S, c->lds_max);
+    }
+    c->Pld = o_ld(P, c->prec, c->prec == NQES_PREC_FP64 ? !c->oreal : cplx);
     drop_sv_graph(c);  // buffers may move
     // factored S*v (mixed path) unless NQES_SV=stored or the shape does not apply; for
     // real weights its int8 form unless NQES_SV=fp64mfma
@@ -530,7 +552,8 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
         HIPCHK(c, c->dsvfscratch.ensure((size_t)(nchunk / 32 + 2) * (P + 1) * 16), "alloc S*v scratch");
         HIPCHK(c, c->dsvftmp.ensure((size_t)3 * (P + 1) * 16), "alloc moments");
     }
-    const size_t obytes = c->svf ? 16 : (size_t)S * c->Pld * o_elem(c->prec, cplx);  // factored: no O
+    const size_t obytes =
+        c->svf ? 16 : (size_t)S * c->Pld * o_elem(c->prec, c->prec == NQES_PREC_FP64 ? !c->oreal : cplx);
     size_t freeb = 0, totalb = 0;
     HIPCHK(c, hipMemGetInfo(&freeb, &totalb), "hipMemGetInfo");
     long long need = (long long)obytes - (long long)c->dO.bytes;
@@ -541,6 +564,8 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     HIPCHK(c, c->dEs.ensure((size_t)(S * 16)), "alloc E");
     HIPCHK(c, c->dspins.ensure((size_t)(S * K * c->NW * 4)), "alloc spins");
     HIPCHK(c, c->dacc.ensure((size_t)C * 8), "alloc accepts");
+    HIPCHK(c, c->dties.ensure((size_t)C * 8), "alloc tie counts");
+    if (c->prec != NQES_PREC_FP64) HIPCHK(c, hipMemsetAsync(c->dties.p, 0, (size_t)C * 8, c->stream), "clear ties");
     c->mchunks = choose_chunks(S, P, 256);
     const size_t mpart = c->svf ? 16 : (size_t)c->mchunks * P * 16;  // factored: moments from the factors
     HIPCHK(c, c->dmpart_o.ensure(mpart), "alloc partials");
@@ -601,10 +626,16 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     c->m = m;
     c->B = B;
     c->S = S;
-    HIPCHK(c, c->dv.ensure((size_t)P * 16), "alloc v");
+    HIPCHK(c, c->dv.ensure((size_t)(P + 1) * 16), "alloc v");
     HIPCHK(c, c->dov.ensure(16), "alloc ov");
     HIPCHK(c, c->dw.ensure((size_t)S * 16), "alloc w");
-    if (c->prec == NQES_PREC_FP64) {
+    if (c->prec == NQES_PREC_FP64 && c->sv64.ok) {  // one pass: per-block partial rows
+        const long long nvec = c->oreal ? c->Pld / 2 : P;
+        const long long nb = c->sv64.blocks;
+        HIPCHK(c, c->dsvpart.ensure((size_t)nb * nvec * 16), "alloc S*v partials");
+        HIPCHK(c, c->dsvwsum.ensure((size_t)nb * 16), "alloc S*v partials");
+        HIPCHK(c, c->dsvscratch.ensure((size_t)(nb / 32 + 2) * (P + 1) * 16), "alloc S*v scratch");
+    } else if (c->prec == NQES_PREC_FP64) {
         c->svchunks = choose_chunks(S, P, 256);
         HIPCHK(c, c->dsvpart.ensure((size_t)c->svchunks * P * 16), "alloc S*v partials");
     } else if (!c->svf) {
@@ -751,7 +782,8 @@ int nqes_get_O(nqes_ctx *c, int64_t row0, int64_t nrows, double *out)
     if (!out || row0 < 0 || nrows < 0 || row0 + nrows > c->S) return c->fail(NQES_EINVAL, "row range out of bounds");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     if (c->svf) return get_O_factored(c, row0, nrows, out);
-    const size_t e = o_elem(c->prec, c->cplx);
+    const bool fp64 = c->prec == NQES_PREC_FP64;
+    const size_t e = o_elem(c->prec, fp64 ? !c->oreal : c->cplx);
     std::vector<unsigned char> buf((size_t)nrows * c->Pld * e);
     HIPCHK(c,
            hipMemcpy(buf.data(), static_cast<char *>(c->dO.p) + (size_t)row0 * c->Pld * e, buf.size(),
@@ -763,6 +795,9 @@ int nqes_get_O(nqes_ctx *c, int64_t row0, int64_t nrows, double *out)
             double *dst = out + 2 * ((size_t)r * c->P + q);
             if (e == 16) {
                 memcpy(dst, src, 16);
+            } else if (fp64) {  // real weights: Re O stored
+                memcpy(dst, src, 8);
+                dst[1] = 0.0;
             } else if (e == 8) {
                 float v[2];
                 memcpy(v, src, 8);
@@ -775,6 +810,22 @@ int nqes_get_O(nqes_ctx *c, int64_t row0, int64_t nrows, double *out)
                 dst[1] = 0.0;
             }
         }
+    return NQES_OK;
+}
+
+int nqes_get_ulp_ties(nqes_ctx *c, uint64_t *per_chain, uint64_t *total)
+{
+    if (!c) return NQES_EINVAL;
+    if (!c->sampled) return c->fail(NQES_ESTATE, "no samples: call nqes_sample first");
+    HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
+    std::vector<uint64_t> t((size_t)c->C);
+    HIPCHK(c, hipMemcpy(t.data(), c->dties.p, (size_t)c->C * 8, hipMemcpyDeviceToHost), "copy tie counts");
+    if (per_chain) memcpy(per_chain, t.data(), (size_t)c->C * 8);
+    if (total) {
+        uint64_t s = 0;
+        for (auto v : t) s += v;
+        *total = s;
+    }
     return NQES_OK;
 }
 
@@ -885,7 +936,29 @@ static int sv_mixed_device(nqes_ctx *c, const void *v, const void *ov, size_t ds
     return NQES_OK;
 }
 
-// real O and o (mixed path, real weights): part 0 / 1 of v (real / imaginary) into hv,
+// device part of the one-pass FP64 S*v: raw sums over the stored O rows (svraw), v
+// given as D = double (real O) or cd; v and o^T v go to the device in one copy
+static int sv_fp64_device(nqes_ctx *c, const void *v, const void *ov, size_t dsize)
+{
+    const long long P = c->P;
+    char *dv = static_cast<char *>(c->dv.p);
+    HIPCHK(c, hipMemcpyAsync(dv, v, dsize * P, hipMemcpyHostToDevice, c->stream), "copy v");
+    HIPCHK(c, hipMemcpyAsync(dv + dsize * P, ov, dsize, hipMemcpyHostToDevice, c->stream), "copy ov");
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
+    const long long nvec = c->oreal ? c->Pld / 2 : P;
+    HIPCHK(c, nq_launch_sv_rows64(!c->oreal, c->sv64, c->dO.p, dv, dv + dsize * P, c->S, nvec, P, c->dsvpart.p,
+                                  c->dsvwsum.p, c->dsvscratch.p, c->dsvout.p, c->stream),
+           "S*v kernels");
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
+    HIPCHK(c, hipMemcpyAsync(c->svraw.data(), c->dsvout.p, dsize * (P + 1), hipMemcpyDeviceToHost, c->stream),
+           "copy S*v");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
+    c->sv_ms += event_ms(c->ev[2], c->ev[3]);
+    c->sv_calls++;
+    return NQES_OK;
+}
+
+// real O and o (real weights, either path): part 0 / 1 of v (real / imaginary) into hv,
 // returns o^T hv
 static double mixed_real_prepare(nqes_ctx *c, const cd *v, int part)
 {
@@ -934,6 +1007,28 @@ static int apply_S(nqes_ctx *c, const cd *v, cd *out)
         HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
         c->sv_ms += event_ms(c->ev[2], c->ev[3]);
         c->sv_calls++;
+        return NQES_OK;
+    }
+    if (c->prec == NQES_PREC_FP64 && c->sv64.ok) {
+        if (!c->oreal) {  // complex O: (1/S) [sum_s conj(O_s) w_s - conj(o) sum_s w_s] + skk .* v
+            int st = sv_fp64_device(c, v, &ov, 16);
+            if (st != NQES_OK) return st;
+            const cd wsum = c->svraw[P];
+            for (long long q = 0; q < P; q++) {
+                cd r = divr(sub(c->svraw[q], mul(conj(c->o[q]), wsum)), Sd);
+                out[q] = add(r, mul(c->skk[q], v[q]));
+            }
+            return NQES_OK;
+        }
+        bool has_im = false;  // real O: the real and imaginary parts of v separately
+        for (long long q = 0; q < P; q++)
+            if (v[q].im != 0.0) has_im = true;
+        for (int part = 0; part < (has_im ? 2 : 1); part++) {
+            double ovp = mixed_real_prepare(c, v, part);
+            int st = sv_fp64_device(c, c->hv.data(), &ovp, 8);
+            if (st != NQES_OK) return st;
+            mixed_real_result(c, out, part);
+        }
         return NQES_OK;
     }
     if (c->prec == NQES_PREC_FP64) {
@@ -1134,10 +1229,13 @@ int nqes_sr_step(nqes_ctx *c, double *W, uint64_t seed, int C, int m, int B, dou
     cd *Wc = reinterpret_cast<cd *>(W);
     for (long long q = 0; q < c->P; q++) Wc[q] = nq64::sub(Wc[q], nq64::scale(lr, x[q]));
     if (stats) {
-        std::vector<unsigned long long> acc(C);
+        std::vector<unsigned long long> acc(C), ties(C);
         HIPCHK(c, hipMemcpy(acc.data(), c->dacc.p, (size_t)C * 8, hipMemcpyDeviceToHost), "copy accepts");
-        uint64_t a = 0;
+        HIPCHK(c, hipMemcpy(ties.data(), c->dties.p, (size_t)C * 8, hipMemcpyDeviceToHost), "copy tie counts");
+        uint64_t a = 0, nt = 0;
         for (auto v : acc) a += v;
+        for (auto v : ties) nt += v;
+        stats->ulp_ties = nt;
         stats->minres_flag = r.flag;
         stats->minres_iters = r.iters;
         stats->accepts = a;

@@ -44,6 +44,7 @@ class Stats(ctypes.Structure):
         ("t_sv_ms", ctypes.c_double),
         ("t_total_ms", ctypes.c_double),
         ("nonfinite", ctypes.c_int),
+        ("ulp_ties", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -96,7 +97,7 @@ EXPORTS = [
     "nqes_create", "nqes_destroy", "nqes_last_error", "nqes_problem_init", "nqes_sr_step",
     "nqes_sample", "nqes_get_samples", "nqes_get_O", "nqes_get_moments", "nqes_apply_S",
     "nqes_solve", "nqes_correlations", "nqes_required_bytes", "nqes_version",
-    "nqes_set_dense_S", "nqes_enumerate",
+    "nqes_set_dense_S", "nqes_enumerate", "nqes_get_ulp_ties",
     "nqes_server_create", "nqes_server_destroy", "nqes_server_last_error", "nqes_server_submit",
     "nqes_server_poll", "nqes_server_wait",
 ]
@@ -127,6 +128,8 @@ def load_library(path=LIB_PATH):
     L.nqes_get_samples.argtypes = [_vp, _vp, _vp, _vp]
     L.nqes_get_O.restype = c_int
     L.nqes_get_O.argtypes = [_vp, c_i64, c_i64, _cp]
+    L.nqes_get_ulp_ties.restype = c_int
+    L.nqes_get_ulp_ties.argtypes = [_vp, _vp, ctypes.POINTER(c_u64)]
     L.nqes_get_moments.restype = c_int
     L.nqes_get_moments.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.nqes_apply_S.restype = c_int
@@ -258,6 +261,13 @@ class Solver:
         acc = np.zeros(self.C, dtype=np.uint64)
         self._check(self.lib.nqes_get_samples(self.ctx, _ptr(spins), _ptr(H), _ptr(acc)))
         return spins, H, acc
+
+    def ulp_ties(self):
+        """(per-chain, total) near-tie accept decisions of the last sample set (FP64)."""
+        per = np.zeros(self.C, dtype=np.uint64)
+        tot = ctypes.c_uint64(0)
+        self._check(self.lib.nqes_get_ulp_ties(self.ctx, _ptr(per), ctypes.byref(tot)))
+        return per, tot.value
 
     def O_rows(self, row0, nrows):
         out = np.zeros((nrows, self.P), dtype=np.complex128)

@@ -826,6 +826,28 @@ void orc_run_chain_model(int model, int N, int M, int K, double h, const double 
     chain_free(&ch);
 }
 
+/*
+ * Chains [0, C) of one SR step in parallel (OpenMP over chains), each as
+ * orc_run_chain_model: spins (C*m*K*NW words), H (C*m*K*K complex), optionally O
+ * (C*m*P complex, NULL: not formed) and per-chain accept counts (C, may be NULL).
+ */
+void orc_sample_chains_model(int model, int N, int M, int K, double h, const double *J, const double *Wd,
+                             uint64_t seed, int C, int m, int B, uint32_t *spins_out, double *H_out,
+                             double *O_out, long *accepts)
+{
+    const int NW = (N + 31) / 32;
+    const long P = (long)K * (N + M + (long)N * M);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int c = 0; c < C; c++) {
+        long acc = 0;
+        orc_run_chain_model(model, N, M, K, h, J, Wd, seed, c, m, B,
+                            spins_out ? spins_out + (long)c * m * K * NW : NULL,
+                            H_out ? H_out + 2L * c * m * K * K : NULL,
+                            O_out ? O_out + 2L * c * m * P : NULL, NULL, NULL, NULL, &acc);
+        if (accepts) accepts[c] = acc;
+    }
+}
+
 void orc_run_chain(int N, int M, int K, double h, const double *J, const double *Wd,
                    uint64_t seed, int c, int m, int B,
                    uint32_t *spins_out, double *H_out, double *O_out,

@@ -67,6 +67,8 @@ def lib():
         L.orc_run_chain.argtypes = [c_int, c_int, c_int, c_dbl, _dp, _cp, c_u64, c_int, c_int, c_int,
                                     vp, vp, vp, vp, vp, vp, ctypes.POINTER(c_long)]
         L.orc_run_chain_model.argtypes = [c_int] + L.orc_run_chain.argtypes
+        L.orc_sample_chains_model.argtypes = [c_int, c_int, c_int, c_int, c_dbl, _dp, _cp, c_u64, c_int, c_int,
+                                              c_int, vp, vp, vp, vp]
         L.orc_minresqlp_dense.restype = c_int
         L.orc_minresqlp_dense.argtypes = [c_long, _cp, _cp, _cp, c_dbl, c_u64, ctypes.POINTER(c_u64)]
         L.orc_sr_step.restype = c_int
@@ -270,6 +272,25 @@ def run_chain(N, M, K, h, J, W, seed, c, m, B, want_O=False, want_state=False, m
         out["O"] = O
     if want_state:
         out.update(phi=phi, phi_inv=phi_inv, tanh=tanh)
+    return out
+
+
+def sample_chains(N, M, K, h, J, W, seed, C, m, B, want_O=False, model=MODEL_TRAPPEDION):
+    """Chains 0..C-1 of an SR step in parallel: dict of spins (S, K, NW), H (S, K, K),
+    accepts (C,) and, with want_O, O (S, P); sample s = c * m + t."""
+    NW = nwords(N)
+    P = nparams(N, M, K)
+    S = C * m
+    spins = np.zeros((S, K, NW), dtype=np.uint32)
+    H = np.zeros((S, K, K), dtype=np.complex128)
+    O = np.zeros((S, P), dtype=np.complex128) if want_O else None
+    acc = np.zeros(C, dtype=np.int64)
+    lib().orc_sample_chains_model(int(model), N, M, K, float(h), np.ascontiguousarray(J, dtype=np.float64).reshape(-1),
+                                  np.ascontiguousarray(W, dtype=np.complex128), int(seed) % (1 << 64), C, m, B,
+                                  _ptr(spins), _ptr(H), _ptr(O), _ptr(acc))
+    out = dict(spins=spins, H=H, accepts=acc.astype(np.uint64))
+    if want_O:
+        out["O"] = O
     return out
 
 

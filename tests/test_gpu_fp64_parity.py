@@ -59,6 +59,79 @@ def test_sampler_bitexact(oracle, nqes, N, M, K, C, m, B):
     assert rel_err(O_g, O_o) == 0.0, rel_err(O_g, O_o)
 
 
+# real weights: O is real, stored as doubles (rows padded to an even length; P = 15
+# and P = 2 (3 + 3 + 9) = 30 are odd / not a multiple of four)
+REAL_CASES = [
+    (8, 16, 2, 6, 4, 3),
+    (3, 3, 1, 5, 4, 2),
+    (3, 3, 2, 4, 3, 2),
+    (6, 12, 3, 4, 3, 2),
+]
+
+
+@pytest.mark.parametrize("N,M,K,C,m,B", REAL_CASES)
+def test_sampler_bitexact_real(oracle, nqes, N, M, K, C, m, B):
+    """Real weights on the FP64 path: accept sequence, spins, H_loc and the O rows
+    (stored as their real parts) bit-identical with the oracle; the near-tie count of
+    the accept decisions is reported per chain and is zero."""
+    print(f"case real N={N} M={M} K={K} C={C} m={m} B={B}", flush=True)
+    h = 0.595
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=N * 100 + M + K, scale=0.15, complex_=False)
+    seed = 1234567
+    sp_o, H_o, O_o, acc_o = oracle_samples(oracle, N, M, K, h, J, W, seed, C, m, B)
+    assert not np.any(O_o.imag) and not np.any(H_o.imag)
+    with nqes.Solver(N, M, K, h, J) as s:
+        s.sample(W, seed, C, m, B, reglam=0.5)
+        sp_g, H_g, acc_g = s.samples()
+        O_g = s.O_rows(0, C * m)
+        ties, total = s.ulp_ties()
+    assert np.array_equal(acc_g, acc_o), (acc_g, acc_o)
+    assert np.array_equal(sp_g, sp_o)
+    assert rel_err(H_g, H_o) == 0.0, rel_err(H_g, H_o)
+    assert np.array_equal(O_g, O_o)
+    assert ties.shape == (C,) and total == 0
+
+
+# the shapes of the benchmark workloads, at chain counts the oracle finishes in seconds:
+# N=32 M=64 K=2 (paper size; one-pass S*v with 4 and 8 waves per row) and N=64 M=128
+# K=4 (large; rows too long for the one-pass kernel: two-pass form)
+@pytest.mark.parametrize("N,M,K,C,m,B,cplx", [(32, 64, 2, 64, 16, 64, False), (32, 64, 2, 64, 16, 64, True),
+                                              (64, 128, 4, 4, 2, 4, True)])
+def test_baseline_shapes_bitexact(oracle, nqes, N, M, K, C, m, B, cplx):
+    """Spins, accepts, H_loc and O bit-exact with the oracle; moments to 1e-12 and S*v
+    (real and complex v) to 1e-12 of the oracle's product over its own O rows."""
+    print(f"case N={N} M={M} K={K} C={C} m={m} B={B} cplx={cplx}", flush=True)
+    h, reglam = 0.595, 1.0
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=31 * N + K, scale=0.1 if cplx else 0.05, complex_=cplx)
+    seed = 20261203
+    ref = oracle.sample_chains(N, M, K, h, J, W, seed, C, m, B, want_O=True)
+    O_o = ref["O"]
+    S = C * m
+    o = O_o.mean(axis=0)
+    sq = (np.abs(O_o) ** 2).mean(axis=0) - np.abs(o) ** 2
+    with nqes.Solver(N, M, K, h, J) as s:
+        s.sample(W, seed, C, m, B, reglam)
+        sp_g, H_g, acc_g = s.samples()
+        O_g = s.O_rows(0, S)
+        mom = s.moments()
+        rng = np.random.default_rng(9)
+        vr = rng.normal(size=s.P)
+        vc = vr + 1j * rng.normal(size=s.P)
+        sv = [s.apply_S(v) for v in (vr.astype(np.complex128), vc)]
+        _, total = s.ulp_ties()
+    assert np.array_equal(acc_g, ref["accepts"])
+    assert np.array_equal(sp_g, ref["spins"])
+    assert rel_err(H_g, ref["H"]) == 0.0
+    assert rel_err(O_g, O_o) == 0.0
+    assert rel_err(mom["o"], o) < 1e-12
+    assert rel_err(mom["skk"].real, reglam * sq) < 1e-11
+    assert total == 0
+    for v, out in zip((vr.astype(np.complex128), vc), sv):
+        assert rel_err(out, oracle.sv_from_O(O_o, mom["o"], mom["skk"], v)) < 1e-12
+
+
 @pytest.mark.parametrize("model", [1, 2])
 @pytest.mark.parametrize("N,M,K,C,m,B", [(8, 16, 2, 4, 3, 2), (6, 12, 3, 3, 2, 1), (5, 70, 1, 3, 2, 1),
                                         (40, 80, 2, 2, 1, 1)])
@@ -80,12 +153,13 @@ def test_sampler_bitexact_pairflip(oracle, nqes, model, N, M, K, C, m, B):
     assert rel_err(O_g, O_o) == 0.0
 
 
+@pytest.mark.parametrize("cplx", [True, False])
 @pytest.mark.parametrize("N,M,K,C,m,B", CASES[:4])
-def test_moments_sv_step(oracle, nqes, N, M, K, C, m, B):
-    print(f"case N={N} M={M} K={K} C={C} m={m} B={B}", flush=True)
+def test_moments_sv_step(oracle, nqes, N, M, K, C, m, B, cplx):
+    print(f"case N={N} M={M} K={K} C={C} m={m} B={B} cplx={cplx}", flush=True)
     h, reglam, lr, rtol = 0.595, 0.7, 0.1, 1e-10
     J = load_J(N)
-    W = random_weights(N, M, K, seed=7 + N + M + K, scale=0.15)
+    W = random_weights(N, M, K, seed=7 + N + M + K, scale=0.15, complex_=cplx)
     seed = 99
     Wo, emat_o, info = oracle.sr_step(N, M, K, h, J, W, seed, C, m, B, lr, reglam, rtol, want_details=True)
     _, _, O_o, _ = oracle_samples(oracle, N, M, K, h, J, W, seed, C, m, B)
