@@ -83,6 +83,7 @@ typedef struct {
 /* Creates a context on HIP device `device`.  Returns NULL on failure. */
 nqes_ctx *nqes_create(int device);
 void nqes_destroy(nqes_ctx *ctx);
+/* ctx == NULL: the calling thread's last nqes_apply_S_rows_noalloc error. */
 const char *nqes_last_error(const nqes_ctx *ctx);
 
 /*
@@ -128,6 +129,26 @@ int nqes_get_moments(nqes_ctx *ctx, double *o, double *skk, double *f, double *e
  * such count; 0 on the mixed path. */
 int nqes_get_ulp_ties(nqes_ctx *ctx, uint64_t *per_chain, uint64_t *total);
 int nqes_apply_S(nqes_ctx *ctx, const double *v, double *out);
+
+/*
+ * No counterpart in the reference: the variant of nqes_apply_S for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The FP64 S*v of the matrix-free operator (rbm.cc:661-684)
+ *   out = (1/S) sum_s conj(O_s) (O_s^T v) + skk .* v - (o^T v) conj(o)
+ * over O rows the caller holds in device memory, by the kernels nqes_apply_S runs on
+ * the FP64 path.  O: S rows, row s at O + s*ldo entries; cplx = 0: real O (doubles,
+ * ldo even), cplx = 1: complex O (interleaved pairs); ldo >= P, entries past P finite.
+ * o, skk, v, out: P entries of the same kind, device memory of the current device.
+ * Enqueued on `stream` (a hipStream_t; NULL: the default stream) without waiting.
+ * Allocates nothing: its partial rows and sums live in one static device array of
+ * 24 MiB per device (each device holds it until the process ends); shapes whose rows
+ * exceed the one-pass kernel (v in LDS, at most 16 x 64 x 9 16-byte vectors) or whose
+ * scratch would exceed the array return NQES_EINVAL, with the reason in
+ * nqes_last_error(NULL).  Calls that overlap on one device share the array and race.
+ */
+int nqes_apply_S_rows_noalloc(const double *O, int64_t S, int64_t P, int64_t ldo, int cplx,
+                              const double *o, const double *skk, const double *v, double *out,
+                              void *stream);
 int nqes_solve(nqes_ctx *ctx, double rtol, uint64_t maxit, double *x, int *flag,
                uint64_t *iters);
 

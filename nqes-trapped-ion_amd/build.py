@@ -44,6 +44,7 @@ SOURCES = [
     ("sv_q8.hip", []),
     ("sv_q8_gemm.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("nqes_api.cpp", EXACT),
+    ("noalloc.hip", EXACT),
     ("serve.cpp", EXACT),
     ("eig.cpp", EXACT),
 ]

@@ -63,6 +63,10 @@ Sv64Shape nq_sv64_shape(long long nvec, long long S, size_t lds_max);
 hipError_t nq_launch_sv_rows64(bool cplx, const Sv64Shape &sh, const void *O, const void *v, const void *ov,
                                long long S, long long nvec, long long P, void *part, void *wsum, void *scratch,
                                void *out, hipStream_t st);
+// the static scratch of nqes_apply_S_rows_noalloc (noalloc.hip): resolves its device
+// address on `device` (current) once; ptr / lds_max may be NULL
+int nq_noalloc_setup(int device, void **ptr, int *lds_max);
+const char *nq_noalloc_last_error();
 // out = o^T v (one D), a fixed summation order
 hipError_t nq_launch_dot64(bool cplx, const void *o, const void *v, long long P, void *out, hipStream_t st);
 // out[p] = (raw[p] - conj(o_p) raw[P]) / S + skk_p v_p

@@ -198,6 +198,7 @@ nqes_ctx *nqes_create(int device)
         }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->lds_max = prop.sharedMemPerBlock;
+    (void)nq_noalloc_setup(device, nullptr, nullptr);  // the no-allocation entry point's scratch address
     return c;
 }
 
@@ -216,7 +217,7 @@ void nqes_destroy(nqes_ctx *c)
     delete c;
 }
 
-const char *nqes_last_error(const nqes_ctx *c) { return c ? c->err.c_str() : "null context"; }
+const char *nqes_last_error(const nqes_ctx *c) { return c ? c->err.c_str() : nq_noalloc_last_error(); }
 
 int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
 {

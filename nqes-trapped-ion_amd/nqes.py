@@ -97,7 +97,7 @@ EXPORTS = [
     "nqes_create", "nqes_destroy", "nqes_last_error", "nqes_problem_init", "nqes_sr_step",
     "nqes_sample", "nqes_get_samples", "nqes_get_O", "nqes_get_moments", "nqes_apply_S",
     "nqes_solve", "nqes_correlations", "nqes_required_bytes", "nqes_version",
-    "nqes_set_dense_S", "nqes_enumerate", "nqes_get_ulp_ties",
+    "nqes_set_dense_S", "nqes_enumerate", "nqes_get_ulp_ties", "nqes_apply_S_rows_noalloc",
     "nqes_server_create", "nqes_server_destroy", "nqes_server_last_error", "nqes_server_submit",
     "nqes_server_poll", "nqes_server_wait",
 ]
@@ -128,6 +128,8 @@ def load_library(path=LIB_PATH):
     L.nqes_get_samples.argtypes = [_vp, _vp, _vp, _vp]
     L.nqes_get_O.restype = c_int
     L.nqes_get_O.argtypes = [_vp, c_i64, c_i64, _cp]
+    L.nqes_apply_S_rows_noalloc.restype = c_int
+    L.nqes_apply_S_rows_noalloc.argtypes = [_vp, c_i64, c_i64, c_i64, c_int, _vp, _vp, _vp, _vp, _vp]
     L.nqes_get_ulp_ties.restype = c_int
     L.nqes_get_ulp_ties.argtypes = [_vp, _vp, ctypes.POINTER(c_u64)]
     L.nqes_get_moments.restype = c_int
