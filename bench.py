@@ -1,30 +1,46 @@
-"""Benchmark: SR steps of the multi-RBM VMC solver on one MI355X.
+"""Benchmark of the multi-RBM VMC solver on one MI355X: SR-step throughput, whole
+converged solves with their correctness gates, and the CPU baseline.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision mixed|fp64]
-                    [--N 32 --M 64 --K 2 --C 4096 --m 256 --B 64] [--no-cpu]
+                    [--N 32 --M 64 --K 2 --C 4096 --m 256 --B 64] [--no-cpu] [--no-gates]
 
-Workload (default): trapped-ion chain N=32 spins, M=64 hidden units, K=2 states,
-h=0.595, J = power law of the N=32 ion chain, 4096 chains x 256 samples (S=2^20)
-with 64 burn-in sweeps, real weights N(0, 0.01^2), lr 0.1, reglam 40, MINRES-QLP
-rtol 1e-9.  A "step" is one full SR step: sampling + H_loc + O, moments, the
-MINRES-QLP solve with its matrix-free S*v passes, and the weight update.
+SR-step line (value): trapped-ion chain N=32 spins, M=64 hidden units, K=2 states,
+h=0.595, J of the N=32 ion chain (the reference's Ising2d_N32_alpha1.0), 4096 chains x
+256 samples (S=2^20) with 64 burn-in sweeps, real weights N(0, 0.01^2), lr 0.1,
+reglam 40, MINRES-QLP rtol 1e-9.  A step is one full SR step: sampling + H_loc + O,
+moments, the MINRES-QLP solve with its matrix-free S*v, and the weight update.
+value = MC samples per second over whole SR steps.
 
-Prints one JSON line.  value = MC samples per second over whole SR steps.
+Converged solves ("converged" in the line): whole solves by the reference's run()
+rules (rbm.cc:1173-1197: chain seeds seed + t*C, lrate 0.1 x 0.99^t >= 0.01, reglam
+1 x 0.9^t >= 0.2, the energies = ascending real eigenvalues of e_mat / N, stop when
+|E_t - E_best| < tol), MINRES-QLP rtol 1e-9 with at most 5000 iterations, weights
+initialised as the reference's init_rbm (rbm.cc:242-260: complex, real and imaginary
+parts N(0,1)/sqrt(N), /sqrt(M), /sqrt(NM), drawn here by numpy from a fixed seed):
+  N8:  N=8  M=16 K=2 h=0.595, 128 chains x 200 samples, 200 SR steps (tol 1e-6), FP64 and
+       mixed; gate: mean E/N of the last 20 steps within 1e-5 of exact diagonalisation;
+       mixed within 1e-4 relative of FP64 at the same step;
+  N16: N=16 M=32 K=2 h=0.595, 1024 chains x 256 samples (a member of the batched field
+       scan), tol 1e-4, FP64 and mixed to convergence: solves/hour of each path, the
+       mixed/FP64 ratio, and the gates at the mixed solve's final step;
+  N32: the paper size (N=32 M=64 K=2, 4096 chains x 256 samples), tol 1e-4: the mixed
+       solve to convergence (solves/hour) and the FP64 path's first two SR steps on the
+       same seeds (per-step time ratio, energies at those steps).
+The published schedule (reglam 1 / 0.9 / 0.2, tol 1e-4) is that of the reference's
+N=32 runs (data/energy/ion_N32_M64_K2_h1_c128_m200:1).  Real N(0, 0.01^2) weights
+with reglam 40 -> 1 diverge at N=8 (non-finite at SR step 43, on the CPU oracle too)
+and lose the excited state at N=32, and a real RBM is positive, so cannot represent
+the excited state: the solves use the reference's complex initialisation.
 
-    python bench.py --workload scan     [--slots 8] [--solve-steps 5]
-    python bench.py --workload serving  [--slots 8] [--solve-steps 5] [--requests 48]
-
-run whole solves through the solve server (continuous batching on one device):
-  scan: 64 solves submitted at once, h ~ U[0.1, 1.5], N in {16, 24, 32, 48}, M = 2N,
-        K in {2, 3, 4}, J_ij = |i-j|^-1 normalised to max 1, 1024 chains x 256 samples;
-  serving: Poisson arrivals (4 requests/s), N in {16, 18, ..., 64}, M in {N, 2N},
-        K in {1..4}, h ~ U[0.2, 1.2], J exponent ~ U[0.5, 1.5], 1024 chains x 256 samples.
-Both use the mixed path and cap every solve at --solve-steps SR steps (the convergence
-test still applies).  value = solves per hour; p50/p99 = arrival -> result latency.
+    python bench.py --workload solve --precision fp64 --N 32 --M 64 ...
+        one whole solve of the chosen shape and path (same rules), printed as JSON;
+    python bench.py --workload scan|serving [--slots 8] [--solve-steps 200] [--requests 300]
+        whole solves through the solve server (continuous batching on one device).
 """
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -36,22 +52,204 @@ sys.path.insert(0, os.path.join(REPO, "nqes-trapped-ion_amd"))
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix, published
 INT8_MFMA_PEAK_TOPS = 5000.0  # MI355X dense int8 matrix, published
+MAXIT_SOLVE = 5000
+SCHED = dict(lrate0=0.1, lratef=0.99, lrate_cutoff=0.01, reglam0=1.0, reglam_decay=0.9, reglam_cutoff=0.2)
 
 
 def load_J(N):
     path = os.path.join(REPO, "tests", "golden", f"J_N{N}.npy")
     if os.path.exists(path):
         return np.load(path)
+    return power_law_J(N, 1.0)
+
+
+def power_law_J(N, alpha):
     i = np.arange(N)
     d = np.abs(i[:, None] - i[None, :]).astype(float)
     J = np.zeros((N, N))
-    J[d > 0] = d[d > 0] ** -1.0
+    J[d > 0] = d[d > 0] ** -alpha
     return J / J.max()
 
 
 def initial_weights(N, M, K, seed):
+    """Real N(0, 0.01^2) weights (the SR-step workload)."""
     rng = np.random.default_rng(seed)
     return (rng.normal(size=K * (N + M + N * M)) * 0.01).astype(np.complex128)
+
+
+def reference_init(N, M, K, seed):
+    """init_multi_rbm's distribution (rbm.cc:242-260): per RBM, a_i, b_j, w_ij complex
+    with real and imaginary parts N(0,1), divided by sqrt(N), sqrt(M), sqrt(N M)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(K):
+        for n, d in ((N, N), (M, M), (N * M, N * M)):
+            out.append((rng.normal(size=n) + 1j * rng.normal(size=n)) / np.sqrt(d))
+    return np.concatenate(out).astype(np.complex128)
+
+
+def ed_energies(N, h, J, K):
+    """K lowest E/N of H = sum_{i != j} J_ij Z_i Z_j - h sum_i X_i (2^N states)."""
+    dim = 1 << N
+    s = np.arange(dim)
+    sig = 1.0 - 2.0 * ((s[:, None] >> np.arange(N)[None, :]) & 1)
+    diag = np.einsum("si,ij,sj->s", sig, J, sig)
+    if N <= 10:
+        H = np.diag(diag)
+        for i in range(N):
+            H[s, s ^ (1 << i)] -= h
+        return np.linalg.eigvalsh(H)[:K] / N
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    rows = np.concatenate([s] + [s] * N)
+    cols = np.concatenate([s] + [s ^ (1 << i) for i in range(N)])
+    vals = np.concatenate([diag] + [np.full(dim, -h)] * N)
+    H = sp.csr_matrix((vals, (rows, cols)), shape=(dim, dim))
+    ev = sla.eigsh(H, k=K, which="SA", tol=1e-12, v0=np.ones(dim))[0]
+    return np.sort(ev) / N
+
+
+def solve(nqes, N, M, K, h, J, W0, prec, C, m, B, tol, niters, seed, rtol=1e-9, maxit=MAXIT_SOLVE, min_steps=0,
+          log=None):
+    """One whole solve by the rules of run() (rbm.cc:1173-1197).  The solve ends at the
+    convergence test or after niters steps; with min_steps it continues (untimed) to
+    that many steps so that another run can be compared at an equal step count."""
+    W = W0.copy()
+    lr, reglam = SCHED["lrate0"], SCHED["reglam0"]
+    best, hist, step_ms, iters = None, [], [], []
+    steps, converged, t_solve, error = None, False, None, None
+    with nqes.Solver(N, M, K, h, J, precision=prec) as s:
+        t0 = time.perf_counter()
+        for t in range(max(niters, min_steps)):
+            try:
+                emat, st = s.monte_carlo(W, seed + t * C, C, m, B, max(lr, SCHED["lrate_cutoff"]),
+                                         max(reglam, SCHED["reglam_cutoff"]), rtol, maxit=maxit, stats=True)
+            except nqes.NqesError as e:  # non-finite energy or gradient: the solve failed
+                if steps is None:
+                    steps, t_solve = t, time.perf_counter() - t0
+                error = str(e)
+                break
+            lr *= SCHED["lratef"]
+            reglam *= SCHED["reglam_decay"]
+            ev = np.sort(np.linalg.eigvals(emat / N).real)
+            hist.append(ev)
+            step_ms.append(st["t_total_ms"])
+            iters.append(st["minres_iters"])
+            if log:
+                log(t, ev, st)
+            if steps is None:
+                if t and np.linalg.norm(ev - best) < tol:
+                    steps, converged, t_solve = t + 1, True, time.perf_counter() - t0
+                elif not t or ev.sum() < best.sum():
+                    best = ev
+                if steps is None and t + 1 == niters:
+                    steps, t_solve = t + 1, time.perf_counter() - t0
+            if steps is not None and t + 1 >= min_steps:
+                break
+    return dict(hist=np.array(hist), steps=steps, converged=converged, wall_s=t_solve, error=error,
+                solves_per_hour=3600.0 / t_solve if steps else 0.0, ms_per_step=step_ms, minres_iters=iters, W=W)
+
+
+def summary(r, ed=None, tail=20):
+    """The solve's final energies (last step, and mean +- standard error of the last
+    `tail` steps), its cost, and its distance to exact diagonalisation."""
+    if not r["steps"]:
+        return {"steps": 0, "error": r["error"]}
+    hist = r["hist"][: r["steps"]]
+    last = hist[-min(tail, len(hist)):]
+    out = {"steps": r["steps"], "converged": r["converged"], "wall_s": r["wall_s"],
+           "solves_per_hour": r["solves_per_hour"], "energies_final": hist[-1].tolist(),
+           f"energies_mean_last{len(last)}": last.mean(axis=0).tolist(),
+           f"energies_sem_last{len(last)}": (last.std(axis=0, ddof=1) / np.sqrt(len(last))).tolist()
+           if len(last) > 1 else None,
+           "minres_iters_max": int(max(r["minres_iters"][: r["steps"]])), "error": r["error"],
+           "ms_per_step_mean": float(np.mean(r["ms_per_step"][: r["steps"]]))}
+    if ed is not None:
+        out["abs_err_vs_ed_final"] = np.abs(hist[-1] - ed).tolist()
+        out[f"abs_err_vs_ed_mean_last{len(last)}"] = np.abs(last.mean(axis=0) - ed).tolist()
+    return out
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def converged_block(args, nqes):
+    """The converged-solve workloads and their gates."""
+    out = {"rules": "run() of rbm.cc:1173-1197; lrate 0.1/0.99/0.01, reglam 1/0.9/0.2, rtol 1e-9, "
+                    f"MINRES-QLP maxit {MAXIT_SOLVE}; weights: the reference's complex init_rbm distribution "
+                    "(rbm.cc:242-260), numpy seed 20261203; chain seeds seed + t*C, seed 20261203",
+           "gate_ed_abs": 1e-5, "gate_mixed_vs_fp64_rel": 1e-4}
+    FP, MX = nqes.PREC_FP64, nqes.PREC_MIXED
+    seed, h = 20261203, 0.595
+
+    # N = 8: exact diagonalisation gate, 200 SR steps
+    N, M, K = 8, 16, 2
+    J = load_J(N)
+    ed = ed_energies(N, h, J, K)
+    W0 = reference_init(N, M, K, seed)
+    res = {}
+    for prec in (FP, MX):
+        res[prec] = solve(nqes, N, M, K, h, J, W0, prec, 128, 200, 64, 1e-6, 200, seed, min_steps=200)
+        progress(f"N=8 {'fp64' if prec == FP else 'mixed'} solve: {res[prec]['steps']} steps, "
+                 f"{res[prec]['wall_s']:.1f}s, E/N {res[prec]['hist'][-1]}")
+    blk = {"workload": "N=8 M=16 K=2 h=0.595 C=128 m=200 B=64, 200 SR steps (tol 1e-6)", "ed": ed.tolist(),
+           "fp64": summary(res[FP], ed), "mixed": summary(res[MX], ed)}
+    tm = res[MX]["steps"]
+    rel = np.abs(res[MX]["hist"][tm - 1] - res[FP]["hist"][tm - 1]) / np.abs(res[FP]["hist"][tm - 1])
+    blk["mixed_vs_fp64_rel_at_step"] = {"step": tm, "rel": rel.tolist()}
+    blk["pass_ed"] = {p: bool(np.all(np.abs(res[q]["hist"][-20:].mean(axis=0) - ed) < 1e-5))
+                      for p, q in (("fp64", FP), ("mixed", MX))}
+    blk["pass_mixed_vs_fp64"] = bool(np.all(rel < 1e-4))
+    out["N8"] = blk
+
+    # N = 16: converged solves of both paths
+    N, M, K = 16, 32, 2
+    J = load_J(N)
+    ed = ed_energies(N, h, J, K)
+    W0 = reference_init(N, M, K, seed)
+    rm = solve(nqes, N, M, K, h, J, W0, MX, 1024, 256, 64, 1e-4, 200, seed)
+    progress(f"N=16 mixed solve: {rm['steps']} steps, {rm['wall_s']:.1f}s")
+    rf = solve(nqes, N, M, K, h, J, W0, FP, 1024, 256, 64, 1e-4, 200, seed, min_steps=rm["steps"])
+    progress(f"N=16 fp64 solve: {rf['steps']} steps, {rf['wall_s']:.1f}s")
+    tm = rm["steps"]
+    rel = np.abs(rm["hist"][tm - 1] - rf["hist"][tm - 1]) / np.abs(rf["hist"][tm - 1])
+    blk = {"workload": "N=16 M=32 K=2 h=0.595 C=1024 m=256 B=64, tol 1e-4, at most 200 SR steps", "ed": ed.tolist(),
+           "fp64": summary(rf, ed), "mixed": summary(rm, ed),
+           "mixed_vs_fp64_rel_at_step": {"step": tm, "rel": rel.tolist()},
+           "pass_mixed_vs_fp64": bool(np.all(rel < 1e-4)),
+           "converged_solves_per_hour_ratio_mixed_over_fp64": rm["solves_per_hour"] / rf["solves_per_hour"]}
+    out["N16"] = blk
+
+    # N = 32: the mixed solve to convergence, the FP64 path's first two steps
+    N, M, K = 32, 64, 2
+    J = load_J(N)
+    W0 = reference_init(N, M, K, seed)
+    rm = solve(nqes, N, M, K, h, J, W0, MX, 4096, 256, 64, 1e-4, 200, seed)
+    progress(f"N=32 mixed solve: {rm['steps']} steps, {rm['wall_s']:.1f}s")
+    nf = 2
+    rf = solve(nqes, N, M, K, h, J, W0, FP, 4096, 256, 64, 1e-4, nf, seed)
+    progress(f"N=32 fp64 first {nf} steps: {rf['wall_s']:.1f}s")
+    rel = np.abs(rm["hist"][:nf] - rf["hist"][:nf]) / np.abs(rf["hist"][:nf])
+    out["N32"] = {"workload": "N=32 M=64 K=2 h=0.595 C=4096 m=256 B=64, tol 1e-4, at most 200 SR steps",
+                  "mixed": summary(rm),
+                  "fp64_first_steps": {"steps": nf, "ms_per_step": rf["ms_per_step"],
+                                       "minres_iters": rf["minres_iters"],
+                                       "mixed_ms_per_step": rm["ms_per_step"][:nf],
+                                       "mixed_minres_iters": rm["minres_iters"][:nf],
+                                       "fp64_over_mixed_time": [a / b for a, b in
+                                                                zip(rf["ms_per_step"], rm["ms_per_step"][:nf])],
+                                       "energies_fp64": rf["hist"].tolist(),
+                                       "energies_mixed": rm["hist"][:nf].tolist(),
+                                       "mixed_vs_fp64_rel": rel.tolist()}}
+    out["converged_solves_per_hour"] = {
+        "N32_mixed": rm["solves_per_hour"],
+        "N16_mixed": out["N16"]["mixed"]["solves_per_hour"],
+        "N16_fp64": out["N16"]["fp64"]["solves_per_hour"],
+        "N8_mixed": out["N8"]["mixed"]["solves_per_hour"],
+        "N8_fp64": out["N8"]["fp64"]["solves_per_hour"],
+    }
+    return out
 
 
 def profiled_traffic(args, real, mode):
@@ -70,12 +268,173 @@ def profiled_traffic(args, real, mode):
     return None
 
 
-def power_law_J(N, alpha):
-    i = np.arange(N)
-    d = np.abs(i[:, None] - i[None, :]).astype(float)
-    J = np.zeros((N, N))
-    J[d > 0] = d[d > 0] ** -alpha
-    return J / J.max()
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(args, J, W0):
+    """The CPU oracle (clean-room FP64 OpenMP restatement of rbm.cc's SR step,
+    matrix-free S*v with O recomputed per MINRES-QLP iteration) on the published
+    run's chain count: 128 chains x 200 samples of the SR-step workload."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle
+    if not os.path.exists(pyoracle.LIB_PATH):
+        pyoracle.build()
+    threads = pyoracle.max_threads()
+    C, m, B = args.cpu_C, args.cpu_m, args.B
+    W = W0.copy()
+    t0 = time.perf_counter()
+    W, emat, info = pyoracle.sr_step(args.N, args.M, args.K, args.h, J, W, args.seed, C, m, B, args.lr,
+                                     args.reglam, args.rtol)
+    dt = time.perf_counter() - t0
+    return {
+        "value": C * m / dt,
+        "unit": "MC samples/s (whole SR step)",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"one SR step of the SR-step workload with C={C} chains x m={m} samples (the reference's published "
+                  f"-c 128 -m 200), B={B}; {dt:.1f}s, {info['iters']} MINRES-QLP iterations; s/step {dt:.2f}",
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+    }
+
+
+def sv_roofline(args, stats, real):
+    """Roofline of the S*v kernels of the SR-step workload (the dominant kernel pair)."""
+    S = args.C * args.m
+    P = args.K * (args.N + args.M + args.N * args.M)
+    sv_env = os.environ.get("NQES_SV")
+    factored = args.precision == "mixed" and args.N <= 64 and sv_env != "stored"
+    q8 = factored and real and sv_env != "fp64mfma"
+    sv_calls = sum(s["sv_calls"] for s in stats)
+    sv_ms = sum(s["t_sv_ms"] for s in stats)
+    sv_avg_ms = sv_ms / max(sv_calls, 1)
+    if args.precision == "fp64":
+        # one streamed pass over O (real weights: FP64 real; else complex128), w per row
+        # inside the pass; v, o, result
+        b_O = 8 if real else 16
+        sv_bytes = S * P * b_O + 3 * P * b_O
+        kernel = f"S*v: k_sv_rows64 (one pass over {'FP64' if real else 'complex128'} O)"
+        achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
+        return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args, real, "stored"),
+                "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes}
+    if q8:
+        # int8 form: each pass reads the quantised factors once (3 digit bytes per row
+        # and column), plus per row its scale (8 B) and spin words, and w once per sample
+        R = S * args.K
+        ucols = 16 * ((args.K * args.M + args.K + 15) // 16)
+        nw = (args.N + 31) // 32
+        sv_bytes = 2 * R * ucols * 3 + 2 * R * 8 + R * 4 * nw + 2 * S * 8
+        nst = (args.N + 15) // 16
+        nsp = 32 if args.N <= 32 else 64
+        ncg64 = (ucols + 63) // 64
+        gemm = args.N <= 32 and ncg64 <= 4 and os.environ.get("NQES_Q8_FWD") != "tiles"
+        bwd_ops = 2 * R * ucols * 18 * 16 * nst
+        fwd_ops = R * ncg64 * 3 * (6 * nst + 1) * 2 * 16 * 64 if gemm else 2 * R * ucols * 6 * nsp
+        kernel = f"S*v: {'k_q8_fwdg' if gemm else 'k_q8_fwd'} + k_q8_bwd (int8 MFMA over quantised c*tanh(theta) factors)"
+        achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
+        return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args, real, "q8"),
+                "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes,
+                "int8_mfma_tops": (fwd_ops + bwd_ops) / (sv_avg_ms * 1e-3) / 1e12 if sv_calls else 0.0,
+                "int8_mfma_peak_tops": INT8_MFMA_PEAK_TOPS}
+    if factored:
+        R = S * args.K
+        sv_flops = 2 * 2 * R * args.N * (args.K * args.M + args.K) * (1 if real else 2)
+        achieved = sv_flops / (sv_avg_ms * 1e-3) / 1e12 if sv_calls else 0.0
+        return {"bound": "mfma", "kernel": "S*v: k_svf_fwd + k_svf_bwd (FP64 MFMA over spins and c*tanh(theta))",
+                "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_MFMA_PEAK_TFLOPS, "traffic": profiled_traffic(args, real, "factored"),
+                "sv_avg_ms": sv_avg_ms, "sv_flops": sv_flops}
+    b_O = 4 if real else 8
+    sv_bytes = S * P * b_O + 3 * P * (8 if real else 16)
+    achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
+    return {"bound": "hbm", "kernel": f"S*v: k_sv_rows_mx (one pass over {'FP32' if real else 'complex64'} O)",
+            "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": profiled_traffic(args, real, "stored"), "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes}
+
+
+def srstep_workload(args, nqes):
+    prec = nqes.PREC_FP64 if args.precision == "fp64" else nqes.PREC_MIXED
+    J = load_J(args.N)
+    W0 = initial_weights(args.N, args.M, args.K, args.seed)
+    W = W0.copy()
+    S = args.C * args.m
+    state = {"reglam": args.reglam, "lr": args.lr, "t": 0}
+    with nqes.Solver(args.N, args.M, args.K, args.h, J, precision=prec) as solver:
+        def one_step():
+            emat, st = solver.monte_carlo(W, args.seed + state["t"] * args.C, args.C, args.m, args.B,
+                                          max(state["lr"], 0.01), max(state["reglam"], 1.0), args.rtol, stats=True)
+            state["reglam"] *= 0.9
+            state["lr"] *= 0.99
+            state["t"] += 1
+            return emat, st
+
+        for _ in range(args.warmup):
+            one_step()
+        stats = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            emat, st = one_step()
+            stats.append(st)
+        dt = time.perf_counter() - t0
+    real = not np.any(W0.imag)
+    out = {
+        "metric": "converged solves/hour (dE<1e-4 vs FP64); MC samples/s; p50/p99 solve latency",
+        "value": S * args.steps / dt,
+        "unit": "MC samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic (random N(0,0.01^2) real weights, seeded)",
+        "config": {"workload": f"trapped-ion SR step N={args.N} M={args.M} K={args.K} h={args.h} "
+                               f"C={args.C} m={args.m} B={args.B} rtol={args.rtol} ({args.precision})",
+                   "N": args.N, "M": args.M, "K": args.K, "chains": args.C, "samples_per_chain": args.m,
+                   "burn_in": args.B, "precision": args.precision},
+        "breakdown_ms_per_step": {
+            "sample": sum(s["t_sample_ms"] for s in stats) / args.steps,
+            "moments": sum(s["t_moments_ms"] for s in stats) / args.steps,
+            "solve": sum(s["t_solve_ms"] for s in stats) / args.steps,
+            "sv_device": sum(s["t_sv_ms"] for s in stats) / args.steps,
+        },
+        "minres_iters_per_step": [s["minres_iters"] for s in stats],
+        "accept_rate": sum(s["accepts"] for s in stats) / max(sum(s["proposals"] for s in stats), 1),
+        "ulp_ties": int(sum(s["ulp_ties"] for s in stats)),
+        "energies_last_step": np.sort(np.linalg.eigvals(emat / args.N).real).tolist(),
+        "roofline": sv_roofline(args, stats, real),
+    }
+    return out, J, W0
+
+
+def solve_workload(args, nqes):
+    prec = nqes.PREC_FP64 if args.precision == "fp64" else nqes.PREC_MIXED
+    J = load_J(args.N)
+    W0 = reference_init(args.N, args.M, args.K, args.seed)
+    log = lambda t, ev, st: progress(f"step {t}: E/N {ev} ({st['minres_iters']} MINRES-QLP iterations, "
+                                     f"{st['t_total_ms']:.0f} ms)")
+    r = solve(nqes, args.N, args.M, args.K, args.h, J, W0, prec, args.C, args.m, args.B, args.tol, args.niters,
+              args.seed, log=log)
+    ed = ed_energies(args.N, args.h, J, args.K) if args.N <= 16 else None
+    out = summary(r, ed)
+    out.update({"workload": f"whole solve N={args.N} M={args.M} K={args.K} h={args.h} C={args.C} m={args.m} "
+                            f"B={args.B} tol={args.tol} niters={args.niters} ({args.precision})",
+                "rules": "run() of rbm.cc:1173-1197; lrate 0.1/0.99/0.01, reglam 1/0.9/0.2, rtol 1e-9, "
+                         f"MINRES-QLP maxit {MAXIT_SOLVE}; complex init_rbm weights (rbm.cc:242-260)",
+                "energies": r["hist"].tolist(), "minres_iters": r["minres_iters"], "ms_per_step": r["ms_per_step"]})
+    print(json.dumps(out))
 
 
 def solve_requests(args):
@@ -97,7 +456,7 @@ def solve_requests(args):
             K = int(rng.integers(1, 5))
             h = float(rng.uniform(0.2, 1.2))
             alpha = float(rng.uniform(0.5, 1.5))
-        W0 = (np.random.default_rng(1000 + r).normal(size=K * (N + M + N * M)) * 0.01).astype(np.complex128)
+        W0 = reference_init(N, M, K, 1000 + r)
         reqs.append((t if args.workload == "serving" else 0.0,
                      dict(N=N, M=M, K=K, h=h, J=power_law_J(N, alpha), W0=W0, seed=args.seed + 7919 * r,
                           C=1024, m=256, B=args.B, niters=args.solve_steps)))
@@ -107,8 +466,9 @@ def solve_requests(args):
 def serve_workload(args):
     import nqes
     reqs = solve_requests(args)
-    sched = dict(tol=1e-6, lrate0=0.1, lratef=0.99, lrate_cutoff=0.01, reglam0=40.0, reglam_decay=0.9,
-                 reglam_cutoff=1.0, rtol=args.rtol)
+    sched = dict(tol=1e-4, lrate0=SCHED["lrate0"], lratef=SCHED["lratef"], lrate_cutoff=SCHED["lrate_cutoff"],
+                 reglam0=SCHED["reglam0"], reglam_decay=SCHED["reglam_decay"], reglam_cutoff=SCHED["reglam_cutoff"],
+                 rtol=args.rtol, maxit=MAXIT_SOLVE)
     with nqes.Server(max_slots=args.slots) as srv:
         # warm-up: one small solve (module load, first allocations)
         w = reqs[0][1]
@@ -124,18 +484,18 @@ def serve_workload(args):
         for k, i in enumerate(ids):
             res.append(srv.wait(i, raise_on_error=False)[1])
             if (k + 1) % 8 == 0:
-                print(f"[bench] {k + 1}/{len(ids)} solves done, {time.perf_counter() - t0:.1f}s",
-                      file=sys.stderr, flush=True)
+                progress(f"{k + 1}/{len(ids)} solves done, {time.perf_counter() - t0:.1f}s")
         wall = time.perf_counter() - t0
     lat = np.array([r["t_done_ms"] - r["t_submit_ms"] for r in res])
     queue = np.array([r["t_admit_ms"] - r["t_submit_ms"] for r in res])
     steps = sum(r["steps"] for r in res)
     samples = sum(r["steps"] * q["C"] * q["m"] for (_, q), r in zip(reqs, res))
-    name = "batched field scan" if args.workload == "scan" else "serving (Poisson arrivals, 4 req/s)"
+    ok = [r for r in res if r["status"] == 0]
+    name = "batched field scan" if args.workload == "scan" else f"serving (Poisson arrivals, {args.rate} req/s)"
     out = {
-        "metric": "MC samples/s (whole SR step); converged solves/hour; p50/p99 solve latency",
-        "value": len(res) * 3600.0 / wall,
-        "unit": f"solves/hour ({args.solve_steps} SR steps max per solve)",
+        "metric": "converged solves/hour (dE<1e-4 vs FP64); MC samples/s; p50/p99 solve latency",
+        "value": len(ok) * 3600.0 / wall,
+        "unit": "solves/hour",
         "n_gpus": 1,
         "steps": steps,
         "warmup": 1,
@@ -144,52 +504,27 @@ def serve_workload(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32+f64",
-        "data": "synthetic (power-law J, random N(0,0.01^2) real weights, seeded)",
-        "config": {"workload": f"{name}: {len(res)} solves, 1024 chains x 256 samples, B={args.B}, "
+        "data": "synthetic (power-law J, complex init_rbm weights, seeded)",
+        "config": {"workload": f"{name}: {len(res)} solves, 1024 chains x 256 samples, B={args.B}, tol 1e-4, "
                                f"<= {args.solve_steps} SR steps each, {args.slots} slots, mixed precision",
                    "solves": len(res), "slots": args.slots, "solve_steps": args.solve_steps},
+        "converged": int(sum(r["converged"] for r in ok)),
         "p50_latency_ms": float(np.percentile(lat, 50)),
         "p99_latency_ms": float(np.percentile(lat, 99)),
         "p50_queue_ms": float(np.percentile(queue, 50)),
         "wall_s": wall,
         "mc_samples_per_s": samples / wall,
         "device_busy_ms_sum": float(sum(r["t_device_ms"] for r in res)),
-        "failed": {"count": int(sum(r["status"] != 0 for r in res)),
-                   "note": "status 4 = non-finite energy or gradient: the SR schedule (lr 0.1, reglam 40 -> 1) "
-                           "diverges for some K=1 requests at small h, identically on the FP64 and mixed paths"},
+        "failed": int(len(res) - len(ok)),
     }
     print(json.dumps(out))
-
-
-def cpu_baseline(args, J, W0):
-    """The CPU oracle (clean-room FP64 OpenMP restatement of rbm.cc's SR step,
-    matrix-free S*v with O recomputed per MINRES iteration) on a bounded sample."""
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import pyoracle
-    if not os.path.exists(pyoracle.LIB_PATH):
-        pyoracle.build()
-    threads = pyoracle.max_threads()
-    C, m, B = args.cpu_C, args.cpu_m, args.B
-    W = W0.copy()
-    t0 = time.perf_counter()
-    W, emat, info = pyoracle.sr_step(args.N, args.M, args.K, args.h, J, W, args.seed, C, m, B, args.lr,
-                                     args.reglam, args.rtol)
-    dt = time.perf_counter() - t0
-    return {
-        "value": C * m / dt,
-        "unit": "MC samples/s (whole SR step)",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"one SR step with C={C} chains x m={m} samples, B={B}, same N/M/K/h/J/weights; "
-                  f"{dt:.1f}s, {info['iters']} MINRES-QLP iterations",
-    }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--N", type=int, default=32)
     ap.add_argument("--M", type=int, default=64)
@@ -202,159 +537,34 @@ def main():
     ap.add_argument("--reglam", type=float, default=40.0)
     ap.add_argument("--rtol", type=float, default=1e-9)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--cpu-C", type=int, default=512)
-    ap.add_argument("--cpu-m", type=int, default=64)
+    ap.add_argument("--tol", type=float, default=1e-4, help="solve: convergence tolerance")
+    ap.add_argument("--niters", type=int, default=200, help="solve: SR steps at most")
+    ap.add_argument("--cpu-C", type=int, default=128)
+    ap.add_argument("--cpu-m", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device copy of MIB mebibytes (HBM counter calibration)")
-    ap.add_argument("--workload", default="srstep", choices=["srstep", "scan", "serving"])
+    ap.add_argument("--no-gates", action="store_true", help="skip the converged-solve workloads")
+    ap.add_argument("--workload", default="srstep", choices=["srstep", "solve", "scan", "serving"])
     ap.add_argument("--slots", type=int, default=8, help="scan/serving: concurrent solves")
-    ap.add_argument("--solve-steps", type=int, default=5, help="scan/serving: SR steps per solve at most")
-    ap.add_argument("--requests", type=int, default=48, help="serving: number of requests")
+    ap.add_argument("--solve-steps", type=int, default=200, help="scan/serving: SR steps per solve at most")
+    ap.add_argument("--requests", type=int, default=300, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: mean arrivals per second")
     args = ap.parse_args()
 
-    if args.copy_calibration:
-        # a plain copy kernel of known size: FETCH_SIZE / WRITE_SIZE of this dispatch
-        # give the counter scale used to turn rocprofv3 readings into HBM bytes
-        import torch
-        n = args.copy_calibration * (1 << 20) // 4
-        x = torch.ones(n, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            y.copy_(x)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibration_bytes_each_way": n * 4}))
-        return
-
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
-                                   "design note only)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (sharding across GPUs is a design "
+                                   "note only)"}))
         sys.exit(2)
-
-    if args.workload != "srstep":
+    if args.workload in ("scan", "serving"):
         serve_workload(args)
         return
-
     import nqes
-    prec = nqes.PREC_FP64 if args.precision == "fp64" else nqes.PREC_MIXED
-    J = load_J(args.N)
-    W0 = initial_weights(args.N, args.M, args.K, args.seed)
-    solver = nqes.Solver(args.N, args.M, args.K, args.h, J, precision=prec)
-    W = W0.copy()
-    S = args.C * args.m
-    reglam, lr = args.reglam, args.lr
-    step_i = 0
-
-    def one_step():
-        nonlocal reglam, lr, step_i
-        emat, st = solver.monte_carlo(W, args.seed + step_i * args.C, args.C, args.m, args.B, max(lr, 0.01),
-                                      max(reglam, 1.0), args.rtol, stats=True)
-        reglam *= 0.9
-        lr *= 0.99
-        step_i += 1
-        return emat, st
-
-    for _ in range(args.warmup):
-        one_step()
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        emat, st = one_step()
-        stats.append(st)
-    dt = time.perf_counter() - t0
-    ms_per_step = dt * 1e3 / args.steps
-
-    P = args.K * (args.N + args.M + args.N * args.M)
-    real = not np.any(W0.imag)
-    sv_env = os.environ.get("NQES_SV")
-    factored = args.precision == "mixed" and args.N <= 64 and sv_env != "stored"
-    q8 = factored and real and sv_env != "fp64mfma"
-    if args.precision == "fp64":
-        # two streamed passes over complex128 O, w written and read, v / result / o
-        b_O, n_pass = 16, 2
-        sv_bytes = n_pass * S * P * b_O + 2 * S * 16 + 3 * P * 16
-        sv_kernel = "S*v: k_sv_rowdot64 + k_sv_colacc64 (two passes over complex128 O)"
-    elif q8:
-        # int8 form: each pass reads the quantised factors once (3 digit bytes per row
-        # and column), plus per row its scale (8 B) and spin words, and w once per sample
-        R = S * args.K
-        ucols = 16 * ((args.K * args.M + args.K + 15) // 16)
-        nw = (args.N + 31) // 32
-        sv_bytes = 2 * R * ucols * 3 + 2 * R * 8 + R * 4 * nw + 2 * S * 8
-        nsp = 32 if args.N <= 32 else 64
-        nst = (args.N + 15) // 16
-        ncg64 = (ucols + 63) // 64
-        gemm = args.N <= 32 and ncg64 <= 4 and os.environ.get("NQES_Q8_FWD") != "tiles"
-        # int8 MFMA ops issued: backward 18 16x16x64 MFMAs per 64 rows, column tile and
-        # spin tile; forward either 6 per 16 rows and column tile (tiled form) or, as one
-        # GEMM, 3 (6 nst + 1) 16x16x64 MFMAs per 16 rows and 64 columns
-        bwd_ops = 2 * R * ucols * 18 * 16 * nst
-        fwd_ops = R * ncg64 * 3 * (6 * nst + 1) * 2 * 16 * 64 if gemm else 2 * R * ucols * 6 * nsp
-        sv_ops = fwd_ops + bwd_ops
-        fwd_name = "k_q8_fwdg" if gemm else "k_q8_fwd"
-        sv_kernel = f"S*v: {fwd_name} + k_q8_bwd (int8 MFMA over quantised c*tanh(theta) factors)"
-    elif factored:
-        # two FP64-MFMA passes over the factors: sigma^T B (R x N x (K M + K)) and the
-        # transposed product, R = S K rows; complex weights double the real products
-        R = S * args.K
-        sv_flops = 2 * 2 * R * args.N * (args.K * args.M + args.K) * (1 if real else 2)
-        sv_kernel = "S*v: k_svf_fwd + k_svf_bwd (FP64 MFMA over spins and c*tanh(theta) factors)"
-    else:
-        # one streamed pass over FP32 (real) or complex64 O, v / result / o in FP64
-        b_O, n_pass = (4 if real else 8), 1
-        sv_bytes = n_pass * S * P * b_O + 3 * P * (8 if real else 16)
-        sv_kernel = f"S*v: k_sv_rows_mx (one pass over {'FP32' if real else 'complex64'} O)"
-    sv_calls = sum(s["sv_calls"] for s in stats)
-    sv_ms = sum(s["t_sv_ms"] for s in stats)
-    sv_avg_ms = sv_ms / max(sv_calls, 1)
-    if q8:
-        achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
-        roofline = {"bound": "hbm", "kernel": sv_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args, real, "q8"),
-                    "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes,
-                    "int8_mfma_tops": sv_ops / (sv_avg_ms * 1e-3) / 1e12 if sv_calls else 0.0,
-                    "int8_mfma_peak_tops": INT8_MFMA_PEAK_TOPS}
-    elif factored:
-        achieved = sv_flops / (sv_avg_ms * 1e-3) / 1e12 if sv_calls else 0.0
-        roofline = {"bound": "mfma", "kernel": sv_kernel, "achieved": achieved, "peak": FP64_MFMA_PEAK_TFLOPS,
-                    "unit": "TFLOP/s", "frac": achieved / FP64_MFMA_PEAK_TFLOPS,
-                    "traffic": profiled_traffic(args, real, "factored"), "sv_avg_ms": sv_avg_ms, "sv_flops": sv_flops}
-    else:
-        achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
-        roofline = {"bound": "hbm", "kernel": sv_kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args, real, "stored"),
-                    "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes}
-    energies = np.sort(np.linalg.eigvals(emat / args.N).real).tolist()
-    out = {
-        "metric": "MC samples/s (whole SR step); converged solves/hour; p50/p99 solve latency",
-        "value": S * args.steps / dt,
-        "unit": "MC samples/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-        "data": "synthetic (random N(0,0.01^2) real weights, seeded)",
-        "config": {"workload": f"trapped-ion SR step N={args.N} M={args.M} K={args.K} h={args.h} "
-                               f"C={args.C} m={args.m} B={args.B} rtol={args.rtol} ({args.precision})",
-                   "N": args.N, "M": args.M, "K": args.K, "chains": args.C, "samples_per_chain": args.m,
-                   "burn_in": args.B, "precision": args.precision},
-        "solves_per_hour_200_steps": 3600e3 / (200 * ms_per_step),
-        "breakdown_ms_per_step": {
-            "sample": sum(s["t_sample_ms"] for s in stats) / args.steps,
-            "moments": sum(s["t_moments_ms"] for s in stats) / args.steps,
-            "solve": sum(s["t_solve_ms"] for s in stats) / args.steps,
-            "sv_device": sv_ms / args.steps,
-        },
-        "minres_iters_per_step": [s["minres_iters"] for s in stats],
-        "accept_rate": sum(s["accepts"] for s in stats) / max(sum(s["proposals"] for s in stats), 1),
-        "energies_last_step": energies,
-        "roofline": roofline,
-    }
+    if args.workload == "solve":
+        solve_workload(args, nqes)
+        return
+    out, J, W0 = srstep_workload(args, nqes)
+    progress(f"SR step: {out['ms_per_step']:.1f} ms")
+    if not args.no_gates:
+        out["converged"] = converged_block(args, nqes)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args, J, W0)
     print(json.dumps(out))
