@@ -365,7 +365,10 @@ def sv_roofline(args, stats, real):
 def srstep_workload(args, nqes):
     prec = nqes.PREC_FP64 if args.precision == "fp64" else nqes.PREC_MIXED
     J = load_J(args.N)
-    W0 = initial_weights(args.N, args.M, args.K, args.seed)
+    if args.init == "real":
+        W0 = initial_weights(args.N, args.M, args.K, args.seed)
+    else:
+        W0 = reference_init(args.N, args.M, args.K, args.seed)
     W = W0.copy()
     S = args.C * args.m
     state = {"reglam": args.reglam, "lr": args.lr, "t": 0}
@@ -399,7 +402,8 @@ def srstep_workload(args, nqes):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
-        "data": "synthetic (random N(0,0.01^2) real weights, seeded)",
+        "data": "synthetic (random N(0,0.01^2) real weights, seeded)" if args.init == "real"
+                else "synthetic (complex init_rbm weights, seeded)",
         "config": {"workload": f"trapped-ion SR step N={args.N} M={args.M} K={args.K} h={args.h} "
                                f"C={args.C} m={args.m} B={args.B} rtol={args.rtol} ({args.precision})",
                    "N": args.N, "M": args.M, "K": args.K, "chains": args.C, "samples_per_chain": args.m,
@@ -537,6 +541,8 @@ def main():
     ap.add_argument("--reglam", type=float, default=40.0)
     ap.add_argument("--rtol", type=float, default=1e-9)
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--init", default="real", choices=["real", "cplx"],
+                    help="SR-step weights: real N(0, 0.01^2) or the reference's complex init_rbm distribution")
     ap.add_argument("--tol", type=float, default=1e-4, help="solve: convergence tolerance")
     ap.add_argument("--niters", type=int, default=200, help="solve: SR steps at most")
     ap.add_argument("--cpu-C", type=int, default=128)
