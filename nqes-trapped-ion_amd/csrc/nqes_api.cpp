@@ -104,6 +104,7 @@ struct nqes_ctx {
     DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
     DevBuf dUq, descale, dvq, dq8w, dq8m;              // its int8 form (real weights)
     DevBuf dUf;                                        // row-major digits (GEMM forward pass)
+    DevBuf dUr, dspins2;                               // complex weights, int8 form: split rows
     DevBuf dUmax;                                      // max |U| per factor row
     hipGraph_t svg = nullptr;                          // int8 S*v sequence, captured once per
     hipGraphExec_t svx = nullptr;                      // sample set and replayed per MINRES step
@@ -112,7 +113,7 @@ struct nqes_ctx {
     bool dense = false;                                // dense S (FP64): oo P x P
     DevBuf doo, dvo, dskk;
     DevBuf denum;                                      // exhaustive evaluation outputs
-    bool svf = false, q8 = false;
+    bool svf = false, q8 = false, q8c = false;        // q8c: complex weights on the int8 form
     bool oreal = false;                                // FP64 path, real weights: O stored as doubles
     Sv64Shape sv64{};                                  // FP64 path: one-pass S*v shape (ok = 0: two passes)
     DevBuf dsvwsum;
@@ -249,10 +250,12 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
         b += (long long)sh.fwd_cgs * S * K * 16;
         b += std::max((long long)sh.bwd_blocks * 16, nchunk * 8) * (P + 1);
         b += (nchunk / 32 + 2) * (P + 1) * 16 + 3 * (P + 1) * 16;
-        const Q8Shape q8 = nq_q8_shape(N, M, K, S * K);  // int8 form, used for real weights
+        // int8 form; complex weights split each factor row into two real rows
+        const Q8Shape q8 = nq_q8_shape(N, M, K, 2 * S * K);
         if (q8.ok)
             b += q8.rpad * (long long)q8.ucols * 3 + q8.rpad * 12 +
-                 (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes + q8.uf_bytes);
+                 (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes + q8.uf_bytes) +
+                 2 * S * K * ((long long)q8.ucols * 4 + NW * 4) + (long long)q8.fwd_cgs * S * K * 32;
     } else {
         b += S * Pld * (long long)o_elem(precision, cplx);  // O
         if (mixed) b += (mch / 32 + 2) * P * 16;              // moments scratch
@@ -374,7 +377,7 @@ static int sample_fp64(nqes_ctx *c, uint64_t seed, int C, int m, int B)
 // arguments of the factored S*v / moments kernels for S samples
 static SvfArgs svf_args(nqes_ctx *c, long long S)
 {
-    SvfArgs f;
+    SvfArgs f{};
     f.N = c->N;
     f.M = c->M;
     f.K = c->K;
@@ -401,6 +404,20 @@ static SvfArgs svf_args(nqes_ctx *c, long long S)
     f.vq = c->dvq.p;
     f.q8w = c->dq8w.p;
     f.q8m = c->dq8m.p;
+    return f;
+}
+
+// the int8 form's arguments for complex weights: the 2 S K split real rows (row 2 r = Re
+// U_r, 2 r + 1 = Im U_r), K = 2 K rows per sample over the column layout of K
+static SvfArgs q8c_args(nqes_ctx *c)
+{
+    SvfArgs f = svf_args(c, c->S);
+    f.K = 2 * c->K;
+    f.kc = c->K;
+    f.U = c->dUr.p;
+    f.spins = static_cast<const uint32_t *>(c->dspins2.p);
+    f.Umax = (const float *)c->dUmax.p;
+    f.Ucols = c->q8_shape.ucols;
     return f;
 }
 
@@ -474,6 +491,12 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
             HIPCHK(c, nq_launch_svf_moments(cplx, f, c->svf_shape, c->dsvfpart.p, c->dsvfscratch.p, c->dsvftmp.p,
                                             c->dmo.p, c->dms.p, c->dmf.p, c->stream),
                    "factored moments kernels");
+            if (c->q8c) {
+                c->S = S;  // q8c_args sizes its rows by the sample count
+                HIPCHK(c, nq_q8c_pack(q8c_args(c), c->q8_shape, c->dU.p, static_cast<const uint32_t *>(c->dspins.p),
+                                      c->stream),
+                       "complex factor quantisation kernels");
+            }
         }
         return NQES_OK;
     }
@@ -518,14 +541,18 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     drop_sv_graph(c);  // buffers may move
     // factored S*v (mixed path) unless NQES_SV=stored or the shape does not apply; for
     // real weights its int8 form unless NQES_SV=fp64mfma
-    c->svf = c->q8 = false;
+    c->svf = c->q8 = c->q8c = false;
     if (c->prec == NQES_PREC_MIXED) {
         const char *e = getenv("NQES_SV");
         c->svf_shape = nq_svf_shape(cplx, c->N, c->M, K);
         c->svf = c->svf_shape.ok && !(e && strcmp(e, "stored") == 0);
+        const bool fp64mfma = e && strcmp(e, "fp64mfma") == 0;
         if (c->svf && !cplx) {
             c->q8_shape = nq_q8_shape(c->N, c->M, K, S * K);
-            c->q8 = c->q8_shape.ok && !(e && strcmp(e, "fp64mfma") == 0);
+            c->q8 = c->q8_shape.ok && !fp64mfma;
+        } else if (c->svf) {  // complex: the int8 form over the 2 S K split real rows
+            c->q8_shape = nq_q8_shape(c->N, c->M, K, 2 * S * K);
+            c->q8c = c->q8_shape.ok && !fp64mfma;
         }
     }
     if (c->svf) {
@@ -534,11 +561,16 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
         const long long npair = K > 1 ? (long long)K * (K - 1) / 2 : 1;
         long long nchunk = nb * npair;
         long long cgs = c->svf_shape.fwd_cgs;
-        if (c->q8) {
+        if (c->q8 || c->q8c) {
             nb = std::max(nb, (long long)c->q8_shape.bwd_blocks);
             nchunk = std::max(nchunk, (long long)c->q8_shape.bwd_blocks);
             cgs = std::max(cgs, (long long)c->q8_shape.fwd_cgs);
             const long long rp = c->q8_shape.rpad;
+            if (c->q8c) {  // split rows, their spins and row maxima; 2 runs x cgs x 2 S K row sums
+                HIPCHK(c, c->dUr.ensure((size_t)2 * S * K * c->q8_shape.ucols * 4), "alloc split factor rows");
+                HIPCHK(c, c->dspins2.ensure((size_t)2 * S * K * c->NW * 4), "alloc split spins");
+                cgs = std::max(cgs, (long long)2 * c->q8_shape.fwd_cgs);  // x 2 S K doubles = cgs S K 16 B
+            }
             HIPCHK(c, c->dUq.ensure((size_t)rp * c->q8_shape.ucols * 3), "alloc quantised factors");
             HIPCHK(c, c->descale.ensure((size_t)rp * 8), "alloc factor scales");
             HIPCHK(c, c->dUmax.ensure((size_t)rp * 4), "alloc factor row maxima");
@@ -959,6 +991,33 @@ static int sv_fp64_device(nqes_ctx *c, const void *v, const void *ov, size_t dsi
     return NQES_OK;
 }
 
+// device part of the complex-weight S*v on the int8 form: Re v | Im v | o^T v to the
+// device in one copy, the kernels of nq_q8c_apply, the raw sums back into svraw
+static int sv_q8c_device(nqes_ctx *c, const cd *v, cd ov)
+{
+    const long long P = c->P;
+    c->hv.resize(2 * P + 2);
+    for (long long q = 0; q < P; q++) {
+        c->hv[q] = v[q].re;
+        c->hv[P + q] = v[q].im;
+    }
+    c->hv[2 * P] = ov.re;
+    c->hv[2 * P + 1] = ov.im;
+    double *dv = static_cast<double *>(c->dv.p);  // 16 (P + 1) bytes
+    HIPCHK(c, hipMemcpyAsync(dv, c->hv.data(), 8 * (2 * P + 2), hipMemcpyHostToDevice, c->stream), "copy v");
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
+    HIPCHK(c, nq_q8c_apply(q8c_args(c), c->q8_shape, dv, dv + P, dv + 2 * P, c->dsvfscratch.p, c->dsvout.p, c->stream),
+           "complex int8 factored S*v kernels");
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
+    std::vector<double> raw(2 * (P + 1));
+    HIPCHK(c, hipMemcpyAsync(raw.data(), c->dsvout.p, 8 * 2 * (P + 1), hipMemcpyDeviceToHost, c->stream), "copy S*v");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
+    for (long long q = 0; q <= P; q++) c->svraw[q] = nq64::mk(raw[q], raw[P + 1 + q]);
+    c->sv_ms += event_ms(c->ev[2], c->ev[3]);
+    c->sv_calls++;
+    return NQES_OK;
+}
+
 // real O and o (real weights, either path): part 0 / 1 of v (real / imaginary) into hv,
 // returns o^T hv
 static double mixed_real_prepare(nqes_ctx *c, const cd *v, int part)
@@ -1051,7 +1110,7 @@ static int apply_S(nqes_ctx *c, const cd *v, cd *out)
     }
     // mixed: centred single pass, (1/S) [sum_s conj(O_s) w_s - conj(o) sum_s w_s]
     if (c->cplx) {
-        int st = sv_mixed_device(c, v, &ov, 16);
+        int st = c->q8c ? sv_q8c_device(c, v, ov) : sv_mixed_device(c, v, &ov, 16);
         if (st != NQES_OK) return st;
         const cd wsum = c->svraw[P];
         for (long long q = 0; q < P; q++) {

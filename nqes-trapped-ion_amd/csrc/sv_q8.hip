@@ -153,7 +153,7 @@ __global__ void k_q8_prep_v(SvfArgs a, int nsp, uint8_t *vfrag, double *colscale
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int UC = a.Ucols;
     if (col >= UC) return;
-    const int N = a.N, M = a.M, K = a.K;
+    const int N = a.N, M = a.M, K = a.kc ? a.kc : a.K;
     const long long KM = (long long)K * M;
     const double *v = (const double *)a.v;
     auto Bval = [&](int i) -> double {
@@ -326,7 +326,8 @@ __global__ void k_q8_masks(const uint32_t *spins, long long R, int N, int NW, in
 }
 
 // Row weights of the backward pass, per call: omega_r = wt(s) 2^(e_r - 30) with
-// wt = w_s, 1 or E_s (WSEL 0, 1, 2).  Block b (one per backward block): G_b from
+// wt = w_s, 1 or E_s (WSEL 0, 1, 2), or for split complex rows wa[s] on even rows and
+// sb wb[s] on odd ones (WSEL 3).  Block b (one per backward block): G_b from
 // max |omega| over its rows (gblk[b]), then per chunk the digit dwords
 // pd[(chunk QW + u) 4 + g] = uint4 over q of Omega = rint(omega 2^(46 - G_b)) (digit u,
 // least significant first); and part[b][P] = sum of wt(s) over its rows with r % K == 0.
@@ -345,7 +346,8 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
     auto wt = [&](long long r) -> double {
         if constexpr (WSEL == 0) return w[r / K];
         else if constexpr (WSEL == 1) return 1.0;
-        else return ((const nq64::cd *)a.E)[r / K].re;
+        else if constexpr (WSEL == 2) return ((const nq64::cd *)a.E)[r / K].re;
+        else return (r & 1) ? a.sb * a.wb[r / K] : a.wa[r / K];  // split complex rows
     };
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double mx = 0.0, sw = 0.0;
@@ -418,6 +420,7 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
                                                 int zw, int nyl)
 {
     const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
+    const int Kc = a.kc ? a.kc : K;  // K of the column layout
     const int NCT = UC / 16;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
@@ -435,7 +438,7 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
     const int zs = blockIdx.z * zw + wv % zw;
     const int i0 = zs * NIT * 16;
     const long long R = a.S * K;
-    const long long KM = (long long)K * M;
+    const long long KM = (long long)Kc * M;
     const long long n64 = (R + 63) / 64;
     long long c0, c1;
     block_chunks(n64, bx, nb, c0, c1);
@@ -510,7 +513,7 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
             if (i < N) {
                 const double val = hsum * osc;
                 if (col < KM) part[(col / M) * a.L + N + M + (long long)i * M + col % M] = val;
-                else if (col < KM + K) part[(col - KM) * a.L + i] = val;
+                else if (col < KM + Kc) part[(col - KM) * a.L + i] = val;
             }
         }
     }
@@ -518,6 +521,61 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
 #pragma unroll
     for (int s = NS - 1; s >= 0; s--) hsum = fma(hsum, 256.0, (double)ts[s]);
     if (g == 0 && col < KM && zs == 0) part[(col / M) * a.L + N + col % M] = hsum * osc;
+}
+
+// Complex factor rows (R x UC complex float, row r = s K + kk) as real rows for the
+// int8 form: Ur[2 r] = Re U_r, Ur[2 r + 1] = Im U_r, with the row maxima of |.| and the
+// spin words of row r copied to both.  Thread per (row r, column); the row maxima by a
+// 16-lane max over the row's columns (UC is a multiple of 16; 64-thread blocks cover
+// four rows' columns in 16-column strips).
+__global__ void k_q8c_split(const float2 *U, long long R, int UC, int NW, const uint32_t *spins, float *Ur,
+                            float *Umax, uint32_t *spins2)
+{
+    const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 4);
+    const int lo = threadIdx.x & 15;
+    if (r >= R) return;
+    float mre = 0.f, mim = 0.f;
+    for (int c = lo; c < UC; c += 16) {
+        const float2 u = U[r * UC + c];
+        Ur[2 * r * UC + c] = u.x;
+        Ur[(2 * r + 1) * UC + c] = u.y;
+        mre = fmaxf(mre, fabsf(u.x));
+        mim = fmaxf(mim, fabsf(u.y));
+    }
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+        mre = fmaxf(mre, __shfl_xor(mre, off));
+        mim = fmaxf(mim, __shfl_xor(mim, off));
+    }
+    if (lo == 0) {
+        Umax[2 * r] = mre;
+        Umax[2 * r + 1] = mim;
+    }
+    for (int w = lo; w < NW; w += 16) {
+        const uint32_t x = spins[r * NW + w];
+        spins2[2 * r * NW + w] = x;
+        spins2[(2 * r + 1) * NW + w] = x;
+    }
+}
+
+// w_s from the two forward runs over the split rows (wa: run on Re v, wb: run on Im v,
+// each ncg column groups' partial sums R apart; rows 2 r, 2 r + 1 of replica row r = s Kc + kk): Re w_s = sum_kk (wa[2 r] - wb[2 r + 1]),
+// Im w_s = sum_kk (wb[2 r] + wa[2 r + 1]), minus o^T v; written planar, wr[s] and wi[s]
+__global__ void k_q8c_combine(const double *wa, const double *wb, long long S, int Kc, int ncg, long long R,
+                              const double *ov, double *wr, double *wi)
+{
+    const double ovr = ov[0], ovi = ov[1];
+    for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < S; s += (long long)gridDim.x * blockDim.x) {
+        double re = 0.0, im = 0.0;
+        for (int g = 0; g < ncg; g++)  // column groups' partial row sums, in group order
+            for (int kk = 0; kk < Kc; kk++) {
+                const long long r = g * R + 2 * (s * Kc + kk);
+                re += wa[r] - wb[r + 1];
+                im += wb[r] + wa[r + 1];
+            }
+        wr[s] = re - ovr;
+        wi[s] = im - ovi;
+    }
 }
 
 int nit_of(int N) { return (N + 15) / 16; }
@@ -668,4 +726,70 @@ hipError_t nq_q8_moments(const SvfArgs &a0, const Q8Shape &sh, const SvfShape &s
     if (e == hipSuccess) e = nq_chunk_reduce(false, part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, rE, st);
     if (e != hipSuccess) return e;
     return nq_launch_svf_sq_pack(false, a0, svf, part, scratch, tmp, so, ss, sf, st);
+}
+
+// Complex weights: the split real rows, quantised and masked as the real form's rows.
+// a: the real form's arguments over the 2 S Kc split rows (a.K = 2 Kc, a.kc = Kc,
+// a.U = the split rows Ur, a.spins = spins2, a.Umax its row maxima); Uc: the complex
+// factor rows, spins: the S Kc replica rows' spin words.
+hipError_t nq_q8c_pack(const SvfArgs &a, const Q8Shape &sh, const void *Uc, const uint32_t *spins, hipStream_t st)
+{
+    const long long Rc = a.S * a.kc;
+    hipLaunchKernelGGL(k_q8c_split, dim3((unsigned)((Rc + 3) / 4)), dim3(64), 0, st, (const float2 *)Uc, Rc, a.Ucols,
+                       a.NW, spins, (float *)a.U, (float *)a.Umax, (uint32_t *)a.spins);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return nq_q8_pack(a, sh, st);
+}
+
+// S*v of complex weights on the int8 form:
+//   forward runs on Re v (a.v = vre) and Im v (vim): row sums wrow[0..R), wrow[R..2R);
+//   w_s = sum_kk U_kk (Y + bias) - o^T v from them (planar wr | wi in a.w);
+//   backward runs with row weights (wr on Re rows, wi on Im rows) and (wi, -wr):
+//   out[0..P] = sum_s Re(conj(O_s) w_s) and sum_s Re w_s, out[P+1..2P+1] = the imaginary
+//   parts (sum_r sigma (Re U wr + Im U wi) and sum_r sigma (Re U wi - Im U wr)).
+// ovc: o^T v as (re, im) on the device.
+hipError_t nq_q8c_apply(const SvfArgs &a0, const Q8Shape &sh, const void *vre, const void *vim, const void *ovc,
+                        void *scratch, void *out, hipStream_t st)
+{
+    SvfArgs a = a0;
+    const long long R = a.S * a.K;
+    const int ncg = sh.gemm ? 1 : sh.fwd_cgs;  // partial row sums per run (column groups)
+    double *wrow = (double *)a0.wrow;
+    double *wr = (double *)a0.w, *wi = wr + a.S;
+    a.ncg = ncg;
+    hipError_t e = hipSuccess;
+    for (int part = 0; part < 2 && e == hipSuccess; part++) {
+        a.v = part ? vim : vre;
+        a.wrow = wrow + (long long)part * ncg * R;
+        if (sh.gemm) {
+            e = nq_q8_fwd_gemm(a, sh, st, true);
+        } else {
+            const int UC = a.Ucols, nct = UC / 16;
+            uint8_t *vfrag = (uint8_t *)a.vq;
+            double *colscale = (double *)(vfrag + (size_t)nct * QV * 64 * 16);
+            hipLaunchKernelGGL(k_q8_prep_v, dim3((UC + 63) / 64), dim3(64), 0, st, a, sh.nsp, vfrag, colscale,
+                               colscale + UC);
+            const dim3 grid(sh.fwd_blocks, sh.fwd_cgs);
+            if (sh.nsp == 32)
+                hipLaunchKernelGGL((k_q8_fwd<32>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
+            else hipLaunchKernelGGL((k_q8_fwd<64>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
+            e = hipGetLastError();
+        }
+    }
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_q8c_combine, dim3((unsigned)((a.S + 255) / 256 < 4096 ? (a.S + 255) / 256 : 4096)), dim3(256),
+                       0, st, (const double *)wrow, (const double *)(wrow + (long long)ncg * R), a.S, a.kc, ncg, R,
+                       (const double *)ovc, wr, wi);
+    e = hipGetLastError();
+    for (int part = 0; part < 2 && e == hipSuccess; part++) {
+        a.wa = part ? wi : wr;
+        a.wb = part ? wr : wi;
+        a.sb = part ? -1.0 : 1.0;
+        e = launch_bwd<3>(a, sh, st);
+        if (e == hipSuccess)
+            e = nq_chunk_reduce(false, a.part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, (double *)out + part * (a.P + 1),
+                                st);
+    }
+    return e;
 }

@@ -56,7 +56,7 @@ __device__ inline double dpp_sum16x4(const double (&h)[4], int l)
 __global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst, uint4 *qg)
 {
     __shared__ double red[4];
-    const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
+    const int N = a.N, M = a.M, K = a.kc ? a.kc : a.K, UC = a.Ucols;  // K of the column layout
     const long long KM = (long long)K * M;
     const double *v = (const double *)a.v;
     auto Bval = [&](int col, int i) -> double {
@@ -275,14 +275,15 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
 
 }  // namespace
 
-hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows)
 {
     hipLaunchKernelGGL(k_q8_prep_vg, dim3(sh.ncg64 * (sh.nst + 1)), dim3(256), 0, st, a, sh.ncg64, sh.nst, (uint4 *)a.vq);
     const size_t lds = sh.qg_bytes - 16;
     const dim3 grid(sh.fwd_blocks), blk(256);
 #define NQ_FWDG(NST, NCG)                                                                              \
     do {                                                                                              \
-        if (a.K == 1) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 1>), grid, blk, lds, st, a);           \
+        if (rows) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0>), grid, blk, lds, st, a);               \
+        else if (a.K == 1) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 1>), grid, blk, lds, st, a);      \
         else if (a.K == 2) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 2>), grid, blk, lds, st, a);      \
         else if (a.K == 4) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 4>), grid, blk, lds, st, a);      \
         else hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0>), grid, blk, lds, st, a);                    \

@@ -138,7 +138,7 @@ def test_factored_sv_matches_stored(nqes, monkeypatch, N, M, K, C, m, B, cplx):
     vs = [rng.normal(size=K * (N + M + N * M)) + 0j,
           rng.normal(size=K * (N + M + N * M)) + 1j * rng.normal(size=K * (N + M + N * M))]
     out, mom, Orows = {}, {}, {}
-    for mode in ("factored", "stored"):
+    for mode in ("fp64mfma", "stored"):
         monkeypatch.setenv("NQES_SV", mode)
         with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
             s.sample(W, 4321, C, m, B, 0.7)
@@ -146,6 +146,7 @@ def test_factored_sv_matches_stored(nqes, monkeypatch, N, M, K, C, m, B, cplx):
             mom[mode] = s.moments()
             Orows[mode] = s.O_rows(0, C * m)
     tol = 1e-12 if K == 1 else 1e-6
+    out["factored"], mom["factored"], Orows["factored"] = out["fp64mfma"], mom["fp64mfma"], Orows["fp64mfma"]
     for a, b in zip(out["factored"], out["stored"]):
         assert rel_err(a, b) < tol, rel_err(a, b)
     # O rows rebuilt from the factors equal the stored rows to FP32 rounding (the
@@ -197,19 +198,54 @@ def test_q8_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
     assert sym < 1e-11, sym
 
 
+@pytest.mark.parametrize("N,M,K,C,m", [(8, 16, 1, 37, 5), (8, 16, 2, 64, 8), (17, 34, 3, 29, 7), (32, 64, 2, 96, 9),
+                                        (40, 80, 2, 33, 5), (64, 128, 4, 21, 3), (24, 24, 4, 50, 4)])
+def test_q8c_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
+    """Complex weights on the int8 form: each complex factor row is split into its real
+    and imaginary rows (quantised separately, three digits each), and S*v runs as two
+    forward and two backward int8 passes (real and imaginary parts of v; row weights
+    (Re w, Im w) and (Im w, -Re w)).  Against the FP64-MFMA form on the same samples:
+    equal up to the factor quantisation (2^-24 of each split row's largest entry; bound
+    5e-7), linear to 1e-11 and Hermitian: <u, S v> = conj(<v, S u>) to 1e-11."""
+    print(f"case complex N={N} M={M} K={K} C={C} m={m}", flush=True)
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=9 + N, scale=0.1, complex_=True)
+    P = K * (N + M + N * M)
+    rng = np.random.default_rng(19)
+    v1 = rng.normal(size=P) + 1j * rng.normal(size=P)
+    v2 = (rng.normal(size=P) + 1j * rng.normal(size=P)) * 1e-3
+    out = {}
+    for mode in ("fp64mfma", "q8"):
+        if mode == "q8":
+            monkeypatch.delenv("NQES_SV", raising=False)
+        else:
+            monkeypatch.setenv("NQES_SV", mode)
+        with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+            s.sample(W, 79, C, m, 3, 0.5)
+            out[mode] = [s.apply_S(v) for v in (v1, v2, v1 + (2.5 - 0.5j) * v2, v1.real + 0j)]
+    a1, a2, a12, ar = out["q8"]
+    b1, b2, _, br = out["fp64mfma"]
+    for a, b in ((a1, b1), (a2, b2), (ar, br)):
+        assert rel_err(a, b) < 5e-7, rel_err(a, b)
+    assert rel_err(a12, a1 + (2.5 - 0.5j) * a2) < 1e-11, rel_err(a12, a1 + (2.5 - 0.5j) * a2)
+    herm = abs(np.vdot(v2, a1) - np.conj(np.vdot(v1, a2))) / (np.linalg.norm(v1) * np.linalg.norm(a2))
+    assert herm < 1e-11, herm
+
+
+@pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("N,M,K,C,m", [(8, 16, 1, 37, 5), (17, 34, 3, 29, 7), (32, 64, 2, 96, 9), (24, 24, 4, 50, 4)])
-def test_q8_forward_passes_agree(nqes, monkeypatch, N, M, K, C, m):
+def test_q8_forward_passes_agree(nqes, monkeypatch, N, M, K, C, m, cplx):
     """For N <= 32 the int8 forward pass runs as one GEMM over the columns (v quantised
     with one scale for all of it, the bias as digit slots of an extra spin tile); the
     tiled form (per-column scales, NQES_Q8_FWD=tiles) computes the same products of the
     same factor digits.  Both quantise v to 2^-47 of a scale, so S*v agrees to ~1e-13
     relative (bound 1e-11); v with blocks of very different size (1e-3) included."""
-    print(f"case N={N} M={M} K={K} C={C} m={m}", flush=True)
+    print(f"case N={N} M={M} K={K} C={C} m={m} complex={cplx}", flush=True)
     J = load_J(N)
-    W = random_weights(N, M, K, seed=11 + N, scale=0.1, complex_=False)
+    W = random_weights(N, M, K, seed=11 + N, scale=0.1, complex_=cplx)
     P = K * (N + M + N * M)
     rng = np.random.default_rng(13)
-    v1 = rng.normal(size=P) + 0j
+    v1 = rng.normal(size=P) + (1j * rng.normal(size=P) if cplx else 0j)
     v2 = v1.copy()
     v2[: P // 3] *= 1e-3
     out = {}
