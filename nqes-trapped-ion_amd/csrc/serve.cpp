@@ -1,16 +1,23 @@
 // serve.cpp -- the solve server of nqes.h: continuous batching of whole solves.
 //
 // One solve is the reference's run() loop (rbm.cc:1173-1197) over nqes_sr_step.
-// The server keeps a FIFO of requests and a pool of max_slots worker threads.
+// The server keeps a FIFO of requests and max_slots slots.  Each slot is a worker
+// thread with its own context (own HIP stream, own device buffers), created once and
+// kept: a solve re-initialises the problem on it, and its buffers, which only grow
+// (to the largest solve the slot has run), are reused by the next solve, so a slot in
+// steady state makes no device allocation.  A slot is accounted at the bytes its
+// context holds: the largest need (nqes_required_bytes plus a fixed per-solve
+// allowance) of the solves it has run.
 // Admission happens under the server's lock whenever a request arrives or a solve
-// ends: a request is admitted while a slot is free and its device bytes fit in the
-// budget minus what running solves hold.  A request that does not fit blocks the
-// ones behind it unless they fit as well; each pass of that kind is counted, and
-// after 2*max_slots passes the blocked request is waited for.  Admitted solves go
-// to a ready list that the workers drain; each worker makes its own context (own
-// HIP stream, own device buffers) for the solve and frees it at the end, so what a
-// solve holds is returned as soon as it finishes and concurrent solves share the
-// device only through the hardware queues.
+// ends.  A request goes to an idle slot whose buffers already hold it (the smallest
+// such slot), else to the idle slot that needs the least growth, if that growth fits
+// the budget minus what all slots hold; if it does not, idle slots whose buffers are
+// larger than the request are trimmed (their contexts re-created empty by their own
+// threads) and the request waits.  A request that does not fit blocks the ones behind
+// it unless they fit as well; each pass of that kind is counted, and after
+// 2*max_slots passes the blocked request is waited for.  Concurrent solves share the
+// device only through the hardware queues, and a solve's result depends only on its
+// request, not on the slot or on what the slot ran before.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -44,8 +51,16 @@ struct Job {
     int64_t bytes = 0;
     int passed = 0;  // times a later request was admitted ahead of this one
     bool done = false;
+    bool claimed = false;  // a waiter has taken it (one waiter per id)
     nqes_result res{};
     std::string err;  // message of a failed solve
+};
+
+struct Slot {
+    nqes_ctx *ctx = nullptr;
+    int64_t reserved = 0;  // bytes its context holds (largest need it has served)
+    Job *job = nullptr;    // assigned, not finished
+    bool trim = false;     // re-create the context empty (idle slot, memory short)
 };
 
 double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
@@ -60,46 +75,81 @@ struct nqes_server {
     std::string err;
 
     std::mutex mu;
-    std::condition_variable cv_ready;  // workers: ready list non-empty or stopping
+    std::condition_variable cv_slot;   // workers: a job or a trim for their slot, or stopping
     std::condition_variable cv_done;   // waiters: some solve finished
     std::deque<Job *> queue;           // submitted, not admitted (arrival order)
-    std::deque<Job *> ready;           // admitted, not yet taken by a worker
     std::map<uint64_t, std::unique_ptr<Job>> jobs;
-    int64_t held = 0;                  // bytes of admitted, unfinished solves
-    int active = 0;                    // admitted, unfinished solves
+    std::vector<Slot> slots;
     uint64_t next_id = 1;
     bool stopping = false;
     std::vector<std::thread> workers;
 
+    int64_t reserved_total() const
+    {
+        int64_t t = 0;
+        for (const Slot &s : slots) t += s.reserved;
+        return t;
+    }
+
+    // under mu: the idle slot for a request of `need` bytes (-1: none fits now)
+    int pick_slot(int64_t need)
+    {
+        int best_fit = -1, least_growth = -1;
+        for (int i = 0; i < (int)slots.size(); i++) {
+            const Slot &s = slots[i];
+            if (s.job || s.trim) continue;
+            if (s.reserved >= need) {
+                if (best_fit < 0 || s.reserved < slots[best_fit].reserved) best_fit = i;
+            } else if (least_growth < 0 || s.reserved > slots[least_growth].reserved) {
+                least_growth = i;
+            }
+        }
+        if (best_fit >= 0) return best_fit;
+        if (least_growth >= 0 && need - slots[least_growth].reserved <= budget - reserved_total()) return least_growth;
+        return -1;
+    }
+
     // under mu
     void admit()
     {
-        int64_t avail = budget - held;
         Job *blocked = nullptr;
-        for (auto it = queue.begin(); it != queue.end() && active < max_slots;) {
+        for (auto it = queue.begin(); it != queue.end();) {
             Job *j = *it;
-            if (j->bytes <= avail && !(blocked && blocked->passed >= 2 * max_slots)) {
+            const int si = (blocked && blocked->passed >= 2 * max_slots) ? -1 : pick_slot(j->bytes);
+            if (si >= 0) {
                 if (blocked) blocked->passed++;
-                avail -= j->bytes;
-                held += j->bytes;
-                active++;
+                Slot &s = slots[si];
+                if (s.reserved < j->bytes) s.reserved = j->bytes;
+                s.job = j;
                 j->res.t_admit_ms = ms_since(t0);
-                ready.push_back(j);
                 it = queue.erase(it);
-                cv_ready.notify_one();
+                cv_slot.notify_all();
             } else {
                 if (!blocked) blocked = j;
                 ++it;
             }
         }
+        // the first waiting request cannot grow any idle slot into the budget: trim idle
+        // slots that hold more than it needs (their threads free the buffers)
+        if (blocked) {
+            bool any_idle = false;
+            for (Slot &s : slots)
+                if (!s.job && !s.trim) any_idle = true;
+            if (any_idle && pick_slot(blocked->bytes) < 0) {
+                for (Slot &s : slots)
+                    if (!s.job && !s.trim && s.reserved > 0) s.trim = true;
+                cv_slot.notify_all();
+            }
+        }
     }
 
-    void run(Job *j)
+    void run(Slot &slot, Job *j)
     {
         const nqes_request &q = j->req;
         nqes_result &r = j->res;
         r.K = q.K;
-        nqes_ctx *ctx = nqes_create(device);
+        if (!slot.ctx) slot.ctx = nqes_create(device);
+        nqes_ctx *ctx = slot.ctx;
         if (!ctx) {
             r.status = NQES_EDEVICE;
             j->err = "cannot create a context on the device";
@@ -127,33 +177,62 @@ struct nqes_server {
             }
         }
         r.status = rc;
-        if (rc != NQES_OK) j->err = nqes_last_error(ctx);
-        nqes_destroy(ctx);
+        if (rc != NQES_OK) {
+            j->err = nqes_last_error(ctx);
+            // a failed step may leave the context mid-solve; start the next one fresh
+            nqes_destroy(ctx);
+            slot.ctx = nullptr;
+        }
     }
 
-    void worker()
+    void worker(int si)
     {
         for (;;) {
             Job *j = nullptr;
+            bool trim = false;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_ready.wait(lk, [&] { return !ready.empty() || (stopping && queue.empty() && active == 0); });
-                if (ready.empty()) return;
-                j = ready.front();
-                ready.pop_front();
+                cv_slot.wait(lk, [&] {
+                    return slots[si].job || slots[si].trim || (stopping && queue.empty() && all_idle());
+                });
+                if (slots[si].job) {
+                    j = slots[si].job;
+                } else if (slots[si].trim) {
+                    trim = true;
+                } else {
+                    break;  // stopping
+                }
             }
-            run(j);
+            if (trim) {
+                if (slots[si].ctx) nqes_destroy(slots[si].ctx);
+                std::lock_guard<std::mutex> lk(mu);
+                slots[si].ctx = nullptr;
+                slots[si].reserved = 0;
+                slots[si].trim = false;
+                admit();
+                continue;
+            }
+            run(slots[si], j);
             {
                 std::lock_guard<std::mutex> lk(mu);
                 j->res.t_done_ms = ms_since(t0);
                 j->done = true;
-                held -= j->bytes;
-                active--;
+                slots[si].job = nullptr;
+                if (!slots[si].ctx) slots[si].reserved = 0;  // failed solve: context dropped
                 admit();
-                if (stopping && queue.empty() && active == 0) cv_ready.notify_all();
+                if (stopping && queue.empty() && all_idle()) cv_slot.notify_all();
             }
             cv_done.notify_all();
         }
+        if (slots[si].ctx) nqes_destroy(slots[si].ctx);
+        slots[si].ctx = nullptr;
+    }
+
+    bool all_idle() const
+    {
+        for (const Slot &s : slots)
+            if (s.job) return false;
+        return true;
     }
 };
 
@@ -175,10 +254,11 @@ nqes_server *nqes_server_create(int device, int max_slots, int64_t budget)
     s->max_slots = max_slots;
     s->budget = budget;
     s->t0 = clk::now();
+    s->slots.resize(max_slots);
     for (int i = 0; i < max_slots; i++)
-        s->workers.emplace_back([s, device] {
+        s->workers.emplace_back([s, device, i] {
             (void)hipSetDevice(device);
-            s->worker();
+            s->worker(i);
         });
     return s;
 }
@@ -190,7 +270,7 @@ void nqes_server_destroy(nqes_server *s)
         std::lock_guard<std::mutex> lk(s->mu);
         s->stopping = true;
     }
-    s->cv_ready.notify_all();
+    s->cv_slot.notify_all();
     for (auto &t : s->workers) t.join();
     delete s;
 }
@@ -210,7 +290,8 @@ int nqes_server_submit(nqes_server *s, const nqes_request *q, uint64_t *id)
     if (q->M % q->N != 0) return bad("M must be a multiple of N (rbm.cc:148)");
     if (!q->J || !q->W0) return bad("J and W0 are required");
     if (q->C < 1 || q->m < 1 || q->B < 0 || q->niters < 0) return bad("C, m, B or niters out of range");
-    if (q->model != NQES_MODEL_TRAPPEDION) return bad("unsupported model");
+    if (q->model != NQES_MODEL_TRAPPEDION && q->model != NQES_MODEL_HS && q->model != NQES_MODEL_XXZ)
+        return bad("unsupported model");
     if (q->precision != NQES_PREC_FP64 && q->precision != NQES_PREC_MIXED) return bad("unsupported precision");
     int64_t need = nqes_required_bytes(q->N, q->M, q->K, q->C, q->m, q->precision);
     if (need < 0) return bad("bad sizes");
@@ -262,9 +343,15 @@ int nqes_server_wait(nqes_server *s, uint64_t id, double *W, nqes_result *res)
             return NQES_EINVAL;
         }
         Job *jp = it->second.get();
+        if (jp->claimed) {  // each id has one waiter; a second one is refused
+            s->err = "request id already being waited for";
+            return NQES_EINVAL;
+        }
+        jp->claimed = true;
         s->cv_done.wait(lk, [&] { return jp->done; });
-        j = std::move(it->second);
-        s->jobs.erase(it);
+        auto it2 = s->jobs.find(id);  // looked up again after the wait
+        j = std::move(it2->second);
+        s->jobs.erase(it2);
         if (j->res.status != NQES_OK) s->err = j->err;
     }
     if (W) std::memcpy(W, j->W.data(), j->W.size() * sizeof(double));

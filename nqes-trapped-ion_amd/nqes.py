@@ -394,9 +394,11 @@ class Server:
     def wait(self, rid, raise_on_error=True):
         """(final weights, result dict).  A failed solve raises, or with
         raise_on_error=False returns its result with the status and message."""
-        W = np.zeros(self._P.pop(rid), dtype=np.complex128)
+        W = np.zeros(self._P.get(rid, 0), dtype=np.complex128)
         res = Result()
-        st = self.lib.nqes_server_wait(self.srv, int(rid), W.ctypes.data, ctypes.byref(res))
+        st = self.lib.nqes_server_wait(self.srv, int(rid), W.ctypes.data if W.size else None, ctypes.byref(res))
+        if st == 0 or res.t_done_ms > 0:
+            self._P.pop(rid, None)
         if st != 0 and raise_on_error:
             raise NqesError(f"solve {rid} failed with status {st}: {self._err()}")
         d = res.as_dict()

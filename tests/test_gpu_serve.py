@@ -129,3 +129,92 @@ def test_refusals():
         bad = dict(q, M=12, W0=random_weights(8, 12, 2, 1))  # M % N != 0
         with pytest.raises(nqes.NqesError, match="status 1"):
             srv.submit(**bad, **SCHED)
+
+
+def scan_and_serving_mix():
+    """Requests drawn as the batched field scan (N in {16, 24, 32, 48}, M = 2N, K in
+    {2, 3, 4}, h ~ U[0.1, 1.5], J exponent 1) and the serving stream (N even in
+    16..64, M in {N, 2N}, K in 1..4, h ~ U[0.2, 1.2], J exponent ~ U[0.5, 1.5],
+    including the hs and xxz pair-flip models), at small chain counts; complex
+    init_rbm-style weights on the mixed path, real small ones on some FP64 requests."""
+    rng = np.random.default_rng(8)
+    out = []
+    for r in range(8):
+        if r < 4:
+            N = int(rng.choice([16, 24, 32, 48]))
+            M, K = 2 * N, int(rng.choice([2, 3, 4]))
+            h, alpha, model = float(rng.uniform(0.1, 1.5)), 1.0, nqes.MODEL_TRAPPEDION
+        else:
+            N = int(rng.choice(np.arange(16, 65, 2)))
+            M, K = N * int(rng.choice([1, 2])), int(rng.integers(1, 5))
+            h, alpha = float(rng.uniform(0.2, 1.2)), float(rng.uniform(0.5, 1.5))
+            model = [nqes.MODEL_TRAPPEDION, nqes.MODEL_HS, nqes.MODEL_XXZ, nqes.MODEL_TRAPPEDION][r - 4]
+        prec = nqes.PREC_FP64 if r in (2, 6) else nqes.PREC_MIXED
+        cplx = prec == nqes.PREC_MIXED or r == 6
+        J = synthetic_J(N, alpha)
+        if model != nqes.MODEL_TRAPPEDION:
+            J = np.tril(J, -1) * (-1.0 if model == nqes.MODEL_XXZ else 1.0)
+        W0 = random_weights(N, M, K, 500 + r, scale=0.3 if cplx else 0.01, complex_=cplx)
+        out.append(dict(N=N, M=M, K=K, h=h, J=J, W0=W0, seed=31 * r + 7, C=32 if prec == nqes.PREC_FP64 else 128,
+                        m=16, B=8, niters=3, precision=prec, model=model))
+    return out
+
+
+SCHED_PUB = dict(tol=1e-4, lrate0=0.1, lratef=0.99, lrate_cutoff=0.01, reglam0=1.0, reglam_decay=0.9,
+                 reglam_cutoff=0.2, rtol=1e-9, maxit=3000)
+
+
+def test_scan_and_serving_mix_alone_equals_mix():
+    """Each request of the scan / serving mix gives the same bits alone (its own fresh
+    server) and in the mix, where three slots keep their contexts across solves of
+    different sizes, K, models and precisions (buffers grown by earlier solves reused)."""
+    reqs = scan_and_serving_mix()
+    alone = []
+    for q in reqs:
+        print("alone:", {k: q[k] for k in ("N", "M", "K", "model", "precision")}, flush=True)
+        with nqes.Server(max_slots=1) as srv:
+            alone.append(srv.wait(srv.submit(**q, **SCHED_PUB)))
+    with nqes.Server(max_slots=3) as srv:
+        ids = []
+        for q in reqs:
+            ids.append(srv.submit(**q, **SCHED_PUB))
+            time.sleep(0.002)
+        mix = [srv.wait(i) for i in ids]
+    for q, (Wa, ra), (Wm, rm) in zip(reqs, alone, mix):
+        assert ra["status"] == 0 and rm["status"] == 0, (ra, rm)
+        assert np.array_equal(Wa, Wm), {k: q[k] for k in ("N", "M", "K", "model", "precision")}
+        assert ra["energies"] == rm["energies"] and ra["steps"] == rm["steps"]
+
+
+def test_pairflip_request_equals_python_loop():
+    """hs requests are served (rbm.cc:1092-1119 through the server) and equal the
+    Python loop over a Solver bit for bit."""
+    N, M, K = 12, 24, 2
+    J = np.tril(synthetic_J(N, 2.0), -1)
+    W0 = random_weights(N, M, K, 77, scale=0.2, complex_=True)
+    q = dict(N=N, M=M, K=K, h=0.0, J=J, W0=W0, seed=99, C=64, m=16, B=8, niters=3, precision=nqes.PREC_MIXED,
+             model=nqes.MODEL_HS)
+    with nqes.Server(max_slots=2) as srv:
+        Ws, rs = srv.wait(srv.submit(**q, **SCHED_PUB))
+    W = W0.copy()
+    with nqes.Solver(N, M, K, 0.0, J, precision=nqes.PREC_MIXED, model=nqes.MODEL_HS) as s:
+        hist, conv, _ = nqes.run(s, W, 99, 64, 16, 8, 3, tol=1e-4, lrate0=0.1, lratef=0.99, lrate_cutoff=0.01,
+                                 reglam0=1.0, reglam_decay=0.9, reglam_cutoff=0.2, rtol=1e-9, maxit=3000)
+    assert rs["status"] == 0 and np.array_equal(W, Ws) and rs["steps"] == len(hist)
+
+
+def test_second_waiter_refused():
+    """Each request id has one waiter: a second wait on an id that is being waited for
+    (or already collected) is refused with NQES_EINVAL, never a use after free."""
+    import threading
+    q = request(MIX[3])
+    with nqes.Server(max_slots=1) as srv:
+        rid = srv.submit(**dict(q, niters=3), **SCHED)
+        first = {}
+        t = threading.Thread(target=lambda: first.update(r=srv.wait(rid)))
+        t.start()
+        time.sleep(0.05)
+        with pytest.raises(nqes.NqesError, match="status 1"):
+            srv.wait(rid)
+        t.join()
+        assert first["r"][1]["status"] == 0
