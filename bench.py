@@ -446,7 +446,7 @@ def solve_requests(args):
     rng = np.random.default_rng(3 if args.workload == "scan" else 4)
     reqs = []
     t = 0.0
-    n = 64 if args.workload == "scan" else args.requests
+    n = args.requests if args.requests else (64 if args.workload == "scan" else 300)
     for r in range(n):
         if args.workload == "scan":
             N = int(rng.choice([16, 24, 32, 48]))
@@ -484,11 +484,21 @@ def serve_workload(args):
             if dt > 0:
                 time.sleep(dt)
             ids.append(srv.submit(**q, precision=nqes.PREC_MIXED, **sched))
-        res = []
-        for k, i in enumerate(ids):
-            res.append(srv.wait(i, raise_on_error=False)[1])
-            if (k + 1) % 8 == 0:
-                progress(f"{k + 1}/{len(ids)} solves done, {time.perf_counter() - t0:.1f}s")
+        # collect in completion order, reporting each as it ends
+        res = [None] * len(ids)
+        left = set(range(len(ids)))
+        while left:
+            for k in sorted(left):
+                if srv.done(ids[k]):
+                    res[k] = srv.wait(ids[k], raise_on_error=False)[1]
+                    left.discard(k)
+                    q = reqs[k][1]
+                    progress(f"{len(ids) - len(left)}/{len(ids)} done at {time.perf_counter() - t0:.1f}s: "
+                             f"N={q['N']} M={q['M']} K={q['K']} steps={res[k]['steps']} "
+                             f"converged={res[k]['converged']} status={res[k]['status']} "
+                             f"latency={(res[k]['t_done_ms'] - res[k]['t_submit_ms']) / 1e3:.1f}s")
+            if left:
+                time.sleep(0.05)
         wall = time.perf_counter() - t0
     lat = np.array([r["t_done_ms"] - r["t_submit_ms"] for r in res])
     queue = np.array([r["t_admit_ms"] - r["t_submit_ms"] for r in res])
@@ -552,7 +562,7 @@ def main():
     ap.add_argument("--workload", default="srstep", choices=["srstep", "solve", "scan", "serving"])
     ap.add_argument("--slots", type=int, default=8, help="scan/serving: concurrent solves")
     ap.add_argument("--solve-steps", type=int, default=200, help="scan/serving: SR steps per solve at most")
-    ap.add_argument("--requests", type=int, default=300, help="serving: number of requests")
+    ap.add_argument("--requests", type=int, default=0, help="scan/serving: number of requests (default 64 / 300)")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: mean arrivals per second")
     args = ap.parse_args()
 

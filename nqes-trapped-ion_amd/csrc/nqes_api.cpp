@@ -877,26 +877,37 @@ int nqes_get_moments(nqes_ctx *c, double *o, double *skk, double *f, double *elo
 // the int8 S*v of the current sample set as one graph: v and o^T v from pinned host
 // memory, the kernels of nq_q8_apply, the raw result back to pinned memory (no memset
 // nodes).  NQES_GRAPH=0 launches the same sequence eagerly instead.
+// doubles in and out of one graphed S*v: real weights v | o^T v in and the P + 1 raw
+// sums out; complex weights (q8c) Re v | Im v | o^T v in and the real then imaginary
+// parts of the P + 1 raw sums out
+static long long sv_in_len(const nqes_ctx *c) { return c->q8c ? 2 * c->P + 2 : c->P + 1; }
+static long long sv_out_len(const nqes_ctx *c) { return c->q8c ? 2 * (c->P + 1) : c->P + 1; }
+
 static int build_sv_graph(nqes_ctx *c)
 {
     const long long P = c->P;
-    if ((size_t)(P + 1) > c->pin_len) {
+    const long long nin = sv_in_len(c), nout = sv_out_len(c), npin = std::max(nin, nout);
+    if ((size_t)npin > c->pin_len) {
         if (c->hv_pin) (void)hipHostFree(c->hv_pin);
         if (c->hr_pin) (void)hipHostFree(c->hr_pin);
         c->hv_pin = c->hr_pin = nullptr;
         c->pin_len = 0;
-        HIPCHK(c, hipHostMalloc((void **)&c->hv_pin, 8 * (P + 1), hipHostMallocDefault), "pinned v");
-        HIPCHK(c, hipHostMalloc((void **)&c->hr_pin, 8 * (P + 1), hipHostMallocDefault), "pinned result");
-        c->pin_len = (size_t)(P + 1);
+        HIPCHK(c, hipHostMalloc((void **)&c->hv_pin, 8 * npin, hipHostMallocDefault), "pinned v");
+        HIPCHK(c, hipHostMalloc((void **)&c->hr_pin, 8 * npin, hipHostMallocDefault), "pinned result");
+        c->pin_len = (size_t)npin;
     }
     // v and o^T v arrive in one copy: the graph's kernels read o^T v right after v (the
-    // v buffer holds 16 P bytes, room for P + 1 doubles)
+    // v buffer holds 16 (P + 1) bytes, room for 2 P + 2 doubles)
     SvfArgs f = svf_args(c, c->S);
     f.ov = (const double *)c->dv.p + P;
+    double *dv = static_cast<double *>(c->dv.p);
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin S*v capture");
-    hipError_t e = hipMemcpyAsync(c->dv.p, c->hv_pin, 8 * (P + 1), hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = nq_q8_apply(f, c->q8_shape, c->dsvfscratch.p, c->dsvout.p, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->hr_pin, c->dsvout.p, 8 * (P + 1), hipMemcpyDeviceToHost, c->stream);
+    hipError_t e = hipMemcpyAsync(c->dv.p, c->hv_pin, 8 * nin, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+        e = c->q8c ? nq_q8c_apply(q8c_args(c), c->q8_shape, dv, dv + P, dv + 2 * P, c->dsvfscratch.p, c->dsvout.p,
+                                  c->stream)
+                   : nq_q8_apply(f, c->q8_shape, c->dsvfscratch.p, c->dsvout.p, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->hr_pin, c->dsvout.p, 8 * nout, hipMemcpyDeviceToHost, c->stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -908,17 +919,16 @@ static int build_sv_graph(nqes_ctx *c)
     return NQES_OK;
 }
 
-// the int8 S*v graph in two halves: launch (v and o^T v to pinned memory, graph on the
-// stream) and finish (wait, raw result to svraw); the host may work in between
-static int sv_graph_launch(nqes_ctx *c, const double *v, double ov)
+// the int8 S*v graph in two halves: launch (the sv_in_len doubles of `in` to pinned
+// memory, graph on the stream) and finish (wait, raw result to svraw); the host may
+// work in between
+static int sv_graph_launch(nqes_ctx *c, const double *in)
 {
-    const long long P = c->P;
     if (!c->svx) {
         int st = build_sv_graph(c);
         if (st != NQES_OK) return st;
     }
-    memcpy(c->hv_pin, v, 8 * P);
-    c->hv_pin[P] = ov;
+    memcpy(c->hv_pin, in, 8 * sv_in_len(c));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
     HIPCHK(c, hipGraphLaunch(c->svx, c->stream), "S*v graph launch");
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
@@ -928,21 +938,28 @@ static int sv_graph_launch(nqes_ctx *c, const double *v, double ov)
 static int sv_graph_finish(nqes_ctx *c)
 {
     HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
-    memcpy(c->svraw.data(), c->hr_pin, 8 * (c->P + 1));
+    const long long P = c->P;
+    if (c->q8c)
+        for (long long q = 0; q <= P; q++) c->svraw[q] = nq64::mk(c->hr_pin[q], c->hr_pin[P + 1 + q]);
+    else
+        memcpy(c->svraw.data(), c->hr_pin, 8 * (P + 1));
     c->sv_ms += event_ms(c->ev[2], c->ev[3]);
     c->sv_calls++;
     return NQES_OK;
 }
 
-static bool sv_graph_on(const nqes_ctx *c) { return c->q8 && env_int("NQES_GRAPH", 1) != 0; }
+static bool sv_graph_on(const nqes_ctx *c) { return (c->q8 || c->q8c) && env_int("NQES_GRAPH", 1) != 0; }
 
 // device part of S*v on the FP32 O of the mixed path: raw[p] = sum_s conj(O_sp) w_s,
 // raw[P] = sum_s w_s, w_s = O_s^T v - o^T v.  v given as D = double (real) or cd.
 static int sv_mixed_device(nqes_ctx *c, const void *v, const void *ov, size_t dsize)
 {
     const long long P = c->P;
-    if (sv_graph_on(c)) {
-        int st = sv_graph_launch(c, (const double *)v, *(const double *)ov);
+    if (sv_graph_on(c)) {  // real weights, int8 form: v (P doubles) then o^T v
+        c->hv.resize(P + 1);
+        if (v != c->hv.data()) memcpy(c->hv.data(), v, 8 * P);
+        c->hv[P] = *(const double *)ov;
+        int st = sv_graph_launch(c, c->hv.data());
         return st != NQES_OK ? st : sv_graph_finish(c);
     }
     HIPCHK(c, hipMemcpyAsync(c->dv.p, v, dsize * P, hipMemcpyHostToDevice, c->stream), "copy v");
@@ -993,7 +1010,8 @@ static int sv_fp64_device(nqes_ctx *c, const void *v, const void *ov, size_t dsi
 
 // device part of the complex-weight S*v on the int8 form: Re v | Im v | o^T v to the
 // device in one copy, the kernels of nq_q8c_apply, the raw sums back into svraw
-static int sv_q8c_device(nqes_ctx *c, const cd *v, cd ov)
+// Re v | Im v | o^T v into hv (the graphed and the eager form read it alike)
+static void q8c_prepare(nqes_ctx *c, const cd *v, cd ov)
 {
     const long long P = c->P;
     c->hv.resize(2 * P + 2);
@@ -1003,6 +1021,16 @@ static int sv_q8c_device(nqes_ctx *c, const cd *v, cd ov)
     }
     c->hv[2 * P] = ov.re;
     c->hv[2 * P + 1] = ov.im;
+}
+
+static int sv_q8c_device(nqes_ctx *c, const cd *v, cd ov)
+{
+    const long long P = c->P;
+    q8c_prepare(c, v, ov);
+    if (sv_graph_on(c)) {
+        int st = sv_graph_launch(c, c->hv.data());
+        return st != NQES_OK ? st : sv_graph_finish(c);
+    }
     double *dv = static_cast<double *>(c->dv.p);  // 16 (P + 1) bytes
     HIPCHK(c, hipMemcpyAsync(dv, c->hv.data(), 8 * (2 * P + 2), hipMemcpyHostToDevice, c->stream), "copy v");
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
@@ -1047,6 +1075,19 @@ static void mixed_real_result(nqes_ctx *c, cd *out, int part)
         } else {
             out[q].im = r;
         }
+    }
+}
+
+// complex O: out = (1/S) [raw - conj(o) raw[P]] + skk .* v from the raw sums in svraw
+static void cplx_result(nqes_ctx *c, const cd *v, cd *out)
+{
+    using namespace nq64;
+    const long long P = c->P;
+    const double Sd = (double)c->S;
+    const cd wsum = c->svraw[P];
+    for (long long q = 0; q < P; q++) {
+        cd r = divr(sub(c->svraw[q], mul(conj(c->o[q]), wsum)), Sd);
+        out[q] = add(r, mul(c->skk[q], v[q]));
     }
 }
 
@@ -1112,11 +1153,7 @@ static int apply_S(nqes_ctx *c, const cd *v, cd *out)
     if (c->cplx) {
         int st = c->q8c ? sv_q8c_device(c, v, ov) : sv_mixed_device(c, v, &ov, 16);
         if (st != NQES_OK) return st;
-        const cd wsum = c->svraw[P];
-        for (long long q = 0; q < P; q++) {
-            cd r = divr(sub(c->svraw[q], mul(conj(c->o[q]), wsum)), Sd);
-            out[q] = add(r, mul(c->skk[q], v[q]));
-        }
+        cplx_result(c, v, out);
         return NQES_OK;
     }
     // real O and o: apply to the real and imaginary parts of v separately
@@ -1144,12 +1181,21 @@ struct SvOperator {
     {
         v = &vv;
         split = false;
-        if (c->dense || c->prec != NQES_PREC_MIXED || c->cplx || !sv_graph_on(c)) return true;
+        if (c->dense || c->prec != NQES_PREC_MIXED || !sv_graph_on(c)) return true;
         const long long P = c->P;
+        if (c->q8c) {  // complex weights: Re v | Im v | o^T v
+            cd ov = nq64::zero();
+            for (long long q = 0; q < P; q++) ov = nq64::add(ov, nq64::mul(c->o[q], vv[q]));
+            q8c_prepare(c, vv.data(), ov);
+            status = sv_graph_launch(c, c->hv.data());
+            split = status == NQES_OK;
+            return split;
+        }
         for (long long q = 0; q < P; q++)
             if (vv[q].im != 0.0) return true;
-        const double ovp = mixed_real_prepare(c, vv.data(), 0);
-        status = sv_graph_launch(c, c->hv.data(), ovp);
+        c->hv.resize(P + 1);
+        c->hv[P] = mixed_real_prepare(c, vv.data(), 0);
+        status = sv_graph_launch(c, c->hv.data());
         split = status == NQES_OK;
         return split;
     }
@@ -1162,7 +1208,8 @@ struct SvOperator {
         split = false;
         status = sv_graph_finish(c);
         if (status != NQES_OK) return false;
-        mixed_real_result(c, out.data(), 0);
+        if (c->q8c) cplx_result(c, v->data(), out.data());
+        else mixed_real_result(c, out.data(), 0);
         return true;
     }
 };

@@ -262,13 +262,15 @@ def test_q8_forward_passes_agree(nqes, monkeypatch, N, M, K, C, m, cplx):
         assert rel_err(a, b) < 1e-11, rel_err(a, b)
 
 
-def test_q8_graph_replays_equal_eager(nqes, monkeypatch):
-    """The int8 S*v runs as a captured HIP graph (one per sample set): every replay,
+@pytest.mark.parametrize("cplx", [False, True])
+def test_q8_graph_replays_equal_eager(nqes, monkeypatch, cplx):
+    """The int8 S*v runs as a captured HIP graph (one per sample set; for complex
+    weights the two forward and two backward passes over the split rows): every replay,
     with eager work and synchronisations between replays and a second sample set
     (graph rebuilt), equals the eagerly launched sequence (NQES_GRAPH=0) bit for bit."""
     N, M, K, C, m = 16, 32, 2, 64, 8
     J = load_J(16)
-    W = random_weights(N, M, K, seed=41, scale=0.1, complex_=False)
+    W = random_weights(N, M, K, seed=41, scale=0.1, complex_=cplx)
     P = K * (N + M + N * M)
     rng = np.random.default_rng(3)
     vs = [rng.normal(size=P) + 0j for _ in range(4)] + [rng.normal(size=P) + 1j * rng.normal(size=P)]
