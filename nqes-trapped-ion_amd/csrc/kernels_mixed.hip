@@ -17,8 +17,11 @@
 //                    FP64 accumulation in registers; next row prefetched, nontemporal loads
 //   k_sv_rowdot_mx / k_sv_colacc_mx   two-pass S*v when v does not fit in LDS
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <stdint.h>
 #include <stdlib.h>
+#include <string.h>
 
 #include "cplx64.hpp"
 #include "kernels.hpp"
@@ -129,6 +132,100 @@ __global__ __launch_bounds__(64) void k_theta_gemm(int N, int M, int K, int NW, 
         for (int q = 0; q < 4; q++) {
             long long orow = r0 + 4 * (lane >> 4) + q;
             if (orow < rows) theta0[orow * ncol + ocol] = acc[q] + bias;
+        }
+    }
+}
+
+// Theta0 = Sigma W + b on BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation.
+// W (FP32) is split into three bf16 terms w1 + w2 + w3: w1 = bf16(W), w2 = bf16(W - w1),
+// w3 = bf16(W - w1 - w2); each residual holds at most the 16 and 8 significant bits left,
+// so it is exact in FP32 and w3 is exact in bf16: W = w1 + w2 + w3 exactly (normal
+// numbers).  The spins are +-1 (exact in bf16), so the MFMAs sum exact products in FP32,
+// as the FP32 form does, in another order.  Block: 4 wavefronts over a 64-row x 64-column
+// output tile; the three terms of the block's 64 columns in LDS as bf16, [term][col][k]
+// with k contiguous (N padded to a multiple of 32 with zeros, rows padded by 8 bf16 so
+// that the 16 lanes of a group read different banks).  Wavefront w: rows 16 w .. 16 w +
+// 15 of the tile, its 4 column tiles.  Operands: lane l, element e = K index 8 (l/16) + e
+// of a 32-wide step (the same bijection for A and B); result register e of lane l = row
+// 4 (l/16) + e, column l % 16.
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+__device__ inline uint16_t bf16_rne(float x)
+{
+    const uint32_t u = __float_as_uint(x);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ __launch_bounds__(256) void k_theta_bf16(int N, int M, int K, int NW, int R, long long rows,
+                                                   const uint32_t *__restrict__ spins0,
+                                                   const float *__restrict__ wT, const float *__restrict__ bT,
+                                                   float *__restrict__ theta0)
+{
+    extern __shared__ __align__(16) uint16_t wl[];  // [3][64][NPP]
+    const int NP = (N + 31) & ~31, NPP = NP + 8;
+    const long long ncol = (long long)K * M * R;
+    const long long c0 = (long long)blockIdx.y * 64;
+    for (int q = threadIdx.x; q < 64 * NP; q += blockDim.x) {
+        const int i = q >> 6, cl = q & 63;  // consecutive threads: consecutive columns of W row i
+        const long long col = c0 + cl;
+        float w = 0.f;
+        if (col < ncol && i < N) {
+            const long long e = col / R;
+            const int part = (int)(col % R), kk = (int)(e / M), j = (int)(e % M);
+            w = wT[(((long long)kk * N + i) * M + j) * R + part];
+        }
+        const uint16_t h1 = bf16_rne(w);
+        const float r1 = w - bf16_to_f(h1);
+        const uint16_t h2 = bf16_rne(r1);
+        const uint16_t h3 = bf16_rne(r1 - bf16_to_f(h2));
+        wl[(0 * 64 + cl) * NPP + i] = h1;
+        wl[(1 * 64 + cl) * NPP + i] = h2;
+        wl[(2 * 64 + cl) * NPP + i] = h3;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = lane >> 4, lo = lane & 15;
+    // row tiles rt0, rt0 + gridDim.x, ... of 64 rows share the staged W terms
+    for (long long rtile = blockIdx.x; rtile * 64 < rows; rtile += gridDim.x) {
+        const long long r0 = rtile * 64;
+        const long long arow = r0 + 16 * wv + lo;
+        const bool aok = arow < rows;
+        const uint32_t *sp = spins0 + (aok ? arow : 0) * NW;
+        v4f acc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) acc[ct] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < NP; k0 += 32) {
+            const int i0 = k0 + 8 * g;  // this lane's 8 spins
+            const uint32_t word = (aok && i0 < N) ? sp[i0 >> 5] : 0u;
+            union {
+                uint16_t h[8];
+                v8bf v;
+            } a;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int i = i0 + e;
+                a.h[e] = (aok && i < N) ? (((word >> (i & 31)) & 1u) ? (uint16_t)0xBF80 : (uint16_t)0x3F80) : (uint16_t)0;
+            }
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+                for (int t = 0; t < 3; t++) {
+                    const v8bf b = *(const v8bf *)(wl + (t * 64 + 16 * ct + lo) * NPP + i0);
+                    acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b, acc[ct], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) {
+            const long long ocol = c0 + 16 * ct + lo;
+            if (ocol >= ncol) continue;
+            const long long e = ocol / R;
+            const int part = (int)(ocol % R), kk = (int)(e / M), j = (int)(e % M);
+            const float bias = bT[((long long)kk * M + j) * R + part];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const long long orow = r0 + 16 * wv + 4 * g + q;
+                if (orow < rows) theta0[orow * ncol + ocol] = acc[ct][q] + bias;
+            }
         }
     }
 }
@@ -455,6 +552,27 @@ hipError_t nq_launch_theta_gemm(bool cplx, int N, int M, int K, int NW, int C, c
     int R = cplx ? 2 : 1;
     long long rows = (long long)C * K;
     long long ncol = (long long)K * M * R;
+    const char *e = getenv("NQES_THETA");  // "fp32": the FP32-MFMA form
+    if (!(e && strcmp(e, "fp32") == 0)) {
+        const int NPP = ((N + 31) & ~31) + 8;
+        // the W terms (101 KB at N = 256) or the output tile, whichever is larger
+        const size_t lds = (size_t)3 * 64 * NPP * 2;  // 101 KB at N = 256
+        if (lds > 65536) {
+            hipError_t ea = hipFuncSetAttribute((const void *)k_theta_bf16, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                (int)lds);
+            if (ea != hipSuccess) return ea;
+        }
+        // about 2048 blocks: a block loops over row tiles so that large sizes stage each
+        // W slice once per several tiles
+        const long long cb = (ncol + 63) / 64, rtiles = (rows + 63) / 64;
+        long long rb = 2048 / cb;
+        if (rb < 1) rb = 1;
+        if (rb > rtiles) rb = rtiles;
+        dim3 grid((unsigned)rb, (unsigned)cb);
+        hipLaunchKernelGGL(k_theta_bf16, grid, dim3(256), lds, st, N, M, K, NW, R, rows, spins0, (const float *)wT,
+                           (const float *)bT, (float *)theta0);
+        return hipGetLastError();
+    }
     dim3 grid((unsigned)((rows + 15) / 16), (unsigned)((ncol + 15) / 16));
     hipLaunchKernelGGL(k_theta_gemm, grid, dim3(64), 0, st, N, M, K, NW, R, rows, spins0, (const float *)wT,
                        (const float *)bT, (float *)theta0);

@@ -335,3 +335,33 @@ def test_mixed_sr_step_matches_fp64(nqes, cplx):
     assert abs(int(sa["accepts"]) - int(sb["accepts"])) <= max(3, sa["accepts"] // 1000)
     if not cplx:
         assert not np.any(Wb.imag)
+
+
+@pytest.mark.parametrize("N,M,K,C,cplx", [(8, 16, 2, 64, True), (32, 64, 2, 128, False), (40, 80, 2, 64, True),
+                                          (64, 128, 4, 32, True), (17, 34, 3, 50, False)])
+def test_theta_gemm_forms_agree(nqes, monkeypatch, N, M, K, C, cplx):
+    """The chains' starting angles Theta0 = Sigma W + b on BF16 MFMA (W split exactly into
+    three bf16 terms, FP32 accumulation; the default) and on FP32 MFMA (NQES_THETA=fp32)
+    are the same exact products summed in FP32 in another order, so they differ by a few
+    ulps; the first sweep's amplitude ratios then differ by ~1e-7 and decide a proposal
+    differently only when a draw falls that close: the sampled spins and accept counts
+    agree on at least 99 % of the chains and H_loc on the chains that agree to 1e-5."""
+    print(f"case theta N={N} M={M} K={K} C={C} complex={cplx}", flush=True)
+    J = load_J(N)
+    # weights of scale 0.1: at 0.3 the FP32 angle arithmetic itself (either theta form)
+    # already decides many proposals differently from FP64 at N = 64, M = 128
+    W = random_weights(N, M, K, seed=3 + N, scale=0.1, complex_=cplx)
+    out = {}
+    for form in ("bf16", "fp32"):
+        if form == "fp32":
+            monkeypatch.setenv("NQES_THETA", "fp32")
+        else:
+            monkeypatch.delenv("NQES_THETA", raising=False)
+        with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+            s.sample(W, 2024, C, 4, 1, 0.5)
+            out[form] = s.samples()
+    (sa, Ha, aa), (sb, Hb, ab) = out["bf16"], out["fp32"]
+    same_chain = np.all((sa == sb).reshape(C, -1), axis=1) & (aa == ab)
+    assert same_chain.mean() >= 0.99, same_chain.mean()
+    rows = np.repeat(same_chain, 4)
+    assert rel_err(Ha[rows], Hb[rows]) < 1e-5
