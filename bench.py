@@ -242,7 +242,7 @@ def converged_block(args, nqes):
     progress(f"N=32 fp64 first {nf} steps: {rf['wall_s']:.1f}s")
     rel = np.abs(rm["hist"][:nf] - rf["hist"][:nf]) / np.abs(rf["hist"][:nf])
     out["N32"] = {"workload": "N=32 M=64 K=2 h=0.595 C=4096 m=256 B=64, tol 1e-4, at most 200 SR steps",
-                  "mixed": summary(rm),
+                  "mixed": dict(summary(rm), energies_per_step=rm["hist"][: rm["steps"]].tolist()),
                   "fp64_first_steps": {"steps": nf, "ms_per_step": rf["ms_per_step"],
                                        "minres_iters": rf["minres_iters"],
                                        "mixed_ms_per_step": rm["ms_per_step"][:nf],

@@ -392,3 +392,18 @@ def test_correlation_operators_rayleigh_ritz(oracle):
     x, z, xx, zz = oracle.correlations(N, M, K, 0.5, synthetic_J(N), W, 5, 3, 6, 2)
     assert np.abs(xx[np.arange(N), np.arange(N)] - np.eye(K)).max() < 1e-12
     assert np.array_equal(xx, xx.transpose(1, 0, 2, 3)) and np.array_equal(zz, zz.transpose(1, 0, 2, 3))
+
+
+def test_sample_chains_equals_run_chain(oracle):
+    """orc_sample_chains_model (chains in parallel) returns exactly the per-chain
+    orc_run_chain_model results, in chain order."""
+    from helpers import load_J, random_weights
+    N, M, K, C, m, B = 8, 16, 2, 7, 4, 3
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=12, scale=0.15)
+    a = oracle.sample_chains(N, M, K, 0.595, J, W, 31, C, m, B, want_O=True)
+    per = [oracle.run_chain(N, M, K, 0.595, J, W, 31, c, m, B, want_O=True) for c in range(C)]
+    assert np.array_equal(a["spins"], np.concatenate([r["spins"] for r in per]))
+    assert np.array_equal(a["H"], np.concatenate([r["H"] for r in per]))
+    assert np.array_equal(a["O"], np.concatenate([r["O"] for r in per]))
+    assert list(a["accepts"]) == [r["accepts"] for r in per]
