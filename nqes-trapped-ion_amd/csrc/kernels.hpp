@@ -127,12 +127,6 @@ struct SvfArgs {
     void *vq;                  // int8 form: v operands, column scales and biases
     void *q8w;                 // int8 form: backward block scales and weight digits
     void *q8m;                 // int8 form: spin masks of the backward A operand
-    // Complex weights on the int8 form: the factor rows are split into real rows (row
-    // 2 r = Re U_r, row 2 r + 1 = Im U_r, both with replica r's spins), so K above counts
-    // the 2 K rows of a sample and kc is the K of the column layout (0: kc = K).
-    int kc;
-    const double *wa, *wb;     // backward weights of those rows: wa[s] (even), sb wb[s] (odd)
-    double sb;
 };
 // ok = 0: the factored form does not apply (N > 64).  The forward pass splits the
 // Ucols/16 column tiles into fwd_cgs groups of fwd_cgt (B of a group fits in LDS); the
@@ -160,13 +154,16 @@ hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void 
 // its forward pass as one GEMM over the columns (sh.gemm; sv_q8_gemm.hip): w_s directly for
 // K = 1, 2, 4 unless rows, else the row sums wrow[r] (a.ncg = 1)
 hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows = false);
-// complex weights on the int8 form (a over the 2 S kc split rows, see SvfArgs::kc): split
-// the complex factor rows Uc and replica spins into a.U / a.Umax / a.spins and quantise;
-// then per call out[0..P] (real parts) and out[P+1..2P+1] (imaginary parts) of the raw
-// sums, a.w holding w_s planar (re | im), a.wrow 2 ncg R row sums
-hipError_t nq_q8c_pack(const SvfArgs &a, const Q8Shape &sh, const void *Uc, const uint32_t *spins, hipStream_t st);
-hipError_t nq_q8c_apply(const SvfArgs &a, const Q8Shape &sh, const void *vre, const void *vim, const void *ovc,
-                        void *scratch, void *out, hipStream_t st);
+// Complex weights on the int8 form (three real passes per product, Gauss's form): the
+// complex factor rows a.U (complex float) are quantised with one scale per row into the
+// digits of X_R = Re, X_I = Im and X_A = X_R + X_I (exact integers), three slices of
+// a.Uq / a.Uf (slice t at t * sh.rpad * Ucols * 3 / t * sh.uf_bytes bytes), masks once.
+hipError_t nq_q8c_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
+// per call: vin = v_A | v_R | v_I (P doubles each: Re v, Im v - Re v, Re v + Im v) | o^T v
+// (re, im) | 0, 0; w3: 3 S doubles; out: the three backward runs' raw sums, (P + 1)
+// doubles each (A, R, I); Re raw = A - R - I, Im raw = R - I (p < P), raw[P] = (I[P], R[P])
+hipError_t nq_q8c_apply(const SvfArgs &a, const Q8Shape &sh, const double *vin, double *w3, void *scratch, double *out,
+                        hipStream_t st);
 // moments from the quantised factors (sums with w = 1 and w = E on the int8 backward
 // pass, |O|^2 on FP64 MFMA); after nq_q8_pack
 hipError_t nq_q8_moments(const SvfArgs &a, const Q8Shape &sh, const SvfShape &svf, void *part, void *scratch,

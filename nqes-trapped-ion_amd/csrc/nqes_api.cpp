@@ -104,7 +104,8 @@ struct nqes_ctx {
     DevBuf dU, dwrow, dsvfpart, dsvfscratch, dsvftmp;  // factored S*v and moments (mixed path)
     DevBuf dUq, descale, dvq, dq8w, dq8m;              // its int8 form (real weights)
     DevBuf dUf;                                        // row-major digits (GEMM forward pass)
-    DevBuf dUr, dspins2;                               // complex weights, int8 form: split rows
+    DevBuf dw3, dvin3, dsvout3;                        // complex weights, int8 form: per-set sample sums,
+                                                       // operands in, the three raw sums out
     DevBuf dUmax;                                      // max |U| per factor row
     hipGraph_t svg = nullptr;                          // int8 S*v sequence, captured once per
     hipGraphExec_t svx = nullptr;                      // sample set and replayed per MINRES step
@@ -250,12 +251,11 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
         b += (long long)sh.fwd_cgs * S * K * 16;
         b += std::max((long long)sh.bwd_blocks * 16, nchunk * 8) * (P + 1);
         b += (nchunk / 32 + 2) * (P + 1) * 16 + 3 * (P + 1) * 16;
-        // int8 form; complex weights split each factor row into two real rows
-        const Q8Shape q8 = nq_q8_shape(N, M, K, 2 * S * K);
+        // int8 form; complex weights keep three digit sets per row
+        const Q8Shape q8 = nq_q8_shape(N, M, K, S * K);
         if (q8.ok)
-            b += q8.rpad * (long long)q8.ucols * 3 + q8.rpad * 12 +
-                 (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes + q8.uf_bytes) +
-                 2 * S * K * ((long long)q8.ucols * 4 + NW * 4) + (long long)q8.fwd_cgs * S * K * 32;
+            b += 3 * (q8.rpad * (long long)q8.ucols * 3 + (long long)q8.uf_bytes) + q8.rpad * 12 +
+                 (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes) + 3 * S * 8 + (3 * P + 4) * 8 + 3 * (P + 1) * 8;
     } else {
         b += S * Pld * (long long)o_elem(precision, cplx);  // O
         if (mixed) b += (mch / 32 + 2) * P * 16;              // moments scratch
@@ -407,15 +407,11 @@ static SvfArgs svf_args(nqes_ctx *c, long long S)
     return f;
 }
 
-// the int8 form's arguments for complex weights: the 2 S K split real rows (row 2 r = Re
-// U_r, 2 r + 1 = Im U_r), K = 2 K rows per sample over the column layout of K
+// the int8 form's arguments for complex weights (the complex factor rows in U, the three
+// digit sets in slices of Uq / Uf)
 static SvfArgs q8c_args(nqes_ctx *c)
 {
     SvfArgs f = svf_args(c, c->S);
-    f.K = 2 * c->K;
-    f.kc = c->K;
-    f.U = c->dUr.p;
-    f.spins = static_cast<const uint32_t *>(c->dspins2.p);
     f.Umax = (const float *)c->dUmax.p;
     f.Ucols = c->q8_shape.ucols;
     return f;
@@ -493,9 +489,7 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
                    "factored moments kernels");
             if (c->q8c) {
                 c->S = S;  // q8c_args sizes its rows by the sample count
-                HIPCHK(c, nq_q8c_pack(q8c_args(c), c->q8_shape, c->dU.p, static_cast<const uint32_t *>(c->dspins.p),
-                                      c->stream),
-                       "complex factor quantisation kernels");
+                HIPCHK(c, nq_q8c_pack(q8c_args(c), c->q8_shape, c->stream), "complex factor quantisation kernels");
             }
         }
         return NQES_OK;
@@ -550,8 +544,8 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
         if (c->svf && !cplx) {
             c->q8_shape = nq_q8_shape(c->N, c->M, K, S * K);
             c->q8 = c->q8_shape.ok && !fp64mfma;
-        } else if (c->svf) {  // complex: the int8 form over the 2 S K split real rows
-            c->q8_shape = nq_q8_shape(c->N, c->M, K, 2 * S * K);
+        } else if (c->svf) {  // complex: the int8 form with three digit sets per row
+            c->q8_shape = nq_q8_shape(c->N, c->M, K, S * K);
             c->q8c = c->q8_shape.ok && !fp64mfma;
         }
     }
@@ -566,18 +560,20 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
             nchunk = std::max(nchunk, (long long)c->q8_shape.bwd_blocks);
             cgs = std::max(cgs, (long long)c->q8_shape.fwd_cgs);
             const long long rp = c->q8_shape.rpad;
-            if (c->q8c) {  // split rows, their spins and row maxima; 2 runs x cgs x 2 S K row sums
-                HIPCHK(c, c->dUr.ensure((size_t)2 * S * K * c->q8_shape.ucols * 4), "alloc split factor rows");
-                HIPCHK(c, c->dspins2.ensure((size_t)2 * S * K * c->NW * 4), "alloc split spins");
-                cgs = std::max(cgs, (long long)2 * c->q8_shape.fwd_cgs);  // x 2 S K doubles = cgs S K 16 B
+            const size_t nsets = c->q8c ? 3 : 1;  // complex: digit sets X_A, X_R, X_I
+            if (c->q8c) {
+                HIPCHK(c, c->dw3.ensure((size_t)3 * S * 8), "alloc S*v sample sums");
+                HIPCHK(c, c->dvin3.ensure((size_t)(3 * P + 4) * 8), "alloc S*v operands");
+                HIPCHK(c, c->dsvout3.ensure((size_t)3 * (P + 1) * 8), "alloc S*v out");
             }
-            HIPCHK(c, c->dUq.ensure((size_t)rp * c->q8_shape.ucols * 3), "alloc quantised factors");
+            HIPCHK(c, c->dUq.ensure(nsets * rp * c->q8_shape.ucols * 3), "alloc quantised factors");
             HIPCHK(c, c->descale.ensure((size_t)rp * 8), "alloc factor scales");
             HIPCHK(c, c->dUmax.ensure((size_t)rp * 4), "alloc factor row maxima");
             HIPCHK(c, c->dvq.ensure(c->q8_shape.vq_bytes), "alloc S*v operands");
             HIPCHK(c, c->dq8w.ensure(c->q8_shape.w_bytes), "alloc S*v weight digits");
             HIPCHK(c, c->dq8m.ensure(c->q8_shape.m_bytes), "alloc spin masks");
-            if (c->q8_shape.gemm) HIPCHK(c, c->dUf.ensure(c->q8_shape.uf_bytes), "alloc row-major factor digits");
+            if (c->q8_shape.gemm)
+                HIPCHK(c, c->dUf.ensure(nsets * c->q8_shape.uf_bytes), "alloc row-major factor digits");
         }
         HIPCHK(c, c->dU.ensure((size_t)S * K * c->svf_shape.ucols * tsz), "alloc S*v factors");
         HIPCHK(c, c->dwrow.ensure((size_t)cgs * S * K * 16), "alloc S*v rows");
@@ -878,10 +874,21 @@ int nqes_get_moments(nqes_ctx *c, double *o, double *skk, double *f, double *elo
 // memory, the kernels of nq_q8_apply, the raw result back to pinned memory (no memset
 // nodes).  NQES_GRAPH=0 launches the same sequence eagerly instead.
 // doubles in and out of one graphed S*v: real weights v | o^T v in and the P + 1 raw
-// sums out; complex weights (q8c) Re v | Im v | o^T v in and the real then imaginary
-// parts of the P + 1 raw sums out
-static long long sv_in_len(const nqes_ctx *c) { return c->q8c ? 2 * c->P + 2 : c->P + 1; }
-static long long sv_out_len(const nqes_ctx *c) { return c->q8c ? 2 * (c->P + 1) : c->P + 1; }
+// sums out; complex weights (q8c) v_A | v_R | v_I | o^T v | 0, 0 in and the three
+// backward runs' P + 1 raw sums out
+static long long sv_in_len(const nqes_ctx *c) { return c->q8c ? 3 * c->P + 4 : c->P + 1; }
+static long long sv_out_len(const nqes_ctx *c) { return c->q8c ? 3 * (c->P + 1) : c->P + 1; }
+
+// complex weights: the raw sums sum_s conj(O_s) w_s (p < P) and sum_s w_s (p = P) from
+// the three backward runs A, R, I (Gauss's form): Re = A - R - I, Im = R - I; at P the
+// runs' weight sums, Re w over set I and Im w over set R
+static void q8c_raw(nqes_ctx *c, const double *r)
+{
+    const long long P = c->P;
+    const double *ra = r, *rr = r + (P + 1), *ri = r + 2 * (P + 1);
+    for (long long q = 0; q < P; q++) c->svraw[q] = nq64::mk((ra[q] - rr[q]) - ri[q], rr[q] - ri[q]);
+    c->svraw[P] = nq64::mk(ri[P], rr[P]);
+}
 
 static int build_sv_graph(nqes_ctx *c)
 {
@@ -900,14 +907,15 @@ static int build_sv_graph(nqes_ctx *c)
     // v buffer holds 16 (P + 1) bytes, room for 2 P + 2 doubles)
     SvfArgs f = svf_args(c, c->S);
     f.ov = (const double *)c->dv.p + P;
-    double *dv = static_cast<double *>(c->dv.p);
+    void *din = c->q8c ? c->dvin3.p : c->dv.p, *dout = c->q8c ? c->dsvout3.p : c->dsvout.p;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin S*v capture");
-    hipError_t e = hipMemcpyAsync(c->dv.p, c->hv_pin, 8 * nin, hipMemcpyHostToDevice, c->stream);
+    hipError_t e = hipMemcpyAsync(din, c->hv_pin, 8 * nin, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
-        e = c->q8c ? nq_q8c_apply(q8c_args(c), c->q8_shape, dv, dv + P, dv + 2 * P, c->dsvfscratch.p, c->dsvout.p,
+        e = c->q8c ? nq_q8c_apply(q8c_args(c), c->q8_shape, static_cast<const double *>(din),
+                                  static_cast<double *>(c->dw3.p), c->dsvfscratch.p, static_cast<double *>(dout),
                                   c->stream)
-                   : nq_q8_apply(f, c->q8_shape, c->dsvfscratch.p, c->dsvout.p, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->hr_pin, c->dsvout.p, 8 * nout, hipMemcpyDeviceToHost, c->stream);
+                   : nq_q8_apply(f, c->q8_shape, c->dsvfscratch.p, dout, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->hr_pin, dout, 8 * nout, hipMemcpyDeviceToHost, c->stream);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(c->stream, &g);
     if (e != hipSuccess || e2 != hipSuccess) {
@@ -939,10 +947,8 @@ static int sv_graph_finish(nqes_ctx *c)
 {
     HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
     const long long P = c->P;
-    if (c->q8c)
-        for (long long q = 0; q <= P; q++) c->svraw[q] = nq64::mk(c->hr_pin[q], c->hr_pin[P + 1 + q]);
-    else
-        memcpy(c->svraw.data(), c->hr_pin, 8 * (P + 1));
+    if (c->q8c) q8c_raw(c, c->hr_pin);
+    else memcpy(c->svraw.data(), c->hr_pin, 8 * (P + 1));
     c->sv_ms += event_ms(c->ev[2], c->ev[3]);
     c->sv_calls++;
     return NQES_OK;
@@ -1008,19 +1014,21 @@ static int sv_fp64_device(nqes_ctx *c, const void *v, const void *ov, size_t dsi
     return NQES_OK;
 }
 
-// device part of the complex-weight S*v on the int8 form: Re v | Im v | o^T v to the
-// device in one copy, the kernels of nq_q8c_apply, the raw sums back into svraw
-// Re v | Im v | o^T v into hv (the graphed and the eager form read it alike)
+// the complex-weight S*v's operands into hv (the graphed and the eager form read it
+// alike): v_A = Re v, v_R = Im v - Re v, v_I = Re v + Im v, o^T v, two zeros
 static void q8c_prepare(nqes_ctx *c, const cd *v, cd ov)
 {
     const long long P = c->P;
-    c->hv.resize(2 * P + 2);
+    c->hv.resize(3 * P + 4);
     for (long long q = 0; q < P; q++) {
         c->hv[q] = v[q].re;
-        c->hv[P + q] = v[q].im;
+        c->hv[P + q] = v[q].im - v[q].re;
+        c->hv[2 * P + q] = v[q].re + v[q].im;
     }
-    c->hv[2 * P] = ov.re;
-    c->hv[2 * P + 1] = ov.im;
+    c->hv[3 * P] = ov.re;
+    c->hv[3 * P + 1] = ov.im;
+    c->hv[3 * P + 2] = 0.0;
+    c->hv[3 * P + 3] = 0.0;
 }
 
 static int sv_q8c_device(nqes_ctx *c, const cd *v, cd ov)
@@ -1031,16 +1039,17 @@ static int sv_q8c_device(nqes_ctx *c, const cd *v, cd ov)
         int st = sv_graph_launch(c, c->hv.data());
         return st != NQES_OK ? st : sv_graph_finish(c);
     }
-    double *dv = static_cast<double *>(c->dv.p);  // 16 (P + 1) bytes
-    HIPCHK(c, hipMemcpyAsync(dv, c->hv.data(), 8 * (2 * P + 2), hipMemcpyHostToDevice, c->stream), "copy v");
+    double *din = static_cast<double *>(c->dvin3.p);
+    HIPCHK(c, hipMemcpyAsync(din, c->hv.data(), 8 * (3 * P + 4), hipMemcpyHostToDevice, c->stream), "copy v");
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
-    HIPCHK(c, nq_q8c_apply(q8c_args(c), c->q8_shape, dv, dv + P, dv + 2 * P, c->dsvfscratch.p, c->dsvout.p, c->stream),
+    HIPCHK(c, nq_q8c_apply(q8c_args(c), c->q8_shape, din, static_cast<double *>(c->dw3.p), c->dsvfscratch.p,
+                           static_cast<double *>(c->dsvout3.p), c->stream),
            "complex int8 factored S*v kernels");
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
-    std::vector<double> raw(2 * (P + 1));
-    HIPCHK(c, hipMemcpyAsync(raw.data(), c->dsvout.p, 8 * 2 * (P + 1), hipMemcpyDeviceToHost, c->stream), "copy S*v");
+    std::vector<double> raw(3 * (P + 1));
+    HIPCHK(c, hipMemcpyAsync(raw.data(), c->dsvout3.p, 8 * 3 * (P + 1), hipMemcpyDeviceToHost, c->stream), "copy S*v");
     HIPCHK(c, hipStreamSynchronize(c->stream), "S*v sync");
-    for (long long q = 0; q <= P; q++) c->svraw[q] = nq64::mk(raw[q], raw[P + 1 + q]);
+    q8c_raw(c, raw.data());
     c->sv_ms += event_ms(c->ev[2], c->ev[3]);
     c->sv_calls++;
     return NQES_OK;

@@ -72,7 +72,11 @@ __device__ inline double xor_sum16(double x)
 // Pack: U (FP32, [R][UC] row-major) -> Uq tiles (and, when Uf is given, the same digits
 // row-major for the GEMM forward pass) and escale[r] = 1 / f_r, the scale from the row
 // maxima Umax when given (else a first pass over U).  One wavefront per 16-row tile,
-// rows [R, Rp) written as zeros.
+// rows [R, Rp) written as zeros.  Complex factor rows (CM = 1, 2, 3: U holds (re, im)
+// float pairs, Umax the row maxima of |re| + |im|): the digits of X_R = rint(re f_r),
+// X_I = rint(im f_r) or X_A = X_R + X_I, with f_r = (XMAX - 1) / 2^e_r so that
+// |X_A| <= XMAX.
+template <int CM>
 __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, long long ntile, int UC, uint32_t *Uq,
                                                  double *escale, uint4 *Uf, const float *Umax)
 {
@@ -93,7 +97,14 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const long long r = rt * 16 + 4 * g + j;
-                    if (r < R) mx[j] = fmaxf(mx[j], fabsf(U[r * UC + 16 * ct + lo]));
+                    if (r < R) {
+                        if constexpr (CM == 0) {
+                            mx[j] = fmaxf(mx[j], fabsf(U[r * UC + 16 * ct + lo]));
+                        } else {
+                            const float2 u = ((const float2 *)U)[r * UC + 16 * ct + lo];
+                            mx[j] = fmaxf(mx[j], fabsf(u.x) + fabsf(u.y));
+                        }
+                    }
                 }
         }
         double sc[4];
@@ -103,7 +114,7 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
             for (int off = 1; off < 16; off <<= 1) mx[j] = fmaxf(mx[j], __shfl_xor(mx[j], off));
             int e = 0;
             if (mx[j] > 0.f) (void)frexpf(mx[j], &e);  // mx < 2^e
-            sc[j] = ldexp(XMAX, -e);
+            sc[j] = ldexp(CM ? XMAX - 1.0 : XMAX, -e);
             const long long r = rt * 16 + 4 * g + j;
             if (lo == 0) escale[r] = (r < R && mx[j] > 0.f) ? 1.0 / sc[j] : 0.0;
         }
@@ -112,8 +123,15 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const long long r = rt * 16 + 4 * g + j;
-                const float u = (r < R) ? U[r * UC + 16 * ct + lo] : 0.f;
-                long long x = (long long)rint((double)u * sc[j]);
+                long long x;
+                if constexpr (CM == 0) {
+                    const float u = (r < R) ? U[r * UC + 16 * ct + lo] : 0.f;
+                    x = (long long)rint((double)u * sc[j]);
+                } else {
+                    const float2 u = (r < R) ? ((const float2 *)U)[r * UC + 16 * ct + lo] : make_float2(0.f, 0.f);
+                    const long long xr = (long long)rint((double)u.x * sc[j]), xi = (long long)rint((double)u.y * sc[j]);
+                    x = CM == 1 ? xr : (CM == 2 ? xi : xr + xi);
+                }
 #pragma unroll
                 for (int t = 0; t < QU; t++) {
                     const long long d = (t < QU - 1) ? bal8l(x) : x;
@@ -153,7 +171,7 @@ __global__ void k_q8_prep_v(SvfArgs a, int nsp, uint8_t *vfrag, double *colscale
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int UC = a.Ucols;
     if (col >= UC) return;
-    const int N = a.N, M = a.M, K = a.kc ? a.kc : a.K;
+    const int N = a.N, M = a.M, K = a.K;
     const long long KM = (long long)K * M;
     const double *v = (const double *)a.v;
     auto Bval = [&](int i) -> double {
@@ -326,8 +344,7 @@ __global__ void k_q8_masks(const uint32_t *spins, long long R, int N, int NW, in
 }
 
 // Row weights of the backward pass, per call: omega_r = wt(s) 2^(e_r - 30) with
-// wt = w_s, 1 or E_s (WSEL 0, 1, 2), or for split complex rows wa[s] on even rows and
-// sb wb[s] on odd ones (WSEL 3).  Block b (one per backward block): G_b from
+// wt = w_s, 1 or E_s (WSEL 0, 1, 2).  Block b (one per backward block): G_b from
 // max |omega| over its rows (gblk[b]), then per chunk the digit dwords
 // pd[(chunk QW + u) 4 + g] = uint4 over q of Omega = rint(omega 2^(46 - G_b)) (digit u,
 // least significant first); and part[b][P] = sum of wt(s) over its rows with r % K == 0.
@@ -346,8 +363,7 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
     auto wt = [&](long long r) -> double {
         if constexpr (WSEL == 0) return w[r / K];
         else if constexpr (WSEL == 1) return 1.0;
-        else if constexpr (WSEL == 2) return ((const nq64::cd *)a.E)[r / K].re;
-        else return (r & 1) ? a.sb * a.wb[r / K] : a.wa[r / K];  // split complex rows
+        else return ((const nq64::cd *)a.E)[r / K].re;
     };
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double mx = 0.0, sw = 0.0;
@@ -420,7 +436,6 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
                                                 int zw, int nyl)
 {
     const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
-    const int Kc = a.kc ? a.kc : K;  // K of the column layout
     const int NCT = UC / 16;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
@@ -438,7 +453,7 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
     const int zs = blockIdx.z * zw + wv % zw;
     const int i0 = zs * NIT * 16;
     const long long R = a.S * K;
-    const long long KM = (long long)Kc * M;
+    const long long KM = (long long)K * M;
     const long long n64 = (R + 63) / 64;
     long long c0, c1;
     block_chunks(n64, bx, nb, c0, c1);
@@ -513,7 +528,7 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
             if (i < N) {
                 const double val = hsum * osc;
                 if (col < KM) part[(col / M) * a.L + N + M + (long long)i * M + col % M] = val;
-                else if (col < KM + Kc) part[(col - KM) * a.L + i] = val;
+                else if (col < KM + K) part[(col - KM) * a.L + i] = val;
             }
         }
     }
@@ -523,58 +538,36 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
     if (g == 0 && col < KM && zs == 0) part[(col / M) * a.L + N + col % M] = hsum * osc;
 }
 
-// Complex factor rows (R x UC complex float, row r = s K + kk) as real rows for the
-// int8 form: Ur[2 r] = Re U_r, Ur[2 r + 1] = Im U_r, with the row maxima of |.| and the
-// spin words of row r copied to both.  Thread per (row r, column); the row maxima by a
-// 16-lane max over the row's columns (UC is a multiple of 16; 64-thread blocks cover
-// four rows' columns in 16-column strips).
-__global__ void k_q8c_split(const float2 *U, long long R, int UC, int NW, const uint32_t *spins, float *Ur,
-                            float *Umax, uint32_t *spins2)
+// Complex factor rows: Umax[r] = max over columns of |re| + |im| (the scale of the three
+// digit sets of row r).  16 lanes per row.
+__global__ void k_q8c_rowmax(const float2 *U, long long R, int UC, float *Umax)
 {
-    const long long r = (long long)blockIdx.x * 4 + (threadIdx.x >> 4);
+    const long long r = (long long)blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4);
     const int lo = threadIdx.x & 15;
     if (r >= R) return;
-    float mre = 0.f, mim = 0.f;
+    float mx = 0.f;
     for (int c = lo; c < UC; c += 16) {
         const float2 u = U[r * UC + c];
-        Ur[2 * r * UC + c] = u.x;
-        Ur[(2 * r + 1) * UC + c] = u.y;
-        mre = fmaxf(mre, fabsf(u.x));
-        mim = fmaxf(mim, fabsf(u.y));
+        mx = fmaxf(mx, fabsf(u.x) + fabsf(u.y));
     }
 #pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-        mre = fmaxf(mre, __shfl_xor(mre, off));
-        mim = fmaxf(mim, __shfl_xor(mim, off));
-    }
-    if (lo == 0) {
-        Umax[2 * r] = mre;
-        Umax[2 * r + 1] = mim;
-    }
-    for (int w = lo; w < NW; w += 16) {
-        const uint32_t x = spins[r * NW + w];
-        spins2[2 * r * NW + w] = x;
-        spins2[(2 * r + 1) * NW + w] = x;
-    }
+    for (int off = 1; off < 16; off <<= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if (lo == 0) Umax[r] = mx;
 }
 
-// w_s from the two forward runs over the split rows (wa: run on Re v, wb: run on Im v,
-// each ncg column groups' partial sums R apart; rows 2 r, 2 r + 1 of replica row r = s Kc + kk): Re w_s = sum_kk (wa[2 r] - wb[2 r + 1]),
-// Im w_s = sum_kk (wb[2 r] + wa[2 r + 1]), minus o^T v; written planar, wr[s] and wi[s]
-__global__ void k_q8c_combine(const double *wa, const double *wb, long long S, int Kc, int ncg, long long R,
-                              const double *ov, double *wr, double *wi)
+// Gauss's form of w_s = sum_kk U (Y + bias) for complex U and v, from the three forward
+// runs' sample sums k1 = A B(Re v), k2 = R B(Im v - Re v), k3 = I B(Re v + Im v) (A = R + I):
+// Re w_s = k1 - k3 - Re o^T v, Im w_s = k1 + k2 - Im o^T v; written in place as the
+// backward runs' sample weights: set A takes Re w + Im w, set R Im w, set I Re w.
+__global__ void k_q8c_combine3(double *w3, long long S, const double *ov)
 {
     const double ovr = ov[0], ovi = ov[1];
     for (long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x; s < S; s += (long long)gridDim.x * blockDim.x) {
-        double re = 0.0, im = 0.0;
-        for (int g = 0; g < ncg; g++)  // column groups' partial row sums, in group order
-            for (int kk = 0; kk < Kc; kk++) {
-                const long long r = g * R + 2 * (s * Kc + kk);
-                re += wa[r] - wb[r + 1];
-                im += wb[r] + wa[r + 1];
-            }
-        wr[s] = re - ovr;
-        wi[s] = im - ovi;
+        const double k1 = w3[s], k2 = w3[S + s], k3 = w3[2 * S + s];
+        const double wr = (k1 - k3) - ovr, wi = (k1 + k2) - ovi;
+        w3[s] = wr + wi;
+        w3[S + s] = wi;
+        w3[2 * S + s] = wr;
     }
 }
 
@@ -670,25 +663,31 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R)
     return sh;
 }
 
-hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+namespace {
+
+// quantise a.U into a.Uq (and a.Uf for the GEMM form) with digit mode CM; masks too if asked
+template <int CM>
+hipError_t q8_pack_rows(const SvfArgs &a, const Q8Shape &sh, bool masks, hipStream_t st)
 {
     const long long R = a.S * a.K;
     const long long ntile = sh.rpad / 16;
     long long blocks = (ntile + 3) / 4;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(k_q8_pack, dim3((unsigned)blocks), dim3(256), 0, st, (const float *)a.U, R, ntile, a.Ucols,
-                       (uint32_t *)a.Uq, (double *)a.escale, sh.gemm ? (uint4 *)a.Uf : (uint4 *)nullptr,
+    hipLaunchKernelGGL((k_q8_pack<CM>), dim3((unsigned)blocks), dim3(256), 0, st, (const float *)a.U, R, ntile,
+                       a.Ucols, (uint32_t *)a.Uq, (double *)a.escale, sh.gemm ? (uint4 *)a.Uf : (uint4 *)nullptr,
                        a.Umax);
-    const int NI = 16 * nit_of(a.N);
-    const long long n64 = sh.rpad / 64, nthr = n64 * NI * 4;
-    hipLaunchKernelGGL(k_q8_masks, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, a.spins, R, a.N, a.NW, NI,
-                       n64, (uint4 *)a.q8m);
+    if (masks) {
+        const int NI = 16 * nit_of(a.N);
+        const long long n64 = sh.rpad / 64, nthr = n64 * NI * 4;
+        hipLaunchKernelGGL(k_q8_masks, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st, a.spins, R, a.N,
+                           a.NW, NI, n64, (uint4 *)a.q8m);
+    }
     return hipGetLastError();
 }
 
-hipError_t nq_q8_apply(const SvfArgs &a0, const Q8Shape &sh, void *scratch, void *out, hipStream_t st)
+// forward pass: a.w[s] = sum over the sample's rows of X (sigma^T Q + bias) / f - *a.ov
+hipError_t q8_forward(SvfArgs a, const Q8Shape &sh, hipStream_t st)
 {
-    SvfArgs a = a0;
     const int UC = a.Ucols, nct = UC / 16;
     if (sh.gemm) {
         a.ncg = 1;
@@ -698,20 +697,35 @@ hipError_t nq_q8_apply(const SvfArgs &a0, const Q8Shape &sh, void *scratch, void
         a.ncg = sh.fwd_cgs;
         uint8_t *vfrag = (uint8_t *)a.vq;
         double *colscale = (double *)(vfrag + (size_t)nct * QV * 64 * 16);
-        hipLaunchKernelGGL(k_q8_prep_v, dim3((UC + 63) / 64), dim3(64), 0, st, a, sh.nsp, vfrag, colscale, colscale + UC);
+        hipLaunchKernelGGL(k_q8_prep_v, dim3((UC + 63) / 64), dim3(64), 0, st, a, sh.nsp, vfrag, colscale,
+                           colscale + UC);
         const dim3 grid(sh.fwd_blocks, sh.fwd_cgs);
         if (sh.nsp == 32) hipLaunchKernelGGL((k_q8_fwd<32>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
         else hipLaunchKernelGGL((k_q8_fwd<64>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (!(sh.gemm && (a.K == 1 || a.K == 2 || a.K == 4))) {  // else the forward pass wrote w
+    if (!(sh.gemm && (a.K == 1 || a.K == 2 || a.K == 4)))  // else the forward pass wrote w
         e = nq_launch_svf_combine(false, a, st);
-        if (e != hipSuccess) return e;
-    }
-    e = launch_bwd<0>(a, sh, st);
+    return e;
+}
+
+// backward pass with sample weights a.w: out[p] = sum_r sigma X omega, out[P] = sum_s w_s
+hipError_t q8_backward(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st)
+{
+    hipError_t e = launch_bwd<0>(a, sh, st);
     if (e != hipSuccess) return e;
     return nq_chunk_reduce(false, a.part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, out, st);
+}
+
+} // namespace
+
+hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st) { return q8_pack_rows<0>(a, sh, true, st); }
+
+hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st)
+{
+    hipError_t e = q8_forward(a, sh, st);
+    return e != hipSuccess ? e : q8_backward(a, sh, scratch, out, st);
 }
 
 hipError_t nq_q8_moments(const SvfArgs &a0, const Q8Shape &sh, const SvfShape &svf, void *part, void *scratch,
@@ -728,68 +742,54 @@ hipError_t nq_q8_moments(const SvfArgs &a0, const Q8Shape &sh, const SvfShape &s
     return nq_launch_svf_sq_pack(false, a0, svf, part, scratch, tmp, so, ss, sf, st);
 }
 
-// Complex weights: the split real rows, quantised and masked as the real form's rows.
-// a: the real form's arguments over the 2 S Kc split rows (a.K = 2 Kc, a.kc = Kc,
-// a.U = the split rows Ur, a.spins = spins2, a.Umax its row maxima); Uc: the complex
-// factor rows, spins: the S Kc replica rows' spin words.
-hipError_t nq_q8c_pack(const SvfArgs &a, const Q8Shape &sh, const void *Uc, const uint32_t *spins, hipStream_t st)
-{
-    const long long Rc = a.S * a.kc;
-    hipLaunchKernelGGL(k_q8c_split, dim3((unsigned)((Rc + 3) / 4)), dim3(64), 0, st, (const float2 *)Uc, Rc, a.Ucols,
-                       a.NW, spins, (float *)a.U, (float *)a.Umax, (uint32_t *)a.spins);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    return nq_q8_pack(a, sh, st);
-}
-
-// S*v of complex weights on the int8 form:
-//   forward runs on Re v (a.v = vre) and Im v (vim): row sums wrow[0..R), wrow[R..2R);
-//   w_s = sum_kk U_kk (Y + bias) - o^T v from them (planar wr | wi in a.w);
-//   backward runs with row weights (wr on Re rows, wi on Im rows) and (wi, -wr):
-//   out[0..P] = sum_s Re(conj(O_s) w_s) and sum_s Re w_s, out[P+1..2P+1] = the imaginary
-//   parts (sum_r sigma (Re U wr + Im U wi) and sum_r sigma (Re U wi - Im U wr)).
-// ovc: o^T v as (re, im) on the device.
-hipError_t nq_q8c_apply(const SvfArgs &a0, const Q8Shape &sh, const void *vre, const void *vim, const void *ovc,
-                        void *scratch, void *out, hipStream_t st)
+// Complex weights: row maxima of |re| + |im|, the digit sets X_A = X_R + X_I, X_R, X_I in
+// slices 0, 1, 2 of Uq / Uf (one scale per row, shared), and the spin masks once.
+hipError_t nq_q8c_pack(const SvfArgs &a0, const Q8Shape &sh, hipStream_t st)
 {
     SvfArgs a = a0;
     const long long R = a.S * a.K;
-    const int ncg = sh.gemm ? 1 : sh.fwd_cgs;  // partial row sums per run (column groups)
-    double *wrow = (double *)a0.wrow;
-    double *wr = (double *)a0.w, *wi = wr + a.S;
-    a.ncg = ncg;
+    hipLaunchKernelGGL(k_q8c_rowmax, dim3((unsigned)((R + 15) / 16)), dim3(256), 0, st, (const float2 *)a.U, R,
+                       a.Ucols, (float *)a.Umax);
+    const size_t qs = (size_t)sh.rpad * a.Ucols * QU, fs = sh.uf_bytes;
+    hipError_t e = hipGetLastError();
+    for (int t = 0; t < 3 && e == hipSuccess; t++) {
+        a.Uq = (const uint8_t *)a0.Uq + t * qs;
+        a.Uf = sh.gemm ? (const void *)((const uint8_t *)a0.Uf + t * fs) : nullptr;
+        if (t == 0) e = q8_pack_rows<3>(a, sh, true, st);
+        else if (t == 1) e = q8_pack_rows<1>(a, sh, false, st);
+        else e = q8_pack_rows<2>(a, sh, false, st);
+    }
+    return e;
+}
+
+// S*v of complex weights on the int8 form, three real products per complex product
+// (Gauss): forward runs A on Re v, R on Im v - Re v, I on Re v + Im v; k_q8c_combine3;
+// backward runs A with Re w + Im w, R with Im w, I with Re w.  The digit sets satisfy
+// X_A = X_R + X_I exactly, so the operator is the exact complex product of the quantised
+// factors, up to the quantisation of v and w (2^-47 of their scale), as the real form.
+hipError_t nq_q8c_apply(const SvfArgs &a0, const Q8Shape &sh, const double *vin, double *w3, void *scratch,
+                        double *out, hipStream_t st)
+{
+    SvfArgs a = a0;
+    const long long P = a.P, S = a.S;
+    const size_t qs = (size_t)sh.rpad * a.Ucols * QU, fs = sh.uf_bytes;
     hipError_t e = hipSuccess;
-    for (int part = 0; part < 2 && e == hipSuccess; part++) {
-        a.v = part ? vim : vre;
-        a.wrow = wrow + (long long)part * ncg * R;
-        if (sh.gemm) {
-            e = nq_q8_fwd_gemm(a, sh, st, true);
-        } else {
-            const int UC = a.Ucols, nct = UC / 16;
-            uint8_t *vfrag = (uint8_t *)a.vq;
-            double *colscale = (double *)(vfrag + (size_t)nct * QV * 64 * 16);
-            hipLaunchKernelGGL(k_q8_prep_v, dim3((UC + 63) / 64), dim3(64), 0, st, a, sh.nsp, vfrag, colscale,
-                               colscale + UC);
-            const dim3 grid(sh.fwd_blocks, sh.fwd_cgs);
-            if (sh.nsp == 32)
-                hipLaunchKernelGGL((k_q8_fwd<32>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
-            else hipLaunchKernelGGL((k_q8_fwd<64>), grid, dim3(sh.fwd_threads), sh.fwd_lds, st, a, sh.fwd_cgt);
-            e = hipGetLastError();
-        }
+    for (int t = 0; t < 3 && e == hipSuccess; t++) {
+        a.Uq = (const uint8_t *)a0.Uq + t * qs;
+        a.Uf = sh.gemm ? (const void *)((const uint8_t *)a0.Uf + t * fs) : nullptr;
+        a.v = vin + t * P;
+        a.ov = vin + 3 * P + 2;  // zero: o^T v enters in the combine
+        a.w = w3 + t * S;
+        e = q8_forward(a, sh, st);
     }
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_q8c_combine, dim3((unsigned)((a.S + 255) / 256 < 4096 ? (a.S + 255) / 256 : 4096)), dim3(256),
-                       0, st, (const double *)wrow, (const double *)(wrow + (long long)ncg * R), a.S, a.kc, ncg, R,
-                       (const double *)ovc, wr, wi);
+    const long long nb = (S + 255) / 256 < 4096 ? (S + 255) / 256 : 4096;
+    hipLaunchKernelGGL(k_q8c_combine3, dim3((unsigned)nb), dim3(256), 0, st, w3, S, vin + 3 * P);
     e = hipGetLastError();
-    for (int part = 0; part < 2 && e == hipSuccess; part++) {
-        a.wa = part ? wi : wr;
-        a.wb = part ? wr : wi;
-        a.sb = part ? -1.0 : 1.0;
-        e = launch_bwd<3>(a, sh, st);
-        if (e == hipSuccess)
-            e = nq_chunk_reduce(false, a.part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, (double *)out + part * (a.P + 1),
-                                st);
+    for (int t = 0; t < 3 && e == hipSuccess; t++) {
+        a.Uq = (const uint8_t *)a0.Uq + t * qs;
+        a.w = w3 + t * S;
+        e = q8_backward(a, sh, scratch, out + t * (P + 1), st);
     }
     return e;
 }

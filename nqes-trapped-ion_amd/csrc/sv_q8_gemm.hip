@@ -56,7 +56,7 @@ __device__ inline double dpp_sum16x4(const double (&h)[4], int l)
 __global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst, uint4 *qg)
 {
     __shared__ double red[4];
-    const int N = a.N, M = a.M, K = a.kc ? a.kc : a.K, UC = a.Ucols;  // K of the column layout
+    const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
     const long long KM = (long long)K * M;
     const double *v = (const double *)a.v;
     auto Bval = [&](int col, int i) -> double {
