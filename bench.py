@@ -322,7 +322,7 @@ def sv_roofline(args, stats, real):
     P = args.K * (args.N + args.M + args.N * args.M)
     sv_env = os.environ.get("NQES_SV")
     factored = args.precision == "mixed" and args.N <= 64 and sv_env != "stored"
-    q8 = factored and real and sv_env != "fp64mfma"
+    q8 = factored and sv_env != "fp64mfma"
     sv_calls = sum(s["sv_calls"] for s in stats)
     sv_ms = sum(s["t_sv_ms"] for s in stats)
     sv_avg_ms = sv_ms / max(sv_calls, 1)
@@ -338,18 +338,23 @@ def sv_roofline(args, stats, real):
                 "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes}
     if q8:
         # int8 form: each pass reads the quantised factors once (3 digit bytes per row
-        # and column), plus per row its scale (8 B) and spin words, and w once per sample
+        # and column), plus per row its scale (8 B) and spin words, and w once per sample;
+        # complex weights run three forward and three backward passes, one per digit set
+        # (X_A = X_R + X_I, X_R, X_I: Gauss's three real products)
+        sets = 1 if real else 3
         R = S * args.K
         ucols = 16 * ((args.K * args.M + args.K + 15) // 16)
         nw = (args.N + 31) // 32
-        sv_bytes = 2 * R * ucols * 3 + 2 * R * 8 + R * 4 * nw + 2 * S * 8
+        sv_bytes = sets * (2 * R * ucols * 3 + 2 * R * 8 + R * 4 * nw + 2 * S * 8)
         nst = (args.N + 15) // 16
         nsp = 32 if args.N <= 32 else 64
         ncg64 = (ucols + 63) // 64
         gemm = args.N <= 32 and ncg64 <= 4 and os.environ.get("NQES_Q8_FWD") != "tiles"
-        bwd_ops = 2 * R * ucols * 18 * 16 * nst
-        fwd_ops = R * ncg64 * 3 * (6 * nst + 1) * 2 * 16 * 64 if gemm else 2 * R * ucols * 6 * nsp
-        kernel = f"S*v: {'k_q8_fwdg' if gemm else 'k_q8_fwd'} + k_q8_bwd (int8 MFMA over quantised c*tanh(theta) factors)"
+        # 15 of the 18 digit pairs (the pairs of weight 256^0 and 256^1 are left out)
+        bwd_ops = sets * 2 * R * ucols * 15 * 16 * nst
+        fwd_ops = sets * (R * ncg64 * (15 * nst + 3) * 2 * 16 * 64 if gemm else 2 * R * ucols * 6 * nsp)
+        kernel = (f"S*v: {'k_q8_fwdg' if gemm else 'k_q8_fwd'} + k_q8_bwd (int8 MFMA over quantised c*tanh(theta) "
+                  f"factors{'' if real else '; complex: 3 digit sets, 3 passes each way'})")
         achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
         return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args, real, "q8"),

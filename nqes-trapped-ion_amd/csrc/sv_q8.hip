@@ -26,8 +26,10 @@
 // MFMAs per tile pair), T (also the b-block column sums) by the same MFMAs on the
 // unmasked weight digits (18 per column tile and chunk); products
 // of equal digit weight 256^s share an int32 accumulator (exact), which goes to FP64
-// once per block.  The operator is the exact product of U~ with v and w quantised to
-// 2^-47 of their scale (w per block of rows): linear and symmetric to ~1e-13.
+// once per block.  Both passes leave out the digit pairs of weight 256^0 and 256^1
+// (QSKIP), which add at most 2^-46 of the largest product, so 15 of the 18 MFMAs of a
+// digit set run.  The operator is the product of U~ with v and w quantised to 2^-47 of
+// their scale (w per block of rows), to that truncation: linear and symmetric to ~1e-13.
 //
 // Layout of the quantised factors Uq: 16 x 16 tiles (row tile rt, column tile ct) of
 // 768 bytes, tile (rt, ct) at dword (rt NCT + ct) 192; lane l of a tile holds dwords
@@ -498,7 +500,8 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
                 const v4i aall = v4i{(int)pw[0], (int)pw[1], (int)pw[2], (int)pw[3]};
 #pragma unroll
                 for (int t = 0; t < QU; t++)
-                    tacc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aall, bp[t], tacc[t + u], 0, 0, 0);
+                    if (t + u >= QSKIP)
+                        tacc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aall, bp[t], tacc[t + u], 0, 0, 0);
             }
 #pragma unroll
             for (int it = 0; it < NIT; it++) {
@@ -506,7 +509,8 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
                                    (int)(pw[3] & cm[it].w)};
 #pragma unroll
                 for (int t = 0; t < QU; t++)
-                    acc[it][t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bp[t], acc[it][t + u], 0, 0, 0);
+                    if (t + u >= QSKIP)
+                        acc[it][t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bp[t], acc[it][t + u], 0, 0, 0);
             }
         }
     }

@@ -223,7 +223,8 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
                         __builtin_memcpy(&bv, &b4, 16);
 #pragma unroll
                         for (int t = 0; t < QU; t++)
-                            acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(at[t], bv, acc[t + u], 0, 0, 0);
+                            if (t + u >= QSKIP)
+                                acc[t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(at[t], bv, acc[t + u], 0, 0, 0);
                     }
                 }
             }
