@@ -499,10 +499,19 @@ def serve_workload(args):
             if dt > 0:
                 time.sleep(dt)
             ids.append(srv.submit(**q, precision=nqes.PREC_MIXED, **sched))
-        # collect in completion order, reporting each as it ends
+        # collect in completion order, reporting each as it ends and every 25 SR steps
+        # completed over the running solves
         res = [None] * len(ids)
         left = set(range(len(ids)))
+        steps_done, steps_shown = 0, 0
         while left:
+            running = sum(srv.progress(ids[k]) for k in left)
+            finished = sum(res[k]["steps"] for k in range(len(ids)) if res[k] is not None)
+            steps_done = running + finished
+            if steps_done >= steps_shown + 25:
+                steps_shown = steps_done
+                progress(f"{steps_done} SR steps done at {time.perf_counter() - t0:.1f}s, "
+                         f"{len(ids) - len(left)}/{len(ids)} solves finished")
             for k in sorted(left):
                 if srv.done(ids[k]):
                     res[k] = srv.wait(ids[k], raise_on_error=False)[1]

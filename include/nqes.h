@@ -246,6 +246,10 @@ const char *nqes_server_last_error(const nqes_server *srv);
 int nqes_server_submit(nqes_server *srv, const nqes_request *req, uint64_t *id);
 /* *done = 1 once solve `id` has finished. */
 int nqes_server_poll(nqes_server *srv, uint64_t id, int *done);
+/* *steps = SR steps solve `id` has completed so far (0 while it is queued), for
+ * progress reports; the reference prints one energy line per SR step instead
+ * (rbm.cc:1188-1190). */
+int nqes_server_progress(nqes_server *srv, uint64_t id, int *steps);
 /* Blocks until solve `id` finishes; copies its final weights (P complex, may be
  * NULL) and result, then forgets the solve.  Returns the solve's status. */
 int nqes_server_wait(nqes_server *srv, uint64_t id, double *W, nqes_result *res);

@@ -20,6 +20,7 @@
 // request, not on the slot or on what the slot ran before.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <chrono>
 #include <complex>
 #include <condition_variable>
@@ -52,6 +53,7 @@ struct Job {
     int passed = 0;  // times a later request was admitted ahead of this one
     bool done = false;
     bool claimed = false;  // a waiter has taken it (one waiter per id)
+    std::atomic<int> progress{0};  // SR steps completed so far (read by nqes_server_progress)
     nqes_result res{};
     std::string err;  // message of a failed solve
 };
@@ -167,6 +169,7 @@ struct nqes_server {
             if (rc != NQES_OK) break;
             r.t_device_ms += st.t_total_ms;
             r.steps = it + 1;
+            j->progress.store(it + 1, std::memory_order_relaxed);
             for (auto &e : emat) e /= (double)q.N;
             sched.advance();
             std::vector<double> ev = srloop::sorted_real_eigs(emat, q.K);
@@ -328,6 +331,19 @@ int nqes_server_poll(nqes_server *s, uint64_t id, int *done)
         return NQES_EINVAL;
     }
     *done = it->second->done ? 1 : 0;
+    return NQES_OK;
+}
+
+int nqes_server_progress(nqes_server *s, uint64_t id, int *steps)
+{
+    if (!s || !steps) return NQES_EINVAL;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) {
+        s->err = "unknown request id";
+        return NQES_EINVAL;
+    }
+    *steps = it->second->progress.load(std::memory_order_relaxed);
     return NQES_OK;
 }
 

@@ -99,7 +99,7 @@ EXPORTS = [
     "nqes_solve", "nqes_correlations", "nqes_required_bytes", "nqes_version",
     "nqes_set_dense_S", "nqes_enumerate", "nqes_get_ulp_ties", "nqes_apply_S_rows_noalloc",
     "nqes_server_create", "nqes_server_destroy", "nqes_server_last_error", "nqes_server_submit",
-    "nqes_server_poll", "nqes_server_wait",
+    "nqes_server_poll", "nqes_server_progress", "nqes_server_wait",
 ]
 
 
@@ -157,6 +157,8 @@ def load_library(path=LIB_PATH):
     L.nqes_server_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(c_u64)]
     L.nqes_server_poll.restype = c_int
     L.nqes_server_poll.argtypes = [_vp, c_u64, ctypes.POINTER(c_int)]
+    L.nqes_server_progress.restype = c_int
+    L.nqes_server_progress.argtypes = [_vp, c_u64, ctypes.POINTER(c_int)]
     L.nqes_server_wait.restype = c_int
     L.nqes_server_wait.argtypes = [_vp, c_u64, _vp, ctypes.POINTER(Result)]
     _lib = L
@@ -390,6 +392,14 @@ class Server:
         if st != 0:
             raise NqesError(f"nqes_server_poll status {st}: {self._err()}")
         return bool(d.value)
+
+    def progress(self, rid):
+        """SR steps solve `rid` has completed so far (0 while queued)."""
+        n = ctypes.c_int(0)
+        st = self.lib.nqes_server_progress(self.srv, int(rid), ctypes.byref(n))
+        if st != 0:
+            raise NqesError(f"nqes_server_progress status {st}: {self._err()}")
+        return n.value
 
     def wait(self, rid, raise_on_error=True):
         """(final weights, result dict).  A failed solve raises, or with

@@ -68,7 +68,15 @@ def test_same_bits_alone_and_in_mix(alone):
         for case in MIX:
             ids.append(srv.submit(**request(case), **SCHED))
             time.sleep(0.005)  # staggered, yet well inside one solve however fast the kernels
+        seen = [srv.progress(i) for i in ids]
+        while not all(srv.done(i) for i in ids):
+            now = [srv.progress(i) for i in ids]
+            assert all(b >= a for a, b in zip(seen, now))  # steps completed only grow
+            seen = now
+            time.sleep(0.01)
+        final = [srv.progress(i) for i in ids]
         res = [srv.wait(i) for i in ids]
+    assert final == [r["steps"] for _, r in res]  # a finished solve reports all its steps
     overlap = 0
     for case, (Wa, ra), (Wm, rm) in zip(MIX, alone, res):
         assert np.array_equal(Wa, Wm), case
