@@ -23,9 +23,9 @@ parts N(0,1)/sqrt(N), /sqrt(M), /sqrt(NM), drawn here by numpy from a fixed seed
   N16: N=16 M=32 K=2 h=0.595, 1024 chains x 256 samples (a member of the batched field
        scan), tol 1e-4, FP64 and mixed to convergence: solves/hour of each path, the
        mixed/FP64 ratio, and the gates at the mixed solve's final step;
-  N32: the paper size (N=32 M=64 K=2, 4096 chains x 256 samples), tol 1e-4: the mixed
-       solve to convergence (solves/hour) and the FP64 path's first two SR steps on the
-       same seeds (per-step time ratio, energies at those steps).
+  N32: the paper size (N=32 M=64 K=2, 4096 chains x 256 samples), tol 1e-4: both paths
+       to convergence on the same seeds (about 3.5 minutes, most of it the FP64 solve):
+       solves/hour of each, their ratio, and the gate at the mixed solve's final step.
 The published schedule (reglam 1 / 0.9 / 0.2, tol 1e-4) is that of the reference's
 N=32 runs (data/energy/ion_N32_M64_K2_h1_c128_m200:1).  Real N(0, 0.01^2) weights
 with reglam 40 -> 1 diverge at N=8 (non-finite at SR step 43, on the CPU oracle too)
@@ -88,13 +88,14 @@ def reference_init(N, M, K, seed):
     return np.concatenate(out).astype(np.complex128)
 
 
-def ed_energies(N, h, J, K):
-    """K lowest E/N of H = sum_{i != j} J_ij Z_i Z_j - h sum_i X_i (2^N states)."""
+def ed_energies(N, h, J, K, dense=None):
+    """K lowest E/N of H = sum_{i != j} J_ij Z_i Z_j - h sum_i X_i (2^N states): dense
+    eigenvalues for N <= 10, sparse Lanczos (scipy eigsh) above."""
     dim = 1 << N
     s = np.arange(dim)
     sig = 1.0 - 2.0 * ((s[:, None] >> np.arange(N)[None, :]) & 1)
     diag = np.einsum("si,ij,sj->s", sig, J, sig)
-    if N <= 10:
+    if dense if dense is not None else N <= 10:
         H = np.diag(diag)
         for i in range(N):
             H[s, s ^ (1 << i)] -= h
@@ -105,7 +106,10 @@ def ed_energies(N, h, J, K):
     cols = np.concatenate([s] + [s ^ (1 << i) for i in range(N)])
     vals = np.concatenate([diag] + [np.full(dim, -h)] * N)
     H = sp.csr_matrix((vals, (rows, cols)), shape=(dim, dim))
-    ev = sla.eigsh(H, k=K, which="SA", tol=1e-12, v0=np.ones(dim))[0]
+    # a start vector with weight in both sectors of the global spin flip (which commutes
+    # with H): a flip-even one such as all ones would find the lowest even states only
+    v0 = np.random.default_rng(0).normal(size=dim)
+    ev = sla.eigsh(H, k=K, which="SA", tol=1e-12, v0=v0)[0]
     return np.sort(ev) / N
 
 
@@ -231,29 +235,32 @@ def converged_block(args, nqes):
     blk.update(compare(rm, rf, rm["steps"]))
     out["N16"] = blk
 
-    # N = 32: the mixed solve to convergence, the FP64 path's first two steps
+    # N = 32: both paths to convergence on the same seeds (the FP64 solve runs on, untimed,
+    # to the mixed solve's step count if it stops first, for the gate at an equal count)
     N, M, K = 32, 64, 2
     J = load_J(N)
     W0 = reference_init(N, M, K, seed)
-    rm = solve(nqes, N, M, K, h, J, W0, MX, 4096, 256, 64, 1e-4, 200, seed)
+    logs = {p: (lambda t, ev, st, p=p: progress(f"N=32 {p} step {t + 1}: E/N {ev}, {st['minres_iters']} MINRES-QLP "
+                                                 f"iterations, {st['t_total_ms']:.0f} ms"))
+            for p in ("mixed", "fp64")}
+    rm = solve(nqes, N, M, K, h, J, W0, MX, 4096, 256, 64, 1e-4, 200, seed, log=logs["mixed"])
     progress(f"N=32 mixed solve: {rm['steps']} steps, {rm['wall_s']:.1f}s")
-    nf = 2
-    rf = solve(nqes, N, M, K, h, J, W0, FP, 4096, 256, 64, 1e-4, nf, seed)
-    progress(f"N=32 fp64 first {nf} steps: {rf['wall_s']:.1f}s")
-    rel = np.abs(rm["hist"][:nf] - rf["hist"][:nf]) / np.abs(rf["hist"][:nf])
-    out["N32"] = {"workload": "N=32 M=64 K=2 h=0.595 C=4096 m=256 B=64, tol 1e-4, at most 200 SR steps",
-                  "mixed": dict(summary(rm), energies_per_step=rm["hist"][: rm["steps"]].tolist()),
-                  "fp64_first_steps": {"steps": nf, "ms_per_step": rf["ms_per_step"],
-                                       "minres_iters": rf["minres_iters"],
-                                       "mixed_ms_per_step": rm["ms_per_step"][:nf],
-                                       "mixed_minres_iters": rm["minres_iters"][:nf],
-                                       "fp64_over_mixed_time": [a / b for a, b in
-                                                                zip(rf["ms_per_step"], rm["ms_per_step"][:nf])],
-                                       "energies_fp64": rf["hist"].tolist(),
-                                       "energies_mixed": rm["hist"][:nf].tolist(),
-                                       "mixed_vs_fp64_rel": rel.tolist()}}
+    rf = solve(nqes, N, M, K, h, J, W0, FP, 4096, 256, 64, 1e-4, 200, seed, min_steps=rm["steps"], log=logs["fp64"])
+    progress(f"N=32 fp64 solve: {rf['steps']} steps, {rf['wall_s']:.1f}s")
+    nf = min(2, rf["steps"] or 0, rm["steps"] or 0)
+    blk = {"workload": "N=32 M=64 K=2 h=0.595 C=4096 m=256 B=64, tol 1e-4, at most 200 SR steps",
+           "fp64": summary(rf), "mixed": dict(summary(rm), energies_per_step=rm["hist"][: rm["steps"]].tolist()),
+           "converged_solves_per_hour_ratio_mixed_over_fp64": rm["solves_per_hour"] / rf["solves_per_hour"]
+           if rf["solves_per_hour"] else None,
+           "first_steps": {"fp64_ms_per_step": rf["ms_per_step"][:nf], "mixed_ms_per_step": rm["ms_per_step"][:nf],
+                           "fp64_minres_iters": rf["minres_iters"][:nf], "mixed_minres_iters": rm["minres_iters"][:nf]}}
+    blk["fp64"]["energies_per_step"] = rf["hist"].tolist()
+    blk.update(compare(rm, rf, rm["steps"]))
+    out["N32"] = blk
     out["converged_solves_per_hour"] = {
         "N32_mixed": rm["solves_per_hour"],
+        "N32_fp64": rf["solves_per_hour"],
+        "N32_mixed_within_1e-4_of_fp64": out["N32"]["pass_mixed_vs_fp64"],
         "N16_mixed": out["N16"]["mixed"]["solves_per_hour"],
         "N16_fp64": out["N16"]["fp64"]["solves_per_hour"],
         "N8_mixed": out["N8"]["mixed"]["solves_per_hour"],
@@ -462,7 +469,7 @@ def solve_requests(args):
     reqs = []
     t = 0.0
     n = args.requests if args.requests else (64 if args.workload == "scan" else 300)
-    for r in range(n):
+    for r in range(args.first + n):
         if args.workload == "scan":
             N = int(rng.choice([16, 24, 32, 48]))
             M, K = 2 * N, int(rng.choice([2, 3, 4]))
@@ -475,10 +482,15 @@ def solve_requests(args):
             K = int(rng.integers(1, 5))
             h = float(rng.uniform(0.2, 1.2))
             alpha = float(rng.uniform(0.5, 1.5))
+        if r < args.first:  # an earlier part of the same stream, run separately
+            continue
         W0 = reference_init(N, M, K, 1000 + r)
         reqs.append((t if args.workload == "serving" else 0.0,
                      dict(N=N, M=M, K=K, h=h, J=power_law_J(N, alpha), W0=W0, seed=args.seed + 7919 * r,
                           C=1024, m=256, B=args.B, niters=args.solve_steps)))
+    if reqs and args.workload == "serving":  # arrivals from the first request of this part on
+        t_first = reqs[0][0]
+        reqs = [(t - t_first, q) for t, q in reqs]
     return reqs
 
 
@@ -554,6 +566,10 @@ def serve_workload(args):
         "mc_samples_per_s": samples / wall,
         "device_busy_ms_sum": float(sum(r["t_device_ms"] for r in res)),
         "failed": int(len(res) - len(ok)),
+        "requests": [args.first, args.first + len(res)],
+        "per_solve": [{"N": q["N"], "M": q["M"], "K": q["K"], "steps": r["steps"], "converged": r["converged"],
+                       "latency_ms": r["t_done_ms"] - r["t_submit_ms"], "queue_ms": r["t_admit_ms"] - r["t_submit_ms"]}
+                      for (_, q), r in zip(reqs, res)],
     }
     print(json.dumps(out))
 
@@ -588,6 +604,8 @@ def main():
     ap.add_argument("--solve-steps", type=int, default=200, help="scan/serving: SR steps per solve at most")
     ap.add_argument("--requests", type=int, default=0, help="scan/serving: number of requests (default 64 / 300)")
     ap.add_argument("--rate", type=float, default=4.0, help="serving: mean arrivals per second")
+    ap.add_argument("--first", type=int, default=0,
+                    help="scan/serving: start at this request of the seeded stream (to run it in parts)")
     args = ap.parse_args()
 
     if args.gpus != 1:
