@@ -532,7 +532,8 @@ def serve_workload(args):
                     progress(f"{len(ids) - len(left)}/{len(ids)} done at {time.perf_counter() - t0:.1f}s: "
                              f"N={q['N']} M={q['M']} K={q['K']} steps={res[k]['steps']} "
                              f"converged={res[k]['converged']} status={res[k]['status']} "
-                             f"latency={(res[k]['t_done_ms'] - res[k]['t_submit_ms']) / 1e3:.1f}s")
+                             f"latency={(res[k]['t_done_ms'] - res[k]['t_submit_ms']) / 1e3:.1f}s"
+                             + (f" error: {res[k].get('error')}" if res[k]["status"] else ""))
             if left:
                 time.sleep(0.05)
         wall = time.perf_counter() - t0
@@ -568,6 +569,7 @@ def serve_workload(args):
         "failed": int(len(res) - len(ok)),
         "requests": [args.first, args.first + len(res)],
         "per_solve": [{"N": q["N"], "M": q["M"], "K": q["K"], "steps": r["steps"], "converged": r["converged"],
+                       "status": r["status"], "error": r.get("error"),
                        "latency_ms": r["t_done_ms"] - r["t_submit_ms"], "queue_ms": r["t_admit_ms"] - r["t_submit_ms"]}
                       for (_, q), r in zip(reqs, res)],
     }

@@ -4,6 +4,42 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <shared_mutex>
+
+// Stream capture and the calls that can end a capture in progress on another thread
+// (device and pinned allocation, synchronous copies, stream and event creation, kernel
+// attributes) exclude each other: a capture holds nq_capture_mutex() exclusively, such
+// calls hold it shared through NqUnsafeHold.  A thread that is capturing (nq_capturing())
+// does not take it again.
+std::shared_mutex &nq_capture_mutex();
+bool &nq_capturing();
+class NqUnsafeHold {
+  public:
+    NqUnsafeHold()
+    {
+        if (!nq_capturing()) {
+            nq_capture_mutex().lock_shared();
+            held_ = true;
+        }
+    }
+    ~NqUnsafeHold()
+    {
+        if (held_) nq_capture_mutex().unlock_shared();
+    }
+    NqUnsafeHold(const NqUnsafeHold &) = delete;
+    NqUnsafeHold &operator=(const NqUnsafeHold &) = delete;
+
+  private:
+    bool held_ = false;
+};
+
+// Lets kernel `fn` launch with `bytes` of dynamic LDS on the current device.  The
+// attribute is set once per kernel and device, to the most the kernel can take, never
+// per launch: kernel attributes are shared by every thread of the process, so a
+// per-launch value set by one thread could undercut another thread's launch in
+// between.  hipErrorInvalidValue if `bytes` exceeds what the kernel can take.
+hipError_t nq_allow_dynamic_lds(const void *fn, size_t bytes);
+
 // Arguments of the FP64 sampler (one wavefront per chain).  Complex arrays are
 // interleaved doubles.  Sample s = chain * m + t.
 struct Fp64SampleArgs {

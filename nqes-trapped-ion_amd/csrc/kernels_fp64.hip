@@ -964,8 +964,7 @@ hipError_t launch_rows64(const void *O, const void *v, const void *ov, long long
 {
     const size_t lds = (size_t)nvec * 16;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_sv_rows64<CPLX, QV, WPR>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = nq_allow_dynamic_lds((const void *)k_sv_rows64<CPLX, QV, WPR>, lds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_sv_rows64<CPLX, QV, WPR>), dim3(nblk), dim3(64 * WPR), lds, st, (const double2 *)O, v, ov,

@@ -19,6 +19,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -26,6 +29,32 @@
 #include "cplx64.hpp"
 #include "kernels.hpp"
 #include "mixed_math.hpp"
+
+hipError_t nq_allow_dynamic_lds(const void *fn, size_t bytes)
+{
+    if (bytes <= 65536) return hipSuccess;  // allowed without the attribute
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, size_t> allowed;  // (device, kernel) -> bytes
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = allowed.find({dev, fn});
+    if (it == allowed.end()) {
+        NqUnsafeHold hold;
+        int lds_max = 0;
+        e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e != hipSuccess) return e;
+        hipFuncAttributes fa{};
+        e = hipFuncGetAttributes(&fa, fn);
+        if (e != hipSuccess) return e;
+        const size_t most = (size_t)lds_max > fa.sharedSizeBytes ? (size_t)lds_max - fa.sharedSizeBytes : 0;
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)most);
+        if (e != hipSuccess) return e;
+        it = allowed.emplace(std::make_pair(dev, fn), most).first;
+    }
+    return bytes <= it->second ? hipSuccess : hipErrorInvalidValue;
+}
 
 using namespace nqm;
 using nq64::cd;
@@ -558,8 +587,7 @@ hipError_t nq_launch_theta_gemm(bool cplx, int N, int M, int K, int NW, int C, c
         // the W terms (101 KB at N = 256) or the output tile, whichever is larger
         const size_t lds = (size_t)3 * 64 * NPP * 2;  // 101 KB at N = 256
         if (lds > 65536) {
-            hipError_t ea = hipFuncSetAttribute((const void *)k_theta_bf16, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                (int)lds);
+            hipError_t ea = nq_allow_dynamic_lds((const void *)k_theta_bf16, lds);
             if (ea != hipSuccess) return ea;
         }
         // about 2048 blocks: a block loops over row tiles so that large sizes stage each
@@ -633,8 +661,7 @@ hipError_t launch_rows(const void *O, const void *v, const void *ov, long long S
                        void *part, void *wpart, size_t lds, hipStream_t st)
 {
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k_sv_rows_mx<CPLX, QV, WPR, NP>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = nq_allow_dynamic_lds((const void *)k_sv_rows_mx<CPLX, QV, WPR, NP>, lds);
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((k_sv_rows_mx<CPLX, QV, WPR, NP>), dim3(nblk), dim3(64 * WPR * NP), lds, st,

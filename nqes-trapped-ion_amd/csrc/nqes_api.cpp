@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -31,7 +32,26 @@
 
 using nq64::cd;
 
+std::shared_mutex &nq_capture_mutex()
+{
+    static std::shared_mutex m;
+    return m;
+}
+
+bool &nq_capturing()
+{
+    thread_local bool capturing = false;
+    return capturing;
+}
+
 namespace {
+
+// a synchronous copy, excluded from stream captures on other threads
+hipError_t copy_sync(void *dst, const void *src, size_t n, hipMemcpyKind kind)
+{
+    NqUnsafeHold hold;
+    return hipMemcpy(dst, src, n, kind);
+}
 
 struct DevBuf {
     void *p = nullptr;
@@ -39,7 +59,10 @@ struct DevBuf {
     ~DevBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            NqUnsafeHold hold;
+            (void)hipFree(p);
+        }
         p = nullptr;
         bytes = 0;
     }
@@ -48,6 +71,7 @@ struct DevBuf {
         if (n <= bytes && p) return hipSuccess;
         release();
         if (n == 0) n = 16;
+        NqUnsafeHold hold;
         hipError_t e = hipMalloc(&p, n);
         if (e != hipSuccess) {
             p = nullptr;
@@ -170,6 +194,7 @@ static long long o_ld(long long P, int prec, bool cplx)
 // the captured S*v graph refers to the sample set's buffers
 static void drop_sv_graph(nqes_ctx *c)
 {
+    NqUnsafeHold hold;
     if (c->svx) (void)hipGraphExecDestroy(c->svx);
     if (c->svg) (void)hipGraphDestroy(c->svg);
     c->svx = nullptr;
@@ -189,6 +214,7 @@ nqes_ctx *nqes_create(int device)
     if (hipSetDevice(device) != hipSuccess) return nullptr;
     nqes_ctx *c = new nqes_ctx();
     c->device = device;
+    NqUnsafeHold hold;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return nullptr;
@@ -209,6 +235,7 @@ void nqes_destroy(nqes_ctx *c)
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    NqUnsafeHold hold;
     if (c->svx) (void)hipGraphExecDestroy(c->svx);
     if (c->svg) (void)hipGraphDestroy(c->svg);
     if (c->hv_pin) (void)hipHostFree(c->hv_pin);
@@ -310,7 +337,7 @@ int nqes_problem_init(nqes_ctx *c, int N, int M, int K, double h, const double *
     c->model = model;
     c->prec = precision;
     HIPCHK(c, c->dJ.ensure(sizeof(double) * N * N), "alloc J");
-    HIPCHK(c, hipMemcpy(c->dJ.p, J, sizeof(double) * N * N, hipMemcpyHostToDevice), "copy J");
+    HIPCHK(c, copy_sync(c->dJ.p, J, sizeof(double) * N * N, hipMemcpyHostToDevice), "copy J");
     HIPCHK(c, c->dW.ensure(16 * c->P), "alloc W");
     HIPCHK(c, c->dchT.ensure((size_t)16 * K * N * M), "alloc tables");
     HIPCHK(c, c->dshT.ensure((size_t)16 * K * N * M), "alloc tables");
@@ -753,9 +780,9 @@ int nqes_get_samples(nqes_ctx *c, uint32_t *spins, double *H, uint64_t *accepts)
     if (!c->sampled) return c->fail(NQES_ESTATE, "no samples: call nqes_sample first");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     if (spins)
-        HIPCHK(c, hipMemcpy(spins, c->dspins.p, (size_t)c->S * c->K * c->NW * 4, hipMemcpyDeviceToHost), "copy spins");
-    if (H) HIPCHK(c, hipMemcpy(H, c->dHs.p, (size_t)c->S * c->K * c->K * 16, hipMemcpyDeviceToHost), "copy H");
-    if (accepts) HIPCHK(c, hipMemcpy(accepts, c->dacc.p, (size_t)c->C * 8, hipMemcpyDeviceToHost), "copy accepts");
+        HIPCHK(c, copy_sync(spins, c->dspins.p, (size_t)c->S * c->K * c->NW * 4, hipMemcpyDeviceToHost), "copy spins");
+    if (H) HIPCHK(c, copy_sync(H, c->dHs.p, (size_t)c->S * c->K * c->K * 16, hipMemcpyDeviceToHost), "copy H");
+    if (accepts) HIPCHK(c, copy_sync(accepts, c->dacc.p, (size_t)c->C * 8, hipMemcpyDeviceToHost), "copy accepts");
     return NQES_OK;
 }
 
@@ -769,10 +796,10 @@ static int get_O_factored(nqes_ctx *c, int64_t row0, int64_t nrows, double *out)
     const size_t tsz = c->cplx ? 8 : 4;
     std::vector<float> U((size_t)nrows * K * UC * (c->cplx ? 2 : 1));
     std::vector<uint32_t> sp((size_t)nrows * K * NW);
-    HIPCHK(c, hipMemcpy(U.data(), static_cast<char *>(c->dU.p) + (size_t)row0 * K * UC * tsz, U.size() * 4,
+    HIPCHK(c, copy_sync(U.data(), static_cast<char *>(c->dU.p) + (size_t)row0 * K * UC * tsz, U.size() * 4,
                         hipMemcpyDeviceToHost),
            "copy S*v factors");
-    HIPCHK(c, hipMemcpy(sp.data(), static_cast<uint32_t *>(c->dspins.p) + (size_t)row0 * K * NW, sp.size() * 4,
+    HIPCHK(c, copy_sync(sp.data(), static_cast<uint32_t *>(c->dspins.p) + (size_t)row0 * K * NW, sp.size() * 4,
                         hipMemcpyDeviceToHost),
            "copy spins");
     const int R = c->cplx ? 2 : 1;
@@ -815,7 +842,7 @@ int nqes_get_O(nqes_ctx *c, int64_t row0, int64_t nrows, double *out)
     const size_t e = o_elem(c->prec, fp64 ? !c->oreal : c->cplx);
     std::vector<unsigned char> buf((size_t)nrows * c->Pld * e);
     HIPCHK(c,
-           hipMemcpy(buf.data(), static_cast<char *>(c->dO.p) + (size_t)row0 * c->Pld * e, buf.size(),
+           copy_sync(buf.data(), static_cast<char *>(c->dO.p) + (size_t)row0 * c->Pld * e, buf.size(),
                      hipMemcpyDeviceToHost),
            "copy O");
     for (int64_t r = 0; r < nrows; r++)
@@ -848,7 +875,7 @@ int nqes_get_ulp_ties(nqes_ctx *c, uint64_t *per_chain, uint64_t *total)
     if (!c->sampled) return c->fail(NQES_ESTATE, "no samples: call nqes_sample first");
     HIPCHK(c, hipSetDevice(c->device), "hipSetDevice");
     std::vector<uint64_t> t((size_t)c->C);
-    HIPCHK(c, hipMemcpy(t.data(), c->dties.p, (size_t)c->C * 8, hipMemcpyDeviceToHost), "copy tie counts");
+    HIPCHK(c, copy_sync(t.data(), c->dties.p, (size_t)c->C * 8, hipMemcpyDeviceToHost), "copy tie counts");
     if (per_chain) memcpy(per_chain, t.data(), (size_t)c->C * 8);
     if (total) {
         uint64_t s = 0;
@@ -890,25 +917,17 @@ static void q8c_raw(nqes_ctx *c, const double *r)
     c->svraw[P] = nq64::mk(ri[P], rr[P]);
 }
 
-static int build_sv_graph(nqes_ctx *c)
+// the int8 S*v sequence on the context's stream: v (and o^T v) from pinned memory, the
+// kernels, the raw result back to pinned memory
+static hipError_t enqueue_sv(nqes_ctx *c)
 {
     const long long P = c->P;
-    const long long nin = sv_in_len(c), nout = sv_out_len(c), npin = std::max(nin, nout);
-    if ((size_t)npin > c->pin_len) {
-        if (c->hv_pin) (void)hipHostFree(c->hv_pin);
-        if (c->hr_pin) (void)hipHostFree(c->hr_pin);
-        c->hv_pin = c->hr_pin = nullptr;
-        c->pin_len = 0;
-        HIPCHK(c, hipHostMalloc((void **)&c->hv_pin, 8 * npin, hipHostMallocDefault), "pinned v");
-        HIPCHK(c, hipHostMalloc((void **)&c->hr_pin, 8 * npin, hipHostMallocDefault), "pinned result");
-        c->pin_len = (size_t)npin;
-    }
-    // v and o^T v arrive in one copy: the graph's kernels read o^T v right after v (the
-    // v buffer holds 16 (P + 1) bytes, room for 2 P + 2 doubles)
+    const long long nin = sv_in_len(c), nout = sv_out_len(c);
+    // v and o^T v arrive in one copy: the kernels read o^T v right after v (the v buffer
+    // holds 16 (P + 1) bytes, room for 2 P + 2 doubles)
     SvfArgs f = svf_args(c, c->S);
     f.ov = (const double *)c->dv.p + P;
     void *din = c->q8c ? c->dvin3.p : c->dv.p, *dout = c->q8c ? c->dsvout3.p : c->dsvout.p;
-    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin S*v capture");
     hipError_t e = hipMemcpyAsync(din, c->hv_pin, 8 * nin, hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess)
         e = c->q8c ? nq_q8c_apply(q8c_args(c), c->q8_shape, static_cast<const double *>(din),
@@ -916,15 +935,54 @@ static int build_sv_graph(nqes_ctx *c)
                                   c->stream)
                    : nq_q8_apply(f, c->q8_shape, c->dsvfscratch.p, dout, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(c->hr_pin, dout, 8 * nout, hipMemcpyDeviceToHost, c->stream);
+    return e;
+}
+
+// pinned staging for the S*v input and result
+static int ensure_sv_pinned(nqes_ctx *c)
+{
+    const long long npin = std::max(sv_in_len(c), sv_out_len(c));
+    if ((size_t)npin <= c->pin_len) return NQES_OK;
+    NqUnsafeHold hold;
+    if (c->hv_pin) (void)hipHostFree(c->hv_pin);
+    if (c->hr_pin) (void)hipHostFree(c->hr_pin);
+    c->hv_pin = c->hr_pin = nullptr;
+    c->pin_len = 0;
+    HIPCHK(c, hipHostMalloc((void **)&c->hv_pin, 8 * npin, hipHostMallocDefault), "pinned v");
+    HIPCHK(c, hipHostMalloc((void **)&c->hr_pin, 8 * npin, hipHostMallocDefault), "pinned result");
+    c->pin_len = (size_t)npin;
+    return NQES_OK;
+}
+
+// Captures enqueue_sv into a graph (c->svx).  The capture holds nq_capture_mutex()
+// exclusively, so this library's allocations and synchronous copies on other threads
+// wait for it; a capture that fails anyway (the runtime can end a capture because of a
+// call made on another thread) leaves c->svx empty, and the caller launches the same
+// sequence directly, as the graph would, and captures again on its next call.
+static void build_sv_graph(nqes_ctx *c)
+{
     hipGraph_t g = nullptr;
-    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-    if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return c->hipfail(e != hipSuccess ? e : e2, "S*v capture");
+    hipGraphExec_t x = nullptr;
+    {
+        std::unique_lock<std::shared_mutex> lk(nq_capture_mutex());
+        nq_capturing() = true;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            e = enqueue_sv(c);
+            const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            if (e == hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        nq_capturing() = false;
+        if (e != hipSuccess) {
+            if (x) (void)hipGraphExecDestroy(x);
+            if (g) (void)hipGraphDestroy(g);
+            (void)hipGetLastError();
+            return;
+        }
     }
     c->svg = g;
-    HIPCHK(c, hipGraphInstantiate(&c->svx, c->svg, nullptr, nullptr, 0), "S*v graph instantiate");
-    return NQES_OK;
+    c->svx = x;
 }
 
 // the int8 S*v graph in two halves: launch (the sv_in_len doubles of `in` to pinned
@@ -932,13 +990,13 @@ static int build_sv_graph(nqes_ctx *c)
 // work in between
 static int sv_graph_launch(nqes_ctx *c, const double *in)
 {
-    if (!c->svx) {
-        int st = build_sv_graph(c);
-        if (st != NQES_OK) return st;
-    }
+    int st = ensure_sv_pinned(c);
+    if (st != NQES_OK) return st;
+    if (!c->svx) build_sv_graph(c);
     memcpy(c->hv_pin, in, 8 * sv_in_len(c));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream), "event");
-    HIPCHK(c, hipGraphLaunch(c->svx, c->stream), "S*v graph launch");
+    if (c->svx) HIPCHK(c, hipGraphLaunch(c->svx, c->stream), "S*v graph launch");
+    else HIPCHK(c, enqueue_sv(c), "S*v kernels");
     HIPCHK(c, hipEventRecord(c->ev[3], c->stream), "event");
     return NQES_OK;
 }
@@ -1346,8 +1404,8 @@ int nqes_sr_step(nqes_ctx *c, double *W, uint64_t seed, int C, int m, int B, dou
     for (long long q = 0; q < c->P; q++) Wc[q] = nq64::sub(Wc[q], nq64::scale(lr, x[q]));
     if (stats) {
         std::vector<unsigned long long> acc(C), ties(C);
-        HIPCHK(c, hipMemcpy(acc.data(), c->dacc.p, (size_t)C * 8, hipMemcpyDeviceToHost), "copy accepts");
-        HIPCHK(c, hipMemcpy(ties.data(), c->dties.p, (size_t)C * 8, hipMemcpyDeviceToHost), "copy tie counts");
+        HIPCHK(c, copy_sync(acc.data(), c->dacc.p, (size_t)C * 8, hipMemcpyDeviceToHost), "copy accepts");
+        HIPCHK(c, copy_sync(ties.data(), c->dties.p, (size_t)C * 8, hipMemcpyDeviceToHost), "copy tie counts");
         uint64_t a = 0, nt = 0;
         for (auto v : acc) a += v;
         for (auto v : ties) nt += v;

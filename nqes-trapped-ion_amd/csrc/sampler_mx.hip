@@ -807,7 +807,7 @@ hipError_t launch_measure_jq(const MixedSampleArgs &a, const void *tabp, const M
     long long blocks = (S + spb - 1) / spb;
     const void *fn = sh.tab ? (const void *)k_measure_mx<CPLX, K, G, JQ, true> : (const void *)k_measure_mx<CPLX, K, G, JQ, false>;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = nq_allow_dynamic_lds(fn, lds);
         if (e != hipSuccess) return e;
     }
     // resident blocks: limited by LDS (160 KB per CU) and by 8 waves per SIMD
@@ -859,7 +859,7 @@ hipError_t launch_jq(const MixedSampleArgs &a, const void *tabp, bool tab, int w
     const int blocks = (a.C + cpb - 1) / cpb;
     const void *fn = tab ? (const void *)k_sample_mx<CPLX, K, G, JQ, true> : (const void *)k_sample_mx<CPLX, K, G, JQ, false>;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = nq_allow_dynamic_lds(fn, lds);
         if (e != hipSuccess) return e;
     }
     if (tab) hipLaunchKernelGGL((k_sample_mx<CPLX, K, G, JQ, true>), dim3(blocks), dim3(64 * wpb), lds, st, a, tabp);

@@ -226,3 +226,47 @@ def test_second_waiter_refused():
             srv.wait(rid)
         t.join()
         assert first["r"][1]["status"] == 0
+
+
+def test_concurrent_launches_of_one_kernel_with_different_lds():
+    """Two contexts on two threads sample with complex weights at N=28 and N=32 (K=2,
+    M=2N): the same sweep and measurement kernels, each launched with more than 64 KB
+    of dynamic LDS and a different amount per N.  Kernel attributes are shared by the
+    whole process, so a per-launch maximum set by one thread could undercut the other
+    thread's launch (an intermittent launch failure of concurrent solves); the library
+    sets it once per kernel.  Every call succeeds and each thread's samples equal those
+    of the same calls made alone."""
+    import threading
+
+    cases = [(28, 56), (32, 64)]
+    reps = 30
+
+    def run(N, M, out, errs):
+        J = synthetic_J(N)
+        W = random_weights(N, M, 2, 7 + N, scale=0.05, complex_=True)
+        try:
+            with nqes.Solver(N, M, 2, 0.5, J, precision=nqes.PREC_MIXED) as s:
+                for r in range(reps):
+                    s.sample(W, 100 * r + N, 256, 8, 4, 1.0)
+                out.append(s.samples())
+        except nqes.NqesError as e:
+            errs.append(f"N={N}: {e}")
+
+    alone = {}
+    for N, M in cases:
+        o, e = [], []
+        run(N, M, o, e)
+        assert not e, e
+        alone[N] = o[0]
+    print("alone done; now concurrent", flush=True)
+    outs = {N: [] for N, _ in cases}
+    errs = []
+    th = [threading.Thread(target=run, args=(N, M, outs[N], errs)) for N, M in cases]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert not errs, errs
+    for N, _ in cases:
+        for a, b in zip(alone[N], outs[N][0]):
+            assert np.array_equal(a, b), N
