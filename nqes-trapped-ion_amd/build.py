@@ -43,6 +43,9 @@ SOURCES = [
     ("sv_factored.hip", []),
     ("sv_q8.hip", []),
     ("sv_q8_gemm.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
+    ("sv_q8_fused.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form", "-DNQES_FZ_K=1"], "sv_q8_fused_k1"),
+    ("sv_q8_fused.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form", "-DNQES_FZ_K=2"], "sv_q8_fused_k2"),
+    ("sv_q8_fused.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form", "-DNQES_FZ_K=4"], "sv_q8_fused_k4"),
     ("nqes_api.cpp", EXACT),
     ("noalloc.hip", EXACT),
     ("serve.cpp", EXACT),

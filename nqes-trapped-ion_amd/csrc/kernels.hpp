@@ -183,8 +183,26 @@ struct Q8Shape {
     size_t fwd_lds, vq_bytes, w_bytes, m_bytes, uf_bytes, qg_bytes;
     long long rpad;  // rows of Uq / escale (multiple of 64)
     int gblk_ints;
+    // one-pass form (sv_q8_fused.hip; real weights, N <= 32, K in {1, 2, 4}): the digits in
+    // layout Ug (held in Uq), the quantised c columns in q8m, fz_blocks blocks of 64-row
+    // chunks, fz_ncg 64-column groups of the K M tanh columns, fz_nst spin tiles
+    int fused, fz_ncg, fz_nst, fz_blocks;
+    size_t ug_bytes;  // layout Ug (3 digit planes x 64 fz_ncg columns per row)
 };
-Q8Shape nq_q8_shape(int N, int M, int K, long long R);
+Q8Shape nq_q8_shape(int N, int M, int K, long long R, bool cplx = false);
+// the one-pass form, per K (sv_q8_fused.hip compiled once per K)
+hipError_t nq_q8f_pack_k1(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
+hipError_t nq_q8f_pack_k2(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
+hipError_t nq_q8f_pack_k4(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
+hipError_t nq_q8f_apply_k1(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st);
+hipError_t nq_q8f_apply_k2(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st);
+hipError_t nq_q8f_apply_k4(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st);
+hipError_t nq_q8f_moment_k1(const SvfArgs &a, const Q8Shape &sh, int wsel, void *part, void *scratch, void *out,
+                            hipStream_t st);
+hipError_t nq_q8f_moment_k2(const SvfArgs &a, const Q8Shape &sh, int wsel, void *part, void *scratch, void *out,
+                            hipStream_t st);
+hipError_t nq_q8f_moment_k4(const SvfArgs &a, const Q8Shape &sh, int wsel, void *part, void *scratch, void *out,
+                            hipStream_t st);
 hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
 hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st);
 // its forward pass as one GEMM over the columns (sh.gemm; sv_q8_gemm.hip): w_s directly for

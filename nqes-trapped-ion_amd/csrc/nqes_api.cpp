@@ -572,7 +572,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
             c->q8_shape = nq_q8_shape(c->N, c->M, K, S * K);
             c->q8 = c->q8_shape.ok && !fp64mfma;
         } else if (c->svf) {  // complex: the int8 form with three digit sets per row
-            c->q8_shape = nq_q8_shape(c->N, c->M, K, S * K);
+            c->q8_shape = nq_q8_shape(c->N, c->M, K, S * K, true);
             c->q8c = c->q8_shape.ok && !fp64mfma;
         }
     }
@@ -593,7 +593,8 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
                 HIPCHK(c, c->dvin3.ensure((size_t)(3 * P + 4) * 8), "alloc S*v operands");
                 HIPCHK(c, c->dsvout3.ensure((size_t)3 * (P + 1) * 8), "alloc S*v out");
             }
-            HIPCHK(c, c->dUq.ensure(nsets * rp * c->q8_shape.ucols * 3), "alloc quantised factors");
+            HIPCHK(c, c->dUq.ensure(std::max((size_t)(nsets * rp * c->q8_shape.ucols * 3), c->q8_shape.ug_bytes)),
+                   "alloc quantised factors");
             HIPCHK(c, c->descale.ensure((size_t)rp * 8), "alloc factor scales");
             HIPCHK(c, c->dUmax.ensure((size_t)rp * 4), "alloc factor row maxima");
             HIPCHK(c, c->dvq.ensure(c->q8_shape.vq_bytes), "alloc S*v operands");

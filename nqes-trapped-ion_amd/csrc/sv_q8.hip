@@ -611,7 +611,7 @@ hipError_t launch_bwd(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
 // stay exact).
 // Work buffers: q8w (block scales, weight digits; per call), q8m (spin masks; per
 // sample set).
-Q8Shape nq_q8_shape(int N, int M, int K, long long R)
+Q8Shape nq_q8_shape(int N, int M, int K, long long R, bool cplx)
 {
     Q8Shape sh{};
     if (N > 64) return sh;
@@ -663,6 +663,37 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R)
     sh.gblk_ints = (sh.bwd_blocks + 3) / 4 * 4;  // keeps pn 16-byte aligned
     sh.w_bytes = (size_t)sh.gblk_ints * 4 + (size_t)n64 * QW * 4 * 16;
     sh.m_bytes = (size_t)n64 * 16 * ((N + 15) / 16) * 4 * 16;
+    // one-pass form: real weights, N <= 32, K | 4, K M <= 128 (the int32 accumulators of a
+    // wavefront's column tiles); NQES_Q8_FUSED=0 keeps the two passes
+    const int fz_ncg = (K * M + 63) / 64, fz_nst = (N + 15) / 16;
+    const char *ef = getenv("NQES_Q8_FUSED");
+    sh.fused = !cplx && N <= 32 && (K == 1 || K == 2 || K == 4) && fz_ncg <= 2 &&
+               !(ef && strcmp(ef, "0") == 0);
+    if (sh.fused) {
+        sh.fz_ncg = fz_ncg;
+        sh.fz_nst = fz_nst;
+        // eight-wavefront blocks (even fz_ncg) take super-chunks of two 64-row chunks: rows
+        // padded to a multiple of 128; one block per CU (LDS), else two
+        const int nkc = fz_ncg % 2 == 0 ? 2 : 1;
+        const long long n64f = (n64 + nkc - 1) / nkc * nkc, nsc = n64f / nkc;
+        sh.rpad = n64f * 64;
+        long long nbf = nkc == 2 ? 256 : 512;
+        if (nbf > nsc) nbf = nsc > 0 ? nsc : 1;
+        while (nbf * MAX_BLOCK_CHUNKS < n64f) nbf *= 2;
+        e = getenv("NQES_Q8_FUSED_BLOCKS");
+        if (e && *e && atoi(e) > 0 && atoi(e) <= nsc && (long long)atoi(e) * MAX_BLOCK_CHUNKS >= n64f) nbf = atoi(e);
+        sh.fz_blocks = (int)nbf;
+        if (sh.bwd_blocks < sh.fz_blocks) sh.bwd_blocks = sh.fz_blocks;  // partial rows allocated for the larger
+        sh.gemm = 0;
+        sh.uf_bytes = 0;
+        sh.ug_bytes = (size_t)sh.rpad * QU * 64 * fz_ncg;
+        // forward B operands and the scale of v
+        const size_t qf = (size_t)fz_ncg * (fz_nst * QV + 1) * 64 * 16 + 16;
+        if (qf > sh.vq_bytes) sh.vq_bytes = qf;
+        // the quantised c columns in the mask buffer
+        const size_t xc = (size_t)sh.rpad * K * 4;
+        if (xc > sh.m_bytes) sh.m_bytes = xc;
+    }
     sh.ok = 1;
     return sh;
 }
@@ -724,10 +755,27 @@ hipError_t q8_backward(const SvfArgs &a, const Q8Shape &sh, void *scratch, void 
 
 } // namespace
 
-hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st) { return q8_pack_rows<0>(a, sh, true, st); }
+hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+{
+    if (sh.fused) {
+        switch (a.K) {
+        case 1: return nq_q8f_pack_k1(a, sh, st);
+        case 2: return nq_q8f_pack_k2(a, sh, st);
+        default: return nq_q8f_pack_k4(a, sh, st);
+        }
+    }
+    return q8_pack_rows<0>(a, sh, true, st);
+}
 
 hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st)
 {
+    if (sh.fused) {
+        switch (a.K) {
+        case 1: return nq_q8f_apply_k1(a, sh, scratch, out, st);
+        case 2: return nq_q8f_apply_k2(a, sh, scratch, out, st);
+        default: return nq_q8f_apply_k4(a, sh, scratch, out, st);
+        }
+    }
     hipError_t e = q8_forward(a, sh, st);
     return e != hipSuccess ? e : q8_backward(a, sh, scratch, out, st);
 }
@@ -738,6 +786,17 @@ hipError_t nq_q8_moments(const SvfArgs &a0, const Q8Shape &sh, const SvfShape &s
     SvfArgs a = a0;
     a.part = part;
     double *r1 = (double *)tmp, *rE = r1 + (a.P + 1);
+    if (sh.fused) {
+        hipError_t e = hipSuccess;
+        for (int wsel = 1; wsel <= 2 && e == hipSuccess; wsel++) {
+            void *o = wsel == 1 ? (void *)r1 : (void *)rE;
+            if (a.K == 1) e = nq_q8f_moment_k1(a, sh, wsel, part, scratch, o, st);
+            else if (a.K == 2) e = nq_q8f_moment_k2(a, sh, wsel, part, scratch, o, st);
+            else e = nq_q8f_moment_k4(a, sh, wsel, part, scratch, o, st);
+        }
+        if (e != hipSuccess) return e;
+        return nq_launch_svf_sq_pack(false, a0, svf, part, scratch, tmp, so, ss, sf, st);
+    }
     hipError_t e = launch_bwd<1>(a, sh, st);
     if (e == hipSuccess) e = nq_chunk_reduce(false, part, a.P + 1, a.P + 1, sh.bwd_blocks, scratch, r1, st);
     if (e == hipSuccess) e = launch_bwd<2>(a, sh, st);

@@ -232,6 +232,45 @@ def test_q8c_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
     assert herm < 1e-11, herm
 
 
+@pytest.mark.parametrize("N,M,K,C,m", [(8, 16, 1, 37, 5), (8, 16, 2, 64, 8), (12, 24, 2, 45, 7), (16, 64, 2, 33, 9),
+                                        (16, 16, 4, 40, 5), (32, 64, 2, 96, 9), (24, 24, 4, 50, 4), (32, 32, 2, 70, 11)])
+def test_q8_one_pass_matches_two_pass(nqes, monkeypatch, N, M, K, C, m):
+    """The one-pass int8 S*v (real weights, N <= 32, K in {1, 2, 4}, K M <= 128: the
+    factor digits read once, the forward products, the row weights and the backward
+    products of each 64-row chunk in one kernel) against the two-pass form
+    (NQES_Q8_FUSED=0) on the same samples: the same factor digits and the same
+    quantisation of v; the one-pass form quantises the row weights with a running scale
+    (at most 2^2 above the largest weight so far) instead of one per block of rows, and
+    adds the a block (the K columns c) in FP64 instead of on int8 MFMA, so the two agree
+    to ~1e-13 relative (bound 1e-11).  The moments (the same backward products with the
+    weights 1 and E) likewise; row counts S K that are not multiples of 128 exercise the
+    padding, and a v with one block 1e-3 of the rest the running scale."""
+    print(f"case N={N} M={M} K={K} C={C} m={m}", flush=True)
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=21 + N + K, scale=0.1, complex_=False)
+    P = K * (N + M + N * M)
+    rng = np.random.default_rng(23)
+    v1 = rng.normal(size=P) + 0j
+    v2 = v1.copy()
+    v2[: P // 3] *= 1e-3
+    out = {}
+    monkeypatch.delenv("NQES_SV", raising=False)
+    for mode in ("1", "0"):
+        monkeypatch.setenv("NQES_Q8_FUSED", mode)
+        with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+            s.sample(W, 81, C, m, 3, 0.5)
+            out[mode] = ([s.apply_S(v) for v in (v1, v2, v1 + 2.5 * v2)], s.moments())
+    (a1, a2, a12), ma = out["1"]
+    (b1, b2, _), mb = out["0"]
+    assert rel_err(a1, b1) < 1e-11, rel_err(a1, b1)
+    assert rel_err(a2, b2) < 1e-11, rel_err(a2, b2)
+    assert rel_err(a12, a1 + 2.5 * a2) < 1e-11, rel_err(a12, a1 + 2.5 * a2)
+    sym = abs(np.vdot(v2, a1) - np.vdot(v1, a2)) / (np.linalg.norm(v1) * np.linalg.norm(a2))
+    assert sym < 1e-11, sym
+    for key in ("o", "skk", "f", "emat"):
+        assert rel_err(ma[key], mb[key]) < 1e-11, (key, rel_err(ma[key], mb[key]))
+
+
 @pytest.mark.parametrize("cplx", [False, True])
 @pytest.mark.parametrize("N,M,K,C,m", [(8, 16, 1, 37, 5), (17, 34, 3, 29, 7), (32, 64, 2, 96, 9), (24, 24, 4, 50, 4)])
 def test_q8_forward_passes_agree(nqes, monkeypatch, N, M, K, C, m, cplx):
