@@ -343,6 +343,30 @@ def sv_roofline(args, stats, real):
         return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args, real, "stored"),
                 "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes}
+    fused = (q8 and real and args.N <= 32 and args.K in (1, 2, 4) and (args.K * args.M + 63) // 64 <= 2
+             and os.environ.get("NQES_Q8_FUSED") != "0")
+    if fused:
+        # one-pass int8 form: the digits of the K M tanh columns (3 bytes per row and column of
+        # the 64-column groups) read once, with per row the quantised c columns (4 B each), the
+        # scale (8 B) and the spin word; the blocks' partial rows written and read back once
+        R = S * args.K
+        ncg, nst = (args.K * args.M + 63) // 64, (args.N + 15) // 16
+        rpad = (R + 127) // 128 * 128
+        nsc = rpad // 128
+        nb = min(256, nsc)
+        while nb * 256 < rpad // 64:
+            nb *= 2
+        sv_bytes = R * (3 * 64 * ncg + 4 * args.K + 8 + 4) + 2 * nb * (P + 1) * 8
+        # int8 MFMAs per 128-row super-chunk: forward 8 row tiles x ncg (15 nst + 3), backward
+        # 4 ncg column tiles x 2 chunks x (15 nst + 3)
+        mfma_ops = nsc * 16 * ncg * (15 * nst + 3) * 16 * 16 * 64 * 2
+        achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
+        return {"bound": "hbm", "kernel": "S*v: k_q8_fused (one pass over the int8 factor digits: forward GEMM, "
+                                          "row weights and backward GEMM per 128-row super-chunk)",
+                "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": profiled_traffic(args, real, "q8f"), "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes,
+                "int8_mfma_tops": mfma_ops / (sv_avg_ms * 1e-3) / 1e12 if sv_calls else 0.0,
+                "int8_mfma_peak_tops": INT8_MFMA_PEAK_TOPS}
     if q8:
         # int8 form: each pass reads the quantised factors once (3 digit bytes per row
         # and column), plus per row its scale (8 B) and spin words, and w once per sample;

@@ -42,8 +42,8 @@
 //
 // LDS per block: the forward B operands (NCG (6 NST + 1) KB), and twice (super-chunk
 // parity) the transposed digits XT [column tile][t][row tile][lane'] dwords (lane' = 16 g'
-// + lo' swizzled), the weight digits [u][g'][row tile] dwords, the spin masks
-// [st][i][g'][row tile] dwords and the wavefronts' largest |omega|.
+// + lo' swizzled), the weight digits [chunk][g'][u][row tile] dwords, the spin masks
+// [st][g'][chunk][i][row tile] dwords and the wavefronts' largest |omega|.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -260,8 +260,8 @@ template <int NCG, int NST, int W>
 struct FzLds {
     static constexpr int NCT = 4 * NCG;
     static constexpr int XT_DW = NCT * QU * W * 64;
-    static constexpr int OD_DW = QW * 4 * W;
-    static constexpr int MK_DW = NST * 16 * 4 * W;
+    static constexpr int OD_DW = 8 * 4 * W;  // [chunk][g'][u, 8 slots][row tile of the chunk]
+    static constexpr int MK_DW = NST * 16 * 4 * W;  // [st][g'][chunk][i][row tile of the chunk]
     static constexpr int BUF_DW = XT_DW + OD_DW + MK_DW + 2 * W;
     static constexpr int NSLOT = NST * QV + 1;
     static constexpr int NQ = NCG * NSLOT * 64;  // uint4 of the forward B operands
@@ -470,9 +470,12 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
                     }
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    double hs = (double)af[NSF - 1][e];
+                    // digit weights in pairs first: |af| <= 6 2^20 (three digit pairs and two
+                    // column groups of 64 products of at most 2^14), so af + 256 af' fits int32
+                    double hs = 0.0;
 #pragma unroll
-                    for (int s = NSF - 2; s >= 0; s--) hs = fma(hs, 256.0, (double)af[s][e]);
+                    for (int s = NSF - 2; s >= 0; s -= 2)
+                        hs = fma(hs, 65536.0, (double)(int)((uint32_t)af[s][e] + ((uint32_t)af[s + 1][e] << 8)));
                     hs *= 65536.0;  // 256^QSKIP
 #pragma unroll
                     for (int k = 0; k < KS; k++) hs = fma((double)xc[e][k], qa[st][k], hs);
@@ -540,20 +543,19 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
             uint32_t bits = 0u;
 #pragma unroll
             for (int j = 0; j < 4; j++) bits |= ((word[j] >> (16 * st + lo)) & 1u) << j;
-            mk[((st * 16 + lo) * 4 + g) * W + wv] = fz_expand4(bits) * 0xffu;
+            mk[((((st * 4 + g) * NKC + (wv >> 2)) * 16 + lo) * 4) + (wv & 3)] = fz_expand4(bits) * 0xffu;
         }
         // weight digits at the block's current scale (rows that do not fit are redone below)
+        // (balanced digits: q + 128 (1 + 256 + ... + 256^5) has the plain base-256 digits
+        // d_u + 128, so byte u of that sum with its top bit flipped is d_u)
         auto put_digits = [&]() {
             long long q = 0;
             if (gvalid && fabs(om) < ldexp(1.0, G)) q = (long long)rint(om * ldexp(1.0, WBITS - G));
+            const unsigned long long b = (unsigned long long)(q + 0x808080808080LL) ^ 0x808080808080ULL;
             if (j4 == 0) {
-                uint8_t *ob = (uint8_t *)od;
+                uint8_t *ob = (uint8_t *)od + (((wv >> 2) * 4 + g) * 8 * 4 + (wv & 3)) * 4 + er;
 #pragma unroll
-                for (int u = 0; u < QW; u++) {
-                    const long long dd = (u < QW - 1) ? bal8l(q) : q;
-                    ob[((u * 4 + g) * W + wv) * 4 + er] = (uint8_t)(dd & 255);
-                    q = (q - dd) >> 8;
-                }
+                for (int u = 0; u < QW; u++) ob[u * 16] = (uint8_t)(b >> (8 * u));
             }
         };
         put_digits();
@@ -599,7 +601,7 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
                 }
             }
             {
-                const uint4 ta = od4[((lo < QW ? lo : 0) * 4 + g) * NKC + kc];
+                const uint4 ta = od4[(kc * 4 + g) * 8 + (lo < QW ? lo : 0)];
                 const v4i tv = lo < QW ? as_v4i(ta) : v4i{0, 0, 0, 0};
 #pragma unroll
                 for (int c = 0; c < CTW; c++)
@@ -609,10 +611,10 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
             }
 #pragma unroll
             for (int st = 0; st < NST; st++) {
-                const uint4 mm = mk4[((st * 16 + lo) * 4 + g) * NKC + kc];
+                const uint4 mm = mk4[((st * 4 + g) * NKC + kc) * 16 + lo];
 #pragma unroll
                 for (int u = 0; u < QW; u++) {
-                    const uint4 o = od4[(u * 4 + g) * NKC + kc];
+                    const uint4 o = od4[(kc * 4 + g) * 8 + u];
                     const v4i av8 = v4i{(int)(o.x & mm.x), (int)(o.y & mm.y), (int)(o.z & mm.z), (int)(o.w & mm.w)};
 #pragma unroll
                     for (int c = 0; c < CTW; c++)
@@ -673,7 +675,7 @@ size_t fz_lds_bytes(int ncg, int nst, int mode)
 {
     const int w = fz_waves(ncg);
     const size_t nq = mode == 0 ? (size_t)ncg * (nst * QV + 1) * 64 * 16 : 0;
-    const size_t buf = ((size_t)4 * ncg * QU * w * 64 + QW * 4 * w + (size_t)nst * 16 * 4 * w + 2 * w) * 4;
+    const size_t buf = ((size_t)4 * ncg * QU * w * 64 + 8 * 4 * w + (size_t)nst * 16 * 4 * w + 2 * w) * 4;
     return nq + 2 * buf;
 }
 
