@@ -35,7 +35,9 @@ the excited state: the solves use the reference's complex initialisation.
     python bench.py --workload solve --precision fp64 --N 32 --M 64 ...
         one whole solve of the chosen shape and path (same rules), printed as JSON;
     python bench.py --workload scan|serving [--slots 8] [--solve-steps 200] [--requests 300]
-        whole solves through the solve server (continuous batching on one device).
+        whole solves through the solve server (continuous batching on one device);
+    python bench.py --workload corr [--cpu-C 128 --cpu-m 200]
+        the drop-in's --correlation-data computation, timed.
 """
 import argparse
 import json
@@ -487,6 +489,25 @@ def solve_workload(args, nqes):
     print(json.dumps(out))
 
 
+def corr_workload(args, nqes):
+    """The correlation functions of the final state (--correlation-data of the drop-in,
+    rbm.cc:722-826 and 1198-1229) on the FP64 path, timed, for the SR-step workload's
+    shape with the reference's complex init weights and --cpu-C x --cpu-m chains (the
+    published -c 128 -m 200 by default)."""
+    J = load_J(args.N)
+    W0 = reference_init(args.N, args.M, args.K, args.seed)
+    C, m = args.cpu_C, args.cpu_m
+    with nqes.Solver(args.N, args.M, args.K, args.h, J, precision=nqes.PREC_FP64) as s:
+        s.correlations(W0, args.seed, 8, 2, 2)  # warm-up: module load, first allocations
+        t0 = time.perf_counter()
+        x, z, xx, zz = s.correlations(W0, args.seed, C, m, args.B)
+        dt = time.perf_counter() - t0
+    print(json.dumps({"workload": f"correlation functions N={args.N} M={args.M} K={args.K} C={C} m={m} B={args.B} (fp64)",
+                      "seconds": dt, "samples_per_s": C * m / dt,
+                      "pair_evaluations_per_s": C * m * args.N * args.N / dt,
+                      "zz_trace": float(np.trace(zz[:, :, 0, 0]).real)}))
+
+
 def solve_requests(args):
     """(arrival time s, request) pairs of the scan or serving workload, seeded."""
     rng = np.random.default_rng(3 if args.workload == "scan" else 4)
@@ -521,6 +542,10 @@ def solve_requests(args):
 def serve_workload(args):
     import nqes
     reqs = solve_requests(args)
+    if args.workload == "scan":
+        # all submitted at once: the client submits the largest solves first (parameters P
+        # times samples), so that no large solve is left to run alone at the end
+        reqs.sort(key=lambda tq: -tq[1]["K"] * (tq[1]["N"] + tq[1]["M"] + tq[1]["N"] * tq[1]["M"]))
     sched = dict(tol=1e-4, lrate0=SCHED["lrate0"], lratef=SCHED["lratef"], lrate_cutoff=SCHED["lrate_cutoff"],
                  reglam0=SCHED["reglam0"], reglam_decay=SCHED["reglam_decay"], reglam_cutoff=SCHED["reglam_cutoff"],
                  rtol=args.rtol, maxit=MAXIT_SOLVE)
@@ -535,17 +560,18 @@ def serve_workload(args):
             if dt > 0:
                 time.sleep(dt)
             ids.append(srv.submit(**q, precision=nqes.PREC_MIXED, **sched))
-        # collect in completion order, reporting each as it ends and every 25 SR steps
-        # completed over the running solves
+        # collect in completion order, reporting each as it ends, every 25 SR steps
+        # completed over the running solves, and each completed step once 10 s passed
+        # since the last report (a few large solves at the end of a batch)
         res = [None] * len(ids)
         left = set(range(len(ids)))
-        steps_done, steps_shown = 0, 0
+        steps_done, steps_shown, t_shown = 0, 0, time.perf_counter()
         while left:
             running = sum(srv.progress(ids[k]) for k in left)
             finished = sum(res[k]["steps"] for k in range(len(ids)) if res[k] is not None)
             steps_done = running + finished
-            if steps_done >= steps_shown + 25:
-                steps_shown = steps_done
+            if steps_done >= steps_shown + 25 or (steps_done > steps_shown and time.perf_counter() - t_shown > 10):
+                steps_shown, t_shown = steps_done, time.perf_counter()
                 progress(f"{steps_done} SR steps done at {time.perf_counter() - t0:.1f}s, "
                          f"{len(ids) - len(left)}/{len(ids)} solves finished")
             for k in sorted(left):
@@ -625,7 +651,7 @@ def main():
     ap.add_argument("--cpu-m", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-gates", action="store_true", help="skip the converged-solve workloads")
-    ap.add_argument("--workload", default="srstep", choices=["srstep", "solve", "scan", "serving"])
+    ap.add_argument("--workload", default="srstep", choices=["srstep", "solve", "scan", "serving", "corr"])
     ap.add_argument("--slots", type=int, default=8, help="scan/serving: concurrent solves")
     ap.add_argument("--solve-steps", type=int, default=200, help="scan/serving: SR steps per solve at most")
     ap.add_argument("--requests", type=int, default=0, help="scan/serving: number of requests (default 64 / 300)")
@@ -644,6 +670,9 @@ def main():
     import nqes
     if args.workload == "solve":
         solve_workload(args, nqes)
+        return
+    if args.workload == "corr":
+        corr_workload(args, nqes)
         return
     out, J, W0 = srstep_workload(args, nqes)
     progress(f"SR step: {out['ms_per_step']:.1f} ms")

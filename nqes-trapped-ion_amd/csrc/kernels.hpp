@@ -134,6 +134,7 @@ struct MixedSampleArgs {
     uint32_t *spins;           // S*K*NW
     unsigned long long *accepts; // C
     unsigned int *flags;       // bit 0: an angle exceeded the FP32 range
+    int jlds;                  // k_measure_mx with its tables in L2: J in LDS
     void *U;                   // (S*K) x Ucols T: S*v factors (NULL: not stored)
     int Ucols;
     float *Umax;               // S*K: max |U| of each factor row (real weights; NULL: not written)
@@ -251,6 +252,7 @@ hipError_t nq_sweep_k(bool cplx, const MixedSampleArgs &a, const void *tabp, con
 // Local energies and O / factor rows (sampler_mx.hip) from the sampler's padded tables
 struct MeasureShape {
     int ok, G, JQ, wpb, tab;
+    int jlds;  // tables in L2: J staged in LDS (it fits next to the per-sample states)
 };
 MeasureShape nq_measure_shape(bool cplx, int N, int K, const SweepShape &sw, size_t lds_max);
 template <int K>

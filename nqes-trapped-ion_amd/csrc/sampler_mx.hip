@@ -245,8 +245,8 @@ __device__ __attribute__((always_inline)) inline void sweep_refresh(const MixedS
     __builtin_amdgcn_wave_barrier();
 }
 
-// Metropolis chains (see the file comment).  Tables in LDS when TAB, else read from
-// the padded global copy (L2-resident).  Replica k's angles and Phi row are held in
+// Metropolis chains (see the file comment).  Tables in LDS when TAB, else the cosh/sinh
+// rows read from the padded global copy (L2-resident) and exp(+-2a), a staged in LDS.  Replica k's angles and Phi row are held in
 // registers for its block of N flips.  After each sampled sweep the spins are stored
 // (sample s = c m + t); the local energies and O rows are computed afterwards by
 // k_measure_mx, one lane group per sample.
@@ -283,6 +283,14 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
     }
     const PadLayout PL = pad_layout(CPLX, N, K, Mp);
     const D *eaP = dtab, *eaN = dtab + KN, *aD = dtab + 2 * KN;
+    if constexpr (!TAB) {  // exp(+-2a) and a in LDS: read every flip
+        D *le = (D *)smem;
+        for (long long q = tid; q < 3 * KN; q += blockDim.x) le[q] = dtab[q];
+        eaP = le;
+        eaN = le + KN;
+        aD = le + 2 * KN;
+        p = smem + ((3 * KN * sizeof(D) + 15) & ~(size_t)15);
+    }
     const ChSh<T> *cs = (const ChSh<T> *)((const unsigned char *)dtab + PL.cs);
     const T *wp = (const T *)((const unsigned char *)dtab + PL.w);
     const T *bp = (const T *)((const unsigned char *)dtab + PL.b);
@@ -333,15 +341,6 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
                     for (int q = 0; q < JQ; q++) tk[kk][q] = tls[((k * K + kk) * JQ + q) * G + g];
                 }
             }
-            // table row of the next flip, loaded one flip ahead from global memory
-            // (L2 latency); from LDS it is loaded at the top of its own flip
-            ChSh<T> nx[TAB ? 1 : K][TAB ? 1 : JQ];
-            if constexpr (!TAB) {
-#pragma unroll
-                for (int kk = 0; kk < K; kk++)
-#pragma unroll
-                    for (int q = 0; q < JQ; q++) nx[kk][q] = cs[((long long)kk * N) * CSR + g + G * q];
-            }
             uint32_t cur = sp[k * NW];
             for (int i = 0; i < N; i++) {
                 ChSh<T> cv[K][JQ];
@@ -354,14 +353,12 @@ __global__ __launch_bounds__(256) void k_sample_mx(MixedSampleArgs a, const void
 #pragma unroll
                         for (int q = 0; q < JQ; q++) cv[kk][q] = r0[kk * N * CSR + G * q];
                 } else {
-                    const int i1 = (i + 1 < N) ? i + 1 : i;
+                    // from L2 at the top of the flip (a one-flip-ahead prefetch into registers
+                    // held 2 K JQ more and, at K = 4, halved the wavefronts per SIMD: slower)
 #pragma unroll
                     for (int kk = 0; kk < K; kk++)
 #pragma unroll
-                        for (int q = 0; q < JQ; q++) {
-                            cv[kk][q] = nx[kk][q];
-                            nx[kk][q] = cs[((long long)kk * N + i1) * CSR + g + G * q];
-                        }
+                        for (int q = 0; q < JQ; q++) cv[kk][q] = cs[((long long)kk * N + i) * CSR + g + G * q];
                 }
                 if (i > 0 && (i & 31) == 0) cur = sp[k * NW + (i >> 5)];
                 const bool si = (cur >> (i & 31)) & 1u;
@@ -771,6 +768,20 @@ __global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a, const voi
         dtab = (const D *)tabp;
     }
     const D *eaP = dtab, *eaN = dtab + KN, *aD = dtab + 2 * KN;
+    if constexpr (!TAB) {  // exp(+-2a), a and (when a.jlds) J in LDS; the cosh/sinh, w, b tables from L2
+        D *le = (D *)smem;
+        for (long long q = tid; q < 3 * KN; q += blockDim.x) le[q] = dtab[q];
+        eaP = le;
+        eaN = le + KN;
+        aD = le + 2 * KN;
+        p = smem + al16((size_t)3 * KN * sizeof(D));
+        if (a.jlds) {
+            double *jl = (double *)p;
+            for (int q = tid; q < N * N; q += blockDim.x) jl[q] = a.J[q];
+            Jm = jl;
+            p += al16((size_t)N * N * sizeof(double));
+        }
+    }
     const ChSh<T> *cs = (const ChSh<T> *)((const unsigned char *)dtab + PL.cs);
     const T *wp = (const T *)((const unsigned char *)dtab + PL.w);
     const T *bp = (const T *)((const unsigned char *)dtab + PL.b);
@@ -803,6 +814,7 @@ hipError_t launch_measure_jq(const MixedSampleArgs &a, const void *tabp, const M
     const int spb = sh.wpb * (64 / G);
     size_t lds = (size_t)spb * chain_lds_bytes<CPLX, K, G, JQ>(a.NW);
     if (sh.tab) lds += al16(pad_tab_bytes(CPLX, a.N, K, G * JQ) + (size_t)a.N * a.N * sizeof(double));
+    else lds += al16((size_t)3 * K * a.N * (CPLX ? 16 : 8)) + (a.jlds ? al16((size_t)a.N * a.N * sizeof(double)) : 0);
     const long long S = (long long)a.C * a.m;
     long long blocks = (S + spb - 1) / spb;
     const void *fn = sh.tab ? (const void *)k_measure_mx<CPLX, K, G, JQ, true> : (const void *)k_measure_mx<CPLX, K, G, JQ, false>;
@@ -856,6 +868,7 @@ hipError_t launch_jq(const MixedSampleArgs &a, const void *tabp, bool tab, int w
     const int cpb = wpb * (64 / G);
     size_t lds = (size_t)cpb * chain_lds_bytes<CPLX, K, G, JQ>(a.NW);
     if (tab) lds += pad_tab_bytes(CPLX, a.N, K, G * JQ);
+    else lds += ((size_t)3 * K * a.N * (CPLX ? 16 : 8) + 15) & ~(size_t)15;
     const int blocks = (a.C + cpb - 1) / cpb;
     const void *fn = tab ? (const void *)k_sample_mx<CPLX, K, G, JQ, true> : (const void *)k_sample_mx<CPLX, K, G, JQ, false>;
     if (lds > 65536) {
@@ -941,9 +954,10 @@ SweepShape nq_sweep_shape(bool cplx, int N, int M, int K, size_t lds_max)
         (size_t)K * K * JQ * G * (cplx ? 8 : 4) + (size_t)K * K * (cplx ? 16 : 8) + (size_t)K * NW * 4;
     auto states = [&](int w) { return (size_t)w * (64 / G) * per_chain; };
     const size_t tb = pad_tab_bytes(cplx, N, K, G * JQ);
+    const size_t ea = al16((size_t)3 * K * N * (cplx ? 16 : 8));  // staged when the tables stay in L2
     if (lds_max > 0) {
-        while (wpb > 1 && states(wpb) > lds_max) wpb--;
-        if (states(wpb) > lds_max) return sh;
+        while (wpb > 1 && states(wpb) + ea > lds_max) wpb--;
+        if (states(wpb) + ea > lds_max) return sh;
     }
     sh.ok = 1;
     sh.G = G;
@@ -979,6 +993,10 @@ MeasureShape nq_measure_shape(bool cplx, int N, int K, const SweepShape &sw, siz
     const size_t tb = al16(pad_tab_bytes(cplx, N, K, Mp) + (size_t)N * N * sizeof(double));
     const char *et = getenv("NQES_TABLDS");
     sh.tab = (tb + states <= lds_max) && !(et && *et && atoi(et) == 0);
+    // tables in L2: exp(+-2a) and a in LDS, and J when it fits too
+    const size_t ea = al16((size_t)3 * K * N * (cplx ? 16 : 8));
+    if (!sh.tab && lds_max > 0 && states + ea > lds_max) return sh;
+    sh.jlds = !sh.tab && (lds_max == 0 || states + ea + al16((size_t)N * N * sizeof(double)) <= lds_max);
     sh.ok = 1;
     return sh;
 }
@@ -987,11 +1005,13 @@ hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, cons
                                 hipStream_t st)
 {
     if (!sh.ok) return hipErrorInvalidValue;
+    MixedSampleArgs b = a;
+    b.jlds = sh.jlds;
     switch (K) {
-    case 1: return nq_measure_k<1>(cplx, a, tabp, sh, st);
-    case 2: return nq_measure_k<2>(cplx, a, tabp, sh, st);
-    case 3: return nq_measure_k<3>(cplx, a, tabp, sh, st);
-    case 4: return nq_measure_k<4>(cplx, a, tabp, sh, st);
+    case 1: return nq_measure_k<1>(cplx, b, tabp, sh, st);
+    case 2: return nq_measure_k<2>(cplx, b, tabp, sh, st);
+    case 3: return nq_measure_k<3>(cplx, b, tabp, sh, st);
+    case 4: return nq_measure_k<4>(cplx, b, tabp, sh, st);
     default: return hipErrorInvalidValue;
     }
 }
