@@ -38,7 +38,8 @@ enum {
     NQES_ENOMEM = 2,     /* device allocation failed / does not fit in HBM  */
     NQES_EDEVICE = 3,    /* HIP runtime error                               */
     NQES_ENONFINITE = 4, /* non-finite energy or gradient (rbm.cc's SIGFPE) */
-    NQES_ESTATE = 5      /* call order violated (e.g. solve before sample)  */
+    NQES_ESTATE = 5,     /* call order violated (e.g. solve before sample)  */
+    NQES_ECANCELED = 6   /* solve cancelled through nqes_server_cancel       */
 };
 
 enum {
@@ -253,6 +254,12 @@ int nqes_server_progress(nqes_server *srv, uint64_t id, int *steps);
 /* Blocks until solve `id` finishes; copies its final weights (P complex, may be
  * NULL) and result, then forgets the solve.  Returns the solve's status. */
 int nqes_server_wait(nqes_server *srv, uint64_t id, double *W, nqes_result *res);
+/* Cancels solve `id` (no counterpart in the reference, whose run() is one solve per
+ * process: a serving deadline).  A queued request finishes at once, a running one
+ * after its current SR step; either way with status NQES_ECANCELED and the weights and
+ * energies of its last completed step.  A finished solve is left as it is.  The id
+ * stays valid for nqes_server_wait. */
+int nqes_server_cancel(nqes_server *srv, uint64_t id);
 
 /* Library version string. */
 const char *nqes_version(void);

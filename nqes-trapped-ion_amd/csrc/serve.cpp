@@ -54,6 +54,7 @@ struct Job {
     bool done = false;
     bool claimed = false;  // a waiter has taken it (one waiter per id)
     std::atomic<int> progress{0};  // SR steps completed so far (read by nqes_server_progress)
+    std::atomic<bool> cancel{false};  // nqes_server_cancel: stop before the next SR step
     nqes_result res{};
     std::string err;  // message of a failed solve
 };
@@ -163,6 +164,10 @@ struct nqes_server {
         srloop::Convergence conv(q.K);
         std::vector<std::complex<double>> emat(q.K * q.K);
         for (int it = 0; rc == NQES_OK && it < q.niters; it++) {
+            if (j->cancel.load(std::memory_order_relaxed)) {
+                rc = NQES_ECANCELED;
+                break;
+            }
             nqes_stats st{};
             rc = nqes_sr_step(ctx, j->W.data(), q.seed + (uint64_t)it * q.C, q.C, q.m, q.B, sched.step_lrate(),
                               sched.step_reglam(), q.rtol, q.maxit, reinterpret_cast<double *>(emat.data()), &st);
@@ -180,7 +185,9 @@ struct nqes_server {
             }
         }
         r.status = rc;
-        if (rc != NQES_OK) {
+        if (rc == NQES_ECANCELED) {
+            j->err = "solve cancelled";
+        } else if (rc != NQES_OK) {
             j->err = nqes_last_error(ctx);
             // a failed step may leave the context mid-solve; start the next one fresh
             nqes_destroy(ctx);
@@ -344,6 +351,37 @@ int nqes_server_progress(nqes_server *s, uint64_t id, int *steps)
         return NQES_EINVAL;
     }
     *steps = it->second->progress.load(std::memory_order_relaxed);
+    return NQES_OK;
+}
+
+int nqes_server_cancel(nqes_server *s, uint64_t id)
+{
+    if (!s) return NQES_EINVAL;
+    bool notify = false;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(id);
+        if (it == s->jobs.end()) {
+            s->err = "unknown request id";
+            return NQES_EINVAL;
+        }
+        Job *j = it->second.get();
+        if (j->done) return NQES_OK;
+        j->cancel.store(true, std::memory_order_relaxed);
+        for (auto qi = s->queue.begin(); qi != s->queue.end(); ++qi)
+            if (*qi == j) {  // still queued: finished now, never admitted
+                s->queue.erase(qi);
+                j->res.K = j->req.K;
+                j->res.status = NQES_ECANCELED;
+                j->err = "solve cancelled";
+                j->res.t_admit_ms = j->res.t_done_ms = ms_since(s->t0);
+                j->done = true;
+                notify = true;
+                break;
+            }
+        if (notify) s->admit();  // a blocked request may fit now
+    }
+    if (notify) s->cv_done.notify_all();
     return NQES_OK;
 }
 

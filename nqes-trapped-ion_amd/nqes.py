@@ -21,6 +21,7 @@ PREC_MIXED = 1
 MODEL_TRAPPEDION = 0  # ising-like: trapped-ion and the diagonal models (J given)
 MODEL_HS = 1          # hs-like, delta = 1 (Haldane-Shastry with its J)
 MODEL_XXZ = 2         # hs-like, delta = -1
+ECANCELED = 6         # status of a solve ended by Server.cancel
 
 _cp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -99,7 +100,7 @@ EXPORTS = [
     "nqes_solve", "nqes_correlations", "nqes_required_bytes", "nqes_version",
     "nqes_set_dense_S", "nqes_enumerate", "nqes_get_ulp_ties", "nqes_apply_S_rows_noalloc",
     "nqes_server_create", "nqes_server_destroy", "nqes_server_last_error", "nqes_server_submit",
-    "nqes_server_poll", "nqes_server_progress", "nqes_server_wait",
+    "nqes_server_poll", "nqes_server_progress", "nqes_server_wait", "nqes_server_cancel",
 ]
 
 
@@ -161,6 +162,8 @@ def load_library(path=LIB_PATH):
     L.nqes_server_progress.argtypes = [_vp, c_u64, ctypes.POINTER(c_int)]
     L.nqes_server_wait.restype = c_int
     L.nqes_server_wait.argtypes = [_vp, c_u64, _vp, ctypes.POINTER(Result)]
+    L.nqes_server_cancel.restype = c_int
+    L.nqes_server_cancel.argtypes = [_vp, c_u64]
     _lib = L
     return L
 
@@ -415,6 +418,13 @@ class Server:
         if st != 0:
             d["error"] = self._err()
         return W, d
+
+    def cancel(self, rid):
+        """Cancels solve rid: a queued request ends at once, a running one after its
+        current SR step (status ECANCELED); a finished one is left as it is."""
+        st = self.lib.nqes_server_cancel(self.srv, int(rid))
+        if st != 0:
+            raise NqesError(f"nqes_server_cancel status {st}: {self._err()}")
 
     def close(self):
         if self.srv:

@@ -270,3 +270,29 @@ def test_concurrent_launches_of_one_kernel_with_different_lds():
     for N, _ in cases:
         for a, b in zip(alone[N], outs[N][0]):
             assert np.array_equal(a, b), N
+
+
+def test_cancel_queued_and_running():
+    """nqes_server_cancel (a serving deadline): a queued request ends at once with status
+    ECANCELED and no step run; a running one ends after its current SR step, with the
+    weights of its completed steps, equal bit for bit to the same request run for that
+    many steps; the server then serves the next request as before."""
+    q = request(MIX[3])
+    sched = dict(SCHED, tol=0.0)  # never converges: only the cancel ends the solve
+    with nqes.Server(max_slots=1) as srv:
+        r1 = srv.submit(**dict(q, niters=400), **sched)
+        r2 = srv.submit(**dict(q, niters=400), **sched)
+        t0 = time.time()
+        while srv.progress(r1) < 2 and time.time() - t0 < 120:
+            time.sleep(0.01)
+        srv.cancel(r2)
+        _, d2 = srv.wait(r2, raise_on_error=False)
+        assert d2["status"] == nqes.ECANCELED and d2["steps"] == 0
+        srv.cancel(r1)
+        W1, d1 = srv.wait(r1, raise_on_error=False)
+        assert d1["status"] == nqes.ECANCELED and 2 <= d1["steps"] < 400, d1
+        Wn, dn = srv.wait(srv.submit(**dict(q, niters=d1["steps"]), **sched))
+        assert dn["status"] == 0 and dn["steps"] == d1["steps"]
+        assert np.array_equal(W1, Wn)
+        assert np.array_equal(np.asarray(d1["energies"]), np.asarray(dn["energies"]))
+
