@@ -437,6 +437,7 @@ def srstep_workload(args, nqes):
             emat, st = one_step()
             stats.append(st)
         dt = time.perf_counter() - t0
+        launch = solver.sampler_launch() if prec == nqes.PREC_MIXED else None
     real = not np.any(W0.imag)
     out = {
         "metric": "converged solves/hour (dE<1e-4 vs FP64); MC samples/s; p50/p99 solve latency",
@@ -466,6 +467,7 @@ def srstep_workload(args, nqes):
         "accept_rate": sum(s["accepts"] for s in stats) / max(sum(s["proposals"] for s in stats), 1),
         "ulp_ties": int(sum(s["ulp_ties"] for s in stats)),
         "energies_last_step": np.sort(np.linalg.eigvals(emat / args.N).real).tolist(),
+        "sampler_launch_last_step": launch,
         "roofline": sv_roofline(args, stats, real),
     }
     return out, J, W0

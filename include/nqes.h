@@ -129,6 +129,18 @@ int nqes_get_moments(nqes_ctx *ctx, double *o, double *skk, double *f, double *e
  * counterpart is the accept test of monte_carlo_step (rbm.cc:540-543), which has no
  * such count; 0 on the mixed path. */
 int nqes_get_ulp_ties(nqes_ctx *ctx, uint64_t *per_chain, uint64_t *total);
+/* Launch shape of the last mixed-precision nqes_sample (the reference runs one chain per
+ * OpenMP thread, rbm.cc:564-600, and has no counterpart): Metropolis chains per block of
+ * the sweep kernel, whether its tables were staged in LDS (1) or read from L2 (0), and
+ * samples per block of the measurement kernel.  These shapes are chosen by the library's
+ * tuner: the first sampling calls of each problem shape (device, real or complex weights,
+ * N, M, K, C, m, B, model) in the process try candidate shapes in turn, timed on the
+ * device, and later calls use the fastest.  No lane computes anything else under another
+ * shape, so results do not depend on the choice.  tuning: 1 while the shape is being tuned,
+ * 0 once its choice is fixed, -1 with the tuner off (NQES_TUNE=0, or NQES_WPB / NQES_TABLDS
+ * set).  Any pointer may be NULL. */
+int nqes_get_sampler_launch(nqes_ctx *ctx, int *chains_per_block, int *tables_in_lds,
+                            int *samples_per_block, int *tuning);
 int nqes_apply_S(nqes_ctx *ctx, const double *v, double *out);
 
 /*

@@ -259,8 +259,10 @@ template <int K>
 hipError_t nq_measure_k(bool cplx, const MixedSampleArgs &a, const void *tabp, const MeasureShape &sh, hipStream_t st);
 hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, const MeasureShape &sh, const void *tabp,
                                 hipStream_t st);
+// tabp: the padded tables (sweeps); tabm: the measurement's copy (same size, cosh/sinh
+// pairs spin-major); both written here
 hipError_t nq_launch_sweep_mx(bool cplx, int K, const MixedSampleArgs &a, const SweepShape &sh, void *tabp,
-                              hipStream_t st);
+                              void *tabm, hipStream_t st);
 
 hipError_t nq_launch_moments_mx(bool cplx, const void *O, const void *E, long long S, long long P, long long Pld,
                                 int nchunk, void *po, void *ps, void *pf, void *scratch, void *out_o, void *out_s,

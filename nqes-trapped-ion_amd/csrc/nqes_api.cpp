@@ -15,12 +15,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -93,6 +95,48 @@ int choose_chunks(long long S, long long ncol, int cols_per_block)
     return (int)nch;
 }
 
+// ---- launch-shape tuner of the mixed sampler.  The sweeps' chains per block (wavefronts
+// per block), the placement of their tables (LDS or L2) and the measurement kernel's
+// samples per block spread the same per-lane work over the CUs differently: no lane
+// computes anything else, so spins, local energies and factor rows are the same bit for
+// bit for every choice.  The first sampling calls of each problem shape in the process
+// (device, real or complex weights, N, M, K, C, m, B, model) try the candidates in turn,
+// timed by the sampling events; later calls take the fastest (ties: the earlier
+// candidate).  NQES_TUNE=0, NQES_WPB or NQES_TABLDS turn it off.
+struct SamplerChoice {
+    int sweep_wpb, sweep_tab, measure_wpb;
+};
+struct TuneEntry {
+    std::vector<SamplerChoice> cand;
+    std::vector<double> ms;  // < 0: not timed yet
+    size_t next = 0;         // next candidate to hand out
+    int best = -1;           // fixed once every candidate is timed
+};
+using TuneKey = std::array<long long, 9>;
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tune;
+
+bool tuner_on()
+{
+    const char *t = getenv("NQES_TUNE"), *w = getenv("NQES_WPB"), *l = getenv("NQES_TABLDS");
+    return !(t && *t && atoi(t) == 0) && !(w && *w) && !(l && *l);
+}
+
+// the candidates around the default shapes: sweep wavefronts per block wpb, wpb / 2; the
+// tables from L2 as well when the default puts them in LDS; measurement 4 or 2 wavefronts
+std::vector<SamplerChoice> tune_candidates(const SweepShape &sw, const MeasureShape &ms)
+{
+    std::vector<int> swpb{sw.wpb}, tabs{sw.tab}, mwpb{ms.wpb};
+    if (sw.wpb > 1) swpb.push_back(sw.wpb / 2);
+    if (sw.tab) tabs.push_back(0);
+    if (ms.wpb > 1) mwpb.push_back(ms.wpb / 2);
+    std::vector<SamplerChoice> out;  // the default shapes first
+    for (int w : swpb)
+        for (int t : tabs)
+            for (int mw : mwpb) out.push_back(SamplerChoice{w, t, mw});
+    return out;
+}
+
 int env_int(const char *name, int dflt)
 {
     const char *v = getenv(name);
@@ -115,7 +159,7 @@ struct nqes_ctx {
     double h = 0;
     DevBuf dJ, dW, dchT, dshT, deaP, deaN;
     DevBuf dwT, dbT, daD, dspins0, drng0, dtheta0, dflags;  // mixed path
-    DevBuf dtabp;                                      // its sampler's padded tables
+    DevBuf dtabp, dtabm;                               // its sampler's padded tables, the measurement's copy
 
     // sample state
     bool sampled = false;
@@ -152,6 +196,12 @@ struct nqes_ctx {
     uint64_t sv_calls = 0;
     double sv_ms = 0;
     int nonfinite = 0;
+    // sampler launch of the last mixed nqes_sample and its tuner state (tune_idx >= 0: a
+    // trial whose time nqes_sample records under tune_key)
+    SamplerChoice launch{0, 0, 0};
+    int tune_state = -1;  // 1: trial, 0: tuned choice, -1: tuner off
+    int tune_idx = -1;
+    TuneKey tune_key{};
 
     int fail(int code, const char *fmt, ...)
     {
@@ -262,7 +312,7 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
     long long b = 8LL * N * N + 16 * P + 32LL * K * N * M + 32LL * K * N;  // J, W, tables
     if (mixed) b += 8LL * K * N * M + 8LL * K * M + 16LL * K * N + 16;
     if (mixed) b += (long long)C * K * NW * 4 + C * 16LL + (long long)C * K * K * M * 8;  // chain starts
-    if (mixed) b += (long long)nq_sweep_shape(cplx, N, M, K, 0).tab_bytes;                 // sampler tables
+    if (mixed) b += 2 * (long long)nq_sweep_shape(cplx, N, M, K, 0).tab_bytes;  // sampler tables, measurement copy
     SvfShape sh{};
     bool svf = false;
     if (mixed) {
@@ -444,6 +494,59 @@ static SvfArgs q8c_args(nqes_ctx *c)
     return f;
 }
 
+// the sampler launch of this call: the tuned choice of its shape, the next untimed
+// candidate while the shape is being tuned, or the default shapes (tuner off)
+static void tune_sampler(nqes_ctx *c, int C, int m, int B, SweepShape &sw, MeasureShape &ms)
+{
+    c->tune_idx = -1;
+    c->tune_state = -1;
+    if (tuner_on()) {
+        const TuneKey key{c->device, c->cplx ? 1 : 0, c->N, c->M, c->K, C, m, B, c->model};
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        TuneEntry &e = g_tune[key];
+        if (e.cand.empty()) {
+            e.cand = tune_candidates(sw, ms);
+            e.ms.assign(e.cand.size(), -1.0);
+        }
+        int idx = e.best;
+        if (idx < 0 && e.next < e.cand.size()) {  // a trial
+            idx = (int)e.next++;
+            c->tune_idx = idx;
+            c->tune_key = key;
+            c->tune_state = 1;
+        } else if (idx < 0) {  // every candidate handed out, not all timed yet: the fastest so far
+            idx = 0;
+            for (size_t q = 1; q < e.cand.size(); q++)
+                if (e.ms[q] >= 0 && (e.ms[idx] < 0 || e.ms[q] < e.ms[idx])) idx = (int)q;
+            c->tune_state = 1;
+        } else {
+            c->tune_state = 0;
+        }
+        sw.wpb = e.cand[idx].sweep_wpb;
+        sw.tab = e.cand[idx].sweep_tab;
+        ms.wpb = e.cand[idx].measure_wpb;
+    }
+    c->launch = SamplerChoice{sw.wpb, sw.tab, ms.wpb};
+}
+
+// a trial's sampling time (chain start, sweeps, measurement); the choice is fixed once
+// every candidate of the shape has one
+static void tune_record(nqes_ctx *c, double ms_sample)
+{
+    if (c->tune_idx < 0) return;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneEntry &e = g_tune[c->tune_key];
+    if ((size_t)c->tune_idx < e.ms.size()) e.ms[c->tune_idx] = ms_sample > 0 ? ms_sample : 1e30;
+    c->tune_idx = -1;
+    if (e.best >= 0 || e.next < e.cand.size()) return;
+    int best = 0;
+    for (size_t q = 0; q < e.ms.size(); q++) {
+        if (e.ms[q] < 0) return;
+        if (e.ms[q] < e.ms[best]) best = (int)q;
+    }
+    e.best = best;
+}
+
 static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
 {
     const long long S = (long long)C * m;
@@ -462,11 +565,13 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     HIPCHK(c, nq_launch_theta_gemm(cplx, c->N, c->M, c->K, c->NW, C, (const uint32_t *)c->dspins0.p, c->dwT.p,
                                    c->dbT.p, c->dtheta0.p, c->stream),
            "theta GEMM kernel");
-    const SweepShape sw = nq_sweep_shape(cplx, c->N, c->M, c->K, c->lds_max);  // k_sample_mx
+    SweepShape sw = nq_sweep_shape(cplx, c->N, c->M, c->K, c->lds_max);  // k_sample_mx
     if (!sw.ok) return c->fail(NQES_EINVAL, "M=%d too large for the sampler", c->M);
     HIPCHK(c, c->dtabp.ensure(sw.tab_bytes), "alloc sampler tables");
-    const MeasureShape ms = nq_measure_shape(cplx, c->N, c->K, sw, c->lds_max);  // k_measure_mx
+    HIPCHK(c, c->dtabm.ensure(sw.tab_bytes), "alloc measurement tables");
+    MeasureShape ms = nq_measure_shape(cplx, c->N, c->K, sw, c->lds_max);  // k_measure_mx
     if (!ms.ok) return c->fail(NQES_EINVAL, "K*K*M=%d hidden-unit angles exceed the LDS per block", c->K * c->K * c->M);
+    tune_sampler(c, C, m, B, sw, ms);
     MixedSampleArgs a{};
     a.N = c->N;
     a.M = c->M;
@@ -500,8 +605,8 @@ static int sample_mixed(nqes_ctx *c, uint64_t seed, int C, int m, int B)
     a.U = c->svf ? c->dU.p : nullptr;
     a.Ucols = c->svf_shape.ucols;
     a.Umax = c->q8 ? (float *)c->dUmax.p : nullptr;
-    HIPCHK(c, nq_launch_sweep_mx(cplx, c->K, a, sw, c->dtabp.p, c->stream), "mixed sample kernel");
-    HIPCHK(c, nq_launch_measure_mx(cplx, c->K, a, ms, c->dtabp.p, c->stream), "mixed measure kernel");
+    HIPCHK(c, nq_launch_sweep_mx(cplx, c->K, a, sw, c->dtabp.p, c->dtabm.p, c->stream), "mixed sample kernel");
+    HIPCHK(c, nq_launch_measure_mx(cplx, c->K, a, ms, c->dtabm.p, c->stream), "mixed measure kernel");
     HIPCHK(c, hipEventRecord(c->ev[1], c->stream), "event");
     if (c->svf) {
         SvfArgs f = svf_args(c, S);
@@ -655,6 +760,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     if (c->prec == NQES_PREC_MIXED)
         HIPCHK(c, hipMemcpyAsync(&flags, c->dflags.p, 4, hipMemcpyDeviceToHost, c->stream), "copy flags");
     HIPCHK(c, hipStreamSynchronize(c->stream), "sample sync");
+    if (c->prec == NQES_PREC_MIXED) tune_record(c, event_ms(c->ev[0], c->ev[1]));
 
     // normalisation (rbm.cc:641-650)
     using namespace nq64;
@@ -883,6 +989,21 @@ int nqes_get_ulp_ties(nqes_ctx *c, uint64_t *per_chain, uint64_t *total)
         for (auto v : t) s += v;
         *total = s;
     }
+    return NQES_OK;
+}
+
+int nqes_get_sampler_launch(nqes_ctx *c, int *chains_per_block, int *tables_in_lds, int *samples_per_block,
+                            int *tuning)
+{
+    if (!c) return NQES_EINVAL;
+    if (!c->sampled || c->prec != NQES_PREC_MIXED)
+        return c->fail(NQES_ESTATE, "no mixed-precision samples: call nqes_sample first");
+    const SweepShape sw = nq_sweep_shape(c->cplx, c->N, c->M, c->K, c->lds_max);
+    const MeasureShape ms = nq_measure_shape(c->cplx, c->N, c->K, sw, c->lds_max);
+    if (chains_per_block) *chains_per_block = c->launch.sweep_wpb * (64 / sw.G);
+    if (tables_in_lds) *tables_in_lds = c->launch.sweep_tab;
+    if (samples_per_block) *samples_per_block = c->launch.measure_wpb * (64 / ms.G);
+    if (tuning) *tuning = c->tune_state;
     return NQES_OK;
 }
 

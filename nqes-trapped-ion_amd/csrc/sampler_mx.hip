@@ -19,6 +19,8 @@
 #include <stdint.h>
 #include <stdlib.h>
 
+#include <type_traits>
+
 #include "cplx64.hpp"
 #include "kernels.hpp"
 #include "mixed_math.hpp"
@@ -55,6 +57,10 @@ __device__ inline cd expD(cd z) { return nq64::cexp_(z); }
 //                            the row stride odd, so lanes reading one hidden unit of
 //                            different rows hit different LDS banks
 //   w   T [K N][Mp],  b  T [K][Mp]
+// The measurement kernel reads a second set with the same layout except that its cs
+// section is transposed, ChSh [K][Mp][N] (spin fastest): its lanes run over spins, so
+// the pairs that one load instruction reads are adjacent (one cache line per 8 or 16
+// lanes instead of one per lane when the tables stay in L2).
 struct PadLayout {
     size_t cs, w, b, total;
 };
@@ -74,7 +80,8 @@ __host__ __device__ inline size_t pad_tab_bytes(bool cplx, int N, int K, int Mp)
 #if !defined(NQES_SWEEP_K) || NQES_SWEEP_K == 1
 template <bool CPLX>
 __global__ void k_pad_tables_mx(int N, int M, int K, int Mp, const void *chT_, const void *shT_, const void *wT_,
-                                const void *bT_, const void *eaP_, const void *eaN_, const void *aD_, void *out)
+                                const void *bT_, const void *eaP_, const void *eaN_, const void *aD_, void *out,
+                                void *outm)
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
@@ -87,6 +94,10 @@ __global__ void k_pad_tables_mx(int N, int M, int K, int Mp, const void *chT_, c
     T *b = (T *)((unsigned char *)out + L.b);
     const long long stride = (long long)gridDim.x * blockDim.x;
     const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    ChSh<T> *csm = (ChSh<T> *)((unsigned char *)outm + L.cs);
+    T *wm = (T *)((unsigned char *)outm + L.w);
+    T *bm = (T *)((unsigned char *)outm + L.b);
+    D *dm = (D *)outm;
     for (long long q = t0; q < KN * CSR; q += stride) {
         const long long row = q / CSR;
         const int j = (int)(q % CSR);
@@ -94,20 +105,21 @@ __global__ void k_pad_tables_mx(int N, int M, int K, int Mp, const void *chT_, c
         v.ch = j < M ? chT[row * M + j] : oneT<T>();
         v.sh = j < M ? shT[row * M + j] : zeroT<T>();
         cs[q] = v;
+        if (j < Mp) csm[((row / N) * Mp + j) * N + row % N] = v;  // row = kk N + i
     }
     for (long long q = t0; q < KN * Mp; q += stride) {
         const long long row = q / Mp;
         const int j = (int)(q % Mp);
-        w[q] = j < M ? wT[row * M + j] : zeroT<T>();
+        w[q] = wm[q] = j < M ? wT[row * M + j] : zeroT<T>();
     }
     for (long long q = t0; q < (long long)K * Mp; q += stride) {
         const int kk = (int)(q / Mp), j = (int)(q % Mp);
-        b[q] = j < M ? bT[(long long)kk * M + j] : zeroT<T>();
+        b[q] = bm[q] = j < M ? bT[(long long)kk * M + j] : zeroT<T>();
     }
     for (long long q = t0; q < KN; q += stride) {
-        d[q] = ((const D *)eaP_)[q];
-        d[KN + q] = ((const D *)eaN_)[q];
-        d[2 * KN + q] = ((const D *)aD_)[q];
+        d[q] = dm[q] = ((const D *)eaP_)[q];
+        d[KN + q] = dm[KN + q] = ((const D *)eaN_)[q];
+        d[2 * KN + q] = dm[2 * KN + q] = ((const D *)aD_)[q];
     }
 }
 #endif
@@ -456,7 +468,6 @@ __device__ inline void hloc_pairflip(const MixedSampleArgs &a, const ChSh<typena
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
-    constexpr int CSR = Mp + 1;
     const int N = a.N, M = a.M, NW = a.NW;
     const double delta = a.delta;
     D res[K * K];
@@ -479,13 +490,13 @@ __device__ inline void hloc_pairflip(const MixedSampleArgs &a, const ChSh<typena
                 }
 #pragma unroll
                 for (int kk = 0; kk < K; kk++) {
-                    const ChSh<T> *ri = cs + ((long long)kk * N + i) * CSR, *rj = cs + ((long long)kk * N + j) * CSR;
+                    const ChSh<T> *ri = cs + (long long)kk * Mp * N + i, *rj = cs + (long long)kk * Mp * N + j;
                     const T *tk = tl + (k * K + kk) * Mp;
                     D prod = oneD<D>();
                     T pf = oneT<T>();
                     for (int hj = 0; hj < M; hj++) {
                         const T t = tk[hj];
-                        const ChSh<T> ci = ri[hj], cj = rj[hj];
+                        const ChSh<T> ci = ri[(long long)hj * N], cj = rj[(long long)hj * N];
                         const T s_i = si ? negT(ci.sh) : ci.sh;
                         const T den1 = subT(ci.ch, mulT(t, s_i));
                         const T t1 = divT(subT(mulT(t, ci.ch), s_i), den1);
@@ -525,7 +536,6 @@ __device__ inline void measure_sample(const MixedSampleArgs &a, const ChSh<typen
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
-    constexpr int CSR = Mp + 1;
     const int N = a.N, M = a.M, NW = a.NW, L = a.L;
     D phi_inv[K * K];
     inv_m<K, D>(phi, phi_inv);
@@ -558,57 +568,65 @@ __device__ inline void measure_sample(const MixedSampleArgs &a, const ChSh<typen
         D sx[K * K];
 #pragma unroll
         for (int q = 0; q < K * K; q++) sx[q] = zeroD<D>();
+        // the terms of spins i0 and (TWO) i1 = i0 + G of replica kk, added to sx in that order
+        auto spin_terms = [&](int kk, int i0, auto two_c) {
+            constexpr bool TWO = decltype(two_c)::value;
+            const int i1 = TWO ? i0 + G : i0;
+            bool dn0[K], dn1[K];
 #pragma unroll
-        for (int kk = 0; kk < K; kk++) {
-            for (int i0 = g; i0 < N; i0 += 2 * G) {
-                const bool two = i0 + G < N;
-                const int i1 = two ? i0 + G : i0;  // i1 == i0: second product discarded
-                bool dn0[K], dn1[K];
+            for (int k = 0; k < K; k++) {
+                dn0[k] = (sp[k * NW + (i0 >> 5)] >> (i0 & 31)) & 1u;
+                dn1[k] = (sp[k * NW + (i1 >> 5)] >> (i1 & 31)) & 1u;
+            }
+            const ChSh<T> *r0 = cs + (long long)kk * Mp * N + i0, *r1 = cs + (long long)kk * Mp * N + i1;
+            D prod0[K], prod1[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                prod0[k] = oneD<D>();
+                prod1[k] = oneD<D>();
+            }
+            for (int j0 = 0; j0 < Mp; j0 += 8) {
+                T pf0[K], pf1[K];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    dn0[k] = (sp[k * NW + (i0 >> 5)] >> (i0 & 31)) & 1u;
-                    dn1[k] = (sp[k * NW + (i1 >> 5)] >> (i1 & 31)) & 1u;
+                    pf0[k] = oneT<T>();
+                    pf1[k] = oneT<T>();
                 }
-                const ChSh<T> *r0 = cs + ((long long)kk * N + i0) * CSR, *r1 = cs + ((long long)kk * N + i1) * CSR;
-                D prod0[K], prod1[K];
 #pragma unroll
-                for (int k = 0; k < K; k++) {
-                    prod0[k] = oneD<D>();
-                    prod1[k] = oneD<D>();
-                }
-                for (int j0 = 0; j0 < Mp; j0 += 8) {
-                    T pf0[K], pf1[K];
+                for (int jj = 0; jj < 8; jj++) {
+                    const ChSh<T> c0 = r0[(long long)(j0 + jj) * N];
+                    ChSh<T> c1;
+                    if constexpr (TWO) c1 = r1[(long long)(j0 + jj) * N];
 #pragma unroll
                     for (int k = 0; k < K; k++) {
-                        pf0[k] = oneT<T>();
-                        pf1[k] = oneT<T>();
-                    }
-#pragma unroll
-                    for (int jj = 0; jj < 8; jj++) {
-                        const ChSh<T> c0 = r0[j0 + jj], c1 = r1[j0 + jj];
-#pragma unroll
-                        for (int k = 0; k < K; k++) {
-                            const T t = tl[(k * K + kk) * Mp + j0 + jj];
-                            pf0[k] = mulT(pf0[k], subT(c0.ch, mulT(t, dn0[k] ? negT(c0.sh) : c0.sh)));
-                            pf1[k] = mulT(pf1[k], subT(c1.ch, mulT(t, dn1[k] ? negT(c1.sh) : c1.sh)));
-                        }
-                    }
-#pragma unroll
-                    for (int k = 0; k < K; k++) {
-                        prod0[k] = mulD(prod0[k], toD(pf0[k]));
-                        prod1[k] = mulD(prod1[k], toD(pf1[k]));
+                        const T t = tl[(k * K + kk) * Mp + j0 + jj];
+                        pf0[k] = mulT(pf0[k], subT(c0.ch, mulT(t, dn0[k] ? negT(c0.sh) : c0.sh)));
+                        if constexpr (TWO) pf1[k] = mulT(pf1[k], subT(c1.ch, mulT(t, dn1[k] ? negT(c1.sh) : c1.sh)));
                     }
                 }
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const D e0 = dn0[k] ? eaP[kk * N + i0] : eaN[kk * N + i0];
-                    sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e0), prod0[k]));
-                    if (two) {
-                        const D e1 = dn1[k] ? eaP[kk * N + i1] : eaN[kk * N + i1];
-                        sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e1), prod1[k]));
-                    }
+                    prod0[k] = mulD(prod0[k], toD(pf0[k]));
+                    if constexpr (TWO) prod1[k] = mulD(prod1[k], toD(pf1[k]));
                 }
             }
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                const D e0 = dn0[k] ? eaP[kk * N + i0] : eaN[kk * N + i0];
+                sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e0), prod0[k]));
+                if constexpr (TWO) {
+                    const D e1 = dn1[k] ? eaP[kk * N + i1] : eaN[kk * N + i1];
+                    sx[k * K + kk] = addD(sx[k * K + kk], mulD(mulD(phi[k * K + kk], e1), prod1[k]));
+                }
+            }
+        };
+        // lane g takes spins g, g + G, g + 2G, ...: two at a time while two remain (every
+        // angle read feeds both), then a last one alone (N <= G: one spin per lane)
+#pragma unroll
+        for (int kk = 0; kk < K; kk++) {
+            int i0 = g;
+            for (; i0 + G < N; i0 += 2 * G) spin_terms(kk, i0, std::true_type{});
+            if (i0 < N) spin_terms(kk, i0, std::false_type{});
         }
         D res[K * K];
 #pragma unroll
@@ -730,13 +748,24 @@ __device__ inline void measure_sample(const MixedSampleArgs &a, const ChSh<typen
     }
 }
 
+// Wavefronts per SIMD the measurement kernel is compiled for: 2 (at most 256 VGPRs) for
+// the real-weight shapes whose unconstrained build lands just above 256 registers (K = 4
+// with up to 4 hidden units per lane, K = 3 with 64 lanes and 8 units per lane), which
+// otherwise run one wavefront per SIMD; capped they spill at most 2 registers.  1 (no
+// bound) elsewhere.
+__host__ __device__ constexpr int measure_min_waves(bool cplx, int K, int G, int JQ)
+{
+    return !cplx && ((K == 4 && JQ <= 4) || (K == 3 && G == 64 && JQ == 8)) ? 2 : 1;
+}
+
 // Local energies and O rows (or factor rows) of all S samples from their stored
 // spins: G lanes per sample, 64/G samples per wavefront, each block looping over
 // sample batches (grid sized to the CUs) so that the tables are staged in LDS once
 // per block.  Angles and Phi from the spins as in the sweeps (sweep_refresh, lanes
 // over hidden units), then measure_sample.
 template <bool CPLX, int K, int G, int JQ, bool TAB>
-__global__ __launch_bounds__(256) void k_measure_mx(MixedSampleArgs a, const void *tabp)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(measure_min_waves(CPLX, K, G, JQ))))
+void k_measure_mx(MixedSampleArgs a, const void *tabp)
 {
     using T = typename Ty<CPLX>::T;
     using D = typename Ty<CPLX>::D;
@@ -1017,7 +1046,7 @@ hipError_t nq_launch_measure_mx(bool cplx, int K, const MixedSampleArgs &a, cons
 }
 
 hipError_t nq_launch_sweep_mx(bool cplx, int K, const MixedSampleArgs &a, const SweepShape &sh, void *tabp,
-                              hipStream_t st)
+                              void *tabm, hipStream_t st)
 {
     if (!sh.ok) return hipErrorInvalidValue;
     const int Mp = sh.G * sh.JQ;
@@ -1026,10 +1055,10 @@ hipError_t nq_launch_sweep_mx(bool cplx, int K, const MixedSampleArgs &a, const 
     if (blocks > 4096) blocks = 4096;
     if (cplx)
         hipLaunchKernelGGL(k_pad_tables_mx<true>, dim3(blocks), dim3(256), 0, st, a.N, a.M, K, Mp, a.chT, a.shT, a.wT,
-                           a.bT, a.eaP, a.eaN, a.aD, tabp);
+                           a.bT, a.eaP, a.eaN, a.aD, tabp, tabm);
     else
         hipLaunchKernelGGL(k_pad_tables_mx<false>, dim3(blocks), dim3(256), 0, st, a.N, a.M, K, Mp, a.chT, a.shT,
-                           a.wT, a.bT, a.eaP, a.eaN, a.aD, tabp);
+                           a.wT, a.bT, a.eaP, a.eaN, a.aD, tabp, tabm);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     switch (K) {

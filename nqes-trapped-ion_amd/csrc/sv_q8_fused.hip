@@ -9,13 +9,15 @@
 //   1. runs the forward GEMM of one 16-row tile, P_ri = sum_c X_rc Q_ci, on
 //      v_mfma_i32_16x16x64_i8 with its A operands straight from HBM (layout Ug below),
 //      and forms the rows' weights omega_r = (sum_kk w_row - o^T v) / f_r;
-//   2. transposes its digits in registers into the backward B layout (a 4x4 byte
-//      transpose across each lane quad: two DPP exchanges and two v_perm_b32 per dword)
-//      and stores them, the rows' weight digits and spin masks in LDS;
+//   2. stores its digits in LDS as row-major 64 x 16 byte matrices, one per column tile,
+//      digit plane and chunk (one dword store per column tile and plane), with the rows'
+//      weight digits and spin masks;
 // and after one block barrier the four wavefronts run the chunk's backward GEMM
 //   out[i][col] = sum_r sigma_r[i] X_r[col] Omega_r = T[col] - 2 sum_r b_r[i] X_r[col] Omega_r
-// from LDS, each for its column tiles, into int32 accumulators.  The digits are read from
-// HBM once per call.
+// from LDS, each for its column tiles, into int32 accumulators.  The backward B operands
+// (16 rows of one column per lane) come out of the row-major matrices through the LDS
+// transpose load ds_read_b64_tr_b8, so no VALU work goes into the transpose.  The digits
+// are read from HBM once per call.
 //
 // Columns.  The MFMAs cover the K M tanh columns of the factor rows (64-column groups).
 // The K columns c_{s,k,kk} (the a block) are added in FP64 on the VALU from their
@@ -34,16 +36,16 @@
 //
 // Layout Ug (written once per sample set by k_q8_packg): per 16-row tile rt, digit plane
 // t and 64-column group cg, 64 lanes x 16 bytes; lane (g, lo) holds row 16 rt + lo, byte
-// 4 d + b = digit t of column 64 cg + 16 b + 4 g + d.  The forward A operand of the
+// 4 b + d = digit t of column 64 cg + 16 b + 4 g + d.  The forward A operand of the
 // int8 MFMA is lane l's 16 bytes as they are; the v digits (k_q8_prep_vf) use the same
-// column order.  Dword d of the lanes of a quad (rows 4 g' .. 4 g' + 3) then holds, for
-// the four columns 16 b + (4 g + d) of column tiles 4 cg + b, the byte block that the
-// quad transpose turns into column dwords (4 rows of one column each).
+// column order (any order of the 64 columns of a group serves the forward sum, as long
+// as both operands share it).  Dword b of a lane then holds four adjacent columns
+// (4 g .. 4 g + 3) of column tile 4 cg + b: one dword of a row of that tile's matrix.
 //
 // LDS per block: the forward B operands (NCG (6 NST + 1) KB), and twice (super-chunk
-// parity) the transposed digits XT [column tile][t][row tile][lane'] dwords (lane' = 16 g'
-// + lo' swizzled), the weight digits [chunk][g'][u][row tile] dwords, the spin masks
-// [st][g'][chunk][i][row tile] dwords and the wavefronts' largest |omega|.
+// parity) the digit matrices XT [column tile][t][chunk][64 rows x 16 bytes], the weight
+// digits [chunk][g'][u][row tile] dwords, the spin masks [st][g'][chunk][i][row tile]
+// dwords and the wavefronts' largest |omega|.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -93,9 +95,13 @@ __device__ inline double fz_sum16x4(const double (&h)[4], int l)
     return b;
 }
 
-// swizzled slot of lane' = 16 g' + lo' in the X^T rows of column tile ct (the quad
-// transpose writes of one instruction then fall in distinct banks, two lanes per bank)
-__device__ inline int xt_pos(int lanep, int ct) { return lanep ^ ((lanep >> 4) | ((ct & 1) << 4)); }
+typedef int v2i __attribute__((ext_vector_type(2)));
+
+// ds_read_b64_tr_b8 at an LDS address (8-byte aligned)
+__device__ inline v2i lds_tr8(const uint32_t *p)
+{
+    return __builtin_amdgcn_ds_read_tr8_b64_v2i32((__attribute__((address_space(3))) v2i *)(p));
+}
 
 __device__ inline v4i as_v4i(uint4 x)
 {
@@ -163,15 +169,15 @@ __global__ __launch_bounds__(256) void k_q8_packg(const float *U, long long R, l
 #pragma unroll
                 for (int d = 0; d < 4; d++) pl[t][d] = 0u;
 #pragma unroll
-            for (int d = 0; d < 4; d++)
+            for (int b = 0; b < 4; b++)
 #pragma unroll
-                for (int b = 0; b < 4; b++) {
+                for (int d = 0; d < 4; d++) {
                     const int col = 64 * cg + 16 * b + 4 * g + d;
                     long long x = col < KM ? xval(col) : 0;
 #pragma unroll
                     for (int t = 0; t < QU; t++) {
                         const long long dg = (t < QU - 1) ? bal8l(x) : x;
-                        pl[t][d] |= ((uint32_t)dg & 255u) << (8 * b);
+                        pl[t][b] |= ((uint32_t)dg & 255u) << (8 * d);
                         x = (x - dg) >> 8;
                     }
                 }
@@ -216,12 +222,13 @@ __global__ __launch_bounds__(256) void k_q8_prep_vf(FzArgs a, int ncg, int nst, 
     int gex = 0;
     if (mx > 0.0) (void)frexp(mx, &gex);
     const double sc = ldexp(1.0, VBITS - gex);
-    const int lane = threadIdx.x & 63, lo = lane & 15, g = lane >> 4, d = threadIdx.x >> 6;
+    // this thread's dword b of lane (g, lo): columns 64 cg + 16 b + 4 g + d, byte d
+    const int lane = threadIdx.x & 63, lo = lane & 15, g = lane >> 4, b = threadIdx.x >> 6;
     const int cg = blockIdx.x / (nst + 1), st = blockIdx.x % (nst + 1);
     const int nslot = nst * QV + 1;
     uint32_t dg[QV] = {};
 #pragma unroll
-    for (int b = 0; b < 4; b++) {
+    for (int d = 0; d < 4; d++) {
         const int col = 64 * cg + 16 * b + 4 * g + d;
         double val = 0.0;
         if (col < KM) {
@@ -237,20 +244,20 @@ __global__ __launch_bounds__(256) void k_q8_prep_vf(FzArgs a, int ncg, int nst, 
 #pragma unroll
         for (int u = 0; u < QV; u++) {
             const long long dd = (u < QV - 1) ? bal8l(x) : x;
-            dg[u] |= ((uint32_t)dd & 255u) << (8 * b);
+            dg[u] |= ((uint32_t)dd & 255u) << (8 * d);
             x = (x - dd) >> 8;
         }
     }
     uint32_t *o = (uint32_t *)(qg + (size_t)cg * nslot * 64);
     if (st < nst) {
 #pragma unroll
-        for (int u = 0; u < QV; u++) o[((st * QV + u) * 64 + lane) * 4 + d] = dg[u];
+        for (int u = 0; u < QV; u++) o[((st * QV + u) * 64 + lane) * 4 + b] = dg[u];
     } else {
         uint32_t bw = 0u;
 #pragma unroll
         for (int u = 0; u < QV; u++)
             if (lo == u) bw = dg[u];
-        o[(nst * QV * 64 + lane) * 4 + d] = bw;
+        o[(nst * QV * 64 + lane) * 4 + b] = bw;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         *(double *)(qg + (size_t)ncg * nslot * 64) = mx > 0.0 ? ldexp(1.0, gex - VBITS) : 0.0;
@@ -259,7 +266,7 @@ __global__ __launch_bounds__(256) void k_q8_prep_vf(FzArgs a, int ncg, int nst, 
 template <int NCG, int NST, int W>
 struct FzLds {
     static constexpr int NCT = 4 * NCG;
-    static constexpr int XT_DW = NCT * QU * W * 64;
+    static constexpr int XT_DW = NCT * QU * (W / 4) * 256;  // [column tile][t][chunk] 64 x 16 bytes
     static constexpr int OD_DW = 8 * 4 * W;  // [chunk][g'][u, 8 slots][row tile of the chunk]
     static constexpr int MK_DW = NST * 16 * 4 * W;  // [st][g'][chunk][i][row tile of the chunk]
     static constexpr int BUF_DW = XT_DW + OD_DW + MK_DW + 2 * W;
@@ -318,10 +325,12 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
     }
     // the lane's bias digit weight (forward bias slot: lane lo holds digit lo)
     const double pw = lo < QV ? ldexp(1.0, 8 * lo) : 0.0;
-    // the quad transpose's byte selectors (v_perm_b32: bytes 0-3 own dword, 4-7 partner)
     const int j4 = lo & 3;
-    const uint32_t sel1 = (j4 & 1) ? 0x03070105u : 0x06020400u;
-    const uint32_t sel2 = (j4 & 2) ? 0x03020706u : 0x05040100u;
+    // the transpose loads (ds_read_b64_tr_b8: within each 16 lanes, lane x gets byte x % 8
+    // of the 8 bytes that lane 2 e + x / 8 read, as its byte e): lane (g, lo) reads the 8
+    // bytes of row 16 (lo / 8) + 4 g + (lo / 2) % 4, columns 8 (lo % 2) .. + 7, so that
+    // lane (g, x) receives column x of rows 4 g + j and 16 + 4 g + j (j < 4) in order
+    const int trofs = (16 * (lo >> 3) + 4 * g + ((lo >> 1) & 3)) * 4 + 2 * (lo & 1);
 
     v4i acc[CTW][NST][NSF];  // sum_r b_r[i] X Omega by digit weight 256^(s + QSKIP)
     v4i tt[CTW][QU];         // sum_r Omega_u X_t: row u of the result
@@ -519,24 +528,21 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
                 }
         }
 
-        // the quad transpose of this tile's digits into X^T: column tile 4 cg + j4,
-        // lane' = 16 (lo / 4) + 4 g + d, row tile wv
+        // this tile's digits into the chunk's column-tile matrices X_t [64 rows][16 columns]
+        // (row-major; the backward products read them back with the LDS transpose load):
+        // dword b of the lane's 16 bytes holds columns 4 g .. 4 g + 3 of column tile 4 cg + b
+        // in row 16 (wv % 4) + lo of chunk wv / 4
+        {
+            uint32_t *xw = xt + (wv >> 2) * 256 + (16 * (wv & 3) + lo) * 4 + g;
 #pragma unroll
-        for (int cg = 0; cg < NCG; cg++)
+            for (int cg = 0; cg < NCG; cg++)
 #pragma unroll
-            for (int t = 0; t < QU; t++) {
-                const uint32_t dw[4] = {av[cg][t].x, av[cg][t].y, av[cg][t].z, av[cg][t].w};
+                for (int t = 0; t < QU; t++) {
+                    const uint32_t dw[4] = {av[cg][t].x, av[cg][t].y, av[cg][t].z, av[cg][t].w};
 #pragma unroll
-                for (int d = 0; d < 4; d++) {
-                    const uint32_t x = dw[d];
-                    const uint32_t p1 = (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, true);
-                    const uint32_t s1 = __builtin_amdgcn_perm(p1, x, sel1);
-                    const uint32_t p2 = (uint32_t)__builtin_amdgcn_mov_dpp((int)s1, 0x4E, 0xF, 0xF, true);
-                    const uint32_t y = __builtin_amdgcn_perm(p2, s1, sel2);
-                    const int ct = 4 * cg + j4, lanep = 16 * (lo >> 2) + 4 * g + d;
-                    xt[((ct * QU + t) * W + wv) * 64 + xt_pos(lanep, ct)] = y;
+                    for (int b = 0; b < 4; b++) xw[((4 * cg + b) * QU + t) * NKC * 256] = dw[b];
                 }
-            }
+        }
         // spin masks: spin 16 st + lo, rows 4 g + j
 #pragma unroll
         for (int st = 0; st < NST; st++) {
@@ -589,15 +595,17 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
         const uint4 *mk4 = (const uint4 *)mk;
 #pragma unroll
         for (int kc = 0; kc < NKC; kc++) {
+            // B operand of lane (g, lo): bytes 4 q + j = row 16 q + 4 g + j of the chunk,
+            // column lo: two transpose loads (q = 0, 1 and q = 2, 3) of the rows' 8-byte halves
             uint4 bv[CTW][QU];
 #pragma unroll
             for (int c = 0; c < CTW; c++) {
                 const int ct = wv * CTW + c;
-                const int pos = xt_pos(lane, ct);
 #pragma unroll
                 for (int t = 0; t < QU; t++) {
-                    const uint32_t *base = xt + ((ct * QU + t) * W + 4 * kc) * 64 + pos;
-                    bv[c][t] = uint4{base[0], base[64], base[128], base[192]};
+                    const uint32_t *base = xt + ((ct * QU + t) * NKC + kc) * 256 + trofs;
+                    const v2i x0 = lds_tr8(base), x1 = lds_tr8(base + 32 * 4);
+                    bv[c][t] = uint4{(uint32_t)x0.x, (uint32_t)x0.y, (uint32_t)x1.x, (uint32_t)x1.y};
                 }
             }
             {

@@ -98,7 +98,8 @@ EXPORTS = [
     "nqes_create", "nqes_destroy", "nqes_last_error", "nqes_problem_init", "nqes_sr_step",
     "nqes_sample", "nqes_get_samples", "nqes_get_O", "nqes_get_moments", "nqes_apply_S",
     "nqes_solve", "nqes_correlations", "nqes_required_bytes", "nqes_version",
-    "nqes_set_dense_S", "nqes_enumerate", "nqes_get_ulp_ties", "nqes_apply_S_rows_noalloc",
+    "nqes_set_dense_S", "nqes_enumerate", "nqes_get_ulp_ties", "nqes_get_sampler_launch",
+    "nqes_apply_S_rows_noalloc",
     "nqes_server_create", "nqes_server_destroy", "nqes_server_last_error", "nqes_server_submit",
     "nqes_server_poll", "nqes_server_progress", "nqes_server_wait", "nqes_server_cancel",
 ]
@@ -133,6 +134,8 @@ def load_library(path=LIB_PATH):
     L.nqes_apply_S_rows_noalloc.argtypes = [_vp, c_i64, c_i64, c_i64, c_int, _vp, _vp, _vp, _vp, _vp]
     L.nqes_get_ulp_ties.restype = c_int
     L.nqes_get_ulp_ties.argtypes = [_vp, _vp, ctypes.POINTER(c_u64)]
+    L.nqes_get_sampler_launch.restype = c_int
+    L.nqes_get_sampler_launch.argtypes = [_vp] + [ctypes.POINTER(c_int)] * 4
     L.nqes_get_moments.restype = c_int
     L.nqes_get_moments.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
     L.nqes_apply_S.restype = c_int
@@ -275,6 +278,14 @@ class Solver:
         tot = ctypes.c_uint64(0)
         self._check(self.lib.nqes_get_ulp_ties(self.ctx, _ptr(per), ctypes.byref(tot)))
         return per, tot.value
+
+    def sampler_launch(self):
+        """Launch shape of the last mixed-precision sample set: {chains_per_block,
+        tables_in_lds, samples_per_block, tuning} (tuning 1: a tuner trial, 0: the tuned
+        choice, -1: tuner off)."""
+        v = [ctypes.c_int(0) for _ in range(4)]
+        self._check(self.lib.nqes_get_sampler_launch(self.ctx, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("chains_per_block", "tables_in_lds", "samples_per_block", "tuning"), (x.value for x in v)))
 
     def O_rows(self, row0, nrows):
         out = np.zeros((nrows, self.P), dtype=np.complex128)
