@@ -133,9 +133,9 @@ int nqes_get_ulp_ties(nqes_ctx *ctx, uint64_t *per_chain, uint64_t *total);
  * OpenMP thread, rbm.cc:564-600, and has no counterpart): Metropolis chains per block of
  * the sweep kernel, whether its tables were staged in LDS (1) or read from L2 (0), and
  * samples per block of the measurement kernel.  These shapes are chosen by the library's
- * tuner: the first sampling calls of each problem shape (device, real or complex weights,
- * N, M, K, C, m, B, model) in the process try candidate shapes in turn, timed on the
- * device, and later calls use the fastest.  No lane computes anything else under another
+ * tuner: after one untimed call with the default shapes, the next sampling calls of each
+ * problem shape (device, real or complex weights, N, M, K, C, m, B, model) in the process
+ * try candidate shapes in turn, timed on the device, and later calls use the fastest.  No lane computes anything else under another
  * shape, so results do not depend on the choice.  tuning: 1 while the shape is being tuned,
  * 0 once its choice is fixed, -1 with the tuner off (NQES_TUNE=0, or NQES_WPB / NQES_TABLDS
  * set).  Any pointer may be NULL. */

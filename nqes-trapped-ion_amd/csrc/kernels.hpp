@@ -208,7 +208,9 @@ hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
 hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st);
 // its forward pass as one GEMM over the columns (sh.gemm; sv_q8_gemm.hip): w_s directly for
 // K = 1, 2, 4 unless rows, else the row sums wrow[r] (a.ncg = 1)
-hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows = false);
+hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows = false, bool prep = true);
+hipError_t nq_q8_prep_vg_sets(const SvfArgs &a, const Q8Shape &sh, int nsets, long long vss, size_t qss_bytes,
+                              hipStream_t st);
 // Complex weights on the int8 form (three real passes per product, Gauss's form): the
 // complex factor rows a.U (complex float) are quantised with one scale per row into the
 // digits of X_R = Re, X_I = Im and X_A = X_R + X_I (exact integers), three slices of
@@ -219,6 +221,7 @@ hipError_t nq_q8c_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
 // doubles each (A, R, I); Re raw = A - R - I, Im raw = R - I (p < P), raw[P] = (I[P], R[P])
 hipError_t nq_q8c_apply(const SvfArgs &a, const Q8Shape &sh, const double *vin, double *w3, void *scratch, double *out,
                         hipStream_t st);
+void nq_q8c_sizes(const Q8Shape &sh, long long P, size_t *vq, size_t *q8w, size_t *part, size_t *scratch);
 // moments from the quantised factors (sums with w = 1 and w = E on the int8 backward
 // pass, |O|^2 on FP64 MFMA); after nq_q8_pack
 hipError_t nq_q8_moments(const SvfArgs &a, const Q8Shape &sh, const SvfShape &svf, void *part, void *scratch,
@@ -233,6 +236,10 @@ hipError_t nq_launch_svf_moments(bool cplx, const SvfArgs &a, const SvfShape &sh
 // complex double); scratch >= (nchunk/32 + 1) * ncol D
 hipError_t nq_chunk_reduce(bool cplx, const void *part, long long ncol, long long stride, int nchunk,
                            void *scratch, void *out, hipStream_t st);
+// nsets sets of FP64 partial rows (set z at part + z pss, scratch: nsets ceil(nchunk / 32)
+// ncol values) reduced into out + z oss, each in the order nq_chunk_reduce uses
+hipError_t nq_chunk_reduce_sets(const double *part, long long ncol, long long stride, int nchunk, double *scratch,
+                                double *out, int nsets, long long pss, long long oss, hipStream_t st);
 
 hipError_t nq_launch_tables_mx(bool cplx, int N, int M, int K, int L, const void *W, void *chT, void *shT, void *wT,
                                void *bT, void *eaP, void *eaN, void *aD, hipStream_t st);

@@ -503,44 +503,49 @@ __device__ inline D chunk_run(D acc, const D *__restrict__ part, long long strid
     return acc;
 }
 
-// out[col] = sum over chunks of part[chunk * stride + col], col < ncol, chunks in order
+// out[col] = sum over chunks of part[chunk * stride + col], col < ncol, chunks in order;
+// blockIdx.z: the set (part + z pss, out + z oss)
 template <typename D>
 __global__ void k_chunksum_mx(const D *__restrict__ part, long long ncol, long long stride, int nchunk,
-                              D *__restrict__ out)
+                              D *__restrict__ out, long long pss, long long oss)
 {
     long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncol) return;
-    out[col] = chunk_run(part[col], part + col, stride, 1, nchunk);
+    part += blockIdx.z * pss;
+    out[blockIdx.z * oss + col] = chunk_run(part[col], part + col, stride, 1, nchunk);
 }
 
-// partial sums over groups of CG consecutive chunks: grid (col tiles, groups)
+// partial sums over groups of CG consecutive chunks: grid (col tiles, groups, sets)
 template <typename D>
 __global__ void k_chunkgroup_mx(const D *__restrict__ part, long long ncol, long long stride, int nchunk, int cg,
-                                D *__restrict__ gout)
+                                D *__restrict__ gout, long long pss)
 {
     long long col = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncol) return;
     int c0 = blockIdx.y * cg, c1 = c0 + cg;
     if (c1 > nchunk) c1 = nchunk;
-    gout[(long long)blockIdx.y * ncol + col] = chunk_run(zeroD<D>(), part + col, stride, c0, c1);
+    gout[((long long)blockIdx.z * gridDim.y + blockIdx.y) * ncol + col] =
+        chunk_run(zeroD<D>(), part + blockIdx.z * pss + col, stride, c0, c1);
 }
 
 // out[col] = sum_chunks part[...], deterministic, parallel over column tiles and
-// chunk groups; scratch holds ngroups * ncol values
+// chunk groups; scratch holds ngroups * ncol values per set.  nsets independent sets of
+// partial rows (part + z pss) reduce into out + z oss in one pair of launches.
 template <typename D>
-void chunk_reduce(const D *part, long long ncol, long long stride, int nchunk, D *scratch, D *out, hipStream_t st)
+void chunk_reduce(const D *part, long long ncol, long long stride, int nchunk, D *scratch, D *out, hipStream_t st,
+                  int nsets = 1, long long pss = 0, long long oss = 0)
 {
     const int cg = 32;
     int ngroups = (nchunk + cg - 1) / cg;
     if (ngroups <= 1) {
-        hipLaunchKernelGGL(k_chunksum_mx<D>, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, st, part, ncol, stride,
-                           nchunk, out);
+        hipLaunchKernelGGL(k_chunksum_mx<D>, dim3((unsigned)((ncol + 63) / 64), 1, (unsigned)nsets), dim3(64), 0, st,
+                           part, ncol, stride, nchunk, out, pss, oss);
         return;
     }
-    hipLaunchKernelGGL(k_chunkgroup_mx<D>, dim3((unsigned)((ncol + 63) / 64), (unsigned)ngroups), dim3(64), 0, st,
-                       part, ncol, stride, nchunk, cg, scratch);
-    hipLaunchKernelGGL(k_chunksum_mx<D>, dim3((unsigned)((ncol + 63) / 64)), dim3(64), 0, st, (const D *)scratch,
-                       ncol, ncol, ngroups, out);
+    hipLaunchKernelGGL(k_chunkgroup_mx<D>, dim3((unsigned)((ncol + 63) / 64), (unsigned)ngroups, (unsigned)nsets),
+                       dim3(64), 0, st, part, ncol, stride, nchunk, cg, scratch, pss);
+    hipLaunchKernelGGL(k_chunksum_mx<D>, dim3((unsigned)((ncol + 63) / 64), 1, (unsigned)nsets), dim3(64), 0, st,
+                       (const D *)scratch, ncol, ncol, ngroups, out, (long long)ngroups * ncol, oss);
 }
 
 } // namespace
@@ -550,6 +555,13 @@ hipError_t nq_chunk_reduce(bool cplx, const void *part, long long ncol, long lon
 {
     if (cplx) chunk_reduce<cd>((const cd *)part, ncol, stride, nchunk, (cd *)scratch, (cd *)out, st);
     else chunk_reduce<double>((const double *)part, ncol, stride, nchunk, (double *)scratch, (double *)out, st);
+    return hipGetLastError();
+}
+
+hipError_t nq_chunk_reduce_sets(const double *part, long long ncol, long long stride, int nchunk, double *scratch,
+                                double *out, int nsets, long long pss, long long oss, hipStream_t st)
+{
+    chunk_reduce<double>(part, ncol, stride, nchunk, scratch, out, st, nsets, pss, oss);
     return hipGetLastError();
 }
 

@@ -101,14 +101,16 @@ int choose_chunks(long long S, long long ncol, int cols_per_block)
 // computes anything else, so spins, local energies and factor rows are the same bit for
 // bit for every choice.  The first sampling calls of each problem shape in the process
 // (device, real or complex weights, N, M, K, C, m, B, model) try the candidates in turn,
-// timed by the sampling events; later calls take the fastest (ties: the earlier
-// candidate).  NQES_TUNE=0, NQES_WPB or NQES_TABLDS turn it off.
+// timed by the sampling events, after one untimed call with the default shapes (a first
+// call pays one-time costs: code loading, cold caches); later calls take the fastest
+// (ties: the earlier candidate).  NQES_TUNE=0, NQES_WPB or NQES_TABLDS turn it off.
 struct SamplerChoice {
     int sweep_wpb, sweep_tab, measure_wpb;
 };
 struct TuneEntry {
     std::vector<SamplerChoice> cand;
     std::vector<double> ms;  // < 0: not timed yet
+    bool warm = false;       // the shape's first call (default shapes) has run, untimed
     size_t next = 0;         // next candidate to hand out
     int best = -1;           // fixed once every candidate is timed
 };
@@ -330,9 +332,15 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
         b += (nchunk / 32 + 2) * (P + 1) * 16 + 3 * (P + 1) * 16;
         // int8 form; complex weights keep three digit sets per row
         const Q8Shape q8 = nq_q8_shape(N, M, K, S * K);
-        if (q8.ok)
+        if (q8.ok) {
             b += 3 * (q8.rpad * (long long)q8.ucols * 3 + (long long)q8.uf_bytes) + q8.rpad * 12 +
                  (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes) + 3 * S * 8 + (3 * P + 4) * 8 + 3 * (P + 1) * 8;
+            if (cplx) {  // the complex S*v's per-set buffers hold three sets
+                size_t vq, q8w, part, scr;
+                nq_q8c_sizes(nq_q8_shape(N, M, K, S * K, true), P, &vq, &q8w, &part, &scr);
+                b += (long long)(vq + q8w + part + scr);
+            }
+        }
     } else {
         b += S * Pld * (long long)o_elem(precision, cplx);  // O
         if (mixed) b += (mch / 32 + 2) * P * 16;              // moments scratch
@@ -509,7 +517,11 @@ static void tune_sampler(nqes_ctx *c, int C, int m, int B, SweepShape &sw, Measu
             e.ms.assign(e.cand.size(), -1.0);
         }
         int idx = e.best;
-        if (idx < 0 && e.next < e.cand.size()) {  // a trial
+        if (idx < 0 && !e.warm) {  // the untimed first call
+            idx = 0;
+            e.warm = true;
+            c->tune_state = 1;
+        } else if (idx < 0 && e.next < e.cand.size()) {  // a trial
             idx = (int)e.next++;
             c->tune_idx = idx;
             c->tune_key = key;
@@ -702,8 +714,12 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
                    "alloc quantised factors");
             HIPCHK(c, c->descale.ensure((size_t)rp * 8), "alloc factor scales");
             HIPCHK(c, c->dUmax.ensure((size_t)rp * 4), "alloc factor row maxima");
-            HIPCHK(c, c->dvq.ensure(c->q8_shape.vq_bytes), "alloc S*v operands");
-            HIPCHK(c, c->dq8w.ensure(c->q8_shape.w_bytes), "alloc S*v weight digits");
+            size_t c_vq = c->q8_shape.vq_bytes, c_q8w = c->q8_shape.w_bytes, c_part = 0, c_scr = 0;
+            if (c->q8c) nq_q8c_sizes(c->q8_shape, P, &c_vq, &c_q8w, &c_part, &c_scr);  // three sets at once
+            HIPCHK(c, c->dvq.ensure(c_vq), "alloc S*v operands");
+            HIPCHK(c, c->dq8w.ensure(c_q8w), "alloc S*v weight digits");
+            HIPCHK(c, c->dsvfpart.ensure(c_part), "alloc S*v partials");
+            HIPCHK(c, c->dsvfscratch.ensure(c_scr), "alloc S*v scratch");
             HIPCHK(c, c->dq8m.ensure(c->q8_shape.m_bytes), "alloc spin masks");
             if (c->q8_shape.gemm)
                 HIPCHK(c, c->dUf.ensure(nsets * c->q8_shape.uf_bytes), "alloc row-major factor digits");

@@ -351,8 +351,15 @@ __global__ void k_q8_masks(const uint32_t *spins, long long R, int N, int NW, in
 // pd[(chunk QW + u) 4 + g] = uint4 over q of Omega = rint(omega 2^(46 - G_b)) (digit u,
 // least significant first); and part[b][P] = sum of wt(s) over its rows with r % K == 0.
 template <int WSEL>
-__global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *pd)
+__global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *pd, long long wss, long long q8ss,
+                                                  long long pss)
 {
+    // blockIdx.z: the digit set (weights a.w + z wss, block scales and digits + z q8ss
+    // bytes, partial rows a.part + z pss doubles) when several sets run in one launch
+    a.w = (double *)a.w + blockIdx.z * wss;
+    a.part = (double *)a.part + blockIdx.z * pss;
+    gblk = (int *)((uint8_t *)gblk + blockIdx.z * q8ss);
+    pd = (uint4 *)((uint8_t *)pd + blockIdx.z * q8ss);
     __shared__ uint32_t stg[4][QW][4][4];  // per wavefront: a chunk's digits in the A layout
     __shared__ double red[4], rsum[4];
     const int K = a.K;
@@ -578,11 +585,21 @@ __global__ void k_q8c_combine3(double *w3, long long S, const double *ov)
 int nit_of(int N) { return (N + 15) / 16; }
 
 template <int WSEL>
-hipError_t launch_bwd(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+hipError_t launch_omega(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, int nsets = 1, long long wss = 0,
+                        long long q8ss = 0, long long pss = 0)
 {
     int *gblk = (int *)a.q8w;
     uint4 *pd = (uint4 *)(gblk + sh.gblk_ints);
-    hipLaunchKernelGGL((k_q8_omega<WSEL>), dim3(sh.bwd_blocks), dim3(256), 0, st, a, gblk, pd);
+    hipLaunchKernelGGL((k_q8_omega<WSEL>), dim3(sh.bwd_blocks, 1, nsets), dim3(256), 0, st, a, gblk, pd, wss, q8ss,
+                       pss);
+    return hipGetLastError();
+}
+
+// the backward products with the block scales and weight digits at a.q8w
+hipError_t launch_bwd_pass(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+{
+    int *gblk = (int *)a.q8w;
+    uint4 *pd = (uint4 *)(gblk + sh.gblk_ints);
     const uint4 *msk = (const uint4 *)a.q8m;
     const dim3 bb(256);
     const int nit = nit_of(a.N), nct = a.Ucols / 16;
@@ -601,6 +618,13 @@ hipError_t launch_bwd(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
         hipLaunchKernelGGL((k_q8_bwd<1>), dim3(nb * ny, 1, nit), bb, 0, st, a, gblk, pd, msk, 1, ny);
     }
     return hipGetLastError();
+}
+
+template <int WSEL>
+hipError_t launch_bwd(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+{
+    const hipError_t e = launch_omega<WSEL>(a, sh, st);
+    return e != hipSuccess ? e : launch_bwd_pass(a, sh, st);
 }
 
 } // namespace
@@ -830,29 +854,59 @@ hipError_t nq_q8c_pack(const SvfArgs &a0, const Q8Shape &sh, hipStream_t st)
 // backward runs A with Re w + Im w, R with Im w, I with Re w.  The digit sets satisfy
 // X_A = X_R + X_I exactly, so the operator is the exact complex product of the quantised
 // factors, up to the quantisation of v and w (2^-47 of their scale), as the real form.
+// The per-set small kernels run once for all three sets: the v digits (GEMM form), the row
+// weights and block scales, and the ordered reduction of the partial rows; the buffers
+// a.vq, a.q8w and a.part hold three sets (nq_q8c_sizes).  Every set's sums are the same
+// operations in the same order as when the sets run one after the other.
 hipError_t nq_q8c_apply(const SvfArgs &a0, const Q8Shape &sh, const double *vin, double *w3, void *scratch,
                         double *out, hipStream_t st)
 {
     SvfArgs a = a0;
     const long long P = a.P, S = a.S;
     const size_t qs = (size_t)sh.rpad * a.Ucols * QU, fs = sh.uf_bytes;
+    const long long pss = (long long)sh.bwd_blocks * (P + 1);  // partial rows of one set (doubles)
     hipError_t e = hipSuccess;
+    a.v = vin;
+    a.ov = vin + 3 * P + 2;  // zero: o^T v enters in the combine
+    if (sh.gemm) e = nq_q8_prep_vg_sets(a, sh, 3, P, sh.vq_bytes, st);
     for (int t = 0; t < 3 && e == hipSuccess; t++) {
         a.Uq = (const uint8_t *)a0.Uq + t * qs;
         a.Uf = sh.gemm ? (const void *)((const uint8_t *)a0.Uf + t * fs) : nullptr;
         a.v = vin + t * P;
-        a.ov = vin + 3 * P + 2;  // zero: o^T v enters in the combine
         a.w = w3 + t * S;
-        e = q8_forward(a, sh, st);
+        a.vq = (uint8_t *)a0.vq + t * sh.vq_bytes;
+        if (sh.gemm) {
+            e = nq_q8_fwd_gemm(a, sh, st, false, false);
+            if (e == hipSuccess && !(a.K == 1 || a.K == 2 || a.K == 4)) e = nq_launch_svf_combine(false, a, st);
+        } else {
+            e = q8_forward(a, sh, st);
+        }
     }
     if (e != hipSuccess) return e;
     const long long nb = (S + 255) / 256 < 4096 ? (S + 255) / 256 : 4096;
     hipLaunchKernelGGL(k_q8c_combine3, dim3((unsigned)nb), dim3(256), 0, st, w3, S, vin + 3 * P);
     e = hipGetLastError();
+    a = a0;
+    a.w = w3;
+    if (e == hipSuccess) e = launch_omega<0>(a, sh, st, 3, S, (long long)sh.w_bytes, pss);
     for (int t = 0; t < 3 && e == hipSuccess; t++) {
         a.Uq = (const uint8_t *)a0.Uq + t * qs;
         a.w = w3 + t * S;
-        e = q8_backward(a, sh, scratch, out + t * (P + 1), st);
+        a.q8w = (uint8_t *)a0.q8w + t * sh.w_bytes;
+        a.part = (double *)a0.part + t * pss;
+        e = launch_bwd_pass(a, sh, st);
     }
+    if (e == hipSuccess)
+        e = nq_chunk_reduce_sets((const double *)a0.part, P + 1, P + 1, sh.bwd_blocks, (double *)scratch, out, 3, pss,
+                                 P + 1, st);
     return e;
+}
+
+// device bytes the three-set complex S*v needs in a.vq, a.q8w, a.part and its scratch
+void nq_q8c_sizes(const Q8Shape &sh, long long P, size_t *vq, size_t *q8w, size_t *part, size_t *scratch)
+{
+    *vq = 3 * sh.vq_bytes;
+    *q8w = 3 * sh.w_bytes;
+    *part = (size_t)3 * sh.bwd_blocks * (P + 1) * 8;
+    *scratch = (size_t)3 * ((sh.bwd_blocks + 31) / 32) * (P + 1) * 8;
 }

@@ -52,13 +52,16 @@ __device__ inline double dpp_sum16x4(const double (&h)[4], int l)
 // pairs cost 3 MFMAs.  Operands: A lane l, byte e = digit of row 16 rt + l%16 at column
 // 64 cg + 16 (l/16) + e (the Uf chunk 64 cg + l); B lane l, byte e = digit of the same
 // column for spin 16 st + l%16 (slot st QV + u), or digit l%16 of bq (slot NST QV).
-// Qg: [cg][slot][lane] uint4, then the scale 2^(g - 46) as a double.
-__global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst, uint4 *qg)
+// Qg: [cg][slot][lane] uint4, then the scale 2^(g - 46) as a double.  blockIdx.z: the
+// set (v + z vss doubles, qg + z qss uint4) when several vectors are prepared at once.
+__global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst, uint4 *qg, long long vss,
+                                                    long long qss)
 {
     __shared__ double red[4];
     const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
     const long long KM = (long long)K * M;
-    const double *v = (const double *)a.v;
+    const double *v = (const double *)a.v + blockIdx.z * vss;
+    qg += blockIdx.z * qss;
     auto Bval = [&](int col, int i) -> double {
         if (col < KM) return v[(long long)(col / M) * a.L + N + M + (long long)i * M + col % M];
         if (col < KM + K) return v[(long long)(col - KM) * a.L + i];
@@ -276,9 +279,20 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
 
 }  // namespace
 
-hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows)
+// the v operands of nsets vectors (a.v + z vss) into a.vq + z qss_bytes, one launch
+hipError_t nq_q8_prep_vg_sets(const SvfArgs &a, const Q8Shape &sh, int nsets, long long vss, size_t qss_bytes,
+                              hipStream_t st)
 {
-    hipLaunchKernelGGL(k_q8_prep_vg, dim3(sh.ncg64 * (sh.nst + 1)), dim3(256), 0, st, a, sh.ncg64, sh.nst, (uint4 *)a.vq);
+    hipLaunchKernelGGL(k_q8_prep_vg, dim3(sh.ncg64 * (sh.nst + 1), 1, nsets), dim3(256), 0, st, a, sh.ncg64, sh.nst,
+                       (uint4 *)a.vq, vss, (long long)(qss_bytes / 16));
+    return hipGetLastError();
+}
+
+hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows, bool prep)
+{
+    if (prep)
+        hipLaunchKernelGGL(k_q8_prep_vg, dim3(sh.ncg64 * (sh.nst + 1)), dim3(256), 0, st, a, sh.ncg64, sh.nst,
+                           (uint4 *)a.vq, 0LL, 0LL);
     const size_t lds = sh.qg_bytes - 16;
     const dim3 grid(sh.fwd_blocks), blk(256);
 #define NQ_FWDG(NST, NCG)                                                                              \

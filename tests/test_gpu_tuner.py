@@ -1,8 +1,8 @@
 """The sampler's launch-shape tuner (mixed path).
 
-The first sampling calls of a problem shape try candidate launch shapes in turn (sweep
-chains per block, sweep tables in LDS or L2, measurement samples per block), timed on the
-device; later calls use the fastest.  No lane computes anything else under another shape,
+The first sampling call of a problem shape runs the default launch shapes untimed; the next
+calls try candidate shapes in turn (sweep chains per block, sweep tables in LDS or L2,
+measurement samples per block), timed on the device; later calls use the fastest.  No lane computes anything else under another shape,
 so every call must give the same spins, accept counts, local energies, factor/O rows and
 moments bit for bit as the default shapes (NQES_TUNE=0).
 """
@@ -46,7 +46,7 @@ def test_tuner_trials_same_bits(nqes, monkeypatch, N, M, K, C, m, B, cplx):
     monkeypatch.delenv("NQES_TUNE")
     launches = []
     with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
-        for _ in range(10):  # at most 8 candidates, then the tuned choice
+        for _ in range(11):  # the untimed first call, at most 8 candidates, then the tuned choice
             out, launch = run(s)
             launches.append(launch)
             assert np.array_equal(out[0], ref[0]) and np.array_equal(out[2], ref[2]), launch
@@ -54,9 +54,10 @@ def test_tuner_trials_same_bits(nqes, monkeypatch, N, M, K, C, m, B, cplx):
             assert np.array_equal(out[3], ref[3]), launch
             for key in ("o", "skk", "f", "emat"):
                 assert np.array_equal(out[4][key], ref[4][key]), (key, launch)
-    trials = [l for l in launches if l["tuning"] == 1]
+    assert launches[0] == dict(launch0, tuning=1)  # the untimed first call: default shapes
+    trials = [l for l in launches[1:] if l["tuning"] == 1]
     shapes = {(l["chains_per_block"], l["tables_in_lds"], l["samples_per_block"]) for l in trials}
-    assert launches[0] == dict(launch0, tuning=1)  # the default shapes are tried first
+    assert trials[0] == launches[0]  # the default shapes are the first timed candidate
     assert len(trials) >= 2 and len(shapes) == len(trials)
     assert launches[-1]["tuning"] == 0 and launches[-2] == launches[-1]
     fixed = (launches[-1]["chains_per_block"], launches[-1]["tables_in_lds"], launches[-1]["samples_per_block"])
