@@ -37,7 +37,7 @@
 // Layout Ug (written once per sample set by k_q8_packg): per 16-row tile rt, digit plane
 // t and 64-column group cg, 64 lanes x 16 bytes; lane (g, lo) holds row 16 rt + lo, byte
 // 4 b + d = digit t of column 64 cg + 16 b + 4 g + d.  The forward A operand of the
-// int8 MFMA is lane l's 16 bytes as they are; the v digits (k_q8_prep_vf) use the same
+// int8 MFMA is lane l's 16 bytes as they are; the v digits (fz_v_operands) use the same
 // column order (any order of the 64 columns of a group serves the forward sum, as long
 // as both operands share it).  Dword b of a lane then holds four adjacent columns
 // (4 g .. 4 g + 3) of column tile 4 cg + b: one dword of a row of that tile's matrix.
@@ -117,7 +117,6 @@ struct FzArgs {
     const uint4 *Ug;
     const float *Xc;        // [rpad][K] quantised c columns
     const double *escale;   // [rpad] 1 / f_r
-    const uint4 *qg;        // forward B operands, then the scale of v
     const double *v;        // v (the a blocks)
     const double *ov;       // o^T v
     const nq64::cd *E;      // local energies (wsel 2)
@@ -190,24 +189,27 @@ __global__ __launch_bounds__(256) void k_q8_packg(const float *U, long long R, l
     }
 }
 
-// ---- v -> forward B operands in Ug's column order.  Block (cg, st), st = nst: the bias
-// slot.  qg[(cg NSLOT + slot) 64 + lane]: lane (g, lo) byte 4 d + b = digit u of v at
-// column 64 cg + 16 b + 4 g + d for spin 16 st + lo (slot st QV + u), or digit lo of the
-// bias of that column (slot nst QV); then the scale 2^(gex - 46) of v as a double.  One
-// scale 2^gex > max |v| for all of v.
-__global__ __launch_bounds__(256) void k_q8_prep_vf(FzArgs a, int ncg, int nst, uint4 *qg)
+// ---- v -> forward B operands in Ug's column order, formed in LDS by every block of the
+// one-pass kernel (34 KB of v read from L2 per block; a separate kernel and its launch
+// cost more).  qs[(cg NSLOT + slot) 64 + lane]: lane (g, lo) byte 4 b + d = digit u of v
+// at column 64 cg + 16 b + 4 g + d for spin 16 st + lo (slot st QV + u), or digit lo of
+// the bias of that column (slot NST QV).  One scale 2^gex > max |v| for all of v; returns
+// 2^(gex - 46).  red: one double per wavefront of LDS, free until the block's next barrier.
+template <int NCG, int NST>
+__device__ inline double fz_v_operands(const FzArgs &a, uint4 *qs, double *red)
 {
-    __shared__ double red[4];
+    constexpr int NSLOT = NST * QV + 1;
     const int N = a.N, M = a.M, K = a.K, KM = a.KM;
     const double *v = a.v;
+    const int nthr = blockDim.x, nwv = nthr >> 6;
     double mx = 0.0;
     {
-        const long long NP = (long long)N + M + (long long)N * M, nv = K * NP, step = blockDim.x;
-        for (long long q0 = threadIdx.x; q0 < nv; q0 += 4 * step) {
+        const long long NP = (long long)N + M + (long long)N * M, nv = K * NP;
+        for (long long q0 = threadIdx.x; q0 < nv; q0 += 4 * nthr) {
             double x[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const long long q = q0 + u * step;
+                const long long q = q0 + (long long)u * nthr;
                 x[u] = q < nv ? v[(q / NP) * a.L + q % NP] : 0.0;
             }
 #pragma unroll
@@ -218,49 +220,54 @@ __global__ __launch_bounds__(256) void k_q8_prep_vf(FzArgs a, int ncg, int nst, 
     for (int off = 1; off < 64; off <<= 1) mx = fmax(mx, __shfl_xor(mx, off));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
     __syncthreads();
-    mx = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    mx = red[0];
+    for (int q = 1; q < nwv; q++) mx = fmax(mx, red[q]);
     int gex = 0;
     if (mx > 0.0) (void)frexp(mx, &gex);
     const double sc = ldexp(1.0, VBITS - gex);
-    // this thread's dword b of lane (g, lo): columns 64 cg + 16 b + 4 g + d, byte d
-    const int lane = threadIdx.x & 63, lo = lane & 15, g = lane >> 4, b = threadIdx.x >> 6;
-    const int cg = blockIdx.x / (nst + 1), st = blockIdx.x % (nst + 1);
-    const int nslot = nst * QV + 1;
-    uint32_t dg[QV] = {};
+    uint32_t *o = (uint32_t *)qs;
+    // item (cg, st, lane, b): dword b of lane (g, lo) in the slots of spin tile st (st = NST:
+    // the bias slot) of column group cg
+    for (int item = threadIdx.x; item < NCG * (NST + 1) * 256; item += nthr) {
+        const int b = item & 3, lane = (item >> 2) & 63, rest = item >> 8;
+        const int st = rest % (NST + 1), cg = rest / (NST + 1);
+        const int lo = lane & 15, g = lane >> 4;
+        uint32_t dg[QV] = {};
 #pragma unroll
-    for (int d = 0; d < 4; d++) {
-        const int col = 64 * cg + 16 * b + 4 * g + d;
-        double val = 0.0;
-        if (col < KM) {
-            const long long base = (long long)(col / M) * a.L;
-            if (st < nst) {
-                const int i = 16 * st + lo;
-                val = i < N ? v[base + N + M + (long long)i * M + col % M] : 0.0;
-            } else {
-                val = v[base + N + col % M];
+        for (int d = 0; d < 4; d++) {
+            const int col = 64 * cg + 16 * b + 4 * g + d;
+            double val = 0.0;
+            if (col < KM) {
+                const long long base = (long long)(col / M) * a.L;
+                if (st < NST) {
+                    const int i = 16 * st + lo;
+                    val = i < N ? v[base + N + M + (long long)i * M + col % M] : 0.0;
+                } else {
+                    val = v[base + N + col % M];
+                }
+            }
+            long long x = (long long)rint(val * sc);
+#pragma unroll
+            for (int u = 0; u < QV; u++) {
+                const long long dd = (u < QV - 1) ? bal8l(x) : x;
+                dg[u] |= ((uint32_t)dd & 255u) << (8 * d);
+                x = (x - dd) >> 8;
             }
         }
-        long long x = (long long)rint(val * sc);
+        uint32_t *oc = o + (size_t)cg * NSLOT * 256;
+        if (st < NST) {
 #pragma unroll
-        for (int u = 0; u < QV; u++) {
-            const long long dd = (u < QV - 1) ? bal8l(x) : x;
-            dg[u] |= ((uint32_t)dd & 255u) << (8 * d);
-            x = (x - dd) >> 8;
+            for (int u = 0; u < QV; u++) oc[((st * QV + u) * 64 + lane) * 4 + b] = dg[u];
+        } else {
+            uint32_t bw = 0u;
+#pragma unroll
+            for (int u = 0; u < QV; u++)
+                if (lo == u) bw = dg[u];
+            oc[(NST * QV * 64 + lane) * 4 + b] = bw;
         }
     }
-    uint32_t *o = (uint32_t *)(qg + (size_t)cg * nslot * 64);
-    if (st < nst) {
-#pragma unroll
-        for (int u = 0; u < QV; u++) o[((st * QV + u) * 64 + lane) * 4 + b] = dg[u];
-    } else {
-        uint32_t bw = 0u;
-#pragma unroll
-        for (int u = 0; u < QV; u++)
-            if (lo == u) bw = dg[u];
-        o[(nst * QV * 64 + lane) * 4 + b] = bw;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        *(double *)(qg + (size_t)ncg * nslot * 64) = mx > 0.0 ? ldexp(1.0, gex - VBITS) : 0.0;
+    __syncthreads();
+    return mx > 0.0 ? ldexp(1.0, gex - VBITS) : 0.0;
 }
 
 template <int NCG, int NST, int W>
@@ -310,9 +317,7 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
     double vsc = 0.0, ov = 0.0;
     double qa[NST][KS];  // v_a[k][16 st + lo] / vsc: the c columns' forward term in digit units
     if constexpr (MODE == 0) {
-        for (int q = threadIdx.x; q < L::NQ; q += blockDim.x) qs[q] = a.qg[q];
-        __syncthreads();
-        vsc = *(const double *)(a.qg + L::NQ);
+        vsc = fz_v_operands<NCG, NST>(a, qs, (double *)bufs);
         ov = *a.ov;
         const double inv = vsc > 0.0 ? 1.0 / vsc : 0.0;  // a power of two
 #pragma unroll
@@ -738,7 +743,6 @@ FzArgs fz_args(const SvfArgs &a, const Q8Shape &sh)
     f.Ug = (const uint4 *)a.Uq;
     f.Xc = (const float *)a.q8m;
     f.escale = a.escale;
-    f.qg = (const uint4 *)a.vq;
     f.v = (const double *)a.v;
     f.ov = (const double *)a.ov;
     f.E = (const nq64::cd *)a.E;
@@ -775,10 +779,7 @@ hipError_t NQ_FZ_CAT(nq_q8f_apply_k, NQES_FZ_K)(const SvfArgs &a, const Q8Shape 
                                                  hipStream_t st)
 {
     FzArgs f = fz_args(a, sh);
-    const int nst = sh.fz_nst, ncg = sh.fz_ncg;
-    hipLaunchKernelGGL(k_q8_prep_vf, dim3(ncg * (nst + 1)), dim3(256), 0, st, f, ncg, nst, (uint4 *)a.vq);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = fz_launch<0>(f, sh, st);
+    hipError_t e = fz_launch<0>(f, sh, st);
     if (e == hipSuccess) e = nq_chunk_reduce(false, a.part, a.P + 1, a.P + 1, sh.fz_blocks, scratch, out, st);
     return e;
 }
