@@ -53,6 +53,9 @@ __device__ inline cf divT(cf a, cf b)
     float r = __builtin_amdgcn_rcpf(b.re * b.re + b.im * b.im);
     return mkf((a.re * b.re + a.im * b.im) * r, (a.im * b.re - a.re * b.im) * r);
 }
+// a + s b for a sign s = +-1: one FMA per component, equal bit for bit to a + b or a - b
+__device__ inline float addsT(float a, float s, float b) { return fmaf(s, b, a); }
+__device__ inline cf addsT(cf a, float s, cf b) { return mkf(fmaf(s, b.re, a.re), fmaf(s, b.im, a.im)); }
 __device__ inline float scaleT(float s, float a) { return s * a; }
 __device__ inline cf scaleT(float s, cf a) { return mkf(s * a.re, s * a.im); }
 template <typename T> __device__ inline T oneT();

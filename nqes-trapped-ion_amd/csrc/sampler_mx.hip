@@ -214,12 +214,12 @@ __device__ __attribute__((always_inline)) inline void sweep_refresh(const MixedS
                     for (int q = 0; q < JQ; q++) wv[kk][q] = wp[((long long)kk * N + i) * Mp + g + G * q];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
-                    const bool dn = (bits[k] >> ii) & 1u;
+                    // theta +- w as one FMA with the spin's sign (exact product by +-1)
+                    const float sg = ((bits[k] >> ii) & 1u) ? -1.0f : 1.0f;
 #pragma unroll
                     for (int kk = 0; kk < K; kk++)
 #pragma unroll
-                        for (int q = 0; q < JQ; q++)
-                            th[k][kk][q] = dn ? subT(th[k][kk][q], wv[kk][q]) : addT(th[k][kk][q], wv[kk][q]);
+                        for (int q = 0; q < JQ; q++) th[k][kk][q] = addsT(th[k][kk][q], sg, wv[kk][q]);
                 }
             }
         }
