@@ -34,6 +34,7 @@ CASES = [
     (40, 80, 2, 16, 3, 2, False),
     (80, 80, 2, 16, 3, 2, False),  # three spin words per replica (streamed S*v above 64 spins)
     (72, 72, 1, 16, 3, 2, True),
+    (64, 128, 4, 16, 3, 2, False),  # the large workload's shape (S reduced: FP64 O is 26 MB here)
 ]
 
 
@@ -201,12 +202,14 @@ def test_q8_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
 @pytest.mark.parametrize("N,M,K,C,m", [(8, 16, 1, 37, 5), (8, 16, 2, 64, 8), (17, 34, 3, 29, 7), (32, 64, 2, 96, 9),
                                         (40, 80, 2, 33, 5), (64, 128, 4, 21, 3), (24, 24, 4, 50, 4)])
 def test_q8c_sv_matches_fp64_mfma(nqes, monkeypatch, N, M, K, C, m):
-    """Complex weights on the int8 form: each complex factor row is split into its real
-    and imaginary rows (quantised separately, three digits each), and S*v runs as two
-    forward and two backward int8 passes (real and imaginary parts of v; row weights
-    (Re w, Im w) and (Im w, -Re w)).  Against the FP64-MFMA form on the same samples:
-    equal up to the factor quantisation (2^-24 of each split row's largest entry; bound
-    5e-7), linear to 1e-11 and Hermitian: <u, S v> = conj(<v, S u>) to 1e-11."""
+    """Complex weights on the int8 form: each complex factor row is quantised with one
+    scale into three digit sets, X_R, X_I and X_A = X_R + X_I (exact integers), and S*v
+    runs Gauss's three real products each way (forward: A on Re v, R on Im v - Re v, I on
+    Re v + Im v; backward: A with Re w + Im w, R with Im w, I with Re w), the three sets'
+    small kernels (v digits, row weights, ordered reductions) in one launch each.
+    Against the FP64-MFMA form on the same samples: equal up to the factor quantisation
+    (2^-24 of each row's largest |Re| + |Im|; bound 5e-7), linear to 1e-11 and Hermitian:
+    <u, S v> = conj(<v, S u>) to 1e-11."""
     print(f"case complex N={N} M={M} K={K} C={C} m={m}", flush=True)
     J = load_J(N)
     W = random_weights(N, M, K, seed=9 + N, scale=0.1, complex_=True)
