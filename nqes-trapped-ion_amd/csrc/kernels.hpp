@@ -208,7 +208,10 @@ hipError_t nq_q8_pack(const SvfArgs &a, const Q8Shape &sh, hipStream_t st);
 hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void *out, hipStream_t st);
 // its forward pass as one GEMM over the columns (sh.gemm; sv_q8_gemm.hip): w_s directly for
 // K = 1, 2, 4 unless rows, else the row sums wrow[r] (a.ncg = 1)
-hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows = false, bool prep = true);
+// nsets > 1 (K in {1, 2, 4}): that many digit sets in one launch, set z at a.Uf + z fss
+// bytes, a.vq + z vss bytes (operands from nq_q8_prep_vg_sets), a.w + z wss doubles
+hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows = false, bool prep = true,
+                          int nsets = 1, long long fss = 0, long long vss = 0, long long wss = 0);
 hipError_t nq_q8_prep_vg_sets(const SvfArgs &a, const Q8Shape &sh, int nsets, long long vss, size_t qss_bytes,
                               hipStream_t st);
 // Complex weights on the int8 form (three real passes per product, Gauss's form): the

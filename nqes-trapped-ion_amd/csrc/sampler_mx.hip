@@ -585,6 +585,7 @@ __device__ inline void measure_sample(const MixedSampleArgs &a, const ChSh<typen
                 prod0[k] = oneD<D>();
                 prod1[k] = oneD<D>();
             }
+#pragma unroll 1
             for (int j0 = 0; j0 < Mp; j0 += 8) {
                 T pf0[K], pf1[K];
 #pragma unroll

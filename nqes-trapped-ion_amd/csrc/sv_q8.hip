@@ -440,10 +440,21 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
 // (166 registers, accumulators in VGPRs) cover each other's loads (measured faster than
 // two wavefronts with the next chunk's operands prefetched into registers).  Writes
 // part[b][0..P) for its columns: a and w blocks T - 2 sum b X Omega, b block T.
+// blockIdx.z = set nzs + slice: digit set `set` (a.Uq + set uqss bytes, block scales and
+// weight digits + set q8ss bytes, partial rows a.part + set pss doubles) when several
+// sets run in one launch; `slice` takes the spin-tile slices of grids with nzs > 1.
 template <int NIT>
 __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, const uint4 *pd, const uint4 *msk,
-                                                int zw, int nyl)
+                                                int zw, int nyl, int nzs, long long uqss, long long q8ss,
+                                                long long pss)
 {
+    {
+        const int set = blockIdx.z / nzs;
+        a.Uq = (const uint8_t *)a.Uq + set * uqss;
+        a.part = (double *)a.part + set * pss;
+        gblk = (const int *)((const uint8_t *)gblk + set * q8ss);
+        pd = (const uint4 *)((const uint8_t *)pd + set * q8ss);
+    }
     const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
     const int NCT = UC / 16;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -459,7 +470,7 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
     const int ct = by * (4 / zw) + wv / zw;
     if (ct >= NCT) return;
     const int NIA = 16 * ((N + 15) / 16);  // spins of the mask array
-    const int zs = blockIdx.z * zw + wv % zw;
+    const int zs = (blockIdx.z % nzs) * zw + wv % zw;
     const int i0 = zs * NIT * 16;
     const long long R = a.S * K;
     const long long KM = (long long)K * M;
@@ -595,27 +606,33 @@ hipError_t launch_omega(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, int
     return hipGetLastError();
 }
 
-// the backward products with the block scales and weight digits at a.q8w
-hipError_t launch_bwd_pass(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
+// the backward products with the block scales and weight digits at a.q8w; nsets digit
+// sets (strides uqss, q8ss bytes and pss doubles) in one launch
+hipError_t launch_bwd_pass(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, int nsets = 1, long long uqss = 0,
+                           long long q8ss = 0, long long pss = 0)
 {
     int *gblk = (int *)a.q8w;
     uint4 *pd = (uint4 *)(gblk + sh.gblk_ints);
     const uint4 *msk = (const uint4 *)a.q8m;
     const dim3 bb(256);
     const int nit = nit_of(a.N), nct = a.Ucols / 16;
-    const unsigned nb = (unsigned)sh.bwd_blocks;
+    const unsigned nb = (unsigned)sh.bwd_blocks, zn = (unsigned)nsets;
     if (nit == 1) {
         const int ny = (nct + 3) / 4;
-        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(nb * ny, 1, 1), bb, 0, st, a, gblk, pd, msk, 1, ny);
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(nb * ny, 1, zn), bb, 0, st, a, gblk, pd, msk, 1, ny, 1, uqss, q8ss,
+                           pss);
     } else if (nit == 2) {  // one slice of two spin tiles
         const int ny = (nct + 3) / 4;
-        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(nb * ny, 1, 1), bb, 0, st, a, gblk, pd, msk, 1, ny);
+        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(nb * ny, 1, zn), bb, 0, st, a, gblk, pd, msk, 1, ny, 1, uqss, q8ss,
+                           pss);
     } else if (nit == 4) {  // two slices of two spin tiles, in one block
         const int ny = (nct + 1) / 2;
-        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(nb * ny, 1, 1), bb, 0, st, a, gblk, pd, msk, 2, ny);
+        hipLaunchKernelGGL((k_q8_bwd<2>), dim3(nb * ny, 1, zn), bb, 0, st, a, gblk, pd, msk, 2, ny, 1, uqss, q8ss,
+                           pss);
     } else {
         const int ny = (nct + 3) / 4;
-        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(nb * ny, 1, nit), bb, 0, st, a, gblk, pd, msk, 1, ny);
+        hipLaunchKernelGGL((k_q8_bwd<1>), dim3(nb * ny, 1, (unsigned)nit * zn), bb, 0, st, a, gblk, pd, msk, 1, ny,
+                           nit, uqss, q8ss, pss);
     }
     return hipGetLastError();
 }
@@ -869,17 +886,24 @@ hipError_t nq_q8c_apply(const SvfArgs &a0, const Q8Shape &sh, const double *vin,
     a.v = vin;
     a.ov = vin + 3 * P + 2;  // zero: o^T v enters in the combine
     if (sh.gemm) e = nq_q8_prep_vg_sets(a, sh, 3, P, sh.vq_bytes, st);
-    for (int t = 0; t < 3 && e == hipSuccess; t++) {
-        a.Uq = (const uint8_t *)a0.Uq + t * qs;
-        a.Uf = sh.gemm ? (const void *)((const uint8_t *)a0.Uf + t * fs) : nullptr;
-        a.v = vin + t * P;
-        a.w = w3 + t * S;
-        a.vq = (uint8_t *)a0.vq + t * sh.vq_bytes;
-        if (sh.gemm) {
-            e = nq_q8_fwd_gemm(a, sh, st, false, false);
-            if (e == hipSuccess && !(a.K == 1 || a.K == 2 || a.K == 4)) e = nq_launch_svf_combine(false, a, st);
-        } else {
-            e = q8_forward(a, sh, st);
+    if (sh.gemm && (a.K == 1 || a.K == 2 || a.K == 4)) {  // the GEMM passes write w: one launch
+        a.Uf = a0.Uf;
+        a.vq = a0.vq;
+        a.w = w3;
+        if (e == hipSuccess) e = nq_q8_fwd_gemm(a, sh, st, false, false, 3, (long long)fs, (long long)sh.vq_bytes, S);
+    } else {
+        for (int t = 0; t < 3 && e == hipSuccess; t++) {
+            a.Uq = (const uint8_t *)a0.Uq + t * qs;
+            a.Uf = sh.gemm ? (const void *)((const uint8_t *)a0.Uf + t * fs) : nullptr;
+            a.v = vin + t * P;
+            a.w = w3 + t * S;
+            a.vq = (uint8_t *)a0.vq + t * sh.vq_bytes;
+            if (sh.gemm) {
+                e = nq_q8_fwd_gemm(a, sh, st, false, false);
+                if (e == hipSuccess) e = nq_launch_svf_combine(false, a, st);
+            } else {
+                e = q8_forward(a, sh, st);
+            }
         }
     }
     if (e != hipSuccess) return e;
@@ -889,13 +913,7 @@ hipError_t nq_q8c_apply(const SvfArgs &a0, const Q8Shape &sh, const double *vin,
     a = a0;
     a.w = w3;
     if (e == hipSuccess) e = launch_omega<0>(a, sh, st, 3, S, (long long)sh.w_bytes, pss);
-    for (int t = 0; t < 3 && e == hipSuccess; t++) {
-        a.Uq = (const uint8_t *)a0.Uq + t * qs;
-        a.w = w3 + t * S;
-        a.q8w = (uint8_t *)a0.q8w + t * sh.w_bytes;
-        a.part = (double *)a0.part + t * pss;
-        e = launch_bwd_pass(a, sh, st);
-    }
+    if (e == hipSuccess) e = launch_bwd_pass(a, sh, st, 3, (long long)qs, (long long)sh.w_bytes, pss);
     if (e == hipSuccess)
         e = nq_chunk_reduce_sets((const double *)a0.part, P + 1, P + 1, sh.bwd_blocks, (double *)scratch, out, 3, pss,
                                  P + 1, st);

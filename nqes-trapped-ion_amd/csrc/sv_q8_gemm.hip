@@ -138,9 +138,16 @@ __global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst,
 // then 16 lanes by DPP.  Output wrow[r].
 // KW = K in 1, 2, 4: the sample weights w_s written directly (no combine step); KW = 0:
 // the row sums wrow[r]
+// blockIdx.z: the digit set (a.Uf + z fss bytes, a.vq + z vss bytes, a.w + z wss
+// doubles) when several sets run in one launch (KW != 0 only: the row sums of KW = 0
+// share one buffer)
 template <int NST, int NCG, int KW>
-__global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs a)
+__global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs a, long long fss, long long vss,
+                                                                         long long wss)
 {
+    a.Uf = (const uint8_t *)a.Uf + blockIdx.z * fss;
+    a.vq = (uint8_t *)a.vq + blockIdx.z * vss;
+    a.w = (double *)a.w + blockIdx.z * wss;
     constexpr int NSF = QU + QV - 1;
     constexpr int NSLOT = NST * QV + 1;
     constexpr int NQ = NCG * NSLOT * 64;
@@ -288,20 +295,22 @@ hipError_t nq_q8_prep_vg_sets(const SvfArgs &a, const Q8Shape &sh, int nsets, lo
     return hipGetLastError();
 }
 
-hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows, bool prep)
+hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows, bool prep, int nsets,
+                          long long fss, long long vss, long long wss)
 {
+    if (nsets > 1 && (rows || !(a.K == 1 || a.K == 2 || a.K == 4))) return hipErrorInvalidValue;
     if (prep)
         hipLaunchKernelGGL(k_q8_prep_vg, dim3(sh.ncg64 * (sh.nst + 1)), dim3(256), 0, st, a, sh.ncg64, sh.nst,
                            (uint4 *)a.vq, 0LL, 0LL);
     const size_t lds = sh.qg_bytes - 16;
-    const dim3 grid(sh.fwd_blocks), blk(256);
-#define NQ_FWDG(NST, NCG)                                                                              \
-    do {                                                                                              \
-        if (rows) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0>), grid, blk, lds, st, a);               \
-        else if (a.K == 1) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 1>), grid, blk, lds, st, a);      \
-        else if (a.K == 2) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 2>), grid, blk, lds, st, a);      \
-        else if (a.K == 4) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 4>), grid, blk, lds, st, a);      \
-        else hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0>), grid, blk, lds, st, a);                    \
+    const dim3 grid(sh.fwd_blocks, 1, nsets), blk(256);
+#define NQ_FWDG(NST, NCG)                                                                                         \
+    do {                                                                                                         \
+        if (rows) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0>), grid, blk, lds, st, a, fss, vss, wss);           \
+        else if (a.K == 1) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 1>), grid, blk, lds, st, a, fss, vss, wss);  \
+        else if (a.K == 2) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 2>), grid, blk, lds, st, a, fss, vss, wss);  \
+        else if (a.K == 4) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 4>), grid, blk, lds, st, a, fss, vss, wss);  \
+        else hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0>), grid, blk, lds, st, a, fss, vss, wss);                \
     } while (0)
     switch (sh.nst * 8 + sh.ncg64) {
     case 9: NQ_FWDG(1, 1); break;
