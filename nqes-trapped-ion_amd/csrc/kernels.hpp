@@ -164,6 +164,9 @@ struct SvfArgs {
     void *vq;                  // int8 form: v operands, column scales and biases
     void *q8w;                 // int8 form: backward block scales and weight digits
     void *q8m;                 // int8 form: spin masks of the backward A operand
+    int qcols;                 // int8 form: digit columns per row (Ucols, or K M when the K c columns
+                               // are kept apart as integers in xc)
+    float *xc;                 // int8 form with qcols < Ucols: the c columns' quantised integers [rows][K]
 };
 // ok = 0: the factored form does not apply (N > 64).  The forward pass splits the
 // Ucols/16 column tiles into fwd_cgs groups of fwd_cgt (B of a group fits in LDS); the
@@ -189,6 +192,11 @@ struct Q8Shape {
     // chunks, fz_ncg 64-column groups of the K M tanh columns, fz_nst spin tiles
     int fused, fz_ncg, fz_nst, fz_blocks;
     size_t ug_bytes;  // layout Ug (3 digit planes x 64 fz_ncg columns per row)
+    // two-pass GEMM form with K in {1, 2, 4}: the K c columns are kept apart (quantised
+    // integers, FP64 arithmetic) when they would start a column tile of their own:
+    // qcols = K M < ucols digit columns, xc_bytes per digit set (else qcols = ucols, 0)
+    int qcols;
+    size_t xc_bytes;
 };
 Q8Shape nq_q8_shape(int N, int M, int K, long long R, bool cplx = false);
 // the one-pass form, per K (sv_q8_fused.hip compiled once per K)
@@ -209,9 +217,10 @@ hipError_t nq_q8_apply(const SvfArgs &a, const Q8Shape &sh, void *scratch, void 
 // its forward pass as one GEMM over the columns (sh.gemm; sv_q8_gemm.hip): w_s directly for
 // K = 1, 2, 4 unless rows, else the row sums wrow[r] (a.ncg = 1)
 // nsets > 1 (K in {1, 2, 4}): that many digit sets in one launch, set z at a.Uf + z fss
-// bytes, a.vq + z vss bytes (operands from nq_q8_prep_vg_sets), a.w + z wss doubles
+// bytes, a.vq + z vss bytes (operands from nq_q8_prep_vg_sets), a.w + z wss doubles,
+// a.xc + z xss floats
 hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows = false, bool prep = true,
-                          int nsets = 1, long long fss = 0, long long vss = 0, long long wss = 0);
+                          int nsets = 1, long long fss = 0, long long vss = 0, long long wss = 0, long long xss = 0);
 hipError_t nq_q8_prep_vg_sets(const SvfArgs &a, const Q8Shape &sh, int nsets, long long vss, size_t qss_bytes,
                               hipStream_t st);
 // Complex weights on the int8 form (three real passes per product, Gauss's form): the

@@ -177,6 +177,7 @@ struct nqes_ctx {
     DevBuf dw3, dvin3, dsvout3;                        // complex weights, int8 form: per-set sample sums,
                                                        // operands in, the three raw sums out
     DevBuf dUmax;                                      // max |U| per factor row
+    DevBuf dxc;                                        // int8 form: the c columns' integers (kept apart)
     hipGraph_t svg = nullptr;                          // int8 S*v sequence, captured once per
     hipGraphExec_t svx = nullptr;                      // sample set and replayed per MINRES step
     double *hv_pin = nullptr, *hr_pin = nullptr;       // pinned v | o^T v and result (P + 1 each)
@@ -333,7 +334,8 @@ int64_t nqes_required_bytes(int N, int M, int K, int C, int m, int precision)
         // int8 form; complex weights keep three digit sets per row
         const Q8Shape q8 = nq_q8_shape(N, M, K, S * K);
         if (q8.ok) {
-            b += 3 * (q8.rpad * (long long)q8.ucols * 3 + (long long)q8.uf_bytes) + q8.rpad * 12 +
+            b += 3 * (q8.rpad * (long long)q8.ucols * 3 + (long long)q8.uf_bytes + (long long)q8.xc_bytes) +
+                 q8.rpad * 12 +
                  (long long)(q8.vq_bytes + q8.w_bytes + q8.m_bytes) + 3 * S * 8 + (3 * P + 4) * 8 + 3 * (P + 1) * 8;
             if (cplx) {  // the complex S*v's per-set buffers hold three sets
                 size_t vq, q8w, part, scr;
@@ -489,6 +491,8 @@ static SvfArgs svf_args(nqes_ctx *c, long long S)
     f.vq = c->dvq.p;
     f.q8w = c->dq8w.p;
     f.q8m = c->dq8m.p;
+    f.qcols = (c->q8 || c->q8c) ? c->q8_shape.qcols : f.Ucols;
+    f.xc = static_cast<float *>(c->dxc.p);
     return f;
 }
 
@@ -723,6 +727,8 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
             HIPCHK(c, c->dq8m.ensure(c->q8_shape.m_bytes), "alloc spin masks");
             if (c->q8_shape.gemm)
                 HIPCHK(c, c->dUf.ensure(nsets * c->q8_shape.uf_bytes), "alloc row-major factor digits");
+            if (c->q8_shape.xc_bytes)
+                HIPCHK(c, c->dxc.ensure(nsets * c->q8_shape.xc_bytes), "alloc factor c columns");
         }
         HIPCHK(c, c->dU.ensure((size_t)S * K * c->svf_shape.ucols * tsz), "alloc S*v factors");
         HIPCHK(c, c->dwrow.ensure((size_t)cgs * S * K * 16), "alloc S*v rows");

@@ -77,15 +77,18 @@ __device__ inline double xor_sum16(double x)
 // rows [R, Rp) written as zeros.  Complex factor rows (CM = 1, 2, 3: U holds (re, im)
 // float pairs, Umax the row maxima of |re| + |im|): the digits of X_R = rint(re f_r),
 // X_I = rint(im f_r) or X_A = X_R + X_I, with f_r = (XMAX - 1) / 2^e_r so that
-// |X_A| <= XMAX.
+// |X_A| <= XMAX.  QC < UC (c columns apart): digits of the first QC columns only, and the
+// integers X of the K c columns (columns KM .. KM + K - 1) into Xc[r K + k] (exact floats:
+// |X| <= XMAX < 2^24).
 template <int CM>
 __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, long long ntile, int UC, uint32_t *Uq,
-                                                 double *escale, uint4 *Uf, const float *Umax)
+                                                 double *escale, uint4 *Uf, const float *Umax, int QC, float *Xc,
+                                                 int KM, int K)
 {
     __shared__ __align__(16) uint8_t tile_t[4][QU * 256];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
-    const int NCT = UC / 16;
+    const int NCT = UC / 16, NQT = QC / 16;
     for (long long rt = (long long)blockIdx.x * 4 + wv; rt < ntile; rt += (long long)gridDim.x * 4) {
         float mx[4] = {0.f, 0.f, 0.f, 0.f};
         if (Umax) {  // the row maxima the measurement kernel wrote with the rows
@@ -120,7 +123,23 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
             const long long r = rt * 16 + 4 * g + j;
             if (lo == 0) escale[r] = (r < R && mx[j] > 0.f) ? 1.0 / sc[j] : 0.0;
         }
-        for (int ct = 0; ct < NCT; ct++) {
+        if (Xc && lo < K) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const long long r = rt * 16 + 4 * g + j;
+                long long x;
+                if constexpr (CM == 0) {
+                    const float u = (r < R) ? U[r * UC + KM + lo] : 0.f;
+                    x = (long long)rint((double)u * sc[j]);
+                } else {
+                    const float2 u = (r < R) ? ((const float2 *)U)[r * UC + KM + lo] : make_float2(0.f, 0.f);
+                    const long long xr = (long long)rint((double)u.x * sc[j]), xi = (long long)rint((double)u.y * sc[j]);
+                    x = CM == 1 ? xr : (CM == 2 ? xi : xr + xi);
+                }
+                Xc[r * K + lo] = (float)x;
+            }
+        }
+        for (int ct = 0; ct < NQT; ct++) {
             uint32_t pl[QU] = {0u, 0u, 0u};
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -141,7 +160,7 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
                     x = (x - d) >> 8;
                 }
             }
-            uint32_t *o = Uq + ((rt * NCT + ct) * 64 + lane) * QU;
+            uint32_t *o = Uq + ((rt * NQT + ct) * 64 + lane) * QU;
 #pragma unroll
             for (int t = 0; t < QU; t++) o[t] = pl[t];
             if (Uf) {
@@ -156,7 +175,7 @@ __global__ __launch_bounds__(256) void k_q8_pack(const float *U, long long R, lo
                 __builtin_amdgcn_wave_barrier();
                 if (lane < 16 * QU) {
                     const int t = lane >> 4, row = lane & 15;
-                    Uf[(rt * QU + t) * NCT * 16 + 16 * ct + row] = *(const uint4 *)(tl + t * 256 + row * 16);
+                    Uf[(rt * QU + t) * NQT * 16 + 16 * ct + row] = *(const uint4 *)(tl + t * 256 + row * 16);
                 }
                 __builtin_amdgcn_wave_barrier();
             }
@@ -350,18 +369,33 @@ __global__ void k_q8_masks(const uint32_t *spins, long long R, int N, int NW, in
 // max |omega| over its rows (gblk[b]), then per chunk the digit dwords
 // pd[(chunk QW + u) 4 + g] = uint4 over q of Omega = rint(omega 2^(46 - G_b)) (digit u,
 // least significant first); and part[b][P] = sum of wt(s) over its rows with r % K == 0.
-template <int WSEL>
-__global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *pd, long long wss, long long q8ss,
-                                                  long long pss)
+// With the c columns apart (a.qcols < a.Ucols; N <= 32, K <= 4) also the a block of
+// part[b], sum_r sigma_r[i] X_c[r][k] Omega_r 2^(G - 46), on int8 MFMA as the backward
+// pass forms the other blocks, T - 2 sum_r b_r[i] X_c Omega: per chunk the B operand
+// holds the three digits d of X_c[r][k] as columns 3 k + d (built in the wavefront's
+// LDS), the A operands are the weight digits u (masked by the spins, and unmasked for
+// T), one int32 accumulator per (spin tile, u): QW (1 + NIT) MFMAs per chunk, every
+// digit pair kept.  At the end each lane forms its column's FP64 sum over u (weight
+// 256^(d + u)), the three digit columns of k are added, and the four wavefronts' sums
+// are added in order.  CS = false: no a block (the kernel keeps its low register count);
+// CS = true compiled for three wavefronts per SIMD (168 registers; unconstrained, 188 and two).
+template <int WSEL, bool CS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CS ? 3 : 1))) void k_q8_omega(SvfArgs a, int *gblk, uint4 *pd, long long wss, long long q8ss,
+                                                  long long pss, long long xss)
 {
     // blockIdx.z: the digit set (weights a.w + z wss, block scales and digits + z q8ss
-    // bytes, partial rows a.part + z pss doubles) when several sets run in one launch
+    // bytes, partial rows a.part + z pss doubles, c columns a.xc + z xss floats) when
+    // several sets run in one launch
     a.w = (double *)a.w + blockIdx.z * wss;
     a.part = (double *)a.part + blockIdx.z * pss;
     gblk = (int *)((uint8_t *)gblk + blockIdx.z * q8ss);
     pd = (uint4 *)((uint8_t *)pd + blockIdx.z * q8ss);
+    const float *xc = CS ? a.xc + blockIdx.z * xss : nullptr;
     __shared__ uint32_t stg[4][QW][4][4];  // per wavefront: a chunk's digits in the A layout
     __shared__ double red[4], rsum[4];
+    __shared__ __align__(16) uint8_t xb[4][16][64];  // per wavefront: X_c digits, column 3 k + d, row t
+    __shared__ __align__(16) uint32_t xs[4][64];     // and the chunk's spin words
+    __shared__ double ared[4][128];                   // the wavefronts' a-block sums [k 32 + i]
     const int K = a.K;
     const long long R = a.S * K;
     const long long n64 = (R + 63) / 64;
@@ -369,17 +403,19 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
     block_chunks(n64, blockIdx.x, gridDim.x, c0, c1);
     const double *w = (const double *)a.w;
     const double *es = a.escale;
+    // sample of row r (R < 2^31, K <= 4): a shift for K = 1, 2, 4, a constant division for 3
+    auto sample = [&](long long r) -> unsigned { return K == 3 ? (unsigned)r / 3u : (unsigned)r >> (K >> 1); };
     auto wt = [&](long long r) -> double {
-        if constexpr (WSEL == 0) return w[r / K];
+        if constexpr (WSEL == 0) return w[sample(r)];
         else if constexpr (WSEL == 1) return 1.0;
-        else return ((const nq64::cd *)a.E)[r / K].re;
+        else return ((const nq64::cd *)a.E)[sample(r)].re;
     };
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     double mx = 0.0, sw = 0.0;
     for (long long r = c0 * 64 + threadIdx.x; r < c1 * 64 && r < R; r += blockDim.x) {
         const double x = wt(r);
         mx = fmax(mx, fabs(x * es[r]));
-        if (r % K == 0) sw += x;
+        if (sample(r) * (unsigned)K == (unsigned)r) sw += x;
     }
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
@@ -406,11 +442,45 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
     // digits into the wavefront's LDS slice, 24 lanes store them in the A layout (LDS
     // operations of one wavefront run in order; no block barrier)
     uint32_t(*st)[4][4] = stg[wv];
+    const int N = a.N, nit = (N + 15) / 16;
+    const int g4 = lane >> 4, lo = lane & 15;
+    v4i cacc[2][QW], ctacc[QW];  // a block: masked (spin tiles) and unmasked (T) products per digit u
+#pragma unroll
+    for (int u = 0; u < QW; u++) {
+        ctacc[u] = v4i{0, 0, 0, 0};
+        cacc[0][u] = v4i{0, 0, 0, 0};
+        cacc[1][u] = v4i{0, 0, 0, 0};
+    }
     for (long long ch = c0 + wv; ch < c1; ch += 4) {
+        // the chunk's global loads first, issued together: the row's weight and scale, its
+        // K c columns (|X| < 2^24; rows past R hold zeros), its spin word
+        const long long r = ch * 64 + lane;
+        const double om = r < R ? wt(r) * es[r] : 0.0;
+        float xv[4] = {0.f, 0.f, 0.f, 0.f};
+        uint32_t sw0 = 0u;
+        if constexpr (CS) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) xv[k] = xc[r * K + (k < K ? k : 0)];
+            sw0 = r < R ? a.spins[r * a.NW] : 0u;
+        }
         {
             const int t = lane;
-            const long long r = ch * 64 + t;
-            long long q = r < R ? (long long)rint(wt(r) * es[r] * wsc) : 0;
+            if constexpr (CS) {  // digits of the c columns, the spin word
+                xs[wv][t] = sw0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    if (k < K) {
+                        int x = (int)xv[k];
+#pragma unroll
+                        for (int d = 0; d < QU; d++) {
+                            const int dg = (d < QU - 1) ? (int)(int8_t)(uint8_t)(x & 255) : x;
+                            xb[wv][3 * k + d][t] = (uint8_t)(dg & 255);
+                            x = (x - dg) >> 8;
+                        }
+                    }
+                }
+            }
+            long long q = (long long)rint(om * wsc);
             const int qq = t >> 4, gg = (t >> 2) & 3, j = t & 3;
 #pragma unroll
             for (int u = 0; u < QW; u++) {
@@ -424,7 +494,68 @@ __global__ __launch_bounds__(256) void k_q8_omega(SvfArgs a, int *gblk, uint4 *p
             const int u = lane >> 2, g = lane & 3;
             pd[(ch * QW + u) * 4 + g] = uint4{st[u][g][0], st[u][g][1], st[u][g][2], st[u][g][3]};
         }
+        if constexpr (CS) {
+            // B: lane (g, col) dword q byte j = digit of row 16 q + 4 g + j at column col;
+            // A: the digit-u dwords of the same rows (lanes of row group g), masked by the
+            // spins 16 it + lo (byte j = 0xff where that spin of row 16 q + 4 g + j is down; spins
+            // >= N give products that are not written)
+            v4i bx;
+            uint4 nm[2];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                bx[q] = lo < 3 * K ? *(const int *)&xb[wv][lo][16 * q + 4 * g4] : 0;
+                const uint4 w4 = *(const uint4 *)&xs[wv][16 * q + 4 * g4];
+#pragma unroll
+                for (int it = 0; it < 2; it++) {
+                    const int i = 16 * it + lo;
+                    const uint32_t m = (((w4.x >> i) & 1u) | (((w4.y >> i) & 1u) << 8) | (((w4.z >> i) & 1u) << 16) |
+                                        (((w4.w >> i) & 1u) << 24)) * 0xffu;
+                    if (q == 0) nm[it].x = m;
+                    else if (q == 1) nm[it].y = m;
+                    else if (q == 2) nm[it].z = m;
+                    else nm[it].w = m;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < QW; u++) {
+                const uint32_t *pu = st[u][g4];
+                const v4i aall = v4i{(int)pu[0], (int)pu[1], (int)pu[2], (int)pu[3]};
+                ctacc[u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aall, bx, ctacc[u], 0, 0, 0);
+#pragma unroll
+                for (int it = 0; it < 2; it++) {
+                    if (it < nit) {
+                        const v4i av = v4i{(int)(pu[0] & nm[it].x), (int)(pu[1] & nm[it].y), (int)(pu[2] & nm[it].z),
+                                           (int)(pu[3] & nm[it].w)};
+                        cacc[it][u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bx, cacc[it][u], 0, 0, 0);
+                    }
+                }
+            }
+        }
         __builtin_amdgcn_wave_barrier();
+    }
+    if constexpr (CS) {
+        // lane (g, col = 3 k + d), register e of spin tile it: spin 16 it + 4 g + e
+        const double osc = ldexp(1.0, G - WBITS);
+        const int d = lo % 3, k = lo / 3;
+#pragma unroll
+        for (int it = 0; it < 2; it++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                double h = 0.0;
+#pragma unroll
+                for (int u = QW - 1; u >= 0; u--) h = fma(h, 256.0, fma(-2.0, (double)cacc[it][u][e], (double)ctacc[u][0]));
+                h = ldexp(h, 8 * d);
+                const double h1 = __shfl(h, (lane & ~15) | ((lo + 1) & 15)), h2 = __shfl(h, (lane & ~15) | ((lo + 2) & 15));
+                const int i = 16 * it + 4 * g4 + e;
+                if (d == 0 && k < K && i < 32) ared[wv][k * 32 + i] = ((h + h1) + h2) * osc;
+            }
+        __syncthreads();
+        for (int p = threadIdx.x; p < K * N; p += blockDim.x) {
+            const int kk = p / N, i = p % N;
+            double sum = 0.0;
+            for (int q = 0; q < (int)(blockDim.x >> 6); q++) sum += ared[q][kk * 32 + i];
+            ((double *)a.part)[(long long)blockIdx.x * (a.P + 1) + (long long)kk * a.L + i] = sum;
+        }
     }
 }
 
@@ -455,8 +586,8 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
         gblk = (const int *)((const uint8_t *)gblk + set * q8ss);
         pd = (const uint4 *)((const uint8_t *)pd + set * q8ss);
     }
-    const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
-    const int NCT = UC / 16;
+    const int N = a.N, M = a.M, K = a.K;
+    const int NCT = a.qcols / 16;  // digit column tiles (the c columns apart: their a block comes from k_q8_omega)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
     // 1-D grid of nb ny blocks: the ny column-tile blocks of one row range are adjacent
@@ -601,8 +732,13 @@ hipError_t launch_omega(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, int
 {
     int *gblk = (int *)a.q8w;
     uint4 *pd = (uint4 *)(gblk + sh.gblk_ints);
-    hipLaunchKernelGGL((k_q8_omega<WSEL>), dim3(sh.bwd_blocks, 1, nsets), dim3(256), 0, st, a, gblk, pd, wss, q8ss,
-                       pss);
+    const long long xss = (long long)(sh.xc_bytes / sizeof(float));
+    if (a.qcols < a.Ucols)
+        hipLaunchKernelGGL((k_q8_omega<WSEL, true>), dim3(sh.bwd_blocks, 1, nsets), dim3(256), 0, st, a, gblk, pd, wss,
+                           q8ss, pss, xss);
+    else
+        hipLaunchKernelGGL((k_q8_omega<WSEL, false>), dim3(sh.bwd_blocks, 1, nsets), dim3(256), 0, st, a, gblk, pd,
+                           wss, q8ss, pss, xss);
     return hipGetLastError();
 }
 
@@ -615,7 +751,7 @@ hipError_t launch_bwd_pass(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, 
     uint4 *pd = (uint4 *)(gblk + sh.gblk_ints);
     const uint4 *msk = (const uint4 *)a.q8m;
     const dim3 bb(256);
-    const int nit = nit_of(a.N), nct = a.Ucols / 16;
+    const int nit = nit_of(a.N), nct = a.qcols / 16;
     const unsigned nb = (unsigned)sh.bwd_blocks, zn = (unsigned)nsets;
     if (nit == 1) {
         const int ny = (nct + 3) / 4;
@@ -655,7 +791,7 @@ hipError_t launch_bwd(const SvfArgs &a, const Q8Shape &sh, hipStream_t st)
 Q8Shape nq_q8_shape(int N, int M, int K, long long R, bool cplx)
 {
     Q8Shape sh{};
-    if (N > 64) return sh;
+    if (N > 64 || K > 4 || R >= (1LL << 31)) return sh;  // k_q8_omega takes 32-bit row indices, K <= 4
     const int UC = 16 * ((K * M + K + 15) / 16);
     const int nct = UC / 16;
     sh.nsp = N <= 32 ? 32 : 64;
@@ -687,14 +823,26 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R, bool cplx)
     if (e && *e && atoi(e) > 0) sh.fwd_blocks = atoi(e);
     sh.rpad = n64 * 64;
     sh.vq_bytes = (size_t)nct * QV * 64 * 16 + 2 * (size_t)UC * sizeof(double);
-    // GEMM forward pass: N <= 32, at most 4 groups of 64 columns (their v digits in
-    // 52 KB of LDS); a row-major copy of the factor digits (3 R Ucols bytes)
-    sh.ncg64 = (UC + 63) / 64;
+    // c columns apart (GEMM form, K in {1, 2, 4}): when the K c columns would open a column
+    // tile of their own (K M a multiple of 16), the digits cover the K M tanh columns only
+    // and the c columns stay integers (xc, rpad K floats per digit set), their products in
+    // FP64 (forward: in k_q8_fwdg before the spin signs; backward: the a block in
+    // k_q8_omega).  NQES_Q8_CSPLIT=0 keeps them as digit columns.
+    const int QC = 16 * ((K * M + 15) / 16);
+    const char *es = getenv("NQES_Q8_CSPLIT");
+    const bool csplit = N <= 32 && (K == 1 || K == 2 || K == 4) && QC < UC && !(es && strcmp(es, "0") == 0);
+    // GEMM forward pass: N <= 32, at most 4 groups of 64 digit columns (their v digits in
+    // 52 KB of LDS); a row-major copy of the factor digits (3 R qcols bytes)
+    sh.ncg64 = ((csplit ? QC : UC) + 63) / 64;
     sh.nst = (N + 15) / 16;
-    sh.qg_bytes = (size_t)sh.ncg64 * (sh.nst * QV + 1) * 64 * 16 + 16;
     const char *eg = getenv("NQES_Q8_FWD");  // "tiles": the tiled forward pass instead
     sh.gemm = N <= 32 && sh.ncg64 <= 4 && !(eg && strcmp(eg, "tiles") == 0);
-    sh.uf_bytes = sh.gemm ? (size_t)(n64 * 4) * QU * UC * 16 : 0;
+    sh.qcols = sh.gemm && csplit ? QC : UC;
+    sh.xc_bytes = sh.qcols < UC ? (size_t)n64 * 64 * K * sizeof(float) : 0;
+    if (!sh.gemm) sh.ncg64 = (UC + 63) / 64;
+    // v digits, their scale and (c columns apart) the K x 32 FP64 B values of the c columns
+    sh.qg_bytes = (size_t)sh.ncg64 * (sh.nst * QV + 1) * 64 * 16 + 16 + (sh.xc_bytes ? (size_t)K * 32 * 8 : 0);
+    sh.uf_bytes = sh.gemm ? (size_t)(n64 * 4) * QU * sh.qcols * 16 : 0;
     if (sh.gemm && sh.qg_bytes > sh.vq_bytes) sh.vq_bytes = sh.qg_bytes;
     if (sh.gemm) {
         sh.fwd_blocks = sh.nst * sh.ncg64 > 6 ? 512 : 768;  // 2 or 3 wavefronts per SIMD, as k_q8_fwdg's bounds
@@ -727,6 +875,8 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R, bool cplx)
         if (sh.bwd_blocks < sh.fz_blocks) sh.bwd_blocks = sh.fz_blocks;  // partial rows allocated for the larger
         sh.gemm = 0;
         sh.uf_bytes = 0;
+        sh.qcols = UC;
+        sh.xc_bytes = 0;
         sh.ug_bytes = (size_t)sh.rpad * QU * 64 * fz_ncg;
         // forward B operands and the scale of v
         const size_t qf = (size_t)fz_ncg * (fz_nst * QV + 1) * 64 * 16 + 16;
@@ -751,7 +901,7 @@ hipError_t q8_pack_rows(const SvfArgs &a, const Q8Shape &sh, bool masks, hipStre
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL((k_q8_pack<CM>), dim3((unsigned)blocks), dim3(256), 0, st, (const float *)a.U, R, ntile,
                        a.Ucols, (uint32_t *)a.Uq, (double *)a.escale, sh.gemm ? (uint4 *)a.Uf : (uint4 *)nullptr,
-                       a.Umax);
+                       a.Umax, sh.qcols, sh.qcols < a.Ucols ? a.xc : (float *)nullptr, a.K * a.M, a.K);
     if (masks) {
         const int NI = 16 * nit_of(a.N);
         const long long n64 = sh.rpad / 64, nthr = n64 * NI * 4;
@@ -859,6 +1009,7 @@ hipError_t nq_q8c_pack(const SvfArgs &a0, const Q8Shape &sh, hipStream_t st)
     for (int t = 0; t < 3 && e == hipSuccess; t++) {
         a.Uq = (const uint8_t *)a0.Uq + t * qs;
         a.Uf = sh.gemm ? (const void *)((const uint8_t *)a0.Uf + t * fs) : nullptr;
+        a.xc = sh.xc_bytes ? a0.xc + t * (sh.xc_bytes / sizeof(float)) : nullptr;
         if (t == 0) e = q8_pack_rows<3>(a, sh, true, st);
         else if (t == 1) e = q8_pack_rows<1>(a, sh, false, st);
         else e = q8_pack_rows<2>(a, sh, false, st);
@@ -890,7 +1041,9 @@ hipError_t nq_q8c_apply(const SvfArgs &a0, const Q8Shape &sh, const double *vin,
         a.Uf = a0.Uf;
         a.vq = a0.vq;
         a.w = w3;
-        if (e == hipSuccess) e = nq_q8_fwd_gemm(a, sh, st, false, false, 3, (long long)fs, (long long)sh.vq_bytes, S);
+        if (e == hipSuccess)
+            e = nq_q8_fwd_gemm(a, sh, st, false, false, 3, (long long)fs, (long long)sh.vq_bytes, S,
+                               (long long)(sh.xc_bytes / sizeof(float)));
     } else {
         for (int t = 0; t < 3 && e == hipSuccess; t++) {
             a.Uq = (const uint8_t *)a0.Uq + t * qs;

@@ -2,8 +2,10 @@
 // over the columns: P_ri = sum_c X_rc Q_ci on v_mfma_i32_16x16x64_i8, then the signs of
 // the spins and the digit weights per row.  The factor digits come in a row-major copy
 // (Uf, written by the pack kernel of sv_q8.hip next to its tiles): per 16-row tile and
-// digit plane t, 16 Ucols bytes, chunk 16 ct + l (l < 16) = the 16 digits of row l of
-// the tile at columns 16 ct .. 16 ct + 15.  Compiled with MFMA results in VGPRs (the
+// digit plane t, 16 qcols bytes, chunk 16 ct + l (l < 16) = the 16 digits of row l of
+// the tile at columns 16 ct .. 16 ct + 15.  When the K c columns would open a column
+// tile of their own (K M a multiple of 16, K in {1, 2, 4}), the digits cover the K M
+// tanh columns only and the c columns' integers (xc) enter in FP64.  Compiled with MFMA results in VGPRs (the
 // sums are read by VALU code once per tile).  Reference: the factored S*v of
 // rbm.cc:661-684, as in sv_q8.hip.
 #include <hip/hip_runtime.h>
@@ -42,7 +44,7 @@ __device__ inline double dpp_sum16x4(const double (&h)[4], int l)
     return b;
 }
 
-// ---- GEMM forward pass (N <= 32, Ucols <= 256).
+// ---- GEMM forward pass (N <= 32, at most 256 digit columns).
 //   w_r = (1 / f_r) 2^(g - 46) sum_c X_rc (sum_i sigma_ri Q_ci + bq_c)
 // with one scale 2^g > max |v| for all of v and its bias (absolute error 2^-47 of the
 // largest entry, the size of FP64 rounding in a dot product).  The sum over columns
@@ -52,13 +54,15 @@ __device__ inline double dpp_sum16x4(const double (&h)[4], int l)
 // pairs cost 3 MFMAs.  Operands: A lane l, byte e = digit of row 16 rt + l%16 at column
 // 64 cg + 16 (l/16) + e (the Uf chunk 64 cg + l); B lane l, byte e = digit of the same
 // column for spin 16 st + l%16 (slot st QV + u), or digit l%16 of bq (slot NST QV).
-// Qg: [cg][slot][lane] uint4, then the scale 2^(g - 46) as a double.  blockIdx.z: the
-// set (v + z vss doubles, qg + z qss uint4) when several vectors are prepared at once.
+// Qg: [cg][slot][lane] uint4, then the scale 2^(g - 46) as a double; with the c columns
+// apart (a.qcols < a.Ucols) then qa[k 32 + i] = v_a[k][i] / 2^(g - 46) (FP64, unquantised;
+// 0 for i >= N), the c columns' B values, 16 bytes after the scale.  blockIdx.z: the set
+// (v + z vss doubles, qg + z qss uint4) when several vectors are prepared at once.
 __global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst, uint4 *qg, long long vss,
                                                     long long qss)
 {
     __shared__ double red[4];
-    const int N = a.N, M = a.M, K = a.K, UC = a.Ucols;
+    const int N = a.N, M = a.M, K = a.K, QC = a.qcols;
     const long long KM = (long long)K * M;
     const double *v = (const double *)a.v + blockIdx.z * vss;
     qg += blockIdx.z * qss;
@@ -102,7 +106,7 @@ __global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst,
     for (int e4 = 0; e4 < 4; e4++) {
         const int col = 64 * cg + 16 * (lane >> 4) + 4 * w + e4;
         double val = 0.0;
-        if (col < UC) {
+        if (col < QC) {
             if (st < nst) val = 16 * st + lo < N ? Bval(col, 16 * st + lo) : 0.0;
             else val = bias(col);
         }
@@ -129,6 +133,10 @@ __global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst,
     }
     if (blockIdx.x == 0 && threadIdx.x == 0)
         *(double *)(qg + (size_t)ncg * nslot * 64) = mx > 0.0 ? ldexp(1.0, gex - VBITS) : 0.0;
+    if (blockIdx.x == 0 && QC < a.Ucols && (int)threadIdx.x < 32 * K) {
+        const int k = threadIdx.x >> 5, i = threadIdx.x & 31;
+        ((double *)(qg + (size_t)ncg * nslot * 64 + 1))[threadIdx.x] = i < N ? v[(long long)k * a.L + i] * sc : 0.0;
+    }
 }
 
 // GEMM forward pass.  Wavefronts loop over 16-row tiles (NCG (NST QV + 1) QU MFMAs
@@ -139,29 +147,38 @@ __global__ __launch_bounds__(256) void k_q8_prep_vg(SvfArgs a, int ncg, int nst,
 // KW = K in 1, 2, 4: the sample weights w_s written directly (no combine step); KW = 0:
 // the row sums wrow[r]
 // blockIdx.z: the digit set (a.Uf + z fss bytes, a.vq + z vss bytes, a.w + z wss
-// doubles) when several sets run in one launch (KW != 0 only: the row sums of KW = 0
-// share one buffer)
-template <int NST, int NCG, int KW>
+// doubles, a.xc + z xss floats) when several sets run in one launch (KW != 0 only: the
+// row sums of KW = 0 share one buffer)
+// CS (KW != 0): the c columns apart, a.qcols = K M digit columns; each lane adds
+// sigma_ri sum_k X_c[r][k] qa[k][i] (integers times the unquantised v_a in units of the v
+// scale, FP64) to its rows after the last spin tile's sums.  A tile's 16 KW X_c values
+// load into one register per lane with the factor digits and pass through the
+// wavefront's 256 bytes of LDS (registers across the MFMA phases would spill).
+template <int NST, int NCG, int KW, bool CS>
 __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs a, long long fss, long long vss,
-                                                                         long long wss)
+                                                                         long long wss, long long xss)
 {
     a.Uf = (const uint8_t *)a.Uf + blockIdx.z * fss;
     a.vq = (uint8_t *)a.vq + blockIdx.z * vss;
     a.w = (double *)a.w + blockIdx.z * wss;
+    a.xc = a.xc + blockIdx.z * xss;
     constexpr int NSF = QU + QV - 1;
     constexpr int NSLOT = NST * QV + 1;
     constexpr int NQ = NCG * NSLOT * 64;
+    constexpr int NQL = CS ? NQ + 1 + 16 * KW : NQ;  // uint4 in LDS: digits (, scale, qa)
     extern __shared__ __align__(16) unsigned char smem_g[];
     uint4 *qs = (uint4 *)smem_g;
     {
         const uint4 *src = (const uint4 *)a.vq;
-        for (int q = threadIdx.x; q < NQ; q += blockDim.x) qs[q] = src[q];
+        for (int q = threadIdx.x; q < NQL; q += blockDim.x) qs[q] = src[q];
     }
     const double vsc = *(const double *)((const uint4 *)a.vq + NQ);
     __syncthreads();
+    const double *qa = (const double *)(qs + NQ + 1);
+    __shared__ __align__(16) float xlds[4][64];  // per wavefront: a tile's X_c, row-major [16][KW]
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int g = lane >> 4, lo = lane & 15;
-    const int NW = a.NW, NCT = a.Ucols / 16;
+    const int NW = a.NW, NCT = a.qcols / 16;
     const long long R = a.S * a.K;
     const long long ntile = (R + 15) / 16;
     const uint4 *Uf = (const uint4 *)a.Uf;
@@ -192,6 +209,9 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
             word[e] = a.spins[rc * NW];
             es[e] = a.escale[rc];
         }
+        // the tile's X_c (rows up to the padded count hold zeros past R); lanes >= 16 KW
+        // read a duplicate
+        const float xl = CS ? a.xc[rt * 16 * KW + (lane & (16 * KW - 1))] : 0.f;
         // an opaque lane offset keeps the v operand reads inside the loop (hoisted, they
         // would hold NCG NSLOT 4 registers)
         int ql = lane;
@@ -213,6 +233,10 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
             __builtin_memcpy(&bb, &b4, 16);
 #pragma unroll
             for (int t = 0; t < QU; t++) accb[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(at[t], bb, accb[t], 0, 0, 0);
+        }
+        if constexpr (CS) {
+            if (lane < 16 * KW) xlds[wv][lane] = xl;
+            __builtin_amdgcn_wave_barrier();
         }
         v4i accp[NSF];  // sums of the previous spin tile
 #pragma unroll
@@ -261,6 +285,34 @@ __global__ __launch_bounds__(256, NST * NCG > 6 ? 2 : 3) void k_q8_fwdg(SvfArgs 
                 for (int s = 0; s < NSF; s++) accp[s] = acc[s];
             }
         }
+        if constexpr (CS) {
+            // the c columns: row 4 g + e, column k at xlds[wv][(4 g + e) KW + k]
+            float xr[4 * KW];
+            const float4 *xp = (const float4 *)(xlds[wv] + 4 * g * KW);
+#pragma unroll
+            for (int q = 0; q < KW; q++) {
+                const float4 x4 = xp[q];
+                xr[4 * q] = x4.x;
+                xr[4 * q + 1] = x4.y;
+                xr[4 * q + 2] = x4.z;
+                xr[4 * q + 3] = x4.w;
+            }
+#pragma unroll
+            for (int st = 0; st < NST; st++) {
+                double qk[KW];
+#pragma unroll
+                for (int k = 0; k < KW; k++) qk[k] = qa[k * 32 + 16 * st + (ql & 15)];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    double c = 0.0;
+#pragma unroll
+                    for (int k = 0; k < KW; k++) c = fma((double)xr[e * KW + k], qk[k], c);
+                    const bool dn = (word[e] >> (16 * st + lo)) & 1u;
+                    h[e] += dn ? -c : c;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
         {
             const double hr = dpp_sum16x4(h, lo);  // row e = lo / 4
             const int e = lo >> 2;
@@ -296,21 +348,28 @@ hipError_t nq_q8_prep_vg_sets(const SvfArgs &a, const Q8Shape &sh, int nsets, lo
 }
 
 hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, bool rows, bool prep, int nsets,
-                          long long fss, long long vss, long long wss)
+                          long long fss, long long vss, long long wss, long long xss)
 {
+    const bool cs = a.qcols < a.Ucols;
     if (nsets > 1 && (rows || !(a.K == 1 || a.K == 2 || a.K == 4))) return hipErrorInvalidValue;
+    if (cs && (rows || !(a.K == 1 || a.K == 2 || a.K == 4) || !a.xc)) return hipErrorInvalidValue;
     if (prep)
         hipLaunchKernelGGL(k_q8_prep_vg, dim3(sh.ncg64 * (sh.nst + 1)), dim3(256), 0, st, a, sh.ncg64, sh.nst,
                            (uint4 *)a.vq, 0LL, 0LL);
-    const size_t lds = sh.qg_bytes - 16;
+    const size_t lds = cs ? sh.qg_bytes : sh.qg_bytes - 16;
     const dim3 grid(sh.fwd_blocks, 1, nsets), blk(256);
-#define NQ_FWDG(NST, NCG)                                                                                         \
-    do {                                                                                                         \
-        if (rows) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0>), grid, blk, lds, st, a, fss, vss, wss);           \
-        else if (a.K == 1) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 1>), grid, blk, lds, st, a, fss, vss, wss);  \
-        else if (a.K == 2) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 2>), grid, blk, lds, st, a, fss, vss, wss);  \
-        else if (a.K == 4) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 4>), grid, blk, lds, st, a, fss, vss, wss);  \
-        else hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0>), grid, blk, lds, st, a, fss, vss, wss);                \
+#define NQ_FWDK(NST, NCG, KW)                                                                                       \
+    do {                                                                                                           \
+        if (cs) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, KW, true>), grid, blk, lds, st, a, fss, vss, wss, xss);   \
+        else hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, KW, false>), grid, blk, lds, st, a, fss, vss, wss, xss);     \
+    } while (0)
+#define NQ_FWDG(NST, NCG)                                                                                              \
+    do {                                                                                                              \
+        if (rows) hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0, false>), grid, blk, lds, st, a, fss, vss, wss, xss);    \
+        else if (a.K == 1) NQ_FWDK(NST, NCG, 1);                                                                      \
+        else if (a.K == 2) NQ_FWDK(NST, NCG, 2);                                                                      \
+        else if (a.K == 4) NQ_FWDK(NST, NCG, 4);                                                                      \
+        else hipLaunchKernelGGL((k_q8_fwdg<NST, NCG, 0, false>), grid, blk, lds, st, a, fss, vss, wss, xss);         \
     } while (0)
     switch (sh.nst * 8 + sh.ncg64) {
     case 9: NQ_FWDG(1, 1); break;
@@ -323,5 +382,6 @@ hipError_t nq_q8_fwd_gemm(const SvfArgs &a, const Q8Shape &sh, hipStream_t st, b
     default: NQ_FWDG(2, 4); break;
     }
 #undef NQ_FWDG
+#undef NQ_FWDK
     return hipGetLastError();
 }

@@ -305,6 +305,44 @@ def test_q8_forward_passes_agree(nqes, monkeypatch, N, M, K, C, m, cplx):
 
 
 @pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("N,M,K,C,m", [(8, 16, 1, 37, 5), (12, 24, 2, 45, 7), (32, 64, 2, 96, 9), (24, 24, 4, 50, 4),
+                                        (16, 64, 4, 20, 3)])
+def test_q8_c_columns_apart_agree(nqes, monkeypatch, N, M, K, C, m, cplx):
+    """Two-pass GEMM form with K M a multiple of 16 (K in {1, 2, 4}): the K c columns of
+    each factor row stay quantised integers outside the digit planes, their forward
+    products and the a block of the backward pass in FP64 with the unquantised v and
+    row weights; NQES_Q8_CSPLIT=0 keeps them as an extra digit column tile.  The same
+    factor integers either way, v and the weights quantised to 2^-47 and 2^-41 of their
+    scale in the digit form, so S*v agrees to ~1e-13 relative (bound 1e-11), and each
+    form stays linear and symmetric (Hermitian) to 1e-11.  K M = 256 (N = 16, M = 64,
+    K = 4) takes the GEMM forward pass only with the c columns apart (four 64-column
+    groups); the other way it runs the tiled forward pass."""
+    print(f"case N={N} M={M} K={K} C={C} m={m} complex={cplx}", flush=True)
+    J = load_J(N)
+    W = random_weights(N, M, K, seed=31 + N + K, scale=0.1, complex_=cplx)
+    P = K * (N + M + N * M)
+    rng = np.random.default_rng(29)
+    v1 = rng.normal(size=P) + (1j * rng.normal(size=P) if cplx else 0j)
+    v2 = v1.copy()
+    v2[: P // 3] *= 1e-3
+    out = {}
+    monkeypatch.delenv("NQES_SV", raising=False)
+    monkeypatch.setenv("NQES_Q8_FUSED", "0")
+    for mode in ("1", "0"):
+        monkeypatch.setenv("NQES_Q8_CSPLIT", mode)
+        with nqes.Solver(N, M, K, 0.595, J, precision=nqes.PREC_MIXED) as s:
+            s.sample(W, 83, C, m, 3, 0.5)
+            out[mode] = [s.apply_S(v) for v in (v1, v2, v1 + 2.5 * v2)]
+    for a, b in zip(out["1"], out["0"]):
+        assert rel_err(a, b) < 1e-11, rel_err(a, b)
+    for mode in ("1", "0"):
+        a1, a2, a12 = out[mode]
+        assert rel_err(a12, a1 + 2.5 * a2) < 1e-11, (mode, rel_err(a12, a1 + 2.5 * a2))
+        sym = abs(np.vdot(v2, a1) - np.conj(np.vdot(v1, a2))) / (np.linalg.norm(v1) * np.linalg.norm(a2))
+        assert sym < 1e-11, (mode, sym)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
 def test_q8_graph_replays_equal_eager(nqes, monkeypatch, cplx):
     """The int8 S*v runs as a captured HIP graph (one per sample set; for complex
     weights the two forward and two backward passes over the split rows): every replay,
