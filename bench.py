@@ -383,16 +383,24 @@ def sv_roofline(args, stats, real):
         R = S * args.K
         ucols = 16 * ((args.K * args.M + args.K + 15) // 16)
         nw = (args.N + 31) // 32
-        sv_bytes = sets * (2 * R * ucols * 3 + 2 * R * 8 + R * 4 * nw + 2 * S * 8)
         nst = (args.N + 15) // 16
         nsp = 32 if args.N <= 32 else 64
-        ncg64 = (ucols + 63) // 64
+        # the K c columns kept apart (GEMM forward pass, K in {1, 2, 4}, K M a multiple of 16):
+        # K M digit columns, the c columns' integers (4 B each) read by the forward pass and
+        # by the omega kernel, which forms their a block on 6 (1 + nst) MFMAs per 64 rows
+        qc = 16 * ((args.K * args.M + 15) // 16)
+        csplit = (args.N <= 32 and args.K in (1, 2, 4) and qc < ucols and (qc + 63) // 64 <= 4
+                  and os.environ.get("NQES_Q8_CSPLIT") != "0" and os.environ.get("NQES_Q8_FWD") != "tiles")
+        qcols = qc if csplit else ucols
+        ncg64 = (qcols + 63) // 64
         gemm = args.N <= 32 and ncg64 <= 4 and os.environ.get("NQES_Q8_FWD") != "tiles"
+        sv_bytes = sets * (2 * R * qcols * 3 + 2 * R * 8 + R * 4 * nw + 2 * S * 8 + (2 * R * args.K * 4 if csplit else 0))
         # 15 of the 18 digit pairs (the pairs of weight 256^0 and 256^1 are left out)
-        bwd_ops = sets * 2 * R * ucols * 15 * 16 * nst
+        bwd_ops = sets * 2 * R * qcols * 15 * 16 * nst + (sets * 2 * R * 256 * 6 * (1 + nst) if csplit else 0)
         fwd_ops = sets * (R * ncg64 * (15 * nst + 3) * 2 * 16 * 64 if gemm else 2 * R * ucols * 6 * nsp)
         kernel = (f"S*v: {'k_q8_fwdg' if gemm else 'k_q8_fwd'} + k_q8_bwd (int8 MFMA over quantised c*tanh(theta) "
-                  f"factors{'' if real else '; complex: 3 digit sets, 3 passes each way'})")
+                  f"factors{'' if real else '; complex: 3 digit sets, 3 passes each way'}"
+                  f"{'; the K c columns apart (FP64 forward terms, a block in k_q8_omega)' if csplit else ''})")
         achieved = sv_bytes / (sv_avg_ms * 1e-3) / 1e9 if sv_calls else 0.0
         return {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(args, real, "q8"),
