@@ -881,8 +881,9 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R, bool cplx)
         // forward B operands and the scale of v
         const size_t qf = (size_t)fz_ncg * (fz_nst * QV + 1) * 64 * 16 + 16;
         if (qf > sh.vq_bytes) sh.vq_bytes = qf;
-        // the quantised c columns in the mask buffer
-        const size_t xc = (size_t)sh.rpad * K * 4;
+        // the quantised c columns and a copy of the spin words padded to rpad rows in the
+        // mask buffer
+        const size_t xc = (size_t)sh.rpad * K * 4 + (size_t)sh.rpad * 4;
         if (xc > sh.m_bytes) sh.m_bytes = xc;
     }
     sh.ok = 1;

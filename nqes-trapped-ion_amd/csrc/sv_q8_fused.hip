@@ -63,6 +63,10 @@ namespace {
 
 constexpr int FZ_RED = 128;  // doubles of the end-of-block reductions per wavefront (NST K 16)
 
+// wavefronts per block: eight (128-row super-chunks, one column tile per wavefront in the
+// backward products) when the column tiles split evenly, else four
+__host__ __device__ constexpr int fz_waves(int ncg) { return (ncg % 2 == 0) ? 8 : 4; }
+
 // four 0/1 bits -> four 0/1 bytes
 __device__ inline uint32_t fz_expand4(uint32_t b) { return ((b & 15u) * 0x00204081u) & 0x01010101u; }
 
@@ -114,6 +118,7 @@ struct FzArgs {
     int N, M, K, L, KM;
     long long S, R, P, n64;
     const uint32_t *spins;  // one word per row (N <= 32)
+    const uint32_t *spad;   // the same, rpad rows (zeros past R), written by the pack
     const uint4 *Ug;
     const float *Xc;        // [rpad][K] quantised c columns
     const double *escale;   // [rpad] 1 / f_r
@@ -130,8 +135,11 @@ struct FzArgs {
 // 2^e_r > Umax[r], as the two-pass pack.
 template <int CM>
 __global__ __launch_bounds__(256) void k_q8_packg(const float *U, long long R, long long ntile, int UC, int KM, int K,
-                                                  int NCG, uint4 *Ug, float *Xc, double *escale, const float *Umax)
+                                                  int NCG, uint4 *Ug, float *Xc, double *escale, const float *Umax,
+                                                  const uint32_t *spins, uint32_t *spad)
 {
+    const int W = fz_waves(NCG), NKC = W / 4;
+    const long long XT_U4 = 4LL * NCG * QU * NKC * 64;  // uint4 of one super-chunk's digit image
     extern __shared__ float lds_pack[];
     constexpr int FW = CM ? 2 : 1;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -141,6 +149,7 @@ __global__ __launch_bounds__(256) void k_q8_packg(const float *U, long long R, l
     for (long long rt = (long long)blockIdx.x * 4 + wv; rt < ntile; rt += (long long)gridDim.x * 4) {
         const long long r = rt * 16 + lo;
         const float mx = r < R ? Umax[r] : 0.f;
+        if (g == 0) spad[r] = r < R ? spins[r] : 0u;
         int e = 0;
         if (mx > 0.f) (void)frexpf(mx, &e);  // mx < 2^e
         const double sc = ldexp(CM ? XMAX - 1.0 : XMAX, -e);
@@ -161,6 +170,10 @@ __global__ __launch_bounds__(256) void k_q8_packg(const float *U, long long R, l
                 return CM == 1 ? xr : (CM == 2 ? xi : xr + xi);
             }
         };
+        // lane (g, lo): row lo of the tile, the 16 columns of column tile 4 cg + g, into the
+        // super-chunk's digit image at [column tile][t][chunk][row 16 q + lo]
+        const long long sck = rt / W;  // the super-chunk
+        const int wq = (int)(rt % W), kc = wq >> 2, q = wq & 3;
         for (int cg = 0; cg < NCG; cg++) {
             uint32_t pl[QU][4];
 #pragma unroll
@@ -171,7 +184,7 @@ __global__ __launch_bounds__(256) void k_q8_packg(const float *U, long long R, l
             for (int b = 0; b < 4; b++)
 #pragma unroll
                 for (int d = 0; d < 4; d++) {
-                    const int col = 64 * cg + 16 * b + 4 * g + d;
+                    const int col = 64 * cg + 16 * g + 4 * b + d;
                     long long x = col < KM ? xval(col) : 0;
 #pragma unroll
                     for (int t = 0; t < QU; t++) {
@@ -180,20 +193,22 @@ __global__ __launch_bounds__(256) void k_q8_packg(const float *U, long long R, l
                         x = (x - dg) >> 8;
                     }
                 }
+            const int ct = 4 * cg + g;
 #pragma unroll
             for (int t = 0; t < QU; t++)
-                Ug[((rt * QU + t) * NCG + cg) * 64 + lane] = uint4{pl[t][0], pl[t][1], pl[t][2], pl[t][3]};
+                Ug[sck * XT_U4 + ((long long)(ct * QU + t) * NKC + kc) * 64 + 16 * q + lo] =
+                    uint4{pl[t][0], pl[t][1], pl[t][2], pl[t][3]};
         }
         if (g == 0)
             for (int k = 0; k < K; k++) Xc[r * K + k] = (float)xval(KM + k);
     }
 }
 
-// ---- v -> forward B operands in Ug's column order, formed in LDS by every block of the
-// one-pass kernel (34 KB of v read from L2 per block; a separate kernel and its launch
-// cost more).  qs[(cg NSLOT + slot) 64 + lane]: lane (g, lo) byte 4 b + d = digit u of v
-// at column 64 cg + 16 b + 4 g + d for spin 16 st + lo (slot st QV + u), or digit lo of
-// the bias of that column (slot NST QV).  One scale 2^gex > max |v| for all of v; returns
+// ---- v -> forward B operands, formed in LDS by every block of the one-pass kernel (34 KB
+// of v read from L2 per block; a separate kernel and its launch cost more).
+// qs[(cg NSLOT + slot) 64 + lane]: lane (g, lo) byte e = digit u of v at column
+// 64 cg + 16 g + e (the A operands' order: lane group g holds column tile 4 cg + g) for
+// spin 16 st + lo (slot st QV + u), or digit lo of the bias of that column (slot NST QV).  One scale 2^gex > max |v| for all of v; returns
 // 2^(gex - 46).  red: one double per wavefront of LDS, free until the block's next barrier.
 template <int NCG, int NST>
 __device__ inline double fz_v_operands(const FzArgs &a, uint4 *qs, double *red)
@@ -235,7 +250,7 @@ __device__ inline double fz_v_operands(const FzArgs &a, uint4 *qs, double *red)
         uint32_t dg[QV] = {};
 #pragma unroll
         for (int d = 0; d < 4; d++) {
-            const int col = 64 * cg + 16 * b + 4 * g + d;
+            const int col = 64 * cg + 16 * g + 4 * b + d;
             double val = 0.0;
             if (col < KM) {
                 const long long base = (long long)(col / M) * a.L;
@@ -270,20 +285,25 @@ __device__ inline double fz_v_operands(const FzArgs &a, uint4 *qs, double *red)
     return mx > 0.0 ? ldexp(1.0, gex - VBITS) : 0.0;
 }
 
-template <int NCG, int NST, int W>
+// One buffer (two, by super-chunk parity) of the one-pass kernel's LDS, in dwords: the
+// super-chunk's digit image XT (copied from Ug as it is), the weight digits, the spin
+// masks, the wavefronts' largest |omega|, and the super-chunk's rows' scales (ES, FP64),
+// c-column integers (XC, KS per row) and spin words (SP), also copied from global memory.
+template <int NCG, int NST, int W, int KS>
 struct FzLds {
     static constexpr int NCT = 4 * NCG;
     static constexpr int XT_DW = NCT * QU * (W / 4) * 256;  // [column tile][t][chunk] 64 x 16 bytes
     static constexpr int OD_DW = 8 * 4 * W;  // [chunk][g'][u, 8 slots][row tile of the chunk]
     static constexpr int MK_DW = NST * 16 * 4 * W;  // [st][g'][chunk][i][row tile of the chunk]
-    static constexpr int BUF_DW = XT_DW + OD_DW + MK_DW + 2 * W;
+    static constexpr int ES_OFF = XT_DW + OD_DW + MK_DW + 2 * W;  // 16-byte aligned
+    static constexpr int XC_OFF = ES_OFF + 32 * W;
+    static constexpr int SP_OFF = XC_OFF + 16 * W * KS;
+    static constexpr int BUF_DW = SP_OFF + 16 * W;
     static constexpr int NSLOT = NST * QV + 1;
     static constexpr int NQ = NCG * NSLOT * 64;  // uint4 of the forward B operands
+    static constexpr int XT_U4 = XT_DW / 4;      // uint4 of one super-chunk's digit image
 };
 
-// wavefronts per block: eight (128-row super-chunks, one column tile per wavefront in the
-// backward products) when the column tiles split evenly, else four
-__host__ __device__ constexpr int fz_waves(int ncg) { return (ncg % 2 == 0) ? 8 : 4; }
 
 // ---- the one-pass S*v (MODE 0: forward and backward; MODE 1: backward with the
 // moment weights 1 or Re E_s).  A block of W wavefronts takes super-chunks of 16 W rows
@@ -298,13 +318,15 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
     constexpr int W = fz_waves(NCG);
     constexpr int CTW = 4 * NCG / W;
     constexpr int NKC = W / 4;  // chunks of 64 rows per super-chunk
-    using L = FzLds<NCG, NST, W>;
+    using L = FzLds<NCG, NST, W, KS>;
     constexpr int NSF = QU + QV - 1 - QSKIP;  // digit weights 256^s, s = QSKIP .. QU + QV - 2
-    extern __shared__ __align__(16) unsigned char smem_fz[];
-    uint4 *qs = (uint4 *)smem_fz;
-    uint32_t *bufs = (uint32_t *)(smem_fz + (MODE == 0 ? (size_t)L::NQ * 16 : 0));
+    // LDS: the forward B operands and the two buffers, as distinct arrays, so that the
+    // compiler sees that the copies into one buffer do not alias the reads of the other
+    // (it would otherwise wait for the copies before every LDS read)
+    __shared__ __align__(16) uint4 qs[MODE == 0 ? L::NQ : 1];
+    __shared__ __align__(16) uint32_t fzb0[L::BUF_DW], fzb1[L::BUF_DW];
 
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = lane >> 4, lo = lane & 15;
     const int N = a.N, M = a.M, KM = a.KM;
     const long long R = a.R, P = a.P;
@@ -317,7 +339,7 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
     double vsc = 0.0, ov = 0.0;
     double qa[NST][KS];  // v_a[k][16 st + lo] / vsc: the c columns' forward term in digit units
     if constexpr (MODE == 0) {
-        vsc = fz_v_operands<NCG, NST>(a, qs, (double *)bufs);
+        vsc = fz_v_operands<NCG, NST>(a, qs, (double *)fzb0);
         ov = *a.ov;
         const double inv = vsc > 0.0 ? 1.0 / vsc : 0.0;  // a power of two
 #pragma unroll
@@ -411,34 +433,75 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
         first = false;
     };
 
-    // operands of this wavefront's row tile of super-chunk ch
+    // Super-chunk ch's digit image, row scales, c-column integers and spin words go from
+    // global memory straight into LDS buffer p (global_load_lds_dwordx4: lane l's 16 bytes
+    // land at the instruction's LDS base + 16 l), issued one super-chunk ahead; the wavefronts
+    // share the copies (the digits W ways, the per-row arrays one wavefront each).
+    auto dma = [&](long long ch, uint32_t *b) {
+        // the lane offset is formed here (an opaque copy): hoisted out of the loop, the
+        // per-lane addresses would hold registers through the whole kernel
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        const uint4 *src = a.Ug + ch * L::XT_U4;
+        for (int i = wv; i < L::XT_U4 / 64; i += W)
+            __builtin_amdgcn_global_load_lds((const void *)(src + i * 64 + ln),
+                                             (__attribute__((address_space(3))) void *)(b + i * 256), 16, 0, 0);
+        const long long r0 = ch * 16 * W;  // the super-chunk's first row
+        if (wv == W - 1 && ln < 8 * W)     // 16 W doubles
+            __builtin_amdgcn_global_load_lds((const void *)((const uint4 *)(a.escale + r0) + ln),
+                                             (__attribute__((address_space(3))) void *)(b + L::ES_OFF), 16, 0, 0);
+        if (wv == W - 2)                   // 16 W KS floats: W KS / 4 instructions of 64 lanes
+            for (int i = 0; i < W * KS / 4; i += 4) {
+                if (ln < 16 * (W * KS / 4 - i < 4 ? W * KS / 4 - i : 4))
+                    __builtin_amdgcn_global_load_lds(
+                        (const void *)((const uint4 *)(a.Xc + r0 * KS) + i * 16 + ln),
+                        (__attribute__((address_space(3))) void *)(b + L::XC_OFF + i * 64), 16, 0, 0);
+            }
+        if (wv == W - 3 && ln < 4 * W)     // 16 W words
+            __builtin_amdgcn_global_load_lds((const void *)((const uint4 *)(a.spad + r0) + ln),
+                                             (__attribute__((address_space(3))) void *)(b + L::SP_OFF), 16, 0, 0);
+    };
+    // operands of this wavefront's row tile of the super-chunk, from the LDS copies:
+    // A operands of lane (g, lo) = row lo of the tile, the 16 columns of column tile
+    // 4 cg + g (the order of the v operands)
     uint4 av[NCG][QU];
     uint32_t word[4];
     double es[4];
     float xc[4][KS];
-    auto load_tile = [&](long long ch) {
-        const long long rt = ch * W + wv;
+    auto load_tile = [&](const uint32_t *xt) {
+        int ln = lane;  // opaque, as in dma
+        asm volatile("" : "+v"(ln));
+        const int gl = ln >> 4, ll = ln & 15;
+        const uint4 *x4 = (const uint4 *)xt;
 #pragma unroll
         for (int cg = 0; cg < NCG; cg++)
 #pragma unroll
-            for (int t = 0; t < QU; t++) av[cg][t] = a.Ug[((rt * QU + t) * NCG + cg) * 64 + lane];
+            for (int t = 0; t < QU; t++)
+                av[cg][t] = x4[(((4 * cg + gl) * QU + t) * NKC + (wv >> 2)) * 64 + 16 * (wv & 3) + ll];
+        const uint32_t *sp = xt + L::SP_OFF;
+        const double *esl = (const double *)(xt + L::ES_OFF);
+        const float *xcl = (const float *)(xt + L::XC_OFF);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            const long long r = rt * 16 + 4 * g + e;
-            word[e] = r < R ? a.spins[r] : 0u;
-            es[e] = a.escale[r];  // zero on padding rows
+            const int row = 16 * wv + 4 * gl + e;
+            word[e] = sp[row];
+            es[e] = esl[row];  // zero on padding rows
 #pragma unroll
-            for (int k = 0; k < KS; k++) xc[e][k] = a.Xc[r * KS + k];
+            for (int k = 0; k < KS; k++) xc[e][k] = xcl[row * KS + k];
         }
     };
 
-    for (long long ch = c0; ch < c1; ch++) {
-        uint32_t *xt = bufs + (size_t)((ch - c0) & 1) * L::BUF_DW;
+    // super-chunk ch in buffer xt; the copies of ch + 1 go to nx
+    auto body = [&](long long ch, uint32_t *xt, uint32_t *nx) {
         uint32_t *od = xt + L::XT_DW;
         uint32_t *mk = od + L::OD_DW;
         double *wmx = (double *)(mk + L::MK_DW);
         const long long rt = ch * W + wv;
-        load_tile(ch);
+        // this super-chunk's copies are complete in every wavefront; every wavefront is past
+        // its reads of the other buffer (the previous super-chunk), which the next copy fills
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        load_tile(xt);
 
         // ---- phase A: this wavefront's 16-row tile
         const int er = lo >> 2;  // this lane's row 4 g + er
@@ -533,21 +596,6 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
                 }
         }
 
-        // this tile's digits into the chunk's column-tile matrices X_t [64 rows][16 columns]
-        // (row-major; the backward products read them back with the LDS transpose load):
-        // dword b of the lane's 16 bytes holds columns 4 g .. 4 g + 3 of column tile 4 cg + b
-        // in row 16 (wv % 4) + lo of chunk wv / 4
-        {
-            uint32_t *xw = xt + (wv >> 2) * 256 + (16 * (wv & 3) + lo) * 4 + g;
-#pragma unroll
-            for (int cg = 0; cg < NCG; cg++)
-#pragma unroll
-                for (int t = 0; t < QU; t++) {
-                    const uint32_t dw[4] = {av[cg][t].x, av[cg][t].y, av[cg][t].z, av[cg][t].w};
-#pragma unroll
-                    for (int b = 0; b < 4; b++) xw[((4 * cg + b) * QU + t) * NKC * 256] = dw[b];
-                }
-        }
         // spin masks: spin 16 st + lo, rows 4 g + j
 #pragma unroll
         for (int st = 0; st < NST; st++) {
@@ -595,6 +643,9 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
             }
         }
 
+        // the next super-chunk's copies run during phase C
+        if (ch + 1 < c1) dma(ch + 1, nx);
+
         // ---- phase C: the super-chunk's backward products for column tiles wv CTW + c
         const uint4 *od4 = (const uint4 *)od;
         const uint4 *mk4 = (const uint4 *)mk;
@@ -640,6 +691,11 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
             }
         }
         if (gvalid) dirty = true;
+    };
+    if (c0 < c1) dma(c0, fzb0);
+    for (long long ch = c0; ch < c1; ch += 2) {
+        body(ch, fzb0, fzb1);
+        if (ch + 1 < c1) body(ch + 1, fzb1, fzb0);
     }
     if (!gvalid) G = 0;
     flush();
@@ -657,7 +713,7 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) wsum += __shfl_xor(wsum, off);
     __syncthreads();  // every wavefront is past its last reads of the buffers
-    double *red = (double *)bufs;
+    double *red = (double *)fzb0;
     if (g == 0) {
 #pragma unroll
         for (int st = 0; st < NST; st++)
@@ -684,13 +740,6 @@ __global__ __launch_bounds__(64 * fz_waves(NCG), 2) void k_q8_fused(FzArgs a)
 template <int NCG, int NST, int MODE>
 const void *fz_kernel() { return (const void *)k_q8_fused<NCG, NST, NQES_FZ_K, MODE>; }
 
-size_t fz_lds_bytes(int ncg, int nst, int mode)
-{
-    const int w = fz_waves(ncg);
-    const size_t nq = mode == 0 ? (size_t)ncg * (nst * QV + 1) * 64 * 16 : 0;
-    const size_t buf = ((size_t)4 * ncg * QU * w * 64 + 8 * 4 * w + (size_t)nst * 16 * 4 * w + 2 * w) * 4;
-    return nq + 2 * buf;
-}
 
 template <int MODE>
 const void *fz_select(const Q8Shape &sh)
@@ -708,14 +757,14 @@ template <int MODE>
 hipError_t fz_allow_lds(const Q8Shape &sh)
 {
     const void *fn = fz_select<MODE>(sh);
-    return fn ? nq_allow_dynamic_lds(fn, fz_lds_bytes(sh.fz_ncg, sh.fz_nst, MODE)) : hipErrorInvalidValue;
+    return fn ? hipSuccess : hipErrorInvalidValue;  // static LDS only (142 KB at most)
 }
 
 template <int MODE>
 hipError_t fz_launch(const FzArgs &f, const Q8Shape &sh, hipStream_t st)
 {
-    const size_t lds = fz_lds_bytes(sh.fz_ncg, sh.fz_nst, MODE);
-    const hipError_t e = fz_allow_lds<MODE>(sh);  // set at pack time; a lookup here
+    const size_t lds = 0;
+    const hipError_t e = fz_allow_lds<MODE>(sh);
     if (e != hipSuccess) return e;
     const dim3 grid(sh.fz_blocks), blk(64 * fz_waves(sh.fz_ncg));
     switch (sh.fz_ncg * 8 + sh.fz_nst) {
@@ -742,6 +791,7 @@ FzArgs fz_args(const SvfArgs &a, const Q8Shape &sh)
     f.spins = a.spins;
     f.Ug = (const uint4 *)a.Uq;
     f.Xc = (const float *)a.q8m;
+    f.spad = (const uint32_t *)((const uint8_t *)a.q8m + (size_t)sh.rpad * a.K * sizeof(float));
     f.escale = a.escale;
     f.v = (const double *)a.v;
     f.ov = (const double *)a.ov;
@@ -770,7 +820,8 @@ hipError_t NQ_FZ_CAT(nq_q8f_pack_k, NQES_FZ_K)(const SvfArgs &a, const Q8Shape &
     if (e == hipSuccess) e = fz_allow_lds<1>(sh);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_q8_packg<0>), dim3((unsigned)blocks), dim3(256), lds, st, (const float *)a.U, R, ntile,
-                       a.Ucols, a.K * a.M, a.K, sh.fz_ncg, (uint4 *)a.Uq, (float *)a.q8m, (double *)a.escale, a.Umax);
+                       a.Ucols, a.K * a.M, a.K, sh.fz_ncg, (uint4 *)a.Uq, (float *)a.q8m, (double *)a.escale, a.Umax,
+                       a.spins, (uint32_t *)((uint8_t *)a.q8m + (size_t)sh.rpad * a.K * sizeof(float)));
     return hipGetLastError();
 }
 
