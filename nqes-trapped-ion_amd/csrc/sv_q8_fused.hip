@@ -5,14 +5,14 @@
 //
 // The two-pass int8 form reads the factor digits twice per call, a row-major copy in the
 // forward GEMM (sv_q8_gemm.hip) and 16x16 tiles in the backward pass (sv_q8.hip).  Here
-// a block takes 64-row chunks.  Each of its four wavefronts
+// a block takes super-chunks of 64-row chunks, copied from HBM into LDS (its digits as
+// row-major 64 x 16-byte matrices, one per column tile, digit plane and chunk) while the
+// previous super-chunk is processed.  Each of its wavefronts
 //   1. runs the forward GEMM of one 16-row tile, P_ri = sum_c X_rc Q_ci, on
-//      v_mfma_i32_16x16x64_i8 with its A operands straight from HBM (layout Ug below),
+//      v_mfma_i32_16x16x64_i8 with its A operands from those matrices (layout Ug below),
 //      and forms the rows' weights omega_r = (sum_kk w_row - o^T v) / f_r;
-//   2. stores its digits in LDS as row-major 64 x 16 byte matrices, one per column tile,
-//      digit plane and chunk (one dword store per column tile and plane), with the rows'
-//      weight digits and spin masks;
-// and after one block barrier the four wavefronts run the chunk's backward GEMM
+//   2. stores the rows' weight digits and spin masks in LDS;
+// and after one block barrier the wavefronts run the chunks' backward GEMM
 //   out[i][col] = sum_r sigma_r[i] X_r[col] Omega_r = T[col] - 2 sum_r b_r[i] X_r[col] Omega_r
 // from LDS, each for its column tiles, into int32 accumulators.  The backward B operands
 // (16 rows of one column per lane) come out of the row-major matrices through the LDS
@@ -34,18 +34,18 @@
 // chunk's exponent + 2 (a few times per block).  Both passes leave out the digit pairs
 // of weight 256^0 and 256^1 (QSKIP), as the two-pass form does.
 //
-// Layout Ug (written once per sample set by k_q8_packg): per 16-row tile rt, digit plane
-// t and 64-column group cg, 64 lanes x 16 bytes; lane (g, lo) holds row 16 rt + lo, byte
-// 4 b + d = digit t of column 64 cg + 16 b + 4 g + d.  The forward A operand of the
-// int8 MFMA is lane l's 16 bytes as they are; the v digits (fz_v_operands) use the same
-// column order (any order of the 64 columns of a group serves the forward sum, as long
-// as both operands share it).  Dword b of a lane then holds four adjacent columns
-// (4 g .. 4 g + 3) of column tile 4 cg + b: one dword of a row of that tile's matrix.
+// Layout Ug (written once per sample set by k_q8_packg): per super-chunk of 16 W rows
+// (W wavefronts per block), the block's LDS image XT of its digits:
+// [column tile][digit plane t][64-row chunk] row-major 64 x 16-byte matrices, row
+// 16 q + lo of chunk kc = row lo of the super-chunk's row tile 4 kc + q.  The kernel copies
+// a super-chunk into LDS as it is (global_load_lds_dwordx4) one super-chunk ahead.  The
+// forward A operand of lane (g, lo) is row lo's 16 bytes of column tile 4 cg + g (columns
+// 64 cg + 16 g + e, e = 0..15); the v digits (fz_v_operands) use the same column order.
 //
-// LDS per block: the forward B operands (NCG (6 NST + 1) KB), and twice (super-chunk
-// parity) the digit matrices XT [column tile][t][chunk][64 rows x 16 bytes], the weight
-// digits [chunk][g'][u][row tile] dwords, the spin masks [st][g'][chunk][i][row tile]
-// dwords and the wavefronts' largest |omega|.
+// LDS per block (static arrays): the forward B operands (NCG (6 NST + 1) KB), and two
+// buffers (super-chunk parity), each the digit image XT, the weight digits
+// [chunk][g'][u][row tile] dwords, the spin masks [st][g'][chunk][i][row tile] dwords, the
+// wavefronts' largest |omega|, and the copied row scales, c-column integers and spin words.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stdlib.h>
