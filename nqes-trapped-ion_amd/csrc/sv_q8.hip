@@ -588,7 +588,7 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
     }
     const int N = a.N, M = a.M, K = a.K;
     const int NCT = a.qcols / 16;  // digit column tiles (the c columns apart: their a block comes from k_q8_omega)
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int g = lane >> 4, lo = lane & 15;
     // 1-D grid of nb ny blocks: the ny column-tile blocks of one row range are adjacent
     // and the row ranges run from the last to the first, the reverse of the forward
@@ -618,21 +618,49 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
     v4i tacc[NS];  // T: the same products with the weight digits unmasked (all 16 rows equal)
 #pragma unroll
     for (int s = 0; s < NS; s++) tacc[s] = v4i{0, 0, 0, 0};
-    const uint3 *Uq = (const uint3 *)a.Uq;
+    // Each wavefront's chunk operands go from global memory straight into its own LDS
+    // slice (global_load_lds_dwordx4: lane l's 16 bytes at the instruction's LDS base + 16 l),
+    // one chunk ahead, double-buffered: the four row tiles' digit words of column tile ct
+    // (4 x 768 B, each contiguous in Uq), the weight digits (QW x 64 B) and the spin masks
+    // of spins i0 .. i0 + 16 NIT (NIT x 1 KB).  No block barrier: a wavefront reads only
+    // what it copied.  The two buffers are distinct arrays and the loop body is
+    // instantiated per buffer, so that the compiler does not wait for the copies into one
+    // before reading the other.
+    constexpr int UQ_DW = 4 * 192, PD_DW = QW * 16, MK_DW = NIT * 256;
+    constexpr int PW = UQ_DW + PD_DW + MK_DW;  // dwords per wavefront and buffer
+    __shared__ __align__(16) uint32_t bqa[4 * PW], bqb[4 * PW];
+    const uint4 *Uq4 = (const uint4 *)a.Uq;
+    auto dma = [&](long long ch, uint32_t *b) {
+        int ln = lane;  // opaque: per-lane addresses formed here, not held through the loop
+        asm volatile("" : "+v"(ln));
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            if (ln < 48)
+                __builtin_amdgcn_global_load_lds((const void *)(Uq4 + ((ch * 4 + q) * NCT + ct) * 48 + ln),
+                                                 (__attribute__((address_space(3))) void *)(b + q * 192), 16, 0, 0);
+        if (ln < QW * 4)
+            __builtin_amdgcn_global_load_lds((const void *)(pd + ch * QW * 4 + ln),
+                                             (__attribute__((address_space(3))) void *)(b + UQ_DW), 16, 0, 0);
+#pragma unroll
+        for (int it = 0; it < NIT; it++)
+            __builtin_amdgcn_global_load_lds((const void *)(msk + (ch * NIA + i0 + 16 * it) * 4 + ln),
+                                             (__attribute__((address_space(3))) void *)(b + UQ_DW + PD_DW + it * 256),
+                                             16, 0, 0);
+    };
     uint3 nw[4];
     uint4 nd[QW], nm[NIT];
-    auto load_chunk = [&](long long ch) {
+    // chunks from the last to the first (the forward pass went the other way)
+    auto body = [&](long long ch, const uint32_t *cur, uint32_t *nxt) {
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");  // this chunk's copies are in LDS
 #pragma unroll
-        for (int q = 0; q < 4; q++) nw[q] = Uq[((ch * 4 + q) * NCT + ct) * 64 + lane];
+        for (int q = 0; q < 4; q++) nw[q] = ((const uint3 *)(cur + q * 192))[lane];
 #pragma unroll
-        for (int u = 0; u < QW; u++) nd[u] = pd[(ch * QW + u) * 4 + g];
+        for (int u = 0; u < QW; u++) nd[u] = ((const uint4 *)(cur + UQ_DW))[u * 4 + g];
 #pragma unroll
-        for (int it = 0; it < NIT; it++) nm[it] = msk[(ch * NIA + i0 + 16 * it + lo) * 4 + g];
-    };
-    // chunks from the last to the first (the forward pass went the other way); no
-    // register prefetch: three wavefronts per SIMD cover each other's loads
-    for (long long ch = c1 - 1; ch >= c0; ch--) {
-        load_chunk(ch);
+        for (int it = 0; it < NIT; it++) nm[it] = ((const uint4 *)(cur + UQ_DW + PD_DW))[(16 * it + lo) * 4 + g];
+        // the next chunk's copies after this chunk's reads (a copy in flight before them
+        // would make the compiler wait for it)
+        if (ch - 1 >= c0) dma(ch - 1, nxt);
         const uint3 *cw = nw;
         const uint4 *cd = nd, *cm = nm;
         v4i bp[QU];
@@ -662,6 +690,12 @@ __global__ __launch_bounds__(256, 3) void k_q8_bwd(SvfArgs a, const int *gblk, c
                         acc[it][t + u] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bp[t], acc[it][t + u], 0, 0, 0);
             }
         }
+    };
+    uint32_t *const sa = bqa + wv * PW, *const sb = bqb + wv * PW;
+    if (c1 - 1 >= c0) dma(c1 - 1, sa);
+    for (long long ch = c1 - 1; ch >= c0; ch -= 2) {
+        body(ch, sa, sb);
+        if (ch - 1 >= c0) body(ch - 1, sb, sa);
     }
     double *part = (double *)a.part + bx * (a.P + 1);
     const double osc = ldexp(1.0, G - WBITS);
