@@ -847,7 +847,11 @@ Q8Shape nq_q8_shape(int N, int M, int K, long long R, bool cplx)
     sh.bwd_cgs = ngrp;
     sh.bwd_waves = (nct + ngrp - 1) / ngrp;
     const long long n64 = (R + 63) / 64;
-    long long nb = 1024;  // measured: fewer, longer row ranges beat more partial sums
+    // about 128 chunks per block, at least 256 blocks (measured: fewer, longer row ranges
+    // write and reduce fewer partial rows; complex N=32 M=64 K=2: 1.19 ms per S*v with 1024
+    // blocks, 1.16 with 512, 1.13 with 256; N=64 M=128 K=4 keeps 1024)
+    long long nb = n64 / 128;
+    if (nb < 256) nb = 256;
     if (nb > n64) nb = n64 > 0 ? n64 : 1;
     while (nb * MAX_BLOCK_CHUNKS < n64) nb *= 2;
     sh.bwd_blocks = (int)nb;
