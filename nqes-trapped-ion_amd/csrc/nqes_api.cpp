@@ -199,6 +199,7 @@ struct nqes_ctx {
     uint64_t sv_calls = 0;
     double sv_ms = 0;
     int nonfinite = 0;
+    int theta_overflow = 0;  // mixed path: an angle |theta| > 80 (FP32 cosh would overflow)
     // sampler launch of the last mixed nqes_sample and its tuner state (tune_idx >= 0: a
     // trial whose time nqes_sample records under tune_key)
     SamplerChoice launch{0, 0, 0};
@@ -807,6 +808,7 @@ int nqes_sample(nqes_ctx *c, const double *W, uint64_t seed, int C, int m, int B
     c->emat.resize(K * K);
     for (int q = 0; q < K * K; q++) c->emat[q] = divr(sh[q], Sd);
     c->nonfinite = nonfinite;
+    c->theta_overflow = (flags & 1u) ? 1 : 0;
     c->C = C;
     c->m = m;
     c->B = B;
@@ -1541,6 +1543,10 @@ int nqes_sr_step(nqes_ctx *c, double *W, uint64_t seed, int C, int m, int B, dou
     memcpy(emat, c->emat.data(), 16 * c->K * c->K);
     if (c->nonfinite) {
         if (stats) stats->nonfinite = 1;
+        if (c->theta_overflow)
+            return c->fail(NQES_ENONFINITE,
+                           "non-finite local energy or gradient: a hidden-unit angle |theta| > 80 exceeds the mixed "
+                           "path's FP32 range (the FP64 path has no such limit)");
         return c->fail(NQES_ENONFINITE, "non-finite local energy or gradient");
     }
     // weight update (rbm.cc:699-717)
